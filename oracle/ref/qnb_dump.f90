@@ -1,0 +1,398 @@
+! qnb_dump: golden-vector writer for the Q nonbonded hot path.
+!
+! This program is test infrastructure of this repository (not reference code).
+! It is linked against the reference's own serial Qdyn6 modules, compiled from
+! /root/reference/src by oracle/ref/Makefile, and runs the same set-up sequence
+! as the reference's `program Qdyn6` (src/qdyn.f90:101-153), then, instead of
+! md_run, performs ONE pair-list generation and ONE evaluation of every
+! nonbonded term of the spherical path, each into a zeroed gradient array, and
+! writes inputs and outputs to a self-describing binary file:
+!
+!   record := int32 namelen, char name(namelen), int32 kind (1=int32, 2=float64),
+!             int32 ndim, int32 dims(ndim), data (Fortran order)
+!
+! Usage:  qnb_dump out.bin input.inp   (Qdyn6 reads its input from the last argument)
+program qnb_dump
+  use MD
+  use SIMPREP
+  use MPIGLOB
+  use NONBONDENE
+  use POTENE
+  implicit none
+  character(len=512) :: outname
+  integer :: u, n, is, i, k, nq, ns
+  real(kind=prec) :: Ekinmax_loc
+  integer, allocatable :: itmp(:)
+  real(kind=prec), allocatable :: dtmp(:)
+  type(NB_ENERGIES) :: e_nb
+  type(RESTRAINT_ENERGIES) :: e_rs
+  real(kind=prec) :: e_lrf
+  type(NB_ENERGIES), allocatable :: eq_nb(:)
+  real(kind=prec), allocatable :: lam(:)
+
+  nodeid = 0
+  numnodes = 1
+  call md_startup
+  if (command_argument_count() < 2) stop 'usage: qnb_dump out.bin input.inp'
+  call get_command_argument(1, outname)
+  ! initialize() reads the input file named by the first argument (src/md.f90:161-170)
+  if (.not. initialize()) call die('Invalid data in input file')
+  call open_files(md=.true.)
+  call topology
+  call prep_coord
+  if (nstates > 0) call get_fep
+  call prep_sim
+  call prep_sim_version(version_pass())
+  call close_input_files
+  call init_constraints
+  call make_nbqqlist
+  call shrink_topology
+  call init_trj
+  if (iseed > 0) then
+    call maxwell
+    call initial_constraint(const_method)
+    call stop_cm_translation
+  end if
+  call distribute_nonbonds
+  call precompute_interactions
+  call nbmonitorlist
+
+  ! temperature (sets Tfree used by restrain_solvent), as md_run does before step 0
+  Ekinmax_loc = 1000.0_prec*Ndegf*Boltz*Temp0/2.0_prec/real(natom, kind=prec)
+  call temperature(tscale(1:ntgroups), Ekinmax_loc, .false.)
+  istep = 0
+
+  open(newunit=u, file=trim(outname), access='stream', form='unformatted', status='replace')
+
+  ! ---------------- scalars ----------------
+  call wi1(u, 'natom', natom)
+  call wi1(u, 'nat_solute', nat_solute)
+  call wi1(u, 'nwat', nwat)
+  call wi1(u, 'solv_atom', solv_atom)
+  call wi1(u, 'ncgp', ncgp)
+  call wi1(u, 'ncgp_solute', ncgp_solute)
+  call wi1(u, 'nqat', nqat)
+  call wi1(u, 'nstates', nstates)
+  call wi1(u, 'natyps', natyps)
+  call wi1(u, 'ivdw_rule', ivdw_rule)
+  call wi1(u, 'iuse_switch_atom', iuse_switch_atom)
+  call wi1(u, 'use_LRF', merge(1, 0, use_LRF))
+  call wi1(u, 'wpol_restr', merge(1, 0, wpol_restr))
+  call wi1(u, 'solvent_type', solvent_type)
+  call wi1(u, 'nwpolr_shell', nwpolr_shell)
+  call wi1(u, 'NBcycle', NBcycle)
+  call wd1(u, 'el14_scale', el14_scale)
+  call wd1(u, 'Rcpp', Rcpp)
+  call wd1(u, 'Rcpw', Rcpw)
+  call wd1(u, 'Rcww', Rcww)
+  call wd1(u, 'Rcq', Rcq)
+  call wd1(u, 'RcLRF', RcLRF)
+  call wd1(u, 'rexcl_o', rexcl_o)
+  call wd1(u, 'rwat', rwat)
+  call wd1(u, 'fk_wsphere', fk_wsphere)
+  call wd1(u, 'Dwmz', Dwmz)
+  call wd1(u, 'awmz', awmz)
+  call wd1(u, 'Boltz', Boltz)
+  call wd1(u, 'Tfree', Tfree)
+  call wd1(u, 'fkwpol', fkwpol)
+  call wd1(u, 'itdis', itdis)
+  call wd1(u, 'dt', dt)
+  call wd(u, 'xpcent', 3, (/xpcent%x, xpcent%y, xpcent%z/), 1, (/3/))
+  call wd(u, 'xwcent', 3, (/xwcent%x, xwcent%y, xwcent%z/), 1, (/3/))
+  if (nstates > 0) call wd(u, 'lambda', nstates, EQ(1:nstates)%lambda, 1, (/nstates/))
+
+  ! ---------------- per-atom arrays ----------------
+  allocate(dtmp(3*natom), itmp(natom))
+  call pack_x(x, natom, dtmp)
+  call wd(u, 'x', 3*natom, dtmp, 2, (/3, natom/))
+  call wd(u, 'crg', natom, crg(1:natom), 1, (/natom/))
+  call wi(u, 'iac', natom, int(iac(1:natom)), 1, (/natom/))
+  call wi(u, 'iqatom', natom, iqatom(1:natom), 1, (/natom/))
+  do i = 1, natom
+    itmp(i) = merge(1, 0, excl(i))
+  end do
+  call wi(u, 'excl', natom, itmp, 1, (/natom/))
+  do i = 1, natom
+    itmp(i) = merge(1, 0, heavy(i))
+  end do
+  call wi(u, 'heavy', natom, itmp, 1, (/natom/))
+  if (allocated(iwhich_cgp)) call wi(u, 'iwhich_cgp', natom, int(iwhich_cgp(1:natom)), 1, (/natom/))
+  call wd(u, 'winv', natom, winv(1:natom), 1, (/natom/))
+  call pack_x(v, natom, dtmp)
+  call wd(u, 'v', 3*natom, dtmp, 2, (/3, natom/))
+
+  ! ---------------- charge groups ----------------
+  call wi(u, 'cgp_first', ncgp, int(cgp(1:ncgp)%first), 1, (/ncgp/))
+  call wi(u, 'cgp_last', ncgp, int(cgp(1:ncgp)%last), 1, (/ncgp/))
+  call wi(u, 'cgp_iswitch', ncgp, int(cgp(1:ncgp)%iswitch), 1, (/ncgp/))
+  n = cgp(ncgp)%last
+  call wi(u, 'cgpatom', n, int(cgpatom(1:n)), 1, (/n/))
+
+  ! ---------------- types and exclusions ----------------
+  call wd(u, 'avdw', 3*natyps, (/((iaclib(i)%avdw(k), k=1,3), i=1,natyps)/), 2, (/3, natyps/))
+  call wd(u, 'bvdw', 3*natyps, (/((iaclib(i)%bvdw(k), k=1,3), i=1,natyps)/), 2, (/3, natyps/))
+  call wi(u, 'ljcod', natyps*natyps, reshape(ljcod(1:natyps,1:natyps), (/natyps*natyps/)), 2, (/natyps, natyps/))
+  if (nat_solute > 0) then
+    n = max_nbr_range*nat_solute
+    call wi(u, 'listex', n, reshape(merge(1, 0, listex(1:max_nbr_range,1:nat_solute)), (/n/)), &
+            2, (/max_nbr_range, nat_solute/))
+    call wi(u, 'list14', n, reshape(merge(1, 0, list14(1:max_nbr_range,1:nat_solute)), (/n/)), &
+            2, (/max_nbr_range, nat_solute/))
+    if (nexlong > 0) call wi(u, 'listexlong', 2*nexlong, &
+            reshape(int(listexlong(1:2,1:nexlong)), (/2*nexlong/)), 2, (/2, nexlong/))
+    if (n14long > 0) call wi(u, 'list14long', 2*n14long, &
+            reshape(int(list14long(1:2,1:n14long)), (/2*n14long/)), 2, (/2, n14long/))
+  end if
+  if (nwat > 0) then
+    call wd(u, 'aLJ_solv', 3*solv_atom, reshape(aLJ_solv(1:solv_atom,1:3), (/3*solv_atom/)), 2, (/solv_atom, 3/))
+    call wd(u, 'bLJ_solv', 3*solv_atom, reshape(bLJ_solv(1:solv_atom,1:3), (/3*solv_atom/)), 2, (/solv_atom, 3/))
+    call wd(u, 'chg_solv', solv_atom, chg_solv(1:solv_atom), 1, (/solv_atom/))
+    n = solv_atom*solv_atom
+    call wd(u, 'ww_A', n, reshape(ww_precomp%vdWA, (/n/)), 2, (/solv_atom, solv_atom/))
+    call wd(u, 'ww_B', n, reshape(ww_precomp%vdWB, (/n/)), 2, (/solv_atom, solv_atom/))
+    call wd(u, 'ww_el', n, reshape(ww_precomp%elec, (/n/)), 2, (/solv_atom, solv_atom/))
+  end if
+
+  ! ---------------- Q-atom tables ----------------
+  nq = nqat
+  ns = nstates
+  if (nq > 0) then
+    call wi(u, 'iqseq', nq, int(iqseq(1:nq)), 1, (/nq/))
+    n = ns*nat_solute*nq
+    if (nat_solute > 0) call wi(u, 'qconn', n, reshape(int(qconn), (/n/)), 3, (/ns, nat_solute, nq/))
+    call wd(u, 'qcrg', nq*ns, reshape(qcrg(1:nq,1:ns), (/nq*ns/)), 2, (/nq, ns/))
+    if (nat_solute > 0) then
+      n = nat_solute*nq*ns
+      call wd(u, 'qp_A', n, reshape(qp_precomp%vdWA, (/n/)), 3, (/nat_solute, nq, ns/))
+      call wd(u, 'qp_B', n, reshape(qp_precomp%vdWB, (/n/)), 3, (/nat_solute, nq, ns/))
+      call wd(u, 'qp_el', n, reshape(qp_precomp%elec, (/n/)), 3, (/nat_solute, nq, ns/))
+      call wd(u, 'qp_sc', n, reshape(qp_precomp%score, (/n/)), 3, (/nat_solute, nq, ns/))
+      call wi(u, 'qp_set', n, reshape(merge(1, 0, qp_precomp%set), (/n/)), 3, (/nat_solute, nq, ns/))
+    end if
+    n = nq*nq*ns
+    call wd(u, 'qq_A', n, reshape(qq_precomp%vdWA, (/n/)), 3, (/nq, nq, ns/))
+    call wd(u, 'qq_B', n, reshape(qq_precomp%vdWB, (/n/)), 3, (/nq, nq, ns/))
+    call wd(u, 'qq_el', n, reshape(qq_precomp%elec, (/n/)), 3, (/nq, nq, ns/))
+    call wd(u, 'qq_sc', n, reshape(qq_precomp%score, (/n/)), 3, (/nq, nq, ns/))
+    call wi(u, 'qq_set', n, reshape(merge(1, 0, qq_precomp%set), (/n/)), 3, (/nq, nq, ns/))
+    call wi(u, 'qq_soft', n, reshape(merge(1, 0, qq_precomp%soft), (/n/)), 3, (/nq, nq, ns/))
+    if (nwat > 0) then
+      n = nq*solv_atom*ns
+      call wd(u, 'qw_A', n, reshape(qw_precomp%vdWA, (/n/)), 3, (/nq, solv_atom, ns/))
+      call wd(u, 'qw_B', n, reshape(qw_precomp%vdWB, (/n/)), 3, (/nq, solv_atom, ns/))
+      call wd(u, 'qw_el', n, reshape(qw_precomp%elec, (/n/)), 3, (/nq, solv_atom, ns/))
+      call wd(u, 'qw_sc', n, reshape(qw_precomp%score, (/n/)), 3, (/nq, solv_atom, ns/))
+    end if
+  end if
+  if (nwat > 0 .and. nwpolr_shell > 0) then
+    n = nwpolr_shell
+    call wd(u, 'wshell_rout', n, wshell(1:n)%rout, 1, (/n/))
+    call wd(u, 'wshell_dr', n, wshell(1:n)%dr, 1, (/n/))
+    call wd(u, 'wshell_cstb', n, wshell(1:n)%cstb, 1, (/n/))
+    call wd(u, 'wshell_theta_corr', n, wshell(1:n)%theta_corr, 1, (/n/))
+  end if
+
+  ! ---------------- pair lists ----------------
+  call make_pair_lists(Rcq, Rcq**2, RcLRF**2, Rcpp**2, Rcpw**2, Rcww**2)
+  call dump_nb(u, 'nbpp', nbpp, nbpp_pair)
+  call dump_nb(u, 'nbpw', nbpw, nbpw_pair)
+  call dump_nb(u, 'nbww', nbww, nbww_pair)
+  if (nq > 0) then
+    call dump_nbqp(u, 'nbqp', nbqp, nbqp_pair, ns)
+    call dump_nbqp(u, 'nbqw', nbqw, nbqw_pair, ns)
+    call wi(u, 'nbqq_pair', ns, nbqq_pair(1:ns), 1, (/ns/))
+    call wi(u, 'nbqqp_pair', ns, nbqqp_pair(1:ns), 1, (/ns/))
+    do is = 1, ns
+      call dump_nbq(u, 'nbqq_s'//char(48+is), nbqq(:, is), nbqq_pair(is))
+      call dump_nbqp(u, 'nbqqp_s'//char(48+is), nbqqp(:, is:is), nbqqp_pair(is), 1)
+    end do
+  end if
+  if (use_LRF) then
+    deallocate(dtmp)
+    allocate(dtmp(43*ncgp))
+    do i = 1, ncgp
+      k = (i-1)*43
+      dtmp(k+1:k+3) = (/lrf(i)%cgp_cent%x, lrf(i)%cgp_cent%y, lrf(i)%cgp_cent%z/)
+      dtmp(k+4) = lrf(i)%phi0
+      dtmp(k+5:k+7) = (/lrf(i)%phi1%x, lrf(i)%phi1%y, lrf(i)%phi1%z/)
+      do n = 1, 3
+        dtmp(k+8+3*(n-1):k+10+3*(n-1)) = (/lrf(i)%phi2(n)%x, lrf(i)%phi2(n)%y, lrf(i)%phi2(n)%z/)
+      end do
+      do n = 1, 9
+        dtmp(k+17+3*(n-1):k+19+3*(n-1)) = (/lrf(i)%phi3(n)%x, lrf(i)%phi3(n)%y, lrf(i)%phi3(n)%z/)
+      end do
+    end do
+    call wd(u, 'lrf', 43*ncgp, dtmp, 2, (/43, ncgp/))
+    deallocate(dtmp)
+    allocate(dtmp(3*natom))
+  end if
+
+  ! ---------------- terms, each into a zeroed gradient ----------------
+  allocate(eq_nb(max(ns,1)), lam(max(ns,1)))
+  if (ns > 0) lam(1:ns) = EQ(1:ns)%lambda
+  if (nwat > 0) then
+    d(:) = zero; e_nb = NB_ENERGIES(zero, zero)
+    if (ivdw_rule == VDW_GEOMETRIC .and. solvent_type == SOLVENT_SPC) then
+      call nonbond_ww_spc(e_nb)
+    else
+      call nonbond_ww(e_nb)
+    end if
+    call dump_term(u, 'ww', (/e_nb%el, e_nb%vdw/), 2)
+    if (nq > 0) then
+      d(:) = zero; eq_nb(:) = NB_ENERGIES(zero, zero)
+      if (ivdw_rule == VDW_GEOMETRIC .and. solvent_type == SOLVENT_SPC) then
+        call nonbond_qw_spc(eq_nb(1:ns), lam(1:ns))
+      else
+        call nonbond_qw(eq_nb(1:ns), lam(1:ns))
+      end if
+      call dump_term(u, 'qw', (/(eq_nb(is)%el, eq_nb(is)%vdw, is=1,ns)/), 2*ns)
+    end if
+    d(:) = zero; e_nb = NB_ENERGIES(zero, zero)
+    call nonbond_pw(e_nb)
+    call dump_term(u, 'pw', (/e_nb%el, e_nb%vdw/), 2)
+  end if
+  d(:) = zero; e_nb = NB_ENERGIES(zero, zero)
+  call nonbond_pp(e_nb)
+  call dump_term(u, 'pp', (/e_nb%el, e_nb%vdw/), 2)
+  if (nq > 0) then
+    d(:) = zero; eq_nb(:) = NB_ENERGIES(zero, zero)
+    call nonbond_qp(eq_nb(1:ns), lam(1:ns))
+    call dump_term(u, 'qp', (/(eq_nb(is)%el, eq_nb(is)%vdw, is=1,ns)/), 2*ns)
+    d(:) = zero; eq_nb(:) = NB_ENERGIES(zero, zero)
+    call nonbond_qq(eq_nb(1:ns), lam(1:ns))
+    call dump_term(u, 'qq', (/(eq_nb(is)%el, eq_nb(is)%vdw, is=1,ns)/), 2*ns)
+    d(:) = zero; eq_nb(:) = NB_ENERGIES(zero, zero)
+    call nonbond_qqp(eq_nb(1:ns), lam(1:ns))
+    call dump_term(u, 'qqp', (/(eq_nb(is)%el, eq_nb(is)%vdw, is=1,ns)/), 2*ns)
+  end if
+  if (use_LRF) then
+    d(:) = zero; e_lrf = zero
+    call lrf_taylor(e_lrf)
+    call dump_term(u, 'lrf', (/e_lrf/), 1)
+  end if
+  if (nwat > 0) then
+    d(:) = zero; e_rs = RESTRAINT_ENERGIES(zero, zero, zero, zero, zero, zero)
+    call restrain_solvent(e_rs)
+    call dump_term(u, 'solvent_radial', (/e_rs%solvent_radial/), 1)
+    if (wpol_restr) then
+      d(:) = zero; e_rs = RESTRAINT_ENERGIES(zero, zero, zero, zero, zero, zero)
+      call watpol(e_rs, .true.)
+      call dump_term(u, 'water_pol', (/e_rs%water_pol/), 1)
+      n = nwpolr_shell
+      call wi(u, 'wshell_n_insh', n, wshell(1:n)%n_insh, 1, (/n/))
+      call wd(u, 'theta', nwat, theta(1:nwat), 1, (/nwat/))
+    end if
+  end if
+  ! full potential energy (bonded terms included) for the printed known answers
+  call pot_energy(E, EQ, .true.)
+  call pack_x(d, natom, dtmp)
+  call wd(u, 'd_total', 3*natom, dtmp, 2, (/3, natom/))
+  call wd(u, 'E_summary', 12, (/E%potential, E%LRF, E%pp%el, E%pp%vdw, E%pw%el, E%pw%vdw, &
+          E%ww%el, E%ww%vdw, E%qx%el, E%qx%vdw, E%restraint%solvent_radial, E%restraint%water_pol/), 1, (/12/))
+  close(u)
+  write(*,'(a)') 'qnb_dump: wrote '//trim(outname)
+
+contains
+
+  subroutine whead(u, name, kind, ndim, dims)
+    integer, intent(in) :: u, kind, ndim, dims(:)
+    character(len=*), intent(in) :: name
+    write(u) int(len_trim(name), 4), trim(name), int(kind, 4), int(ndim, 4), int(dims(1:ndim), 4)
+  end subroutine whead
+
+  subroutine wi(u, name, n, a, ndim, dims)
+    integer, intent(in) :: u, n, ndim, dims(:)
+    character(len=*), intent(in) :: name
+    integer, intent(in) :: a(n)
+    call whead(u, name, 1, ndim, dims)
+    if (n > 0) write(u) int(a(1:n), 4)
+  end subroutine wi
+
+  subroutine wd(u, name, n, a, ndim, dims)
+    integer, intent(in) :: u, n, ndim, dims(:)
+    character(len=*), intent(in) :: name
+    real(kind=prec), intent(in) :: a(n)
+    call whead(u, name, 2, ndim, dims)
+    if (n > 0) write(u) real(a(1:n), 8)
+  end subroutine wd
+
+  subroutine wi1(u, name, a)
+    integer, intent(in) :: u, a
+    character(len=*), intent(in) :: name
+    call wi(u, name, 1, (/a/), 1, (/1/))
+  end subroutine wi1
+
+  subroutine wd1(u, name, a)
+    integer, intent(in) :: u
+    real(kind=prec), intent(in) :: a
+    character(len=*), intent(in) :: name
+    call wd(u, name, 1, (/a/), 1, (/1/))
+  end subroutine wd1
+
+  subroutine pack_x(xv, n, out)
+    integer, intent(in) :: n
+    type(qr_vec), intent(in) :: xv(:)
+    real(kind=prec), intent(out) :: out(:)
+    integer :: j
+    do j = 1, n
+      out(3*j-2) = xv(j)%x
+      out(3*j-1) = xv(j)%y
+      out(3*j)   = xv(j)%z
+    end do
+  end subroutine pack_x
+
+  subroutine dump_term(u, name, e, ne)
+    integer, intent(in) :: u, ne
+    character(len=*), intent(in) :: name
+    real(kind=prec), intent(in) :: e(ne)
+    call wd(u, 'E_'//name, ne, e, 1, (/ne/))
+    call pack_x(d, natom, dtmp)
+    call wd(u, 'd_'//name, 3*natom, dtmp, 2, (/3, natom/))
+  end subroutine dump_term
+
+  subroutine dump_nb(u, name, l, np)
+    integer, intent(in) :: u, np
+    character(len=*), intent(in) :: name
+    type(NB_TYPE), intent(in) :: l(:)
+    integer :: j
+    real(kind=prec), allocatable :: p(:)
+    call wi(u, name//'_ij', 2*np, (/(int(l(j)%i), int(l(j)%j), j=1,np)/), 2, (/2, np/))
+    allocate(p(3*np))
+    do j = 1, np
+      p(3*j-2) = l(j)%vdWA; p(3*j-1) = l(j)%vdWB; p(3*j) = l(j)%elec
+    end do
+    call wd(u, name//'_par', 3*np, p, 2, (/3, np/))
+  end subroutine dump_nb
+
+  subroutine dump_nbqp(u, name, l, np, nst)
+    integer, intent(in) :: u, np, nst
+    character(len=*), intent(in) :: name
+    type(NBQP_TYPE), intent(in) :: l(:,:)
+    integer :: j, s
+    real(kind=prec), allocatable :: p(:)
+    call wi(u, name//'_ij', 2*np, (/(int(l(j,1)%i), int(l(j,1)%j), j=1,np)/), 2, (/2, np/))
+    allocate(p(4*np*nst))
+    do s = 1, nst
+      do j = 1, np
+        k = 4*((s-1)*np + j - 1)
+        p(k+1) = l(j,s)%vdWA; p(k+2) = l(j,s)%vdWB; p(k+3) = l(j,s)%elec; p(k+4) = l(j,s)%score
+      end do
+    end do
+    call wd(u, name//'_par', 4*np*nst, p, 3, (/4, np, nst/))
+  end subroutine dump_nbqp
+
+  subroutine dump_nbq(u, name, l, np)
+    integer, intent(in) :: u, np
+    character(len=*), intent(in) :: name
+    type(NBQ_TYPE), intent(in) :: l(:)
+    integer :: j
+    real(kind=prec), allocatable :: p(:)
+    call wi(u, name//'_ij', 3*np, (/(int(l(j)%iq), int(l(j)%jq), merge(1, 0, l(j)%soft), j=1,np)/), 2, (/3, np/))
+    allocate(p(4*np))
+    do j = 1, np
+      p(4*j-3) = l(j)%vdWA; p(4*j-2) = l(j)%vdWB; p(4*j-1) = l(j)%elec; p(4*j) = l(j)%score
+    end do
+    call wd(u, name//'_par', 4*np, p, 2, (/4, np/))
+  end subroutine dump_nbq
+
+end program qnb_dump

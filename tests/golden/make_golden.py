@@ -1,0 +1,219 @@
+"""Writes the golden fixtures of the nonbonded hot path from the reference itself.
+
+Runs in the development container only (it needs /root/reference and the
+reference build under oracle/_ref/, made by `make -C oracle/ref`).  For each case
+it prepares a Qdyn6 input from the reference's own SPH test system
+(tests/basic_tests/prep_SPH/lig_w.{top,fep}, tests/basic_tests/eq1.inp with the
+substitutions tests/basic_tests/run_test.sh:171-185 makes for SPH), runs
+oracle/_ref/qnb_dump (this repository's driver linked against the reference's
+modules) and converts the dump into a compressed .npz holding:
+
+  * the system as set up by Qdyn6 (coordinates, scaled charges, types, charge
+    groups, exclusions, Q-atom precompute tables, SCAAS constants, cutoffs);
+  * the pair lists, as (i, j) arrays; water-water as molecule pairs because each
+    one expands to solv_atom**2 atom pairs in a fixed order
+    (src/nonbondene.f90:4366-4375, checked when the fixture is written);
+  * the LRF expansion table after the list update;
+  * energy and gradient of every nonbonded term, evaluated into a zeroed d.
+
+Cases:
+  lig_eq1  the reference test exactly (1 state, all 46 solute atoms are Q-atoms);
+           its Q-surr. energies are the reference's own known answer
+           (tests/basic_tests/eval_test.sh:97-101).
+  lig_fep2 the same topology with a 2-state FEP file written here: 8 Q-atoms
+           with changed charges and types, soft-core alphas, one exponential
+           soft pair and an off-diagonal, lambda (0.7, 0.3), solute cutoffs 16 A (the
+           ligand has a 13.4 A charge group), water-water 9 A,
+           q_atom 12 A (so the Q lists are radius-limited) -> exercises pp, pw,
+           qp, qqp, qq, qw, ww, LRF and SCAAS together.
+  lig_fep2_nolrf  as lig_fep2 with LRF off (cut-off-only list builders).
+
+Usage: python tests/golden/make_golden.py  (writes tests/golden/*.npz)
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference/tests/basic_tests"
+DUMP = os.path.join(REPO, "oracle", "_ref", "qnb_dump")
+sys.path.insert(0, HERE)
+from qdump import read_dump  # noqa: E402
+
+EQ1_SPH = """[MD]
+steps                    100
+stepsize                   0.1
+temperature                0.1
+bath_coupling              1
+thermostat	berendsen
+integrator	leap-frog
+separate_scaling        on
+random_seed                1
+initial_temperature        0.1
+lrf                       {lrf}
+
+[sphere]
+shell_radius	16
+shell_force	20
+[solvent]
+pol_update	100
+
+[cut-offs]
+q_atom	{rq}
+lrf	99
+{extra_cut}
+[intervals]
+non_bond                  25
+output                    50
+
+[files]
+topology                  lig_w.top
+fep                       {fep}
+final                     eq1.re
+{lambdas}
+[sequence_restraints]
+1 46 0.5 0 1
+"""
+
+# Q-atoms of the 2-state case (topology numbers): a cluster around atom 1 and
+# a second one around atom 20, ten bonds away, so a soft pair can join them.
+FEP2_ATOMS = [1, 2, 3, 22, 25, 19, 20, 45]
+FEP2_DQ = [0.10, -0.20, 0.05, 0.15, -0.10, 0.30, -0.25, 0.12]
+
+
+def fep2_text(top_charges):
+    types = [
+        ("QN1", 971.75, 28.31, 180.0, 2.5, 687.13, 20.02, 14.01),
+        ("QC1", 1059.13, 23.67, 0.0, 0.0, 748.92, 16.74, 12.01),
+        ("QC2", 944.52, 22.03, 250.0, 2.5, 667.88, 15.58, 12.01),
+        ("QO1", 616.44, 23.77, 90.0, 1.8, 435.89, 16.81, 16.00),
+        ("QH1", 84.57, 5.41, 0.0, 0.0, 59.8, 3.83, 1.008),
+    ]
+    st1 = ["QN1", "QC1", "QC1", "QC2", "QO1", "QC2", "QO1", "QH1"]
+    st2 = ["QC1", "QC2", "QC1", "QC2", "QO1", "QO1", "QC2", "QH1"]
+    alphas = [(0.0, 5.0), (2.0, 2.0), (0.0, 0.0), (1.0, 3.0), (0.0, 0.0), (4.0, 0.0), (0.0, 2.5), (0.0, 0.0)]
+    out = ["[FEP]", "states 2", "", "[atoms]"]
+    for k, a in enumerate(FEP2_ATOMS):
+        out.append(f"{k + 1} {a}")
+    out += ["", "[change_charges]"]
+    for k, a in enumerate(FEP2_ATOMS):
+        q = top_charges[a - 1]
+        out.append(f"{k + 1} {q:.4f} {q + FEP2_DQ[k]:.4f}")
+    out += ["", "[atom_types]"]
+    for t in types:
+        out.append("%-6s %8.2f %8.2f %7.1f %6.2f %8.2f %8.2f %8.3f" % t)
+    out += ["", "[change_atoms]"]
+    for k in range(len(FEP2_ATOMS)):
+        out.append(f"{k + 1} {st1[k]} {st2[k]}")
+    out += ["", "[soft_pairs]", "1 7", "", "[softcore]"]
+    for k, (a1, a2) in enumerate(alphas):
+        out.append(f"{k + 1} {a1} {a2}")
+    out += ["", "[off_diagonals]", "1 2 1 7 80.0 0.0", ""]
+    return "\n".join(out)
+
+
+def top_charges():
+    lines = open(os.path.join(REF, "prep_SPH", "lig_w.top")).read().split("\n")
+    for k, ln in enumerate(lines):
+        if "No. of atomic charges" in ln:
+            n = int(ln.split()[0])
+            vals = []
+            j = k + 1
+            while len(vals) < n:
+                vals += [float(t) for t in lines[j].split()]
+                j += 1
+            return vals
+    raise RuntimeError("charges not found")
+
+
+def run_case(name, inp_text, fep_text=None):
+    tmp = tempfile.mkdtemp(prefix="qnbgold_")
+    try:
+        shutil.copy(os.path.join(REF, "prep_SPH", "lig_w.top"), tmp)
+        shutil.copy(os.path.join(REF, "prep_SPH", "lig_w.fep"), tmp)
+        if fep_text is not None:
+            with open(os.path.join(tmp, "fep2.fep"), "w") as f:
+                f.write(fep_text)
+        with open(os.path.join(tmp, "run.inp"), "w") as f:
+            f.write(inp_text)
+        log = subprocess.run([DUMP, "out.bin", "run.inp"], cwd=tmp, capture_output=True, text=True)
+        if log.returncode != 0 or not os.path.exists(os.path.join(tmp, "out.bin")):
+            print(log.stdout[-3000:], log.stderr[-3000:])
+            raise RuntimeError("qnb_dump failed for " + name)
+        return read_dump(os.path.join(tmp, "out.bin")), log.stdout
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def compact(d):
+    """Pack a dump into fixture arrays (0-based atom indices everywhere)."""
+    out = {}
+    for k, v in d.items():
+        if k.startswith("nbww"):
+            continue
+        if k.endswith("_ij") and k.startswith("nb"):
+            continue
+        out[k] = np.asarray(v)
+    # 0-based copies of index arrays; Fortran 1-based values are kept verbatim
+    # under their own names so the fixture is a faithful record of the dump
+    sa = d["solv_atom"]
+    ns_ = d["nat_solute"]
+    ww = d["nbww_ij"]
+    if ww.shape[1]:
+        assert ww.shape[1] % (sa * sa) == 0
+        blk = ww.reshape(2, sa * sa, -1, order="F").transpose(0, 2, 1)
+        iw = (blk[0, :, 0] - ns_ - 1) // sa
+        jw = (blk[1, :, 0] - ns_ - 1) // sa
+        # the fixed expansion (la, ka) of each molecule pair
+        la = np.repeat(np.arange(sa), sa)
+        ka = np.tile(np.arange(sa), sa)
+        assert np.array_equal(blk[0] - 1, ns_ + sa * iw[:, None] + la[None, :])
+        assert np.array_equal(blk[1] - 1, ns_ + sa * jw[:, None] + ka[None, :])
+        out["ww_mol_pairs"] = np.stack([iw, jw]).astype(np.int32)
+        par = d["nbww_par"].reshape(3, sa * sa, -1, order="F").transpose(0, 2, 1)
+        assert np.all(par == par[:, :1, :])
+        out["ww_blk_par"] = par[:, 0, :]
+    else:
+        out["ww_mol_pairs"] = np.zeros((2, 0), np.int32)
+    for lst in ("nbpp", "nbpw", "nbqp", "nbqw"):
+        out[lst + "_ij"] = d[lst + "_ij"].astype(np.int32)
+    for k in d:
+        if k.startswith("nbqq") and k.endswith("_ij"):
+            out[k] = d[k].astype(np.int32)
+    return out
+
+
+def main():
+    if not os.path.exists(DUMP):
+        sys.exit("build the reference first: make -C oracle/ref")
+    q = top_charges()
+    cases = {
+        "lig_eq1": (EQ1_SPH.format(lrf="on", rq=99, extra_cut="", fep="lig_w.fep", lambdas=""), None),
+        "lig_fep2": (EQ1_SPH.format(lrf="on", rq=12,
+                                    extra_cut="solute_solute 16\nsolvent_solvent 9\nsolute_solvent 16\n",
+                                    fep="fep2.fep", lambdas="[lambdas]\n0.7 0.3\n"), fep2_text(q)),
+        "lig_fep2_nolrf": (EQ1_SPH.format(lrf="off", rq=12,
+                                          extra_cut="solute_solute 16\nsolvent_solvent 9\nsolute_solvent 16\n",
+                                          fep="fep2.fep", lambdas="[lambdas]\n0.7 0.3\n"), fep2_text(q)),
+    }
+    for name, (inp, fep) in cases.items():
+        d, log = run_case(name, inp, fep)
+        if name == "lig_eq1":
+            # the reference's own known answer (tests/basic_tests/eval_test.sh:97-101)
+            el, vdw = d["E_qw"]
+            assert "%.2f %.2f" % (el, vdw) == "3.12 139.43", (el, vdw)
+        arrs = compact(d)
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **arrs)
+        print(name, "->", path, os.path.getsize(path) // 1024, "KiB;",
+              "pp", d["nbpp_ij"].shape[1], "pw", d["nbpw_ij"].shape[1], "ww", d["nbww_ij"].shape[1],
+              "qp", d["nbqp_ij"].shape[1], "qw", d["nbqw_ij"].shape[1])
+
+
+if __name__ == "__main__":
+    main()
