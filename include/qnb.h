@@ -1,0 +1,271 @@
+/*
+ * qnb.h -- C ABI of the MI355X nonbonded engine for Q's spherical-boundary MD.
+ *
+ * The engine replaces the nonbonded half of Qdyn6's potential energy: the
+ * pair-list builders, the nonbonded energy/gradient loops, the per-state Q-atom
+ * FEP/EVB energies, the local-reaction-field (LRF) expansion and the SCAAS
+ * water-sphere restraints.  Each entry point below names the reference routine
+ * it stands in for (paths relative to the reference's src/).
+ *
+ * Conventions
+ *  - Plain C types only; every pointer is borrowed for the duration of the call.
+ *  - Atom, charge-group, Q-atom and type indices are 0-based.  The one
+ *    exception is iqatom[], which keeps the reference meaning: 0 = not a
+ *    Q-atom, k > 0 = Q-atom number k (1-based), as src/globals.f90:65.
+ *  - Coordinates and gradients crossing the ABI are the reference layout:
+ *    natom x {x,y,z} float64 (TYPE(qr_vec), src/sizes.f90:109-111).  The
+ *    gradient is d = dE/dx (the force is -d), src/potene.f90:109.
+ *  - Charges are already scaled by sqrt(coulomb constant), as prep_sim does
+ *    (src/simprep.f90:3714-3721).
+ *  - Multi-dimensional tables keep the reference's Fortran (column-major)
+ *    order, so the Fortran shim can pass its arrays unchanged; the layout of
+ *    each one is spelled out next to it.
+ *  - Every function returns 0 on success and a nonzero QNB_E* code otherwise;
+ *    qnb_last_error() gives the message.  The Fortran side turns a nonzero code
+ *    into die(msg) (src/qalloc.f90:609), keeping the reference's semantics.
+ *  - A handle is used from one host thread at a time.  It owns its device
+ *    memory and HIP stream.
+ */
+#ifndef QNB_H
+#define QNB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QNB_ABI_VERSION 1
+
+enum qnb_status {
+  QNB_OK = 0,
+  QNB_EINVAL = 1,     /* bad argument or inconsistent system description */
+  QNB_EDEVICE = 2,    /* HIP runtime error */
+  QNB_ENOMEM = 3,     /* allocation failed */
+  QNB_ENONFINITE = 4, /* non-finite energy (reference: NaN check, src/md.f90:1861) */
+  QNB_EOVERFLOW = 5,  /* a list exceeded its capacity after regrowth */
+  QNB_EUNSUPPORTED = 6
+};
+
+enum qnb_precision {
+  QNB_PREC_FP64 = 0,  /* plain: FP64 throughout */
+  QNB_PREC_MIXED = 1  /* FP32 pair arithmetic, FP64 accumulation */
+};
+
+enum qnb_vdw_rule { QNB_VDW_GEOMETRIC = 1, QNB_VDW_ARITHMETIC = 2 };
+
+#define QNB_MAX_NBR_RANGE 25 /* src/topo.f90:37 */
+#define QNB_MAX_STATES 8
+
+/* Everything the engine needs from Qdyn6's set-up (topology, FEP file,
+ * prep_sim, precompute_interactions).  Sizes first. */
+typedef struct qnb_system {
+  int32_t natom;       /* all atoms (solute + solvent)                  */
+  int32_t nat_solute;  /* solute atoms come first: 0 .. nat_solute-1   */
+  int32_t nwat;        /* solvent molecules (each one charge group)     */
+  int32_t solv_atom;   /* atoms per solvent molecule (3 for TIP3P/SPC)  */
+  int32_t ncgp;        /* charge groups; solvent groups follow solute   */
+  int32_t ncgp_solute;
+  int32_t nqat;        /* Q-atoms                                        */
+  int32_t nstates;     /* FEP/EVB states (1..QNB_MAX_STATES)             */
+  int32_t ntypes;      /* atom types (natyps)                            */
+  int32_t ivdw_rule;   /* qnb_vdw_rule, src/topo.f90:185                 */
+  int32_t solvent_spc; /* 1: SPC-like solvent, only atom 0 carries LJ
+                          (nonbond_ww_spc / nonbond_qw_spc paths,
+                          src/potene.f90:347-356); 0: general loops     */
+  int32_t nexlong;     /* long (> QNB_MAX_NBR_RANGE) exclusions          */
+  int32_t n14long;     /* long 1-4 pairs                                 */
+
+  /* per atom [natom] */
+  const double *crg;     /* scaled charge                                  */
+  const int32_t *iac;    /* atom type, 0-based                             */
+  const int32_t *iqatom; /* 0 or 1-based Q-atom number                     */
+  const uint8_t *excl;   /* 1 = excluded atom (outside the sphere)         */
+  const uint8_t *heavy;  /* 1 = heavy atom (SCAAS radial acts on heavy
+                            solvent atoms, src/nonbondene.f90:6510-6540)   */
+
+  /* charge groups [ncgp]: atom ranges into cgpatom[], switch atom */
+  const int32_t *cgp_first;   /* index into cgpatom                        */
+  const int32_t *cgp_last;    /* index into cgpatom (inclusive)            */
+  const int32_t *cgp_iswitch; /* atom index                                */
+  const int32_t *cgpatom;     /* [cgp_last[ncgp-1]+1] atom indices          */
+
+  /* atom types (iaclib(t)%avdw(k), src/topo.f90:79-83): avdw/bvdw
+   * [ntypes][3], element (type t, LJ code k) at [t*3 + k], with code k = 0
+   * normal, 1 type-2, 2 the 1-4 parameters.  ljcod [ntypes][ntypes] holds the
+   * 1-based code (1 or 2) of a type pair at [ti + tj*ntypes]
+   * (src/globals.f90:136). */
+  const double *avdw;
+  const double *bvdw;
+  const int32_t *ljcod;
+  double el14_scale;
+
+  /* solute exclusions, the topology's own encoding (src/topo.f90:200-204):
+   * listex/list14 [nat_solute][QNB_MAX_NBR_RANGE]: element (k-1, i) at
+   * [i*25 + k-1] is 1 when pair (i, i+k) is excluded / is a 1-4 pair;
+   * listexlong/list14long [n][2] atom pairs at [2*m], [2*m+1]. */
+  const uint8_t *listex;
+  const uint8_t *list14;
+  const int32_t *listexlong;
+  const int32_t *list14long;
+
+  /* solute-solvent parameters: pw_* [solv_atom][nat_solute], element
+   * (solute atom i, solvent atom j) at [i + j*nat_solute]
+   * (pw_precomp(i-pw_low, j), src/simprep.f90:3069-3108, offset removed). */
+  const double *pw_A, *pw_B, *pw_el;
+  /* solvent-solvent: ww_* [solv_atom][solv_atom], (la, ka) at
+   * [la + ka*solv_atom] (ww_precomp, src/simprep.f90:3296-3317). */
+  const double *ww_A, *ww_B, *ww_el;
+
+  /* Q-atoms */
+  const int32_t *iqseq; /* [nqat] atom index of each Q-atom                */
+  /* qconn3 [nat_solute]: 1 when the solute atom is bonded or angled
+   * (qconn <= 3) to any Q-atom in any state, i.e. any(qconn(:,i,:) <= 3)
+   * (src/nonbondene.f90:3717); such atoms go on the Q-"bonded neighbour"
+   * list instead of the Q-solute list. */
+  const uint8_t *qconn3;
+  /* qp_* [nstates][nqat][nat_solute]: (i, iq, s) at [i + nat_solute*(iq +
+   * nqat*s)] (qp_precomp, src/simprep.f90:3112-3148, 3420-3461) */
+  const double *qp_A, *qp_B, *qp_el, *qp_sc;
+  const uint8_t *qp_set;
+  /* qq_* [nstates][nqat][nqat]: (iq, jq, s) at [iq + nqat*(jq + nqat*s)]
+   * (qq_precomp, src/simprep.f90:3152-3268) */
+  const double *qq_A, *qq_B, *qq_el, *qq_sc;
+  const uint8_t *qq_set, *qq_soft;
+  /* qw_* [nstates][solv_atom][nqat]: (iq, j, s) at [iq + nqat*(j +
+   * solv_atom*s)] (qw_precomp, src/simprep.f90:3272-3292) */
+  const double *qw_A, *qw_B, *qw_el, *qw_sc;
+} qnb_system;
+
+/* Run parameters: cutoffs (Angstrom, not squared), sphere, LRF, SCAAS. */
+typedef struct qnb_params {
+  double Rcpp, Rcpw, Rcww, Rcq, RcLRF; /* src/globals.f90:132            */
+  int32_t use_LRF;
+  int32_t precision;                   /* qnb_precision                  */
+  /* bit 0: with use_LRF = 0, reproduce nbpwlist's distance test exactly as
+   * the reference executes it: its charge-group loop reuses the switch-atom
+   * variable `ia`, so after the first water inside the cut-off the test
+   * point becomes atom cgp_last+1 (src/nonbondene.f90:2686-2691). */
+  int32_t ref_quirks;
+  double xpcent[3];                    /* sphere centre (Q-list origin)  */
+  double xwcent[3];                    /* water-sphere centre            */
+  double rexcl_o;                      /* exclusion radius (Q lists are
+                                          static when Rcq > rexcl_o)     */
+  double lambda[QNB_MAX_STATES];       /* EQ(s)%lambda                   */
+  /* SCAAS radial restraint (src/nonbondene.f90:6466-6544) */
+  double rwat, fk_wsphere, Dwmz, awmz, Boltz;
+  /* SCAAS polarisation restraint (src/nonbondene.f90:6548-6746) */
+  int32_t wpol_restr;
+  int32_t nwpolr_shell;                /* <= QNB_MAX_SHELLS              */
+  double fkwpol;
+  double itdis;                        /* theta_corr update cadence      */
+  double wshell_rout[8], wshell_dr[8], wshell_cstb[8];
+  double wshell_theta_corr[8];         /* initial (restart) values       */
+} qnb_params;
+
+#define QNB_MAX_SHELLS 8
+
+/* E%pp, E%pw, E%ww, E%LRF and the two SCAAS restraint terms
+ * (TYPE(ENERGIES), src/nrgy.f90:48-54). */
+typedef struct qnb_energies {
+  double pp_el, pp_vdw;
+  double pw_el, pw_vdw;
+  double ww_el, ww_vdw;
+  double lrf;
+  double solvent_radial, water_pol;
+} qnb_energies;
+
+/* Per-state Q-atom energies EQ(s)%qq, %qp, %qw (src/nrgy.f90:64-71). */
+typedef struct qnb_qenergies {
+  double qq_el, qq_vdw;
+  double qp_el, qp_vdw;
+  double qw_el, qw_vdw;
+} qnb_qenergies;
+
+typedef struct qnb_handle qnb_handle;
+
+/* Which list qnb_get_pairs exports (src/globals.f90:414-437). */
+enum qnb_list {
+  QNB_LIST_PP = 0,  /* nbpp  (i, j) solute-solute atom pairs              */
+  QNB_LIST_PW = 1,  /* nbpw  (i, j) solute atom - solvent atom            */
+  QNB_LIST_WW = 2,  /* nbww  as molecule pairs (iw, jw), 0-based waters   */
+  QNB_LIST_QP = 3,  /* nbqp  (iq, j) 0-based Q-atom number, solute atom   */
+  QNB_LIST_QW = 4,  /* nbqw  as (iq, iw) Q-atom number, 0-based water     */
+  QNB_LIST_QQ = 5,  /* nbqq  (iq, jq) of state 0                          */
+  QNB_LIST_QQP = 6  /* nbqqp (iq, j) of state 0                           */
+};
+
+int32_t qnb_abi_version(void);
+const char *qnb_last_error(void);
+/* Number of visible devices (0 when no GPU). */
+int32_t qnb_device_count(void);
+
+/* Copies the system to device memory and prepares static lists (Q-Q and
+ * Q-bonded-neighbour lists: make_nbqqlist + nbqqlist,
+ * src/nonbondene.f90:729-745, 3231-3282).  Call once after Qdyn6's
+ * precompute_interactions (src/qdyn.f90:153). */
+int32_t qnb_create(const qnb_system *sys, const qnb_params *par, qnb_handle **out);
+int32_t qnb_destroy(qnb_handle *h);
+
+/* make_pair_lists (src/nonbondene.f90:749-837), spherical boundary,
+ * switch-atom lists: charge-group centres, solute-solute, solute-water,
+ * water-water, Q-solute and Q-water lists, and the LRF expansion of every
+ * charge-group pair beyond the cut-off when use_LRF.  x: natom*3 (host). */
+int32_t qnb_make_pair_lists(qnb_handle *h, const double *x);
+
+/* pot_energy_nonbonds (src/potene.f90:320-380) for in_md = .true.:
+ * nonbond_ww(_spc), nonbond_qw(_spc), nonbond_pw, nonbond_pp, nonbond_qp
+ * and lrf_taylor.  Adds the gradient into d (natom*3, host) and writes
+ * E (pp, pw, ww, lrf) and EQ[nstates] (qp, qw). */
+int32_t qnb_nonbonded(qnb_handle *h, const double *x, double *d, qnb_energies *E, qnb_qenergies *EQ);
+
+/* nonbond_qq + nonbond_qqp (src/nonbondene.f90:5013-5153): adds into d and
+ * writes EQ[s].qq and adds the Q-bonded-neighbour terms to EQ[s].qp. */
+int32_t qnb_qq(qnb_handle *h, const double *x, double *d, qnb_qenergies *EQ);
+
+/* restrain_solvent + watpol (src/nonbondene.f90:6466-6746) for one MD step:
+ * Tfree is the previous step's free temperature (src/md.f90:1803); istep
+ * drives the theta_corr update cadence.  Adds into d, writes
+ * E->solvent_radial and E->water_pol. */
+int32_t qnb_restraints(qnb_handle *h, const double *x, double *d, double Tfree, int32_t istep, qnb_energies *E);
+
+/* Term selection for qnb_eval_terms: one bit per reference routine. */
+enum qnb_term {
+  QNB_T_WW = 1 << 0,  /* nonbond_ww_spc / nonbond_ww   src/nonbondene.f90:5886, 5509 */
+  QNB_T_QW = 1 << 1,  /* nonbond_qw_spc / nonbond_qw   :5658, 5339 */
+  QNB_T_PW = 1 << 2,  /* nonbond_pw                    :4856 */
+  QNB_T_PP = 1 << 3,  /* nonbond_pp                    :4694 */
+  QNB_T_QP = 1 << 4,  /* nonbond_qp                    :5157 */
+  QNB_T_LRF = 1 << 5, /* lrf_taylor                    :507  */
+  QNB_T_QQ = 1 << 6,  /* nonbond_qq                    :5013 */
+  QNB_T_QQP = 1 << 7, /* nonbond_qqp                   :5085 */
+  QNB_T_RAD = 1 << 8, /* restrain_solvent              :6466 */
+  QNB_T_POL = 1 << 9  /* watpol                        :6548 */
+};
+#define QNB_T_NONBONDED (QNB_T_WW | QNB_T_QW | QNB_T_PW | QNB_T_PP | QNB_T_QP | QNB_T_LRF)
+
+/* Evaluates the selected terms (mask of qnb_term) at x, adds their gradient
+ * into d and writes their energies; unselected fields are left untouched.
+ * Tfree/istep are used by the restraint terms only. */
+int32_t qnb_eval_terms(qnb_handle *h, uint32_t mask, const double *x, double *d, double Tfree, int32_t istep,
+                       qnb_energies *E, qnb_qenergies *EQ);
+
+/* Exports the current list `which` as n (a, b) int32 pairs into ab[2*cap];
+ * *n receives the length (also when it exceeds cap, then nothing is copied). */
+int32_t qnb_get_pairs(qnb_handle *h, int32_t which, int32_t *ab, int64_t cap, int64_t *n);
+
+/* LRF table after the last list update: [ncgp][43] float64 in the order
+ * centre(3), phi0, phi1(3), phi2(3x3), phi3(9x3) of LRF_TYPE
+ * (src/globals.f90:503-509). */
+int32_t qnb_get_lrf(qnb_handle *h, double *lrf);
+
+/* Polarisation restraint state that survives a restart (theta_corr,
+ * src/qalloc.f90:840-871) plus the running averages: [3*nwpolr_shell] as
+ * theta_corr[], avtheta[], avn_insh[]. */
+int32_t qnb_get_shell_state(qnb_handle *h, double *state);
+int32_t qnb_set_shell_state(qnb_handle *h, const double *state);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QNB_H */

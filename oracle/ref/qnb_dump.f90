@@ -153,6 +153,29 @@ program qnb_dump
     call wd(u, 'ww_el', n, reshape(ww_precomp%elec, (/n/)), 2, (/solv_atom, solv_atom/))
   end if
 
+  ! solute-solvent table pw_precomp(i-pw_low, j) spread to all solute atoms
+  if (nwat > 0 .and. nat_solute > 0 .and. allocated(pw_precomp)) then
+    deallocate(dtmp)
+    allocate(dtmp(3*nat_solute*solv_atom))
+    dtmp(:) = zero
+    do k = 1, solv_atom
+      do i = 1, nat_solute
+        if (i - pw_low >= 1 .and. i - pw_low <= size(pw_precomp, 1)) then
+          n = i + nat_solute*(k-1)
+          dtmp(n) = pw_precomp(i-pw_low, k)%vdWA
+          dtmp(n + nat_solute*solv_atom) = pw_precomp(i-pw_low, k)%vdWB
+          dtmp(n + 2*nat_solute*solv_atom) = pw_precomp(i-pw_low, k)%elec
+        end if
+      end do
+    end do
+    n = nat_solute*solv_atom
+    call wd(u, 'pw_A', n, dtmp(1:n), 2, (/nat_solute, solv_atom/))
+    call wd(u, 'pw_B', n, dtmp(n+1:2*n), 2, (/nat_solute, solv_atom/))
+    call wd(u, 'pw_el', n, dtmp(2*n+1:3*n), 2, (/nat_solute, solv_atom/))
+    deallocate(dtmp)
+    allocate(dtmp(3*natom))
+  end if
+
   ! ---------------- Q-atom tables ----------------
   nq = nqat
   ns = nstates

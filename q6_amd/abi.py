@@ -1,0 +1,135 @@
+"""ctypes mirror of include/qnb.h and loader of the HIP engine library.
+
+The engine is the in-tree shared library q6_amd/libq6nb.so (built by
+__graft_entry__.build()).  Loading fails loudly when it is missing: there is no
+CPU fallback in the product path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libq6nb.so")
+
+QNB_MAX_STATES = 8
+QNB_MAX_SHELLS = 8
+QNB_MAX_NBR_RANGE = 25
+
+QNB_PREC_FP64 = 0
+QNB_PREC_MIXED = 1
+
+LIST_PP, LIST_PW, LIST_WW, LIST_QP, LIST_QW, LIST_QQ, LIST_QQP = range(7)
+
+T_WW, T_QW, T_PW, T_PP, T_QP, T_LRF, T_QQ, T_QQP, T_RAD, T_POL = (1 << k for k in range(10))
+T_NONBONDED = T_WW | T_QW | T_PW | T_PP | T_QP | T_LRF
+T_ALL = (1 << 10) - 1
+
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int32)
+_pu8 = C.POINTER(C.c_uint8)
+
+
+class QnbSystem(C.Structure):
+    _fields_ = [
+        ("natom", C.c_int32), ("nat_solute", C.c_int32), ("nwat", C.c_int32), ("solv_atom", C.c_int32),
+        ("ncgp", C.c_int32), ("ncgp_solute", C.c_int32), ("nqat", C.c_int32), ("nstates", C.c_int32),
+        ("ntypes", C.c_int32), ("ivdw_rule", C.c_int32), ("solvent_spc", C.c_int32),
+        ("nexlong", C.c_int32), ("n14long", C.c_int32),
+        ("crg", _pd), ("iac", _pi), ("iqatom", _pi), ("excl", _pu8), ("heavy", _pu8),
+        ("cgp_first", _pi), ("cgp_last", _pi), ("cgp_iswitch", _pi), ("cgpatom", _pi),
+        ("avdw", _pd), ("bvdw", _pd), ("ljcod", _pi), ("el14_scale", C.c_double),
+        ("listex", _pu8), ("list14", _pu8), ("listexlong", _pi), ("list14long", _pi),
+        ("pw_A", _pd), ("pw_B", _pd), ("pw_el", _pd),
+        ("ww_A", _pd), ("ww_B", _pd), ("ww_el", _pd),
+        ("iqseq", _pi), ("qconn3", _pu8),
+        ("qp_A", _pd), ("qp_B", _pd), ("qp_el", _pd), ("qp_sc", _pd), ("qp_set", _pu8),
+        ("qq_A", _pd), ("qq_B", _pd), ("qq_el", _pd), ("qq_sc", _pd), ("qq_set", _pu8), ("qq_soft", _pu8),
+        ("qw_A", _pd), ("qw_B", _pd), ("qw_el", _pd), ("qw_sc", _pd),
+    ]
+
+
+class QnbParams(C.Structure):
+    _fields_ = [
+        ("Rcpp", C.c_double), ("Rcpw", C.c_double), ("Rcww", C.c_double), ("Rcq", C.c_double),
+        ("RcLRF", C.c_double), ("use_LRF", C.c_int32), ("precision", C.c_int32), ("ref_quirks", C.c_int32),
+        ("xpcent", C.c_double * 3), ("xwcent", C.c_double * 3), ("rexcl_o", C.c_double),
+        ("lambda_", C.c_double * QNB_MAX_STATES),
+        ("rwat", C.c_double), ("fk_wsphere", C.c_double), ("Dwmz", C.c_double), ("awmz", C.c_double),
+        ("Boltz", C.c_double),
+        ("wpol_restr", C.c_int32), ("nwpolr_shell", C.c_int32), ("fkwpol", C.c_double), ("itdis", C.c_double),
+        ("wshell_rout", C.c_double * QNB_MAX_SHELLS), ("wshell_dr", C.c_double * QNB_MAX_SHELLS),
+        ("wshell_cstb", C.c_double * QNB_MAX_SHELLS), ("wshell_theta_corr", C.c_double * QNB_MAX_SHELLS),
+    ]
+
+
+class QnbEnergies(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "pp_el", "pp_vdw", "pw_el", "pw_vdw", "ww_el", "ww_vdw", "lrf", "solvent_radial", "water_pol")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class QnbQEnergies(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("qq_el", "qq_vdw", "qp_el", "qp_vdw", "qw_el", "qw_vdw")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+# every symbol include/qnb.h declares (checked by the CPU test suite)
+EXPORTS = [
+    "qnb_abi_version", "qnb_last_error", "qnb_device_count", "qnb_create", "qnb_destroy",
+    "qnb_make_pair_lists", "qnb_nonbonded", "qnb_qq", "qnb_restraints", "qnb_eval_terms",
+    "qnb_get_pairs", "qnb_get_lrf", "qnb_get_shell_state", "qnb_set_shell_state",
+]
+
+_lib = None
+
+
+def _declare(lib):
+    vp = C.c_void_p
+    lib.qnb_abi_version.restype = C.c_int32
+    lib.qnb_last_error.restype = C.c_char_p
+    lib.qnb_device_count.restype = C.c_int32
+    lib.qnb_create.argtypes = [C.POINTER(QnbSystem), C.POINTER(QnbParams), C.POINTER(vp)]
+    lib.qnb_destroy.argtypes = [vp]
+    lib.qnb_make_pair_lists.argtypes = [vp, _pd]
+    lib.qnb_nonbonded.argtypes = [vp, _pd, _pd, C.POINTER(QnbEnergies), C.POINTER(QnbQEnergies)]
+    lib.qnb_qq.argtypes = [vp, _pd, _pd, C.POINTER(QnbQEnergies)]
+    lib.qnb_restraints.argtypes = [vp, _pd, _pd, C.c_double, C.c_int32, C.POINTER(QnbEnergies)]
+    lib.qnb_eval_terms.argtypes = [vp, C.c_uint32, _pd, _pd, C.c_double, C.c_int32,
+                                   C.POINTER(QnbEnergies), C.POINTER(QnbQEnergies)]
+    lib.qnb_get_pairs.argtypes = [vp, C.c_int32, _pi, C.c_int64, C.POINTER(C.c_int64)]
+    lib.qnb_get_lrf.argtypes = [vp, _pd]
+    lib.qnb_get_shell_state.argtypes = [vp, _pd]
+    lib.qnb_set_shell_state.argtypes = [vp, _pd]
+    for n in EXPORTS:
+        if n not in ("qnb_abi_version", "qnb_last_error", "qnb_device_count"):
+            getattr(lib, n).restype = C.c_int32
+    return lib
+
+
+def load(path=None):
+    """Load the engine library; raises when it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"HIP engine library not built: {p} (run __graft_entry__.build())")
+    lib = _declare(C.CDLL(p))
+    if lib.qnb_abi_version() != 1:
+        raise RuntimeError("qnb ABI version mismatch")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class QnbError(RuntimeError):
+    pass
+
+
+def check(lib, rc, what):
+    if rc != 0:
+        msg = lib.qnb_last_error()
+        raise QnbError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")

@@ -1,0 +1,82 @@
+"""The C restatement (oracle/) against golden vectors of the reference itself.
+
+The fixtures were written by stock Qdyn6 modules built from the reference
+sources (tests/golden/make_golden.py).  Every list, the LRF table, and the
+energy and gradient of every term must agree BIT FOR BIT: the restatement
+follows the reference's loop and summation order exactly.
+"""
+import numpy as np
+import pytest
+
+from conftest import CASES, golden_path
+from oracle.oracle import Oracle
+from q6_amd import abi
+from q6_amd.system import from_golden
+
+TERMS = [("ww", abi.T_WW), ("qw", abi.T_QW), ("pw", abi.T_PW), ("pp", abi.T_PP), ("qp", abi.T_QP),
+         ("lrf", abi.T_LRF), ("qq", abi.T_QQ), ("qqp", abi.T_QQP), ("solvent_radial", abi.T_RAD),
+         ("water_pol", abi.T_POL)]
+
+
+def term_energy(t, E, EQ):
+    if t in ("ww", "pw", "pp"):
+        return [E[t + "_el"], E[t + "_vdw"]]
+    if t in ("qw", "qp", "qq"):
+        return sum([[q[t + "_el"], q[t + "_vdw"]] for q in EQ], [])
+    if t == "qqp":
+        return sum([[q["qp_el"], q["qp_vdw"]] for q in EQ], [])
+    return [E[t]]
+
+
+@pytest.fixture(scope="module", params=CASES)
+def case(request):
+    s = from_golden(golden_path(request.param))
+    o = Oracle(s, ref_quirks=1)
+    o.make_pair_lists(s.x)
+    return request.param, s, o
+
+
+def test_known_answer_qsurr():
+    # tests/basic_tests/eval_test.sh:97-101 of the reference: "Q-surr. 1 1.0000 3.12 139.43"
+    s = from_golden(golden_path("lig_eq1"))
+    o = Oracle(s)
+    o.make_pair_lists(s.x)
+    _, _, EQ = o.eval_terms(abi.T_QW | abi.T_QP, s.x)
+    assert "%.2f %.2f" % (EQ[0]["qw_el"] + EQ[0]["qp_el"], EQ[0]["qw_vdw"] + EQ[0]["qp_vdw"]) == "3.12 139.43"
+
+
+@pytest.mark.parametrize("which,key", [(abi.LIST_PP, "nbpp_ij"), (abi.LIST_PW, "nbpw_ij"),
+                                       (abi.LIST_WW, "ww_mol_pairs"), (abi.LIST_QP, "nbqp_ij")])
+def test_lists_exact_order(case, which, key):
+    name, s, o = case
+    ref = s.ref[key].T
+    if key.startswith("nb"):
+        ref = ref - 1
+    got = o.pairs(which)
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)  # same pairs, same orientation, same append order
+
+
+def test_list_parameters(case):
+    name, s, o = case
+    par = o.list_params(abi.LIST_PP, 3)
+    assert np.array_equal(par.T, s.ref["nbpp_par"].reshape(3, -1))
+    par = o.list_params(abi.LIST_PW, 3)
+    assert np.array_equal(par.T, s.ref["nbpw_par"].reshape(3, -1))
+
+
+def test_lrf_table_exact(case):
+    name, s, o = case
+    if not s.par["use_LRF"]:
+        pytest.skip("LRF off")
+    assert np.array_equal(o.lrf(), s.ref["lrf"].T)
+
+
+@pytest.mark.parametrize("term,mask", TERMS)
+def test_terms_exact(case, term, mask):
+    name, s, o = case
+    if "d_" + term not in s.ref:
+        pytest.skip("term not dumped for this case")
+    d, E, EQ = o.eval_terms(mask, s.x, Tfree=s.Tfree, istep=0)
+    assert np.array_equal(d, s.ref["d_" + term].T)
+    assert np.array_equal(np.array(term_energy(term, E, EQ)), np.atleast_1d(s.ref["E_" + term]))
