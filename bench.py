@@ -1,0 +1,184 @@
+"""Benchmark: batched mixed-precision lambda windows vs the plain FP64 path.
+
+Workload (the metric's headline case): 51 lambda windows (lambda_1 = 0, 0.02,
+..., 1; lambda_2 = 1 - lambda_1) of the 30 A sphere system (400 ten-atom
+solute charge groups, 30 Q-atoms in 2 EVB states, TIP3P), each window with its
+own seeded Maxwell-Boltzmann velocities at 300 K, 1 fs steps, pair-list / LRF
+update every 10 steps, all windows advanced in the same kernel launches with
+FP32 pair arithmetic and FP64 accumulation, one HIP graph per list cycle.
+Reported next to it: the plain FP64 GPU path running one window at a time
+(51 windows run sequentially give that same aggregate ns/day), and the C
+oracle (single thread) on a bounded sample of the same system.
+
+A "step" is one MD step of every window: nonbonded engine (forces, per-state
+Q energies, LRF Taylor term, SCAAS restraints; lists + LRF expansion every
+10th step), solute bonds/angles, leap-frog, SHAKE of the waters, Berendsen.
+ns/day = steps/s x dt(fs) x 0.0864, summed over windows.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X peaks (vendor figures): FP64 vector 78.6 TFLOP/s, FP32 vector 157.3 TFLOP/s
+PEAK_FP64, PEAK_FP32 = 78.6, 157.3
+# algorithmic FLOPs per unit (div and sqrt count as one)
+F_LRF = 89        # LRF atom-centre update as restated in the engine (20 independent components)
+F_TEST = 8        # group-pair distance test of the list sweep
+F_LJC, F_COUL = 33, 22
+F_Q_SHARED, F_Q_STATE, F_QH_STATE = 12, 27, 9
+
+
+def lrf_units(s, x):
+    """Atom-centre updates and pair tests of one list update (host count from
+    positions: every receiver group gets every non-Q source atom of the groups
+    beyond its class cut-off)."""
+    n = s.n
+    ncs, ng = n["ncgp_solute"], n["ncgp"]
+    ref = np.concatenate([s.a["cgp_iswitch"][:ncs], n["nat_solute"] + 3 * np.arange(n["nwat"])]).astype(int)
+    p = x[ref]
+    nonq = np.array([sum(1 for k in range(s.a["cgp_first"][g], s.a["cgp_last"][g] + 1)
+                         if s.a["iqatom"][s.a["cgpatom"][k]] == 0) for g in range(ng)])
+    sol = np.arange(ng) < ncs
+    units = 0
+    for a in range(ng):
+        r2 = ((p - p[a]) ** 2).sum(1)
+        cut = np.where(sol & sol[a], s.par["Rcpp"] ** 2, np.where(~sol & ~sol[a], s.par["Rcww"] ** 2, s.par["Rcpw"] ** 2))
+        far = (r2 > cut) & (r2 <= s.par["RcLRF"] ** 2)
+        far[a] = False
+        units += int(nonq[far].sum())
+    return units, ng * ng
+
+
+def force_flops(counts, ns):
+    ww_mol = counts["ww"] // 9
+    return (ww_mol * (F_LJC + 8 * F_COUL) + (counts["pp"] + counts["pw"]) * F_LJC
+            + counts["qp"] * (F_Q_SHARED + F_Q_STATE * ns)
+            + (counts["qw"] // 3) * ((F_Q_SHARED + F_Q_STATE * ns) + 2 * (F_Q_SHARED + F_QH_STATE * ns))
+            + (counts["qq"] + counts["qqp"]) * (F_Q_SHARED + F_Q_STATE))
+
+
+def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True):
+    from q6_amd.md import BatchMD
+    b = BatchMD(systems, lambdas, vels, precision=precision, nbcycle=10, ene_every=10,
+                max_frames=(warmup + steps) // 10 + 2)
+    b.run(warmup, graph=graph)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(steps, graph=graph)
+    b.sync()
+    dt = time.perf_counter() - t0
+    return b, dt
+
+
+def cpu_baseline(s, ncycles=1):
+    """Oracle (C restatement, 1 thread) on one window: one list/LRF update and
+    10 full evaluations per cycle = 10 steps of the hot path."""
+    from oracle.oracle import Oracle
+    from q6_amd import abi
+    o = Oracle(s)
+    t0 = time.perf_counter()
+    for _ in range(ncycles):
+        o.make_pair_lists(s.x)
+        for k in range(10):
+            o.eval_terms(abi.T_ALL, s.x, Tfree=300.0, istep=k)
+    dt = time.perf_counter() - t0
+    steps = 10 * ncycles
+    return steps / dt, dt, steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--windows", type=int, default=51)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: lambda windows shard across GPUs with no per-step exchange "
+              "(see DESIGN.md); only --gpus 1 is measured.", file=sys.stderr)
+        sys.exit(2)
+    if args.steps % 10 or args.warmup % 10:
+        sys.exit("--steps and --warmup must be multiples of the 10-step list cycle")
+
+    from q6_amd import abi, synth
+    s = synth.cfg2()
+    nw = args.windows
+    lam1 = np.linspace(0.0, 1.0, nw) if nw > 1 else np.array([0.5])
+    lambdas = [np.array([l, 1.0 - l]) for l in lam1]
+    vels = [synth.maxwell_velocities(s, 300.0, 100 + w) for w in range(nw)]
+    print(f"# {synth.describe(s)}", file=sys.stderr, flush=True)
+
+    # batched mixed precision, all windows in one launch per kernel, graphs
+    bm, tm = run_md([s] * nw, lambdas, vels, abi.QNB_PREC_MIXED, args.steps, args.warmup)
+    steps_per_s = args.steps / tm
+    ns_day = nw * steps_per_s * 1.0 * 0.0864
+    counts = bm.pair_counts(0)
+    pairs_step = s.pair_count_per_step(counts)
+    print(f"# batched mixed: {args.steps} steps x {nw} windows in {tm:.3f} s -> {ns_day:.1f} ns/day aggregate; "
+          f"pairs/step/window {pairs_step}", file=sys.stderr, flush=True)
+    # kernel timings on the same stream (HIP events), same batch
+    t_lists = bm.time_kernel(0, 20)
+    t_forces = bm.time_kernel(1, 20)
+    e = bm.energies(0)
+    temps = e[:, abi.F_TEMP]
+    bm.close()
+
+    units, tests = lrf_units(s, s.x)
+    lists_flop = nw * (units * F_LRF + tests * F_TEST)
+    forces_flop = nw * force_flops(counts, int(s.n["nstates"]))
+    share_lists = t_lists / 10.0
+    if share_lists >= t_forces:
+        dom, t_dom, flop_dom = "k_lists_lrf", t_lists, lists_flop
+    else:
+        dom, t_dom, flop_dom = "k_forces", t_forces, forces_flop
+    achieved = flop_dom / (t_dom * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP32,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4), "traffic": None,
+                "ms_per_launch": {"k_lists_lrf": round(t_lists, 4), "k_forces": round(t_forces, 4)},
+                "flop_per_launch": {"k_lists_lrf": lists_flop, "k_forces": forces_flop}}
+
+    plain = None
+    if not args.no_plain:
+        bp, tp = run_md([s], [lambdas[nw // 2]], [vels[nw // 2]], abi.QNB_PREC_FP64, args.steps, args.warmup)
+        plain = args.steps / tp * 1.0 * 0.0864
+        bp.close()
+        print(f"# plain FP64, one window: {args.steps} steps in {tp:.3f} s -> {plain:.1f} ns/day", file=sys.stderr,
+              flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        sps, tc, nsteps = cpu_baseline(s, ncycles=5)
+        cpu = {"value": round(sps * 1.0 * 0.0864, 4), "unit": "ns/day", "cores": 1, "kind": "port",
+               "sample": f"oracle (C restatement, 1 thread) on one window of the same system: 1 list/LRF update + "
+                         f"{nsteps} full nonbonded+restraint evaluations ({tc:.1f} s); integration not included"}
+
+    out = {
+        "metric": "ns/day + pair-interactions/s, batched mixed-prec lambda-windows vs plain FP64, 1 GPU",
+        "value": round(ns_day, 2), "unit": "ns/day", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(tm / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32-pair/fp64-accumulate", "data": "synthetic (seeded sphere, random init)",
+        "config": {"workload": f"{nw} lambda windows of a 30 A sphere: {s.n['natom']} atoms ({s.n['nat_solute']} solute "
+                               f"in 400 ten-atom groups, {s.n['nwat']} TIP3P), 30 Q-atoms x 2 EVB states, "
+                               f"1 fs, lists/LRF every 10 steps",
+                   "windows": nw, "natom": int(s.n["natom"]), "precision": "mixed", "graph": True},
+        "pair_interactions_per_s": round(nw * pairs_step * steps_per_s, 1),
+        "plain_fp64_ns_day": None if plain is None else round(plain, 2),
+        "speedup_vs_plain_fp64": None if plain is None else round(ns_day / plain, 2),
+        "temperature_range_K": [round(float(temps.min()), 1), round(float(temps.max()), 1)] if len(temps) else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -265,6 +265,73 @@ int32_t qnb_get_lrf(qnb_handle *h, double *lrf);
 int32_t qnb_get_shell_state(qnb_handle *h, double *state);
 int32_t qnb_set_shell_state(qnb_handle *h, const double *state);
 
+/* ------------------------------------------------------------------------
+ * Device-resident MD over a batch of instances (lambda windows of one system,
+ * or independent systems of uneven size).  The step follows md_run
+ * (src/md.f90:1657-1871) for leap-frog + Berendsen: nonbonded engine (list
+ * update every nbcycle steps), solute bonds and angles, newvel_ber
+ * (src/md.f90:1451-1466), SHAKE of the rigid waters (src/bondene.f90:
+ * 1069-1151), v = (x - xx)/dt, temperature() (src/md.f90:1379-1447) which
+ * gives the next step's scaling factor and Tfree.  All instances advance in
+ * the same kernel launches; a list-update cycle is captured once as a HIP
+ * graph and replayed.  Energies are sampled on the device every ene_every
+ * steps into a per-instance history. */
+typedef struct qnb_md_params {
+  double dt_fs;       /* time step in fs; internal dt = 0.020462*dt_fs (src/md.f90:207) */
+  double T0;          /* bath temperature (K)                                   */
+  double tau_T_fs;    /* Berendsen coupling time (fs)                           */
+  int32_t nbcycle;    /* pair-list update interval (NBcycle)                    */
+  int32_t ene_every;  /* energy sampling interval (iene_cycle)                  */
+  int32_t max_frames; /* capacity of the energy history                         */
+  int32_t nbonds, nangles;
+  const int32_t *bond_ij;   /* [nbonds][2]                                        */
+  const double *bond_r0, *bond_k;     /* E = k/2 (r - r0)^2 (src/bondene.f90:182-244) */
+  const int32_t *angle_ijk; /* [nangles][3], j the vertex                          */
+  const double *angle_t0, *angle_k;   /* radians; E = k/2 (t - t0)^2            */
+  const double *winv;       /* [natom] inverse masses                            */
+  double water_roh, water_rhh; /* rigid 3-site water geometry (SHAKE constraints) */
+} qnb_md_params;
+
+/* per-frame record of the energy history (float64 each) */
+#define QNB_FRAME_LEN 72
+enum qnb_frame_field {
+  QNB_F_PP_EL = 0, QNB_F_PP_VDW, QNB_F_PW_EL, QNB_F_PW_VDW, QNB_F_WW_EL, QNB_F_WW_VDW, QNB_F_LRF,
+  QNB_F_RAD, QNB_F_POL,
+  QNB_F_Q0 = 9,          /* per state s: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw at 9 + 6s (8 states) */
+  QNB_F_BOND = 57, QNB_F_ANGLE, QNB_F_EKIN, QNB_F_TEMP, QNB_F_TFREE, QNB_F_STEP
+};
+
+typedef struct qnb_batch qnb_batch;
+
+/* n instances; instance k runs system sys[k] with params par[k] (its own
+ * lambda, SCAAS constants) and MD params md[k] from x0[k], v0[k]
+ * (natom*3 each, velocities in A per internal time unit).  Instances whose
+ * sys[k] pointers are equal share one device copy of the topology. */
+int32_t qnb_batch_create(int32_t n, const qnb_system *const *sys, const qnb_params *const *par,
+                         const qnb_md_params *const *md, const double *const *x0, const double *const *v0,
+                         qnb_batch **out);
+int32_t qnb_batch_destroy(qnb_batch *b);
+/* Advances every instance by nsteps MD steps (device-resident).  With
+ * use_graph, each list-update cycle of nbcycle steps is one HIP-graph launch. */
+int32_t qnb_batch_run(qnb_batch *b, int32_t nsteps, int32_t use_graph);
+/* Waits for queued work; returns QNB_ENONFINITE / QNB_EOVERFLOW if an instance
+ * went non-finite, a constraint failed or a list overflowed. */
+int32_t qnb_batch_sync(qnb_batch *b);
+int32_t qnb_batch_get_state(qnb_batch *b, int32_t k, double *x, double *v);
+/* Copies instance k's energy history (nframes x QNB_FRAME_LEN float64). */
+int32_t qnb_batch_energy_history(qnb_batch *b, int32_t k, double *out, int32_t cap, int32_t *nframes);
+/* Pair interactions of instance k's current lists, counted as the reference's
+ * explicit lists would hold them: out[0..6] = nbpp, nbpw, nbww, nbqp, nbqw
+ * (atom pairs, one state), nbqq, nbqqp (summed over states). */
+int32_t qnb_batch_pair_counts(qnb_batch *b, int32_t k, int64_t *out);
+
+/* Times one kernel of the step on the batch's stream with HIP events: which =
+ * 0 the list/LRF update (k_lists_lrf), 1 the receiver-side force kernel
+ * (k_forces); the kernel is launched `reps` times over all instances (both
+ * recompute their outputs from the current coordinates, so the state does not
+ * change) and *ms_per_launch receives the average launch duration. */
+int32_t qnb_batch_time_kernel(qnb_batch *b, int32_t which, int32_t reps, double *ms_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

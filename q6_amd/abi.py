@@ -76,11 +76,27 @@ class QnbQEnergies(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class QnbMdParams(C.Structure):
+    _fields_ = [
+        ("dt_fs", C.c_double), ("T0", C.c_double), ("tau_T_fs", C.c_double),
+        ("nbcycle", C.c_int32), ("ene_every", C.c_int32), ("max_frames", C.c_int32),
+        ("nbonds", C.c_int32), ("nangles", C.c_int32),
+        ("bond_ij", _pi), ("bond_r0", _pd), ("bond_k", _pd),
+        ("angle_ijk", _pi), ("angle_t0", _pd), ("angle_k", _pd),
+        ("winv", _pd), ("water_roh", C.c_double), ("water_rhh", C.c_double),
+    ]
+
+
+QNB_FRAME_LEN = 72
+F_Q0, F_BOND, F_ANGLE, F_EKIN, F_TEMP, F_TFREE, F_STEP = 9, 57, 58, 59, 60, 61, 62
+
 # every symbol include/qnb.h declares (checked by the CPU test suite)
 EXPORTS = [
     "qnb_abi_version", "qnb_last_error", "qnb_device_count", "qnb_create", "qnb_destroy",
     "qnb_make_pair_lists", "qnb_nonbonded", "qnb_qq", "qnb_restraints", "qnb_eval_terms",
     "qnb_get_pairs", "qnb_get_lrf", "qnb_get_shell_state", "qnb_set_shell_state",
+    "qnb_batch_create", "qnb_batch_destroy", "qnb_batch_run", "qnb_batch_sync", "qnb_batch_get_state",
+    "qnb_batch_energy_history", "qnb_batch_pair_counts", "qnb_batch_time_kernel",
 ]
 
 _lib = None
@@ -103,6 +119,15 @@ def _declare(lib):
     lib.qnb_get_lrf.argtypes = [vp, _pd]
     lib.qnb_get_shell_state.argtypes = [vp, _pd]
     lib.qnb_set_shell_state.argtypes = [vp, _pd]
+    pp = C.POINTER(C.c_void_p)
+    lib.qnb_batch_create.argtypes = [C.c_int32, pp, pp, pp, pp, pp, C.POINTER(vp)]
+    lib.qnb_batch_destroy.argtypes = [vp]
+    lib.qnb_batch_run.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.qnb_batch_sync.argtypes = [vp]
+    lib.qnb_batch_get_state.argtypes = [vp, C.c_int32, _pd, _pd]
+    lib.qnb_batch_energy_history.argtypes = [vp, C.c_int32, _pd, C.c_int32, C.POINTER(C.c_int32)]
+    lib.qnb_batch_pair_counts.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
+    lib.qnb_batch_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     for n in EXPORTS:
         if n not in ("qnb_abi_version", "qnb_last_error", "qnb_device_count"):
             getattr(lib, n).restype = C.c_int32

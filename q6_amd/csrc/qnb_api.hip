@@ -1,0 +1,676 @@
+// qnb_api.hip -- C ABI of the engine (include/qnb.h): handle, validation,
+// device tables, launches and the reference-order export of pair lists.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qnb_host.h"
+
+namespace qnb {
+template <typename R>
+__global__ void k_lists_lrf(const DevSys*);
+__global__ void k_qlists(const DevSys*);
+template <typename R, int NS>
+__global__ void k_forces(const DevSys*, uint32_t, const double*);
+template <typename R, int NS>
+__global__ void k_qforces(const DevSys*, uint32_t, int);
+template <typename R>
+__global__ void k_qq(const DevSys*, uint32_t);
+__global__ void k_watpol(const DevSys*, const int*);
+__global__ void k_reduce_energies(const DevSys*, int, int);
+
+static thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+int validate_system(const qnb_system* s, const qnb_params* p) {
+  if (!s || !p) return fail(QNB_EINVAL, "null system or params");
+  if (s->natom <= 0 || s->nat_solute < 0 || s->nat_solute > s->natom) return fail(QNB_EINVAL, "bad atom counts");
+  if (s->nstates < 1 || s->nstates > QNB_MAX_STATES) return fail(QNB_EINVAL, "nstates must be 1..%d", QNB_MAX_STATES);
+  if (s->nwat > 0 && s->solv_atom != 3)
+    return fail(QNB_EUNSUPPORTED, "only 3-site solvent is supported (solv_atom=%d)", s->solv_atom);
+  if (s->nat_solute + s->nwat * s->solv_atom != s->natom) return fail(QNB_EINVAL, "natom != nat_solute + nwat*solv_atom");
+  if (s->ncgp != s->ncgp_solute + s->nwat) return fail(QNB_EINVAL, "each solvent molecule must be one charge group");
+  if (p->nwpolr_shell < 0 || p->nwpolr_shell > QNB_MAX_SHELLS) return fail(QNB_EINVAL, "nwpolr_shell out of range");
+  if (!p->use_LRF && (p->ref_quirks & 1) && s->nwat > 0 && s->ncgp_solute > 0)
+    return fail(QNB_EUNSUPPORTED, "the nbpwlist index-reuse reproduction is only available in the oracle");
+  if (s->ntypes <= 0) return fail(QNB_EINVAL, "no atom types");
+  for (int i = 0; i < s->natom; i++)
+    if (s->iac[i] < 0 || s->iac[i] >= s->ntypes) return fail(QNB_EINVAL, "atom %d has type %d out of range", i, s->iac[i]);
+  for (int g = 0; g < s->ncgp; g++) {
+    if (s->cgp_first[g] > s->cgp_last[g]) return fail(QNB_EINVAL, "empty charge group %d", g);
+    if (g >= s->ncgp_solute) {
+      int w = g - s->ncgp_solute, o = s->nat_solute + s->solv_atom * w;
+      if (s->cgp_last[g] - s->cgp_first[g] + 1 != s->solv_atom || s->cgp_iswitch[g] != o)
+        return fail(QNB_EINVAL, "solvent group %d is not molecule %d", g, w);
+      for (int k = 0; k < s->solv_atom; k++)
+        if (s->cgpatom[s->cgp_first[g] + k] != o + k) return fail(QNB_EINVAL, "solvent group %d atom order", g);
+    } else {
+      for (int k = s->cgp_first[g]; k <= s->cgp_last[g]; k++)
+        if (s->cgpatom[k] < 0 || s->cgpatom[k] >= s->nat_solute)
+          return fail(QNB_EINVAL, "solute group %d holds a non-solute atom", g);
+    }
+  }
+  for (int i = s->nat_solute; i < s->natom; i++)
+    if (s->iqatom[i] != 0) return fail(QNB_EUNSUPPORTED, "Q-atoms in the solvent are not supported");
+  for (int q = 0; q < s->nqat; q++)
+    if (s->iqseq[q] < 0 || s->iqseq[q] >= s->nat_solute) return fail(QNB_EINVAL, "Q-atom %d is not a solute atom", q);
+  return 0;
+}
+
+void fill_params(DevSys& S, const qnb_system* s, const qnb_params* p) {
+  const int ns = s->nstates;
+  S.natom = s->natom;
+  S.nat_solute = s->nat_solute;
+  S.nwat = s->nwat;
+  S.sa = s->solv_atom;
+  S.ncgp = s->ncgp;
+  S.ncgp_solute = s->ncgp_solute;
+  S.nqat = s->nqat;
+  S.nstates = ns;
+  S.ntypes = s->ntypes;
+  S.ivdw_rule = s->ivdw_rule;
+  S.spc = s->solvent_spc;
+  S.use_lrf = p->use_LRF;
+  S.prec = p->precision;
+  S.quirk = p->ref_quirks;
+  S.rcpp2 = p->Rcpp * p->Rcpp;
+  S.rcpw2 = p->Rcpw * p->Rcpw;
+  S.rcww2 = p->Rcww * p->Rcww;
+  S.rcq2 = p->Rcq * p->Rcq;
+  S.rclrf2 = p->RcLRF * p->RcLRF;
+  S.rexcl2 = p->rexcl_o * p->rexcl_o;
+  for (int k = 0; k < QNB_MAXST; k++) S.lambda[k] = k < ns ? p->lambda[k] : 0.0;
+  S.el14 = s->el14_scale;
+  for (int c = 0; c < 3; c++) {
+    S.xp[c] = p->xpcent[c];
+    S.xw[c] = p->xwcent[c];
+  }
+  S.rwat = p->rwat;
+  S.fk_wsphere = p->fk_wsphere;
+  S.Dwmz = p->Dwmz;
+  S.awmz = p->awmz;
+  S.Boltz = p->Boltz;
+  S.fkwpol = p->fkwpol;
+  S.itdis = p->itdis;
+  S.wpol = p->wpol_restr;
+  S.nshell = p->nwpolr_shell;
+  for (int k = 0; k < QNB_MAXSH; k++) {
+    S.sh_rout[k] = k < S.nshell ? p->wshell_rout[k] : 0.0;
+    S.sh_dr[k] = k < S.nshell ? p->wshell_dr[k] : 0.0;
+    S.sh_cstb[k] = k < S.nshell ? p->wshell_cstb[k] : 0.0;
+  }
+}
+
+int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
+  const int nat = s->natom, nsol = s->nat_solute, nq = s->nqat, ns = s->nstates, sa = s->solv_atom;
+  T.ntypes = s->ntypes;
+  T.ivdw_rule = s->ivdw_rule;
+  T.gfirst.assign(s->cgp_first, s->cgp_first + s->ncgp);
+  T.glast.assign(s->cgp_last, s->cgp_last + s->ncgp);
+  T.gswitch.assign(s->cgp_iswitch, s->cgp_iswitch + s->ncgp);
+  const int ncgpat = s->cgp_last[s->ncgp - 1] + 1;
+  T.cgpatom.assign(s->cgpatom, s->cgpatom + ncgpat);
+  T.iqatom.assign(s->iqatom, s->iqatom + nat);
+  T.iac.assign(s->iac, s->iac + nat);
+  T.excl.assign(s->excl, s->excl + nat);
+  T.crg.assign(s->crg, s->crg + nat);
+  T.avdw.assign(s->avdw, s->avdw + 3 * s->ntypes);
+  T.bvdw.assign(s->bvdw, s->bvdw + 3 * s->ntypes);
+  T.ljcod.assign(s->ljcod, s->ljcod + s->ntypes * s->ntypes);
+  std::vector<int> atom_group(nat, -1);
+  for (int g = 0; g < s->ncgp; g++)
+    for (int k = s->cgp_first[g]; k <= s->cgp_last[g]; k++) atom_group[s->cgpatom[k]] = g;
+  for (int i = 0; i < nat; i++)
+    if (atom_group[i] < 0) return fail(QNB_EINVAL, "atom %d is in no charge group", i);
+  const int nsol1 = std::max(nsol, 1);
+  T.exmask.assign(nsol1, 0u);
+  T.m14mask.assign(nsol1, 0u);
+  for (int i = 0; i < nsol; i++)
+    for (int k = 1; k <= 25; k++) {
+      if (s->listex[i * 25 + k - 1]) T.exmask[i] |= 1u << (k - 1);
+      if (s->list14[i * 25 + k - 1]) T.m14mask[i] |= 1u << (k - 1);
+    }
+  {
+    std::vector<std::vector<std::pair<int, int>>> lx(nsol1);
+    for (int m = 0; m < s->nexlong; m++) {
+      int a = s->listexlong[2 * m], b = s->listexlong[2 * m + 1];
+      lx[std::min(a, b)].push_back({std::max(a, b), 0});
+    }
+    for (int m = 0; m < s->n14long; m++) {
+      int a = s->list14long[2 * m], b = s->list14long[2 * m + 1];
+      lx[std::min(a, b)].push_back({std::max(a, b), 1});
+    }
+    T.lx_off.assign(nsol1 + 1, 0);
+    for (int i = 0; i < nsol1; i++) {
+      T.lx_off[i + 1] = T.lx_off[i] + (int)lx[i].size();
+      for (auto& e : lx[i]) {
+        T.lx_j.push_back(e.first);
+        T.lx_kind.push_back((uint8_t)e.second);
+      }
+    }
+  }
+  if (nq > 0) T.iqseq.assign(s->iqseq, s->iqseq + nq);
+  if (nsol > 0 && nq > 0) T.qconn3.assign(s->qconn3, s->qconn3 + nsol);
+  else T.qconn3.assign(nsol1, 0);
+  // static Q lists (nbqqlist, nonbondene.f90:3231-3282), per state in reference order
+  std::vector<int> qq_iq, qq_jq, qq_st, qqp_iq, qqp_j, qqp_st;
+  std::vector<uint8_t> qq_soft;
+  std::vector<double> qq_par, qqp_par;
+  for (int st = 0; st < ns && nq > 0; st++) {
+    for (int iq = 0; iq < nq - 1; iq++)
+      for (int jq = iq + 1; jq < nq; jq++) {
+        size_t k = (size_t)iq + nq * ((size_t)jq + (size_t)nq * st);
+        if (!s->qq_set[k]) continue;
+        qq_iq.push_back(iq);
+        qq_jq.push_back(jq);
+        qq_st.push_back(st);
+        qq_soft.push_back(s->qq_soft[k]);
+        qq_par.insert(qq_par.end(), {s->qq_A[k], s->qq_B[k], s->qq_el[k], s->qq_sc[k]});
+        if (st == 0) {
+          T.qq_iq.push_back(iq);
+          T.qq_jq.push_back(jq);
+        }
+      }
+    for (int ja = 0; ja < nsol; ja++) {
+      if (s->iqatom[ja] != 0 || !s->qconn3[ja]) continue;
+      for (int iq = 0; iq < nq; iq++) {
+        size_t k = (size_t)ja + nsol * ((size_t)iq + (size_t)nq * st);
+        if (!s->qp_set[k]) continue;
+        qqp_iq.push_back(iq);
+        qqp_j.push_back(ja);
+        qqp_st.push_back(st);
+        qqp_par.insert(qqp_par.end(), {s->qp_A[k], s->qp_B[k], s->qp_el[k], s->qp_sc[k]});
+        if (st == 0) {
+          T.qqp_iq.push_back(iq);
+          T.qqp_j.push_back(ja);
+        }
+      }
+    }
+  }
+  int rc = 0;
+#define UP(field, src, n)                                    \
+  do {                                                       \
+    if ((rc = M.upload(&S.field, (src), (size_t)(n)))) return rc; \
+  } while (0)
+  UP(crg, s->crg, nat);
+  UP(iac, s->iac, nat);
+  UP(iqatom, s->iqatom, nat);
+  UP(excl, s->excl, nat);
+  UP(heavy, s->heavy, nat);
+  UP(gfirst, s->cgp_first, s->ncgp);
+  UP(glast, s->cgp_last, s->ncgp);
+  UP(gswitch, s->cgp_iswitch, s->ncgp);
+  UP(cgpatom, s->cgpatom, ncgpat);
+  UP(atom_group, atom_group.data(), nat);
+  UP(avdw, s->avdw, 3 * s->ntypes);
+  UP(bvdw, s->bvdw, 3 * s->ntypes);
+  UP(ljcod, s->ljcod, s->ntypes * s->ntypes);
+  UP(exmask, T.exmask.data(), T.exmask.size());
+  UP(m14mask, T.m14mask.data(), T.m14mask.size());
+  UP(lx_off, T.lx_off.data(), T.lx_off.size());
+  UP(lx_j, T.lx_j.data(), T.lx_j.size());
+  UP(lx_kind, T.lx_kind.data(), T.lx_kind.size());
+  if (nsol > 0 && s->nwat > 0) {
+    UP(pwA, s->pw_A, (size_t)nsol * sa);
+    UP(pwB, s->pw_B, (size_t)nsol * sa);
+    UP(pwEl, s->pw_el, (size_t)nsol * sa);
+  }
+  if (s->nwat > 0) {
+    UP(wwA, s->ww_A, sa * sa);
+    UP(wwB, s->ww_B, sa * sa);
+    UP(wwEl, s->ww_el, sa * sa);
+  }
+  UP(qconn3, T.qconn3.data(), T.qconn3.size());
+  if (nq > 0) {
+    UP(iqseq, s->iqseq, nq);
+    if (nsol > 0) {
+      size_t n = (size_t)nsol * nq * ns;
+      UP(qpA, s->qp_A, n);
+      UP(qpB, s->qp_B, n);
+      UP(qpEl, s->qp_el, n);
+      UP(qpSc, s->qp_sc, n);
+    }
+    if (s->nwat > 0) {
+      size_t n = (size_t)nq * sa * ns;
+      UP(qwA, s->qw_A, n);
+      UP(qwB, s->qw_B, n);
+      UP(qwEl, s->qw_el, n);
+      UP(qwSc, s->qw_sc, n);
+    }
+    S.nqq = (int)qq_iq.size();
+    UP(qq_iq, qq_iq.data(), qq_iq.size());
+    UP(qq_jq, qq_jq.data(), qq_jq.size());
+    UP(qq_st, qq_st.data(), qq_st.size());
+    UP(qq_soft, qq_soft.data(), qq_soft.size());
+    UP(qq_par, qq_par.data(), qq_par.size());
+    S.nqqp = (int)qqp_iq.size();
+    UP(qqp_iq, qqp_iq.data(), qqp_iq.size());
+    UP(qqp_j, qqp_j.data(), qqp_j.size());
+    UP(qqp_st, qqp_st.data(), qqp_st.size());
+    UP(qqp_par, qqp_par.data(), qqp_par.size());
+  }
+#undef UP
+  return 0;
+}
+
+int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots) {
+  int rc = 0;
+#define AL(field, n)                                          \
+  do {                                                        \
+    if ((rc = M.alloc(&S.field, (size_t)(n)))) return rc;     \
+  } while (0)
+  const int nat = S.natom, nsol1 = std::max(S.nat_solute, 1), nw1 = std::max(S.nwat, 1);
+  AL(x, 3 * nat);
+  AL(d, 3 * nat);
+  AL(lrf, (size_t)QNB_NLRF * S.ncgp);
+  if (S.cap <= 0) S.cap = 256;
+  AL(nbr, (size_t)S.cap * S.ncgp);
+  AL(nnbr, S.ncgp);
+  AL(need, 1);
+  AL(qw_in, nw1);
+  AL(qp_in, std::max(S.ncgp_solute, 1));
+  AL(qw_list, nw1);
+  AL(qp_list, nsol1);
+  AL(nq_lists, 2);
+  S.nslots = nslots;
+  AL(epart, (size_t)QNB_NE * nslots);
+  AL(ered, QNB_NE);
+  AL(sh_state, 3 * QNB_MAXSH);
+  AL(w_theta, nw1);
+  AL(w_shell, nw1);
+  AL(sh_list, (size_t)QNB_MAXSH * nw1);
+  AL(sh_rank, (size_t)QNB_MAXSH * nw1);
+  AL(sh_n, QNB_MAXSH);
+#undef AL
+  double sh0[3 * QNB_MAXSH] = {0};
+  for (int k = 0; k < S.nshell; k++) sh0[k] = p->wshell_theta_corr[k];
+  HIPCHK(hipMemcpy(S.sh_state, sh0, sizeof sh0, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(S.nq_lists, 0, 2 * sizeof(int)));
+  HIPCHK(hipMemset(S.need, 0, sizeof(int)));
+  HIPCHK(hipMemset(S.nnbr, 0, sizeof(int) * S.ncgp));
+  return 0;
+}
+
+}  // namespace qnb
+
+using namespace qnb;
+
+struct qnb_handle {
+  DevSys hs;  // host copy of the device record (device pointers)
+  DevSys* dsys = nullptr;
+  DevAlloc mem;
+  HostTopo T;
+  hipStream_t stream = nullptr;
+  double* d_tfree = nullptr;
+  int* d_istep = nullptr;
+  bool qlists_built = false;
+  bool lists_built = false;
+};
+
+// host twin of the device pp_param filter, for list export
+bool qnb::host_pp_param(const HostTopo& T, int i, int j) {
+  int lo = std::min(i, j), hi = std::max(i, j), dk = hi - lo;
+  bool is14 = false;
+  if (dk <= 25) {
+    uint32_t bit = 1u << (dk - 1);
+    if (T.exmask[lo] & bit) return false;
+    is14 = (T.m14mask[lo] & bit) != 0;
+  } else {
+    bool ex = false;
+    for (int m = T.lx_off[lo]; m < T.lx_off[lo + 1]; m++)
+      if (T.lx_j[m] == hi) {
+        if (T.lx_kind[m] == 0) ex = true;
+        else is14 = true;
+      }
+    if (ex) return false;
+  }
+  int ti = T.iac[i], tj = T.iac[j];
+  int vdw = is14 ? 3 : T.ljcod[ti + tj * T.ntypes];
+  int k = vdw - 1;
+  double A, B;
+  if (T.ivdw_rule == 1) {
+    A = T.avdw[ti * 3 + k] * T.avdw[tj * 3 + k];
+    B = T.bvdw[ti * 3 + k] * T.bvdw[tj * 3 + k];
+  } else {
+    double tA = T.avdw[ti * 3 + k] + T.avdw[tj * 3 + k];
+    tA = tA * tA;
+    tA = tA * tA * tA;
+    double tB = T.bvdw[ti * 3 + k] * T.bvdw[tj * 3 + k];
+    A = (tA * tA) * tB;
+    B = 2.0 * tA * tB;
+  }
+  double el = T.crg[i] * T.crg[j];
+  return std::fabs(A) > 1.0e-10 || std::fabs(B) > 1.0e-10 || std::fabs(el) > 1.0e-10;
+}
+
+extern "C" int32_t qnb_abi_version(void) { return QNB_ABI_VERSION; }
+extern "C" const char* qnb_last_error(void) { return qnb::last_error(); }
+extern "C" int32_t qnb_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" int32_t qnb_destroy(qnb_handle* h) {
+  if (!h) return 0;
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+extern "C" int32_t qnb_create(const qnb_system* s, const qnb_params* p, qnb_handle** out) {
+  if (!out) return fail(QNB_EINVAL, "null out");
+  *out = nullptr;
+  int rc = validate_system(s, p);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(QNB_EDEVICE, "no HIP device");
+  qnb_handle* h = new qnb_handle();
+  DevSys& S = h->hs;
+  memset(&S, 0, sizeof S);
+  fill_params(S, s, p);
+  if ((rc = topo_build(S, s, h->T, h->mem)) || (rc = instance_alloc(S, h->mem, p, s->ncgp + 2)) ||
+      (rc = h->mem.alloc(&h->d_tfree, 1)) || (rc = h->mem.alloc(&h->d_istep, 1)) || (rc = h->mem.alloc(&h->dsys, 1))) {
+    delete h;
+    return rc;
+  }
+  if (hipStreamCreate(&h->stream) != hipSuccess || hipMemcpy(h->dsys, &S, sizeof S, hipMemcpyHostToDevice) != hipSuccess) {
+    delete h;
+    return fail(QNB_EDEVICE, "stream or system record set-up failed");
+  }
+  *out = h;
+  return 0;
+}
+
+static int upload_x(qnb_handle* h, const double* x) {
+  if (!x) return fail(QNB_EINVAL, "null coordinates");
+  HIPCHK(hipMemcpyAsync(h->hs.x, x, sizeof(double) * 3 * h->hs.natom, hipMemcpyHostToDevice, h->stream));
+  return 0;
+}
+
+template <typename R>
+static void launch_lists(qnb_handle* h) {
+  const int wpb = 4;
+  dim3 grid((h->hs.ncgp + wpb - 1) / wpb, 1);
+  hipLaunchKernelGGL(k_lists_lrf<R>, grid, dim3(64 * wpb), 0, h->stream, h->dsys);
+}
+
+extern "C" int32_t qnb_make_pair_lists(qnb_handle* h, const double* x) {
+  if (!h) return fail(QNB_EINVAL, "null handle");
+  int rc = upload_x(h, x);
+  if (rc) return rc;
+  for (int attempt = 0; attempt < 4; attempt++) {
+    HIPCHK(hipMemsetAsync(h->hs.need, 0, sizeof(int), h->stream));
+    if (h->hs.prec == QNB_PREC_MIXED) launch_lists<float>(h);
+    else launch_lists<double>(h);
+    HIPCHK(hipGetLastError());
+    int need = 0;
+    HIPCHK(hipMemcpyAsync(&need, h->hs.need, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (need <= h->hs.cap) break;
+    // reallocate_nonbondlist_* (qalloc.f90:370-605): grow and rebuild
+    int ncap = ((need * 5 / 4) + 63) / 64 * 64;
+    h->mem.release(h->hs.nbr);
+    h->hs.nbr = nullptr;
+    rc = h->mem.alloc(&h->hs.nbr, (size_t)ncap * h->hs.ncgp);
+    if (rc) return rc;
+    h->hs.cap = ncap;
+    HIPCHK(hipMemcpy(h->dsys, &h->hs, sizeof(DevSys), hipMemcpyHostToDevice));
+    if (attempt == 3) return fail(QNB_EOVERFLOW, "neighbour list capacity");
+  }
+  // Q lists: rebuilt every update unless Rcq exceeds the exclusion radius,
+  // then built once (the `save`d list_done of nbqplist/nbqwlist)
+  if (h->hs.nqat > 0 && !(h->qlists_built && h->hs.rcq2 > h->hs.rexcl2)) {
+    hipLaunchKernelGGL(k_qlists, dim3(1), dim3(1024), 0, h->stream, h->dsys);
+    HIPCHK(hipGetLastError());
+    h->qlists_built = true;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->lists_built = true;
+  return 0;
+}
+
+template <typename R, int NS>
+static void launch_eval(qnb_handle* h, uint32_t mask) {
+  const DevSys& S = h->hs;
+  const int wpb = 4;
+  hipLaunchKernelGGL((k_forces<R, NS>), dim3((S.ncgp + wpb - 1) / wpb, 1), dim3(64 * wpb), 0, h->stream, h->dsys,
+                     mask, (const double*)h->d_tfree);
+  if (S.nqat > 0 && (mask & (QNB_T_QW | QNB_T_QP))) {
+    // partners per Q-atom: at most nwat + nat_solute; chunks of 64 per wave
+    int maxp = S.nwat + S.nat_solute;
+    int chunks = std::max(1, std::min((maxp + 63) / 64, 64));
+    int waves = S.nqat * chunks;
+    hipLaunchKernelGGL((k_qforces<R, NS>), dim3((waves + wpb - 1) / wpb, 1), dim3(64 * wpb), 0, h->stream, h->dsys,
+                       mask, chunks);
+  }
+  const int with_qq = S.nqat > 0 && (mask & (QNB_T_QQ | QNB_T_QQP));
+  const int with_pol = S.nwat > 0 && S.wpol && S.nshell > 0 && (mask & QNB_T_POL);
+  if (with_qq) hipLaunchKernelGGL((k_qq<R>), dim3(1), dim3(64), 0, h->stream, h->dsys, mask);
+  if (with_pol) hipLaunchKernelGGL(k_watpol, dim3(1), dim3(1024), 0, h->stream, h->dsys, (const int*)h->d_istep);
+  hipLaunchKernelGGL(k_reduce_energies, dim3(1), dim3(64), 0, h->stream, h->dsys, with_qq, with_pol);
+}
+
+template <typename R>
+static void launch_eval_r(qnb_handle* h, uint32_t mask) {
+  int ns = h->hs.nstates;
+  if (ns <= 1) launch_eval<R, 1>(h, mask);
+  else if (ns <= 2) launch_eval<R, 2>(h, mask);
+  else if (ns <= 4) launch_eval<R, 4>(h, mask);
+  else launch_eval<R, 8>(h, mask);
+}
+
+extern "C" int32_t qnb_eval_terms(qnb_handle* h, uint32_t mask, const double* x, double* d, double Tfree, int32_t istep,
+                       qnb_energies* E, qnb_qenergies* EQ) {
+  if (!h) return fail(QNB_EINVAL, "null handle");
+  if (!h->lists_built) return fail(QNB_EINVAL, "make_pair_lists must run before the first evaluation");
+  if (!d) return fail(QNB_EINVAL, "null gradient");
+  int rc = upload_x(h, x);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(h->d_tfree, &Tfree, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->d_istep, &istep, sizeof(int), hipMemcpyHostToDevice, h->stream));
+  if (h->hs.prec == QNB_PREC_MIXED) launch_eval_r<float>(h, mask);
+  else launch_eval_r<double>(h, mask);
+  HIPCHK(hipGetLastError());
+  const int nat = h->hs.natom;
+  std::vector<double> dd(3 * (size_t)nat);
+  double er[QNB_NE];
+  HIPCHK(hipMemcpyAsync(dd.data(), h->hs.d, sizeof(double) * 3 * nat, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(er, h->hs.ered, sizeof er, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (size_t k = 0; k < dd.size(); k++) d[k] += dd[k];
+  bool finite = true;
+  for (int k = 0; k < QNB_NE; k++) finite = finite && std::isfinite(er[k]);
+  if (E) {
+    if (mask & QNB_T_PP) { E->pp_el = er[E_PP_EL]; E->pp_vdw = er[E_PP_VDW]; }
+    if (mask & QNB_T_PW) { E->pw_el = er[E_PW_EL]; E->pw_vdw = er[E_PW_VDW]; }
+    if (mask & QNB_T_WW) { E->ww_el = er[E_WW_EL]; E->ww_vdw = er[E_WW_VDW]; }
+    if (mask & QNB_T_LRF) E->lrf = er[E_LRF];
+    if (mask & QNB_T_RAD) E->solvent_radial = er[E_RAD];
+    if (mask & QNB_T_POL) E->water_pol = er[E_POL];
+  }
+  if (EQ)
+    for (int s = 0; s < h->hs.nstates; s++) {
+      const double* q = er + E_Q0 + 6 * s;
+      if (mask & QNB_T_QQ) { EQ[s].qq_el = q[0]; EQ[s].qq_vdw = q[1]; }
+      // nonbond_qp and nonbond_qqp both add into EQ(s)%qp: with QP selected the
+      // field is set (Q-solute plus, when selected, Q-bonded-neighbour terms);
+      // with QQP alone the Q-bonded-neighbour energy is added to it
+      if (mask & QNB_T_QP) {
+        EQ[s].qp_el = q[2];
+        EQ[s].qp_vdw = q[3];
+      } else if (mask & QNB_T_QQP) {
+        EQ[s].qp_el += q[2];
+        EQ[s].qp_vdw += q[3];
+      }
+      if (mask & QNB_T_QW) { EQ[s].qw_el = q[4]; EQ[s].qw_vdw = q[5]; }
+    }
+  if (!finite) return fail(QNB_ENONFINITE, "non-finite energy");
+  return 0;
+}
+
+extern "C" int32_t qnb_nonbonded(qnb_handle* h, const double* x, double* d, qnb_energies* E, qnb_qenergies* EQ) {
+  return qnb_eval_terms(h, QNB_T_NONBONDED, x, d, 0.0, 0, E, EQ);
+}
+
+extern "C" int32_t qnb_qq(qnb_handle* h, const double* x, double* d, qnb_qenergies* EQ) {
+  if (!EQ) return fail(QNB_EINVAL, "null EQ");
+  return qnb_eval_terms(h, QNB_T_QQ | QNB_T_QQP, x, d, 0.0, 0, nullptr, EQ);
+}
+
+extern "C" int32_t qnb_restraints(qnb_handle* h, const double* x, double* d, double Tfree, int32_t istep, qnb_energies* E) {
+  return qnb_eval_terms(h, QNB_T_RAD | QNB_T_POL, x, d, Tfree, istep, E, nullptr);
+}
+
+extern "C" int32_t qnb_get_lrf(qnb_handle* h, double* lrf) {
+  if (!h || !lrf) return fail(QNB_EINVAL, "null argument");
+  HIPCHK(hipMemcpy(lrf, h->hs.lrf, sizeof(double) * QNB_NLRF * h->hs.ncgp, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int32_t qnb_get_shell_state(qnb_handle* h, double* st) {
+  if (!h || !st) return fail(QNB_EINVAL, "null argument");
+  double buf[3 * QNB_MAXSH];
+  HIPCHK(hipMemcpy(buf, h->hs.sh_state, sizeof buf, hipMemcpyDeviceToHost));
+  int n = h->hs.nshell;
+  for (int k = 0; k < n; k++) {
+    st[k] = buf[k];
+    st[n + k] = buf[QNB_MAXSH + k];
+    st[2 * n + k] = buf[2 * QNB_MAXSH + k];
+  }
+  return 0;
+}
+
+extern "C" int32_t qnb_set_shell_state(qnb_handle* h, const double* st) {
+  if (!h || !st) return fail(QNB_EINVAL, "null argument");
+  double buf[3 * QNB_MAXSH] = {0};
+  int n = h->hs.nshell;
+  for (int k = 0; k < n; k++) {
+    buf[k] = st[k];
+    buf[QNB_MAXSH + k] = st[n + k];
+    buf[2 * QNB_MAXSH + k] = st[2 * n + k];
+  }
+  HIPCHK(hipMemcpy(h->hs.sh_state, buf, sizeof buf, hipMemcpyHostToDevice));
+  return 0;
+}
+
+static inline bool owns_h(int a, int b) { return (a < b && ((a + b) & 1) == 0) || (a > b && ((a + b) & 1) == 1); }
+
+// Expands the device's group-level lists into the reference's atom-pair lists,
+// in the reference's append order (make_pair_lists, nonbondene.f90:749-837).
+extern "C" int32_t qnb_get_pairs(qnb_handle* h, int32_t which, int32_t* ab, int64_t cap, int64_t* n) {
+  if (!h || !n) return fail(QNB_EINVAL, "null argument");
+  if (!h->lists_built) return fail(QNB_EINVAL, "no lists built yet");
+  const DevSys& S = h->hs;
+  std::vector<int32_t> out;
+  std::vector<int> nn(S.ncgp), nbr((size_t)S.cap * S.ncgp);
+  HIPCHK(hipMemcpy(nn.data(), S.nnbr, sizeof(int) * S.ncgp, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(nbr.data(), S.nbr, sizeof(int) * nbr.size(), hipMemcpyDeviceToHost));
+  auto row = [&](int g) { return nbr.data() + (size_t)g * S.cap; };
+  const int ns = S.ncgp_solute;
+  switch (which) {
+    case QNB_LIST_PP:
+      for (int ig = 0; ig < ns; ig++) {
+        if (h->T.excl[h->T.gswitch[ig]]) continue;
+        std::vector<int> partners;
+        for (int k = 0; k < nn[ig]; k++) {
+          int b = row(ig)[k];
+          if (b < ns && owns_h(ig, b)) partners.push_back(b);
+        }
+        partners.push_back(ig);
+        std::sort(partners.begin(), partners.end());
+        for (int jg : partners)
+          for (int ia = h->T.gfirst[ig]; ia <= h->T.glast[ig]; ia++) {
+            int i = h->T.cgpatom[ia];
+            if (h->T.iqatom[i]) continue;
+            for (int ja = h->T.gfirst[jg]; ja <= h->T.glast[jg]; ja++) {
+              int j = h->T.cgpatom[ja];
+              if (h->T.iqatom[j]) continue;
+              if (ig == jg && i >= j) continue;
+              if (!host_pp_param(h->T, i, j)) continue;
+              out.push_back(i);
+              out.push_back(j);
+            }
+          }
+      }
+      break;
+    case QNB_LIST_PW:
+      for (int ig = 0; ig < ns; ig++) {
+        if (h->T.excl[h->T.gswitch[ig]]) continue;
+        for (int k = 0; k < nn[ig]; k++) {
+          int b = row(ig)[k];
+          if (b < ns) continue;
+          int o = S.nat_solute + S.sa * (b - ns);
+          for (int ia = h->T.gfirst[ig]; ia <= h->T.glast[ig]; ia++) {
+            int i = h->T.cgpatom[ia];
+            if (h->T.iqatom[i]) continue;
+            for (int j = 0; j < S.sa; j++) {
+              out.push_back(i);
+              out.push_back(o + j);
+            }
+          }
+        }
+      }
+      break;
+    case QNB_LIST_WW:
+      for (int a = ns; a < S.ncgp; a++)
+        for (int k = 0; k < nn[a]; k++) {
+          int b = row(a)[k];
+          if (b >= ns && owns_h(a, b)) {
+            out.push_back(a - ns);
+            out.push_back(b - ns);
+          }
+        }
+      break;
+    case QNB_LIST_QP:
+    case QNB_LIST_QW: {
+      int nql[2] = {0, 0};
+      HIPCHK(hipMemcpy(nql, S.nq_lists, sizeof nql, hipMemcpyDeviceToHost));
+      int cnt = which == QNB_LIST_QW ? nql[0] : nql[1];
+      std::vector<int> lst(std::max(cnt, 1));
+      if (cnt) HIPCHK(hipMemcpy(lst.data(), which == QNB_LIST_QW ? S.qw_list : S.qp_list, sizeof(int) * cnt,
+                                hipMemcpyDeviceToHost));
+      for (int m = 0; m < cnt; m++)
+        for (int iq = 0; iq < S.nqat; iq++) {
+          out.push_back(iq);
+          out.push_back(lst[m]);
+        }
+      break;
+    }
+    case QNB_LIST_QQ:
+      for (size_t m = 0; m < h->T.qq_iq.size(); m++) {
+        out.push_back(h->T.qq_iq[m]);
+        out.push_back(h->T.qq_jq[m]);
+      }
+      break;
+    case QNB_LIST_QQP:
+      for (size_t m = 0; m < h->T.qqp_iq.size(); m++) {
+        out.push_back(h->T.qqp_iq[m]);
+        out.push_back(h->T.qqp_j[m]);
+      }
+      break;
+    default:
+      return fail(QNB_EINVAL, "unknown list %d", which);
+  }
+  *n = (int64_t)(out.size() / 2);
+  if (ab && *n <= cap) memcpy(ab, out.data(), sizeof(int32_t) * out.size());
+  return 0;
+}
+

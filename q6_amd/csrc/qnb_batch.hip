@@ -1,0 +1,635 @@
+// qnb_batch.hip -- device-resident MD of a batch of instances (lambda windows
+// or independent systems) on top of the nonbonded engine: step kernels
+// (solute bonded terms, leap-frog + Berendsen, SHAKE of rigid waters,
+// temperature, energy sampling) and the batch C ABI.  One launch of each
+// kernel covers every instance (grid.y or grid.x = instance), and a
+// list-update cycle of nbcycle steps is captured once as a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "qnb_host.h"
+
+namespace qnb {
+template <typename R>
+__global__ void k_lists_lrf(const DevSys*);
+__global__ void k_qlists(const DevSys*);
+template <typename R, int NS>
+__global__ void k_forces(const DevSys*, uint32_t, const double*);
+template <typename R, int NS>
+__global__ void k_qforces(const DevSys*, uint32_t, int);
+template <typename R>
+__global__ void k_qq(const DevSys*, uint32_t);
+__global__ void k_watpol(const DevSys*, const int*);
+__global__ void k_reduce_energies(const DevSys*, int, int);
+
+// per-instance MD record
+struct MdDev {
+  double dt, dt_over_tau, T0, Boltz;
+  int nbcycle, ene_every, max_frames;
+  int nbonds, nangles;
+  const int* bond_ij;
+  const double* bond_r0;
+  const double* bond_k;
+  const int* angle_ijk;
+  const double* angle_t0;
+  const double* angle_k;
+  const double* winv;
+  double roh2, rhh2;
+  double ndegf, ndegfree;
+  double* v;
+  double* xx;
+  double* hist;   // [max_frames][QNB_FRAME_LEN]
+  double* st;     // [8]: sfact, Tfree, Temp, Ekin, Ebond, Eangle
+  int* ctr;       // [4]: frame, error flags
+  double* tfree;  // &tfree_all[k]
+  int* istep;     // &istep_all[k]
+};
+
+enum { ERR_NONFINITE = 1, ERR_SHAKE = 2, ERR_OVERFLOW = 4 };
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x / QNB_WAVE, l = threadIdx.x % QNB_WAVE, nw = blockDim.x / QNB_WAVE;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < nw; k++) s += red[k];
+  return s;
+}
+
+// harmonic bonds and angles of the solute (bond, angle: src/bondene.f90:41-246)
+__global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
+  const DevSys& S = systems[blockIdx.x];
+  const MdDev& M = mds[blockIdx.x];
+  __shared__ double red[4];
+  const double* x = S.x;
+  double* d = S.d;
+  double eb = 0.0, ea = 0.0;
+  for (int b = threadIdx.x; b < M.nbonds; b += blockDim.x) {
+    const int i = M.bond_ij[2 * b], j = M.bond_ij[2 * b + 1];
+    const double ax = x[3 * j] - x[3 * i], ay = x[3 * j + 1] - x[3 * i + 1], az = x[3 * j + 2] - x[3 * i + 2];
+    const double r = sqrt(ax * ax + ay * ay + az * az);
+    const double db = r - M.bond_r0[b];
+    eb += 0.5 * M.bond_k[b] * db * db;
+    const double dv = M.bond_k[b] * db / r;
+    unsafeAtomicAdd(&d[3 * i], -ax * dv);
+    unsafeAtomicAdd(&d[3 * i + 1], -ay * dv);
+    unsafeAtomicAdd(&d[3 * i + 2], -az * dv);
+    unsafeAtomicAdd(&d[3 * j], ax * dv);
+    unsafeAtomicAdd(&d[3 * j + 1], ay * dv);
+    unsafeAtomicAdd(&d[3 * j + 2], az * dv);
+  }
+  for (int a = threadIdx.x; a < M.nangles; a += blockDim.x) {
+    const int i = M.angle_ijk[3 * a], j = M.angle_ijk[3 * a + 1], k = M.angle_ijk[3 * a + 2];
+    double u[3], w[3];
+    for (int c = 0; c < 3; c++) {
+      u[c] = x[3 * i + c] - x[3 * j + c];
+      w[c] = x[3 * k + c] - x[3 * j + c];
+    }
+    const double ru = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double rw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    double ct = (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / (ru * rw);
+    ct = fmin(1.0, fmax(-1.0, ct));
+    const double th = acos(ct);
+    const double st = fmax(sin(th), 1.0e-12);
+    const double da = th - M.angle_t0[a];
+    ea += 0.5 * M.angle_k[a] * da * da;
+    const double dv = M.angle_k[a] * da;
+    // dtheta/dx_i = -(w/(ru rw) - ct u/ru^2)/sin(theta), likewise for k; j balances
+    for (int c = 0; c < 3; c++) {
+      const double gi = -(w[c] / (ru * rw) - ct * u[c] / (ru * ru)) / st;
+      const double gk = -(u[c] / (ru * rw) - ct * w[c] / (rw * rw)) / st;
+      unsafeAtomicAdd(&d[3 * i + c], dv * gi);
+      unsafeAtomicAdd(&d[3 * k + c], dv * gk);
+      unsafeAtomicAdd(&d[3 * j + c], -dv * (gi + gk));
+    }
+  }
+  const double sb = block_sum(eb, red);
+  const double sa = block_sum(ea, red);
+  if (threadIdx.x == 0) {
+    M.st[4] = sb;
+    M.st[5] = sa;
+  }
+}
+
+// newvel_ber (src/md.f90:1451-1466): v = (v - d winv dt) s; xx = x; x += v dt
+__global__ __launch_bounds__(256) void k_leapfrog(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
+  const DevSys& S = systems[blockIdx.y];
+  const MdDev& M = mds[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S.natom) return;
+  const double s = M.st[0], dt = M.dt, wi = M.winv[i];
+  for (int c = 0; c < 3; c++) {
+    double v = (M.v[3 * i + c] - S.d[3 * i + c] * wi * dt) * s;
+    M.v[3 * i + c] = v;
+    M.xx[3 * i + c] = S.x[3 * i + c];
+    S.x[3 * i + c] = S.x[3 * i + c] + v * dt;
+  }
+}
+
+// shake (src/bondene.f90:1069-1151) on each rigid water (O-H1, O-H2, H1-H2),
+// then v = (x - xx)/dt for every atom (src/md.f90:1787-1793)
+__global__ __launch_bounds__(256) void k_shake(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
+  const DevSys& S = systems[blockIdx.y];
+  const MdDev& M = mds[blockIdx.y];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double* x = S.x;
+  const double* xx = M.xx;
+  if (t < S.nwat) {
+    const int o = S.nat_solute + 3 * t;
+    const int ci[3] = {o, o, o + 1}, cj[3] = {o + 1, o + 2, o + 2};
+    const double d2[3] = {M.roh2, M.roh2, M.rhh2};
+    bool ready[3] = {false, false, false};
+    int nits = 0;
+    for (; nits < 1000; nits++) {
+      for (int c = 0; c < 3; c++) {
+        if (ready[c]) continue;
+        const int i = ci[c], j = cj[c];
+        double vx = x[3 * i] - x[3 * j], vy = x[3 * i + 1] - x[3 * j + 1], vz = x[3 * i + 2] - x[3 * j + 2];
+        const double diff = d2[c] - (vx * vx + vy * vy + vz * vz);
+        if (fabs(diff) < 0.0001 * d2[c]) ready[c] = true;
+        const double px = xx[3 * i] - xx[3 * j], py = xx[3 * i + 1] - xx[3 * j + 1], pz = xx[3 * i + 2] - xx[3 * j + 2];
+        const double scp = vx * px + vy * py + vz * pz;
+        const double wi = M.winv[i], wj = M.winv[j];
+        const double corr = diff / (2.0 * scp * (wi + wj));
+        x[3 * i] += px * corr * wi;
+        x[3 * i + 1] += py * corr * wi;
+        x[3 * i + 2] += pz * corr * wi;
+        x[3 * j] -= px * corr * wj;
+        x[3 * j + 1] -= py * corr * wj;
+        x[3 * j + 2] -= pz * corr * wj;
+      }
+      if (ready[0] && ready[1] && ready[2]) break;
+    }
+    if (nits >= 1000) atomicOr(&M.ctr[1], ERR_SHAKE);
+    for (int k = 0; k < 3; k++)
+      for (int c = 0; c < 3; c++) M.v[3 * (o + k) + c] = (x[3 * (o + k) + c] - xx[3 * (o + k) + c]) / M.dt;
+  } else if (t < S.nwat + S.nat_solute) {
+    const int i = t - S.nwat;
+    for (int c = 0; c < 3; c++) M.v[3 * i + c] = (x[3 * i + c] - xx[3 * i + c]) / M.dt;
+  }
+}
+
+// temperature (src/md.f90:1379-1447), one temperature group: Temp, Tfree and
+// the Berendsen factor of the next step; advances the step counter
+__global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds,
+                                                     int advance) {
+  const DevSys& S = systems[blockIdx.x];
+  const MdDev& M = mds[blockIdx.x];
+  __shared__ double red[4];
+  double ek = 0.0, ekf = 0.0;
+  for (int i = threadIdx.x; i < S.natom; i += blockDim.x) {
+    const double m = 1.0 / M.winv[i];
+    const double v2 = M.v[3 * i] * M.v[3 * i] + M.v[3 * i + 1] * M.v[3 * i + 1] + M.v[3 * i + 2] * M.v[3 * i + 2];
+    const double e = 0.5 * m * v2;
+    ek += e;
+    if (!S.excl[i]) ekf += e;
+  }
+  ek = block_sum(ek, red);
+  ekf = block_sum(ekf, red);
+  if (threadIdx.x == 0) {
+    const double T = 2.0 * ek / M.Boltz / M.ndegf;
+    const double Tf = 2.0 * ekf / M.Boltz / M.ndegfree;
+    double sf = 0.0;
+    if (Tf != 0.0) sf = M.T0 / Tf - 1.0;
+    sf = sqrt(1.0 + M.dt_over_tau * sf);
+    M.st[0] = sf;
+    M.st[1] = Tf;
+    M.st[2] = T;
+    M.st[3] = ek;
+    *M.tfree = Tf;
+    if (!isfinite(T) || !isfinite(S.ered[E_PP_EL] + S.ered[E_WW_EL])) atomicOr(&M.ctr[1], ERR_NONFINITE);
+    if (*S.need > S.cap) atomicOr(&M.ctr[1], ERR_OVERFLOW);
+    if (advance) *M.istep = *M.istep + 1;
+  }
+}
+
+// energy sampling every ene_every steps (put_ene cadence, src/md.f90:1829-1834)
+__global__ void k_record(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
+  const DevSys& S = systems[blockIdx.x];
+  const MdDev& M = mds[blockIdx.x];
+  const int step = *M.istep;
+  if (M.ene_every <= 0 || step % M.ene_every != 0) return;
+  const int f = M.ctr[0];
+  if (f >= M.max_frames) return;
+  double* h = M.hist + (size_t)f * QNB_FRAME_LEN;
+  const int k = threadIdx.x;
+  if (k < QNB_NE) h[k] = S.ered[k];
+  __syncthreads();
+  if (k == 0) {
+    h[QNB_F_BOND] = M.st[4];
+    h[QNB_F_ANGLE] = M.st[5];
+    h[QNB_F_EKIN] = M.st[3];
+    h[QNB_F_TEMP] = M.st[2];
+    h[QNB_F_TFREE] = M.st[1];
+    h[QNB_F_STEP] = (double)step;
+    for (int z = QNB_F_STEP + 1; z < QNB_FRAME_LEN; z++) h[z] = 0.0;
+    M.ctr[0] = f + 1;
+  }
+}
+
+}  // namespace qnb
+
+using namespace qnb;
+
+struct qnb_batch {
+  int n = 0;
+  std::vector<DevSys> hs;
+  std::vector<MdDev> hm;
+  DevSys* dsys = nullptr;
+  MdDev* dmd = nullptr;
+  double* tfree = nullptr;
+  int* istep = nullptr;
+  DevAlloc mem;
+  std::vector<std::unique_ptr<HostTopo>> topos;
+  std::vector<int> topo_of;  // instance -> index into topos
+  hipStream_t stream = nullptr;
+  hipGraphExec_t cycle_exec = nullptr;
+  hipGraph_t cycle_graph = nullptr;
+  int maxg = 0, maxat = 0, maxsw = 0, ns_max = 1, prec = 0;
+  bool anyq = false, anypol = false, static_q = true;
+  int qchunks = 1, qwaves = 0;
+  std::vector<int> nbcycle;
+  long long steps_done = 0;
+  ~qnb_batch() {
+    if (cycle_exec) (void)hipGraphExecDestroy(cycle_exec);
+    if (cycle_graph) (void)hipGraphDestroy(cycle_graph);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+template <typename R, int NS>
+static void enqueue_forces(qnb_batch* b) {
+  const int wpb = 4, B = b->n;
+  const uint32_t mask = QNB_T_NONBONDED | QNB_T_QQ | QNB_T_QQP | QNB_T_RAD | QNB_T_POL;
+  hipLaunchKernelGGL((k_forces<R, NS>), dim3((b->maxg + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream, b->dsys,
+                     mask, (const double*)b->tfree);
+  if (b->anyq) {
+    hipLaunchKernelGGL((k_qforces<R, NS>), dim3((b->qwaves + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream,
+                       b->dsys, mask, b->qchunks);
+    hipLaunchKernelGGL((k_qq<R>), dim3(B), dim3(64), 0, b->stream, b->dsys, mask);
+  }
+  if (b->anypol) hipLaunchKernelGGL(k_watpol, dim3(B), dim3(1024), 0, b->stream, b->dsys, (const int*)b->istep);
+}
+
+template <typename R>
+static void enqueue_step(qnb_batch* b, bool lists) {
+  const int wpb = 4, B = b->n;
+  if (lists) {
+    hipLaunchKernelGGL(k_lists_lrf<R>, dim3((b->maxg + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream, b->dsys);
+    if (b->anyq && !b->static_q) hipLaunchKernelGGL(k_qlists, dim3(B), dim3(1024), 0, b->stream, b->dsys);
+  }
+  switch (b->ns_max) {
+    case 1: enqueue_forces<R, 1>(b); break;
+    case 2: enqueue_forces<R, 2>(b); break;
+    case 3:
+    case 4: enqueue_forces<R, 4>(b); break;
+    default: enqueue_forces<R, 8>(b); break;
+  }
+  hipLaunchKernelGGL(k_bonded, dim3(B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(64), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
+  hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
+  hipLaunchKernelGGL(k_leapfrog, dim3((b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  hipLaunchKernelGGL(k_shake, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  hipLaunchKernelGGL(k_temperature, dim3(B), dim3(256), 0, b->stream, b->dsys, b->dmd, 1);
+}
+
+static void enqueue(qnb_batch* b, bool lists) {
+  if (b->prec == QNB_PREC_MIXED) enqueue_step<float>(b, lists);
+  else enqueue_step<double>(b, lists);
+}
+
+extern "C" int32_t qnb_batch_destroy(qnb_batch* b) {
+  delete b;
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, const qnb_params* const* par,
+                                    const qnb_md_params* const* md, const double* const* x0, const double* const* v0,
+                                    qnb_batch** out) {
+  if (!out || n <= 0 || !sys || !par || !md || !x0 || !v0) return fail(QNB_EINVAL, "bad batch arguments");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(QNB_EDEVICE, "no HIP device");
+  std::unique_ptr<qnb_batch> b(new qnb_batch());
+  b->n = n;
+  b->hs.resize(n);
+  b->hm.resize(n);
+  b->prec = par[0]->precision;
+  std::map<const qnb_system*, int> shared;  // system pointer -> instance holding its topology
+  int rc = 0;
+  for (int k = 0; k < n; k++) {
+    if ((rc = validate_system(sys[k], par[k]))) return rc;
+    if (par[k]->precision != b->prec) return fail(QNB_EINVAL, "all instances need the same precision");
+    DevSys& S = b->hs[k];
+    memset(&S, 0, sizeof S);
+    auto it = shared.find(sys[k]);
+    if (it == shared.end()) {
+      fill_params(S, sys[k], par[k]);
+      b->topos.emplace_back(new HostTopo());
+      if ((rc = topo_build(S, sys[k], *b->topos.back(), b->mem))) return rc;
+      shared[sys[k]] = k;
+      b->topo_of.push_back((int)b->topos.size() - 1);
+    } else {
+      S = b->hs[it->second];  // topology pointers shared; instance fields below
+      fill_params(S, sys[k], par[k]);
+      b->topo_of.push_back(b->topo_of[it->second]);
+    }
+    S.cap = 0;
+    if ((rc = instance_alloc(S, b->mem, par[k], sys[k]->ncgp + 2))) return rc;
+    HIPCHK(hipMemcpy(S.x, x0[k], sizeof(double) * 3 * S.natom, hipMemcpyHostToDevice));
+    b->maxg = std::max(b->maxg, S.ncgp);
+    b->maxat = std::max(b->maxat, S.natom);
+    b->maxsw = std::max(b->maxsw, S.nwat + S.nat_solute);
+    b->ns_max = std::max(b->ns_max, S.nstates);
+    b->anyq = b->anyq || S.nqat > 0;
+    b->anypol = b->anypol || (S.nwat > 0 && S.wpol && S.nshell > 0);
+    b->static_q = b->static_q && (S.rcq2 > S.rexcl2);
+    b->qchunks = std::max(b->qchunks, std::min((S.nwat + S.nat_solute + 63) / 64, 64));
+    b->nbcycle.push_back(md[k]->nbcycle);
+    if (md[k]->nbcycle != md[0]->nbcycle) return fail(QNB_EINVAL, "all instances need the same nbcycle");
+  }
+  for (int k = 0; k < n; k++) b->qwaves = std::max(b->qwaves, b->hs[k].nqat * b->qchunks);
+  HIPCHK(hipStreamCreate(&b->stream));
+  if ((rc = b->mem.alloc(&b->tfree, n)) || (rc = b->mem.alloc(&b->istep, n))) return rc;
+  HIPCHK(hipMemset(b->istep, 0, sizeof(int) * n));
+  // MD records
+  for (int k = 0; k < n; k++) {
+    const qnb_md_params* m = md[k];
+    MdDev& M = b->hm[k];
+    memset(&M, 0, sizeof M);
+    const DevSys& S = b->hs[k];
+    M.dt = 0.020462 * m->dt_fs;
+    M.dt_over_tau = m->dt_fs / m->tau_T_fs;
+    M.T0 = m->T0;
+    M.Boltz = par[k]->Boltz;
+    M.nbcycle = m->nbcycle;
+    M.ene_every = m->ene_every;
+    M.max_frames = m->max_frames;
+    M.nbonds = m->nbonds;
+    M.nangles = m->nangles;
+    M.roh2 = m->water_roh * m->water_roh;
+    M.rhh2 = m->water_rhh * m->water_rhh;
+    const double nconstr = 3.0 * S.nwat;
+    M.ndegf = 3.0 * S.natom - nconstr;
+    int nexcl = 0;
+    std::vector<uint8_t> ex(S.natom);
+    for (int i = 0; i < S.natom; i++) nexcl += sys[k]->excl[i] != 0;
+    M.ndegfree = M.ndegf - 3.0 * nexcl;
+    if ((rc = b->mem.upload(&M.bond_ij, m->bond_ij, 2 * (size_t)m->nbonds)) ||
+        (rc = b->mem.upload(&M.bond_r0, m->bond_r0, m->nbonds)) || (rc = b->mem.upload(&M.bond_k, m->bond_k, m->nbonds)) ||
+        (rc = b->mem.upload(&M.angle_ijk, m->angle_ijk, 3 * (size_t)m->nangles)) ||
+        (rc = b->mem.upload(&M.angle_t0, m->angle_t0, m->nangles)) ||
+        (rc = b->mem.upload(&M.angle_k, m->angle_k, m->nangles)) || (rc = b->mem.upload(&M.winv, m->winv, S.natom)) ||
+        (rc = b->mem.alloc(&M.v, 3 * (size_t)S.natom)) || (rc = b->mem.alloc(&M.xx, 3 * (size_t)S.natom)) ||
+        (rc = b->mem.alloc(&M.hist, (size_t)std::max(m->max_frames, 1) * QNB_FRAME_LEN)) ||
+        (rc = b->mem.alloc(&M.st, 8)) || (rc = b->mem.alloc(&M.ctr, 4)))
+      return rc;
+    HIPCHK(hipMemcpy(M.v, v0[k], sizeof(double) * 3 * S.natom, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(M.st, 0, 8 * sizeof(double)));
+    HIPCHK(hipMemset(M.ctr, 0, 4 * sizeof(int)));
+    M.tfree = b->tfree + k;
+    M.istep = b->istep + k;
+  }
+  if ((rc = b->mem.alloc(&b->dsys, n)) || (rc = b->mem.alloc(&b->dmd, n))) return rc;
+  // first list build, regrown until every row fits, then rows sized to 1.5x
+  // the largest neighbour count seen (re-checked at every update; an overflow
+  // during a run is reported by qnb_batch_sync)
+  for (int k = 0; k < n; k++) {
+    DevSys& S = b->hs[k];
+    b->mem.release(S.nbr);
+    S.cap = std::min(((S.ncgp + 63) / 64) * 64, 1024);
+    if ((rc = b->mem.alloc(&S.nbr, (size_t)S.cap * S.ncgp))) return rc;
+  }
+  for (int attempt = 0;; attempt++) {
+    HIPCHK(hipMemcpy(b->dsys, b->hs.data(), sizeof(DevSys) * n, hipMemcpyHostToDevice));
+    for (int k = 0; k < n; k++) HIPCHK(hipMemset(b->hs[k].need, 0, sizeof(int)));
+    const int wpb = 4;
+    if (b->prec == QNB_PREC_MIXED)
+      hipLaunchKernelGGL(k_lists_lrf<float>, dim3((b->maxg + wpb - 1) / wpb, n), dim3(64 * wpb), 0, b->stream, b->dsys);
+    else
+      hipLaunchKernelGGL(k_lists_lrf<double>, dim3((b->maxg + wpb - 1) / wpb, n), dim3(64 * wpb), 0, b->stream, b->dsys);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->stream));
+    bool again = false;
+    for (int k = 0; k < n; k++) {
+      DevSys& S = b->hs[k];
+      int need = 0;
+      HIPCHK(hipMemcpy(&need, S.need, sizeof(int), hipMemcpyDeviceToHost));
+      if (need > S.cap) {
+        b->mem.release(S.nbr);
+        S.cap = ((need + 63) / 64) * 64;
+        if ((rc = b->mem.alloc(&S.nbr, (size_t)S.cap * S.ncgp))) return rc;
+        again = true;
+      }
+    }
+    if (!again) break;
+    if (attempt > 3) return fail(QNB_EOVERFLOW, "neighbour list capacity");
+  }
+  for (int k = 0; k < n; k++) {
+    DevSys& S = b->hs[k];
+    std::vector<int> nn(S.ncgp);
+    HIPCHK(hipMemcpy(nn.data(), S.nnbr, sizeof(int) * S.ncgp, hipMemcpyDeviceToHost));
+    int mx = 0;
+    for (int v : nn) mx = std::max(mx, v);
+    const int cap = ((mx * 3 / 2 + 64) + 63) / 64 * 64;
+    if (cap < S.cap) {
+      // shrink: rebuild happens at the first step of the run
+      b->mem.release(S.nbr);
+      S.cap = cap;
+      if ((rc = b->mem.alloc(&S.nbr, (size_t)S.cap * S.ncgp))) return rc;
+    }
+    HIPCHK(hipMemset(S.need, 0, sizeof(int)));
+  }
+  HIPCHK(hipMemcpy(b->dsys, b->hs.data(), sizeof(DevSys) * n, hipMemcpyHostToDevice));
+  {
+    const int wpb = 4;
+    if (b->prec == QNB_PREC_MIXED)
+      hipLaunchKernelGGL(k_lists_lrf<float>, dim3((b->maxg + wpb - 1) / wpb, n), dim3(64 * wpb), 0, b->stream, b->dsys);
+    else
+      hipLaunchKernelGGL(k_lists_lrf<double>, dim3((b->maxg + wpb - 1) / wpb, n), dim3(64 * wpb), 0, b->stream, b->dsys);
+  }
+  if (b->anyq) hipLaunchKernelGGL(k_qlists, dim3(n), dim3(1024), 0, b->stream, b->dsys);
+  // initial temperature (md_run calls temperature() before the loop)
+  hipLaunchKernelGGL(k_temperature, dim3(n), dim3(256), 0, b->stream, b->dsys, b->dmd, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  *out = b.release();
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_run(qnb_batch* b, int32_t nsteps, int32_t use_graph) {
+  if (!b) return fail(QNB_EINVAL, "null batch");
+  const int nb = b->nbcycle[0];
+  int done = 0;
+  // steps up to the next list update, eagerly
+  while (done < nsteps && (b->steps_done % nb) != 0) {
+    enqueue(b, false);
+    b->steps_done++;
+    done++;
+  }
+  if (use_graph && nsteps - done >= nb) {
+    if (!b->cycle_exec) {
+      HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      for (int s = 0; s < nb; s++) enqueue(b, s == 0);
+      HIPCHK(hipStreamEndCapture(b->stream, &b->cycle_graph));
+      HIPCHK(hipGraphInstantiate(&b->cycle_exec, b->cycle_graph, nullptr, nullptr, 0));
+    }
+    while (nsteps - done >= nb) {
+      HIPCHK(hipGraphLaunch(b->cycle_exec, b->stream));
+      b->steps_done += nb;
+      done += nb;
+    }
+  }
+  while (done < nsteps) {
+    enqueue(b, (b->steps_done % nb) == 0);
+    b->steps_done++;
+    done++;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_sync(qnb_batch* b) {
+  if (!b) return fail(QNB_EINVAL, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (int k = 0; k < b->n; k++) {
+    int ctr[4];
+    HIPCHK(hipMemcpy(ctr, b->hm[k].ctr, sizeof ctr, hipMemcpyDeviceToHost));
+    if (ctr[1] & ERR_OVERFLOW) return fail(QNB_EOVERFLOW, "instance %d: neighbour list overflow", k);
+    if (ctr[1] & ERR_SHAKE) return fail(QNB_ENONFINITE, "instance %d: shake failure", k);
+    if (ctr[1] & ERR_NONFINITE) return fail(QNB_ENONFINITE, "instance %d: non-finite energy", k);
+  }
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_get_state(qnb_batch* b, int32_t k, double* x, double* v) {
+  if (!b || k < 0 || k >= b->n) return fail(QNB_EINVAL, "bad instance");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (x) HIPCHK(hipMemcpy(x, b->hs[k].x, sizeof(double) * 3 * b->hs[k].natom, hipMemcpyDeviceToHost));
+  if (v) HIPCHK(hipMemcpy(v, b->hm[k].v, sizeof(double) * 3 * b->hs[k].natom, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_energy_history(qnb_batch* b, int32_t k, double* out, int32_t cap, int32_t* nframes) {
+  if (!b || k < 0 || k >= b->n || !nframes) return fail(QNB_EINVAL, "bad instance");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  int ctr[4];
+  HIPCHK(hipMemcpy(ctr, b->hm[k].ctr, sizeof ctr, hipMemcpyDeviceToHost));
+  *nframes = ctr[0];
+  if (out && ctr[0] <= cap && ctr[0] > 0)
+    HIPCHK(hipMemcpy(out, b->hm[k].hist, sizeof(double) * QNB_FRAME_LEN * ctr[0], hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_pair_counts(qnb_batch* b, int32_t k, int64_t* out) {
+  if (!b || k < 0 || k >= b->n || !out) return fail(QNB_EINVAL, "bad instance");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const DevSys& S = b->hs[k];
+  const HostTopo* T = b->topos[b->topo_of[k]].get();
+  std::vector<int> nn(S.ncgp), nbr((size_t)S.cap * S.ncgp);
+  HIPCHK(hipMemcpy(nn.data(), S.nnbr, sizeof(int) * S.ncgp, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(nbr.data(), S.nbr, sizeof(int) * nbr.size(), hipMemcpyDeviceToHost));
+  int nql[2];
+  HIPCHK(hipMemcpy(nql, S.nq_lists, sizeof nql, hipMemcpyDeviceToHost));
+  // atom pairs of a group pair, filtered as the list builders filter them
+  auto pp_pairs = [&](int ig, int jg) {
+    int64_t c = 0;
+    for (int ia = T->gfirst[ig]; ia <= T->glast[ig]; ia++) {
+      const int i = T->cgpatom[ia];
+      if (T->iqatom[i]) continue;
+      for (int ja = T->gfirst[jg]; ja <= T->glast[jg]; ja++) {
+        const int j = T->cgpatom[ja];
+        if (T->iqatom[j] || (ig == jg && i >= j)) continue;
+        c += host_pp_param(*T, i, j);
+      }
+    }
+    return c;
+  };
+  auto nonq = [&](int g) {
+    int64_t c = 0;
+    for (int ia = T->gfirst[g]; ia <= T->glast[g]; ia++) c += T->iqatom[T->cgpatom[ia]] == 0;
+    return c;
+  };
+  int64_t pp = 0, pw = 0, ww = 0;
+  const int ns = S.ncgp_solute;
+  for (int a = 0; a < S.ncgp; a++) {
+    if (a < ns && T->excl[T->gswitch[a]]) continue;
+    for (int m = 0; m < std::min(nn[a], S.cap); m++) {
+      const int g = nbr[(size_t)a * S.cap + m];
+      const bool own = (a < g && ((a + g) & 1) == 0) || (a > g && ((a + g) & 1) == 1);
+      if (a < ns && g < ns) {
+        if (own) pp += pp_pairs(a, g);
+      } else if (a < ns) {
+        pw += nonq(a) * S.sa;
+      } else if (g >= ns && own) {
+        ww += (int64_t)S.sa * S.sa;
+      }
+    }
+    if (a < ns) pp += pp_pairs(a, a);
+  }
+  out[0] = pp;
+  out[1] = pw;
+  out[2] = ww;
+  out[3] = (int64_t)nql[1] * S.nqat;
+  out[4] = (int64_t)nql[0] * S.nqat * S.sa;
+  out[5] = S.nqq;
+  out[6] = S.nqqp;
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_time_kernel(qnb_batch* b, int32_t which, int32_t reps, double* ms_per_launch) {
+  if (!b || reps <= 0 || !ms_per_launch) return fail(QNB_EINVAL, "bad arguments");
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int wpb = 4, B = b->n;
+  const uint32_t mask = QNB_T_NONBONDED | QNB_T_QQ | QNB_T_QQP | QNB_T_RAD | QNB_T_POL;
+  auto launch = [&]() {
+    if (which == 0) {
+      if (b->prec == QNB_PREC_MIXED)
+        hipLaunchKernelGGL(k_lists_lrf<float>, dim3((b->maxg + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream, b->dsys);
+      else
+        hipLaunchKernelGGL(k_lists_lrf<double>, dim3((b->maxg + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream, b->dsys);
+    } else {
+      dim3 g((b->maxg + wpb - 1) / wpb, B), t(64 * wpb);
+      const double* tf = b->tfree;
+      const bool mixed = b->prec == QNB_PREC_MIXED;
+      switch (b->ns_max) {
+        case 1:
+          if (mixed) hipLaunchKernelGGL((k_forces<float, 1>), g, t, 0, b->stream, b->dsys, mask, tf);
+          else hipLaunchKernelGGL((k_forces<double, 1>), g, t, 0, b->stream, b->dsys, mask, tf);
+          break;
+        case 2:
+          if (mixed) hipLaunchKernelGGL((k_forces<float, 2>), g, t, 0, b->stream, b->dsys, mask, tf);
+          else hipLaunchKernelGGL((k_forces<double, 2>), g, t, 0, b->stream, b->dsys, mask, tf);
+          break;
+        case 3:
+        case 4:
+          if (mixed) hipLaunchKernelGGL((k_forces<float, 4>), g, t, 0, b->stream, b->dsys, mask, tf);
+          else hipLaunchKernelGGL((k_forces<double, 4>), g, t, 0, b->stream, b->dsys, mask, tf);
+          break;
+        default:
+          if (mixed) hipLaunchKernelGGL((k_forces<float, 8>), g, t, 0, b->stream, b->dsys, mask, tf);
+          else hipLaunchKernelGGL((k_forces<double, 8>), g, t, 0, b->stream, b->dsys, mask, tf);
+      }
+    }
+  };
+  launch();  // warm-up
+  HIPCHK(hipEventRecord(e0, b->stream));
+  for (int r = 0; r < reps; r++) launch();
+  HIPCHK(hipEventRecord(e1, b->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_launch = (double)ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}

@@ -1,0 +1,222 @@
+// qnb_dev.h -- device-side data layout and pair arithmetic of the MI355X
+// nonbonded engine (gfx950, wave64).
+//
+// Layout in HBM (one record per system instance; topology arrays may be shared
+// between instances of a batch, per-instance state is private):
+//   x, d        natom x {x,y,z} float64 (reference qr_vec layout)
+//   lrf         ncgp x 43 float64 (centre, phi0, phi1, phi2(3x3), phi3(9x3),
+//               the reference's LRF_TYPE order)
+//   nbr/nnbr    full neighbour lists in ELL form: row g holds every charge group
+//               within the cut-off of group g (both directions of a pair),
+//               nnbr[g] entries, row stride `cap`
+//   epart       per-wave energy partials [slot][QNB_NE] reduced in fixed order
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define QNB_WAVE 64
+#define QNB_NLRF 43
+#define QNB_MAXST 8
+#define QNB_MAXSH 8
+
+// energy slots
+enum {
+  E_PP_EL = 0, E_PP_VDW, E_PW_EL, E_PW_VDW, E_WW_EL, E_WW_VDW, E_LRF, E_RAD, E_POL,
+  E_Q0  // then per state s: qq_el, qq_vdw, qp_el, qp_vdw, qw_el, qw_vdw at E_Q0 + 6*s
+};
+#define QNB_NE (E_Q0 + 6 * QNB_MAXST)
+
+struct DevSys {
+  // sizes
+  int natom, nat_solute, nwat, sa, ncgp, ncgp_solute, nqat, nstates, ntypes, ivdw_rule, spc;
+  int use_lrf, prec, quirk;
+  double rcpp2, rcpw2, rcww2, rcq2, rclrf2, rexcl2;
+  double lambda[QNB_MAXST];
+  double el14;
+  double xp[3], xw[3];
+  // SCAAS
+  double rwat, fk_wsphere, Dwmz, awmz, Boltz, fkwpol, itdis;
+  int wpol, nshell;
+  double sh_rout[QNB_MAXSH], sh_dr[QNB_MAXSH], sh_cstb[QNB_MAXSH];
+
+  // topology (may be shared across a batch)
+  const double* crg;
+  const int* iac;
+  const int* iqatom;
+  const uint8_t* excl;
+  const uint8_t* heavy;
+  const int* gfirst;   // into cgpatom
+  const int* glast;    // into cgpatom (inclusive)
+  const int* gswitch;  // atom
+  const int* cgpatom;
+  const int* atom_group;
+  const double* avdw;  // [t*3+k]
+  const double* bvdw;
+  const int* ljcod;    // [ti + tj*ntypes], 1-based code
+  const uint32_t* exmask;  // [nat_solute] bit k-1: pair (i, i+k) excluded
+  const uint32_t* m14mask; // [nat_solute] bit k-1: pair (i, i+k) is 1-4
+  const int* lx_off;       // [nat_solute+1] long exclusion/1-4 partners of the lower atom
+  const int* lx_j;
+  const uint8_t* lx_kind;  // 0 excluded, 1 1-4
+  const double* pwA;       // [i + j*nat_solute]
+  const double* pwB;
+  const double* pwEl;
+  const double* wwA;       // [la + ka*sa]
+  const double* wwB;
+  const double* wwEl;
+  const int* iqseq;
+  const uint8_t* qconn3;
+  const double* qpA;  // [i + nsol*(iq + nq*s)]
+  const double* qpB;
+  const double* qpEl;
+  const double* qpSc;
+  const double* qwA;  // [iq + nq*(j + sa*s)]
+  const double* qwB;
+  const double* qwEl;
+  const double* qwSc;
+  // static Q-Q and Q-bonded-neighbour lists, all states concatenated
+  const int* qq_iq;
+  const int* qq_jq;
+  const int* qq_st;
+  const uint8_t* qq_soft;
+  const double* qq_par;  // [p*4]: A, B, el, score
+  int nqq;
+  const int* qqp_iq;
+  const int* qqp_j;
+  const int* qqp_st;
+  const double* qqp_par;
+  int nqqp;
+
+  // per-instance state
+  double* x;
+  double* d;
+  double* lrf;
+  int* nbr;
+  int* nnbr;
+  int cap;
+  int* need;        // [1] max neighbours seen (overflow detection)
+  uint8_t* qw_in;   // [nwat] water on the Q-water list
+  uint8_t* qp_in;   // [ncgp_solute] group on the Q-solute list
+  int* qw_list;     // compacted waters on the Q list
+  int* qp_list;     // compacted solute atoms on the Q list
+  int* nq_lists;    // [2]: n_qw, n_qp
+  double* epart;    // [nslots][QNB_NE]
+  int nslots;
+  double* ered;     // [QNB_NE]
+  // polarisation restraint state
+  double* sh_state;  // [3*QNB_MAXSH]: theta_corr, avtheta, avn_insh
+  double* w_theta;   // [nwat]
+  int* w_shell;      // [nwat]
+  int* sh_list;      // [QNB_MAXSH][nwat]
+  int* sh_rank;      // [QNB_MAXSH][nwat]
+  int* sh_n;         // [QNB_MAXSH]
+};
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, QNB_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// q_dist4 (math.f90:254-262): |b - a|^2, ((dx^2 + dy^2) + dz^2), no contraction
+__device__ __forceinline__ double dist4(const double* __restrict__ x, int a, int b) {
+#pragma clang fp contract(off)
+  double dx = x[3 * b] - x[3 * a];
+  double dy = x[3 * b + 1] - x[3 * a + 1];
+  double dz = x[3 * b + 2] - x[3 * a + 2];
+  double r = dx * dx;
+  r = r + dy * dy;
+  r = r + dz * dz;
+  return r;
+}
+
+__device__ __forceinline__ double dist4p(const double* __restrict__ x, int a, const double* p) {
+#pragma clang fp contract(off)
+  double dx = p[0] - x[3 * a];
+  double dy = p[1] - x[3 * a + 1];
+  double dz = p[2] - x[3 * a + 2];
+  double r = dx * dx;
+  r = r + dy * dy;
+  r = r + dz * dz;
+  return r;
+}
+
+// Solute-solute pair parameters computed from atom data exactly as
+// pp_int_comp / precompute_set_values_pp (simprep.f90:2952-3065, 3321-3350)
+// would tabulate them.  Returns false when the pair does not interact.
+__device__ __forceinline__ bool pp_param(const DevSys& S, int i, int j, double& A, double& B, double& el) {
+  int lo = i < j ? i : j, hi = i < j ? j : i, dk = hi - lo;
+  bool is14 = false;
+  if (dk <= 25) {
+    uint32_t bit = 1u << (dk - 1);
+    if (S.exmask[lo] & bit) return false;
+    is14 = (S.m14mask[lo] & bit) != 0;
+  } else {
+    bool ex = false;
+    for (int m = S.lx_off[lo]; m < S.lx_off[lo + 1]; m++)
+      if (S.lx_j[m] == hi) {
+        if (S.lx_kind[m] == 0) ex = true;
+        else is14 = true;
+      }
+    if (ex) return false;
+  }
+  int ti = S.iac[i], tj = S.iac[j];
+  int vdw = is14 ? 3 : S.ljcod[ti + tj * S.ntypes];
+  int k = vdw - 1;
+  if (S.ivdw_rule == 1) {
+    A = S.avdw[ti * 3 + k] * S.avdw[tj * 3 + k];
+    B = S.bvdw[ti * 3 + k] * S.bvdw[tj * 3 + k];
+  } else {
+    double tA = S.avdw[ti * 3 + k] + S.avdw[tj * 3 + k];
+    tA = tA * tA;
+    tA = tA * tA * tA;
+    double tB = S.bvdw[ti * 3 + k] * S.bvdw[tj * 3 + k];
+    A = (tA * tA) * tB;
+    B = 2.0 * tA * tB;
+  }
+  el = S.crg[i] * S.crg[j];
+  bool set = fabs(A) > 1.0e-10 || fabs(B) > 1.0e-10 || fabs(el) > 1.0e-10;
+  if (vdw == 3) el = el * S.el14;
+  return set;
+}
+
+// Pair arithmetic in precision R (double: plain path; float: mixed path).
+// u = x_j - x_i arrives in FP64 (the difference of FP64 positions), is rounded
+// to R, and the returned scalar dv is in R; callers accumulate u*dv in FP64.
+template <typename R>
+struct PairMath {
+  // nbe (nonbonded.f90:45-75): LJ + Coulomb.  Adds V_el, V_a - V_b.
+  static __device__ __forceinline__ R nbe(R ux, R uy, R uz, R A, R B, R q, double& el, double& vdw) {
+    R r2 = R(1) / (ux * ux + uy * uy + uz * uz);
+    R r = sqrt(r2);
+    R r6 = r2 * r2 * r2;
+    R r12 = r6 * r6;
+    R Vel = q * r, Va = A * r12, Vb = B * r6;
+    el += (double)Vel;
+    vdw += (double)Va - (double)Vb;
+    return r2 * (-Vel - R(12) * Va + R(6) * Vb);
+  }
+  // nbe_spc (nonbonded.f90:247-267): Coulomb only
+  static __device__ __forceinline__ R nbe_spc(R ux, R uy, R uz, R q, double& el) {
+    R r2 = R(1) / (ux * ux + uy * uy + uz * uz);
+    R r = sqrt(r2);
+    R Vel = q * r;
+    el += (double)Vel;
+    return -r2 * Vel;
+  }
+  // nbe_qx (nonbonded.f90:200-222), soft-core score sc, state weight lam
+  static __device__ __forceinline__ R nbe_qx(R r2, R r, R r6in, R A, R B, R q, R sc, R lam, double& el,
+                                             double& vdw) {
+    R r6_hc = R(1) / r6in;
+    R r6 = R(1) / (r6_hc + sc);
+    R r12 = r6 * r6;
+    R Vel = q * r, Va = A * r12, Vb = B * r6;
+    el += (double)Vel;
+    vdw += (double)Va - (double)Vb;
+    return r2 * (-Vel - (R(12) * Va - R(6) * Vb) * r6 * r6_hc) * lam;
+  }
+};

@@ -1,0 +1,941 @@
+// qnb_kernels.hip -- HIP kernels of the nonbonded engine for gfx950.
+//
+// Work decomposition is receiver-centric: one wavefront owns one charge group
+// (a solute group or a water) and computes every interaction of its atoms with
+// all partners, so pair forces need no atomics and the result does not depend
+// on scheduling.  Each unordered pair is evaluated from both sides; its energy
+// is counted once, on the side that owns the pair under the reference's parity
+// rule (nonbondene.f90:1589-1591), which also decides which side's list the
+// pair is exported in.  Q-atoms, whose partners are all waters and solute
+// atoms within Rcq, are the one many-to-one case: their forces are summed per
+// wavefront and added with one atomic per wave and component.
+#include "qnb_dev.h"
+
+namespace qnb {
+
+__device__ __forceinline__ int group_ref_atom(const DevSys& S, int g) {
+  return g < S.ncgp_solute ? S.gswitch[g] : S.nat_solute + S.sa * (g - S.ncgp_solute);
+}
+
+// the owner side of a charge-group pair (a != b) under the parity rule
+__device__ __forceinline__ bool owns(int a, int b) {
+  return (a < b && ((a + b) & 1) == 0) || (a > b && ((a + b) & 1) == 1);
+}
+
+// ---------------------------------------------------------------------------
+// LRF expansion (lrf_update, nonbondene.f90:628-725), restated on the 20
+// independent components of the symmetric tensors: acc[0] phi0, acc[1..3]
+// phi1, acc[4..9] phi2 (xx xy xz yy yz zz), acc[10..19] phi3 (xxx xxy xxz xyy
+// xyz xzz yyy yyz yzz zzz).  dr = x_i - centre in FP64, terms in R, sums FP64.
+template <typename R>
+__device__ __forceinline__ void lrf_accumulate(double* acc, double drx, double dry, double drz, double q) {
+  R dx = (R)drx, dy = (R)dry, dz = (R)drz;
+  R r2 = dx * dx + dy * dy + dz * dz;
+  R f0 = (R)q / (r2 * sqrt(r2));
+  R f1 = R(3) * f0 / r2;
+  R f2 = -f1 / r2;
+  acc[0] += (double)(f0 * r2);
+  acc[1] -= (double)(dx * f0);
+  acc[2] -= (double)(dy * f0);
+  acc[3] -= (double)(dz * f0);
+  R xx = dx * dx, xy = dx * dy, xz = dx * dz, yy = dy * dy, yz = dy * dz, zz = dz * dz;
+  acc[4] += (double)(xx * f1 - f0);
+  acc[5] += (double)(xy * f1);
+  acc[6] += (double)(xz * f1);
+  acc[7] += (double)(yy * f1 - f0);
+  acc[8] += (double)(yz * f1);
+  acc[9] += (double)(zz * f1 - f0);
+  R g = R(5) * f2, h = f2 * r2;
+  R gx = g * dx, gy = g * dy, gz = g * dz;
+  acc[10] += (double)(gx * xx - R(3) * h * dx);
+  acc[11] += (double)(gx * xy - h * dy);
+  acc[12] += (double)(gx * xz - h * dz);
+  acc[13] += (double)(gx * yy - h * dx);
+  acc[14] += (double)(gx * yz);
+  acc[15] += (double)(gx * zz - h * dx);
+  acc[16] += (double)(gy * yy - R(3) * h * dy);
+  acc[17] += (double)(gy * yz - h * dz);
+  acc[18] += (double)(gy * zz - h * dy);
+  acc[19] += (double)(gz * zz - R(3) * h * dz);
+}
+
+// index of the symmetric third-rank component (a<=b<=c sorted) in acc[10..]
+__device__ __forceinline__ int t3index(int a, int b, int c) {
+  // sort three indices
+  if (a > b) { int t = a; a = b; b = t; }
+  if (b > c) { int t = b; b = c; c = t; }
+  if (a > b) { int t = a; a = b; b = t; }
+  // lexicographic rank among (000 001 002 011 012 022 111 112 122 222)
+  const int base[3] = {0, 6, 9};
+  if (a == 0) {
+    const int r0[3][3] = {{0, 1, 2}, {0, 3, 4}, {0, 0, 5}};
+    return 10 + r0[b][c];
+  }
+  if (a == 1) return 10 + (b == 1 ? (c == 1 ? 6 : 7) : 8);
+  (void)base;
+  return 19;
+}
+
+// ---------------------------------------------------------------------------
+// Pair-list update with LRF (make_pair_lists, nonbondene.f90:749-837, SPH):
+// one wavefront per receiver charge group a.  Lanes sweep every other group b
+// (LDS-free: group reference atoms are L2-resident); a pair inside its class
+// cut-off goes to a's neighbour row through a ballot/prefix compaction, a
+// pair beyond it (and inside RcLRF) adds b's non-Q charges to a's expansion.
+template <typename R>
+__global__ __launch_bounds__(256) void k_lists_lrf(const DevSys* __restrict__ systems) {
+  const DevSys& S = systems[blockIdx.y];
+  const int a = blockIdx.x * (blockDim.x / QNB_WAVE) + threadIdx.x / QNB_WAVE;
+  if (a >= S.ncgp) return;
+  const int lane = threadIdx.x % QNB_WAVE;
+  const double* __restrict__ x = S.x;
+  const int ra = group_ref_atom(S, a);
+  const bool a_sol = a < S.ncgp_solute;
+  // centre of a (cgp_centers, nonbondene.f90:43-78): sum in atom order, then divide
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  for (int k = S.gfirst[a]; k <= S.glast[a]; k++) {
+    int at = S.cgpatom[k];
+    cx = cx + x[3 * at];
+    cy = cy + x[3 * at + 1];
+    cz = cz + x[3 * at + 2];
+  }
+  double nat = (double)(S.glast[a] - S.gfirst[a] + 1);
+  cx = cx / nat;
+  cy = cy / nat;
+  cz = cz / nat;
+  double acc[20];
+#pragma unroll
+  for (int k = 0; k < 20; k++) acc[k] = 0.0;
+  int count = 0;
+  int* __restrict__ row = S.nbr + (size_t)a * S.cap;
+  if (!S.excl[ra]) {
+    for (int b0 = 0; b0 < S.ncgp; b0 += QNB_WAVE) {
+      const int b = b0 + lane;
+      bool in = false;
+      if (b < S.ncgp && b != a) {
+        const int rb = group_ref_atom(S, b);
+        if (!S.excl[rb]) {
+          const bool b_sol = b < S.ncgp_solute;
+          const double cut2 = (a_sol && b_sol) ? S.rcpp2 : ((!a_sol && !b_sol) ? S.rcww2 : S.rcpw2);
+          const double r2 = dist4(x, ra, rb);
+          if (r2 <= cut2) {
+            in = true;
+          } else if (S.use_lrf && r2 <= S.rclrf2) {
+            if (b_sol) {
+              for (int k = S.gfirst[b]; k <= S.glast[b]; k++) {
+                int i = S.cgpatom[k];
+                if (S.iqatom[i] != 0) continue;
+                lrf_accumulate<R>(acc, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
+              }
+            } else {
+              for (int k = 0; k < S.sa; k++) {
+                int i = rb + k;
+                lrf_accumulate<R>(acc, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
+              }
+            }
+          }
+        }
+      }
+      const uint64_t m = __ballot(in);
+      const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+      if (in && pos < S.cap) row[pos] = b;
+      count += __popcll(m);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 20; k++) acc[k] = wave_sum(acc[k]);
+  if (lane == 0) {
+    S.nnbr[a] = count;
+    if (count > S.cap) atomicMax(S.need, count);
+    double* L = S.lrf + (size_t)QNB_NLRF * a;
+    L[0] = cx;
+    L[1] = cy;
+    L[2] = cz;
+    L[3] = acc[0];
+    L[4] = acc[1];
+    L[5] = acc[2];
+    L[6] = acc[3];
+    const int p2[3][3] = {{4, 5, 6}, {5, 7, 8}, {6, 8, 9}};
+    for (int k = 0; k < 3; k++)
+      for (int c = 0; c < 3; c++) L[7 + 3 * k + c] = acc[p2[k][c]];
+    for (int n = 0; n < 3; n++)
+      for (int j = 0; j < 3; j++)
+        for (int c = 0; c < 3; c++) L[16 + 3 * (j + 3 * n) + c] = acc[t3index(n, j, c)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Q-atom lists (nbqplist nonbondene.f90:3647-3742, nbqwlist 3862-3944): flags
+// and ordered compaction of the waters and solute atoms within Rcq of xpcent.
+// One block of 1024 threads per system; each thread takes a contiguous chunk
+// so the compacted order is the reference's append order.
+__global__ __launch_bounds__(1024) void k_qlists(const DevSys* __restrict__ systems) {
+  const DevSys& S = systems[blockIdx.x];
+  __shared__ int scan[1024];
+  const int t = threadIdx.x, nt = blockDim.x;
+  // waters
+  {
+    const int per = (S.nwat + nt - 1) / nt, w0 = t * per, w1 = min(S.nwat, w0 + per);
+    int c = 0;
+    for (int w = w0; w < w1; w++) {
+      int o = S.nat_solute + S.sa * w;
+      bool in = !S.excl[o] && dist4p(S.x, o, S.xp) <= S.rcq2;
+      S.qw_in[w] = in;
+      c += in;
+    }
+    scan[t] = c;
+    __syncthreads();
+    for (int off = 1; off < nt; off <<= 1) {
+      int v = t >= off ? scan[t - off] : 0;
+      __syncthreads();
+      scan[t] += v;
+      __syncthreads();
+    }
+    int pos = scan[t] - c;
+    for (int w = w0; w < w1; w++)
+      if (S.qw_in[w]) S.qw_list[pos++] = w;
+    if (t == nt - 1) S.nq_lists[0] = scan[t];
+    __syncthreads();
+  }
+  // solute atoms, group by group
+  {
+    const int per = (S.ncgp_solute + nt - 1) / nt, g0 = t * per, g1 = min(S.ncgp_solute, g0 + per);
+    int c = 0;
+    for (int g = g0; g < g1; g++) {
+      int sw = S.gswitch[g];
+      bool in = !S.excl[sw] && dist4p(S.x, sw, S.xp) <= S.rcq2;
+      S.qp_in[g] = in;
+      if (in)
+        for (int k = S.gfirst[g]; k <= S.glast[g]; k++) c += !S.qconn3[S.cgpatom[k]];
+    }
+    scan[t] = c;
+    __syncthreads();
+    for (int off = 1; off < nt; off <<= 1) {
+      int v = t >= off ? scan[t - off] : 0;
+      __syncthreads();
+      scan[t] += v;
+      __syncthreads();
+    }
+    int pos = scan[t] - c;
+    for (int g = g0; g < g1; g++)
+      if (S.qp_in[g])
+        for (int k = S.gfirst[g]; k <= S.glast[g]; k++) {
+          int i = S.cgpatom[k];
+          if (!S.qconn3[i]) S.qp_list[pos++] = i;
+        }
+    if (t == nt - 1) S.nq_lists[1] = scan[t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LRF Taylor application for one atom (lrf_taylor, nonbondene.f90:507-571)
+__device__ __forceinline__ void lrf_taylor_atom(const DevSys& S, int i, double* f, double& e) {
+  const double* L = S.lrf + (size_t)QNB_NLRF * S.atom_group[i];
+  const double* x = S.x;
+  const double drx = L[0] - x[3 * i], dry = L[1] - x[3 * i + 1], drz = L[2] - x[3 * i + 2];
+  const double* p1 = L + 4;
+  const double* p2 = L + 7;
+  const double* p3 = L + 16;
+  double t0 = drx * p2[0] + dry * p2[1] + drz * p2[2];
+  double t1 = drx * p2[3] + dry * p2[4] + drz * p2[5];
+  double t2 = drx * p2[6] + dry * p2[7] + drz * p2[8];
+  double V = L[3] + (drx * p1[0] + dry * p1[1] + drz * p1[2]) + 0.5 * (drx * t0 + dry * t1 + drz * t2);
+  const double q = S.crg[i];
+  e += 0.5 * q * V;
+  double df[3];
+  for (int k = 0; k < 3; k++) {
+    const double* P = p3 + 9 * k;
+    double u0 = drx * P[0] + dry * P[1] + drz * P[2];
+    double u1 = drx * P[3] + dry * P[4] + drz * P[5];
+    double u2 = drx * P[6] + dry * P[7] + drz * P[8];
+    df[k] = p1[k] + (drx * p2[3 * k] + dry * p2[3 * k + 1] + drz * p2[3 * k + 2]) +
+            0.5 * (drx * u0 + dry * u1 + drz * u2);
+  }
+  f[0] -= df[0] * q;
+  f[1] -= df[1] * q;
+  f[2] -= df[2] * q;
+}
+
+// SCAAS radial restraint on one water oxygen (restrain_solvent,
+// nonbondene.f90:6466-6544, 3-site branch)
+__device__ __forceinline__ void radial_restraint(const DevSys& S, int i, double shift, double* f, double& e) {
+  const double* x = S.x;
+  double vx = x[3 * i] - S.xw[0], vy = x[3 * i + 1] - S.xw[1], vz = x[3 * i + 2] - S.xw[2];
+  double b = sqrt(vx * vx + vy * vy + vz * vz);
+  double db = b - (S.rwat - shift);
+  double erst, dv;
+  if (db > 0.0) {
+    erst = 0.5 * S.fk_wsphere * (db * db) - S.Dwmz;
+    dv = S.fk_wsphere * db / b;
+  } else if (b > 0.0) {
+    double fexp = exp(S.awmz * db);
+    erst = S.Dwmz * (fexp * fexp - 2.0 * fexp);
+    dv = -2.0 * S.Dwmz * S.awmz * (fexp - fexp * fexp) / b;
+  } else {
+    erst = 0.0;
+    dv = 0.0;
+  }
+  f[0] += vx * dv;
+  f[1] += vy * dv;
+  f[2] += vz * dv;
+  e += erst;
+}
+
+// term bits (include/qnb.h enum qnb_term)
+enum : uint32_t { T_WW = 1, T_QW = 2, T_PW = 4, T_PP = 8, T_QP = 16, T_LRF = 32, T_QQ = 64, T_QQP = 128,
+                  T_RAD = 256, T_POL = 512 };
+
+// ---------------------------------------------------------------------------
+// Receiver-side forces: one wavefront per charge group a.  Water receivers
+// take the rigid 3-site fast path (9 atom pairs per water partner, the O-O pair
+// with LJ, nonbond_ww_spc nonbondene.f90:5886-5975); solute receivers walk
+// their atoms one at a time.  Writes (not adds) d for every atom of a, and the
+// wave's energy partials into epart[a].
+template <typename R, int NS>
+__global__ __launch_bounds__(256) void k_forces(const DevSys* __restrict__ systems, uint32_t mask,
+                                                const double* __restrict__ tfree) {
+  const DevSys& S = systems[blockIdx.y];
+  const int a = blockIdx.x * (blockDim.x / QNB_WAVE) + threadIdx.x / QNB_WAVE;
+  if (a >= S.ncgp) return;
+  const int lane = threadIdx.x % QNB_WAVE;
+  const double* __restrict__ x = S.x;
+  double* __restrict__ d = S.d;
+  const int nn = min(S.nnbr[a], S.cap);
+  const int* __restrict__ row = S.nbr + (size_t)a * S.cap;
+  double e[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) e[k] = 0.0;
+  double eq[4 * NS];  // per state: qp_el, qp_vdw, qw_el, qw_vdw
+#pragma unroll
+  for (int k = 0; k < 4 * NS; k++) eq[k] = 0.0;
+  const int ra = group_ref_atom(S, a);
+  const bool aexcl = S.excl[ra];
+
+  if (a >= S.ncgp_solute) {
+    // ------------------------------------------------ water receiver
+    const int aw = a - S.ncgp_solute;
+    const int o = ra, sa = S.sa;
+    double f[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) f[k][0] = f[k][1] = f[k][2] = 0.0;
+    double xi[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      xi[k][0] = x[3 * (o + k)];
+      xi[k][1] = x[3 * (o + k) + 1];
+      xi[k][2] = x[3 * (o + k) + 2];
+    }
+    if (sa == 3 && S.spc) {
+      for (int p = lane; p < nn; p += QNB_WAVE) {
+        const int b = row[p];
+        if (b >= S.ncgp_solute) {
+          if (!(mask & T_WW)) continue;
+          const int jo = S.nat_solute + 3 * (b - S.ncgp_solute);
+          const bool own = owns(a, b);
+          double el = 0.0, vdw = 0.0;
+#pragma unroll
+          for (int la = 0; la < 3; la++)
+#pragma unroll
+            for (int ka = 0; ka < 3; ka++) {
+              const int j = jo + ka;
+              const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
+              const int k = la + 3 * ka;
+              R dv;
+              if (la == 0 && ka == 0)
+                dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.wwA[0], (R)S.wwB[0], (R)S.wwEl[0], el, vdw);
+              else
+                dv = PairMath<R>::nbe_spc((R)ux, (R)uy, (R)uz, (R)S.wwEl[k], el);
+              f[la][0] -= (double)((R)ux * dv);
+              f[la][1] -= (double)((R)uy * dv);
+              f[la][2] -= (double)((R)uz * dv);
+            }
+          if (own) {
+            e[E_WW_EL] += el;
+            e[E_WW_VDW] += vdw;
+          }
+        } else {
+          if (!(mask & T_PW)) continue;
+          for (int k = S.gfirst[b]; k <= S.glast[b]; k++) {
+            const int j = S.cgpatom[k];
+            if (S.iqatom[j] != 0) continue;
+            double el = 0.0, vdw = 0.0;
+#pragma unroll
+            for (int la = 0; la < 3; la++) {
+              const size_t t = (size_t)j + (size_t)S.nat_solute * la;
+              const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
+              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], el, vdw);
+              f[la][0] -= (double)((R)ux * dv);
+              f[la][1] -= (double)((R)uy * dv);
+              f[la][2] -= (double)((R)uz * dv);
+            }
+          }
+        }
+      }
+    } else {
+      // general solvent: every atom pair with its table entry
+      for (int p = lane; p < nn; p += QNB_WAVE) {
+        const int b = row[p];
+        if (b >= S.ncgp_solute) {
+          if (!(mask & T_WW)) continue;
+          const int jo = S.nat_solute + sa * (b - S.ncgp_solute);
+          const bool own = owns(a, b);
+          double el = 0.0, vdw = 0.0;
+          for (int la = 0; la < sa && la < 3; la++)
+            for (int ka = 0; ka < sa; ka++) {
+              const int j = jo + ka, k = la + sa * ka;
+              const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
+              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.wwA[k], (R)S.wwB[k], (R)S.wwEl[k], el, vdw);
+              f[la][0] -= (double)((R)ux * dv);
+              f[la][1] -= (double)((R)uy * dv);
+              f[la][2] -= (double)((R)uz * dv);
+            }
+          if (own) {
+            e[E_WW_EL] += el;
+            e[E_WW_VDW] += vdw;
+          }
+        } else {
+          if (!(mask & T_PW)) continue;
+          for (int k = S.gfirst[b]; k <= S.glast[b]; k++) {
+            const int j = S.cgpatom[k];
+            if (S.iqatom[j] != 0) continue;
+            double el = 0.0, vdw = 0.0;
+            for (int la = 0; la < sa && la < 3; la++) {
+              const size_t t = (size_t)j + (size_t)S.nat_solute * la;
+              const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
+              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], el, vdw);
+              f[la][0] -= (double)((R)ux * dv);
+              f[la][1] -= (double)((R)uy * dv);
+              f[la][2] -= (double)((R)uz * dv);
+            }
+          }
+        }
+      }
+    }
+    // Q-atom - water (nonbond_qw_spc nonbondene.f90:5658-5767; nonbond_qw 5339-5418)
+    if ((mask & T_QW) && S.nqat > 0 && S.qw_in[aw]) {
+      const int nq = S.nqat, ns = S.nstates;
+      for (int iq = lane; iq < nq; iq += QNB_WAVE) {
+        const int i = S.iqseq[iq];
+        for (int k = 0; k < sa && k < 3; k++) {
+          const double ux = xi[k][0] - x[3 * i], uy = xi[k][1] - x[3 * i + 1], uz = xi[k][2] - x[3 * i + 2];
+          const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
+          R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
+          R r = sqrt(r2);
+          R dvs = R(0);
+          if (k == 0 || !S.spc) {
+            R r6 = r2 * r2 * r2;
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+              if (s >= ns) break;
+              const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
+              dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)S.qwA[t], (R)S.qwB[t], (R)S.qwEl[t], (R)S.qwSc[t],
+                                         (R)S.lambda[s], eq[4 * s + 2], eq[4 * s + 3]);
+            }
+          } else {
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+              if (s >= ns) break;
+              const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
+              R Vel = (R)S.qwEl[t] * r;  // nbe_qspc (nonbonded.f90:226-243)
+              eq[4 * s + 2] += (double)Vel;
+              dvs += -r2 * Vel * (R)S.lambda[s];
+            }
+          }
+          f[k][0] += (double)(rux * dvs);
+          f[k][1] += (double)(ruy * dvs);
+          f[k][2] += (double)(ruz * dvs);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      f[k][0] = wave_sum(f[k][0]);
+      f[k][1] = wave_sum(f[k][1]);
+      f[k][2] = wave_sum(f[k][2]);
+    }
+    if (lane < 3 && lane < sa) {
+      const int i = o + lane;
+      double fi[3] = {f[lane][0], f[lane][1], f[lane][2]};
+      if ((mask & T_LRF) && S.use_lrf && !S.excl[i] && S.iqatom[i] == 0) lrf_taylor_atom(S, i, fi, e[E_LRF]);
+      if ((mask & T_RAD) && lane == 0 && !aexcl && S.sa == 3) {
+        const double T = tfree[blockIdx.y];
+        const double shift = S.fk_wsphere != 0.0 ? sqrt(S.Boltz * T / S.fk_wsphere) : 0.0;
+        radial_restraint(S, i, shift, fi, e[E_RAD]);
+      }
+      d[3 * i] = fi[0];
+      d[3 * i + 1] = fi[1];
+      d[3 * i + 2] = fi[2];
+    }
+  } else {
+    // ------------------------------------------------ solute receiver
+    const bool qin = (mask & T_QP) && S.nqat > 0 && S.qp_in[a];
+    for (int ka = S.gfirst[a]; ka <= S.glast[a]; ka++) {
+      const int i = S.cgpatom[ka];
+      double fi[3] = {0.0, 0.0, 0.0};
+      const bool iq_atom = S.iqatom[i] != 0;
+      if (!iq_atom && !aexcl) {
+        const double xix = x[3 * i], xiy = x[3 * i + 1], xiz = x[3 * i + 2];
+        for (int p = lane; p < nn; p += QNB_WAVE) {
+          const int b = row[p];
+          if (b < S.ncgp_solute) {
+            if (!(mask & T_PP)) continue;
+            const bool own = owns(a, b);
+            for (int kb = S.gfirst[b]; kb <= S.glast[b]; kb++) {
+              const int j = S.cgpatom[kb];
+              if (S.iqatom[j] != 0) continue;
+              double A, B, el;
+              if (!pp_param(S, i, j, A, B, el)) continue;
+              const double ux = x[3 * j] - xix, uy = x[3 * j + 1] - xiy, uz = x[3 * j + 2] - xiz;
+              double ee = 0.0, ev = 0.0;
+              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)A, (R)B, (R)el, ee, ev);
+              fi[0] -= (double)((R)ux * dv);
+              fi[1] -= (double)((R)uy * dv);
+              fi[2] -= (double)((R)uz * dv);
+              if (own) {
+                e[E_PP_EL] += ee;
+                e[E_PP_VDW] += ev;
+              }
+            }
+          } else {
+            if (!(mask & T_PW)) continue;
+            const int jo = S.nat_solute + S.sa * (b - S.ncgp_solute);
+            for (int k = 0; k < S.sa; k++) {
+              const int j = jo + k;
+              const size_t t = (size_t)i + (size_t)S.nat_solute * k;
+              const double ux = x[3 * j] - xix, uy = x[3 * j + 1] - xiy, uz = x[3 * j + 2] - xiz;
+              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], e[E_PW_EL],
+                                      e[E_PW_VDW]);
+              fi[0] -= (double)((R)ux * dv);
+              fi[1] -= (double)((R)uy * dv);
+              fi[2] -= (double)((R)uz * dv);
+            }
+          }
+        }
+        // pairs inside the group itself (ig == jgr, i < j, nonbondene.f90:1610)
+        if (mask & T_PP) {
+          const int n_in = S.glast[a] - S.gfirst[a] + 1;
+          for (int p = lane; p < n_in; p += QNB_WAVE) {
+            const int j = S.cgpatom[S.gfirst[a] + p];
+            if (j == i || S.iqatom[j] != 0) continue;
+            double A, B, el;
+            if (!pp_param(S, i, j, A, B, el)) continue;
+            const double ux = x[3 * j] - xix, uy = x[3 * j + 1] - xiy, uz = x[3 * j + 2] - xiz;
+            double ee = 0.0, ev = 0.0;
+            R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)A, (R)B, (R)el, ee, ev);
+            fi[0] -= (double)((R)ux * dv);
+            fi[1] -= (double)((R)uy * dv);
+            fi[2] -= (double)((R)uz * dv);
+            if (i < j) {
+              e[E_PP_EL] += ee;
+              e[E_PP_VDW] += ev;
+            }
+          }
+        }
+      }
+      // Q-atom - solute atom (nonbond_qp, nonbondene.f90:5157-5230)
+      if (qin && !iq_atom && !S.qconn3[i]) {
+        const int nq = S.nqat, ns = S.nstates, nsol = S.nat_solute;
+        const double xix = x[3 * i], xiy = x[3 * i + 1], xiz = x[3 * i + 2];
+        for (int iq = lane; iq < nq; iq += QNB_WAVE) {
+          const int q = S.iqseq[iq];
+          const double ux = xix - x[3 * q], uy = xiy - x[3 * q + 1], uz = xiz - x[3 * q + 2];
+          const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
+          R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
+          R r = sqrt(r2);
+          R r6 = r2 * r2 * r2;
+#pragma unroll
+          for (int s = 0; s < NS; s++) {
+            if (s >= ns) break;
+            const size_t t = (size_t)i + (size_t)nsol * ((size_t)iq + (size_t)nq * s);
+            R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)S.qpA[t], (R)S.qpB[t], (R)S.qpEl[t], (R)S.qpSc[t],
+                                       (R)S.lambda[s], eq[4 * s + 0], eq[4 * s + 1]);
+            fi[0] += (double)(rux * dv);
+            fi[1] += (double)(ruy * dv);
+            fi[2] += (double)(ruz * dv);
+          }
+        }
+      }
+      fi[0] = wave_sum(fi[0]);
+      fi[1] = wave_sum(fi[1]);
+      fi[2] = wave_sum(fi[2]);
+      if (lane == 0) {
+        if ((mask & T_LRF) && S.use_lrf && !S.excl[i] && !iq_atom) lrf_taylor_atom(S, i, fi, e[E_LRF]);
+        d[3 * i] = fi[0];
+        d[3 * i + 1] = fi[1];
+        d[3 * i + 2] = fi[2];
+      }
+    }
+  }
+  // energy partials of this wave (lane 0 holds the LRF/radial terms)
+  double* ep = S.epart + (size_t)a * QNB_NE;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    double v = wave_sum(e[k]);
+    if (lane == 0) ep[k] = v;
+  }
+#pragma unroll
+  for (int s = 0; s < QNB_MAXST; s++) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (s < NS) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) v[k] = wave_sum(eq[4 * (s < NS ? s : 0) + k]);
+    }
+    if (lane == 0) {
+      ep[E_Q0 + 6 * s + 0] = 0.0;
+      ep[E_Q0 + 6 * s + 1] = 0.0;
+      ep[E_Q0 + 6 * s + 2] = v[0];
+      ep[E_Q0 + 6 * s + 3] = v[1];
+      ep[E_Q0 + 6 * s + 4] = v[2];
+      ep[E_Q0 + 6 * s + 5] = v[3];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Q-atom side of the Q-water and Q-solute pairs: wave = (Q-atom, 64 partners);
+// the wave's sum is added to the Q-atom's gradient with atomics (few per step).
+template <typename R, int NS>
+__global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ systems, uint32_t mask,
+                                                 int chunks_per_q) {
+  const DevSys& S = systems[blockIdx.y];
+  const int wv = blockIdx.x * (blockDim.x / QNB_WAVE) + threadIdx.x / QNB_WAVE;
+  const int lane = threadIdx.x % QNB_WAVE;
+  const int iq = wv / chunks_per_q, ch = wv % chunks_per_q;
+  if (iq >= S.nqat) return;
+  const int nqw = S.nq_lists[0], nqp = S.nq_lists[1];
+  const int nw_ch = (mask & T_QW) ? (nqw + QNB_WAVE - 1) / QNB_WAVE : 0;
+  const int np_ch = (mask & T_QP) ? (nqp + QNB_WAVE - 1) / QNB_WAVE : 0;
+  const double* __restrict__ x = S.x;
+  const int i = S.iqseq[iq];
+  const int nq = S.nqat, ns = S.nstates, sa = S.sa;
+  double f[3] = {0.0, 0.0, 0.0};
+  double dummy_e = 0.0, dummy_v = 0.0;
+  for (int c = ch; c < nw_ch + np_ch; c += chunks_per_q) {
+    const int p = c * QNB_WAVE + lane;
+    if (c < nw_ch) {
+      if (p >= nqw) continue;
+      const int o = S.nat_solute + sa * S.qw_list[p];
+      for (int k = 0; k < sa; k++) {
+        const int j = o + k;
+        const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
+        const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
+        R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
+        R r = sqrt(r2);
+        R dvs = R(0);
+        if (k == 0 || !S.spc) {
+          R r6 = r2 * r2 * r2;
+#pragma unroll
+          for (int s = 0; s < NS; s++) {
+            if (s >= ns) break;
+            const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
+            dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)S.qwA[t], (R)S.qwB[t], (R)S.qwEl[t], (R)S.qwSc[t],
+                                       (R)S.lambda[s], dummy_e, dummy_v);
+          }
+        } else {
+#pragma unroll
+          for (int s = 0; s < NS; s++) {
+            if (s >= ns) break;
+            const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
+            dvs += -r2 * ((R)S.qwEl[t] * r) * (R)S.lambda[s];
+          }
+        }
+        f[0] -= (double)(rux * dvs);
+        f[1] -= (double)(ruy * dvs);
+        f[2] -= (double)(ruz * dvs);
+      }
+    } else {
+      const int pp = p - nw_ch * QNB_WAVE;
+      if (pp >= nqp) continue;
+      const int j = S.qp_list[pp];
+      const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
+      const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
+      R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
+      R r = sqrt(r2);
+      R r6 = r2 * r2 * r2;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        if (s >= ns) break;
+        const size_t t = (size_t)j + (size_t)S.nat_solute * ((size_t)iq + (size_t)nq * s);
+        R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)S.qpA[t], (R)S.qpB[t], (R)S.qpEl[t], (R)S.qpSc[t],
+                                   (R)S.lambda[s], dummy_e, dummy_v);
+        f[0] -= (double)(rux * dv);
+        f[1] -= (double)(ruy * dv);
+        f[2] -= (double)(ruz * dv);
+      }
+    }
+  }
+  f[0] = wave_sum(f[0]);
+  f[1] = wave_sum(f[1]);
+  f[2] = wave_sum(f[2]);
+  if (lane == 0 && (f[0] != 0.0 || f[1] != 0.0 || f[2] != 0.0)) {
+    unsafeAtomicAdd(&S.d[3 * i], f[0]);
+    unsafeAtomicAdd(&S.d[3 * i + 1], f[1]);
+    unsafeAtomicAdd(&S.d[3 * i + 2], f[2]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Static Q-Q (nonbond_qq, nonbondene.f90:5013-5082, pair kernel nbe_qq
+// nonbonded.f90:111-150) and Q-bonded-neighbour (nonbond_qqp, 5085-5153)
+// lists: one wave per system; both atoms updated with atomics.
+template <typename R>
+__global__ __launch_bounds__(64) void k_qq(const DevSys* __restrict__ systems, uint32_t mask) {
+  const DevSys& S = systems[blockIdx.x];
+  const int slot = S.ncgp;
+  const int lane = threadIdx.x;
+  const double* __restrict__ x = S.x;
+  double eq[QNB_MAXST][4];
+  for (int s = 0; s < QNB_MAXST; s++) eq[s][0] = eq[s][1] = eq[s][2] = eq[s][3] = 0.0;
+  if (mask & T_QQ)
+    for (int p = lane; p < S.nqq; p += QNB_WAVE) {
+      const int s = S.qq_st[p];
+      const int i = S.iqseq[S.qq_iq[p]], j = S.iqseq[S.qq_jq[p]];
+      const double* pr = S.qq_par + 4 * p;
+      const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
+      const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
+      R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
+      R r = sqrt(r2);
+      R r6d = r2 * r2 * r2;
+      R r6_hc = R(1) / r6d;
+      R r6 = R(1) / (r6_hc + (R)pr[3]);
+      R r12 = r6 * r6;
+      R lam = (R)S.lambda[s];
+      R Vel = (R)pr[2] * r, Va, Vb, dv;
+      if (S.qq_soft[p]) {
+        Vb = R(0);
+        Va = (R)pr[0] * exp(-(R)pr[1] / r);
+        dv = r2 * (-Vel - (R)pr[1] * Va / r) * lam;
+      } else {
+        Va = (R)pr[0] * r12;
+        Vb = (R)pr[1] * r6;
+        dv = r2 * (-Vel - (R(12) * Va - R(6) * Vb) * r6 * r6_hc) * lam;
+      }
+      eq[s][0] += (double)Vel;
+      eq[s][1] += (double)Va - (double)Vb;
+      const double fx = (double)(rux * dv), fy = (double)(ruy * dv), fz = (double)(ruz * dv);
+      unsafeAtomicAdd(&S.d[3 * i], -fx);
+      unsafeAtomicAdd(&S.d[3 * i + 1], -fy);
+      unsafeAtomicAdd(&S.d[3 * i + 2], -fz);
+      unsafeAtomicAdd(&S.d[3 * j], fx);
+      unsafeAtomicAdd(&S.d[3 * j + 1], fy);
+      unsafeAtomicAdd(&S.d[3 * j + 2], fz);
+    }
+  if (mask & T_QQP)
+    for (int p = lane; p < S.nqqp; p += QNB_WAVE) {
+      const int s = S.qqp_st[p];
+      const int iq = S.qqp_iq[p];
+      const int i = S.iqseq[iq], j = S.qqp_j[p];
+      const double* pr = S.qqp_par + 4 * p;
+      const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
+      const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
+      R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
+      R r = sqrt(r2);
+      R r6 = r2 * r2 * r2;
+      R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)pr[0], (R)pr[1], (R)pr[2], (R)pr[3], (R)S.lambda[s], eq[s][2],
+                                 eq[s][3]);
+      const double fx = (double)(rux * dv), fy = (double)(ruy * dv), fz = (double)(ruz * dv);
+      unsafeAtomicAdd(&S.d[3 * i], -fx);
+      unsafeAtomicAdd(&S.d[3 * i + 1], -fy);
+      unsafeAtomicAdd(&S.d[3 * i + 2], -fz);
+      unsafeAtomicAdd(&S.d[3 * j], fx);
+      unsafeAtomicAdd(&S.d[3 * j + 1], fy);
+      unsafeAtomicAdd(&S.d[3 * j + 2], fz);
+    }
+  double* ep = S.epart + (size_t)slot * QNB_NE;
+  for (int k = 0; k < E_Q0; k++)
+    if (lane == 0) ep[k] = 0.0;
+  for (int s = 0; s < QNB_MAXST; s++) {
+    double v0 = wave_sum(eq[s][0]), v1 = wave_sum(eq[s][1]), v2 = wave_sum(eq[s][2]), v3 = wave_sum(eq[s][3]);
+    if (lane == 0) {
+      ep[E_Q0 + 6 * s + 0] = v0;
+      ep[E_Q0 + 6 * s + 1] = v1;
+      ep[E_Q0 + 6 * s + 2] = v2;
+      ep[E_Q0 + 6 * s + 3] = v3;
+      ep[E_Q0 + 6 * s + 4] = 0.0;
+      ep[E_Q0 + 6 * s + 5] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SCAAS polarisation restraint (watpol, nonbondene.f90:6548-6746, 3-site
+// solvent, md = .true.): one block per system.  The reference's per-shell
+// selection sort by theta is a stable ascending sort; here each member's rank
+// is counted directly (ties broken by water index, as the selection sort does).
+__device__ __forceinline__ void wat_vectors(const DevSys& S, int i, double* rmu, double& rm, double* rcu,
+                                            double& rc) {
+  const double* x = S.x;
+  double m[3];
+  for (int c = 0; c < 3; c++) m[c] = (x[3 * (i + 1) + c] + x[3 * (i + 2) + c]) - x[3 * i + c] * 2.0;
+  rm = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  for (int c = 0; c < 3; c++) rmu[c] = m[c] / rm;
+  double v[3];
+  for (int c = 0; c < 3; c++) v[c] = x[3 * i + c] - S.xw[c];
+  rc = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  for (int c = 0; c < 3; c++) rcu[c] = v[c] / rc;
+}
+
+__global__ __launch_bounds__(1024) void k_watpol(const DevSys* __restrict__ systems, const int* __restrict__ istep) {
+  const DevSys& S = systems[blockIdx.x];
+  const int slot = S.ncgp + 1;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int nw = S.nwat, nsh = S.nshell;
+  const double pi = 3.141592653589793;
+  __shared__ int cnt[QNB_MAXSH];
+  __shared__ double red[1024 / QNB_WAVE][2 * QNB_MAXSH + 1];
+  if (t < QNB_MAXSH) cnt[t] = 0;
+  __syncthreads();
+  // theta and shell of every water, in water order per shell (atomic slot,
+  // then ordered by rank over (theta, index) so slot order does not matter)
+  for (int w = t; w < nw; w += nt) {
+    const int i = S.nat_solute + S.sa * w;
+    int sh = -1;
+    double th = 0.0;
+    if (!S.excl[i]) {
+      double rmu[3], rcu[3], rm, rc;
+      wat_vectors(S, i, rmu, rm, rcu, rc);
+      double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+      scp = scp > 1.0 ? 1.0 : (scp < -1.0 ? -1.0 : scp);
+      th = acos(scp);
+      if (rc > S.sh_rout[nsh - 1] - S.sh_dr[nsh - 1]) {
+        int is;
+        for (is = nsh - 1; is >= 1; is--)
+          if (rc <= S.sh_rout[is]) break;
+        if (is < 1) is = 0;
+        sh = is;
+      }
+    }
+    S.w_theta[w] = th;
+    S.w_shell[w] = sh;
+    if (sh >= 0) {
+      int pos = atomicAdd(&cnt[sh], 1);
+      S.sh_list[(size_t)sh * nw + pos] = w;
+    }
+  }
+  __syncthreads();
+  // rank of each member inside its shell
+  for (int sh = 0; sh < nsh; sh++) {
+    const int n = cnt[sh];
+    const int* L = S.sh_list + (size_t)sh * nw;
+    for (int m = t; m < n; m += nt) {
+      const int w = L[m];
+      const double th = S.w_theta[w];
+      int r = 0;
+      for (int k = 0; k < n; k++) {
+        const int w2 = L[k];
+        const double t2 = S.w_theta[w2];
+        r += (t2 < th) || (t2 == th && w2 < w);
+      }
+      S.sh_rank[(size_t)sh * nw + m] = r;
+    }
+  }
+  // theta_corr feedback every itdis steps (nonbondene.f90:6653-6666)
+  double* st = S.sh_state;  // theta_corr[8], avtheta[8], avn_insh[8]
+  const int step = istep[blockIdx.x];
+  const int itd = (int)llround(S.itdis);
+  if (t == 0 && step > 0 && itd > 0 && step % itd == 0) {
+    for (int is = 0; is < nsh; is++) {
+      st[QNB_MAXSH + is] = st[QNB_MAXSH + is] / S.itdis;
+      st[2 * QNB_MAXSH + is] = st[2 * QNB_MAXSH + is] / S.itdis;
+      st[is] = st[is] + st[QNB_MAXSH + is] - acos(S.sh_cstb[is]);
+      st[QNB_MAXSH + is] = 0.0;
+      st[2 * QNB_MAXSH + is] = 0.0;
+    }
+  }
+  __syncthreads();
+  double epol = 0.0;
+  double avt[QNB_MAXSH];
+  for (int is = 0; is < QNB_MAXSH; is++) avt[is] = 0.0;
+  for (int sh = 0; sh < nsh; sh++) {
+    const int n = cnt[sh];
+    const double tc = st[sh];
+    for (int m = t; m < n; m += nt) {
+      const int w = S.sh_list[(size_t)sh * nw + m];
+      const int il = S.sh_rank[(size_t)sh * nw + m] + 1;
+      const double th = S.w_theta[w];
+      double arg = 1.0 + (1.0 - 2.0 * (double)il) / (double)n;
+      double th0 = acos(arg);
+      th0 = th0 - 3.0 * sin(th0) * S.sh_cstb[sh] / 2.0;
+      if (th0 < 0.0) th0 = 0.0;
+      if (th0 > pi) th0 = pi;
+      avt[sh] += th;
+      const double dth = th - th0 + tc;
+      epol += 0.5 * S.fkwpol * (dth * dth);
+      const double dv = S.fkwpol * dth;
+      const int i = S.nat_solute + S.sa * w;
+      double rmu[3], rcu[3], rm, rc;
+      wat_vectors(S, i, rmu, rm, rcu, rc);
+      double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+      scp = scp > 1.0 ? 1.0 : (scp < -1.0 ? -1.0 : scp);
+      double f0 = sin(acos(scp));
+      if (fabs(f0) < 1.0e-10) f0 = 1.0e-10;
+      f0 = -1.0 / f0;
+      f0 = dv * f0;
+      for (int c = 0; c < 3; c++) {
+        const double av = rcu[c] - rmu[c] * scp;
+        const double f1 = (av * (-2.0)) / rm, f3 = av / rm, f2 = (rmu[c] - rcu[c] * scp) / rc;
+        S.d[3 * i + c] += (f1 + f2) * f0;
+        S.d[3 * (i + 1) + c] += f3 * f0;
+        S.d[3 * (i + 2) + c] += f3 * f0;
+      }
+    }
+  }
+  // block reductions: energy and per-shell theta sums
+  const int wid = t / QNB_WAVE, lane = t % QNB_WAVE;
+  double v = wave_sum(epol);
+  if (lane == 0) red[wid][0] = v;
+  for (int is = 0; is < nsh; is++) {
+    double s2 = wave_sum(avt[is]);
+    if (lane == 0) red[wid][1 + is] = s2;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double etot = 0.0;
+    for (int k = 0; k < nt / QNB_WAVE; k++) etot += red[k][0];
+    for (int is = 0; is < nsh; is++) {
+      if (cnt[is] == 0) continue;
+      double s2 = 0.0;
+      for (int k = 0; k < nt / QNB_WAVE; k++) s2 += red[k][1 + is];
+      st[QNB_MAXSH + is] += s2 / (double)cnt[is];
+      st[2 * QNB_MAXSH + is] += (double)cnt[is];
+    }
+    for (int is = 0; is < nsh; is++) S.sh_n[is] = cnt[is];
+    double* ep = S.epart + (size_t)slot * QNB_NE;
+    for (int k = 0; k < QNB_NE; k++) ep[k] = 0.0;
+    ep[E_POL] = etot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-order reduction of the energy partials: slots [0, ncgp) from
+// k_forces, plus the qq slot (ncgp) and the watpol slot (ncgp+1) when those
+// kernels ran.
+__global__ void k_reduce_energies(const DevSys* __restrict__ systems, int with_qq, int with_pol) {
+  const DevSys& S = systems[blockIdx.x];
+  const int k = threadIdx.x;
+  if (k >= QNB_NE) return;
+  double s = 0.0;
+  for (int a = 0; a < S.ncgp; a++) s += S.epart[(size_t)a * QNB_NE + k];
+  if (with_qq && S.nqat > 0) s += S.epart[(size_t)S.ncgp * QNB_NE + k];
+  if (with_pol && S.nwat > 0 && S.wpol && S.nshell > 0) s += S.epart[(size_t)(S.ncgp + 1) * QNB_NE + k];
+  S.ered[k] = s;
+}
+
+// explicit instantiations used by the host code
+template __global__ void k_lists_lrf<double>(const DevSys*);
+template __global__ void k_lists_lrf<float>(const DevSys*);
+#define QNB_INST(R, NS)                                                            \
+  template __global__ void k_forces<R, NS>(const DevSys*, uint32_t, const double*); \
+  template __global__ void k_qforces<R, NS>(const DevSys*, uint32_t, int);
+QNB_INST(double, 1)
+QNB_INST(double, 2)
+QNB_INST(double, 4)
+QNB_INST(double, 8)
+QNB_INST(float, 1)
+QNB_INST(float, 2)
+QNB_INST(float, 4)
+QNB_INST(float, 8)
+template __global__ void k_qq<double>(const DevSys*, uint32_t);
+template __global__ void k_qq<float>(const DevSys*, uint32_t);
+
+}  // namespace qnb

@@ -1,0 +1,106 @@
+"""Host-side mirror of the reference's nonbonded interface over the HIP engine.
+
+`NonbondedEngine` exposes the entry points Qdyn6's MD loop calls in the
+reference's NONBONDENE/POTENE modules (make_pair_lists, pot_energy_nonbonds,
+nonbond_qq/qqp, restrain_solvent/watpol), implemented by the C ABI of
+q6_amd/libq6nb.so.  There is no CPU path: construction fails when the library
+or a GPU is missing.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+_pd = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+
+
+class NonbondedEngine:
+    def __init__(self, system, precision=abi.QNB_PREC_FP64, **par_overrides):
+        self.lib = abi.load()
+        self.sys = system
+        self._s, self._p = system.to_structs(precision=precision, **par_overrides)
+        self._p.precision = precision
+        h = C.c_void_p()
+        abi.check(self.lib, self.lib.qnb_create(C.byref(self._s), C.byref(self._p), C.byref(h)), "qnb_create")
+        self.h = h
+        self.nst = int(system.n["nstates"])
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qnb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # make_pair_lists (src/nonbondene.f90:749-837)
+    def make_pair_lists(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        abi.check(self.lib, self.lib.qnb_make_pair_lists(self.h, _pd(x)), "qnb_make_pair_lists")
+
+    def eval_terms(self, mask, x, Tfree=0.0, istep=0, d=None):
+        x = np.ascontiguousarray(x, np.float64)
+        if d is None:
+            d = np.zeros_like(x)
+        E = abi.QnbEnergies()
+        EQ = (abi.QnbQEnergies * self.nst)()
+        abi.check(self.lib, self.lib.qnb_eval_terms(self.h, mask, _pd(x), _pd(d), Tfree, istep, C.byref(E), EQ),
+                  "qnb_eval_terms")
+        return d, E.as_dict(), [q.as_dict() for q in EQ]
+
+    # pot_energy_nonbonds (src/potene.f90:320-380)
+    def pot_energy_nonbonds(self, x, d=None):
+        x = np.ascontiguousarray(x, np.float64)
+        if d is None:
+            d = np.zeros_like(x)
+        E = abi.QnbEnergies()
+        EQ = (abi.QnbQEnergies * self.nst)()
+        abi.check(self.lib, self.lib.qnb_nonbonded(self.h, _pd(x), _pd(d), C.byref(E), EQ), "qnb_nonbonded")
+        return d, E.as_dict(), [q.as_dict() for q in EQ]
+
+    # nonbond_qq + nonbond_qqp (src/nonbondene.f90:5013-5153)
+    def nonbond_qq(self, x, d=None, EQ=None):
+        x = np.ascontiguousarray(x, np.float64)
+        if d is None:
+            d = np.zeros_like(x)
+        eq = (abi.QnbQEnergies * self.nst)()
+        if EQ is not None:
+            for k, q in enumerate(EQ):
+                for n, v in q.items():
+                    setattr(eq[k], n, v)
+        abi.check(self.lib, self.lib.qnb_qq(self.h, _pd(x), _pd(d), eq), "qnb_qq")
+        return d, [q.as_dict() for q in eq]
+
+    # restrain_solvent + watpol (src/nonbondene.f90:6466-6746)
+    def restraints(self, x, Tfree, istep, d=None):
+        x = np.ascontiguousarray(x, np.float64)
+        if d is None:
+            d = np.zeros_like(x)
+        E = abi.QnbEnergies()
+        abi.check(self.lib, self.lib.qnb_restraints(self.h, _pd(x), _pd(d), Tfree, istep, C.byref(E)),
+                  "qnb_restraints")
+        return d, E.as_dict()
+
+    def pairs(self, which):
+        n = C.c_int64()
+        abi.check(self.lib, self.lib.qnb_get_pairs(self.h, which, None, 0, C.byref(n)), "qnb_get_pairs")
+        ab = np.zeros((max(n.value, 1), 2), np.int32)
+        abi.check(self.lib, self.lib.qnb_get_pairs(self.h, which, ab.ctypes.data_as(C.POINTER(C.c_int32)), n.value,
+                                                   C.byref(n)), "qnb_get_pairs")
+        return ab[:n.value]
+
+    def lrf(self):
+        out = np.zeros((int(self.sys.n["ncgp"]), 43))
+        abi.check(self.lib, self.lib.qnb_get_lrf(self.h, _pd(out)), "qnb_get_lrf")
+        return out
+
+    def shell_state(self):
+        n = int(self.sys.par["nwpolr_shell"])
+        st = np.zeros(3 * max(n, 1))
+        abi.check(self.lib, self.lib.qnb_get_shell_state(self.h, _pd(st)), "qnb_get_shell_state")
+        return st[:3 * n]
+
+    def set_shell_state(self, st):
+        st = np.ascontiguousarray(st, np.float64)
+        abi.check(self.lib, self.lib.qnb_set_shell_state(self.h, _pd(st)), "qnb_set_shell_state")

@@ -1,0 +1,142 @@
+"""Batched device-resident MD (lambda windows, uneven systems) over the C ABI.
+
+`BatchMD` advances several instances in the same kernel launches
+(qnb_batch_*): every lambda window of an FEP run, or independent systems of
+different size.  Energies are sampled on the device; `energies()` returns the
+per-instance time series, including the per-state Q-atom energies the Zwanzig
+and EVB reductions need (q6_amd.fep).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .synth import BOLTZ
+
+_pd = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+_pi = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
+
+
+def scaas_cstb(s, lam):
+    """Polarisation-shell constant cstb for the lambda-weighted Q charge
+    (wat_sphere crgQtot, src/simprep.f90:4640-4650; wat_shells
+    src/simprep.f90:4883)."""
+    qcrg = s.md.get("qcrg")
+    if qcrg is None or qcrg.size == 0:
+        return np.asarray(s.par["wshell_cstb"])
+    from .synth import RHO_WAT, SQRT332, TIP3P_HOH, TIP3P_ROH, TIP3P_Q
+    import math
+    crgQtot = float((qcrg * SQRT332 * np.asarray(lam)[None, :]).sum())
+    mu_w = -TIP3P_Q[0] * SQRT332 * TIP3P_ROH * math.cos(math.radians(TIP3P_HOH) / 2)
+    out = []
+    for rout, dr in zip(s.par["wshell_rout"], s.par["wshell_dr"]):
+        ri = rout - dr
+        rshell = (0.5 * (rout ** 3 + ri ** 3)) ** (1.0 / 3.0)
+        out.append(crgQtot * (1.0 - 1.0 / 80.0) / (RHO_WAT * mu_w * 4.0 * math.pi * rshell ** 2))
+    return np.array(out)
+
+
+class BatchMD:
+    def __init__(self, systems, lambdas, velocities, precision=abi.QNB_PREC_FP64, dt_fs=1.0, T0=300.0,
+                 tau_T_fs=100.0, nbcycle=10, ene_every=10, max_frames=1000, positions=None):
+        """systems[k]: QSystem of instance k (repeat the same object for the
+        windows of one system: they share its device topology); lambdas[k]:
+        state weights; velocities[k]: (natom, 3) initial velocities."""
+        self.lib = abi.load()
+        n = len(systems)
+        self.n = n
+        self.systems = systems
+        self._keep = []
+        sys_structs = {}
+        sp, pp, mp, xp, vp = ((C.c_void_p * n)() for _ in range(5))
+        for k, s in enumerate(systems):
+            key = id(s)
+            if key not in sys_structs:
+                st, _ = s.to_structs(precision=precision)
+                sys_structs[key] = st
+                self._keep.append(st)
+            lam = np.asarray(lambdas[k], float)
+            _, par = s.to_structs(precision=precision, **{"lambda": lam, "wshell_cstb": scaas_cstb(s, lam),
+                                                          "itdis": 100.0 / dt_fs})
+            par.precision = precision
+            md = s.md
+            bonds = np.ascontiguousarray(md["bonds"], np.int32).reshape(-1, 2)
+            angles = np.ascontiguousarray(md["angles"], np.int32).reshape(-1, 3)
+            arrs = dict(bond_ij=bonds.ravel(), bond_r0=np.full(len(bonds), md["bond_r0"]),
+                        bond_k=np.full(len(bonds), md["bond_k"]), angle_ijk=angles.ravel(),
+                        angle_t0=np.full(len(angles), md["angle_t0"]), angle_k=np.full(len(angles), md["angle_k"]),
+                        winv=1.0 / np.asarray(md["mass"], float))
+            arrs = {a: np.ascontiguousarray(v) for a, v in arrs.items()}
+            m = abi.QnbMdParams(dt_fs=dt_fs, T0=T0, tau_T_fs=tau_T_fs, nbcycle=nbcycle, ene_every=ene_every,
+                                max_frames=max_frames, nbonds=len(bonds), nangles=len(angles),
+                                water_roh=md["water_roh"], water_rhh=md["water_rhh"])
+            m.bond_ij, m.angle_ijk = _pi(arrs["bond_ij"]), _pi(arrs["angle_ijk"])
+            for a in ("bond_r0", "bond_k", "angle_t0", "angle_k", "winv"):
+                setattr(m, a, _pd(arrs[a]))
+            x0 = np.ascontiguousarray(positions[k] if positions is not None else s.x, np.float64)
+            v0 = np.ascontiguousarray(velocities[k], np.float64)
+            self._keep += [par, m, arrs, x0, v0]
+            sp[k] = C.addressof(sys_structs[key])
+            pp[k] = C.addressof(par)
+            mp[k] = C.addressof(m)
+            xp[k] = x0.ctypes.data
+            vp[k] = v0.ctypes.data
+        h = C.c_void_p()
+        abi.check(self.lib, self.lib.qnb_batch_create(n, sp, pp, mp, xp, vp, C.byref(h)), "qnb_batch_create")
+        self.h = h
+        self.max_frames = max_frames
+        self.dt_fs = dt_fs
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qnb_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, nsteps, graph=True):
+        abi.check(self.lib, self.lib.qnb_batch_run(self.h, nsteps, int(graph)), "qnb_batch_run")
+
+    def sync(self):
+        abi.check(self.lib, self.lib.qnb_batch_sync(self.h), "qnb_batch_sync")
+
+    def state(self, k):
+        nat = self.systems[k].natom
+        x = np.zeros((nat, 3))
+        v = np.zeros((nat, 3))
+        abi.check(self.lib, self.lib.qnb_batch_get_state(self.h, k, _pd(x), _pd(v)), "qnb_batch_get_state")
+        return x, v
+
+    def energies(self, k):
+        """(nframes, QNB_FRAME_LEN) history of instance k."""
+        nf = C.c_int32()
+        abi.check(self.lib, self.lib.qnb_batch_energy_history(self.h, k, None, 0, C.byref(nf)), "history")
+        out = np.zeros((max(nf.value, 1), abi.QNB_FRAME_LEN))
+        abi.check(self.lib, self.lib.qnb_batch_energy_history(self.h, k, _pd(out), nf.value, C.byref(nf)), "history")
+        return out[:nf.value]
+
+    def pair_counts(self, k):
+        out = np.zeros(7, np.int64)
+        abi.check(self.lib, self.lib.qnb_batch_pair_counts(self.h, k, out.ctypes.data_as(C.POINTER(C.c_int64))),
+                  "qnb_batch_pair_counts")
+        keys = ("pp", "pw", "ww", "qp", "qw", "qq", "qqp")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def time_kernel(self, which, reps):
+        ms = C.c_double()
+        abi.check(self.lib, self.lib.qnb_batch_time_kernel(self.h, which, reps, C.byref(ms)), "time_kernel")
+        return ms.value
+
+
+def q_state_energies(frames, nstates):
+    """Per-state Q-atom nonbonded energies EQ(s)%qx = qq + qp + qw (el + vdw),
+    the state energies of the FEP/EVB analysis (src/potene.f90:226-229)."""
+    out = np.zeros((len(frames), nstates))
+    for s in range(nstates):
+        b = abi.F_Q0 + 6 * s
+        out[:, s] = frames[:, b:b + 6].sum(1)
+    return out
+
+
+__all__ = ["BatchMD", "q_state_energies", "scaas_cstb", "BOLTZ"]

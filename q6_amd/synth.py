@@ -1,0 +1,500 @@
+"""Seeded synthetic spherical systems for the benchmark workloads.
+
+Builds, directly in the engine's data layout (q6_amd.system.QSystem), the
+systems the workloads are defined on: a sphere of rigid TIP3P water on a
+jittered cubic lattice at 0.0334 molecules/A^3 with random orientations, around
+a solute of self-avoiding bead chains (one neutral charge group per chain),
+with Q-atoms in 2-3 FEP/EVB states.  Everything Qdyn6's set-up would derive is
+derived the same way here:
+  * charges scaled by sqrt(332) (prep_sim, src/simprep.f90:3714-3721);
+  * geometric LJ A_i = 2 sqrt(eps) sigma^6, B_i = 2 sqrt(eps) sigma^3, pair
+    products as precompute_set_values_* (src/simprep.f90:3321-3589);
+  * exclusions 1-2 and 1-3, 1-4 list from the bond graph (as Qprep6 writes
+    listex/list14), Q connectivity levels (make_qconn, src/nonbondene.f90:3087);
+  * SCAAS constants: Dwmz, awmz (src/simprep.f90:4667-4672), polarisation
+    shells and cstb (src/simprep.f90:4707-4891).
+Exact atom and pair counts are printed by `describe`.
+"""
+import math
+
+import numpy as np
+
+from .system import QSystem
+
+SQRT332 = math.sqrt(332.0)
+RHO_WAT = 0.0334
+BOLTZ = 0.001986  # src/globals.f90:46
+# TIP3P (ff/oplsaa/qoplsaa.prm:41,69, qoplsaa.lib:1484-1497 of the reference)
+TIP3P_Q = (-0.834, 0.417, 0.417)
+TIP3P_ROH = 0.9572
+TIP3P_HOH = 104.52
+OW_A, OW_B = 762.89, 24.39
+M_O, M_H = 15.9994, 1.008
+
+
+def _lj(sigma, eps):
+    return 2.0 * math.sqrt(eps) * sigma ** 6, 2.0 * math.sqrt(eps) * sigma ** 3
+
+
+def _rand_rot(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+class _Grid:
+    """Spatial hash for rejection tests."""
+
+    def __init__(self, cell):
+        self.cell = cell
+        self.d = {}
+
+    def _key(self, p):
+        return tuple(np.floor(p / self.cell).astype(int))
+
+    def add(self, p, idx):
+        self.d.setdefault(self._key(p), []).append((idx, p))
+
+    def near(self, p, r, skip=()):
+        k = self._key(p)
+        r2 = r * r
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for idx, q in self.d.get((k[0] + dx, k[1] + dy, k[2] + dz), ()):
+                        if idx in skip:
+                            continue
+                        t = p - q
+                        if t @ t < r2:
+                            return True
+        return False
+
+
+def bead_chains(rng, nchain, nbead, r_region, bond=1.5, angle=109.5, dmin=2.2, max_region=60.0):
+    """Self-avoiding bead chains grown inside a sphere.  Non-bonded beads
+    (beyond 1-3) stay at least dmin apart.  When a chain cannot be placed the
+    region radius grows by 0.5 A (the returned radius says what was used)."""
+    grid = _Grid(max(dmin, 2.0))
+    pos = np.zeros((nchain * nbead, 3))
+    cos_t = math.cos(math.radians(angle))
+    sin_t = math.sin(math.radians(angle))
+    R = r_region
+    for c in range(nchain):
+        placed = False
+        for attempt in range(400):
+            if attempt and attempt % 100 == 0:
+                R += 0.5
+            chain = []
+            p0 = rng.uniform(-R, R, 3)
+            while p0 @ p0 > R * R:
+                p0 = rng.uniform(-R, R, 3)
+            if grid.near(p0, dmin):
+                continue
+            chain.append(p0)
+            ok = True
+            for b in range(1, nbead):
+                good = False
+                for _ in range(60):
+                    if b == 1:
+                        u = rng.normal(size=3)
+                        u /= np.linalg.norm(u)
+                    else:
+                        prev = chain[-1] - chain[-2]
+                        prev /= np.linalg.norm(prev)
+                        a = rng.normal(size=3)
+                        a -= (a @ prev) * prev
+                        a /= np.linalg.norm(a)
+                        # the new bond makes the bond angle `angle` with the previous one
+                        u = -cos_t * prev + sin_t * a
+                    p = chain[-1] + bond * u
+                    if p @ p > R * R:
+                        continue
+                    # self-avoidance against everything except the last two beads of this chain
+                    if grid.near(p, dmin):
+                        continue
+                    if len(chain) >= 3 and any((p - q) @ (p - q) < dmin * dmin for q in chain[:-2]):
+                        continue
+                    chain.append(p)
+                    good = True
+                    break
+                if not good:
+                    ok = False
+                    break
+            if ok:
+                for b, p in enumerate(chain):
+                    grid.add(p, c * nbead + b)
+                pos[c * nbead:(c + 1) * nbead] = np.array(chain)
+                placed = True
+                break
+        if not placed:
+            raise RuntimeError("could not place chain %d" % c)
+        if R > max_region:
+            raise RuntimeError("solute region exceeded %.1f A" % max_region)
+    return pos, R
+
+
+def tip3p_geometry():
+    th = math.radians(TIP3P_HOH) / 2
+    return np.array([[0.0, 0.0, 0.0],
+                     [TIP3P_ROH * math.sin(th), 0.0, TIP3P_ROH * math.cos(th)],
+                     [-TIP3P_ROH * math.sin(th), 0.0, TIP3P_ROH * math.cos(th)]])
+
+
+def water_sphere(rng_lat, rng_rot, radius, solute_pos, jitter=0.3, dmin_o=2.4, dmin_h=1.8):
+    """Water centres on a jittered cubic lattice at RHO_WAT within radius,
+    random orientations, dropped where they overlap the solute."""
+    a = RHO_WAT ** (-1.0 / 3.0)
+    n = int(math.ceil(radius / a)) + 1
+    g = np.arange(-n, n + 1) * a
+    cen = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    cen = cen + rng_lat.uniform(-jitter, jitter, cen.shape)
+    cen = cen[np.einsum("ij,ij->i", cen, cen) <= radius * radius]
+    rot = _rand_rot(rng_rot, len(cen))
+    geo = tip3p_geometry()
+    at = cen[:, None, :] + np.einsum("nij,kj->nki", rot, geo)
+    if len(solute_pos):
+        grid = _Grid(3.0)
+        for k, p in enumerate(solute_pos):
+            grid.add(p, k)
+        keep = np.array([not (grid.near(w[0], dmin_o) or grid.near(w[1], dmin_h) or grid.near(w[2], dmin_h))
+                         for w in at])
+        at = at[keep]
+    return at
+
+
+def _exclusions(nat, bonds):
+    """1-2/1-3 exclusions and 1-4 pairs from the bond graph."""
+    adj = [[] for _ in range(nat)]
+    for i, j in bonds:
+        adj[i].append(j)
+        adj[j].append(i)
+    ex, p14 = set(), set()
+    for s in range(nat):
+        dist = {s: 0}
+        frontier = [s]
+        for lev in range(1, 4):
+            nxt = []
+            for u in frontier:
+                for w in adj[u]:
+                    if w not in dist:
+                        dist[w] = lev
+                        nxt.append(w)
+            frontier = nxt
+        for t, lev in dist.items():
+            if t <= s:
+                continue
+            if lev in (1, 2):
+                ex.add((s, t))
+            elif lev == 3:
+                p14.add((s, t))
+    return ex, p14, adj
+
+
+def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q_sigma=0.3,
+          cut=10.0, rcq=99.0, rclrf=99.0, use_lrf=True, lambdas=None, q_state0_zero=False,
+          name="synthetic"):
+    """A sphere of `radius` A: `nchain` neutral bead chains of `nbead` atoms (one
+    charge group each) in the central region, the first `n_q_chains` of them
+    (the ones closest to the centre) made of Q-atoms with `nstates` states,
+    and TIP3P water filling the rest."""
+    rng = np.random.default_rng(seed)
+    rng_lat = np.random.default_rng(seed + 1)
+    rng_rot = np.random.default_rng(seed + 2)
+    rng_q = np.random.default_rng(seed + 3)
+    # bead types: 6 LJ types, sigma in [3.0, 3.8], eps in [0.05, 0.2]
+    sig = np.linspace(3.0, 3.8, 6)
+    eps = np.linspace(0.05, 0.2, 6)[::-1]
+    bead_masses = [12.011, 14.007, 15.999, 12.011, 32.06, 14.007]
+    if nchain:
+        spos, r_used = bead_chains(rng, nchain, nbead, chain_region or radius * 0.6)
+    else:
+        spos, r_used = np.zeros((0, 3)), 0.0
+    # Q chains: the chains whose centres are closest to the sphere centre
+    if nchain:
+        cc = spos.reshape(nchain, nbead, 3).mean(1)
+        order = np.argsort(np.einsum("ij,ij->i", cc, cc), kind="stable")
+        spos = spos.reshape(nchain, nbead, 3)[order].reshape(-1, 3)
+    nsol = nchain * nbead
+    wat = water_sphere(rng_lat, rng_rot, radius - 0.5, spos)
+    nwat = len(wat)
+    nat = nsol + 3 * nwat
+    x = np.concatenate([spos, wat.reshape(-1, 3)]) if nwat else spos.copy()
+
+    s = QSystem()
+    s.name = name
+    ntypes = 2 + 6
+    avdw = np.zeros((ntypes, 3))
+    bvdw = np.zeros((ntypes, 3))
+    mass_t = np.zeros(ntypes)
+    avdw[0, :] = OW_A
+    bvdw[0, :] = OW_B
+    mass_t[0], mass_t[1] = M_O, M_H
+    for t in range(6):
+        A, B = _lj(sig[t], eps[t])
+        avdw[2 + t, :] = A
+        bvdw[2 + t, :] = B
+        mass_t[2 + t] = bead_masses[t]
+    iac = np.zeros(nat, np.int32)
+    iac[:nsol] = 2 + rng.integers(0, 6, nsol)
+    iac[nsol::3] = 0
+    iac[nsol + 1::3] = 1
+    iac[nsol + 2::3] = 1
+    crg = np.zeros(nat)
+    for c in range(nchain):
+        q = rng.normal(0.0, q_sigma, nbead)
+        crg[c * nbead:(c + 1) * nbead] = q - q.mean()
+    crg[nsol::3] = TIP3P_Q[0]
+    crg[nsol + 1::3] = TIP3P_Q[1]
+    crg[nsol + 2::3] = TIP3P_Q[2]
+    crg_unscaled = crg.copy()
+    crg = crg * SQRT332
+
+    # bonds, angles along each chain; water geometry constrained
+    bonds, angles = [], []
+    for c in range(nchain):
+        o = c * nbead
+        bonds += [(o + k, o + k + 1) for k in range(nbead - 1)]
+        angles += [(o + k, o + k + 1, o + k + 2) for k in range(nbead - 2)]
+    ex, p14, adj = _exclusions(nsol, bonds)
+    listex = np.zeros((max(nsol, 1), 25), np.uint8)
+    list14 = np.zeros((max(nsol, 1), 25), np.uint8)
+    exlong, l14long = [], []
+    for (i, j) in ex:
+        if j - i <= 25:
+            listex[i, j - i - 1] = 1
+        else:
+            exlong.append((i, j))
+    for (i, j) in p14:
+        if j - i <= 25:
+            list14[i, j - i - 1] = 1
+        else:
+            l14long.append((i, j))
+
+    # charge groups: one per chain (switch atom in the middle), one per water
+    ncgp = nchain + nwat
+    first = np.zeros(ncgp, np.int32)
+    last = np.zeros(ncgp, np.int32)
+    sw = np.zeros(ncgp, np.int32)
+    for c in range(nchain):
+        first[c], last[c], sw[c] = c * nbead, c * nbead + nbead - 1, c * nbead + nbead // 2
+    for w in range(nwat):
+        g = nchain + w
+        first[g], last[g], sw[g] = nsol + 3 * w, nsol + 3 * w + 2, nsol + 3 * w
+    cgpatom = np.arange(nat, dtype=np.int32)
+
+    # Q-atoms
+    nq = n_q_chains * nbead
+    iqseq = np.arange(nq, dtype=np.int32)
+    iqatom = np.zeros(nat, np.int32)
+    iqatom[:nq] = np.arange(1, nq + 1)
+    qcrg = np.zeros((nq, nstates))
+    for st in range(nstates):
+        for c in range(n_q_chains):
+            sl = slice(c * nbead, (c + 1) * nbead)
+            if st == 0:
+                qcrg[sl, 0] = crg_unscaled[sl]
+            elif q_state0_zero:
+                qcrg[sl, st] = 0.0
+            else:
+                q = rng_q.normal(0.0, q_sigma, nbead)
+                qcrg[sl, st] = q - q.mean()
+    if q_state0_zero:  # cfg1: state 1 charged, state 2 neutral
+        pass
+    qcrg_s = qcrg * SQRT332
+    # qconn levels (make_qconn): 1 self, 2 bonded, 3 angle, 4 torsion, 9 far
+    qconn = np.full((nstates, max(nsol, 1), max(nq, 1)), 9, np.int32)
+    for iq in range(nq):
+        i = iqseq[iq]
+        dist = {i: 1}
+        frontier = [i]
+        for lev in (2, 3, 4):
+            nxt = []
+            for u in frontier:
+                for w in adj[u]:
+                    if w not in dist:
+                        dist[w] = lev
+                        nxt.append(w)
+            frontier = nxt
+        for t, lev in dist.items():
+            qconn[:, t, iq] = lev
+    qconn3 = (qconn[:, :nsol, :] <= 3).any(axis=(0, 2)).astype(np.uint8) if nq else np.zeros(nsol, np.uint8)
+
+    sa = 3
+    aLJ_solv = np.array([avdw[0], avdw[1], avdw[1]])
+    bLJ_solv = np.array([bvdw[0], bvdw[1], bvdw[1]])
+    chg_solv = np.array(TIP3P_Q) * SQRT332
+    # ww (precompute_set_values_ww, geometric, code ljcod=1)
+    ww_A = np.zeros((sa, sa))
+    ww_B = np.zeros((sa, sa))
+    ww_el = np.zeros((sa, sa))
+    for la in range(sa):
+        for ka in range(sa):
+            ww_A[la, ka] = aLJ_solv[la, 0] * aLJ_solv[ka, 0]
+            ww_B[la, ka] = bLJ_solv[la, 0] * bLJ_solv[ka, 0]
+            ww_el[la, ka] = chg_solv[la] * chg_solv[ka]
+    # pw (precompute_set_values_pw)
+    pw_A = np.zeros((max(nsol, 1), sa))
+    pw_B = np.zeros((max(nsol, 1), sa))
+    pw_el = np.zeros((max(nsol, 1), sa))
+    for i in range(nsol):
+        if iqatom[i]:
+            continue
+        for j in range(sa):
+            pw_A[i, j] = avdw[iac[i], 0] * aLJ_solv[j, 0]
+            pw_B[i, j] = bvdw[iac[i], 0] * bLJ_solv[j, 0]
+            pw_el[i, j] = crg[i] * chg_solv[j]
+    # qp (precompute_set_values_qp, qvdw_flag off: the Q-atom's own type)
+    qp_A = np.zeros((max(nsol, 1), max(nq, 1), nstates))
+    qp_B = np.zeros_like(qp_A)
+    qp_el = np.zeros_like(qp_A)
+    qp_set = np.zeros(qp_A.shape, np.uint8)
+    # rows of non-Q atoms only (qp_int_comp skips atoms with any qconn <= 3 and
+    # Q-atoms; qq_int_comp adds the qconn >= 4 pairs of Q-bonded neighbours)
+    for j in range(nsol):
+        if iqatom[j]:
+            continue
+        for iq in range(nq):
+            i = iqseq[iq]
+            for st in range(nstates):
+                lev = qconn[st, j, iq]
+                if lev <= 3:
+                    continue
+                vdw = 3 if lev == 4 else 1
+                k = vdw - 1
+                A = avdw[iac[i], k] * avdw[iac[j], k]
+                B = bvdw[iac[i], k] * bvdw[iac[j], k]
+                el = qcrg_s[iq, st] * crg[j]
+                setf = abs(A) > 1e-10 or abs(B) > 1e-10 or abs(el) > 1e-10
+                if vdw == 3:
+                    el *= 0.5
+                qp_A[j, iq, st], qp_B[j, iq, st], qp_el[j, iq, st], qp_set[j, iq, st] = A, B, el, setf
+    # qq (precompute_set_values_qq) for Q pairs with qconn >= 4
+    qq_A = np.zeros((max(nq, 1), max(nq, 1), nstates))
+    qq_B = np.zeros_like(qq_A)
+    qq_el = np.zeros_like(qq_A)
+    qq_set = np.zeros(qq_A.shape, np.uint8)
+    for iq in range(nq - 1):
+        for jq in range(iq + 1, nq):
+            i, j = iqseq[iq], iqseq[jq]
+            for st in range(nstates):
+                lev = qconn[st, j, iq]
+                if lev < 4:
+                    continue
+                vdw = 3 if lev == 4 else 1
+                k = vdw - 1
+                A = avdw[iac[i], k] * avdw[iac[j], k]
+                B = bvdw[iac[i], k] * bvdw[iac[j], k]
+                el = qcrg_s[iq, st] * qcrg_s[jq, st]
+                setf = abs(A) > 1e-10 or abs(B) > 1e-10 or abs(el) > 1e-10
+                if vdw == 3:
+                    el *= 0.5
+                qq_A[iq, jq, st], qq_B[iq, jq, st], qq_el[iq, jq, st], qq_set[iq, jq, st] = A, B, el, setf
+    # qw (precompute_set_values_qw)
+    qw_A = np.zeros((max(nq, 1), sa, nstates))
+    qw_B = np.zeros_like(qw_A)
+    qw_el = np.zeros_like(qw_A)
+    for iq in range(nq):
+        i = iqseq[iq]
+        for j in range(sa):
+            for st in range(nstates):
+                qw_A[iq, j, st] = avdw[iac[i], 0] * aLJ_solv[j, 0]
+                qw_B[iq, j, st] = bvdw[iac[i], 0] * bLJ_solv[j, 0]
+                qw_el[iq, j, st] = qcrg_s[iq, st] * chg_solv[j]
+
+    a = s.a
+    a["crg"], a["iac"], a["iqatom"] = crg, iac, iqatom
+    a["excl"] = np.zeros(nat, np.uint8)
+    heavy = np.ones(nat, np.uint8)
+    heavy[nsol + 1::3] = 0
+    heavy[nsol + 2::3] = 0
+    a["heavy"] = heavy
+    a["cgp_first"], a["cgp_last"], a["cgp_iswitch"], a["cgpatom"] = first, last, sw, cgpatom
+    a["avdw"], a["bvdw"] = avdw.ravel(), bvdw.ravel()
+    a["ljcod"] = np.ones(ntypes * ntypes, np.int32)
+    a["listex"], a["list14"] = listex.ravel(), list14.ravel()
+    a["listexlong"] = np.array(exlong, np.int32).ravel()
+    a["list14long"] = np.array(l14long, np.int32).ravel()
+    s.n["nexlong"], s.n["n14long"] = len(exlong), len(l14long)
+    a["pw_A"], a["pw_B"], a["pw_el"] = pw_A.ravel(order="F"), pw_B.ravel(order="F"), pw_el.ravel(order="F")
+    a["ww_A"], a["ww_B"], a["ww_el"] = ww_A.ravel(order="F"), ww_B.ravel(order="F"), ww_el.ravel(order="F")
+    a["iqseq"], a["qconn3"] = iqseq, qconn3
+    a["qp_A"], a["qp_B"], a["qp_el"] = (v.ravel(order="F") for v in (qp_A, qp_B, qp_el))
+    a["qp_sc"], a["qp_set"] = np.zeros(qp_A.size), qp_set.ravel(order="F")
+    a["qq_A"], a["qq_B"], a["qq_el"] = (v.ravel(order="F") for v in (qq_A, qq_B, qq_el))
+    a["qq_sc"], a["qq_set"], a["qq_soft"] = np.zeros(qq_A.size), qq_set.ravel(order="F"), np.zeros(qq_A.size, np.uint8)
+    a["qw_A"], a["qw_B"], a["qw_el"] = (v.ravel(order="F") for v in (qw_A, qw_B, qw_el))
+    a["qw_sc"] = np.zeros(qw_A.size)
+    s.n.update(natom=nat, nat_solute=nsol, nwat=nwat, solv_atom=sa, ncgp=ncgp, ncgp_solute=nchain, nqat=nq,
+               nstates=nstates, ntypes=ntypes, ivdw_rule=1, solvent_spc=1)
+    s.el14_scale = 0.5
+    s.x = x
+    lam = np.asarray(lambdas if lambdas is not None else [1.0 / nstates] * nstates, float)
+    # SCAAS (wat_sphere / wat_shells)
+    rwat = radius
+    Dwmz = 0.26 * math.exp(-0.19 * (rwat - 15.0)) + 0.74
+    awmz = 0.2 / (1.0 + math.exp(0.4 * (rwat - 25.0))) + 0.3
+    crgQtot = float((qcrg_s * lam[None, :]).sum()) if nq else 0.0
+    mu_w = -chg_solv[0] * TIP3P_ROH * math.cos(math.radians(TIP3P_HOH) / 2)
+    nsh = int(-0.5 + math.sqrt(2.0 * (3.0001 / 0.5) + 0.25))
+    rout = rwat
+    sh_rout, sh_dr, sh_cstb = [], [], []
+    eps_diel = 1.0 - 1.0 / 80.0
+    for k in range(1, nsh + 1):
+        dr = 0.5 * k
+        ri = rout - dr
+        rshell = (0.5 * (rout ** 3 + ri ** 3)) ** (1.0 / 3.0)
+        sh_rout.append(rout)
+        sh_dr.append(dr)
+        sh_cstb.append(crgQtot * eps_diel / (RHO_WAT * mu_w * 4.0 * math.pi * rshell ** 2))
+        rout -= dr
+    s.par = dict(Rcpp=cut, Rcpw=cut, Rcww=cut, Rcq=rcq, RcLRF=rclrf, use_LRF=int(use_lrf), rexcl_o=radius,
+                 rwat=rwat, fk_wsphere=60.0, Dwmz=Dwmz, awmz=awmz, Boltz=BOLTZ, fkwpol=20.0, itdis=100.0,
+                 wpol_restr=1, nwpolr_shell=nsh, xpcent=np.zeros(3), xwcent=np.zeros(3),
+                 wshell_rout=np.array(sh_rout), wshell_dr=np.array(sh_dr), wshell_cstb=np.array(sh_cstb),
+                 wshell_theta_corr=np.zeros(nsh))
+    s.par["lambda"] = lam
+    # MD data: masses, bonded terms, constraints
+    mass = mass_t[iac]
+    s.md = dict(mass=mass, bonds=np.array(bonds, np.int32).reshape(-1, 2), bond_r0=1.5, bond_k=600.0,
+                angles=np.array(angles, np.int32).reshape(-1, 3), angle_t0=math.radians(109.5), angle_k=80.0,
+                water_roh=TIP3P_ROH, water_rhh=2 * TIP3P_ROH * math.sin(math.radians(TIP3P_HOH) / 2),
+                chain_region=r_used, qcrg=qcrg)
+    s.Tfree = 300.0
+    return s.finalize()
+
+
+def maxwell_velocities(s, T, seed):
+    """Seeded Maxwell-Boltzmann velocities in Q units (A per internal time unit,
+    dt_internal = 0.020462 * fs, src/md.f90:207), net momentum removed."""
+    rng = np.random.default_rng(seed)
+    m = s.md["mass"]
+    sd = np.sqrt(BOLTZ * T / m)
+    v = rng.normal(size=(len(m), 3)) * sd[:, None]
+    p = (v * m[:, None]).sum(0) / m.sum()
+    return v - p
+
+
+# workload definitions ----------------------------------------------------
+
+def cfg1(seed=1):
+    """20 A sphere of TIP3P (~1,100 waters) around a 12-atom neutral chain, all
+    12 atoms Q-atoms in 2 states (state 1 charges N(0,0.4), state 2 zero)."""
+    return build(20.0, 1, 12, 1, 2, seed, chain_region=2.0, q_sigma=0.4, lambdas=[0.5, 0.5],
+                 q_state0_zero=True, name="cfg1: 20 A TIP3P sphere + 12 Q-atoms")
+
+
+def cfg2(seed=11, lambdas=(0.5, 0.5)):
+    """30 A sphere: 400 neutral 10-atom bead-chain groups around the centre,
+    30 Q-atoms in 2 EVB states, TIP3P fill."""
+    return build(30.0, 400, 10, 3, 2, seed, chain_region=18.0, q_sigma=0.3, lambdas=list(lambdas),
+                 name="cfg2: 30 A sphere, 400x10-atom solute groups, 30 Q-atoms, TIP3P")
+
+
+def describe(s):
+    n = s.n
+    return (f"{s.name}: natom={n['natom']} solute={n['nat_solute']} waters={n['nwat']} groups={n['ncgp']} "
+            f"Q-atoms={n['nqat']} states={n['nstates']} chain_region={s.md.get('chain_region', 0):.1f} A")

@@ -101,12 +101,13 @@ class QSystem:
         return int(self.n["natom"])
 
     def pair_count_per_step(self, nlist):
-        """Pair interactions per step as BASELINE.md defines them:
-        nbpp + nbpw + nbww + nstates*(nbqp + nbqw + nbqq) + sum_s nbqqp_s,
-        where nlist holds atom-pair counts."""
+        """Pair interactions per step: nbpp + nbpw + nbww + nstates*(nbqp +
+        nbqw) + sum_s (nbqq_s + nbqqp_s), the lengths of the reference's
+        explicit lists (src/globals.f90:414-437); nlist holds atom-pair counts
+        with qq and qqp already summed over states."""
         ns = int(self.n["nstates"])
-        return (nlist["pp"] + nlist["pw"] + nlist["ww"]
-                + ns * (nlist["qp"] + nlist["qw"] + nlist["qq"]) + nlist["qqp"])
+        return (nlist["pp"] + nlist["pw"] + nlist["ww"] + ns * (nlist["qp"] + nlist["qw"])
+                + nlist["qq"] + nlist["qqp"])
 
 
 def from_golden(path):

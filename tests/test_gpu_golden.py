@@ -1,0 +1,141 @@
+"""HIP engine against the reference's golden vectors (GPU).
+
+Tolerances (FP64 plain path): pair lists bit-exact as sets (and in the
+reference's order); energies and gradients within 1e-10 relative, the bar of
+the FP64 form.  The gradient tolerance is relative to the largest component of
+the term's gradient; LRF table entries relative to the largest entry of the
+same kind (phi0, phi1, phi2, phi3).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_path
+from oracle.oracle import Oracle
+from q6_amd import abi
+from q6_amd.system import from_golden
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+TERMS = [("ww", abi.T_WW), ("qw", abi.T_QW), ("pw", abi.T_PW), ("pp", abi.T_PP), ("qp", abi.T_QP),
+         ("lrf", abi.T_LRF), ("qq", abi.T_QQ), ("qqp", abi.T_QQP), ("solvent_radial", abi.T_RAD),
+         ("water_pol", abi.T_POL)]
+
+
+def engine_for(s, **kw):
+    from q6_amd.engine import NonbondedEngine
+    return NonbondedEngine(s, **kw)
+
+
+def term_energy(t, E, EQ):
+    if t in ("ww", "pw", "pp"):
+        return [E[t + "_el"], E[t + "_vdw"]]
+    if t in ("qw", "qp", "qq"):
+        return sum([[q[t + "_el"], q[t + "_vdw"]] for q in EQ], [])
+    if t == "qqp":
+        return sum([[q["qp_el"], q["qp_vdw"]] for q in EQ], [])
+    return [E[t]]
+
+
+@pytest.fixture(scope="module", params=["lig_eq1", "lig_fep2"])
+def gcase(request):
+    print(f"\n[gpu] golden case {request.param}", flush=True)
+    s = from_golden(golden_path(request.param))
+    eng = engine_for(s)
+    eng.make_pair_lists(s.x)
+    yield request.param, s, eng
+    eng.close()
+
+
+@pytest.mark.parametrize("which,key", [(abi.LIST_PP, "nbpp_ij"), (abi.LIST_PW, "nbpw_ij"),
+                                       (abi.LIST_WW, "ww_mol_pairs"), (abi.LIST_QP, "nbqp_ij")])
+def test_lists_match_reference(gcase, which, key):
+    name, s, eng = gcase
+    ref = s.ref[key].T
+    if key.startswith("nb"):
+        ref = ref - 1
+    got = eng.pairs(which)
+    assert got.shape == ref.shape
+    # sorted-set equality is the contract; the export also keeps the reference's order
+    assert np.array_equal(np.unique(got, axis=0), np.unique(ref, axis=0))
+    assert np.array_equal(got, ref)
+
+
+def test_qw_list_matches_reference(gcase):
+    name, s, eng = gcase
+    ref = s.ref["nbqw_ij"].T - 1  # (iq, water atom) in blocks of solv_atom
+    sa = int(s.n["solv_atom"])
+    ref_mol = np.stack([ref[::sa, 0], (ref[::sa, 1] - int(s.n["nat_solute"])) // sa], axis=1)
+    got = eng.pairs(abi.LIST_QW)
+    assert np.array_equal(got, ref_mol)
+
+
+def test_lrf_table(gcase):
+    name, s, eng = gcase
+    got, ref = eng.lrf(), s.ref["lrf"].T
+    for lo, hi in ((0, 3), (3, 4), (4, 7), (7, 16), (16, 43)):
+        scale = np.max(np.abs(ref[:, lo:hi]))
+        assert np.max(np.abs(got[:, lo:hi] - ref[:, lo:hi])) <= REL * scale, (lo, hi)
+
+
+@pytest.mark.parametrize("term,mask", TERMS)
+def test_terms_match_reference(gcase, term, mask):
+    name, s, eng = gcase
+    d, E, EQ = eng.eval_terms(mask, s.x, Tfree=s.Tfree, istep=0)
+    dref = s.ref["d_" + term].T
+    eref = np.atleast_1d(s.ref["E_" + term])
+    e = np.array(term_energy(term, E, EQ))
+    escale = max(np.max(np.abs(eref)), 1e-12)
+    assert np.max(np.abs(e - eref)) <= REL * escale, (e, eref)
+    dscale = max(np.max(np.abs(dref)), 1e-12)
+    assert np.max(np.abs(d - dref)) <= REL * dscale
+
+
+def test_full_nonbonded_call(gcase):
+    """qnb_nonbonded = the whole of pot_energy_nonbonds in one call."""
+    name, s, eng = gcase
+    d, E, EQ = eng.pot_energy_nonbonds(s.x)
+    dref = sum(s.ref["d_" + t].T for t in ("ww", "qw", "pw", "pp", "qp", "lrf"))
+    assert np.max(np.abs(d - dref)) <= REL * np.max(np.abs(dref))
+    for t in ("ww", "pw", "pp"):
+        assert abs(E[t + "_el"] - s.ref["E_" + t][0]) <= REL * max(abs(s.ref["E_" + t][0]), 1.0)
+        assert abs(E[t + "_vdw"] - s.ref["E_" + t][1]) <= REL * max(abs(s.ref["E_" + t][1]), 1.0)
+
+
+def test_nolrf_lists_against_oracle():
+    """LRF off: cut-off-only builders.  The engine follows the switch-atom
+    distance test; the oracle without its reference-quirk flag does the same."""
+    print("\n[gpu] lig_fep2_nolrf", flush=True)
+    s = from_golden(golden_path("lig_fep2_nolrf"))
+    eng = engine_for(s)
+    eng.make_pair_lists(s.x)
+    o = Oracle(s, ref_quirks=0)
+    o.make_pair_lists(s.x)
+    for which in (abi.LIST_PP, abi.LIST_PW, abi.LIST_WW, abi.LIST_QP, abi.LIST_QW):
+        assert np.array_equal(eng.pairs(which), o.pairs(which)), which
+    d, E, EQ = eng.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree)
+    do, Eo, EQo = o.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree)
+    assert np.max(np.abs(d - do)) <= REL * np.max(np.abs(do))
+    eng.close()
+
+
+def test_mixed_precision_error_budget():
+    """FP32 pair arithmetic with FP64 accumulation: the error against the FP64
+    reference is reported and bounded.  Bounds: forces 1e-4 relative to the
+    largest gradient component; energy components 2e-4 relative (each pair
+    energy carries FP32 rounding, ~1e-7 relative, and a class sum adds up to
+    ~1e5 terms of +-10..100 kcal/mol that largely cancel)."""
+    s = from_golden(golden_path("lig_fep2"))
+    eng = engine_for(s, precision=abi.QNB_PREC_MIXED)
+    eng.make_pair_lists(s.x)
+    d, E, EQ = eng.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree)
+    dref = sum(s.ref["d_" + t].T for t, _ in TERMS)
+    err = np.max(np.abs(d - dref)) / np.max(np.abs(dref))
+    print(f"\n[gpu] mixed precision max force error {err:.2e}", flush=True)
+    assert err < 1e-4
+    for t in ("ww", "pw", "pp"):
+        for k, suf in enumerate(("_el", "_vdw")):
+            ref = s.ref["E_" + t][k]
+            print(f"[gpu] mixed {t}{suf}: {E[t + suf]:.6f} vs {ref:.6f}", flush=True)
+            assert abs(E[t + suf] - ref) <= 2e-4 * max(abs(ref), 1.0), (t, suf, E[t + suf], ref)
+    eng.close()
