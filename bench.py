@@ -64,10 +64,30 @@ def force_flops(counts, ns):
             + (counts["qq"] + counts["qqp"]) * (F_Q_SHARED + F_Q_STATE))
 
 
-def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True):
+def equilibrate(s, steps=300, min_steps=300):
+    """Relax the seeded start (untimed set-up): steepest descent, then one FP64
+    instance at 300 K with a 10 fs bath coupling; its final coordinates start
+    every window."""
+    from q6_amd import synth
+    from q6_amd.md import BatchMD
+    m = BatchMD([s], [np.asarray(s.par["lambda"])], [np.zeros((s.natom, 3))], nbcycle=10, ene_every=10, max_frames=2)
+    m.minimize(min_steps, 0.02)
+    m.sync()
+    x, _ = m.state(0)
+    m.close()
+    b = BatchMD([s], [np.asarray(s.par["lambda"])], [synth.maxwell_velocities(s, 300.0, 1)], nbcycle=10,
+                ene_every=10, max_frames=steps // 10 + 2, tau_T_fs=10.0, positions=[x])
+    b.run(steps)
+    b.sync()
+    x, _ = b.state(0)
+    b.close()
+    return x
+
+
+def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True, positions=None):
     from q6_amd.md import BatchMD
     b = BatchMD(systems, lambdas, vels, precision=precision, nbcycle=10, ene_every=10,
-                max_frames=(warmup + steps) // 10 + 2)
+                max_frames=(warmup + steps) // 10 + 2, positions=positions)
     b.run(warmup, graph=graph)
     b.sync()
     t0 = time.perf_counter()
@@ -116,9 +136,11 @@ def main():
     lambdas = [np.array([l, 1.0 - l]) for l in lam1]
     vels = [synth.maxwell_velocities(s, 300.0, 100 + w) for w in range(nw)]
     print(f"# {synth.describe(s)}", file=sys.stderr, flush=True)
+    s.x = equilibrate(s)
+    xs = [s.x] * nw
 
     # batched mixed precision, all windows in one launch per kernel, graphs
-    bm, tm = run_md([s] * nw, lambdas, vels, abi.QNB_PREC_MIXED, args.steps, args.warmup)
+    bm, tm = run_md([s] * nw, lambdas, vels, abi.QNB_PREC_MIXED, args.steps, args.warmup, positions=xs)
     steps_per_s = args.steps / tm
     ns_day = nw * steps_per_s * 1.0 * 0.0864
     counts = bm.pair_counts(0)
@@ -148,7 +170,8 @@ def main():
 
     plain = None
     if not args.no_plain:
-        bp, tp = run_md([s], [lambdas[nw // 2]], [vels[nw // 2]], abi.QNB_PREC_FP64, args.steps, args.warmup)
+        bp, tp = run_md([s], [lambdas[nw // 2]], [vels[nw // 2]], abi.QNB_PREC_FP64, args.steps, args.warmup,
+                        positions=xs[:1])
         plain = args.steps / tp * 1.0 * 0.0864
         bp.close()
         print(f"# plain FP64, one window: {args.steps} steps in {tp:.3f} s -> {plain:.1f} ns/day", file=sys.stderr,

@@ -325,6 +325,12 @@ int32_t qnb_batch_energy_history(qnb_batch *b, int32_t k, double *out, int32_t c
  * (atom pairs, one state), nbqq, nbqqp (summed over states). */
 int32_t qnb_batch_pair_counts(qnb_batch *b, int32_t k, int64_t *out);
 
+/* Relaxes the current structures before MD: nsteps steepest-descent moves
+ * along the full gradient (nonbonded engine + solute bonds/angles), each atom
+ * moving at most max_disp A per step, waters kept rigid by SHAKE.  Set-up aid
+ * for synthetic starts (Qdyn6 relies on a prepared structure instead). */
+int32_t qnb_batch_minimize(qnb_batch *b, int32_t nsteps, double max_disp);
+
 /* Times one kernel of the step on the batch's stream with HIP events: which =
  * 0 the list/LRF update (k_lists_lrf), 1 the receiver-side force kernel
  * (k_forces); the kernel is launched `reps` times over all instances (both

@@ -305,6 +305,7 @@ int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots) {
   HIPCHK(hipMemset(S.nq_lists, 0, 2 * sizeof(int)));
   HIPCHK(hipMemset(S.need, 0, sizeof(int)));
   HIPCHK(hipMemset(S.nnbr, 0, sizeof(int) * S.ncgp));
+  HIPCHK(hipMemset(S.ered, 0, sizeof(double) * QNB_NE));
   return 0;
 }
 

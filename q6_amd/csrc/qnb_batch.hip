@@ -176,6 +176,28 @@ __global__ __launch_bounds__(256) void k_shake(const DevSys* __restrict__ system
   }
 }
 
+// steepest-descent move used to relax a start structure before MD: every atom
+// moves along -d by k0 * |d|, capped at max_disp (A); the previous positions
+// go to xx so k_shake restores the rigid waters afterwards
+__global__ __launch_bounds__(256) void k_sd_move(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds,
+                                                 double k0, double max_disp) {
+  const DevSys& S = systems[blockIdx.y];
+  const MdDev& M = mds[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S.natom) return;
+  const double gx = S.d[3 * i], gy = S.d[3 * i + 1], gz = S.d[3 * i + 2];
+  const double g = sqrt(gx * gx + gy * gy + gz * gz);
+  double sc = k0;
+  if (g * k0 > max_disp) sc = max_disp / g;
+  if (!isfinite(sc)) sc = 0.0;
+  M.xx[3 * i] = S.x[3 * i];
+  M.xx[3 * i + 1] = S.x[3 * i + 1];
+  M.xx[3 * i + 2] = S.x[3 * i + 2];
+  S.x[3 * i] -= gx * sc;
+  S.x[3 * i + 1] -= gy * sc;
+  S.x[3 * i + 2] -= gz * sc;
+}
+
 // temperature (src/md.f90:1379-1447), one temperature group: Temp, Tfree and
 // the Berendsen factor of the next step; advances the step counter
 __global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds,
@@ -300,6 +322,25 @@ static void enqueue_step(qnb_batch* b, bool lists) {
   hipLaunchKernelGGL(k_temperature, dim3(B), dim3(256), 0, b->stream, b->dsys, b->dmd, 1);
 }
 
+template <typename R>
+static void enqueue_min_step(qnb_batch* b, bool lists, double k0, double max_disp) {
+  const int wpb = 4, B = b->n;
+  if (lists) {
+    hipLaunchKernelGGL(k_lists_lrf<R>, dim3((b->maxg + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream, b->dsys);
+    if (b->anyq && !b->static_q) hipLaunchKernelGGL(k_qlists, dim3(B), dim3(1024), 0, b->stream, b->dsys);
+  }
+  switch (b->ns_max) {
+    case 1: enqueue_forces<R, 1>(b); break;
+    case 2: enqueue_forces<R, 2>(b); break;
+    case 3:
+    case 4: enqueue_forces<R, 4>(b); break;
+    default: enqueue_forces<R, 8>(b); break;
+  }
+  hipLaunchKernelGGL(k_bonded, dim3(B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  hipLaunchKernelGGL(k_sd_move, dim3((b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd, k0, max_disp);
+  hipLaunchKernelGGL(k_shake, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+}
+
 static void enqueue(qnb_batch* b, bool lists) {
   if (b->prec == QNB_PREC_MIXED) enqueue_step<float>(b, lists);
   else enqueue_step<double>(b, lists);
@@ -398,6 +439,9 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     M.istep = b->istep + k;
   }
   if ((rc = b->mem.alloc(&b->dsys, n)) || (rc = b->mem.alloc(&b->dmd, n))) return rc;
+  // the MD records never change after this point; every kernel that takes
+  // b->dmd runs after this copy
+  HIPCHK(hipMemcpy(b->dmd, b->hm.data(), sizeof(MdDev) * n, hipMemcpyHostToDevice));
   // first list build, regrown until every row fits, then rows sized to 1.5x
   // the largest neighbour count seen (re-checked at every update; an overflow
   // during a run is reported by qnb_batch_sync)
@@ -631,5 +675,24 @@ extern "C" int32_t qnb_batch_time_kernel(qnb_batch* b, int32_t which, int32_t re
   *ms_per_launch = (double)ms / reps;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_minimize(qnb_batch* b, int32_t nsteps, double max_disp) {
+  if (!b || nsteps < 0 || !(max_disp > 0.0)) return fail(QNB_EINVAL, "bad arguments");
+  const double k0 = 1.0e-3;  // A per kcal/mol/A
+  for (int it = 0; it < nsteps; it++) {
+    const bool lists = (it % b->nbcycle[0]) == 0;
+    if (b->prec == QNB_PREC_MIXED) enqueue_min_step<float>(b, lists, k0, max_disp);
+    else enqueue_min_step<double>(b, lists, k0, max_disp);
+  }
+  // lists for the relaxed positions before MD continues
+  const int wpb = 4;
+  if (b->prec == QNB_PREC_MIXED)
+    hipLaunchKernelGGL(k_lists_lrf<float>, dim3((b->maxg + wpb - 1) / wpb, b->n), dim3(64 * wpb), 0, b->stream, b->dsys);
+  else
+    hipLaunchKernelGGL(k_lists_lrf<double>, dim3((b->maxg + wpb - 1) / wpb, b->n), dim3(64 * wpb), 0, b->stream, b->dsys);
+  if (b->anyq && !b->static_q) hipLaunchKernelGGL(k_qlists, dim3(b->n), dim3(1024), 0, b->stream, b->dsys);
+  HIPCHK(hipGetLastError());
   return 0;
 }

@@ -98,6 +98,10 @@ class BatchMD:
     def run(self, nsteps, graph=True):
         abi.check(self.lib, self.lib.qnb_batch_run(self.h, nsteps, int(graph)), "qnb_batch_run")
 
+    def minimize(self, nsteps, max_disp=0.02):
+        """Steepest-descent relaxation of the start structures (set-up aid)."""
+        abi.check(self.lib, self.lib.qnb_batch_minimize(self.h, nsteps, max_disp), "qnb_batch_minimize")
+
     def sync(self):
         abi.check(self.lib, self.lib.qnb_batch_sync(self.h), "qnb_batch_sync")
 

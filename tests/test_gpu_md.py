@@ -63,13 +63,16 @@ def test_windows_are_independent_and_graph_matches_eager(small):
 
 
 def test_md_stays_stable_and_rigid(small):
-    """200 steps at 300 K: waters stay rigid (SHAKE), temperature stays near the bath."""
+    """The seeded lattice start relaxes (random water orientations release
+    potential energy); with a 10 fs Berendsen coupling the 300 K bath absorbs it
+    within a few hundred steps.  Waters stay rigid (SHAKE, relative tolerance
+    1e-4 on d^2) and every sampled energy stays finite."""
     from q6_amd.md import BatchMD
     s, v = small
     print("\n[gpu] batch stability", flush=True)
     b = BatchMD([s], [np.array([0.5, 0.5])], [v], precision=abi.QNB_PREC_MIXED, nbcycle=10, ene_every=10,
-                max_frames=30)
-    b.run(200)
+                max_frames=60, tau_T_fs=10.0)
+    b.run(500)
     b.sync()
     x, _ = b.state(0)
     e = b.energies(0)
@@ -78,6 +81,6 @@ def test_md_stays_stable_and_rigid(small):
     roh = np.linalg.norm(x[o + 1] - x[o], axis=1)
     assert np.all(np.abs(roh - 0.9572) < 2e-3)
     T = e[:, abi.F_TEMP]
-    print(f"[gpu] T range {T.min():.1f}..{T.max():.1f}", flush=True)
+    print(f"[gpu] T first {T[0]:.1f} max {T.max():.1f} last {T[-1]:.1f}", flush=True)
     assert np.all(np.isfinite(e[:, :abi.F_STEP]))
-    assert 150.0 < T[-1] < 450.0
+    assert 200.0 < T[-1] < 450.0
