@@ -84,10 +84,12 @@ def equilibrate(s, steps=300, min_steps=300):
     return x
 
 
-def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True, positions=None):
+def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True, positions=None, tune=True):
     from q6_amd.md import BatchMD
     b = BatchMD(systems, lambdas, vels, precision=precision, nbcycle=10, ene_every=10,
                 max_frames=(warmup + steps) // 10 + 2, positions=positions)
+    if tune:
+        b.shapes = b.autotune()
     b.run(warmup, graph=graph)
     b.sync()
     t0 = time.perf_counter()
@@ -121,6 +123,8 @@ def main():
     ap.add_argument("--windows", type=int, default=51)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", choices=["windows", "uneven"], default="windows",
+                    help="windows: 51 lambda windows of the 30 A system (headline); uneven: 16 spheres of 15-45 A")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: lambda windows shard across GPUs with no per-step exchange "
@@ -128,6 +132,8 @@ def main():
         sys.exit(2)
     if args.steps % 10 or args.warmup % 10:
         sys.exit("--steps and --warmup must be multiples of the 10-step list cycle")
+    if args.workload == "uneven":
+        return uneven(args)
 
     from q6_amd import abi, synth
     s = synth.cfg2()
@@ -148,6 +154,7 @@ def main():
     print(f"# batched mixed: {args.steps} steps x {nw} windows in {tm:.3f} s -> {ns_day:.1f} ns/day aggregate; "
           f"pairs/step/window {pairs_step}", file=sys.stderr, flush=True)
     # kernel timings on the same stream (HIP events), same batch
+    bm_shapes = bm.shapes
     t_lists = bm.time_kernel(0, 20)
     t_forces = bm.time_kernel(1, 20)
     e = bm.energies(0)
@@ -192,7 +199,8 @@ def main():
         "config": {"workload": f"{nw} lambda windows of a 30 A sphere: {s.n['natom']} atoms ({s.n['nat_solute']} solute "
                                f"in 400 ten-atom groups, {s.n['nwat']} TIP3P), 30 Q-atoms x 2 EVB states, "
                                f"1 fs, lists/LRF every 10 steps",
-                   "windows": nw, "natom": int(s.n["natom"]), "precision": "mixed", "graph": True},
+                   "windows": nw, "natom": int(s.n["natom"]), "precision": "mixed", "graph": True,
+                   "launch_shape_waves_per_block": list(bm_shapes)},
         "pair_interactions_per_s": round(nw * pairs_step * steps_per_s, 1),
         "plain_fp64_ns_day": None if plain is None else round(plain, 2),
         "speedup_vs_plain_fp64": None if plain is None else round(ns_day / plain, 2),
@@ -200,6 +208,38 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out))
+
+
+def uneven(args):
+    """16 independent spheres of 15-45 A in one batch (mixed, autotuned
+    shapes) against the same 16 run one at a time on the plain FP64 path."""
+    from q6_amd import abi, synth
+    systems = synth.cfg5()
+    vels = [synth.maxwell_velocities(s, 300.0, 700 + k) for k, s in enumerate(systems)]
+    lams = [np.asarray(s.par["lambda"]) for s in systems]
+    xs = [equilibrate(s, steps=100, min_steps=200) for s in systems]
+    b, t = run_md(systems, lams, vels, abi.QNB_PREC_MIXED, args.steps, args.warmup, positions=xs)
+    shapes = b.shapes
+    b.close()
+    agg = len(systems) * args.steps / t * 0.0864
+    seq_time = 0.0
+    if not args.no_plain:
+        for k, s in enumerate(systems):
+            bp, tp = run_md([s], [lams[k]], [vels[k]], abi.QNB_PREC_FP64, args.steps, args.warmup, positions=[xs[k]])
+            bp.close()
+            seq_time += tp
+    seq = len(systems) * args.steps / seq_time * 0.0864 if seq_time else None
+    out = {"metric": "ns/day + pair-interactions/s, batched mixed-prec lambda-windows vs plain FP64, 1 GPU",
+           "value": round(agg, 2), "unit": "ns/day", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(t / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32-pair/fp64-accumulate", "data": "synthetic (seeded spheres)",
+           "config": {"workload": "16 independent spheres, radii 15-45 A, " +
+                                  f"{min(s.natom for s in systems)}-{max(s.natom for s in systems)} atoms, "
+                                  "2-3 states, one batch", "launch_shape_waves_per_block": list(shapes)},
+           "sequential_plain_fp64_ns_day": None if seq is None else round(seq, 2),
+           "speedup_vs_sequential_plain_fp64": None if seq is None else round(agg / seq, 2),
+           "roofline": None, "cpu_baseline": None}
     print(json.dumps(out))
 
 

@@ -331,6 +331,12 @@ int32_t qnb_batch_pair_counts(qnb_batch *b, int32_t k, int64_t *out);
  * for synthetic starts (Qdyn6 relies on a prepared structure instead). */
 int32_t qnb_batch_minimize(qnb_batch *b, int32_t nsteps, double max_disp);
 
+/* Autotuner: picks the launch shape (charge groups = wavefronts per block,
+ * 1/2/4/8) of the list/LRF sweep and of the force kernel by timing each
+ * candidate `reps` times on the live batch; wpb_out[2] receives the choice
+ * (lists, forces).  Call before qnb_batch_run; it drops a captured graph. */
+int32_t qnb_batch_autotune(qnb_batch *b, int32_t reps, int32_t *wpb_out);
+
 /* Times one kernel of the step on the batch's stream with HIP events: which =
  * 0 the list/LRF update (k_lists_lrf), 1 the receiver-side force kernel
  * (k_forces); the kernel is launched `reps` times over all instances (both

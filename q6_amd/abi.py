@@ -96,7 +96,7 @@ EXPORTS = [
     "qnb_make_pair_lists", "qnb_nonbonded", "qnb_qq", "qnb_restraints", "qnb_eval_terms",
     "qnb_get_pairs", "qnb_get_lrf", "qnb_get_shell_state", "qnb_set_shell_state",
     "qnb_batch_create", "qnb_batch_destroy", "qnb_batch_run", "qnb_batch_sync", "qnb_batch_get_state",
-    "qnb_batch_energy_history", "qnb_batch_pair_counts", "qnb_batch_time_kernel", "qnb_batch_minimize",
+    "qnb_batch_energy_history", "qnb_batch_pair_counts", "qnb_batch_time_kernel", "qnb_batch_minimize", "qnb_batch_autotune",
 ]
 
 _lib = None
@@ -129,6 +129,7 @@ def _declare(lib):
     lib.qnb_batch_pair_counts.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
     lib.qnb_batch_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     lib.qnb_batch_minimize.argtypes = [vp, C.c_int32, C.c_double]
+    lib.qnb_batch_autotune.argtypes = [vp, C.c_int32, _pi]
     for n in EXPORTS:
         if n not in ("qnb_abi_version", "qnb_last_error", "qnb_device_count"):
             getattr(lib, n).restype = C.c_int32

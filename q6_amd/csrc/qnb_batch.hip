@@ -277,6 +277,7 @@ struct qnb_batch {
   int maxg = 0, maxat = 0, maxsw = 0, ns_max = 1, prec = 0;
   bool anyq = false, anypol = false, static_q = true;
   int qchunks = 1, qwaves = 0;
+  int wpb_lists = 4, wpb_forces = 4;  // waves per block, picked by qnb_batch_autotune
   std::vector<int> nbcycle;
   long long steps_done = 0;
   ~qnb_batch() {
@@ -286,12 +287,37 @@ struct qnb_batch {
   }
 };
 
+// launchers shared by the step, the minimiser, the timer and the autotuner
+static void launch_lists(qnb_batch* b, int wpb) {
+  dim3 g((b->maxg + wpb - 1) / wpb, b->n), t(64 * wpb);
+  if (b->prec == QNB_PREC_MIXED) hipLaunchKernelGGL(k_lists_lrf<float>, g, t, 0, b->stream, b->dsys);
+  else hipLaunchKernelGGL(k_lists_lrf<double>, g, t, 0, b->stream, b->dsys);
+}
+
+static const uint32_t kStepMask = QNB_T_NONBONDED | QNB_T_QQ | QNB_T_QQP | QNB_T_RAD | QNB_T_POL;
+
+template <typename R, int NS>
+static void launch_forces_t(qnb_batch* b, int wpb) {
+  hipLaunchKernelGGL((k_forces<R, NS>), dim3((b->maxg + wpb - 1) / wpb, b->n), dim3(64 * wpb), 0, b->stream, b->dsys,
+                     kStepMask, (const double*)b->tfree);
+}
+
+static void launch_forces(qnb_batch* b, int wpb) {
+  const bool mixed = b->prec == QNB_PREC_MIXED;
+  switch (b->ns_max) {
+    case 1: mixed ? launch_forces_t<float, 1>(b, wpb) : launch_forces_t<double, 1>(b, wpb); break;
+    case 2: mixed ? launch_forces_t<float, 2>(b, wpb) : launch_forces_t<double, 2>(b, wpb); break;
+    case 3:
+    case 4: mixed ? launch_forces_t<float, 4>(b, wpb) : launch_forces_t<double, 4>(b, wpb); break;
+    default: mixed ? launch_forces_t<float, 8>(b, wpb) : launch_forces_t<double, 8>(b, wpb); break;
+  }
+}
+
 template <typename R, int NS>
 static void enqueue_forces(qnb_batch* b) {
   const int wpb = 4, B = b->n;
-  const uint32_t mask = QNB_T_NONBONDED | QNB_T_QQ | QNB_T_QQP | QNB_T_RAD | QNB_T_POL;
-  hipLaunchKernelGGL((k_forces<R, NS>), dim3((b->maxg + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream, b->dsys,
-                     mask, (const double*)b->tfree);
+  const uint32_t mask = kStepMask;
+  launch_forces_t<R, NS>(b, b->wpb_forces);
   if (b->anyq) {
     hipLaunchKernelGGL((k_qforces<R, NS>), dim3((b->qwaves + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream,
                        b->dsys, mask, b->qchunks);
@@ -302,9 +328,9 @@ static void enqueue_forces(qnb_batch* b) {
 
 template <typename R>
 static void enqueue_step(qnb_batch* b, bool lists) {
-  const int wpb = 4, B = b->n;
+  const int B = b->n;
   if (lists) {
-    hipLaunchKernelGGL(k_lists_lrf<R>, dim3((b->maxg + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream, b->dsys);
+    launch_lists(b, b->wpb_lists);
     if (b->anyq && !b->static_q) hipLaunchKernelGGL(k_qlists, dim3(B), dim3(1024), 0, b->stream, b->dsys);
   }
   switch (b->ns_max) {
@@ -324,9 +350,9 @@ static void enqueue_step(qnb_batch* b, bool lists) {
 
 template <typename R>
 static void enqueue_min_step(qnb_batch* b, bool lists, double k0, double max_disp) {
-  const int wpb = 4, B = b->n;
+  const int B = b->n;
   if (lists) {
-    hipLaunchKernelGGL(k_lists_lrf<R>, dim3((b->maxg + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream, b->dsys);
+    launch_lists(b, b->wpb_lists);
     if (b->anyq && !b->static_q) hipLaunchKernelGGL(k_qlists, dim3(B), dim3(1024), 0, b->stream, b->dsys);
   }
   switch (b->ns_max) {
@@ -633,37 +659,9 @@ extern "C" int32_t qnb_batch_time_kernel(qnb_batch* b, int32_t which, int32_t re
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int wpb = 4, B = b->n;
-  const uint32_t mask = QNB_T_NONBONDED | QNB_T_QQ | QNB_T_QQP | QNB_T_RAD | QNB_T_POL;
   auto launch = [&]() {
-    if (which == 0) {
-      if (b->prec == QNB_PREC_MIXED)
-        hipLaunchKernelGGL(k_lists_lrf<float>, dim3((b->maxg + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream, b->dsys);
-      else
-        hipLaunchKernelGGL(k_lists_lrf<double>, dim3((b->maxg + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream, b->dsys);
-    } else {
-      dim3 g((b->maxg + wpb - 1) / wpb, B), t(64 * wpb);
-      const double* tf = b->tfree;
-      const bool mixed = b->prec == QNB_PREC_MIXED;
-      switch (b->ns_max) {
-        case 1:
-          if (mixed) hipLaunchKernelGGL((k_forces<float, 1>), g, t, 0, b->stream, b->dsys, mask, tf);
-          else hipLaunchKernelGGL((k_forces<double, 1>), g, t, 0, b->stream, b->dsys, mask, tf);
-          break;
-        case 2:
-          if (mixed) hipLaunchKernelGGL((k_forces<float, 2>), g, t, 0, b->stream, b->dsys, mask, tf);
-          else hipLaunchKernelGGL((k_forces<double, 2>), g, t, 0, b->stream, b->dsys, mask, tf);
-          break;
-        case 3:
-        case 4:
-          if (mixed) hipLaunchKernelGGL((k_forces<float, 4>), g, t, 0, b->stream, b->dsys, mask, tf);
-          else hipLaunchKernelGGL((k_forces<double, 4>), g, t, 0, b->stream, b->dsys, mask, tf);
-          break;
-        default:
-          if (mixed) hipLaunchKernelGGL((k_forces<float, 8>), g, t, 0, b->stream, b->dsys, mask, tf);
-          else hipLaunchKernelGGL((k_forces<double, 8>), g, t, 0, b->stream, b->dsys, mask, tf);
-      }
-    }
+    if (which == 0) launch_lists(b, b->wpb_lists);
+    else launch_forces(b, b->wpb_forces);
   };
   launch();  // warm-up
   HIPCHK(hipEventRecord(e0, b->stream));
@@ -687,12 +685,59 @@ extern "C" int32_t qnb_batch_minimize(qnb_batch* b, int32_t nsteps, double max_d
     else enqueue_min_step<double>(b, lists, k0, max_disp);
   }
   // lists for the relaxed positions before MD continues
-  const int wpb = 4;
-  if (b->prec == QNB_PREC_MIXED)
-    hipLaunchKernelGGL(k_lists_lrf<float>, dim3((b->maxg + wpb - 1) / wpb, b->n), dim3(64 * wpb), 0, b->stream, b->dsys);
-  else
-    hipLaunchKernelGGL(k_lists_lrf<double>, dim3((b->maxg + wpb - 1) / wpb, b->n), dim3(64 * wpb), 0, b->stream, b->dsys);
+  launch_lists(b, b->wpb_lists);
   if (b->anyq && !b->static_q) hipLaunchKernelGGL(k_qlists, dim3(b->n), dim3(1024), 0, b->stream, b->dsys);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Autotuner: times the list/LRF sweep and the force kernel on the live batch
+// for 1, 2, 4 and 8 receiver groups (wavefronts) per block and keeps the
+// fastest shape of each.  Both kernels recompute their outputs from the
+// current coordinates, so the trials leave the state unchanged.
+extern "C" int32_t qnb_batch_autotune(qnb_batch* b, int32_t reps, int32_t* wpb_out) {
+  if (!b || reps <= 0) return fail(QNB_EINVAL, "bad arguments");
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int cand[4] = {1, 2, 4, 8};
+  for (int which = 0; which < 2; which++) {
+    float best = 1e30f;
+    int pick = 4;
+    for (int c : cand) {
+      if (which == 0) launch_lists(b, c);
+      else launch_forces(b, c);
+      HIPCHK(hipEventRecord(e0, b->stream));
+      for (int r = 0; r < reps; r++) {
+        if (which == 0) launch_lists(b, c);
+        else launch_forces(b, c);
+      }
+      HIPCHK(hipEventRecord(e1, b->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
+    }
+    if (which == 0) b->wpb_lists = pick;
+    else b->wpb_forces = pick;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // the step graph embeds launch shapes: drop it so the next run recaptures
+  if (b->cycle_exec) {
+    (void)hipGraphExecDestroy(b->cycle_exec);
+    b->cycle_exec = nullptr;
+  }
+  if (b->cycle_graph) {
+    (void)hipGraphDestroy(b->cycle_graph);
+    b->cycle_graph = nullptr;
+  }
+  if (wpb_out) {
+    wpb_out[0] = b->wpb_lists;
+    wpb_out[1] = b->wpb_forces;
+  }
   return 0;
 }

@@ -83,7 +83,7 @@ __device__ __forceinline__ int t3index(int a, int b, int c) {
 // cut-off goes to a's neighbour row through a ballot/prefix compaction, a
 // pair beyond it (and inside RcLRF) adds b's non-Q charges to a's expansion.
 template <typename R>
-__global__ __launch_bounds__(256) void k_lists_lrf(const DevSys* __restrict__ systems) {
+__global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ systems) {
   const DevSys& S = systems[blockIdx.y];
   const int a = blockIdx.x * (blockDim.x / QNB_WAVE) + threadIdx.x / QNB_WAVE;
   if (a >= S.ncgp) return;
@@ -292,7 +292,7 @@ enum : uint32_t { T_WW = 1, T_QW = 2, T_PW = 4, T_PP = 8, T_QP = 16, T_LRF = 32,
 // their atoms one at a time.  Writes (not adds) d for every atom of a, and the
 // wave's energy partials into epart[a].
 template <typename R, int NS>
-__global__ __launch_bounds__(256) void k_forces(const DevSys* __restrict__ systems, uint32_t mask,
+__global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ systems, uint32_t mask,
                                                 const double* __restrict__ tfree) {
   const DevSys& S = systems[blockIdx.y];
   const int a = blockIdx.x * (blockDim.x / QNB_WAVE) + threadIdx.x / QNB_WAVE;

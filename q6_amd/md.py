@@ -102,6 +102,13 @@ class BatchMD:
         """Steepest-descent relaxation of the start structures (set-up aid)."""
         abi.check(self.lib, self.lib.qnb_batch_minimize(self.h, nsteps, max_disp), "qnb_batch_minimize")
 
+    def autotune(self, reps=3):
+        """Pick the launch shapes of the two heavy kernels by timing (returns
+        waves per block for the list/LRF sweep and for the force kernel)."""
+        out = np.zeros(2, np.int32)
+        abi.check(self.lib, self.lib.qnb_batch_autotune(self.h, reps, _pi(out)), "qnb_batch_autotune")
+        return int(out[0]), int(out[1])
+
     def sync(self):
         abi.check(self.lib, self.lib.qnb_batch_sync(self.h), "qnb_batch_sync")
 

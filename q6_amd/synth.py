@@ -498,3 +498,21 @@ def describe(s):
     n = s.n
     return (f"{s.name}: natom={n['natom']} solute={n['nat_solute']} waters={n['nwat']} groups={n['ncgp']} "
             f"Q-atoms={n['nqat']} states={n['nstates']} chain_region={s.md.get('chain_region', 0):.1f} A")
+
+
+def cfg5(seed=500, radii=None):
+    """Uneven batch: 16 spheres of radius 15, 17, ..., 45 A built as cfg2
+    (ten-atom bead chains, solute about 0.3 of the atoms, TIP3P fill), each
+    with 10-60 Q-atoms (1-6 chains) in 2-3 states (seeded)."""
+    rng = np.random.default_rng(seed)
+    radii = list(radii) if radii is not None else list(range(15, 47, 2))
+    out = []
+    for k, R in enumerate(radii):
+        vol = 4.0 / 3.0 * math.pi * R ** 3
+        nchain = max(2, int(round(0.3 * 3 * RHO_WAT * vol / 10 / 1.15)))
+        nq = int(rng.integers(1, 7))
+        nst = int(rng.integers(2, 4))
+        lam = np.full(nst, 1.0 / nst)
+        out.append(build(float(R), nchain, 10, min(nq, nchain), nst, seed + 10 * k, chain_region=0.55 * R,
+                         q_sigma=0.3, lambdas=lam, name=f"cfg5[{k}]: {R} A sphere"))
+    return out
