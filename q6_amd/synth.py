@@ -347,64 +347,16 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
             pw_A[i, j] = avdw[iac[i], 0] * aLJ_solv[j, 0]
             pw_B[i, j] = bvdw[iac[i], 0] * bLJ_solv[j, 0]
             pw_el[i, j] = crg[i] * chg_solv[j]
-    # qp (precompute_set_values_qp, qvdw_flag off: the Q-atom's own type)
-    qp_A = np.zeros((max(nsol, 1), max(nq, 1), nstates))
-    qp_B = np.zeros_like(qp_A)
-    qp_el = np.zeros_like(qp_A)
-    qp_set = np.zeros(qp_A.shape, np.uint8)
-    # rows of non-Q atoms only (qp_int_comp skips atoms with any qconn <= 3 and
-    # Q-atoms; qq_int_comp adds the qconn >= 4 pairs of Q-bonded neighbours)
-    for j in range(nsol):
-        if iqatom[j]:
-            continue
-        for iq in range(nq):
-            i = iqseq[iq]
-            for st in range(nstates):
-                lev = qconn[st, j, iq]
-                if lev <= 3:
-                    continue
-                vdw = 3 if lev == 4 else 1
-                k = vdw - 1
-                A = avdw[iac[i], k] * avdw[iac[j], k]
-                B = bvdw[iac[i], k] * bvdw[iac[j], k]
-                el = qcrg_s[iq, st] * crg[j]
-                setf = abs(A) > 1e-10 or abs(B) > 1e-10 or abs(el) > 1e-10
-                if vdw == 3:
-                    el *= 0.5
-                qp_A[j, iq, st], qp_B[j, iq, st], qp_el[j, iq, st], qp_set[j, iq, st] = A, B, el, setf
-    # qq (precompute_set_values_qq) for Q pairs with qconn >= 4
-    qq_A = np.zeros((max(nq, 1), max(nq, 1), nstates))
-    qq_B = np.zeros_like(qq_A)
-    qq_el = np.zeros_like(qq_A)
-    qq_set = np.zeros(qq_A.shape, np.uint8)
-    for iq in range(nq - 1):
-        for jq in range(iq + 1, nq):
-            i, j = iqseq[iq], iqseq[jq]
-            for st in range(nstates):
-                lev = qconn[st, j, iq]
-                if lev < 4:
-                    continue
-                vdw = 3 if lev == 4 else 1
-                k = vdw - 1
-                A = avdw[iac[i], k] * avdw[iac[j], k]
-                B = bvdw[iac[i], k] * bvdw[iac[j], k]
-                el = qcrg_s[iq, st] * qcrg_s[jq, st]
-                setf = abs(A) > 1e-10 or abs(B) > 1e-10 or abs(el) > 1e-10
-                if vdw == 3:
-                    el *= 0.5
-                qq_A[iq, jq, st], qq_B[iq, jq, st], qq_el[iq, jq, st], qq_set[iq, jq, st] = A, B, el, setf
-    # qw (precompute_set_values_qw)
-    qw_A = np.zeros((max(nq, 1), sa, nstates))
-    qw_B = np.zeros_like(qw_A)
-    qw_el = np.zeros_like(qw_A)
+    # Q-atom tables (precompute_set_values_qp/qq/qw) from the Q charges and
+    # the Q-side LJ factors; qvdw_flag off: a Q-atom's factors are its type's
+    qA = np.zeros((max(nq, 1), nstates, 3))
+    qB = np.zeros_like(qA)
     for iq in range(nq):
-        i = iqseq[iq]
-        for j in range(sa):
-            for st in range(nstates):
-                qw_A[iq, j, st] = avdw[iac[i], 0] * aLJ_solv[j, 0]
-                qw_B[iq, j, st] = bvdw[iac[i], 0] * bLJ_solv[j, 0]
-                qw_el[iq, j, st] = qcrg_s[iq, st] * chg_solv[j]
-
+        qA[iq, :, :] = avdw[iac[iqseq[iq]], :][None, :]
+        qB[iq, :, :] = bvdw[iac[iqseq[iq]], :][None, :]
+    qctx = dict(qconn=qconn, iqseq=iqseq, iqatom=iqatom, iac=iac, avdw=avdw, bvdw=bvdw, crg=crg,
+                aLJ_solv=aLJ_solv, bLJ_solv=bLJ_solv, chg_solv=chg_solv, nsol=nsol, nstates=nstates, el14=0.5)
+    qt = q_tables(qctx, qcrg, qA, qB)
     a = s.a
     a["crg"], a["iac"], a["iqatom"] = crg, iac, iqatom
     a["excl"] = np.zeros(nat, np.uint8)
@@ -422,12 +374,7 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
     a["pw_A"], a["pw_B"], a["pw_el"] = pw_A.ravel(order="F"), pw_B.ravel(order="F"), pw_el.ravel(order="F")
     a["ww_A"], a["ww_B"], a["ww_el"] = ww_A.ravel(order="F"), ww_B.ravel(order="F"), ww_el.ravel(order="F")
     a["iqseq"], a["qconn3"] = iqseq, qconn3
-    a["qp_A"], a["qp_B"], a["qp_el"] = (v.ravel(order="F") for v in (qp_A, qp_B, qp_el))
-    a["qp_sc"], a["qp_set"] = np.zeros(qp_A.size), qp_set.ravel(order="F")
-    a["qq_A"], a["qq_B"], a["qq_el"] = (v.ravel(order="F") for v in (qq_A, qq_B, qq_el))
-    a["qq_sc"], a["qq_set"], a["qq_soft"] = np.zeros(qq_A.size), qq_set.ravel(order="F"), np.zeros(qq_A.size, np.uint8)
-    a["qw_A"], a["qw_B"], a["qw_el"] = (v.ravel(order="F") for v in (qw_A, qw_B, qw_el))
-    a["qw_sc"] = np.zeros(qw_A.size)
+    a.update(qt)
     s.n.update(natom=nat, nat_solute=nsol, nwat=nwat, solv_atom=sa, ncgp=ncgp, ncgp_solute=nchain, nqat=nq,
                nstates=nstates, ntypes=ntypes, ivdw_rule=1, solvent_spc=1)
     s.el14_scale = 0.5
@@ -462,8 +409,93 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
     s.md = dict(mass=mass, bonds=np.array(bonds, np.int32).reshape(-1, 2), bond_r0=1.5, bond_k=600.0,
                 angles=np.array(angles, np.int32).reshape(-1, 3), angle_t0=math.radians(109.5), angle_k=80.0,
                 water_roh=TIP3P_ROH, water_rhh=2 * TIP3P_ROH * math.sin(math.radians(TIP3P_HOH) / 2),
-                chain_region=r_used, qcrg=qcrg)
+                chain_region=r_used, qcrg=qcrg, qA=qA, qB=qB, qctx=qctx)
     s.Tfree = 300.0
+    return s.finalize()
+
+
+def q_tables(ctx, qcrg, qA, qB):
+    """Q-atom precompute tables in the engine's layout (precompute_set_values_qp
+    /qq/qw, src/simprep.f90:3420-3589) from the Q charges qcrg[nq, ns]
+    (unscaled), the Q-side LJ factors qA/qB[nq, ns, code] and the Q
+    connectivity levels (make_qconn: 1 self, 2 bonded, 3 angle, 4 1-4, 9 far).
+    Pairs at level 4 use LJ code 3 and scaled electrostatics."""
+    qconn, iqseq, iqatom, iac = ctx["qconn"], ctx["iqseq"], ctx["iqatom"], ctx["iac"]
+    avdw, bvdw, crg = ctx["avdw"], ctx["bvdw"], ctx["crg"]
+    nsol, ns, el14 = ctx["nsol"], ctx["nstates"], ctx["el14"]
+    nq = len(iqseq)
+    qs = np.asarray(qcrg, float) * SQRT332
+    nq1, nsol1 = max(nq, 1), max(nsol, 1)
+    # qp rows of non-Q solute atoms with qconn >= 4 (qp_int_comp, qq_int_comp)
+    qp = {k: np.zeros((nsol1, nq1, ns)) for k in ("A", "B", "el")}
+    qp_set = np.zeros((nsol1, nq1, ns), np.uint8)
+    if nq and nsol:
+        lev = np.transpose(qconn[:, :nsol, :nq], (1, 2, 0))            # (j, iq, s)
+        code = np.where(lev == 4, 2, 0)
+        nonq = (iqatom[:nsol] == 0)[:, None, None]
+        ok = nonq & (lev >= 4)
+        tj = iac[:nsol]
+        Aj = avdw[tj][:, None, None, :]                                 # (j,1,1,code)
+        Bj = bvdw[tj][:, None, None, :]
+        Aq = np.take_along_axis(np.broadcast_to(qA[None, :nq], (nsol, nq, ns, 3)), code[..., None], 3)[..., 0]
+        Bq = np.take_along_axis(np.broadcast_to(qB[None, :nq], (nsol, nq, ns, 3)), code[..., None], 3)[..., 0]
+        Ajc = np.take_along_axis(np.broadcast_to(Aj, (nsol, nq, ns, 3)), code[..., None], 3)[..., 0]
+        Bjc = np.take_along_axis(np.broadcast_to(Bj, (nsol, nq, ns, 3)), code[..., None], 3)[..., 0]
+        A = Aq * Ajc
+        B = Bq * Bjc
+        el = qs[None, :nq, :] * crg[:nsol][:, None, None]
+        setf = (np.abs(A) > 1e-10) | (np.abs(B) > 1e-10) | (np.abs(el) > 1e-10)
+        el = np.where(lev == 4, el * el14, el)
+        qp["A"][:nsol, :nq] = np.where(ok, A, 0.0)
+        qp["B"][:nsol, :nq] = np.where(ok, B, 0.0)
+        qp["el"][:nsol, :nq] = np.where(ok, el, 0.0)
+        qp_set[:nsol, :nq] = (ok & setf).astype(np.uint8)
+    # qq for pairs iq < jq with qconn >= 4
+    qq = {k: np.zeros((nq1, nq1, ns)) for k in ("A", "B", "el")}
+    qq_set = np.zeros((nq1, nq1, ns), np.uint8)
+    for iq in range(nq - 1):
+        for jq in range(iq + 1, nq):
+            for st in range(ns):
+                lv = qconn[st, iqseq[jq], iq]
+                if lv < 4:
+                    continue
+                k = 2 if lv == 4 else 0
+                A = qA[iq, st, k] * qA[jq, st, k]
+                B = qB[iq, st, k] * qB[jq, st, k]
+                el = qs[iq, st] * qs[jq, st]
+                setf = abs(A) > 1e-10 or abs(B) > 1e-10 or abs(el) > 1e-10
+                if lv == 4:
+                    el *= el14
+                qq["A"][iq, jq, st], qq["B"][iq, jq, st], qq["el"][iq, jq, st] = A, B, el
+                qq_set[iq, jq, st] = setf
+    # qw (precompute_set_values_qw): water atoms with code 1
+    sa = len(ctx["chg_solv"])
+    qw = {k: np.zeros((nq1, sa, ns)) for k in ("A", "B", "el")}
+    if nq:
+        qw["A"][:nq] = qA[:nq, None, :, 0] * ctx["aLJ_solv"][None, :, 0, None]
+        qw["B"][:nq] = qB[:nq, None, :, 0] * ctx["bLJ_solv"][None, :, 0, None]
+        qw["el"][:nq] = qs[:nq, None, :] * ctx["chg_solv"][None, :, None]
+    out = {"qp_" + k: v.ravel(order="F") for k, v in qp.items()}
+    out["qp_sc"], out["qp_set"] = np.zeros(qp["A"].size), qp_set.ravel(order="F")
+    out.update({"qq_" + k: v.ravel(order="F") for k, v in qq.items()})
+    out["qq_sc"], out["qq_set"] = np.zeros(qq["A"].size), qq_set.ravel(order="F")
+    out["qq_soft"] = np.zeros(qq["A"].size, np.uint8)
+    out.update({"qw_" + k: v.ravel(order="F") for k, v in qw.items()})
+    out["qw_sc"] = np.zeros(qw["A"].size)
+    return out
+
+
+def set_q_params(s, qcrg=None, qA=None, qB=None):
+    """Rebuild the Q tables of a synthetic system from (perturbed) Q charges
+    and Q-side LJ factors; returns the system (arrays replaced, not mutated)."""
+    md = s.md
+    if qcrg is not None:
+        md["qcrg"] = np.asarray(qcrg, float)
+    if qA is not None:
+        md["qA"] = np.asarray(qA, float)
+    if qB is not None:
+        md["qB"] = np.asarray(qB, float)
+    s.a.update(q_tables(md["qctx"], md["qcrg"], md["qA"], md["qB"]))
     return s.finalize()
 
 
