@@ -344,6 +344,63 @@ int32_t qnb_batch_autotune(qnb_batch *b, int32_t reps, int32_t *wpb_out);
  * change) and *ms_per_launch receives the average launch duration. */
 int32_t qnb_batch_time_kernel(qnb_batch *b, int32_t which, int32_t reps, double *ms_per_launch);
 
+/* ------------------------------------------------------------------------
+ * Adjoint of the per-state Q-atom energies E_s = EQ(s)%qq + %qp + %qw
+ * (src/potene.f90:224-229) over stored frames, for gradients of energy gaps
+ * and Zwanzig/EVB free energies (src/qfep.f90:402-827) with respect to the
+ * Q-side parameters that EVB calibration fits.  No counterpart in the
+ * reference (Qfep6 has no derivatives).
+ *
+ * Every Q table entry (qp_*, qw_*, qq_* of qnb_system) is a product of one
+ * Q-atom's parameter in one state and a partner factor
+ * (precompute_set_values_qp/qq/qw, src/simprep.f90:3420-3589).  The caller
+ * passes, per entry, the derivative of the entry with respect to that Q-side
+ * parameter: for the charge, d el/d q_iq (the partner's scaled charge, times
+ * el14_scale for 1-4 pairs); for LJ, d A/d A_iq and d B/d B_iq (with the
+ * geometric rule the partner's A and B factors of the pair's LJ code).  qq
+ * entries carry both members' tangents.  Entries that do not interact take
+ * zeros.  Layouts are those of the matching qnb_system tables. */
+typedef struct qnb_qtangent {
+  const double *qp_dq, *qp_dA, *qp_dB;   /* [j + nat_solute*(iq + nqat*s)]       */
+  const uint8_t *qp_is14;                /* 1: the pair is 1-4 (separate slots)   */
+  const double *qw_dq, *qw_dA, *qw_dB;   /* [iq + nqat*(k + solv_atom*s)]         */
+  const double *qq_dq_i, *qq_dA_i, *qq_dB_i; /* [iq + nqat*(jq + nqat*s)], wrt iq */
+  const double *qq_dq_j, *qq_dA_j, *qq_dB_j; /* same entries, wrt jq              */
+  const uint8_t *qq_is14;
+} qnb_qtangent;
+
+/* per (frame, state, Q-atom) output record, float64 each */
+#define QNB_ADJ_NV 7
+enum qnb_adj_field {
+  QNB_ADJ_DQ = 0,   /* dE_s / d q_iq,s   (scaled charge)                  */
+  QNB_ADJ_DA = 1,   /* dE_s / d A_iq,s   normal pairs                     */
+  QNB_ADJ_DB = 2,   /* dE_s / d B_iq,s   normal pairs                     */
+  QNB_ADJ_DA14 = 3, /* dE_s / d A_iq,s   1-4 pairs                        */
+  QNB_ADJ_DB14 = 4, /* dE_s / d B_iq,s   1-4 pairs                        */
+  QNB_ADJ_EEL = 5,  /* iq's share of E_s: electrostatic                   */
+  QNB_ADJ_EVDW = 6  /* iq's share of E_s: van der Waals                   */
+};
+
+/* Host-supplied frames: frames[nframes][natom*3]; out[nframes][nstates][nqat]
+ * [QNB_ADJ_NV].  The Q lists are rebuilt per frame from Rcq and xpcent;
+ * pair math runs in the handle's precision (QNB_PREC_MIXED: FP32 pairs, FP64
+ * sums), every sum in FP64.  Summing QNB_ADJ_EEL + QNB_ADJ_EVDW over Q-atoms
+ * gives E_s of the frame. */
+int32_t qnb_adjoint(qnb_handle *h, const qnb_qtangent *tan, int32_t nframes, const double *frames, double *out);
+
+/* Coordinate snapshots during qnb_batch_run: every `every` steps (at the
+ * point where the step's energies are sampled) up to max_frames per
+ * instance, kept in device memory.  every = 0 stops storing. */
+int32_t qnb_batch_store_frames(qnb_batch *b, int32_t every, int32_t max_frames);
+/* Copies instance k's stored frames (nframes x natom*3 float64). */
+int32_t qnb_batch_frames(qnb_batch *b, int32_t k, double *out, int32_t cap, int32_t *nframes);
+/* Adjoint over every instance's stored frames in one launch, in the batch's
+ * precision.  tan[k]: tangents of instance k (pointers may repeat).  out
+ * receives the instances' records one after another (instance k:
+ * nframes_k x nstates_k x nqat_k x QNB_ADJ_NV); *n the number of float64
+ * written (also when it exceeds cap, then nothing is copied). */
+int32_t qnb_batch_adjoint(qnb_batch *b, const qnb_qtangent *const *tan, double *out, int64_t cap, int64_t *n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,19 @@ class QnbMdParams(C.Structure):
     ]
 
 
+_pu8 = C.POINTER(C.c_uint8)
+
+QTANGENT_FIELDS = ("qp_dq", "qp_dA", "qp_dB", "qp_is14", "qw_dq", "qw_dA", "qw_dB",
+                   "qq_dq_i", "qq_dA_i", "qq_dB_i", "qq_dq_j", "qq_dA_j", "qq_dB_j", "qq_is14")
+
+
+class QnbQTangent(C.Structure):
+    _fields_ = [(n, _pu8 if n.endswith("is14") else _pd) for n in QTANGENT_FIELDS]
+
+
+QNB_ADJ_NV = 7
+ADJ_DQ, ADJ_DA, ADJ_DB, ADJ_DA14, ADJ_DB14, ADJ_EEL, ADJ_EVDW = range(7)
+
 QNB_FRAME_LEN = 72
 F_Q0, F_BOND, F_ANGLE, F_EKIN, F_TEMP, F_TFREE, F_STEP = 9, 57, 58, 59, 60, 61, 62
 
@@ -97,6 +110,7 @@ EXPORTS = [
     "qnb_get_pairs", "qnb_get_lrf", "qnb_get_shell_state", "qnb_set_shell_state",
     "qnb_batch_create", "qnb_batch_destroy", "qnb_batch_run", "qnb_batch_sync", "qnb_batch_get_state",
     "qnb_batch_energy_history", "qnb_batch_pair_counts", "qnb_batch_time_kernel", "qnb_batch_minimize", "qnb_batch_autotune",
+    "qnb_adjoint", "qnb_batch_store_frames", "qnb_batch_frames", "qnb_batch_adjoint",
 ]
 
 _lib = None
@@ -130,6 +144,10 @@ def _declare(lib):
     lib.qnb_batch_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     lib.qnb_batch_minimize.argtypes = [vp, C.c_int32, C.c_double]
     lib.qnb_batch_autotune.argtypes = [vp, C.c_int32, _pi]
+    lib.qnb_adjoint.argtypes = [vp, C.POINTER(QnbQTangent), C.c_int32, _pd, _pd]
+    lib.qnb_batch_store_frames.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.qnb_batch_frames.argtypes = [vp, C.c_int32, _pd, C.c_int32, C.POINTER(C.c_int32)]
+    lib.qnb_batch_adjoint.argtypes = [vp, pp, _pd, C.c_int64, C.POINTER(C.c_int64)]
     for n in EXPORTS:
         if n not in ("qnb_abi_version", "qnb_last_error", "qnb_device_count"):
             getattr(lib, n).restype = C.c_int32

@@ -540,6 +540,38 @@ extern "C" int32_t qnb_restraints(qnb_handle* h, const double* x, double* d, dou
   return qnb_eval_terms(h, QNB_T_RAD | QNB_T_POL, x, d, Tfree, istep, E, nullptr);
 }
 
+extern "C" int32_t qnb_adjoint(qnb_handle* h, const qnb_qtangent* tan, int32_t nframes, const double* frames,
+                               double* out) {
+  if (!h) return fail(QNB_EINVAL, "null handle");
+  if (nframes < 0 || (nframes > 0 && (!frames || !out))) return fail(QNB_EINVAL, "bad frames or output");
+  const DevSys& S = h->hs;
+  if (S.nqat <= 0 || nframes == 0) return 0;
+  DevAlloc tmp;
+  AdjDev A{};
+  int rc = adjoint_upload(tmp, S, tan, A);
+  if (rc) return rc;
+  const int chunk = 256;
+  const size_t fl = 3 * (size_t)S.natom, ol = (size_t)S.nstates * S.nqat * QNB_ADJ_NV;
+  double *dfr = nullptr, *dout = nullptr;
+  AdjDev* dadj = nullptr;
+  if ((rc = tmp.alloc(&dfr, fl * std::min(nframes, chunk)))) return rc;
+  if ((rc = tmp.alloc(&dout, ol * std::min(nframes, chunk)))) return rc;
+  if ((rc = tmp.alloc(&dadj, 1))) return rc;
+  for (int f0 = 0; f0 < nframes; f0 += chunk) {
+    const int nf = std::min(chunk, nframes - f0);
+    A.frames = dfr;
+    A.nframes = nf;
+    A.out = dout;
+    HIPCHK(hipMemcpyAsync(dfr, frames + fl * f0, sizeof(double) * fl * nf, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(dadj, &A, sizeof A, hipMemcpyHostToDevice, h->stream));
+    launch_adjoint(h->stream, S.prec, S.nstates, (long)nf * S.nqat, 1, h->dsys, dadj);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out + ol * f0, dout, sizeof(double) * ol * nf, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  return 0;
+}
+
 extern "C" int32_t qnb_get_lrf(qnb_handle* h, double* lrf) {
   if (!h || !lrf) return fail(QNB_EINVAL, "null argument");
   HIPCHK(hipMemcpy(lrf, h->hs.lrf, sizeof(double) * QNB_NLRF * h->hs.ncgp, hipMemcpyDeviceToHost));

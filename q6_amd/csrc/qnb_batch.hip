@@ -48,6 +48,8 @@ struct MdDev {
   int* ctr;       // [4]: frame, error flags
   double* tfree;  // &tfree_all[k]
   int* istep;     // &istep_all[k]
+  double* xs;     // [xs_max][natom*3] coordinate snapshots (counter ctr[2])
+  int xs_every, xs_max;
 };
 
 enum { ERR_NONFINITE = 1, ERR_SHAKE = 2, ERR_OVERFLOW = 4 };
@@ -233,10 +235,24 @@ __global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ 
 }
 
 // energy sampling every ene_every steps (put_ene cadence, src/md.f90:1829-1834)
+// coordinate snapshot of the sampled step into slot ctr[2] (k_record, next in
+// the stream, advances the counter)
+__global__ void k_snap(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
+  const DevSys& S = systems[blockIdx.y];
+  const MdDev& M = mds[blockIdx.y];
+  if (M.xs_every <= 0 || *M.istep % M.xs_every != 0) return;
+  const int f = M.ctr[2];
+  if (f >= M.xs_max) return;
+  const int n = 3 * S.natom;
+  double* dst = M.xs + (size_t)f * n;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) dst[k] = S.x[k];
+}
+
 __global__ void k_record(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
   const DevSys& S = systems[blockIdx.x];
   const MdDev& M = mds[blockIdx.x];
   const int step = *M.istep;
+  if (threadIdx.x == 0 && M.xs_every > 0 && step % M.xs_every == 0 && M.ctr[2] < M.xs_max) M.ctr[2] += 1;
   if (M.ene_every <= 0 || step % M.ene_every != 0) return;
   const int f = M.ctr[0];
   if (f >= M.max_frames) return;
@@ -342,6 +358,7 @@ static void enqueue_step(qnb_batch* b, bool lists) {
   }
   hipLaunchKernelGGL(k_bonded, dim3(B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(64), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
+  hipLaunchKernelGGL(k_snap, dim3((3 * b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_leapfrog, dim3((b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_shake, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
@@ -465,8 +482,8 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     M.istep = b->istep + k;
   }
   if ((rc = b->mem.alloc(&b->dsys, n)) || (rc = b->mem.alloc(&b->dmd, n))) return rc;
-  // the MD records never change after this point; every kernel that takes
-  // b->dmd runs after this copy
+  // the MD records change after this point only in qnb_batch_store_frames
+  // (after a stream sync); every kernel that takes b->dmd runs after this copy
   HIPCHK(hipMemcpy(b->dmd, b->hm.data(), sizeof(MdDev) * n, hipMemcpyHostToDevice));
   // first list build, regrown until every row fits, then rows sized to 1.5x
   // the largest neighbour count seen (re-checked at every update; an overflow
@@ -595,6 +612,92 @@ extern "C" int32_t qnb_batch_energy_history(qnb_batch* b, int32_t k, double* out
   *nframes = ctr[0];
   if (out && ctr[0] <= cap && ctr[0] > 0)
     HIPCHK(hipMemcpy(out, b->hm[k].hist, sizeof(double) * QNB_FRAME_LEN * ctr[0], hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_store_frames(qnb_batch* b, int32_t every, int32_t max_frames) {
+  if (!b || every < 0 || max_frames < 0) return fail(QNB_EINVAL, "bad frame-store arguments");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (int k = 0; k < b->n; k++) {
+    MdDev& M = b->hm[k];
+    if (M.xs) {
+      b->mem.release(M.xs);
+      M.xs = nullptr;
+    }
+    M.xs_every = 0;
+    M.xs_max = 0;
+    if (every > 0 && max_frames > 0) {
+      int rc = b->mem.alloc(&M.xs, (size_t)max_frames * 3 * b->hs[k].natom);
+      if (rc) return rc;
+      M.xs_every = every;
+      M.xs_max = max_frames;
+    }
+    int zero = 0;
+    HIPCHK(hipMemcpy(M.ctr + 2, &zero, sizeof zero, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(b->dmd, b->hm.data(), sizeof(MdDev) * b->n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int frames_stored(qnb_batch* b, int k, int* nf) {
+  int ctr[4];
+  HIPCHK(hipMemcpy(ctr, b->hm[k].ctr, sizeof ctr, hipMemcpyDeviceToHost));
+  *nf = b->hm[k].xs ? ctr[2] : 0;
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_frames(qnb_batch* b, int32_t k, double* out, int32_t cap, int32_t* nframes) {
+  if (!b || k < 0 || k >= b->n || !nframes) return fail(QNB_EINVAL, "bad instance");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  int nf = 0, rc = frames_stored(b, k, &nf);
+  if (rc) return rc;
+  *nframes = nf;
+  if (out && nf > 0 && nf <= cap)
+    HIPCHK(hipMemcpy(out, b->hm[k].xs, sizeof(double) * 3 * b->hs[k].natom * nf, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_adjoint(qnb_batch* b, const qnb_qtangent* const* tan, double* out, int64_t cap,
+                                     int64_t* n) {
+  if (!b || !tan || !n) return fail(QNB_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<int> nf(b->n);
+  std::vector<size_t> off(b->n + 1, 0);
+  long waves = 0;
+  for (int k = 0; k < b->n; k++) {
+    int rc = frames_stored(b, k, &nf[k]);
+    if (rc) return rc;
+    const DevSys& S = b->hs[k];
+    off[k + 1] = off[k] + (size_t)nf[k] * S.nstates * S.nqat * QNB_ADJ_NV;
+    waves = std::max(waves, (long)nf[k] * S.nqat);
+  }
+  *n = (int64_t)off[b->n];
+  if (!out || *n == 0 || *n > cap) return 0;
+  DevAlloc tmp;
+  std::vector<AdjDev> A(b->n);
+  double* dout = nullptr;
+  int rc = tmp.alloc(&dout, off[b->n]);
+  if (rc) return rc;
+  for (int k = 0; k < b->n; k++) {
+    int same = -1;
+    for (int m = 0; m < k && same < 0; m++)
+      if (tan[m] == tan[k]) same = m;
+    if (same >= 0) {
+      if (b->topo_of[same] != b->topo_of[k]) return fail(QNB_EINVAL, "instances %d and %d share tangents but not a system", same, k);
+      A[k] = A[same];
+    } else if (b->hs[k].nqat > 0 && (rc = adjoint_upload(tmp, b->hs[k], tan[k], A[k]))) {
+      return rc;
+    }
+    A[k].frames = b->hm[k].xs;
+    A[k].nframes = nf[k];
+    A[k].out = dout + off[k];
+  }
+  AdjDev* dadj = nullptr;
+  if ((rc = tmp.upload((const AdjDev**)&dadj, A.data(), A.size()))) return rc;
+  launch_adjoint(b->stream, b->prec, b->ns_max, waves, b->n, b->dsys, dadj);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * off[b->n], hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
 

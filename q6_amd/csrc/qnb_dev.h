@@ -220,3 +220,29 @@ struct PairMath {
     return r2 * (-Vel - (R(12) * Va - R(6) * Vb) * r6 * r6_hc) * lam;
   }
 };
+
+// Adjoint of the per-state Q-atom energies over a set of frames: tangent
+// tables (derivative of each Q table entry with respect to the Q-side
+// parameter of that entry) and the per-frame output.
+#define QNB_ADJ_NV 7
+enum { ADJ_DQ = 0, ADJ_DA, ADJ_DB, ADJ_DA14, ADJ_DB14, ADJ_EEL, ADJ_EVDW };
+
+struct AdjDev {
+  const double* frames;  // [nframes][natom*3]
+  int nframes;
+  const double* qp_dq;   // layout of qpA: [j + nsol*(iq + nq*s)]
+  const double* qp_dA;
+  const double* qp_dB;
+  const uint8_t* qp_is14;
+  const double* qw_dq;   // layout of qwA: [iq + nq*(k + sa*s)]
+  const double* qw_dA;
+  const double* qw_dB;
+  const double* qq_dq_i;  // layout of the qq tables: [iq + nq*(jq + nq*s)]
+  const double* qq_dA_i;
+  const double* qq_dB_i;
+  const double* qq_dq_j;
+  const double* qq_dA_j;
+  const double* qq_dB_j;
+  const uint8_t* qq_is14;
+  double* out;  // [nframes][nstates][nqat][QNB_ADJ_NV]
+};

@@ -72,5 +72,8 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M);
 bool host_pp_param(const HostTopo& T, int i, int j);
 // per-instance device state (coordinates, gradient, lists, LRF, energies)
 int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots);
+// adjoint of the Q-atom energies: tangent upload and launch over frames
+int adjoint_upload(DevAlloc& M, const DevSys& S, const qnb_qtangent* t, AdjDev& A);
+void launch_adjoint(hipStream_t st, int prec, int ns_max, long waves, int B, const DevSys* dsys, const AdjDev* dadj);
 
 }  // namespace qnb

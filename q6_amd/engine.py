@@ -104,3 +104,13 @@ class NonbondedEngine:
     def set_shell_state(self, st):
         st = np.ascontiguousarray(st, np.float64)
         abi.check(self.lib, self.lib.qnb_set_shell_state(self.h, _pd(st)), "qnb_set_shell_state")
+
+    def adjoint(self, tangent, frames):
+        """Adjoint of the per-state Q-atom energies over frames (nframes,
+        natom, 3): (nframes, nstates, nqat, QNB_ADJ_NV) records (qnb_adjoint)."""
+        fr = np.ascontiguousarray(frames, np.float64)
+        nf = fr.shape[0]
+        nq = int(self.sys.n["nqat"])
+        out = np.zeros((nf, self.nst, nq, abi.QNB_ADJ_NV))
+        abi.check(self.lib, self.lib.qnb_adjoint(self.h, C.byref(tangent), nf, _pd(fr), _pd(out)), "qnb_adjoint")
+        return out

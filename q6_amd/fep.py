@@ -34,7 +34,8 @@ def evb_ground(V, H12):
     V = np.asarray(V, np.float64)
     v1, v2 = V[:, 0], V[:, 1]
     eg = 0.5 * (v1 + v2) - 0.5 * np.sqrt((v1 - v2) ** 2 + 4.0 * np.asarray(H12) ** 2)
-    c1 = 1.0 / (1.0 + ((eg - v1) / H12) ** 2)
+    with np.errstate(divide="ignore", invalid="ignore"):  # H12 = 0: c1 undefined, as in the reference
+        c1 = 1.0 / (1.0 + ((eg - v1) / H12) ** 2)
     return eg, c1
 
 
@@ -64,7 +65,7 @@ def zwanzig(Vs, lambdas, kT, nskip=0):
     return dgf, dgr, dG
 
 
-def zwanzig_grad(Vs, dVs, lambdas, kT, nskip=0):
+def zwanzig_grad(Vs, dVs, lambdas, kT, nskip=0, per_window=False):
     """Derivative of the Zwanzig dG (last window) with respect to parameters,
     given per-frame derivatives dVs[i] (npts, nstates, nparam) of the state
     energies of window i, with the frames held fixed (the perturbation
@@ -90,4 +91,132 @@ def zwanzig_grad(Vs, dVs, lambdas, kT, nskip=0):
         dv = V @ dl
         w = np.exp(-(dv - dv.min()) / kT)
         ddgr[i - 1] = (w[:, None] * np.einsum("psk,s->pk", dV, dl)).sum(0) / w.sum()
+    if per_window:  # d dG_i for every window (dG_0 = 0)
+        return np.concatenate([np.zeros((1, npar)), np.cumsum(0.5 * (ddgf - ddgr), axis=0)])
     return 0.5 * (ddgf.sum(0) - ddgr.sum(0))
+
+
+def _evb_terms(V, H12):
+    """E_g and its derivatives with respect to V1, V2 and H12 (2 states)."""
+    v1, v2 = V[:, 0], V[:, 1]
+    root = np.sqrt((v1 - v2) ** 2 + 4.0 * H12 ** 2)
+    eg = 0.5 * (v1 + v2) - 0.5 * root
+    d1 = 0.5 - 0.5 * (v1 - v2) / root
+    d2 = 0.5 + 0.5 * (v1 - v2) / root
+    dh = -2.0 * H12 / root
+    return eg, d1, d2, dh
+
+
+def _ground_dv(V, dV, lam, H12, dH):
+    """dvg = E_g - sum_s lam_s V_s and its parameter derivative per frame."""
+    eg, d1, d2, dh = _evb_terms(V, H12)
+    dvg = eg - V @ lam
+    ddvg = d1[:, None] * dV[:, 0, :] + d2[:, None] * dV[:, 1, :] + dh[:, None] * dH[None, :] \
+        - np.einsum("psk,s->pk", dV, lam)
+    return dvg, ddvg
+
+
+def evb_window_free_energy(Vs, lambdas, H12, kT, nskip=0):
+    """Ground-state free energy at every window, Qfep6 Part 2 with the whole
+    window in one bin (src/qfep.f90:716-795): dGg_i = dG_i - kT ln <exp(-(E_g
+    - eps_i)/kT)>_i, eps_i = sum_s lambda_s V_s, taken about the mean of
+    E_g - eps_i as the reference does."""
+    lam = np.asarray(lambdas, np.float64)
+    _, _, dG = zwanzig(Vs, lam, kT, nskip)
+    out = np.zeros(len(Vs))
+    for i, V in enumerate(Vs):
+        V = np.asarray(V, np.float64)[nskip:]
+        eg, _ = evb_ground(V, H12)
+        dvg = eg - V @ lam[i]
+        av = dvg.sum() / len(dvg)
+        out[i] = dG[i] + (-kT * np.log(np.sum(np.exp(-(dvg - av) / kT)) / len(dvg)) + av)
+    return out
+
+
+def evb_window_free_energy_grad(Vs, dVs, dH, lambdas, H12, kT, nskip=0):
+    """Derivative of evb_window_free_energy (nwin, nparam) for fixed frames;
+    dH[nparam] is the derivative of H12 with respect to the parameters."""
+    lam = np.asarray(lambdas, np.float64)
+    dH = np.asarray(dH, np.float64)
+    gdG = zwanzig_grad(Vs, dVs, lam, kT, nskip, per_window=True)
+    out = np.zeros_like(gdG)
+    for i, (V, dV) in enumerate(zip(Vs, dVs)):
+        V = np.asarray(V, np.float64)[nskip:]
+        dV = np.asarray(dV, np.float64)[nskip:]
+        dvg, ddvg = _ground_dv(V, dV, lam[i], H12, dH)
+        w = np.exp(-(dvg - dvg.min()) / kT)
+        out[i] = gdG[i] + (w[:, None] * ddvg).sum(0) / w.sum()
+    return out
+
+
+def evb_pmf(Vs, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):
+    """Reaction free energy along the energy gap, Qfep6 Parts 2 and 3
+    (src/qfep.f90:700-827), 2-state EVB.  The gap sum_s coeff_s V_s of every
+    frame after nskip is binned between the smallest and largest gap of all
+    windows (bin = int((gap - gapmin)/xint), 0-based; the largest gap lands in
+    an extra bin that is never reported, as in the reference).  Per window and
+    bin with at least nmin points: dGg = dG_i - kT ln<exp(-(E_g -
+    eps_i)/kT)>_bin; Part 3 averages dGg over windows weighted by points.
+    frame_bins (list of int arrays) overrides the binning, e.g. to hold it
+    fixed under a parameter perturbation.  Returns a dict: gap centres,
+    dGg[nwin, nbins], npts[nwin, nbins], binned[nbins], ptsum[nbins],
+    frame_bins; entries below nmin are NaN."""
+    lam = np.asarray(lambdas, np.float64)
+    coeff = np.asarray(coeff, np.float64)
+    _, _, dG = zwanzig(Vs, lam, kT, nskip)
+    gaps = [np.asarray(V, np.float64)[nskip:] @ coeff for V in Vs]
+    gapmin = min(g.min() for g in gaps)
+    gapmax = max(g.max() for g in gaps)
+    xint = (gapmax - gapmin) / nbins
+    if frame_bins is None:
+        frame_bins = [np.floor((g - gapmin) / xint).astype(np.int64) for g in gaps]
+    n = len(Vs)
+    dGg = np.full((n, nbins), np.nan)
+    npts = np.zeros((n, nbins), np.int64)
+    num = np.zeros(nbins)
+    ptsum = np.zeros(nbins, np.int64)
+    for i, V in enumerate(Vs):
+        V = np.asarray(V, np.float64)[nskip:]
+        eg, _ = evb_ground(V, H12)
+        dvg = eg - V @ lam[i]
+        b = frame_bins[i]
+        for k in range(nbins):
+            sel = b == k
+            c = int(sel.sum())
+            npts[i, k] = c
+            if c == 0 or c < nmin:
+                continue
+            av = dvg[sel].sum() / c
+            dGg[i, k] = dG[i] + (-kT * np.log(np.sum(np.exp(-(dvg[sel] - av) / kT)) / c) + av)
+            num[k] += dGg[i, k] * c
+            ptsum[k] += c
+    binned = np.where(ptsum >= max(nmin, 1), num / np.maximum(ptsum, 1), np.nan)
+    centres = gapmin + (np.arange(nbins) + 0.5) * xint
+    return dict(centres=centres, dGg=dGg, npts=npts, binned=binned, ptsum=ptsum, frame_bins=frame_bins)
+
+
+def evb_pmf_grad(Vs, dVs, dH, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):
+    """Derivative of evb_pmf's Part 3 curve (nbins, nparam) for fixed frames
+    and fixed bin membership; rows of unreported bins are NaN."""
+    lam = np.asarray(lambdas, np.float64)
+    dH = np.asarray(dH, np.float64)
+    res = evb_pmf(Vs, lam, H12, kT, coeff, nbins, nmin, nskip, frame_bins)
+    gdG = zwanzig_grad(Vs, dVs, lam, kT, nskip, per_window=True)
+    npar = gdG.shape[1]
+    num = np.zeros((nbins, npar))
+    for i, (V, dV) in enumerate(zip(Vs, dVs)):
+        V = np.asarray(V, np.float64)[nskip:]
+        dV = np.asarray(dV, np.float64)[nskip:]
+        dvg, ddvg = _ground_dv(V, dV, lam[i], H12, dH)
+        b = res["frame_bins"][i]
+        for k in range(nbins):
+            c = res["npts"][i, k]
+            if c == 0 or c < nmin:
+                continue
+            sel = b == k
+            w = np.exp(-(dvg[sel] - dvg[sel].min()) / kT)
+            num[k] += c * (gdG[i] + (w[:, None] * ddvg[sel]).sum(0) / w.sum())
+    out = np.full((nbins, npar), np.nan)
+    ok = res["ptsum"] >= max(nmin, 1)
+    out[ok] = num[ok] / res["ptsum"][ok, None]
+    return out

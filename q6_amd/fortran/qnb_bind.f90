@@ -49,6 +49,13 @@ module qnb_bind
     real(c_double) :: qq_el, qq_vdw, qp_el, qp_vdw, qw_el, qw_vdw
   end type qnb_qenergies
 
+  ! tangent tables of the Q tables for the adjoint (c_loc of real(8) /
+  ! integer(1) arrays in the layouts of the matching qnb_system tables)
+  type, bind(c) :: qnb_qtangent
+    type(c_ptr) :: qp_dq, qp_dA, qp_dB, qp_is14, qw_dq, qw_dA, qw_dB
+    type(c_ptr) :: qq_dq_i, qq_dA_i, qq_dB_i, qq_dq_j, qq_dA_j, qq_dB_j, qq_is14
+  end type qnb_qtangent
+
   interface
     integer(c_int32_t) function qnb_abi_version() bind(c, name='qnb_abi_version')
       import :: c_int32_t
@@ -122,6 +129,16 @@ module qnb_bind
       import :: c_int32_t, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(in) :: st(*)
+    end function
+    ! adjoint of the per-state Q-atom energies over frames(natom*3, nframes);
+    ! out(7, nqat, nstates, nframes)
+    integer(c_int32_t) function qnb_adjoint(h, tan, nframes, frames, out) bind(c, name='qnb_adjoint')
+      import :: c_int32_t, c_ptr, c_double, qnb_qtangent
+      type(c_ptr), value :: h
+      type(qnb_qtangent), intent(in) :: tan
+      integer(c_int32_t), value :: nframes
+      real(c_double), intent(in) :: frames(*)
+      real(c_double), intent(out) :: out(*)
     end function
   end interface
 

@@ -134,6 +134,41 @@ class BatchMD:
         keys = ("pp", "pw", "ww", "qp", "qw", "qq", "qqp")
         return dict(zip(keys, (int(v) for v in out)))
 
+    def store_frames(self, every, max_frames):
+        """Keep a coordinate snapshot every `every` steps (max_frames per
+        instance) on the device, for the adjoint."""
+        abi.check(self.lib, self.lib.qnb_batch_store_frames(self.h, every, max_frames), "qnb_batch_store_frames")
+
+    def frames(self, k):
+        nf = C.c_int32()
+        abi.check(self.lib, self.lib.qnb_batch_frames(self.h, k, None, 0, C.byref(nf)), "qnb_batch_frames")
+        out = np.zeros((max(nf.value, 1), self.systems[k].natom, 3))
+        abi.check(self.lib, self.lib.qnb_batch_frames(self.h, k, _pd(out), nf.value, C.byref(nf)), "qnb_batch_frames")
+        return out[:nf.value]
+
+    def adjoint(self, tangents):
+        """Adjoint over every instance's stored frames (one launch); tangents[k]
+        is instance k's qnb_qtangent.  Returns per instance an array
+        (nframes, nstates, nqat, QNB_ADJ_NV)."""
+        tp = (C.c_void_p * self.n)(*[C.addressof(t) for t in tangents])
+        n = C.c_int64()
+        abi.check(self.lib, self.lib.qnb_batch_adjoint(self.h, tp, None, 0, C.byref(n)), "qnb_batch_adjoint")
+        buf = np.zeros(max(n.value, 1))
+        abi.check(self.lib, self.lib.qnb_batch_adjoint(self.h, tp, _pd(buf), n.value, C.byref(n)), "qnb_batch_adjoint")
+        res, off = [], 0
+        for k, s in enumerate(self.systems):
+            ns, nq = s.n["nstates"], s.n["nqat"]
+            per = ns * nq * abi.QNB_ADJ_NV
+            nf = (self._nframes(k))
+            res.append(buf[off:off + nf * per].reshape(nf, ns, nq, abi.QNB_ADJ_NV))
+            off += nf * per
+        return res
+
+    def _nframes(self, k):
+        nf = C.c_int32()
+        abi.check(self.lib, self.lib.qnb_batch_frames(self.h, k, None, 0, C.byref(nf)), "qnb_batch_frames")
+        return nf.value
+
     def time_kernel(self, which, reps):
         ms = C.c_double()
         abi.check(self.lib, self.lib.qnb_batch_time_kernel(self.h, which, reps, C.byref(ms)), "time_kernel")

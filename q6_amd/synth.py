@@ -196,11 +196,13 @@ def _exclusions(nat, bonds):
 
 def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q_sigma=0.3,
           cut=10.0, rcq=99.0, rclrf=99.0, use_lrf=True, lambdas=None, q_state0_zero=False,
-          name="synthetic"):
+          name="synthetic", q_per_chain=None):
     """A sphere of `radius` A: `nchain` neutral bead chains of `nbead` atoms (one
     charge group each) in the central region, the first `n_q_chains` of them
     (the ones closest to the centre) made of Q-atoms with `nstates` states,
-    and TIP3P water filling the rest."""
+    and TIP3P water filling the rest.  With q_per_chain, only the first
+    q_per_chain beads of those chains are Q-atoms (so Q-atoms have bonded
+    non-Q neighbours: Q-bonded-neighbour and 1-4 Q-solute pairs)."""
     rng = np.random.default_rng(seed)
     rng_lat = np.random.default_rng(seed + 1)
     rng_rot = np.random.default_rng(seed + 2)
@@ -287,20 +289,21 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
     cgpatom = np.arange(nat, dtype=np.int32)
 
     # Q-atoms
-    nq = n_q_chains * nbead
-    iqseq = np.arange(nq, dtype=np.int32)
+    qpc = nbead if q_per_chain is None else int(q_per_chain)
+    nq = n_q_chains * qpc
+    iqseq = np.array([c * nbead + k for c in range(n_q_chains) for k in range(qpc)], np.int32)
     iqatom = np.zeros(nat, np.int32)
-    iqatom[:nq] = np.arange(1, nq + 1)
+    iqatom[iqseq] = np.arange(1, nq + 1)
     qcrg = np.zeros((nq, nstates))
     for st in range(nstates):
         for c in range(n_q_chains):
-            sl = slice(c * nbead, (c + 1) * nbead)
+            sl = slice(c * qpc, (c + 1) * qpc)
             if st == 0:
-                qcrg[sl, 0] = crg_unscaled[sl]
+                qcrg[sl, 0] = crg_unscaled[iqseq[sl]]
             elif q_state0_zero:
                 qcrg[sl, st] = 0.0
             else:
-                q = rng_q.normal(0.0, q_sigma, nbead)
+                q = rng_q.normal(0.0, q_sigma, qpc)
                 qcrg[sl, st] = q - q.mean()
     if q_state0_zero:  # cfg1: state 1 charged, state 2 neutral
         pass
@@ -410,6 +413,11 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
                 angles=np.array(angles, np.int32).reshape(-1, 3), angle_t0=math.radians(109.5), angle_k=80.0,
                 water_roh=TIP3P_ROH, water_rhh=2 * TIP3P_ROH * math.sin(math.radians(TIP3P_HOH) / 2),
                 chain_region=r_used, qcrg=qcrg, qA=qA, qB=qB, qctx=qctx)
+    # Q-atom LJ types (the FEP file's Q atom types): one per bead type that
+    # carries a Q-atom, sigma/eps as the topology type's
+    q_types = np.unique(iac[iqseq]) if nq else np.zeros(0, np.int32)
+    s.md.update(q_types=q_types, q_type_of=np.searchsorted(q_types, iac[iqseq]) if nq else np.zeros(0, np.int64),
+                q_sig=sig[q_types - 2] if nq else np.zeros(0), q_eps=eps[q_types - 2] if nq else np.zeros(0))
     s.Tfree = 300.0
     return s.finalize()
 
@@ -483,6 +491,74 @@ def q_tables(ctx, qcrg, qA, qB):
     out.update({"qw_" + k: v.ravel(order="F") for k, v in qw.items()})
     out["qw_sc"] = np.zeros(qw["A"].size)
     return out
+
+
+def q_tangents(ctx, qcrg, qA, qB):
+    """Tangent tables of the Q tables (include/qnb.h qnb_qtangent): the
+    derivative of every entry with respect to the Q-side parameter it
+    contains -- the scaled Q charge (partner charge x el14 for 1-4 pairs) and
+    the Q-atom's geometric LJ factors A, B (partner factor of the pair's code)."""
+    qconn, iqseq, iqatom, iac = ctx["qconn"], ctx["iqseq"], ctx["iqatom"], ctx["iac"]
+    avdw, bvdw, crg = ctx["avdw"], ctx["bvdw"], ctx["crg"]
+    nsol, ns, el14 = ctx["nsol"], ctx["nstates"], ctx["el14"]
+    nq = len(iqseq)
+    qs = np.asarray(qcrg, float) * SQRT332
+    nq1, nsol1 = max(nq, 1), max(nsol, 1)
+    qp = {k: np.zeros((nsol1, nq1, ns)) for k in ("dq", "dA", "dB")}
+    qp14 = np.zeros((nsol1, nq1, ns), np.uint8)
+    if nq and nsol:
+        lev = np.transpose(qconn[:, :nsol, :nq], (1, 2, 0))
+        code = np.where(lev == 4, 2, 0)
+        ok = (iqatom[:nsol] == 0)[:, None, None] & (lev >= 4)
+        tj = iac[:nsol]
+        Ajc = np.take_along_axis(np.broadcast_to(avdw[tj][:, None, None, :], (nsol, nq, ns, 3)), code[..., None], 3)[..., 0]
+        Bjc = np.take_along_axis(np.broadcast_to(bvdw[tj][:, None, None, :], (nsol, nq, ns, 3)), code[..., None], 3)[..., 0]
+        dq = np.broadcast_to(crg[:nsol][:, None, None], (nsol, nq, ns)) * np.where(lev == 4, el14, 1.0)
+        qp["dq"][:nsol, :nq] = np.where(ok, dq, 0.0)
+        qp["dA"][:nsol, :nq] = np.where(ok, Ajc, 0.0)
+        qp["dB"][:nsol, :nq] = np.where(ok, Bjc, 0.0)
+        qp14[:nsol, :nq] = (ok & (lev == 4)).astype(np.uint8)
+    qq = {k: np.zeros((nq1, nq1, ns)) for k in ("dq_i", "dA_i", "dB_i", "dq_j", "dA_j", "dB_j")}
+    qq14 = np.zeros((nq1, nq1, ns), np.uint8)
+    for iq in range(nq - 1):
+        for jq in range(iq + 1, nq):
+            for st in range(ns):
+                lv = qconn[st, iqseq[jq], iq]
+                if lv < 4:
+                    continue
+                k = 2 if lv == 4 else 0
+                f = el14 if lv == 4 else 1.0
+                qq["dq_i"][iq, jq, st], qq["dq_j"][iq, jq, st] = qs[jq, st] * f, qs[iq, st] * f
+                qq["dA_i"][iq, jq, st], qq["dA_j"][iq, jq, st] = qA[jq, st, k], qA[iq, st, k]
+                qq["dB_i"][iq, jq, st], qq["dB_j"][iq, jq, st] = qB[jq, st, k], qB[iq, st, k]
+                qq14[iq, jq, st] = lv == 4
+    sa = len(ctx["chg_solv"])
+    qw = {k: np.zeros((nq1, sa, ns)) for k in ("dq", "dA", "dB")}
+    if nq:
+        qw["dq"][:nq] = ctx["chg_solv"][None, :, None]
+        qw["dA"][:nq] = ctx["aLJ_solv"][None, :, 0, None]
+        qw["dB"][:nq] = ctx["bLJ_solv"][None, :, 0, None]
+    out = {"qp_" + k: v.ravel(order="F") for k, v in qp.items()}
+    out["qp_is14"] = qp14.ravel(order="F")
+    out.update({"qw_" + k: v.ravel(order="F") for k, v in qw.items()})
+    out.update({"qq_" + k: v.ravel(order="F") for k, v in qq.items()})
+    out["qq_is14"] = qq14.ravel(order="F")
+    return out
+
+
+def q_lj_factors(s, sig=None, eps=None):
+    """Q-side geometric LJ factors qA/qB[nq, ns, code] from per-Q-type sigma
+    and epsilon (A = 2 sqrt(eps) sigma^6, B = 2 sqrt(eps) sigma^3, every code)."""
+    md = s.md
+    sig = np.asarray(md["q_sig"] if sig is None else sig, float)
+    eps = np.asarray(md["q_eps"] if eps is None else eps, float)
+    t = md["q_type_of"]
+    ns = s.n["nstates"]
+    A = 2.0 * np.sqrt(eps[t]) * sig[t] ** 6
+    B = 2.0 * np.sqrt(eps[t]) * sig[t] ** 3
+    qA = np.repeat(np.repeat(A[:, None, None], ns, 1), 3, 2)
+    qB = np.repeat(np.repeat(B[:, None, None], ns, 1), 3, 2)
+    return qA, qB
 
 
 def set_q_params(s, qcrg=None, qA=None, qB=None):

@@ -34,7 +34,8 @@ def test_struct_sizes_match_header():
     src = r'''
 #include <stdio.h>
 #include "qnb.h"
-int main(){printf("%zu %zu %zu %zu\n", sizeof(qnb_system), sizeof(qnb_params), sizeof(qnb_energies), sizeof(qnb_qenergies));return 0;}
+int main(){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(qnb_system), sizeof(qnb_params), sizeof(qnb_energies),
+              sizeof(qnb_qenergies), sizeof(qnb_md_params), sizeof(qnb_qtangent));return 0;}
 '''
     import subprocess
     import tempfile
@@ -43,7 +44,8 @@ int main(){printf("%zu %zu %zu %zu\n", sizeof(qnb_system), sizeof(qnb_params), s
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "-o", os.path.join(d, "t"), os.path.join(d, "t.c")])
     out = subprocess.check_output([os.path.join(d, "t")]).decode().split()
     assert [int(v) for v in out] == [C.sizeof(abi.QnbSystem), C.sizeof(abi.QnbParams), C.sizeof(abi.QnbEnergies),
-                                     C.sizeof(abi.QnbQEnergies)]
+                                     C.sizeof(abi.QnbQEnergies), C.sizeof(abi.QnbMdParams),
+                                     C.sizeof(abi.QnbQTangent)]
 
 
 def test_create_fails_loudly_without_gpu():

@@ -63,3 +63,49 @@ def test_zwanzig_gradient_matches_finite_differences():
         e[k] = h
         fd = (dG(p0 + e) - dG(p0 - e)) / (2 * h)
         assert abs(fd - g[k]) < 1e-7 * max(1, abs(fd))
+
+
+def test_evb_ground_state_free_energy_of_uncoupled_states_is_zero():
+    """H12 = 0 and V2 = V1 + c with c > 0: E_g = V1 on every frame, the
+    mapping free energy is dG_i = lambda2_i c exactly, and the ground-state
+    free energy dG_i - kT ln<exp(-(E_g - eps_i)/kT)> vanishes at every window."""
+    rng = np.random.default_rng(5)
+    nwin, c, kT = 7, 3.5, 0.596
+    lam1 = np.linspace(1, 0, nwin)
+    lams = np.stack([lam1, 1 - lam1], 1)
+    Vs = []
+    for _ in range(nwin):
+        v1 = rng.normal(-40.0, 2.0, 50)
+        Vs.append(np.stack([v1, v1 + c], 1))
+    _, _, dG = fep.zwanzig(Vs, lams, kT)
+    assert np.allclose(dG, lams[:, 1] * c, atol=1e-12)
+    G = fep.evb_window_free_energy(Vs, lams, 0.0, kT)
+    assert np.abs(G).max() < 1e-10
+
+
+def test_pmf_single_bin_is_point_weighted_window_average():
+    """With one bin every frame but the largest-gap one (which falls in the
+    reference's unreported extra bin) is binned together, so Part 3's value
+    is the point-weighted mean over windows of the single-bin Part 2 value."""
+    rng = np.random.default_rng(6)
+    nwin, kT, H12 = 4, 0.596, 7.0
+    lam1 = np.linspace(1, 0, nwin)
+    lams = np.stack([lam1, 1 - lam1], 1)
+    Vs = [np.stack([rng.normal(-5 * i, 2, 40), rng.normal(5 * i - 20, 2, 40)], 1) for i in range(nwin)]
+    coeff = np.array([1.0, -1.0])
+    res = fep.evb_pmf(Vs, lams, H12, kT, coeff, nbins=1, nmin=1)
+    gaps = [V @ coeff for V in Vs]
+    top = max(range(nwin), key=lambda i: gaps[i].max())
+    _, _, dG = fep.zwanzig(Vs, lams, kT)
+    num = den = 0.0
+    for i, V in enumerate(Vs):
+        keep = np.ones(len(V), bool)
+        if i == top:
+            keep[np.argmax(gaps[i])] = False
+        eg, _ = fep.evb_ground(V[keep], H12)
+        dvg = eg - V[keep] @ lams[i]
+        g = dG[i] - kT * np.log(np.mean(np.exp(-dvg / kT)))
+        num += g * keep.sum()
+        den += keep.sum()
+    assert res["ptsum"][0] == den
+    assert abs(res["binned"][0] - num / den) < 1e-9
