@@ -159,8 +159,10 @@ def evb_pmf(Vs, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):
     eps_i)/kT)>_bin; Part 3 averages dGg over windows weighted by points.
     frame_bins (list of int arrays) overrides the binning, e.g. to hold it
     fixed under a parameter perturbation.  Returns a dict: gap centres,
-    dGg[nwin, nbins], npts[nwin, nbins], binned[nbins], ptsum[nbins],
-    frame_bins; entries below nmin are NaN."""
+    dGg[nwin, nbins] (ground state), dGv[nwin, nbins, nstates] (diabatic
+    curves), c1[nwin, nbins] (mean state-1 weight), npts[nwin, nbins],
+    binned[nbins] and binned_c1[nbins] (Part 3), ptsum[nbins], frame_bins;
+    entries below nmin are NaN."""
     lam = np.asarray(lambdas, np.float64)
     coeff = np.asarray(coeff, np.float64)
     _, _, dG = zwanzig(Vs, lam, kT, nskip)
@@ -171,14 +173,20 @@ def evb_pmf(Vs, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):
     if frame_bins is None:
         frame_bins = [np.floor((g - gapmin) / xint).astype(np.int64) for g in gaps]
     n = len(Vs)
+    ns = lam.shape[1]
     dGg = np.full((n, nbins), np.nan)
+    dGv = np.full((n, nbins, ns), np.nan)
+    c1 = np.full((n, nbins), np.nan)
     npts = np.zeros((n, nbins), np.int64)
     num = np.zeros(nbins)
+    numc = np.zeros(nbins)
     ptsum = np.zeros(nbins, np.int64)
     for i, V in enumerate(Vs):
         V = np.asarray(V, np.float64)[nskip:]
-        eg, _ = evb_ground(V, H12)
-        dvg = eg - V @ lam[i]
+        eg, w1 = evb_ground(V, H12)
+        veff = V @ lam[i]
+        dvg = eg - veff
+        dvv = V - veff[:, None]
         b = frame_bins[i]
         for k in range(nbins):
             sel = b == k
@@ -188,11 +196,18 @@ def evb_pmf(Vs, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):
                 continue
             av = dvg[sel].sum() / c
             dGg[i, k] = dG[i] + (-kT * np.log(np.sum(np.exp(-(dvg[sel] - av) / kT)) / c) + av)
+            avv = dvv[sel].sum(0) / c
+            dGv[i, k] = dG[i] + (-kT * np.log(np.exp(-(dvv[sel] - avv) / kT).sum(0) / c) + avv)
+            c1[i, k] = w1[sel].sum() / c
             num[k] += dGg[i, k] * c
+            numc[k] += c1[i, k] * c
             ptsum[k] += c
-    binned = np.where(ptsum >= max(nmin, 1), num / np.maximum(ptsum, 1), np.nan)
+    ok = ptsum >= max(nmin, 1)
+    binned = np.where(ok, num / np.maximum(ptsum, 1), np.nan)
+    binned_c1 = np.where(ok, numc / np.maximum(ptsum, 1), np.nan)
     centres = gapmin + (np.arange(nbins) + 0.5) * xint
-    return dict(centres=centres, dGg=dGg, npts=npts, binned=binned, ptsum=ptsum, frame_bins=frame_bins)
+    return dict(centres=centres, dGg=dGg, dGv=dGv, c1=c1, npts=npts, binned=binned, binned_c1=binned_c1,
+                ptsum=ptsum, frame_bins=frame_bins)
 
 
 def evb_pmf_grad(Vs, dVs, dH, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):
