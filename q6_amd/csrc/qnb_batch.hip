@@ -99,9 +99,14 @@ __global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ syste
     double ct = (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / (ru * rw);
     ct = fmin(1.0, fmax(-1.0, ct));
     const double th = acos(ct);
-    const double st = fmax(sin(th), 1.0e-12);
+    // sin(theta) from the cross product, consistent with the numerator of the
+    // derivative below even for nearly linear angles, where sin(acos(ct))
+    // loses all digits; at a linear angle the derivative has no direction
+    const double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
+    const double st = sqrt(cx * cx + cy * cy + cz * cz) / (ru * rw);
     const double da = th - M.angle_t0[a];
     ea += 0.5 * M.angle_k[a] * da * da;
+    if (!(st > 1.0e-8)) continue;
     const double dv = M.angle_k[a] * da;
     // dtheta/dx_i = -(w/(ru rw) - ct u/ru^2)/sin(theta), likewise for k; j balances
     for (int c = 0; c < 3; c++) {
@@ -189,12 +194,15 @@ __global__ __launch_bounds__(256) void k_sd_move(const DevSys* __restrict__ syst
   if (i >= S.natom) return;
   const double gx = S.d[3 * i], gy = S.d[3 * i + 1], gz = S.d[3 * i + 2];
   const double g = sqrt(gx * gx + gy * gy + gz * gz);
-  double sc = k0;
-  if (g * k0 > max_disp) sc = max_disp / g;
-  if (!isfinite(sc)) sc = 0.0;
   M.xx[3 * i] = S.x[3 * i];
   M.xx[3 * i + 1] = S.x[3 * i + 1];
   M.xx[3 * i + 2] = S.x[3 * i + 2];
+  if (!isfinite(g)) {  // leave the atom in place and report
+    atomicOr(&M.ctr[1], ERR_NONFINITE);
+    return;
+  }
+  double sc = k0;
+  if (g * k0 > max_disp) sc = max_disp / g;
   S.x[3 * i] -= gx * sc;
   S.x[3 * i + 1] -= gy * sc;
   S.x[3 * i + 2] -= gz * sc;
@@ -590,8 +598,8 @@ extern "C" int32_t qnb_batch_sync(qnb_batch* b) {
     int ctr[4];
     HIPCHK(hipMemcpy(ctr, b->hm[k].ctr, sizeof ctr, hipMemcpyDeviceToHost));
     if (ctr[1] & ERR_OVERFLOW) return fail(QNB_EOVERFLOW, "instance %d: neighbour list overflow", k);
+    if (ctr[1] & ERR_NONFINITE) return fail(QNB_ENONFINITE, "instance %d: non-finite energy or gradient", k);
     if (ctr[1] & ERR_SHAKE) return fail(QNB_ENONFINITE, "instance %d: shake failure", k);
-    if (ctr[1] & ERR_NONFINITE) return fail(QNB_ENONFINITE, "instance %d: non-finite energy", k);
   }
   return 0;
 }

@@ -74,10 +74,12 @@ class _Grid:
         return False
 
 
-def bead_chains(rng, nchain, nbead, r_region, bond=1.5, angle=109.5, dmin=2.2, max_region=60.0):
+def bead_chains(rng, nchain, nbead, r_region, bond=1.5, angle=109.5, dmin=2.2, max_region=60.0, compact=None):
     """Self-avoiding bead chains grown inside a sphere.  Non-bonded beads
     (beyond 1-3) stay at least dmin apart.  When a chain cannot be placed the
-    region radius grows by 0.5 A (the returned radius says what was used)."""
+    region radius grows by 0.5 A (the returned radius says what was used).
+    With `compact`, every bead of a chain stays within that distance of the
+    chain's first bead (a folded chain: a compact charge group)."""
     grid = _Grid(max(dmin, 2.0))
     pos = np.zeros((nchain * nbead, 3))
     cos_t = math.cos(math.radians(angle))
@@ -112,6 +114,8 @@ def bead_chains(rng, nchain, nbead, r_region, bond=1.5, angle=109.5, dmin=2.2, m
                         u = -cos_t * prev + sin_t * a
                     p = chain[-1] + bond * u
                     if p @ p > R * R:
+                        continue
+                    if compact is not None and (p - chain[0]) @ (p - chain[0]) > compact * compact:
                         continue
                     # self-avoidance against everything except the last two beads of this chain
                     if grid.near(p, dmin):
@@ -196,13 +200,17 @@ def _exclusions(nat, bonds):
 
 def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q_sigma=0.3,
           cut=10.0, rcq=99.0, rclrf=99.0, use_lrf=True, lambdas=None, q_state0_zero=False,
-          name="synthetic", q_per_chain=None):
+          name="synthetic", q_per_chain=None, compact=None):
     """A sphere of `radius` A: `nchain` neutral bead chains of `nbead` atoms (one
     charge group each) in the central region, the first `n_q_chains` of them
     (the ones closest to the centre) made of Q-atoms with `nstates` states,
     and TIP3P water filling the rest.  With q_per_chain, only the first
     q_per_chain beads of those chains are Q-atoms (so Q-atoms have bonded
-    non-Q neighbours: Q-bonded-neighbour and 1-4 Q-solute pairs)."""
+    non-Q neighbours: Q-bonded-neighbour and 1-4 Q-solute pairs).  With
+    `compact`, each chain is folded within that radius of its first bead and
+    its switch atom is the bead with the smallest largest distance to the
+    others, so every atom of a group lies within `compact` of the switch atom
+    and no atom pair in contact is missed by the switch-atom cut-off."""
     rng = np.random.default_rng(seed)
     rng_lat = np.random.default_rng(seed + 1)
     rng_rot = np.random.default_rng(seed + 2)
@@ -212,7 +220,7 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
     eps = np.linspace(0.05, 0.2, 6)[::-1]
     bead_masses = [12.011, 14.007, 15.999, 12.011, 32.06, 14.007]
     if nchain:
-        spos, r_used = bead_chains(rng, nchain, nbead, chain_region or radius * 0.6)
+        spos, r_used = bead_chains(rng, nchain, nbead, chain_region or radius * 0.6, compact=compact)
     else:
         spos, r_used = np.zeros((0, 3)), 0.0
     # Q chains: the chains whose centres are closest to the sphere centre
@@ -283,6 +291,10 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
     sw = np.zeros(ncgp, np.int32)
     for c in range(nchain):
         first[c], last[c], sw[c] = c * nbead, c * nbead + nbead - 1, c * nbead + nbead // 2
+        if compact is not None:
+            ch = x[c * nbead:(c + 1) * nbead]
+            far = np.sqrt(((ch[:, None, :] - ch[None, :, :]) ** 2).sum(-1)).max(1)
+            sw[c] = c * nbead + int(np.argmin(far))
     for w in range(nwat):
         g = nchain + w
         first[g], last[g], sw[g] = nsol + 3 * w, nsol + 3 * w + 2, nsol + 3 * w
@@ -598,7 +610,7 @@ def cfg1(seed=1):
 def cfg2(seed=11, lambdas=(0.5, 0.5)):
     """30 A sphere: 400 neutral 10-atom bead-chain groups around the centre,
     30 Q-atoms in 2 EVB states, TIP3P fill."""
-    return build(30.0, 400, 10, 3, 2, seed, chain_region=18.0, q_sigma=0.3, lambdas=list(lambdas),
+    return build(30.0, 400, 10, 3, 2, seed, chain_region=18.0, q_sigma=0.3, lambdas=list(lambdas), compact=3.0,
                  name="cfg2: 30 A sphere, 400x10-atom solute groups, 30 Q-atoms, TIP3P")
 
 
@@ -622,5 +634,5 @@ def cfg5(seed=500, radii=None):
         nst = int(rng.integers(2, 4))
         lam = np.full(nst, 1.0 / nst)
         out.append(build(float(R), nchain, 10, min(nq, nchain), nst, seed + 10 * k, chain_region=0.55 * R,
-                         q_sigma=0.3, lambdas=lam, name=f"cfg5[{k}]: {R} A sphere"))
+                         q_sigma=0.3, lambdas=lam, compact=3.0, name=f"cfg5[{k}]: {R} A sphere"))
     return out
