@@ -84,10 +84,10 @@ def equilibrate(s, steps=300, min_steps=300):
     return x
 
 
-def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True, positions=None, tune=True):
+def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True, positions=None, tune=True, extra=0):
     from q6_amd.md import BatchMD
     b = BatchMD(systems, lambdas, vels, precision=precision, nbcycle=10, ene_every=10,
-                max_frames=(warmup + steps) // 10 + 2, positions=positions)
+                max_frames=(warmup + steps + extra) // 10 + 2, positions=positions)
     if tune:
         b.shapes = b.autotune()
     b.run(warmup, graph=graph)
@@ -97,6 +97,56 @@ def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True, positio
     b.sync()
     dt = time.perf_counter() - t0
     return b, dt
+
+
+def free_energy_precision(bm, s, lambdas, nfr):
+    """Runs nfr*10 more steps storing a frame and an energy record every 10;
+    the per-window state energies the mixed run recorded and the plain FP64
+    engine's state energies of the same frames give two Zwanzig dG curves
+    over the same frames: their difference is the precision error of dG.
+    Then times the adjoint over the stored frames (one launch, all windows)."""
+    from q6_amd import abi, adjoint, fep
+    from q6_amd.engine import NonbondedEngine
+    from q6_amd.md import q_state_energies
+    nw = len(lambdas)
+    bm.store_frames(10, nfr)
+    bm.run(10 * nfr)
+    bm.sync()
+    frames = [bm.frames(k) for k in range(nw)]
+    V_mixed = [q_state_energies(bm.energies(k)[-nfr:], 2) for k in range(nw)]
+    eng = NonbondedEngine(s)
+    eng.make_pair_lists(frames[0][0])
+    qmask = abi.T_QP | abi.T_QW | abi.T_QQ | abi.T_QQP
+    V64 = []
+    for k in range(nw):
+        E = np.zeros((nfr, 2))
+        for f in range(nfr):
+            _, _, EQ = eng.eval_terms(qmask, frames[k][f])
+            E[f] = [sum(q.values()) for q in EQ]
+        V64.append(E)
+    eng.close()
+    kT = 0.001986 * 300.0
+    lam = np.asarray(lambdas)
+    _, _, dG_m = fep.zwanzig(V_mixed, lam, kT)
+    _, _, dG_64 = fep.zwanzig(V64, lam, kT)
+    tg, keep = adjoint.tangent_struct(s)
+    t0 = time.perf_counter()
+    outs = bm.adjoint([tg] * nw)
+    t_adj = time.perf_counter() - t0
+    space = adjoint.QParamSpace(s, H12=80.0)
+    VdV = [adjoint.energy_derivatives(o, space) for o in outs]
+    t1 = time.perf_counter()
+    g = adjoint.gradients([v for v, _ in VdV], [d for _, d in VdV], space, lam, kT, [1.0, -1.0])
+    t_red = time.perf_counter() - t1
+    E_adj = [v for v, _ in VdV]
+    return {"frames_per_window": nfr, "windows": nw,
+            "dG_end_mixed": round(float(dG_m[-1]), 4), "dG_end_fp64": round(float(dG_64[-1]), 4),
+            "max_abs_dG_diff_kcal_mol": float(np.abs(dG_m - dG_64).max()),
+            "max_abs_state_energy_diff_kcal_mol": float(max(np.abs(a - b).max() for a, b in zip(V_mixed, V64))),
+            "adjoint": {"launch_s": round(t_adj, 4), "frames": nw * nfr, "frames_per_s": round(nw * nfr / t_adj, 1),
+                        "params": space.npar, "host_reduction_s": round(t_red, 4),
+                        "max_abs_E_adjoint_vs_fp64": float(max(np.abs(a - b).max() for a, b in zip(E_adj, V64))),
+                        "grad_dG_norm": float(np.linalg.norm(g["dG"]))}}
 
 
 def cpu_baseline(s, ncycles=1):
@@ -123,6 +173,8 @@ def main():
     ap.add_argument("--windows", type=int, default=51)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--frames", type=int, default=20,
+                    help="frames per window stored after the timed run for the dG precision check and the adjoint")
     ap.add_argument("--workload", choices=["windows", "uneven"], default="windows",
                     help="windows: 51 lambda windows of the 30 A system (headline); uneven: 16 spheres of 15-45 A")
     args = ap.parse_args()
@@ -146,7 +198,8 @@ def main():
     xs = [s.x] * nw
 
     # batched mixed precision, all windows in one launch per kernel, graphs
-    bm, tm = run_md([s] * nw, lambdas, vels, abi.QNB_PREC_MIXED, args.steps, args.warmup, positions=xs)
+    bm, tm = run_md([s] * nw, lambdas, vels, abi.QNB_PREC_MIXED, args.steps, args.warmup, positions=xs,
+                    extra=10 * args.frames)
     steps_per_s = args.steps / tm
     ns_day = nw * steps_per_s * 1.0 * 0.0864
     counts = bm.pair_counts(0)
@@ -159,6 +212,11 @@ def main():
     t_forces = bm.time_kernel(1, 20)
     e = bm.energies(0)
     temps = e[:, abi.F_TEMP]
+    prec = free_energy_precision(bm, s, lambdas, args.frames) if args.frames > 0 else None
+    if prec:
+        print(f"# dG precision: max |dG_mixed - dG_fp64| = {prec['max_abs_dG_diff_kcal_mol']:.2e} kcal/mol; "
+              f"adjoint {prec['adjoint']['frames']} frames in {prec['adjoint']['launch_s']:.3f} s",
+              file=sys.stderr, flush=True)
     bm.close()
 
     units, tests = lrf_units(s, s.x)
@@ -207,6 +265,7 @@ def main():
         "temperature_range_K": [round(float(temps.min()), 1), round(float(temps.max()), 1)] if len(temps) else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "free_energy_precision": prec,
     }
     print(json.dumps(out))
 

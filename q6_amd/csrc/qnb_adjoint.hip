@@ -24,8 +24,8 @@ __device__ __forceinline__ void adj_pair(double (&g)[NS][QNB_ADJ_NV], int s, R r
   R rq = tq * r;
   double e_vdw = 0.0, gA = 0.0, gB = 0.0;
   if (lj) {
-    R r6_hc = R(1) / r6in;
-    R r6 = R(1) / (r6_hc + sc);
+    R r6_hc = recip(r6in);
+    R r6 = recip(r6_hc + sc);
     R r12 = r6 * r6;
     R Va = A * r12, Vb = B * r6;
     e_vdw = (double)Va - (double)Vb;
@@ -72,8 +72,8 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevSys* __restrict__ syst
     for (int k = 0; k < sa; k++) {
       const int j = o + k;
       const R ux = (R)(x[3 * j] - xi0), uy = (R)(x[3 * j + 1] - xi1), uz = (R)(x[3 * j + 2] - xi2);
-      const R r2 = R(1) / (ux * ux + uy * uy + uz * uz);
-      const R r = sqrt(r2);
+      R r2, r;
+      rinv(ux * ux + uy * uy + uz * uz, r2, r);
       const R r6in = r2 * r2 * r2;
       const bool lj = k == 0 || !S.spc;
 #pragma unroll
@@ -92,8 +92,8 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevSys* __restrict__ syst
     const int sw = S.gswitch[S.atom_group[j]];
     if (S.excl[sw] || dist4p(x, sw, S.xp) > S.rcq2) continue;
     const R ux = (R)(x[3 * j] - xi0), uy = (R)(x[3 * j + 1] - xi1), uz = (R)(x[3 * j + 2] - xi2);
-    const R r2 = R(1) / (ux * ux + uy * uy + uz * uz);
-    const R r = sqrt(r2);
+    R r2, r;
+    rinv(ux * ux + uy * uy + uz * uz, r2, r);
     const R r6in = r2 * r2 * r2;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -110,8 +110,8 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevSys* __restrict__ syst
     const int st = S.qq_st[p];
     const int a = S.iqseq[ia], b = S.iqseq[ja];
     const R ux = (R)(x[3 * b] - x[3 * a]), uy = (R)(x[3 * b + 1] - x[3 * a + 1]), uz = (R)(x[3 * b + 2] - x[3 * a + 2]);
-    const R r2 = R(1) / (ux * ux + uy * uy + uz * uz);
-    const R r = sqrt(r2);
+    R r2, r;
+    rinv(ux * ux + uy * uy + uz * uz, r2, r);
     const double* pr = S.qq_par + 4 * p;
     const size_t t = (size_t)ia + (size_t)nq * ((size_t)ja + (size_t)nq * st);
     const bool mine = ia == iq;
@@ -141,8 +141,8 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevSys* __restrict__ syst
     if (S.qqp_iq[p] != iq) continue;
     const int j = S.qqp_j[p], st = S.qqp_st[p];
     const R ux = (R)(x[3 * j] - xi0), uy = (R)(x[3 * j + 1] - xi1), uz = (R)(x[3 * j + 2] - xi2);
-    const R r2 = R(1) / (ux * ux + uy * uy + uz * uz);
-    const R r = sqrt(r2);
+    R r2, r;
+    rinv(ux * ux + uy * uy + uz * uz, r2, r);
     const double* pr = S.qqp_par + 4 * p;
     const size_t t = (size_t)j + (size_t)nsol * ((size_t)iq + (size_t)nq * st);
     adj_pair<R, NS>(g, st, r2, r, r2 * r2 * r2, (R)pr[0], (R)pr[1], (R)pr[2], (R)pr[3], (R)A.qp_dq[t],

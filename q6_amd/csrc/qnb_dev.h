@@ -184,6 +184,21 @@ __device__ __forceinline__ bool pp_param(const DevSys& S, int i, int j, double& 
   return set;
 }
 
+// 1/|u|^2 and 1/|u| from |u|^2: IEEE division and square root in FP64 (the
+// plain path evaluates them as q_dist does); in FP32 the hardware reciprocal
+// square root (v_rsq_f32, 1 ulp), the usual FP32 pair-kernel form
+__device__ __forceinline__ void rinv(double d2, double& r2, double& r) {
+  r2 = 1.0 / d2;
+  r = sqrt(r2);
+}
+__device__ __forceinline__ void rinv(float d2, float& r2, float& r) {
+  r = __builtin_amdgcn_rsqf(d2);
+  r2 = r * r;
+}
+// 1/v: IEEE in FP64, hardware reciprocal (v_rcp_f32, 1 ulp) in FP32
+__device__ __forceinline__ double recip(double v) { return 1.0 / v; }
+__device__ __forceinline__ float recip(float v) { return __builtin_amdgcn_rcpf(v); }
+
 // Pair arithmetic in precision R (double: plain path; float: mixed path).
 // u = x_j - x_i arrives in FP64 (the difference of FP64 positions), is rounded
 // to R, and the returned scalar dv is in R; callers accumulate u*dv in FP64.
@@ -191,8 +206,8 @@ template <typename R>
 struct PairMath {
   // nbe (nonbonded.f90:45-75): LJ + Coulomb.  Adds V_el, V_a - V_b.
   static __device__ __forceinline__ R nbe(R ux, R uy, R uz, R A, R B, R q, double& el, double& vdw) {
-    R r2 = R(1) / (ux * ux + uy * uy + uz * uz);
-    R r = sqrt(r2);
+    R r2, r;
+    rinv(ux * ux + uy * uy + uz * uz, r2, r);
     R r6 = r2 * r2 * r2;
     R r12 = r6 * r6;
     R Vel = q * r, Va = A * r12, Vb = B * r6;
@@ -202,8 +217,8 @@ struct PairMath {
   }
   // nbe_spc (nonbonded.f90:247-267): Coulomb only
   static __device__ __forceinline__ R nbe_spc(R ux, R uy, R uz, R q, double& el) {
-    R r2 = R(1) / (ux * ux + uy * uy + uz * uz);
-    R r = sqrt(r2);
+    R r2, r;
+    rinv(ux * ux + uy * uy + uz * uz, r2, r);
     R Vel = q * r;
     el += (double)Vel;
     return -r2 * Vel;
@@ -211,8 +226,8 @@ struct PairMath {
   // nbe_qx (nonbonded.f90:200-222), soft-core score sc, state weight lam
   static __device__ __forceinline__ R nbe_qx(R r2, R r, R r6in, R A, R B, R q, R sc, R lam, double& el,
                                              double& vdw) {
-    R r6_hc = R(1) / r6in;
-    R r6 = R(1) / (r6_hc + sc);
+    R r6_hc = recip(r6in);
+    R r6 = recip(r6_hc + sc);
     R r12 = r6 * r6;
     R Vel = q * r, Va = A * r12, Vb = B * r6;
     el += (double)Vel;

@@ -26,37 +26,48 @@ __device__ __forceinline__ bool owns(int a, int b) {
 // LRF expansion (lrf_update, nonbondene.f90:628-725), restated on the 20
 // independent components of the symmetric tensors: acc[0] phi0, acc[1..3]
 // phi1, acc[4..9] phi2 (xx xy xz yy yz zz), acc[10..19] phi3 (xxx xxy xxz xyy
-// xyz xzz yyy yyz yzz zzz).  dr = x_i - centre in FP64, terms in R, sums FP64.
+// xyz xzz yyy yyz yzz zzz).  dr = x_i - centre in FP64, terms in R; the
+// caller sums one source group's terms in R (at most a group's atoms) and
+// adds the group's partial to its FP64 accumulators.
 template <typename R>
-__device__ __forceinline__ void lrf_accumulate(double* acc, double drx, double dry, double drz, double q) {
+__device__ __forceinline__ void lrf_accumulate(R* acc, double drx, double dry, double drz, double q) {
   R dx = (R)drx, dy = (R)dry, dz = (R)drz;
   R r2 = dx * dx + dy * dy + dz * dz;
-  R f0 = (R)q / (r2 * sqrt(r2));
-  R f1 = R(3) * f0 / r2;
-  R f2 = -f1 / r2;
-  acc[0] += (double)(f0 * r2);
-  acc[1] -= (double)(dx * f0);
-  acc[2] -= (double)(dy * f0);
-  acc[3] -= (double)(dz * f0);
+  R f0, f1, f2;
+  if constexpr (sizeof(R) == 4) {
+    R ri2, ri;
+    rinv(r2, ri2, ri);
+    f0 = (R)q * ri * ri2;
+    f1 = R(3) * f0 * ri2;
+    f2 = -f1 * ri2;
+  } else {
+    f0 = (R)q / (r2 * sqrt(r2));
+    f1 = R(3) * f0 / r2;
+    f2 = -f1 / r2;
+  }
+  acc[0] += f0 * r2;
+  acc[1] -= dx * f0;
+  acc[2] -= dy * f0;
+  acc[3] -= dz * f0;
   R xx = dx * dx, xy = dx * dy, xz = dx * dz, yy = dy * dy, yz = dy * dz, zz = dz * dz;
-  acc[4] += (double)(xx * f1 - f0);
-  acc[5] += (double)(xy * f1);
-  acc[6] += (double)(xz * f1);
-  acc[7] += (double)(yy * f1 - f0);
-  acc[8] += (double)(yz * f1);
-  acc[9] += (double)(zz * f1 - f0);
+  acc[4] += xx * f1 - f0;
+  acc[5] += xy * f1;
+  acc[6] += xz * f1;
+  acc[7] += yy * f1 - f0;
+  acc[8] += yz * f1;
+  acc[9] += zz * f1 - f0;
   R g = R(5) * f2, h = f2 * r2;
   R gx = g * dx, gy = g * dy, gz = g * dz;
-  acc[10] += (double)(gx * xx - R(3) * h * dx);
-  acc[11] += (double)(gx * xy - h * dy);
-  acc[12] += (double)(gx * xz - h * dz);
-  acc[13] += (double)(gx * yy - h * dx);
-  acc[14] += (double)(gx * yz);
-  acc[15] += (double)(gx * zz - h * dx);
-  acc[16] += (double)(gy * yy - R(3) * h * dy);
-  acc[17] += (double)(gy * yz - h * dz);
-  acc[18] += (double)(gy * zz - h * dy);
-  acc[19] += (double)(gz * zz - R(3) * h * dz);
+  acc[10] += gx * xx - R(3) * h * dx;
+  acc[11] += gx * xy - h * dy;
+  acc[12] += gx * xz - h * dz;
+  acc[13] += gx * yy - h * dx;
+  acc[14] += gx * yz;
+  acc[15] += gx * zz - h * dx;
+  acc[16] += gy * yy - R(3) * h * dy;
+  acc[17] += gy * yz - h * dz;
+  acc[18] += gy * zz - h * dy;
+  acc[19] += gz * zz - R(3) * h * dz;
 }
 
 // index of the symmetric third-rank component (a<=b<=c sorted) in acc[10..]
@@ -121,18 +132,23 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
           if (r2 <= cut2) {
             in = true;
           } else if (S.use_lrf && r2 <= S.rclrf2) {
+            R part[20];
+#pragma unroll
+            for (int k = 0; k < 20; k++) part[k] = R(0);
             if (b_sol) {
               for (int k = S.gfirst[b]; k <= S.glast[b]; k++) {
                 int i = S.cgpatom[k];
                 if (S.iqatom[i] != 0) continue;
-                lrf_accumulate<R>(acc, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
+                lrf_accumulate<R>(part, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
               }
             } else {
               for (int k = 0; k < S.sa; k++) {
                 int i = rb + k;
-                lrf_accumulate<R>(acc, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
+                lrf_accumulate<R>(part, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
               }
             }
+#pragma unroll
+            for (int k = 0; k < 20; k++) acc[k] += (double)part[k];
           }
         }
       }
@@ -326,32 +342,87 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
       xi[k][2] = x[3 * (o + k) + 2];
     }
     if (sa == 3 && S.spc) {
+      // FP32 pair path: partner atoms relative to the receiver's oxygen are
+      // formed in FP64 and rounded once; a molecule pair's 9 atom pairs are
+      // summed in FP32 and added to the FP64 accumulators once per partner
+      float ro[3][3], wel[9];
+      if constexpr (sizeof(R) == 4) {
+#pragma unroll
+        for (int la = 0; la < 3; la++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) ro[la][c] = (float)(xi[la][c] - xi[0][c]);
+#pragma unroll
+        for (int k = 0; k < 9; k++) wel[k] = (float)S.wwEl[k];
+      }
+      const float wA = (float)S.wwA[0], wB = (float)S.wwB[0];
       for (int p = lane; p < nn; p += QNB_WAVE) {
         const int b = row[p];
         if (b >= S.ncgp_solute) {
           if (!(mask & T_WW)) continue;
           const int jo = S.nat_solute + 3 * (b - S.ncgp_solute);
           const bool own = owns(a, b);
-          double el = 0.0, vdw = 0.0;
+          if constexpr (sizeof(R) == 4) {
+            float pj[3][3];
 #pragma unroll
-          for (int la = 0; la < 3; la++)
+            for (int ka = 0; ka < 3; ka++)
 #pragma unroll
-            for (int ka = 0; ka < 3; ka++) {
-              const int j = jo + ka;
-              const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
-              const int k = la + 3 * ka;
-              R dv;
-              if (la == 0 && ka == 0)
-                dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.wwA[0], (R)S.wwB[0], (R)S.wwEl[0], el, vdw);
-              else
-                dv = PairMath<R>::nbe_spc((R)ux, (R)uy, (R)uz, (R)S.wwEl[k], el);
-              f[la][0] -= (double)((R)ux * dv);
-              f[la][1] -= (double)((R)uy * dv);
-              f[la][2] -= (double)((R)uz * dv);
+              for (int c = 0; c < 3; c++) pj[ka][c] = (float)(x[3 * (jo + ka) + c] - xi[0][c]);
+            float fl[3][3], elf = 0.f, vdwf = 0.f;
+#pragma unroll
+            for (int la = 0; la < 3; la++) fl[la][0] = fl[la][1] = fl[la][2] = 0.f;
+#pragma unroll
+            for (int la = 0; la < 3; la++)
+#pragma unroll
+              for (int ka = 0; ka < 3; ka++) {
+                const float ux = pj[ka][0] - ro[la][0], uy = pj[ka][1] - ro[la][1], uz = pj[ka][2] - ro[la][2];
+                float r2, r;
+                rinv(ux * ux + uy * uy + uz * uz, r2, r);
+                const float Vel = wel[la + 3 * ka] * r;
+                float dv;
+                elf += Vel;
+                if (la == 0 && ka == 0) {
+                  const float r6 = r2 * r2 * r2, Va = wA * r6 * r6, Vb = wB * r6;
+                  vdwf += Va - Vb;
+                  dv = r2 * (-Vel - 12.f * Va + 6.f * Vb);
+                } else {
+                  dv = -r2 * Vel;
+                }
+                fl[la][0] -= ux * dv;
+                fl[la][1] -= uy * dv;
+                fl[la][2] -= uz * dv;
+              }
+#pragma unroll
+            for (int la = 0; la < 3; la++) {
+              f[la][0] += (double)fl[la][0];
+              f[la][1] += (double)fl[la][1];
+              f[la][2] += (double)fl[la][2];
             }
-          if (own) {
-            e[E_WW_EL] += el;
-            e[E_WW_VDW] += vdw;
+            if (own) {
+              e[E_WW_EL] += (double)elf;
+              e[E_WW_VDW] += (double)vdwf;
+            }
+          } else {
+            double el = 0.0, vdw = 0.0;
+#pragma unroll
+            for (int la = 0; la < 3; la++)
+#pragma unroll
+              for (int ka = 0; ka < 3; ka++) {
+                const int j = jo + ka;
+                const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
+                const int k = la + 3 * ka;
+                R dv;
+                if (la == 0 && ka == 0)
+                  dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.wwA[0], (R)S.wwB[0], (R)S.wwEl[0], el, vdw);
+                else
+                  dv = PairMath<R>::nbe_spc((R)ux, (R)uy, (R)uz, (R)S.wwEl[k], el);
+                f[la][0] -= (double)((R)ux * dv);
+                f[la][1] -= (double)((R)uy * dv);
+                f[la][2] -= (double)((R)uz * dv);
+              }
+            if (own) {
+              e[E_WW_EL] += el;
+              e[E_WW_VDW] += vdw;
+            }
           }
         } else {
           if (!(mask & T_PW)) continue;
@@ -419,8 +490,8 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
         for (int k = 0; k < sa && k < 3; k++) {
           const double ux = xi[k][0] - x[3 * i], uy = xi[k][1] - x[3 * i + 1], uz = xi[k][2] - x[3 * i + 2];
           const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
-          R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
-          R r = sqrt(r2);
+          R r2, r;
+          rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
           R dvs = R(0);
           if (k == 0 || !S.spc) {
             R r6 = r2 * r2 * r2;
@@ -540,8 +611,8 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
           const int q = S.iqseq[iq];
           const double ux = xix - x[3 * q], uy = xiy - x[3 * q + 1], uz = xiz - x[3 * q + 2];
           const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
-          R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
-          R r = sqrt(r2);
+          R r2, r;
+          rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
           R r6 = r2 * r2 * r2;
 #pragma unroll
           for (int s = 0; s < NS; s++) {
@@ -619,8 +690,8 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
         const int j = o + k;
         const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
         const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
-        R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
-        R r = sqrt(r2);
+        R r2, r;
+        rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
         R dvs = R(0);
         if (k == 0 || !S.spc) {
           R r6 = r2 * r2 * r2;
@@ -649,8 +720,8 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
       const int j = S.qp_list[pp];
       const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
       const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
-      R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
-      R r = sqrt(r2);
+      R r2, r;
+      rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
       R r6 = r2 * r2 * r2;
 #pragma unroll
       for (int s = 0; s < NS; s++) {
@@ -693,11 +764,11 @@ __global__ __launch_bounds__(64) void k_qq(const DevSys* __restrict__ systems, u
       const double* pr = S.qq_par + 4 * p;
       const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
       const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
-      R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
-      R r = sqrt(r2);
+      R r2, r;
+      rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
       R r6d = r2 * r2 * r2;
-      R r6_hc = R(1) / r6d;
-      R r6 = R(1) / (r6_hc + (R)pr[3]);
+      R r6_hc = recip(r6d);
+      R r6 = recip(r6_hc + (R)pr[3]);
       R r12 = r6 * r6;
       R lam = (R)S.lambda[s];
       R Vel = (R)pr[2] * r, Va, Vb, dv;
@@ -728,8 +799,8 @@ __global__ __launch_bounds__(64) void k_qq(const DevSys* __restrict__ systems, u
       const double* pr = S.qqp_par + 4 * p;
       const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
       const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
-      R r2 = R(1) / (rux * rux + ruy * ruy + ruz * ruz);
-      R r = sqrt(r2);
+      R r2, r;
+      rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
       R r6 = r2 * r2 * r2;
       R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)pr[0], (R)pr[1], (R)pr[2], (R)pr[3], (R)S.lambda[s], eq[s][2],
                                  eq[s][3]);

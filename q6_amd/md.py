@@ -60,10 +60,17 @@ class BatchMD:
                                                           "itdis": 100.0 / dt_fs})
             par.precision = precision
             md = s.md
+            # harmonic bonds, then the harmonic distance restraints (same form)
             bonds = np.ascontiguousarray(md["bonds"], np.int32).reshape(-1, 2)
+            rst = np.asarray(md.get("restraints", np.zeros((0, 2), np.int32)), np.int32).reshape(-1, 2)
+            r0 = np.concatenate([np.broadcast_to(md["bond_r0"], len(bonds)),
+                                 np.broadcast_to(md.get("restraint_r0", 0.0), len(rst))])
+            kb = np.concatenate([np.broadcast_to(md["bond_k"], len(bonds)),
+                                 np.broadcast_to(md.get("restraint_k", 0.0), len(rst))])
+            bonds = np.concatenate([bonds, rst])
             angles = np.ascontiguousarray(md["angles"], np.int32).reshape(-1, 3)
-            arrs = dict(bond_ij=bonds.ravel(), bond_r0=np.full(len(bonds), md["bond_r0"]),
-                        bond_k=np.full(len(bonds), md["bond_k"]), angle_ijk=angles.ravel(),
+            arrs = dict(bond_ij=bonds.ravel(), bond_r0=r0.astype(np.float64), bond_k=kb.astype(np.float64),
+                        angle_ijk=angles.ravel(),
                         angle_t0=np.full(len(angles), md["angle_t0"]), angle_k=np.full(len(angles), md["angle_k"]),
                         winv=1.0 / np.asarray(md["mass"], float))
             arrs = {a: np.ascontiguousarray(v) for a, v in arrs.items()}

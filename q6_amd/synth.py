@@ -421,7 +421,19 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
     s.par["lambda"] = lam
     # MD data: masses, bonded terms, constraints
     mass = mass_t[iac]
+    # compact groups keep their shape through distance restraints from every
+    # bead to the switch atom at the built distance (Qdyn6 [distance_restraints]
+    # with d1 = d2, p_restrain src/bondene.f90:905-950): bead chains have no
+    # torsions and would otherwise unfold beyond the switch-atom cut-off
+    rst = []
+    if compact is not None:
+        for c in range(nchain):
+            for k in range(first[c], last[c] + 1):
+                if k != sw[c]:
+                    rst.append((sw[c], k, float(np.linalg.norm(x[k] - x[sw[c]]))))
     s.md = dict(mass=mass, bonds=np.array(bonds, np.int32).reshape(-1, 2), bond_r0=1.5, bond_k=600.0,
+                restraints=np.array([(i, j) for i, j, _ in rst], np.int32).reshape(-1, 2),
+                restraint_r0=np.array([r for _, _, r in rst]), restraint_k=20.0,
                 angles=np.array(angles, np.int32).reshape(-1, 3), angle_t0=math.radians(109.5), angle_k=80.0,
                 water_roh=TIP3P_ROH, water_rhh=2 * TIP3P_ROH * math.sin(math.radians(TIP3P_HOH) / 2),
                 chain_region=r_used, qcrg=qcrg, qA=qA, qB=qB, qctx=qctx)
