@@ -488,6 +488,9 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     HIPCHK(hipMemset(M.ctr, 0, 4 * sizeof(int)));
     M.tfree = b->tfree + k;
     M.istep = b->istep + k;
+    // the nonbonded kernels accumulate energies on the sampled steps only
+    b->hs[k].step_ptr = M.istep;
+    b->hs[k].ene_every = m->ene_every;
   }
   if ((rc = b->mem.alloc(&b->dsys, n)) || (rc = b->mem.alloc(&b->dmd, n))) return rc;
   // the MD records change after this point only in qnb_batch_store_frames

@@ -110,7 +110,15 @@ struct DevSys {
   int* sh_list;      // [QNB_MAXSH][nwat]
   int* sh_rank;      // [QNB_MAXSH][nwat]
   int* sh_n;         // [QNB_MAXSH]
+  // energy sampling: energies are accumulated when ene_every <= 0 or when
+  // the instance's step counter is a multiple of ene_every
+  const int* step_ptr;
+  int ene_every;
 };
+
+__device__ __forceinline__ bool want_energies(const DevSys& S) {
+  return S.ene_every <= 0 || (*S.step_ptr % S.ene_every) == 0;
+}
 
 // ---------------------------------------------------------------- helpers
 
@@ -205,33 +213,39 @@ __device__ __forceinline__ float recip(float v) { return __builtin_amdgcn_rcpf(v
 template <typename R>
 struct PairMath {
   // nbe (nonbonded.f90:45-75): LJ + Coulomb.  Adds V_el, V_a - V_b.
-  static __device__ __forceinline__ R nbe(R ux, R uy, R uz, R A, R B, R q, double& el, double& vdw) {
+  // `want`: accumulate the energies (only the sampled steps need them)
+  static __device__ __forceinline__ R nbe(R ux, R uy, R uz, R A, R B, R q, double& el, double& vdw,
+                                          bool want = true) {
     R r2, r;
     rinv(ux * ux + uy * uy + uz * uz, r2, r);
     R r6 = r2 * r2 * r2;
     R r12 = r6 * r6;
     R Vel = q * r, Va = A * r12, Vb = B * r6;
-    el += (double)Vel;
-    vdw += (double)Va - (double)Vb;
+    if (want) {
+      el += (double)Vel;
+      vdw += (double)Va - (double)Vb;
+    }
     return r2 * (-Vel - R(12) * Va + R(6) * Vb);
   }
   // nbe_spc (nonbonded.f90:247-267): Coulomb only
-  static __device__ __forceinline__ R nbe_spc(R ux, R uy, R uz, R q, double& el) {
+  static __device__ __forceinline__ R nbe_spc(R ux, R uy, R uz, R q, double& el, bool want = true) {
     R r2, r;
     rinv(ux * ux + uy * uy + uz * uz, r2, r);
     R Vel = q * r;
-    el += (double)Vel;
+    if (want) el += (double)Vel;
     return -r2 * Vel;
   }
   // nbe_qx (nonbonded.f90:200-222), soft-core score sc, state weight lam
   static __device__ __forceinline__ R nbe_qx(R r2, R r, R r6in, R A, R B, R q, R sc, R lam, double& el,
-                                             double& vdw) {
+                                             double& vdw, bool want = true) {
     R r6_hc = recip(r6in);
     R r6 = recip(r6_hc + sc);
     R r12 = r6 * r6;
     R Vel = q * r, Va = A * r12, Vb = B * r6;
-    el += (double)Vel;
-    vdw += (double)Va - (double)Vb;
+    if (want) {
+      el += (double)Vel;
+      vdw += (double)Va - (double)Vb;
+    }
     return r2 * (-Vel - (R(12) * Va - R(6) * Vb) * r6 * r6_hc) * lam;
   }
 };

@@ -326,6 +326,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
   for (int k = 0; k < 4 * NS; k++) eq[k] = 0.0;
   const int ra = group_ref_atom(S, a);
   const bool aexcl = S.excl[ra];
+  const bool want_e = want_energies(S);
 
   if (a >= S.ncgp_solute) {
     // ------------------------------------------------ water receiver
@@ -367,7 +368,9 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             for (int ka = 0; ka < 3; ka++)
 #pragma unroll
               for (int c = 0; c < 3; c++) pj[ka][c] = (float)(x[3 * (jo + ka) + c] - xi[0][c]);
-            float fl[3][3], elf = 0.f, vdwf = 0.f;
+            float fl[3][3];
+            double el = 0.0, vdw = 0.0;
+            const bool acc_e = want_e && own;
 #pragma unroll
             for (int la = 0; la < 3; la++) fl[la][0] = fl[la][1] = fl[la][2] = 0.f;
 #pragma unroll
@@ -379,10 +382,10 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
                 rinv(ux * ux + uy * uy + uz * uz, r2, r);
                 const float Vel = wel[la + 3 * ka] * r;
                 float dv;
-                elf += Vel;
+                if (acc_e) el += (double)Vel;
                 if (la == 0 && ka == 0) {
                   const float r6 = r2 * r2 * r2, Va = wA * r6 * r6, Vb = wB * r6;
-                  vdwf += Va - Vb;
+                  if (acc_e) vdw += (double)Va - (double)Vb;
                   dv = r2 * (-Vel - 12.f * Va + 6.f * Vb);
                 } else {
                   dv = -r2 * Vel;
@@ -397,9 +400,9 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
               f[la][1] += (double)fl[la][1];
               f[la][2] += (double)fl[la][2];
             }
-            if (own) {
-              e[E_WW_EL] += (double)elf;
-              e[E_WW_VDW] += (double)vdwf;
+            if (acc_e) {
+              e[E_WW_EL] += el;
+              e[E_WW_VDW] += vdw;
             }
           } else {
             double el = 0.0, vdw = 0.0;
@@ -412,9 +415,10 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
                 const int k = la + 3 * ka;
                 R dv;
                 if (la == 0 && ka == 0)
-                  dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.wwA[0], (R)S.wwB[0], (R)S.wwEl[0], el, vdw);
+                  dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.wwA[0], (R)S.wwB[0], (R)S.wwEl[0], el, vdw,
+                                        want_e && own);
                 else
-                  dv = PairMath<R>::nbe_spc((R)ux, (R)uy, (R)uz, (R)S.wwEl[k], el);
+                  dv = PairMath<R>::nbe_spc((R)ux, (R)uy, (R)uz, (R)S.wwEl[k], el, want_e && own);
                 f[la][0] -= (double)((R)ux * dv);
                 f[la][1] -= (double)((R)uy * dv);
                 f[la][2] -= (double)((R)uz * dv);
@@ -434,7 +438,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             for (int la = 0; la < 3; la++) {
               const size_t t = (size_t)j + (size_t)S.nat_solute * la;
               const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
-              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], el, vdw);
+              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], el, vdw, false);
               f[la][0] -= (double)((R)ux * dv);
               f[la][1] -= (double)((R)uy * dv);
               f[la][2] -= (double)((R)uz * dv);
@@ -455,7 +459,8 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             for (int ka = 0; ka < sa; ka++) {
               const int j = jo + ka, k = la + sa * ka;
               const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
-              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.wwA[k], (R)S.wwB[k], (R)S.wwEl[k], el, vdw);
+              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.wwA[k], (R)S.wwB[k], (R)S.wwEl[k], el, vdw,
+                                      want_e && own);
               f[la][0] -= (double)((R)ux * dv);
               f[la][1] -= (double)((R)uy * dv);
               f[la][2] -= (double)((R)uz * dv);
@@ -473,7 +478,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             for (int la = 0; la < sa && la < 3; la++) {
               const size_t t = (size_t)j + (size_t)S.nat_solute * la;
               const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
-              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], el, vdw);
+              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], el, vdw, false);
               f[la][0] -= (double)((R)ux * dv);
               f[la][1] -= (double)((R)uy * dv);
               f[la][2] -= (double)((R)uz * dv);
@@ -500,7 +505,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
               if (s >= ns) break;
               const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
               dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)S.qwA[t], (R)S.qwB[t], (R)S.qwEl[t], (R)S.qwSc[t],
-                                         (R)S.lambda[s], eq[4 * s + 2], eq[4 * s + 3]);
+                                         (R)S.lambda[s], eq[4 * s + 2], eq[4 * s + 3], want_e);
             }
           } else {
 #pragma unroll
@@ -508,7 +513,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
               if (s >= ns) break;
               const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
               R Vel = (R)S.qwEl[t] * r;  // nbe_qspc (nonbonded.f90:226-243)
-              eq[4 * s + 2] += (double)Vel;
+              if (want_e) eq[4 * s + 2] += (double)Vel;
               dvs += -r2 * Vel * (R)S.lambda[s];
             }
           }
@@ -558,7 +563,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
               if (!pp_param(S, i, j, A, B, el)) continue;
               const double ux = x[3 * j] - xix, uy = x[3 * j + 1] - xiy, uz = x[3 * j + 2] - xiz;
               double ee = 0.0, ev = 0.0;
-              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)A, (R)B, (R)el, ee, ev);
+              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)A, (R)B, (R)el, ee, ev, want_e && own);
               fi[0] -= (double)((R)ux * dv);
               fi[1] -= (double)((R)uy * dv);
               fi[2] -= (double)((R)uz * dv);
@@ -575,7 +580,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
               const size_t t = (size_t)i + (size_t)S.nat_solute * k;
               const double ux = x[3 * j] - xix, uy = x[3 * j + 1] - xiy, uz = x[3 * j + 2] - xiz;
               R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], e[E_PW_EL],
-                                      e[E_PW_VDW]);
+                                      e[E_PW_VDW], want_e);
               fi[0] -= (double)((R)ux * dv);
               fi[1] -= (double)((R)uy * dv);
               fi[2] -= (double)((R)uz * dv);
@@ -592,7 +597,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             if (!pp_param(S, i, j, A, B, el)) continue;
             const double ux = x[3 * j] - xix, uy = x[3 * j + 1] - xiy, uz = x[3 * j + 2] - xiz;
             double ee = 0.0, ev = 0.0;
-            R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)A, (R)B, (R)el, ee, ev);
+            R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)A, (R)B, (R)el, ee, ev, want_e && i < j);
             fi[0] -= (double)((R)ux * dv);
             fi[1] -= (double)((R)uy * dv);
             fi[2] -= (double)((R)uz * dv);
@@ -619,7 +624,7 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             if (s >= ns) break;
             const size_t t = (size_t)i + (size_t)nsol * ((size_t)iq + (size_t)nq * s);
             R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)S.qpA[t], (R)S.qpB[t], (R)S.qpEl[t], (R)S.qpSc[t],
-                                       (R)S.lambda[s], eq[4 * s + 0], eq[4 * s + 1]);
+                                       (R)S.lambda[s], eq[4 * s + 0], eq[4 * s + 1], want_e);
             fi[0] += (double)(rux * dv);
             fi[1] += (double)(ruy * dv);
             fi[2] += (double)(ruz * dv);
@@ -700,7 +705,7 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
             if (s >= ns) break;
             const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
             dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)S.qwA[t], (R)S.qwB[t], (R)S.qwEl[t], (R)S.qwSc[t],
-                                       (R)S.lambda[s], dummy_e, dummy_v);
+                                       (R)S.lambda[s], dummy_e, dummy_v, false);
           }
         } else {
 #pragma unroll
@@ -728,7 +733,7 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
         if (s >= ns) break;
         const size_t t = (size_t)j + (size_t)S.nat_solute * ((size_t)iq + (size_t)nq * s);
         R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)S.qpA[t], (R)S.qpB[t], (R)S.qpEl[t], (R)S.qpSc[t],
-                                   (R)S.lambda[s], dummy_e, dummy_v);
+                                   (R)S.lambda[s], dummy_e, dummy_v, false);
         f[0] -= (double)(rux * dv);
         f[1] -= (double)(ruy * dv);
         f[2] -= (double)(ruz * dv);
