@@ -466,7 +466,7 @@ static void launch_eval(qnb_handle* h, uint32_t mask) {
   const int with_pol = S.nwat > 0 && S.wpol && S.nshell > 0 && (mask & QNB_T_POL);
   if (with_qq) hipLaunchKernelGGL((k_qq<R>), dim3(1), dim3(64), 0, h->stream, h->dsys, mask);
   if (with_pol) hipLaunchKernelGGL(k_watpol, dim3(1), dim3(1024), 0, h->stream, h->dsys, (const int*)h->d_istep);
-  hipLaunchKernelGGL(k_reduce_energies, dim3(1), dim3(64), 0, h->stream, h->dsys, with_qq, with_pol);
+  hipLaunchKernelGGL(k_reduce_energies, dim3(1), dim3(1024), 0, h->stream, h->dsys, with_qq, with_pol);
 }
 
 template <typename R>

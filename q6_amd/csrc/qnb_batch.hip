@@ -365,7 +365,7 @@ static void enqueue_step(qnb_batch* b, bool lists) {
     default: enqueue_forces<R, 8>(b); break;
   }
   hipLaunchKernelGGL(k_bonded, dim3(B), dim3(256), 0, b->stream, b->dsys, b->dmd);
-  hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(64), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
+  hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(1024), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
   hipLaunchKernelGGL(k_snap, dim3((3 * b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_leapfrog, dim3((b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);

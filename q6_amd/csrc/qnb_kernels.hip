@@ -302,204 +302,157 @@ enum : uint32_t { T_WW = 1, T_QW = 2, T_PW = 4, T_PP = 8, T_QP = 16, T_LRF = 32,
                   T_RAD = 256, T_POL = 512 };
 
 // ---------------------------------------------------------------------------
-// Receiver-side forces: one wavefront per charge group a.  Water receivers
-// take the rigid 3-site fast path (9 atom pairs per water partner, the O-O pair
-// with LJ, nonbond_ww_spc nonbondene.f90:5886-5975); solute receivers walk
-// their atoms one at a time.  Writes (not adds) d for every atom of a, and the
-// wave's energy partials into epart[a].
+// Receiver-side forces: one wavefront per charge group a.  The wave expands
+// its neighbour row into partner atoms 64 groups at a time (a wave prefix sum
+// over the groups' atom counts, staged in LDS) and gives every lane one
+// partner atom at a time, so lanes stay balanced however the row mixes 3-atom
+// waters and larger solute groups.  A lane forms the partner's position
+// relative to the receiver's reference atom once in FP64, rounds it to R and
+// evaluates it against the receiver atoms (offsets from the same reference
+// point), accumulating per-lane forces in R; the wave reduces them in FP64.
+// Pair forms: water-water as nonbond_ww_spc / nonbond_ww
+// (nonbondene.f90:5886-5975, 5509-5573: O-O LJ + Coulomb, the other 8
+// Coulomb only for SPC), pw from the pw table (nonbond_pw 4856-4916), pp from
+// pp_param (nonbond_pp 4694-4760).  Writes (not adds) d for every atom of a,
+// and the wave's energy partials into epart[a] on the sampled steps.
+#define FC_WPB 8    // waves per block at most (launch bound 512)
+// solute receiver atoms per sweep over the partners (per-lane accumulators
+// for each: 3 values of R)
+#define FC_CH 10
+
+// partner-atom expansion of 64 neighbour groups: returns the chunk's atom
+// count; s_off/s_grp hold each lane's group and its first atom's rank
+__device__ __forceinline__ int expand_groups(const DevSys& S, const int* __restrict__ row, int p0, int nn,
+                                            uint32_t use_sol, uint32_t use_wat, int* s_off, int* s_grp) {
+  const int lane = threadIdx.x % QNB_WAVE;
+  const int p = p0 + lane;
+  int b = -1, cnt = 0;
+  if (p < nn) {
+    b = row[p];
+    if (b < S.ncgp_solute ? use_sol : use_wat) cnt = S.glast[b] - S.gfirst[b] + 1;
+  }
+  int incl = cnt;
+#pragma unroll
+  for (int o2 = 1; o2 < QNB_WAVE; o2 <<= 1) {
+    const int v = __shfl_up(incl, o2, QNB_WAVE);
+    if (lane >= o2) incl += v;
+  }
+  s_off[lane] = incl - cnt;
+  s_grp[lane] = b;
+  __builtin_amdgcn_wave_barrier();
+  return __shfl(incl, QNB_WAVE - 1, QNB_WAVE);
+}
+
+// the group holding expanded atom t (largest g with s_off[g] <= t)
+__device__ __forceinline__ int owner_slot(const int* s_off, int t) {
+  int lo = 0, hi = QNB_WAVE - 1;
+#pragma unroll
+  for (int it = 0; it < 6; it++) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s_off[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 template <typename R, int NS>
-__global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ systems, uint32_t mask,
-                                                const double* __restrict__ tfree) {
+__global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict__ systems, uint32_t mask,
+                                                        const double* __restrict__ tfree) {
   const DevSys& S = systems[blockIdx.y];
-  const int a = blockIdx.x * (blockDim.x / QNB_WAVE) + threadIdx.x / QNB_WAVE;
+  const int wib = threadIdx.x / QNB_WAVE;
+  const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / QNB_WAVE) + wib);
   if (a >= S.ncgp) return;
   const int lane = threadIdx.x % QNB_WAVE;
+  __shared__ int s_off[FC_WPB][QNB_WAVE];
+  __shared__ int s_grp[FC_WPB][QNB_WAVE];
+  __shared__ int s_ia[FC_WPB][FC_CH];
+  __shared__ R s_o[FC_WPB][FC_CH][3];
   const double* __restrict__ x = S.x;
   double* __restrict__ d = S.d;
   const int nn = min(S.nnbr[a], S.cap);
   const int* __restrict__ row = S.nbr + (size_t)a * S.cap;
+  const bool want_e = want_energies(S);
+  const int nsol = S.nat_solute, sa = S.sa, ncs = S.ncgp_solute;
+  const bool a_sol = a < ncs;
+  const int ra = group_ref_atom(S, a);
+  const bool aexcl = S.excl[ra];
   double e[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) e[k] = 0.0;
   double eq[4 * NS];  // per state: qp_el, qp_vdw, qw_el, qw_vdw
 #pragma unroll
   for (int k = 0; k < 4 * NS; k++) eq[k] = 0.0;
-  const int ra = group_ref_atom(S, a);
-  const bool aexcl = S.excl[ra];
-  const bool want_e = want_energies(S);
+  const double xr0 = x[3 * ra], xr1 = x[3 * ra + 1], xr2 = x[3 * ra + 2];
+  const int g0 = S.gfirst[a], na = S.glast[a] - g0 + 1;
+  const int nq = S.nqat, ns = S.nstates;
 
-  if (a >= S.ncgp_solute) {
-    // ------------------------------------------------ water receiver
-    const int aw = a - S.ncgp_solute;
-    const int o = ra, sa = S.sa;
-    double f[3][3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) f[k][0] = f[k][1] = f[k][2] = 0.0;
-    double xi[3][3];
+  if (!a_sol) {
+    // ------------------------------------------------ water receiver (3 atoms)
+    R o[3][3], f[3][3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      xi[k][0] = x[3 * (o + k)];
-      xi[k][1] = x[3 * (o + k) + 1];
-      xi[k][2] = x[3 * (o + k) + 2];
+      o[k][0] = (R)(x[3 * (ra + k)] - xr0);
+      o[k][1] = (R)(x[3 * (ra + k) + 1] - xr1);
+      o[k][2] = (R)(x[3 * (ra + k) + 2] - xr2);
+      f[k][0] = f[k][1] = f[k][2] = R(0);
     }
-    if (sa == 3 && S.spc) {
-      // FP32 pair path: partner atoms relative to the receiver's oxygen are
-      // formed in FP64 and rounded once; a molecule pair's 9 atom pairs are
-      // summed in FP32 and added to the FP64 accumulators once per partner
-      float ro[3][3], wel[9];
-      if constexpr (sizeof(R) == 4) {
+    if (!aexcl) {
+      for (int p0 = 0; p0 < nn; p0 += QNB_WAVE) {
+        const int total = expand_groups(S, row, p0, nn, mask & T_PW, mask & T_WW, s_off[wib], s_grp[wib]);
+        for (int t = lane; t < total; t += QNB_WAVE) {
+          const int sl = owner_slot(s_off[wib], t);
+          const int bg = s_grp[wib][sl], kb = t - s_off[wib][sl];
+          const int j = S.cgpatom[S.gfirst[bg] + kb];
+          const R dx = (R)(x[3 * j] - xr0), dy = (R)(x[3 * j + 1] - xr1), dz = (R)(x[3 * j + 2] - xr2);
+          if (bg >= ncs) {
+            // water-water: receiver atom k, partner atom kb
+            const bool acc = want_e && owns(a, bg);
 #pragma unroll
-        for (int la = 0; la < 3; la++)
-#pragma unroll
-          for (int c = 0; c < 3; c++) ro[la][c] = (float)(xi[la][c] - xi[0][c]);
-#pragma unroll
-        for (int k = 0; k < 9; k++) wel[k] = (float)S.wwEl[k];
-      }
-      const float wA = (float)S.wwA[0], wB = (float)S.wwB[0];
-      for (int p = lane; p < nn; p += QNB_WAVE) {
-        const int b = row[p];
-        if (b >= S.ncgp_solute) {
-          if (!(mask & T_WW)) continue;
-          const int jo = S.nat_solute + 3 * (b - S.ncgp_solute);
-          const bool own = owns(a, b);
-          if constexpr (sizeof(R) == 4) {
-            float pj[3][3];
-#pragma unroll
-            for (int ka = 0; ka < 3; ka++)
-#pragma unroll
-              for (int c = 0; c < 3; c++) pj[ka][c] = (float)(x[3 * (jo + ka) + c] - xi[0][c]);
-            float fl[3][3];
-            double el = 0.0, vdw = 0.0;
-            const bool acc_e = want_e && own;
-#pragma unroll
-            for (int la = 0; la < 3; la++) fl[la][0] = fl[la][1] = fl[la][2] = 0.f;
-#pragma unroll
-            for (int la = 0; la < 3; la++)
-#pragma unroll
-              for (int ka = 0; ka < 3; ka++) {
-                const float ux = pj[ka][0] - ro[la][0], uy = pj[ka][1] - ro[la][1], uz = pj[ka][2] - ro[la][2];
-                float r2, r;
-                rinv(ux * ux + uy * uy + uz * uz, r2, r);
-                const float Vel = wel[la + 3 * ka] * r;
-                float dv;
-                if (acc_e) el += (double)Vel;
-                if (la == 0 && ka == 0) {
-                  const float r6 = r2 * r2 * r2, Va = wA * r6 * r6, Vb = wB * r6;
-                  if (acc_e) vdw += (double)Va - (double)Vb;
-                  dv = r2 * (-Vel - 12.f * Va + 6.f * Vb);
-                } else {
-                  dv = -r2 * Vel;
-                }
-                fl[la][0] -= ux * dv;
-                fl[la][1] -= uy * dv;
-                fl[la][2] -= uz * dv;
-              }
-#pragma unroll
-            for (int la = 0; la < 3; la++) {
-              f[la][0] += (double)fl[la][0];
-              f[la][1] += (double)fl[la][1];
-              f[la][2] += (double)fl[la][2];
-            }
-            if (acc_e) {
-              e[E_WW_EL] += el;
-              e[E_WW_VDW] += vdw;
+            for (int k = 0; k < 3; k++) {
+              const int tt = k + 3 * kb;
+              const R ux = dx - o[k][0], uy = dy - o[k][1], uz = dz - o[k][2];
+              R dv;
+              if (!S.spc || (k == 0 && kb == 0))
+                dv = PairMath<R>::nbe(ux, uy, uz, (R)S.wwA[tt], (R)S.wwB[tt], (R)S.wwEl[tt], e[E_WW_EL],
+                                      e[E_WW_VDW], acc);
+              else
+                dv = PairMath<R>::nbe_spc(ux, uy, uz, (R)S.wwEl[tt], e[E_WW_EL], acc);
+              f[k][0] += ux * dv;
+              f[k][1] += uy * dv;
+              f[k][2] += uz * dv;
             }
           } else {
-            double el = 0.0, vdw = 0.0;
-#pragma unroll
-            for (int la = 0; la < 3; la++)
-#pragma unroll
-              for (int ka = 0; ka < 3; ka++) {
-                const int j = jo + ka;
-                const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
-                const int k = la + 3 * ka;
-                R dv;
-                if (la == 0 && ka == 0)
-                  dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.wwA[0], (R)S.wwB[0], (R)S.wwEl[0], el, vdw,
-                                        want_e && own);
-                else
-                  dv = PairMath<R>::nbe_spc((R)ux, (R)uy, (R)uz, (R)S.wwEl[k], el, want_e && own);
-                f[la][0] -= (double)((R)ux * dv);
-                f[la][1] -= (double)((R)uy * dv);
-                f[la][2] -= (double)((R)uz * dv);
-              }
-            if (own) {
-              e[E_WW_EL] += el;
-              e[E_WW_VDW] += vdw;
-            }
-          }
-        } else {
-          if (!(mask & T_PW)) continue;
-          for (int k = S.gfirst[b]; k <= S.glast[b]; k++) {
-            const int j = S.cgpatom[k];
+            // solute partner j: pw table, energy counted on the solute side
             if (S.iqatom[j] != 0) continue;
-            double el = 0.0, vdw = 0.0;
 #pragma unroll
-            for (int la = 0; la < 3; la++) {
-              const size_t t = (size_t)j + (size_t)S.nat_solute * la;
-              const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
-              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], el, vdw, false);
-              f[la][0] -= (double)((R)ux * dv);
-              f[la][1] -= (double)((R)uy * dv);
-              f[la][2] -= (double)((R)uz * dv);
+            for (int k = 0; k < 3; k++) {
+              const size_t tt = (size_t)j + (size_t)nsol * k;
+              const R ux = dx - o[k][0], uy = dy - o[k][1], uz = dz - o[k][2];
+              double ee = 0.0, ev = 0.0;
+              const R dv = PairMath<R>::nbe(ux, uy, uz, (R)S.pwA[tt], (R)S.pwB[tt], (R)S.pwEl[tt], ee, ev, false);
+              f[k][0] += ux * dv;
+              f[k][1] += uy * dv;
+              f[k][2] += uz * dv;
             }
           }
         }
-      }
-    } else {
-      // general solvent: every atom pair with its table entry
-      for (int p = lane; p < nn; p += QNB_WAVE) {
-        const int b = row[p];
-        if (b >= S.ncgp_solute) {
-          if (!(mask & T_WW)) continue;
-          const int jo = S.nat_solute + sa * (b - S.ncgp_solute);
-          const bool own = owns(a, b);
-          double el = 0.0, vdw = 0.0;
-          for (int la = 0; la < sa && la < 3; la++)
-            for (int ka = 0; ka < sa; ka++) {
-              const int j = jo + ka, k = la + sa * ka;
-              const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
-              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.wwA[k], (R)S.wwB[k], (R)S.wwEl[k], el, vdw,
-                                      want_e && own);
-              f[la][0] -= (double)((R)ux * dv);
-              f[la][1] -= (double)((R)uy * dv);
-              f[la][2] -= (double)((R)uz * dv);
-            }
-          if (own) {
-            e[E_WW_EL] += el;
-            e[E_WW_VDW] += vdw;
-          }
-        } else {
-          if (!(mask & T_PW)) continue;
-          for (int k = S.gfirst[b]; k <= S.glast[b]; k++) {
-            const int j = S.cgpatom[k];
-            if (S.iqatom[j] != 0) continue;
-            double el = 0.0, vdw = 0.0;
-            for (int la = 0; la < sa && la < 3; la++) {
-              const size_t t = (size_t)j + (size_t)S.nat_solute * la;
-              const double ux = x[3 * j] - xi[la][0], uy = x[3 * j + 1] - xi[la][1], uz = x[3 * j + 2] - xi[la][2];
-              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], el, vdw, false);
-              f[la][0] -= (double)((R)ux * dv);
-              f[la][1] -= (double)((R)uy * dv);
-              f[la][2] -= (double)((R)uz * dv);
-            }
-          }
-        }
+        __builtin_amdgcn_wave_barrier();
       }
     }
-    // Q-atom - water (nonbond_qw_spc nonbondene.f90:5658-5767; nonbond_qw 5339-5418)
-    if ((mask & T_QW) && S.nqat > 0 && S.qw_in[aw]) {
-      const int nq = S.nqat, ns = S.nstates;
+    // Q-atom - water, receiver side (nonbond_qw_spc nonbondene.f90:5658-5767;
+    // nonbond_qw 5339-5418)
+    if ((mask & T_QW) && nq > 0 && S.qw_in[a - ncs]) {
       for (int iq = lane; iq < nq; iq += QNB_WAVE) {
-        const int i = S.iqseq[iq];
-        for (int k = 0; k < sa && k < 3; k++) {
-          const double ux = xi[k][0] - x[3 * i], uy = xi[k][1] - x[3 * i + 1], uz = xi[k][2] - x[3 * i + 2];
-          const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
+        const int q = S.iqseq[iq];
+        const R qx = (R)(x[3 * q] - xr0), qy = (R)(x[3 * q + 1] - xr1), qz = (R)(x[3 * q + 2] - xr2);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const R ux = o[k][0] - qx, uy = o[k][1] - qy, uz = o[k][2] - qz;
           R r2, r;
-          rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
+          rinv(ux * ux + uy * uy + uz * uz, r2, r);
           R dvs = R(0);
           if (k == 0 || !S.spc) {
-            R r6 = r2 * r2 * r2;
+            const R r6 = r2 * r2 * r2;
 #pragma unroll
             for (int s = 0; s < NS; s++) {
               if (s >= ns) break;
@@ -512,137 +465,159 @@ __global__ __launch_bounds__(512) void k_forces(const DevSys* __restrict__ syste
             for (int s = 0; s < NS; s++) {
               if (s >= ns) break;
               const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
-              R Vel = (R)S.qwEl[t] * r;  // nbe_qspc (nonbonded.f90:226-243)
+              const R Vel = (R)S.qwEl[t] * r;  // nbe_qspc (nonbonded.f90:226-243)
               if (want_e) eq[4 * s + 2] += (double)Vel;
               dvs += -r2 * Vel * (R)S.lambda[s];
             }
           }
-          f[k][0] += (double)(rux * dvs);
-          f[k][1] += (double)(ruy * dvs);
-          f[k][2] += (double)(ruz * dvs);
+          f[k][0] += ux * dvs;
+          f[k][1] += uy * dvs;
+          f[k][2] += uz * dvs;
         }
       }
     }
+    // reduce; LRF Taylor term (lrf_taylor) and SCAAS radial restraint on O
+    // (restrain_solvent) on the atom's own lane
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      f[k][0] = wave_sum(f[k][0]);
-      f[k][1] = wave_sum(f[k][1]);
-      f[k][2] = wave_sum(f[k][2]);
-    }
-    if (lane < 3 && lane < sa) {
-      const int i = o + lane;
-      double fi[3] = {f[lane][0], f[lane][1], f[lane][2]};
-      if ((mask & T_LRF) && S.use_lrf && !S.excl[i] && S.iqatom[i] == 0) lrf_taylor_atom(S, i, fi, e[E_LRF]);
-      if ((mask & T_RAD) && lane == 0 && !aexcl && S.sa == 3) {
-        const double T = tfree[blockIdx.y];
-        const double shift = S.fk_wsphere != 0.0 ? sqrt(S.Boltz * T / S.fk_wsphere) : 0.0;
-        radial_restraint(S, i, shift, fi, e[E_RAD]);
-      }
-      d[3 * i] = fi[0];
-      d[3 * i + 1] = fi[1];
-      d[3 * i + 2] = fi[2];
-    }
-  } else {
-    // ------------------------------------------------ solute receiver
-    const bool qin = (mask & T_QP) && S.nqat > 0 && S.qp_in[a];
-    for (int ka = S.gfirst[a]; ka <= S.glast[a]; ka++) {
-      const int i = S.cgpatom[ka];
-      double fi[3] = {0.0, 0.0, 0.0};
-      const bool iq_atom = S.iqatom[i] != 0;
-      if (!iq_atom && !aexcl) {
-        const double xix = x[3 * i], xiy = x[3 * i + 1], xiz = x[3 * i + 2];
-        for (int p = lane; p < nn; p += QNB_WAVE) {
-          const int b = row[p];
-          if (b < S.ncgp_solute) {
-            if (!(mask & T_PP)) continue;
-            const bool own = owns(a, b);
-            for (int kb = S.gfirst[b]; kb <= S.glast[b]; kb++) {
-              const int j = S.cgpatom[kb];
-              if (S.iqatom[j] != 0) continue;
-              double A, B, el;
-              if (!pp_param(S, i, j, A, B, el)) continue;
-              const double ux = x[3 * j] - xix, uy = x[3 * j + 1] - xiy, uz = x[3 * j + 2] - xiz;
-              double ee = 0.0, ev = 0.0;
-              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)A, (R)B, (R)el, ee, ev, want_e && own);
-              fi[0] -= (double)((R)ux * dv);
-              fi[1] -= (double)((R)uy * dv);
-              fi[2] -= (double)((R)uz * dv);
-              if (own) {
-                e[E_PP_EL] += ee;
-                e[E_PP_VDW] += ev;
-              }
-            }
-          } else {
-            if (!(mask & T_PW)) continue;
-            const int jo = S.nat_solute + S.sa * (b - S.ncgp_solute);
-            for (int k = 0; k < S.sa; k++) {
-              const int j = jo + k;
-              const size_t t = (size_t)i + (size_t)S.nat_solute * k;
-              const double ux = x[3 * j] - xix, uy = x[3 * j + 1] - xiy, uz = x[3 * j + 2] - xiz;
-              R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)S.pwA[t], (R)S.pwB[t], (R)S.pwEl[t], e[E_PW_EL],
-                                      e[E_PW_VDW], want_e);
-              fi[0] -= (double)((R)ux * dv);
-              fi[1] -= (double)((R)uy * dv);
-              fi[2] -= (double)((R)uz * dv);
-            }
-          }
+      const double fx = wave_sum((double)f[k][0]), fy = wave_sum((double)f[k][1]), fz = wave_sum((double)f[k][2]);
+      if (lane == k) {
+        const int i = ra + k;
+        double fi[3] = {-fx, -fy, -fz};
+        if ((mask & T_LRF) && S.use_lrf && !S.excl[i] && S.iqatom[i] == 0) lrf_taylor_atom(S, i, fi, e[E_LRF]);
+        if (k == 0 && (mask & T_RAD) && !aexcl) {
+          const double T = tfree[blockIdx.y];
+          const double shift = S.fk_wsphere != 0.0 ? sqrt(S.Boltz * T / S.fk_wsphere) : 0.0;
+          radial_restraint(S, i, shift, fi, e[E_RAD]);
         }
-        // pairs inside the group itself (ig == jgr, i < j, nonbondene.f90:1610)
-        if (mask & T_PP) {
-          const int n_in = S.glast[a] - S.gfirst[a] + 1;
-          for (int p = lane; p < n_in; p += QNB_WAVE) {
-            const int j = S.cgpatom[S.gfirst[a] + p];
-            if (j == i || S.iqatom[j] != 0) continue;
-            double A, B, el;
-            if (!pp_param(S, i, j, A, B, el)) continue;
-            const double ux = x[3 * j] - xix, uy = x[3 * j + 1] - xiy, uz = x[3 * j + 2] - xiz;
-            double ee = 0.0, ev = 0.0;
-            R dv = PairMath<R>::nbe((R)ux, (R)uy, (R)uz, (R)A, (R)B, (R)el, ee, ev, want_e && i < j);
-            fi[0] -= (double)((R)ux * dv);
-            fi[1] -= (double)((R)uy * dv);
-            fi[2] -= (double)((R)uz * dv);
-            if (i < j) {
-              e[E_PP_EL] += ee;
-              e[E_PP_VDW] += ev;
-            }
-          }
-        }
-      }
-      // Q-atom - solute atom (nonbond_qp, nonbondene.f90:5157-5230)
-      if (qin && !iq_atom && !S.qconn3[i]) {
-        const int nq = S.nqat, ns = S.nstates, nsol = S.nat_solute;
-        const double xix = x[3 * i], xiy = x[3 * i + 1], xiz = x[3 * i + 2];
-        for (int iq = lane; iq < nq; iq += QNB_WAVE) {
-          const int q = S.iqseq[iq];
-          const double ux = xix - x[3 * q], uy = xiy - x[3 * q + 1], uz = xiz - x[3 * q + 2];
-          const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
-          R r2, r;
-          rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
-          R r6 = r2 * r2 * r2;
-#pragma unroll
-          for (int s = 0; s < NS; s++) {
-            if (s >= ns) break;
-            const size_t t = (size_t)i + (size_t)nsol * ((size_t)iq + (size_t)nq * s);
-            R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)S.qpA[t], (R)S.qpB[t], (R)S.qpEl[t], (R)S.qpSc[t],
-                                       (R)S.lambda[s], eq[4 * s + 0], eq[4 * s + 1], want_e);
-            fi[0] += (double)(rux * dv);
-            fi[1] += (double)(ruy * dv);
-            fi[2] += (double)(ruz * dv);
-          }
-        }
-      }
-      fi[0] = wave_sum(fi[0]);
-      fi[1] = wave_sum(fi[1]);
-      fi[2] = wave_sum(fi[2]);
-      if (lane == 0) {
-        if ((mask & T_LRF) && S.use_lrf && !S.excl[i] && !iq_atom) lrf_taylor_atom(S, i, fi, e[E_LRF]);
         d[3 * i] = fi[0];
         d[3 * i + 1] = fi[1];
         d[3 * i + 2] = fi[2];
       }
     }
+  } else {
+    // ------------------------------------------------ solute receiver
+    const bool qin = (mask & T_QP) && nq > 0 && S.qp_in[a];
+    constexpr int CH = sizeof(R) == 4 ? FC_CH : FC_CH / 2;
+    for (int c0 = 0; c0 < na; c0 += CH) {
+      const int nk = min(CH, na - c0);
+      // receiver atoms of this sweep in LDS: index (-1: takes no pair terms)
+      // and offset from the reference atom
+      if (lane < nk) {
+        const int i = S.cgpatom[g0 + c0 + lane];
+        s_ia[wib][lane] = (!aexcl && S.iqatom[i] == 0) ? i : -1;
+        s_o[wib][lane][0] = (R)(x[3 * i] - xr0);
+        s_o[wib][lane][1] = (R)(x[3 * i + 1] - xr1);
+        s_o[wib][lane][2] = (R)(x[3 * i + 2] - xr2);
+      }
+      __builtin_amdgcn_wave_barrier();
+      R f[CH][3];
+#pragma unroll
+      for (int k = 0; k < CH; k++) f[k][0] = f[k][1] = f[k][2] = R(0);
+      if (!aexcl) {
+        for (int p0 = 0; p0 < nn; p0 += QNB_WAVE) {
+          const int total = expand_groups(S, row, p0, nn, mask & T_PP, mask & T_PW, s_off[wib], s_grp[wib]);
+          for (int t = lane; t < total; t += QNB_WAVE) {
+            const int sl = owner_slot(s_off[wib], t);
+            const int bg = s_grp[wib][sl], kb = t - s_off[wib][sl];
+            const int j = S.cgpatom[S.gfirst[bg] + kb];
+            const bool b_sol = bg < ncs;
+            if (b_sol && S.iqatom[j] != 0) continue;
+            const bool acc = want_e && owns(a, bg);
+            const R dx = (R)(x[3 * j] - xr0), dy = (R)(x[3 * j + 1] - xr1), dz = (R)(x[3 * j + 2] - xr2);
+#pragma unroll
+            for (int k = 0; k < CH; k++) {
+              if (k >= nk) break;
+              const int i = s_ia[wib][k];
+              if (i < 0) continue;
+              const R ux = dx - s_o[wib][k][0], uy = dy - s_o[wib][k][1], uz = dz - s_o[wib][k][2];
+              R dv;
+              if (b_sol) {
+                double A, B, el;
+                if (!pp_param(S, i, j, A, B, el)) continue;
+                dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], acc);
+              } else {
+                const size_t tt = (size_t)i + (size_t)nsol * kb;
+                dv = PairMath<R>::nbe(ux, uy, uz, (R)S.pwA[tt], (R)S.pwB[tt], (R)S.pwEl[tt], e[E_PW_EL],
+                                      e[E_PW_VDW], want_e);
+              }
+              f[k][0] += ux * dv;
+              f[k][1] += uy * dv;
+              f[k][2] += uz * dv;
+            }
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+        // pairs inside the group (ig == jgr, i < j, nonbondene.f90:1610)
+        if (mask & T_PP) {
+          for (int q = lane; q < na; q += QNB_WAVE) {
+            const int j = S.cgpatom[g0 + q];
+            if (S.iqatom[j] != 0) continue;
+            const R dx = (R)(x[3 * j] - xr0), dy = (R)(x[3 * j + 1] - xr1), dz = (R)(x[3 * j + 2] - xr2);
+#pragma unroll
+            for (int k = 0; k < CH; k++) {
+              if (k >= nk) break;
+              const int i = s_ia[wib][k];
+              if (i < 0 || i == j) continue;
+              double A, B, el;
+              if (!pp_param(S, i, j, A, B, el)) continue;
+              const R ux = dx - s_o[wib][k][0], uy = dy - s_o[wib][k][1], uz = dz - s_o[wib][k][2];
+              const R dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], want_e && i < j);
+              f[k][0] += ux * dv;
+              f[k][1] += uy * dv;
+              f[k][2] += uz * dv;
+            }
+          }
+        }
+      }
+      // Q-atom - solute atom, receiver side (nonbond_qp, nonbondene.f90:5157-5230)
+      if (qin) {
+        for (int iq = lane; iq < nq; iq += QNB_WAVE) {
+          const int q = S.iqseq[iq];
+          const R qx = (R)(x[3 * q] - xr0), qy = (R)(x[3 * q + 1] - xr1), qz = (R)(x[3 * q + 2] - xr2);
+#pragma unroll
+          for (int k = 0; k < CH; k++) {
+            if (k >= nk) break;
+            const int i = s_ia[wib][k];
+            if (i < 0 || S.qconn3[i]) continue;
+            const R ux = s_o[wib][k][0] - qx, uy = s_o[wib][k][1] - qy, uz = s_o[wib][k][2] - qz;
+            R r2, r;
+            rinv(ux * ux + uy * uy + uz * uz, r2, r);
+            const R r6 = r2 * r2 * r2;
+            R dvs = R(0);
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+              if (s >= ns) break;
+              const size_t t = (size_t)i + (size_t)nsol * ((size_t)iq + (size_t)nq * s);
+              dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)S.qpA[t], (R)S.qpB[t], (R)S.qpEl[t], (R)S.qpSc[t],
+                                         (R)S.lambda[s], eq[4 * s + 0], eq[4 * s + 1], want_e);
+            }
+            f[k][0] += ux * dvs;
+            f[k][1] += uy * dvs;
+            f[k][2] += uz * dvs;
+          }
+        }
+      }
+      // reduce; LRF Taylor term on the atom's own lane
+#pragma unroll
+      for (int k = 0; k < CH; k++) {
+        if (k >= nk) break;
+        const double fx = wave_sum((double)f[k][0]), fy = wave_sum((double)f[k][1]), fz = wave_sum((double)f[k][2]);
+        if (lane == k) {
+          const int i = S.cgpatom[g0 + c0 + k];
+          double fi[3] = {-fx, -fy, -fz};
+          if ((mask & T_LRF) && S.use_lrf && !S.excl[i] && S.iqatom[i] == 0) lrf_taylor_atom(S, i, fi, e[E_LRF]);
+          d[3 * i] = fi[0];
+          d[3 * i + 1] = fi[1];
+          d[3 * i + 2] = fi[2];
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
   }
-  // energy partials of this wave (lane 0 holds the LRF/radial terms)
+
+  // energy partials of this wave, on the sampled steps
+  if (!want_e) return;
   double* ep = S.epart + (size_t)a * QNB_NE;
 #pragma unroll
   for (int k = 0; k < 9; k++) {
@@ -986,15 +961,30 @@ __global__ __launch_bounds__(1024) void k_watpol(const DevSys* __restrict__ syst
 // Fixed-order reduction of the energy partials: slots [0, ncgp) from
 // k_forces, plus the qq slot (ncgp) and the watpol slot (ncgp+1) when those
 // kernels ran.
-__global__ void k_reduce_energies(const DevSys* __restrict__ systems, int with_qq, int with_pol) {
+// Fixed-order reduction of the per-wave energy partials: wave c of the block
+// sums slots [c*n/16, (c+1)*n/16) with lane k on energy k (a wave reads one
+// 57-double slot per iteration), then thread k adds the 16 chunk sums in
+// order.  Runs on the sampled steps only.
+#define QNB_RED_CHUNKS 16
+__global__ __launch_bounds__(64 * QNB_RED_CHUNKS) void k_reduce_energies(const DevSys* __restrict__ systems,
+                                                                        int with_qq, int with_pol) {
   const DevSys& S = systems[blockIdx.x];
-  const int k = threadIdx.x;
-  if (k >= QNB_NE) return;
+  if (!want_energies(S)) return;
+  __shared__ double part[QNB_RED_CHUNKS][QNB_WAVE];
+  const int k = threadIdx.x % QNB_WAVE, c = threadIdx.x / QNB_WAVE;
+  const int n = S.ncgp, a0 = (int)((long)n * c / QNB_RED_CHUNKS), a1 = (int)((long)n * (c + 1) / QNB_RED_CHUNKS);
   double s = 0.0;
-  for (int a = 0; a < S.ncgp; a++) s += S.epart[(size_t)a * QNB_NE + k];
-  if (with_qq && S.nqat > 0) s += S.epart[(size_t)S.ncgp * QNB_NE + k];
-  if (with_pol && S.nwat > 0 && S.wpol && S.nshell > 0) s += S.epart[(size_t)(S.ncgp + 1) * QNB_NE + k];
-  S.ered[k] = s;
+  if (k < QNB_NE)
+    for (int a = a0; a < a1; a++) s += S.epart[(size_t)a * QNB_NE + k];
+  part[c][k] = s;
+  __syncthreads();
+  if (c == 0 && k < QNB_NE) {
+    double t = 0.0;
+    for (int m = 0; m < QNB_RED_CHUNKS; m++) t += part[m][k];
+    if (with_qq && S.nqat > 0) t += S.epart[(size_t)S.ncgp * QNB_NE + k];
+    if (with_pol && S.nwat > 0 && S.wpol && S.nshell > 0) t += S.epart[(size_t)(S.ncgp + 1) * QNB_NE + k];
+    S.ered[k] = t;
+  }
 }
 
 // explicit instantiations used by the host code
