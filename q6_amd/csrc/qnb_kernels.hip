@@ -447,7 +447,8 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
         const R qx = (R)(x[3 * q] - xr0), qy = (R)(x[3 * q + 1] - xr1), qz = (R)(x[3 * q + 2] - xr2);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-          const R ux = o[k][0] - qx, uy = o[k][1] - qy, uz = o[k][2] - qz;
+          // u = x_q - x_i, the partner-minus-receiver convention of the pair terms
+          const R ux = qx - o[k][0], uy = qy - o[k][1], uz = qz - o[k][2];
           R r2, r;
           rinv(ux * ux + uy * uy + uz * uz, r2, r);
           R dvs = R(0);
@@ -580,7 +581,7 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
             if (k >= nk) break;
             const int i = s_ia[wib][k];
             if (i < 0 || S.qconn3[i]) continue;
-            const R ux = s_o[wib][k][0] - qx, uy = s_o[wib][k][1] - qy, uz = s_o[wib][k][2] - qz;
+            const R ux = qx - s_o[wib][k][0], uy = qy - s_o[wib][k][1], uz = qz - s_o[wib][k][2];
             R r2, r;
             rinv(ux * ux + uy * uy + uz * uz, r2, r);
             const R r6 = r2 * r2 * r2;
