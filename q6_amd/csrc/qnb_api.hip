@@ -55,6 +55,8 @@ int validate_system(const qnb_system* s, const qnb_params* p) {
     if (s->iac[i] < 0 || s->iac[i] >= s->ntypes) return fail(QNB_EINVAL, "atom %d has type %d out of range", i, s->iac[i]);
   for (int g = 0; g < s->ncgp; g++) {
     if (s->cgp_first[g] > s->cgp_last[g]) return fail(QNB_EINVAL, "empty charge group %d", g);
+    if (s->cgp_last[g] - s->cgp_first[g] + 1 > FC_CAP)
+      return fail(QNB_EUNSUPPORTED, "charge group %d has more than %d atoms", g, FC_CAP);
     if (g >= s->ncgp_solute) {
       int w = g - s->ncgp_solute, o = s->nat_solute + s->solv_atom * w;
       if (s->cgp_last[g] - s->cgp_first[g] + 1 != s->solv_atom || s->cgp_iswitch[g] != o)
@@ -452,8 +454,8 @@ template <typename R, int NS>
 static void launch_eval(qnb_handle* h, uint32_t mask) {
   const DevSys& S = h->hs;
   const int wpb = 4;
-  hipLaunchKernelGGL((k_forces<R, NS>), dim3((S.ncgp + wpb - 1) / wpb, 1), dim3(64 * wpb), 0, h->stream, h->dsys,
-                     mask, (const double*)h->d_tfree);
+  hipLaunchKernelGGL((k_forces<R, NS>), dim3((S.ncgp + wpb - 1) / wpb, 1), dim3(64 * wpb), fc_wave_bytes<R>() * wpb,
+                     h->stream, h->dsys, mask, (const double*)h->d_tfree);
   if (S.nqat > 0 && (mask & (QNB_T_QW | QNB_T_QP))) {
     // partners per Q-atom: at most nwat + nat_solute; chunks of 64 per wave
     int maxp = S.nwat + S.nat_solute;

@@ -320,10 +320,18 @@ static void launch_lists(qnb_batch* b, int wpb) {
 
 static const uint32_t kStepMask = QNB_T_NONBONDED | QNB_T_QQ | QNB_T_QQP | QNB_T_RAD | QNB_T_POL;
 
+// waves per block of the force kernel, kept within 64 KiB of LDS per block
+template <typename R>
+static int forces_wpb(int wpb) {
+  const int cap = (int)(65536 / fc_wave_bytes<R>());
+  return std::max(1, std::min(std::min(wpb, FC_WPB), cap));
+}
+
 template <typename R, int NS>
 static void launch_forces_t(qnb_batch* b, int wpb) {
-  hipLaunchKernelGGL((k_forces<R, NS>), dim3((b->maxg + wpb - 1) / wpb, b->n), dim3(64 * wpb), 0, b->stream, b->dsys,
-                     kStepMask, (const double*)b->tfree);
+  wpb = forces_wpb<R>(wpb);
+  hipLaunchKernelGGL((k_forces<R, NS>), dim3((b->maxg + wpb - 1) / wpb, b->n), dim3(64 * wpb),
+                     fc_wave_bytes<R>() * wpb, b->stream, b->dsys, kStepMask, (const double*)b->tfree);
 }
 
 static void launch_forces(qnb_batch* b, int wpb) {

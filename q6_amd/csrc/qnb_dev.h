@@ -250,6 +250,18 @@ struct PairMath {
   }
 };
 
+// Receiver-side force kernel: at most FC_WPB wavefronts per block, FC_CAP
+// staged partner atoms and FC_CH receiver atoms per sweep; per-wave LDS:
+// partner offsets and receiver offsets (R x 3), partner index and tag,
+// receiver index
+#define FC_WPB 8
+#define FC_CAP 384
+#define FC_CH 10
+template <typename R>
+__host__ __device__ constexpr size_t fc_wave_bytes() {
+  return ((sizeof(R) * 3 * (FC_CAP + FC_CH) + sizeof(int) * (2 * FC_CAP + FC_CH)) + 15) / 16 * 16;
+}
+
 // Adjoint of the per-state Q-atom energies over a set of frames: tangent
 // tables (derivative of each Q table entry with respect to the Q-side
 // parameter of that entry) and the per-frame output.

@@ -315,15 +315,22 @@ enum : uint32_t { T_WW = 1, T_QW = 2, T_PW = 4, T_PP = 8, T_QP = 16, T_LRF = 32,
 // Coulomb only for SPC), pw from the pw table (nonbond_pw 4856-4916), pp from
 // pp_param (nonbond_pp 4694-4760).  Writes (not adds) d for every atom of a,
 // and the wave's energy partials into epart[a] on the sampled steps.
-#define FC_WPB 8    // waves per block at most (launch bound 512)
-// solute receiver atoms per sweep over the partners (per-lane accumulators
-// for each: 3 values of R)
-#define FC_CH 10
 
-// partner-atom expansion of 64 neighbour groups: returns the chunk's atom
-// count; s_off/s_grp hold each lane's group and its first atom's rank
-__device__ __forceinline__ int expand_groups(const DevSys& S, const int* __restrict__ row, int p0, int nn,
-                                            uint32_t use_sol, uint32_t use_wat, int* s_off, int* s_grp) {
+// Partner staging: the wave takes up to 64 neighbour groups whose atoms fit
+// in FC_CAP slots (wave prefix sum over the groups' atom counts; groups
+// larger than FC_CAP are refused at set-up) and writes, per partner atom,
+// its index, its position relative to the receiver's reference atom (formed
+// in FP64, rounded to R) and a tag: bits 0-1 the atom's rank in its group,
+// bit 2 solute partner, bit 3 the receiver owns the group pair's energy,
+// bit 4 a Q-atom (its pairs belong to the Q kernels).  Returns the number of
+// staged atoms; *ngroups receives the number of groups consumed.
+#define FT_SOL 4
+#define FT_OWN 8
+#define FT_Q 16
+template <typename R>
+__device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int* __restrict__ row, int p0, int nn,
+                                              uint32_t use_sol, uint32_t use_wat, double xr0, double xr1,
+                                              double xr2, int* s_j, R (*s_d)[3], int* s_t, int* ngroups) {
   const int lane = threadIdx.x % QNB_WAVE;
   const int p = p0 + lane;
   int b = -1, cnt = 0;
@@ -337,22 +344,26 @@ __device__ __forceinline__ int expand_groups(const DevSys& S, const int* __restr
     const int v = __shfl_up(incl, o2, QNB_WAVE);
     if (lane >= o2) incl += v;
   }
-  s_off[lane] = incl - cnt;
-  s_grp[lane] = b;
-  __builtin_amdgcn_wave_barrier();
-  return __shfl(incl, QNB_WAVE - 1, QNB_WAVE);
-}
-
-// the group holding expanded atom t (largest g with s_off[g] <= t)
-__device__ __forceinline__ int owner_slot(const int* s_off, int t) {
-  int lo = 0, hi = QNB_WAVE - 1;
-#pragma unroll
-  for (int it = 0; it < 6; it++) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (s_off[mid] <= t) lo = mid;
-    else hi = mid - 1;
+  // inclusive sums never decrease, so the groups that fit are a prefix
+  const int ng = __popcll(__ballot(incl <= FC_CAP));
+  *ngroups = ng;
+  if (ng == 0) return 0;
+  const int total = __shfl(incl, ng - 1, QNB_WAVE);
+  if (lane < ng && cnt > 0) {
+    const bool b_sol = b < S.ncgp_solute;
+    const int tag0 = (b_sol ? FT_SOL : 0) | (owns(a, b) ? FT_OWN : 0);
+    const int f0 = S.gfirst[b];
+    for (int m = 0, slot = incl - cnt; m < cnt; m++, slot++) {
+      const int j = S.cgpatom[f0 + m];
+      s_j[slot] = j;
+      s_d[slot][0] = (R)(S.x[3 * j] - xr0);
+      s_d[slot][1] = (R)(S.x[3 * j + 1] - xr1);
+      s_d[slot][2] = (R)(S.x[3 * j + 2] - xr2);
+      s_t[slot] = tag0 | (m & 3) | ((b_sol && S.iqatom[j] != 0) ? FT_Q : 0);
+    }
   }
-  return lo;
+  __builtin_amdgcn_wave_barrier();
+  return total;
 }
 
 template <typename R, int NS>
@@ -363,10 +374,14 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
   const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / QNB_WAVE) + wib);
   if (a >= S.ncgp) return;
   const int lane = threadIdx.x % QNB_WAVE;
-  __shared__ int s_off[FC_WPB][QNB_WAVE];
-  __shared__ int s_grp[FC_WPB][QNB_WAVE];
-  __shared__ int s_ia[FC_WPB][FC_CH];
-  __shared__ R s_o[FC_WPB][FC_CH][3];
+  // per-wave LDS (dynamic, fc_wave_bytes<R>() per wave of the block)
+  extern __shared__ __align__(16) unsigned char fc_smem[];
+  unsigned char* wbase = fc_smem + (size_t)wib * fc_wave_bytes<R>();
+  R(*s_pd)[3] = reinterpret_cast<R(*)[3]>(wbase);
+  R(*s_o)[3] = reinterpret_cast<R(*)[3]>(wbase + sizeof(R) * 3 * FC_CAP);
+  int* s_pj = reinterpret_cast<int*>(wbase + sizeof(R) * 3 * (FC_CAP + FC_CH));
+  int* s_pt = s_pj + FC_CAP;
+  int* s_ia = s_pt + FC_CAP;
   const double* __restrict__ x = S.x;
   double* __restrict__ d = S.d;
   const int nn = min(S.nnbr[a], S.cap);
@@ -397,16 +412,16 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
       f[k][0] = f[k][1] = f[k][2] = R(0);
     }
     if (!aexcl) {
-      for (int p0 = 0; p0 < nn; p0 += QNB_WAVE) {
-        const int total = expand_groups(S, row, p0, nn, mask & T_PW, mask & T_WW, s_off[wib], s_grp[wib]);
+      for (int p0 = 0, ng = 0; p0 < nn; p0 += ng) {
+        const int total = stage_partners<R>(S, a, row, p0, nn, mask & T_PW, mask & T_WW, xr0, xr1, xr2, s_pj,
+                                            s_pd, s_pt, &ng);
+        if (ng == 0) break;
         for (int t = lane; t < total; t += QNB_WAVE) {
-          const int sl = owner_slot(s_off[wib], t);
-          const int bg = s_grp[wib][sl], kb = t - s_off[wib][sl];
-          const int j = S.cgpatom[S.gfirst[bg] + kb];
-          const R dx = (R)(x[3 * j] - xr0), dy = (R)(x[3 * j + 1] - xr1), dz = (R)(x[3 * j + 2] - xr2);
-          if (bg >= ncs) {
+          const int tag = s_pt[t], j = s_pj[t], kb = tag & 3;
+          const R dx = s_pd[t][0], dy = s_pd[t][1], dz = s_pd[t][2];
+          if (!(tag & FT_SOL)) {
             // water-water: receiver atom k, partner atom kb
-            const bool acc = want_e && owns(a, bg);
+            const bool acc = want_e && (tag & FT_OWN);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
               const int tt = k + 3 * kb;
@@ -423,7 +438,7 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
             }
           } else {
             // solute partner j: pw table, energy counted on the solute side
-            if (S.iqatom[j] != 0) continue;
+            if (tag & FT_Q) continue;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
               const size_t tt = (size_t)j + (size_t)nsol * k;
@@ -506,32 +521,32 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
       // and offset from the reference atom
       if (lane < nk) {
         const int i = S.cgpatom[g0 + c0 + lane];
-        s_ia[wib][lane] = (!aexcl && S.iqatom[i] == 0) ? i : -1;
-        s_o[wib][lane][0] = (R)(x[3 * i] - xr0);
-        s_o[wib][lane][1] = (R)(x[3 * i + 1] - xr1);
-        s_o[wib][lane][2] = (R)(x[3 * i + 2] - xr2);
+        s_ia[lane] = (!aexcl && S.iqatom[i] == 0) ? i : -1;
+        s_o[lane][0] = (R)(x[3 * i] - xr0);
+        s_o[lane][1] = (R)(x[3 * i + 1] - xr1);
+        s_o[lane][2] = (R)(x[3 * i + 2] - xr2);
       }
       __builtin_amdgcn_wave_barrier();
       R f[CH][3];
 #pragma unroll
       for (int k = 0; k < CH; k++) f[k][0] = f[k][1] = f[k][2] = R(0);
       if (!aexcl) {
-        for (int p0 = 0; p0 < nn; p0 += QNB_WAVE) {
-          const int total = expand_groups(S, row, p0, nn, mask & T_PP, mask & T_PW, s_off[wib], s_grp[wib]);
+        for (int p0 = 0, ng = 0; p0 < nn; p0 += ng) {
+          const int total = stage_partners<R>(S, a, row, p0, nn, mask & T_PP, mask & T_PW, xr0, xr1, xr2,
+                                              s_pj, s_pd, s_pt, &ng);
+          if (ng == 0) break;
           for (int t = lane; t < total; t += QNB_WAVE) {
-            const int sl = owner_slot(s_off[wib], t);
-            const int bg = s_grp[wib][sl], kb = t - s_off[wib][sl];
-            const int j = S.cgpatom[S.gfirst[bg] + kb];
-            const bool b_sol = bg < ncs;
-            if (b_sol && S.iqatom[j] != 0) continue;
-            const bool acc = want_e && owns(a, bg);
-            const R dx = (R)(x[3 * j] - xr0), dy = (R)(x[3 * j + 1] - xr1), dz = (R)(x[3 * j + 2] - xr2);
+            const int tag = s_pt[t], j = s_pj[t], kb = tag & 3;
+            if (tag & FT_Q) continue;
+            const bool b_sol = (tag & FT_SOL) != 0;
+            const bool acc = want_e && (tag & FT_OWN);
+            const R dx = s_pd[t][0], dy = s_pd[t][1], dz = s_pd[t][2];
 #pragma unroll
             for (int k = 0; k < CH; k++) {
               if (k >= nk) break;
-              const int i = s_ia[wib][k];
+              const int i = s_ia[k];
               if (i < 0) continue;
-              const R ux = dx - s_o[wib][k][0], uy = dy - s_o[wib][k][1], uz = dz - s_o[wib][k][2];
+              const R ux = dx - s_o[k][0], uy = dy - s_o[k][1], uz = dz - s_o[k][2];
               R dv;
               if (b_sol) {
                 double A, B, el;
@@ -558,11 +573,11 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
 #pragma unroll
             for (int k = 0; k < CH; k++) {
               if (k >= nk) break;
-              const int i = s_ia[wib][k];
+              const int i = s_ia[k];
               if (i < 0 || i == j) continue;
               double A, B, el;
               if (!pp_param(S, i, j, A, B, el)) continue;
-              const R ux = dx - s_o[wib][k][0], uy = dy - s_o[wib][k][1], uz = dz - s_o[wib][k][2];
+              const R ux = dx - s_o[k][0], uy = dy - s_o[k][1], uz = dz - s_o[k][2];
               const R dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], want_e && i < j);
               f[k][0] += ux * dv;
               f[k][1] += uy * dv;
@@ -579,9 +594,9 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
 #pragma unroll
           for (int k = 0; k < CH; k++) {
             if (k >= nk) break;
-            const int i = s_ia[wib][k];
+            const int i = s_ia[k];
             if (i < 0 || S.qconn3[i]) continue;
-            const R ux = qx - s_o[wib][k][0], uy = qy - s_o[wib][k][1], uz = qz - s_o[wib][k][2];
+            const R ux = qx - s_o[k][0], uy = qy - s_o[k][1], uz = qz - s_o[k][2];
             R r2, r;
             rinv(ux * ux + uy * uy + uz * uz, r2, r);
             const R r6 = r2 * r2 * r2;
