@@ -16,7 +16,7 @@ namespace qnb {
 template <typename R>
 __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
-template <typename R, int NS>
+template <typename R, int NS, bool WAT>
 __global__ void k_forces(const DevSys*, uint32_t, const double*);
 template <typename R, int NS>
 __global__ void k_qforces(const DevSys*, uint32_t, int);
@@ -24,6 +24,7 @@ template <typename R>
 __global__ void k_qq(const DevSys*, uint32_t);
 __global__ void k_watpol(const DevSys*, const int*);
 __global__ void k_reduce_energies(const DevSys*, int, int);
+__global__ void k_atom_terms(const DevSys*, uint32_t, const double*);
 
 static thread_local std::string g_err;
 
@@ -294,6 +295,7 @@ int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots) {
   S.nslots = nslots;
   AL(epart, (size_t)QNB_NE * nslots);
   AL(ered, QNB_NE);
+  AL(eatom, 2 * (size_t)nat);
   AL(sh_state, 3 * QNB_MAXSH);
   AL(w_theta, nw1);
   AL(w_shell, nw1);
@@ -454,8 +456,14 @@ template <typename R, int NS>
 static void launch_eval(qnb_handle* h, uint32_t mask) {
   const DevSys& S = h->hs;
   const int wpb = 4;
-  hipLaunchKernelGGL((k_forces<R, NS>), dim3((S.ncgp + wpb - 1) / wpb, 1), dim3(64 * wpb), fc_wave_bytes<R>() * wpb,
-                     h->stream, h->dsys, mask, (const double*)h->d_tfree);
+  if (S.nwat > 0)
+    hipLaunchKernelGGL((k_forces<R, NS, true>), dim3((S.nwat + wpb - 1) / wpb, 1), dim3(64 * wpb),
+                       fc_wave_bytes<R>() * wpb, h->stream, h->dsys, mask, (const double*)h->d_tfree);
+  if (S.ncgp_solute > 0)
+    hipLaunchKernelGGL((k_forces<R, NS, false>), dim3((S.ncgp_solute + wpb - 1) / wpb, 1), dim3(64 * wpb),
+                       fc_wave_bytes<R>() * wpb, h->stream, h->dsys, mask, (const double*)h->d_tfree);
+  hipLaunchKernelGGL(k_atom_terms, dim3((S.natom + 255) / 256, 1), dim3(256), 0, h->stream, h->dsys, mask,
+                     (const double*)h->d_tfree);
   if (S.nqat > 0 && (mask & (QNB_T_QW | QNB_T_QP))) {
     // partners per Q-atom: at most nwat + nat_solute; chunks of 64 per wave
     int maxp = S.nwat + S.nat_solute;

@@ -18,7 +18,7 @@ namespace qnb {
 template <typename R>
 __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
-template <typename R, int NS>
+template <typename R, int NS, bool WAT>
 __global__ void k_forces(const DevSys*, uint32_t, const double*);
 template <typename R, int NS>
 __global__ void k_qforces(const DevSys*, uint32_t, int);
@@ -26,6 +26,7 @@ template <typename R>
 __global__ void k_qq(const DevSys*, uint32_t);
 __global__ void k_watpol(const DevSys*, const int*);
 __global__ void k_reduce_energies(const DevSys*, int, int);
+__global__ void k_atom_terms(const DevSys*, uint32_t, const double*);
 
 // per-instance MD record
 struct MdDev {
@@ -298,7 +299,7 @@ struct qnb_batch {
   hipStream_t stream = nullptr;
   hipGraphExec_t cycle_exec = nullptr;
   hipGraph_t cycle_graph = nullptr;
-  int maxg = 0, maxat = 0, maxsw = 0, ns_max = 1, prec = 0;
+  int maxg = 0, maxat = 0, maxsw = 0, maxw = 0, maxs = 0, ns_max = 1, prec = 0;
   bool anyq = false, anypol = false, static_q = true;
   int qchunks = 1, qwaves = 0;
   int wpb_lists = 4, wpb_forces = 4;  // waves per block, picked by qnb_batch_autotune
@@ -330,8 +331,14 @@ static int forces_wpb(int wpb) {
 template <typename R, int NS>
 static void launch_forces_t(qnb_batch* b, int wpb) {
   wpb = forces_wpb<R>(wpb);
-  hipLaunchKernelGGL((k_forces<R, NS>), dim3((b->maxg + wpb - 1) / wpb, b->n), dim3(64 * wpb),
-                     fc_wave_bytes<R>() * wpb, b->stream, b->dsys, kStepMask, (const double*)b->tfree);
+  if (b->maxw > 0)
+    hipLaunchKernelGGL((k_forces<R, NS, true>), dim3((b->maxw + wpb - 1) / wpb, b->n), dim3(64 * wpb),
+                       fc_wave_bytes<R>() * wpb, b->stream, b->dsys, kStepMask, (const double*)b->tfree);
+  if (b->maxs > 0)
+    hipLaunchKernelGGL((k_forces<R, NS, false>), dim3((b->maxs + wpb - 1) / wpb, b->n), dim3(64 * wpb),
+                       fc_wave_bytes<R>() * wpb, b->stream, b->dsys, kStepMask, (const double*)b->tfree);
+  hipLaunchKernelGGL(k_atom_terms, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys, kStepMask,
+                     (const double*)b->tfree);
 }
 
 static void launch_forces(qnb_batch* b, int wpb) {
@@ -445,6 +452,8 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     if ((rc = instance_alloc(S, b->mem, par[k], sys[k]->ncgp + 2))) return rc;
     HIPCHK(hipMemcpy(S.x, x0[k], sizeof(double) * 3 * S.natom, hipMemcpyHostToDevice));
     b->maxg = std::max(b->maxg, S.ncgp);
+    b->maxw = std::max(b->maxw, S.nwat);
+    b->maxs = std::max(b->maxs, S.ncgp_solute);
     b->maxat = std::max(b->maxat, S.natom);
     b->maxsw = std::max(b->maxsw, S.nwat + S.nat_solute);
     b->ns_max = std::max(b->ns_max, S.nstates);

@@ -103,6 +103,7 @@ struct DevSys {
   double* epart;    // [nslots][QNB_NE]
   int nslots;
   double* ered;     // [QNB_NE]
+  double* eatom;    // [natom][2]: per-atom LRF and radial-restraint energies
   // polarisation restraint state
   double* sh_state;  // [3*QNB_MAXSH]: theta_corr, avtheta, avn_insh
   double* w_theta;   // [nwat]

@@ -366,13 +366,16 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
   return total;
 }
 
-template <typename R, int NS>
+// WAT: the water receivers (groups ncgp_solute..ncgp-1), else the solute
+// receivers; the two launches write disjoint parts of d and epart and get
+// their own register allocation
+template <typename R, int NS, bool WAT>
 __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict__ systems, uint32_t mask,
                                                         const double* __restrict__ tfree) {
   const DevSys& S = systems[blockIdx.y];
   const int wib = threadIdx.x / QNB_WAVE;
-  const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / QNB_WAVE) + wib);
-  if (a >= S.ncgp) return;
+  const int a = __builtin_amdgcn_readfirstlane((WAT ? S.ncgp_solute : 0) + blockIdx.x * (blockDim.x / QNB_WAVE) + wib);
+  if (a >= (WAT ? S.ncgp : S.ncgp_solute)) return;
   const int lane = threadIdx.x % QNB_WAVE;
   // per-wave LDS (dynamic, fc_wave_bytes<R>() per wave of the block)
   extern __shared__ __align__(16) unsigned char fc_smem[];
@@ -401,8 +404,14 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
   const int g0 = S.gfirst[a], na = S.glast[a] - g0 + 1;
   const int nq = S.nqat, ns = S.nstates;
 
-  if (!a_sol) {
+  if constexpr (WAT) {
     // ------------------------------------------------ water receiver (3 atoms)
+    // water-water parameters, wave-uniform: charge products of receiver atom
+    // k with partner atom kb, and the O-O LJ pair
+    R wel[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) wel[k] = (R)S.wwEl[k];
+    const R wA = (R)S.wwA[0], wB = (R)S.wwB[0];
     R o[3][3], f[3][3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -427,11 +436,17 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
               const int tt = k + 3 * kb;
               const R ux = dx - o[k][0], uy = dy - o[k][1], uz = dz - o[k][2];
               R dv;
-              if (!S.spc || (k == 0 && kb == 0))
+              if (S.spc) {
+                // SPC: Coulomb for every pair, LJ for O-O only
+                const R q = kb == 0 ? wel[k] : (kb == 1 ? wel[k + 3] : wel[k + 6]);
+                if (k == 0 && kb == 0)
+                  dv = PairMath<R>::nbe(ux, uy, uz, wA, wB, q, e[E_WW_EL], e[E_WW_VDW], acc);
+                else
+                  dv = PairMath<R>::nbe_spc(ux, uy, uz, q, e[E_WW_EL], acc);
+              } else {
                 dv = PairMath<R>::nbe(ux, uy, uz, (R)S.wwA[tt], (R)S.wwB[tt], (R)S.wwEl[tt], e[E_WW_EL],
                                       e[E_WW_VDW], acc);
-              else
-                dv = PairMath<R>::nbe_spc(ux, uy, uz, (R)S.wwEl[tt], e[E_WW_EL], acc);
+              }
               f[k][0] += ux * dv;
               f[k][1] += uy * dv;
               f[k][2] += uz * dv;
@@ -492,23 +507,15 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
         }
       }
     }
-    // reduce; LRF Taylor term (lrf_taylor) and SCAAS radial restraint on O
-    // (restrain_solvent) on the atom's own lane
+    // reduce (the per-atom LRF Taylor and restraint terms follow in k_atom_terms)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const double fx = wave_sum((double)f[k][0]), fy = wave_sum((double)f[k][1]), fz = wave_sum((double)f[k][2]);
       if (lane == k) {
         const int i = ra + k;
-        double fi[3] = {-fx, -fy, -fz};
-        if ((mask & T_LRF) && S.use_lrf && !S.excl[i] && S.iqatom[i] == 0) lrf_taylor_atom(S, i, fi, e[E_LRF]);
-        if (k == 0 && (mask & T_RAD) && !aexcl) {
-          const double T = tfree[blockIdx.y];
-          const double shift = S.fk_wsphere != 0.0 ? sqrt(S.Boltz * T / S.fk_wsphere) : 0.0;
-          radial_restraint(S, i, shift, fi, e[E_RAD]);
-        }
-        d[3 * i] = fi[0];
-        d[3 * i + 1] = fi[1];
-        d[3 * i + 2] = fi[2];
+        d[3 * i] = -fx;
+        d[3 * i + 1] = -fy;
+        d[3 * i + 2] = -fz;
       }
     }
   } else {
@@ -614,18 +621,16 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
           }
         }
       }
-      // reduce; LRF Taylor term on the atom's own lane
+      // reduce (the per-atom LRF Taylor term follows in k_atom_terms)
 #pragma unroll
       for (int k = 0; k < CH; k++) {
         if (k >= nk) break;
         const double fx = wave_sum((double)f[k][0]), fy = wave_sum((double)f[k][1]), fz = wave_sum((double)f[k][2]);
         if (lane == k) {
           const int i = S.cgpatom[g0 + c0 + k];
-          double fi[3] = {-fx, -fy, -fz};
-          if ((mask & T_LRF) && S.use_lrf && !S.excl[i] && S.iqatom[i] == 0) lrf_taylor_atom(S, i, fi, e[E_LRF]);
-          d[3 * i] = fi[0];
-          d[3 * i + 1] = fi[1];
-          d[3 * i + 2] = fi[2];
+          d[3 * i] = -fx;
+          d[3 * i + 1] = -fy;
+          d[3 * i + 2] = -fz;
         }
       }
       __builtin_amdgcn_wave_barrier();
@@ -655,6 +660,35 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
       ep[E_Q0 + 6 * s + 4] = v[2];
       ep[E_Q0 + 6 * s + 5] = v[3];
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-atom terms after the pair kernels, one thread per atom: the LRF Taylor
+// term (lrf_taylor, nonbondene.f90:507-571) of every non-Q, non-excluded
+// atom and the SCAAS radial restraint (restrain_solvent, 6466-6544) of every
+// non-excluded water oxygen; adds to d.  On the sampled steps the atom's two
+// energies go to eatom[2i], eatom[2i+1] for the fixed-order reduction.
+__global__ __launch_bounds__(256) void k_atom_terms(const DevSys* __restrict__ systems, uint32_t mask,
+                                                    const double* __restrict__ tfree) {
+  const DevSys& S = systems[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S.natom) return;
+  double f[3] = {0.0, 0.0, 0.0};
+  double el = 0.0, er = 0.0;
+  if ((mask & T_LRF) && S.use_lrf && !S.excl[i] && S.iqatom[i] == 0) lrf_taylor_atom(S, i, f, el);
+  const int w = i - S.nat_solute;
+  if ((mask & T_RAD) && w >= 0 && w % S.sa == 0 && S.sa == 3 && !S.excl[i]) {
+    const double T = tfree[blockIdx.y];
+    const double shift = S.fk_wsphere != 0.0 ? sqrt(S.Boltz * T / S.fk_wsphere) : 0.0;
+    radial_restraint(S, i, shift, f, er);
+  }
+  S.d[3 * i] += f[0];
+  S.d[3 * i + 1] += f[1];
+  S.d[3 * i + 2] += f[2];
+  if (want_energies(S)) {
+    S.eatom[2 * i] = el;
+    S.eatom[2 * i + 1] = er;
   }
 }
 
@@ -993,10 +1027,28 @@ __global__ __launch_bounds__(64 * QNB_RED_CHUNKS) void k_reduce_energies(const D
   if (k < QNB_NE)
     for (int a = a0; a < a1; a++) s += S.epart[(size_t)a * QNB_NE + k];
   part[c][k] = s;
+  // per-atom LRF and radial-restraint energies: thread t sums a contiguous
+  // atom range, then the ranges are added in order
+  __shared__ double apart[64 * QNB_RED_CHUNKS][2];
+  {
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int i0 = (int)((long)S.natom * t / nt), i1 = (int)((long)S.natom * (t + 1) / nt);
+    double l = 0.0, r = 0.0;
+    for (int i = i0; i < i1; i++) {
+      l += S.eatom[2 * i];
+      r += S.eatom[2 * i + 1];
+    }
+    apart[t][0] = l;
+    apart[t][1] = r;
+  }
   __syncthreads();
   if (c == 0 && k < QNB_NE) {
     double t = 0.0;
     for (int m = 0; m < QNB_RED_CHUNKS; m++) t += part[m][k];
+    if (k == E_LRF || k == E_RAD) {
+      const int col = k == E_LRF ? 0 : 1;
+      for (int m = 0; m < (int)blockDim.x; m++) t += apart[m][col];
+    }
     if (with_qq && S.nqat > 0) t += S.epart[(size_t)S.ncgp * QNB_NE + k];
     if (with_pol && S.nwat > 0 && S.wpol && S.nshell > 0) t += S.epart[(size_t)(S.ncgp + 1) * QNB_NE + k];
     S.ered[k] = t;
@@ -1007,7 +1059,8 @@ __global__ __launch_bounds__(64 * QNB_RED_CHUNKS) void k_reduce_energies(const D
 template __global__ void k_lists_lrf<double>(const DevSys*);
 template __global__ void k_lists_lrf<float>(const DevSys*);
 #define QNB_INST(R, NS)                                                            \
-  template __global__ void k_forces<R, NS>(const DevSys*, uint32_t, const double*); \
+  template __global__ void k_forces<R, NS, true>(const DevSys*, uint32_t, const double*); \
+  template __global__ void k_forces<R, NS, false>(const DevSys*, uint32_t, const double*); \
   template __global__ void k_qforces<R, NS>(const DevSys*, uint32_t, int);
 QNB_INST(double, 1)
 QNB_INST(double, 2)
