@@ -338,8 +338,10 @@ int32_t qnb_batch_minimize(qnb_batch *b, int32_t nsteps, double max_disp);
 int32_t qnb_batch_autotune(qnb_batch *b, int32_t reps, int32_t *wpb_out);
 
 /* Times one kernel of the step on the batch's stream with HIP events: which =
- * 0 the list/LRF update (k_lists_lrf), 1 the receiver-side force kernel
- * (k_forces); the kernel is launched `reps` times over all instances (both
+ * 0 the list/LRF update (k_lists_lrf), 1 the receiver-side force kernels
+ * (k_forces for water and for solute receivers, in the form used on steps
+ * that sample no energies, then the per-atom LRF/restraint kernel); the
+ * kernels are launched `reps` times over all instances (all
  * recompute their outputs from the current coordinates, so the state does not
  * change) and *ms_per_launch receives the average launch duration. */
 int32_t qnb_batch_time_kernel(qnb_batch *b, int32_t which, int32_t reps, double *ms_per_launch);

@@ -16,7 +16,7 @@ namespace qnb {
 template <typename R>
 __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
-template <typename R, int NS, bool WAT>
+template <typename R, int NS, bool WAT, bool E>
 __global__ void k_forces(const DevSys*, uint32_t, const double*);
 template <typename R, int NS>
 __global__ void k_qforces(const DevSys*, uint32_t, int);
@@ -162,6 +162,8 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
     }
     T.lx_off.assign(nsol1 + 1, 0);
     for (int i = 0; i < nsol1; i++) {
+      // bit 31 of the exclusion mask: the atom has partners beyond the 25-atom window
+      if (!lx[i].empty()) T.exmask[i] |= 1u << 31;
       T.lx_off[i + 1] = T.lx_off[i] + (int)lx[i].size();
       for (auto& e : lx[i]) {
         T.lx_j.push_back(e.first);
@@ -457,10 +459,10 @@ static void launch_eval(qnb_handle* h, uint32_t mask) {
   const DevSys& S = h->hs;
   const int wpb = 4;
   if (S.nwat > 0)
-    hipLaunchKernelGGL((k_forces<R, NS, true>), dim3((S.nwat + wpb - 1) / wpb, 1), dim3(64 * wpb),
+    hipLaunchKernelGGL((k_forces<R, NS, true, true>), dim3((S.nwat + wpb - 1) / wpb, 1), dim3(64 * wpb),
                        fc_wave_bytes<R>() * wpb, h->stream, h->dsys, mask, (const double*)h->d_tfree);
   if (S.ncgp_solute > 0)
-    hipLaunchKernelGGL((k_forces<R, NS, false>), dim3((S.ncgp_solute + wpb - 1) / wpb, 1), dim3(64 * wpb),
+    hipLaunchKernelGGL((k_forces<R, NS, false, true>), dim3((S.ncgp_solute + wpb - 1) / wpb, 1), dim3(64 * wpb),
                        fc_wave_bytes<R>() * wpb, h->stream, h->dsys, mask, (const double*)h->d_tfree);
   hipLaunchKernelGGL(k_atom_terms, dim3((S.natom + 255) / 256, 1), dim3(256), 0, h->stream, h->dsys, mask,
                      (const double*)h->d_tfree);

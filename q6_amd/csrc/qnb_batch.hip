@@ -18,7 +18,7 @@ namespace qnb {
 template <typename R>
 __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
-template <typename R, int NS, bool WAT>
+template <typename R, int NS, bool WAT, bool E>
 __global__ void k_forces(const DevSys*, uint32_t, const double*);
 template <typename R, int NS>
 __global__ void k_qforces(const DevSys*, uint32_t, int);
@@ -304,6 +304,7 @@ struct qnb_batch {
   int qchunks = 1, qwaves = 0;
   int wpb_lists = 4, wpb_forces = 4;  // waves per block, picked by qnb_batch_autotune
   std::vector<int> nbcycle;
+  std::vector<int> ene_every;  // per instance energy sampling interval
   long long steps_done = 0;
   ~qnb_batch() {
     if (cycle_exec) (void)hipGraphExecDestroy(cycle_exec);
@@ -328,35 +329,44 @@ static int forces_wpb(int wpb) {
   return std::max(1, std::min(std::min(wpb, FC_WPB), cap));
 }
 
-template <typename R, int NS>
-static void launch_forces_t(qnb_batch* b, int wpb) {
-  wpb = forces_wpb<R>(wpb);
+template <typename R, int NS, bool E>
+static void launch_forces_e(qnb_batch* b, int wpb) {
   if (b->maxw > 0)
-    hipLaunchKernelGGL((k_forces<R, NS, true>), dim3((b->maxw + wpb - 1) / wpb, b->n), dim3(64 * wpb),
+    hipLaunchKernelGGL((k_forces<R, NS, true, E>), dim3((b->maxw + wpb - 1) / wpb, b->n), dim3(64 * wpb),
                        fc_wave_bytes<R>() * wpb, b->stream, b->dsys, kStepMask, (const double*)b->tfree);
   if (b->maxs > 0)
-    hipLaunchKernelGGL((k_forces<R, NS, false>), dim3((b->maxs + wpb - 1) / wpb, b->n), dim3(64 * wpb),
+    hipLaunchKernelGGL((k_forces<R, NS, false, E>), dim3((b->maxs + wpb - 1) / wpb, b->n), dim3(64 * wpb),
                        fc_wave_bytes<R>() * wpb, b->stream, b->dsys, kStepMask, (const double*)b->tfree);
+}
+
+// energy: the step samples energies in at least one instance (or the host
+// cannot tell, e.g. inside a graph whose period is not a multiple of every
+// instance's sampling interval)
+template <typename R, int NS>
+static void launch_forces_t(qnb_batch* b, int wpb, bool energy = true) {
+  wpb = forces_wpb<R>(wpb);
+  if (energy) launch_forces_e<R, NS, true>(b, wpb);
+  else launch_forces_e<R, NS, false>(b, wpb);
   hipLaunchKernelGGL(k_atom_terms, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys, kStepMask,
                      (const double*)b->tfree);
 }
 
-static void launch_forces(qnb_batch* b, int wpb) {
+static void launch_forces(qnb_batch* b, int wpb, bool energy = true) {
   const bool mixed = b->prec == QNB_PREC_MIXED;
   switch (b->ns_max) {
-    case 1: mixed ? launch_forces_t<float, 1>(b, wpb) : launch_forces_t<double, 1>(b, wpb); break;
-    case 2: mixed ? launch_forces_t<float, 2>(b, wpb) : launch_forces_t<double, 2>(b, wpb); break;
+    case 1: mixed ? launch_forces_t<float, 1>(b, wpb, energy) : launch_forces_t<double, 1>(b, wpb, energy); break;
+    case 2: mixed ? launch_forces_t<float, 2>(b, wpb, energy) : launch_forces_t<double, 2>(b, wpb, energy); break;
     case 3:
-    case 4: mixed ? launch_forces_t<float, 4>(b, wpb) : launch_forces_t<double, 4>(b, wpb); break;
-    default: mixed ? launch_forces_t<float, 8>(b, wpb) : launch_forces_t<double, 8>(b, wpb); break;
+    case 4: mixed ? launch_forces_t<float, 4>(b, wpb, energy) : launch_forces_t<double, 4>(b, wpb, energy); break;
+    default: mixed ? launch_forces_t<float, 8>(b, wpb, energy) : launch_forces_t<double, 8>(b, wpb, energy); break;
   }
 }
 
 template <typename R, int NS>
-static void enqueue_forces(qnb_batch* b) {
+static void enqueue_forces(qnb_batch* b, bool energy = true) {
   const int wpb = 4, B = b->n;
   const uint32_t mask = kStepMask;
-  launch_forces_t<R, NS>(b, b->wpb_forces);
+  launch_forces_t<R, NS>(b, b->wpb_forces, energy);
   if (b->anyq) {
     hipLaunchKernelGGL((k_qforces<R, NS>), dim3((b->qwaves + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream,
                        b->dsys, mask, b->qchunks);
@@ -366,18 +376,18 @@ static void enqueue_forces(qnb_batch* b) {
 }
 
 template <typename R>
-static void enqueue_step(qnb_batch* b, bool lists) {
+static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
   const int B = b->n;
   if (lists) {
     launch_lists(b, b->wpb_lists);
     if (b->anyq && !b->static_q) hipLaunchKernelGGL(k_qlists, dim3(B), dim3(1024), 0, b->stream, b->dsys);
   }
   switch (b->ns_max) {
-    case 1: enqueue_forces<R, 1>(b); break;
-    case 2: enqueue_forces<R, 2>(b); break;
+    case 1: enqueue_forces<R, 1>(b, energy); break;
+    case 2: enqueue_forces<R, 2>(b, energy); break;
     case 3:
-    case 4: enqueue_forces<R, 4>(b); break;
-    default: enqueue_forces<R, 8>(b); break;
+    case 4: enqueue_forces<R, 4>(b, energy); break;
+    default: enqueue_forces<R, 8>(b, energy); break;
   }
   hipLaunchKernelGGL(k_bonded, dim3(B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(1024), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
@@ -407,9 +417,16 @@ static void enqueue_min_step(qnb_batch* b, bool lists, double k0, double max_dis
   hipLaunchKernelGGL(k_shake, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
 }
 
-static void enqueue(qnb_batch* b, bool lists) {
-  if (b->prec == QNB_PREC_MIXED) enqueue_step<float>(b, lists);
-  else enqueue_step<double>(b, lists);
+// does MD step `step` sample energies in any instance
+static bool energy_step(const qnb_batch* b, long long step) {
+  for (int e : b->ene_every)
+    if (e <= 0 || step % e == 0) return true;
+  return false;
+}
+
+static void enqueue(qnb_batch* b, bool lists, bool energy) {
+  if (b->prec == QNB_PREC_MIXED) enqueue_step<float>(b, lists, energy);
+  else enqueue_step<double>(b, lists, energy);
 }
 
 extern "C" int32_t qnb_batch_destroy(qnb_batch* b) {
@@ -508,6 +525,7 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     // the nonbonded kernels accumulate energies on the sampled steps only
     b->hs[k].step_ptr = M.istep;
     b->hs[k].ene_every = m->ene_every;
+    b->ene_every.push_back(m->ene_every);
   }
   if ((rc = b->mem.alloc(&b->dsys, n)) || (rc = b->mem.alloc(&b->dmd, n))) return rc;
   // the MD records change after this point only in qnb_batch_store_frames
@@ -585,14 +603,18 @@ extern "C" int32_t qnb_batch_run(qnb_batch* b, int32_t nsteps, int32_t use_graph
   int done = 0;
   // steps up to the next list update, eagerly
   while (done < nsteps && (b->steps_done % nb) != 0) {
-    enqueue(b, false);
+    enqueue(b, false, energy_step(b, b->steps_done));
     b->steps_done++;
     done++;
   }
   if (use_graph && nsteps - done >= nb) {
     if (!b->cycle_exec) {
+      // a cycle starts at a multiple of nb: its sampled steps sit at fixed
+      // offsets when every sampling interval divides nb
+      bool fixed = true;
+      for (int e : b->ene_every) fixed = fixed && e > 0 && nb % e == 0;
       HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      for (int s = 0; s < nb; s++) enqueue(b, s == 0);
+      for (int s = 0; s < nb; s++) enqueue(b, s == 0, fixed ? energy_step(b, s) : true);
       HIPCHK(hipStreamEndCapture(b->stream, &b->cycle_graph));
       HIPCHK(hipGraphInstantiate(&b->cycle_exec, b->cycle_graph, nullptr, nullptr, 0));
     }
@@ -603,7 +625,7 @@ extern "C" int32_t qnb_batch_run(qnb_batch* b, int32_t nsteps, int32_t use_graph
     }
   }
   while (done < nsteps) {
-    enqueue(b, (b->steps_done % nb) == 0);
+    enqueue(b, (b->steps_done % nb) == 0, energy_step(b, b->steps_done));
     b->steps_done++;
     done++;
   }
@@ -792,7 +814,7 @@ extern "C" int32_t qnb_batch_time_kernel(qnb_batch* b, int32_t which, int32_t re
   HIPCHK(hipEventCreate(&e1));
   auto launch = [&]() {
     if (which == 0) launch_lists(b, b->wpb_lists);
-    else launch_forces(b, b->wpb_forces);
+    else launch_forces(b, b->wpb_forces, false);
   };
   launch();  // warm-up
   HIPCHK(hipEventRecord(e0, b->stream));
@@ -837,11 +859,11 @@ extern "C" int32_t qnb_batch_autotune(qnb_batch* b, int32_t reps, int32_t* wpb_o
     int pick = 4;
     for (int c : cand) {
       if (which == 0) launch_lists(b, c);
-      else launch_forces(b, c);
+      else launch_forces(b, c, false);
       HIPCHK(hipEventRecord(e0, b->stream));
       for (int r = 0; r < reps; r++) {
         if (which == 0) launch_lists(b, c);
-        else launch_forces(b, c);
+        else launch_forces(b, c, false);
       }
       HIPCHK(hipEventRecord(e1, b->stream));
       HIPCHK(hipEventSynchronize(e1));

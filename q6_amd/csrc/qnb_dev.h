@@ -53,7 +53,7 @@ struct DevSys {
   const double* avdw;  // [t*3+k]
   const double* bvdw;
   const int* ljcod;    // [ti + tj*ntypes], 1-based code
-  const uint32_t* exmask;  // [nat_solute] bit k-1: pair (i, i+k) excluded
+  const uint32_t* exmask;  // [nat_solute] bit k-1: pair (i, i+k) excluded; bit 31: i has long partners
   const uint32_t* m14mask; // [nat_solute] bit k-1: pair (i, i+k) is 1-4
   const int* lx_off;       // [nat_solute+1] long exclusion/1-4 partners of the lower atom
   const int* lx_j;
@@ -260,8 +260,11 @@ struct PairMath {
 #define FC_CH 10
 template <typename R>
 __host__ __device__ constexpr size_t fc_wave_bytes() {
-  return ((sizeof(R) * 3 * (FC_CAP + FC_CH) + sizeof(int) * (2 * FC_CAP + FC_CH)) + 15) / 16 * 16;
+  return ((sizeof(R) * 3 * (FC_CAP + FC_CH) + sizeof(double) * FC_CH + sizeof(int) * (2 * FC_CAP + 4 * FC_CH)) + 15) /
+         16 * 16;
 }
+// type tables of the solute-solute parameters held in LDS up to this many types
+#define FC_MAXT 64
 
 // Adjoint of the per-state Q-atom energies over a set of frames: tangent
 // tables (derivative of each Q table entry with respect to the Q-side

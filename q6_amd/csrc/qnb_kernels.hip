@@ -327,6 +327,55 @@ enum : uint32_t { T_WW = 1, T_QW = 2, T_PW = 4, T_PP = 8, T_QP = 16, T_LRF = 32,
 #define FT_SOL 4
 #define FT_OWN 8
 #define FT_Q 16
+// pp_param from data at hand: the two atoms' types, charges and exclusion
+// masks (bit 31: partners beyond the 25-atom window, looked up in the long
+// list) and the type tables (LJ factors, ljcod) in LDS.  Same result as
+// pp_param.
+struct PPAtom {
+  int i, t;
+  uint32_t xm, m14;
+  double q;
+};
+__device__ __forceinline__ bool pp_param_fast(const DevSys& S, const PPAtom& P, const PPAtom& Q,
+                                              const double* __restrict__ avdw, const double* __restrict__ bvdw,
+                                              const uint8_t* __restrict__ ljc, double& A, double& B, double& el) {
+  const bool pl = P.i < Q.i;
+  const int dk = pl ? Q.i - P.i : P.i - Q.i;
+  const uint32_t xm = pl ? P.xm : Q.xm, m14 = pl ? P.m14 : Q.m14;
+  bool is14 = false;
+  if (dk <= 25) {
+    const uint32_t bit = 1u << (dk - 1);
+    if (xm & bit) return false;
+    is14 = (m14 & bit) != 0;
+  } else if (xm >> 31) {
+    const int lo = pl ? P.i : Q.i, hi = pl ? Q.i : P.i;
+    bool ex = false;
+    for (int m = S.lx_off[lo]; m < S.lx_off[lo + 1]; m++)
+      if (S.lx_j[m] == hi) {
+        if (S.lx_kind[m] == 0) ex = true;
+        else is14 = true;
+      }
+    if (ex) return false;
+  }
+  const int vdw = is14 ? 3 : ljc[P.t + Q.t * S.ntypes];
+  const int k = vdw - 1;
+  if (S.ivdw_rule == 1) {
+    A = avdw[P.t * 3 + k] * avdw[Q.t * 3 + k];
+    B = bvdw[P.t * 3 + k] * bvdw[Q.t * 3 + k];
+  } else {
+    double tA = avdw[P.t * 3 + k] + avdw[Q.t * 3 + k];
+    tA = tA * tA;
+    tA = tA * tA * tA;
+    double tB = bvdw[P.t * 3 + k] * bvdw[Q.t * 3 + k];
+    A = (tA * tA) * tB;
+    B = 2.0 * tA * tB;
+  }
+  el = P.q * Q.q;
+  const bool set = fabs(A) > 1.0e-10 || fabs(B) > 1.0e-10 || fabs(el) > 1.0e-10;
+  if (vdw == 3) el = el * S.el14;
+  return set;
+}
+
 template <typename R>
 __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int* __restrict__ row, int p0, int nn,
                                               uint32_t use_sol, uint32_t use_wat, double xr0, double xr1,
@@ -369,12 +418,30 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
 // WAT: the water receivers (groups ncgp_solute..ncgp-1), else the solute
 // receivers; the two launches write disjoint parts of d and epart and get
 // their own register allocation
-template <typename R, int NS, bool WAT>
+// E: false for launches on steps that sample no energies (the host knows);
+// true evaluates the device-side sampling test
+template <typename R, int NS, bool WAT, bool E>
 __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict__ systems, uint32_t mask,
                                                         const double* __restrict__ tfree) {
   const DevSys& S = systems[blockIdx.y];
   const int wib = threadIdx.x / QNB_WAVE;
   const int a = __builtin_amdgcn_readfirstlane((WAT ? S.ncgp_solute : 0) + blockIdx.x * (blockDim.x / QNB_WAVE) + wib);
+  // solute receivers: LJ factor and ljcod tables in LDS (every thread of the
+  // block takes part before any wave leaves)
+  __shared__ double s_avdw[3 * FC_MAXT], s_bvdw[3 * FC_MAXT];
+  __shared__ uint8_t s_ljc[FC_MAXT * FC_MAXT];
+  const bool lds_types = !WAT && S.ntypes <= FC_MAXT;
+  if constexpr (!WAT) {
+    if (lds_types) {
+      const int nt = S.ntypes;
+      for (int k = threadIdx.x; k < 3 * nt; k += blockDim.x) {
+        s_avdw[k] = S.avdw[k];
+        s_bvdw[k] = S.bvdw[k];
+      }
+      for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) s_ljc[k] = (uint8_t)S.ljcod[k];
+    }
+    __syncthreads();
+  }
   if (a >= (WAT ? S.ncgp : S.ncgp_solute)) return;
   const int lane = threadIdx.x % QNB_WAVE;
   // per-wave LDS (dynamic, fc_wave_bytes<R>() per wave of the block)
@@ -382,14 +449,18 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
   unsigned char* wbase = fc_smem + (size_t)wib * fc_wave_bytes<R>();
   R(*s_pd)[3] = reinterpret_cast<R(*)[3]>(wbase);
   R(*s_o)[3] = reinterpret_cast<R(*)[3]>(wbase + sizeof(R) * 3 * FC_CAP);
-  int* s_pj = reinterpret_cast<int*>(wbase + sizeof(R) * 3 * (FC_CAP + FC_CH));
+  double* s_rq = reinterpret_cast<double*>(wbase + sizeof(R) * 3 * (FC_CAP + FC_CH));
+  int* s_pj = reinterpret_cast<int*>(s_rq + FC_CH);
   int* s_pt = s_pj + FC_CAP;
   int* s_ia = s_pt + FC_CAP;
+  int* s_rt = s_ia + FC_CH;
+  uint32_t* s_rx = reinterpret_cast<uint32_t*>(s_rt + FC_CH);
+  uint32_t* s_rm = s_rx + FC_CH;
   const double* __restrict__ x = S.x;
   double* __restrict__ d = S.d;
   const int nn = min(S.nnbr[a], S.cap);
   const int* __restrict__ row = S.nbr + (size_t)a * S.cap;
-  const bool want_e = want_energies(S);
+  const bool want_e = E && want_energies(S);
   const int nsol = S.nat_solute, sa = S.sa, ncs = S.ncgp_solute;
   const bool a_sol = a < ncs;
   const int ra = group_ref_atom(S, a);
@@ -532,6 +603,10 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
         s_o[lane][0] = (R)(x[3 * i] - xr0);
         s_o[lane][1] = (R)(x[3 * i + 1] - xr1);
         s_o[lane][2] = (R)(x[3 * i + 2] - xr2);
+        s_rt[lane] = S.iac[i];
+        s_rx[lane] = S.exmask[i];
+        s_rm[lane] = S.m14mask[i];
+        s_rq[lane] = S.crg[i];
       }
       __builtin_amdgcn_wave_barrier();
       R f[CH][3];
@@ -548,6 +623,13 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
             const bool b_sol = (tag & FT_SOL) != 0;
             const bool acc = want_e && (tag & FT_OWN);
             const R dx = s_pd[t][0], dy = s_pd[t][1], dz = s_pd[t][2];
+            PPAtom Q{j, 0, 0u, 0u, 0.0};
+            if (b_sol && lds_types) {
+              Q.t = S.iac[j];
+              Q.xm = S.exmask[j];
+              Q.m14 = S.m14mask[j];
+              Q.q = S.crg[j];
+            }
 #pragma unroll
             for (int k = 0; k < CH; k++) {
               if (k >= nk) break;
@@ -557,7 +639,12 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
               R dv;
               if (b_sol) {
                 double A, B, el;
-                if (!pp_param(S, i, j, A, B, el)) continue;
+                if (lds_types) {
+                  const PPAtom P{i, s_rt[k], s_rx[k], s_rm[k], s_rq[k]};
+                  if (!pp_param_fast(S, P, Q, s_avdw, s_bvdw, s_ljc, A, B, el)) continue;
+                } else if (!pp_param(S, i, j, A, B, el)) {
+                  continue;
+                }
                 dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], acc);
               } else {
                 const size_t tt = (size_t)i + (size_t)nsol * kb;
@@ -1059,8 +1146,10 @@ __global__ __launch_bounds__(64 * QNB_RED_CHUNKS) void k_reduce_energies(const D
 template __global__ void k_lists_lrf<double>(const DevSys*);
 template __global__ void k_lists_lrf<float>(const DevSys*);
 #define QNB_INST(R, NS)                                                            \
-  template __global__ void k_forces<R, NS, true>(const DevSys*, uint32_t, const double*); \
-  template __global__ void k_forces<R, NS, false>(const DevSys*, uint32_t, const double*); \
+  template __global__ void k_forces<R, NS, true, true>(const DevSys*, uint32_t, const double*); \
+  template __global__ void k_forces<R, NS, false, true>(const DevSys*, uint32_t, const double*); \
+  template __global__ void k_forces<R, NS, true, false>(const DevSys*, uint32_t, const double*); \
+  template __global__ void k_forces<R, NS, false, false>(const DevSys*, uint32_t, const double*); \
   template __global__ void k_qforces<R, NS>(const DevSys*, uint32_t, int);
 QNB_INST(double, 1)
 QNB_INST(double, 2)
