@@ -467,9 +467,9 @@ static void launch_eval(qnb_handle* h, uint32_t mask) {
   hipLaunchKernelGGL(k_atom_terms, dim3((S.natom + 255) / 256, 1), dim3(256), 0, h->stream, h->dsys, mask,
                      (const double*)h->d_tfree);
   if (S.nqat > 0 && (mask & (QNB_T_QW | QNB_T_QP))) {
-    // partners per Q-atom: at most nwat + nat_solute; chunks of 64 per wave
+    // partners per Q-atom: at most nwat + nat_solute; about 8 per lane
     int maxp = S.nwat + S.nat_solute;
-    int chunks = std::max(1, std::min((maxp + 63) / 64, 64));
+    int chunks = std::max(1, std::min((maxp + 511) / 512, 64));
     int waves = S.nqat * chunks;
     hipLaunchKernelGGL((k_qforces<R, NS>), dim3((waves + wpb - 1) / wpb, 1), dim3(64 * wpb), 0, h->stream, h->dsys,
                        mask, chunks);

@@ -51,6 +51,8 @@ struct MdDev {
   int* istep;     // &istep_all[k]
   double* xs;     // [xs_max][natom*3] coordinate snapshots (counter ctr[2])
   int xs_every, xs_max;
+  double* bpart;  // [nbpart][2] bond and angle energy per k_bonded block
+  int nbpart;
 };
 
 enum { ERR_NONFINITE = 1, ERR_SHAKE = 2, ERR_OVERFLOW = 4 };
@@ -67,14 +69,17 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // harmonic bonds and angles of the solute (bond, angle: src/bondene.f90:41-246)
+// Thread t of the grid row takes bond t and angle t; each block's energy
+// sums go to bpart[block] and k_record adds them in block order.
 __global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
-  const DevSys& S = systems[blockIdx.x];
-  const MdDev& M = mds[blockIdx.x];
+  const DevSys& S = systems[blockIdx.y];
+  const MdDev& M = mds[blockIdx.y];
   __shared__ double red[4];
   const double* x = S.x;
   double* d = S.d;
   double eb = 0.0, ea = 0.0;
-  for (int b = threadIdx.x; b < M.nbonds; b += blockDim.x) {
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x, tstride = gridDim.x * blockDim.x;
+  for (int b = t0; b < M.nbonds; b += tstride) {
     const int i = M.bond_ij[2 * b], j = M.bond_ij[2 * b + 1];
     const double ax = x[3 * j] - x[3 * i], ay = x[3 * j + 1] - x[3 * i + 1], az = x[3 * j + 2] - x[3 * i + 2];
     const double r = sqrt(ax * ax + ay * ay + az * az);
@@ -88,7 +93,7 @@ __global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ syste
     unsafeAtomicAdd(&d[3 * j + 1], ay * dv);
     unsafeAtomicAdd(&d[3 * j + 2], az * dv);
   }
-  for (int a = threadIdx.x; a < M.nangles; a += blockDim.x) {
+  for (int a = t0; a < M.nangles; a += tstride) {
     const int i = M.angle_ijk[3 * a], j = M.angle_ijk[3 * a + 1], k = M.angle_ijk[3 * a + 2];
     double u[3], w[3];
     for (int c = 0; c < 3; c++) {
@@ -120,9 +125,9 @@ __global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ syste
   }
   const double sb = block_sum(eb, red);
   const double sa = block_sum(ea, red);
-  if (threadIdx.x == 0) {
-    M.st[4] = sb;
-    M.st[5] = sa;
+  if (threadIdx.x == 0 && blockIdx.x < M.nbpart) {
+    M.bpart[2 * blockIdx.x] = sb;
+    M.bpart[2 * blockIdx.x + 1] = sa;
   }
 }
 
@@ -270,8 +275,13 @@ __global__ void k_record(const DevSys* __restrict__ systems, const MdDev* __rest
   if (k < QNB_NE) h[k] = S.ered[k];
   __syncthreads();
   if (k == 0) {
-    h[QNB_F_BOND] = M.st[4];
-    h[QNB_F_ANGLE] = M.st[5];
+    double eb = 0.0, ea = 0.0;
+    for (int p = 0; p < M.nbpart; p++) {
+      eb += M.bpart[2 * p];
+      ea += M.bpart[2 * p + 1];
+    }
+    h[QNB_F_BOND] = eb;
+    h[QNB_F_ANGLE] = ea;
     h[QNB_F_EKIN] = M.st[3];
     h[QNB_F_TEMP] = M.st[2];
     h[QNB_F_TFREE] = M.st[1];
@@ -300,6 +310,7 @@ struct qnb_batch {
   hipGraphExec_t cycle_exec = nullptr;
   hipGraph_t cycle_graph = nullptr;
   int maxg = 0, maxat = 0, maxsw = 0, maxw = 0, maxs = 0, ns_max = 1, prec = 0;
+  int nbblk = 1;  // k_bonded blocks per instance
   bool anyq = false, anypol = false, static_q = true;
   int qchunks = 1, qwaves = 0;
   int wpb_lists = 4, wpb_forces = 4;  // waves per block, picked by qnb_batch_autotune
@@ -389,7 +400,7 @@ static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
     case 4: enqueue_forces<R, 4>(b, energy); break;
     default: enqueue_forces<R, 8>(b, energy); break;
   }
-  hipLaunchKernelGGL(k_bonded, dim3(B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  hipLaunchKernelGGL(k_bonded, dim3(b->nbblk, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(1024), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
   hipLaunchKernelGGL(k_snap, dim3((3 * b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
@@ -412,7 +423,7 @@ static void enqueue_min_step(qnb_batch* b, bool lists, double k0, double max_dis
     case 4: enqueue_forces<R, 4>(b); break;
     default: enqueue_forces<R, 8>(b); break;
   }
-  hipLaunchKernelGGL(k_bonded, dim3(B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  hipLaunchKernelGGL(k_bonded, dim3(b->nbblk, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_sd_move, dim3((b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd, k0, max_disp);
   hipLaunchKernelGGL(k_shake, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
 }
@@ -477,7 +488,8 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     b->anyq = b->anyq || S.nqat > 0;
     b->anypol = b->anypol || (S.nwat > 0 && S.wpol && S.nshell > 0);
     b->static_q = b->static_q && (S.rcq2 > S.rexcl2);
-    b->qchunks = std::max(b->qchunks, std::min((S.nwat + S.nat_solute + 63) / 64, 64));
+    // about 8 partners per lane
+    b->qchunks = std::max(b->qchunks, std::min((S.nwat + S.nat_solute + 511) / 512, 64));
     b->nbcycle.push_back(md[k]->nbcycle);
     if (md[k]->nbcycle != md[0]->nbcycle) return fail(QNB_EINVAL, "all instances need the same nbcycle");
   }
@@ -517,6 +529,7 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
         (rc = b->mem.alloc(&M.hist, (size_t)std::max(m->max_frames, 1) * QNB_FRAME_LEN)) ||
         (rc = b->mem.alloc(&M.st, 8)) || (rc = b->mem.alloc(&M.ctr, 4)))
       return rc;
+    b->nbblk = std::max(b->nbblk, (std::max(m->nbonds, m->nangles) + 255) / 256);
     HIPCHK(hipMemcpy(M.v, v0[k], sizeof(double) * 3 * S.natom, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(M.st, 0, 8 * sizeof(double)));
     HIPCHK(hipMemset(M.ctr, 0, 4 * sizeof(int)));
@@ -526,6 +539,12 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     b->hs[k].step_ptr = M.istep;
     b->hs[k].ene_every = m->ene_every;
     b->ene_every.push_back(m->ene_every);
+  }
+  for (int k = 0; k < n; k++) {
+    MdDev& M = b->hm[k];
+    M.nbpart = b->nbblk;
+    if ((rc = b->mem.alloc(&M.bpart, 2 * (size_t)b->nbblk))) return rc;
+    HIPCHK(hipMemset(M.bpart, 0, sizeof(double) * 2 * b->nbblk));
   }
   if ((rc = b->mem.alloc(&b->dsys, n)) || (rc = b->mem.alloc(&b->dmd, n))) return rc;
   // the MD records change after this point only in qnb_batch_store_frames

@@ -1001,20 +1001,36 @@ __global__ __launch_bounds__(1024) void k_watpol(const DevSys* __restrict__ syst
     }
   }
   __syncthreads();
-  // rank of each member inside its shell
+  // rank of each member inside its shell; the comparands are staged through
+  // LDS in chunks of WP_CHUNK (theta and water index)
+  constexpr int WP_CHUNK = 1024;
+  __shared__ double c_th[WP_CHUNK];
+  __shared__ int c_w[WP_CHUNK];
   for (int sh = 0; sh < nsh; sh++) {
     const int n = cnt[sh];
     const int* L = S.sh_list + (size_t)sh * nw;
-    for (int m = t; m < n; m += nt) {
-      const int w = L[m];
-      const double th = S.w_theta[w];
+    for (int m0 = 0; m0 < n; m0 += nt) {
+      const int m = m0 + t;
+      const int w = m < n ? L[m] : -1;
+      const double th = m < n ? S.w_theta[w] : 0.0;
       int r = 0;
-      for (int k = 0; k < n; k++) {
-        const int w2 = L[k];
-        const double t2 = S.w_theta[w2];
-        r += (t2 < th) || (t2 == th && w2 < w);
+      for (int k0 = 0; k0 < n; k0 += WP_CHUNK) {
+        const int nk = min(WP_CHUNK, n - k0);
+        __syncthreads();
+        for (int k = t; k < nk; k += nt) {
+          const int w2 = L[k0 + k];
+          c_w[k] = w2;
+          c_th[k] = S.w_theta[w2];
+        }
+        __syncthreads();
+        if (m < n)
+          for (int k = 0; k < nk; k++) {
+            const double t2 = c_th[k];
+            const int w2 = c_w[k];
+            r += (t2 < th) || (t2 == th && w2 < w);
+          }
       }
-      S.sh_rank[(size_t)sh * nw + m] = r;
+      if (m < n) S.sh_rank[(size_t)sh * nw + m] = r;
     }
   }
   // theta_corr feedback every itdis steps (nonbondene.f90:6653-6666)
