@@ -260,8 +260,8 @@ struct PairMath {
 #define FC_CH 10
 template <typename R>
 __host__ __device__ constexpr size_t fc_wave_bytes() {
-  return ((sizeof(R) * 3 * (FC_CAP + FC_CH) + sizeof(double) * FC_CH + sizeof(int) * (2 * FC_CAP + 4 * FC_CH)) + 15) /
-         16 * 16;
+  return ((sizeof(R) * (3 * (FC_CAP + FC_CH) + 9 * FC_CH) + sizeof(double) * FC_CH +
+           sizeof(int) * (2 * FC_CAP + 4 * FC_CH)) + 15) / 16 * 16;
 }
 // type tables of the solute-solute parameters held in LDS up to this many types
 #define FC_MAXT 64

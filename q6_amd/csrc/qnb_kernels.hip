@@ -449,7 +449,8 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
   unsigned char* wbase = fc_smem + (size_t)wib * fc_wave_bytes<R>();
   R(*s_pd)[3] = reinterpret_cast<R(*)[3]>(wbase);
   R(*s_o)[3] = reinterpret_cast<R(*)[3]>(wbase + sizeof(R) * 3 * FC_CAP);
-  double* s_rq = reinterpret_cast<double*>(wbase + sizeof(R) * 3 * (FC_CAP + FC_CH));
+  R(*s_pw)[3][3] = reinterpret_cast<R(*)[3][3]>(wbase + sizeof(R) * 3 * (FC_CAP + FC_CH));
+  double* s_rq = reinterpret_cast<double*>(wbase + sizeof(R) * (3 * (FC_CAP + FC_CH) + 9 * FC_CH));
   int* s_pj = reinterpret_cast<int*>(s_rq + FC_CH);
   int* s_pt = s_pj + FC_CAP;
   int* s_ia = s_pt + FC_CAP;
@@ -607,6 +608,14 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
         s_rx[lane] = S.exmask[i];
         s_rm[lane] = S.m14mask[i];
         s_rq[lane] = S.crg[i];
+        // the atom's solute-water parameters against the 3 water sites
+        if (S.nwat > 0)
+          for (int kb = 0; kb < 3; kb++) {
+            const size_t tt = (size_t)i + (size_t)nsol * kb;
+            s_pw[lane][kb][0] = (R)S.pwA[tt];
+            s_pw[lane][kb][1] = (R)S.pwB[tt];
+            s_pw[lane][kb][2] = (R)S.pwEl[tt];
+          }
       }
       __builtin_amdgcn_wave_barrier();
       R f[CH][3];
@@ -647,8 +656,7 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
                 }
                 dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], acc);
               } else {
-                const size_t tt = (size_t)i + (size_t)nsol * kb;
-                dv = PairMath<R>::nbe(ux, uy, uz, (R)S.pwA[tt], (R)S.pwB[tt], (R)S.pwEl[tt], e[E_PW_EL],
+                dv = PairMath<R>::nbe(ux, uy, uz, s_pw[k][kb][0], s_pw[k][kb][1], s_pw[k][kb][2], e[E_PW_EL],
                                       e[E_PW_VDW], want_e);
               }
               f[k][0] += ux * dv;
