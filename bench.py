@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--windows", type=int, default=51)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--frames", type=int, default=20,
+    ap.add_argument("--frames", type=int, default=200,
                     help="frames per window stored after the timed run for the dG precision check and the adjoint")
     ap.add_argument("--workload", choices=["windows", "uneven"], default="windows",
                     help="windows: 51 lambda windows of the 30 A system (headline); uneven: 16 spheres of 15-45 A")
