@@ -89,141 +89,67 @@ __device__ __forceinline__ int t3index(int a, int b, int c) {
 
 // ---------------------------------------------------------------------------
 // Pair-list update with LRF (make_pair_lists, nonbondene.f90:749-837, SPH):
-// one wavefront per receiver charge group a; the waves of a block sweep the
-// source groups b together, 64 at a time (fewer when their non-Q atoms
-// exceed LL_CAP): the block stages each source group's reference atom,
-// flags and non-Q atoms (position, charge) in LDS once for all its waves.
-// A wave tests its receiver against the 64 reference atoms (q_dist4, one
-// lane per group); a pair inside its class cut-off goes to a's neighbour row
-// through a ballot/prefix compaction, a pair beyond it (and inside RcLRF)
-// adds b's non-Q charges to a's expansion.  The far groups' atoms are spread
-// over the lanes through a prefix sum of their counts, so lanes stay
-// balanced; terms are summed in R per lane over a chunk, then in FP64.
-#define LL_CAP 1024
+// one wavefront per receiver charge group a.  Lanes test 64 source groups b
+// at a time (q_dist4 of the reference atoms); a pair inside its class
+// cut-off goes to a's neighbour row through a ballot/prefix compaction, a
+// pair beyond it (and inside RcLRF) adds b's non-Q charges to a's
+// expansion.  The far groups' non-Q atoms (per-group lists built at set-up)
+// are spread over the lanes through a wave prefix sum of their counts, so
+// lanes stay balanced; terms are summed in R per lane over a chunk of
+// groups, then added in FP64.
 template <typename R>
 __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ systems) {
   const DevSys& S = systems[blockIdx.y];
-  const int W = blockDim.x / QNB_WAVE;
   const int wib = threadIdx.x / QNB_WAVE;
+  const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / QNB_WAVE) + wib);
+  if (a >= S.ncgp) return;
   const int lane = threadIdx.x % QNB_WAVE;
-  const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * W + wib);
-  const int ng_all = S.ncgp, ncs = S.ncgp_solute;
-  __shared__ double c_ref[QNB_WAVE][3];      // reference atom of each staged source group
-  __shared__ int c_flag[QNB_WAVE];           // bit 0 excluded, bit 1 solute
-  __shared__ int c_off[QNB_WAVE + 1];        // first staged atom of each group
-  __shared__ double c_at[LL_CAP][4];         // non-Q atoms: x, y, z, charge
-  __shared__ int c_ng;                       // groups staged in this chunk
-  __shared__ int w_fp[512 / QNB_WAVE][QNB_WAVE + 1];  // per wave: prefix of its far groups' atoms
+  __shared__ int w_fp[512 / QNB_WAVE][QNB_WAVE];   // per wave: first far atom of each lane's group
+  __shared__ int w_nq[512 / QNB_WAVE][QNB_WAVE];   // per wave: start of that group's non-Q atom list
   const double* __restrict__ x = S.x;
-  const bool active = a < ng_all;
-  const int ra = active ? group_ref_atom(S, a) : 0;
-  const bool work = active && !S.excl[ra];
-  const bool a_sol = a < ncs;
+  const int ra = group_ref_atom(S, a);
+  const bool a_sol = a < S.ncgp_solute;
   // centre of a (cgp_centers, nonbondene.f90:43-78): sum in atom order, then divide
   double cx = 0.0, cy = 0.0, cz = 0.0;
-  if (active) {
-    for (int k = S.gfirst[a]; k <= S.glast[a]; k++) {
-      int at = S.cgpatom[k];
-      cx = cx + x[3 * at];
-      cy = cy + x[3 * at + 1];
-      cz = cz + x[3 * at + 2];
-    }
-    double nat = (double)(S.glast[a] - S.gfirst[a] + 1);
-    cx = cx / nat;
-    cy = cy / nat;
-    cz = cz / nat;
+  for (int k = S.gfirst[a]; k <= S.glast[a]; k++) {
+    int at = S.cgpatom[k];
+    cx = cx + x[3 * at];
+    cy = cy + x[3 * at + 1];
+    cz = cz + x[3 * at + 2];
   }
-  const double xa0 = x[3 * ra], xa1 = x[3 * ra + 1], xa2 = x[3 * ra + 2];
+  double nat = (double)(S.glast[a] - S.gfirst[a] + 1);
+  cx = cx / nat;
+  cy = cy / nat;
+  cz = cz / nat;
   double acc[20];
 #pragma unroll
   for (int k = 0; k < 20; k++) acc[k] = 0.0;
   int count = 0;
-  int* __restrict__ row = S.nbr + (size_t)(active ? a : 0) * S.cap;
-
-  for (int b0 = 0; b0 < ng_all;) {
-    // --- stage: wave 0 sizes the chunk, then the block copies it
-    __syncthreads();
-    if (wib == 0) {
-      const int b = b0 + lane;
-      int cnt = 0;
-      if (b < ng_all) {
-        if (b < ncs) {
-          for (int k = S.gfirst[b]; k <= S.glast[b]; k++) cnt += S.iqatom[S.cgpatom[k]] == 0;
-        } else {
-          cnt = S.sa;
-        }
-      }
-      int incl = cnt;
-#pragma unroll
-      for (int o2 = 1; o2 < QNB_WAVE; o2 <<= 1) {
-        const int v = __shfl_up(incl, o2, QNB_WAVE);
-        if (lane >= o2) incl += v;
-      }
-      const int ng = max(1, __popcll(__ballot(incl <= LL_CAP)));
-      c_off[lane] = incl - cnt;
-      if (lane == ng - 1) c_off[QNB_WAVE] = incl;
-      if (lane == 0) c_ng = ng;
-    }
-    __syncthreads();
-    const int ng = c_ng;
-    for (int g = threadIdx.x; g < ng; g += blockDim.x) {
-      const int b = b0 + g;
-      if (b < ng_all) {
-        const int rb = group_ref_atom(S, b);
-        c_ref[g][0] = x[3 * rb];
-        c_ref[g][1] = x[3 * rb + 1];
-        c_ref[g][2] = x[3 * rb + 2];
-        c_flag[g] = (S.excl[rb] ? 1 : 0) | (b < ncs ? 2 : 0);
-        int slot = c_off[g];
-        if (b < ncs) {
-          for (int k = S.gfirst[b]; k <= S.glast[b]; k++) {
-            const int i = S.cgpatom[k];
-            if (S.iqatom[i] != 0 || slot >= LL_CAP) continue;
-            c_at[slot][0] = x[3 * i];
-            c_at[slot][1] = x[3 * i + 1];
-            c_at[slot][2] = x[3 * i + 2];
-            c_at[slot][3] = S.crg[i];
-            slot++;
-          }
-        } else {
-          for (int k = 0; k < S.sa && slot < LL_CAP; k++, slot++) {
-            const int i = rb + k;
-            c_at[slot][0] = x[3 * i];
-            c_at[slot][1] = x[3 * i + 1];
-            c_at[slot][2] = x[3 * i + 2];
-            c_at[slot][3] = S.crg[i];
-          }
-        }
-      } else {
-        c_flag[g] = 1;
-      }
-    }
-    __syncthreads();
-
-    if (work) {
-      // --- distance tests, one lane per staged group
+  int* __restrict__ row = S.nbr + (size_t)a * S.cap;
+  if (!S.excl[ra]) {
+    for (int b0 = 0; b0 < S.ncgp; b0 += QNB_WAVE) {
       const int b = b0 + lane;
       bool in = false, far = false;
-      if (lane < ng && b < ng_all && b != a && !(c_flag[lane] & 1)) {
-        const bool b_sol = (c_flag[lane] & 2) != 0;
-        const double cut2 = (a_sol && b_sol) ? S.rcpp2 : ((!a_sol && !b_sol) ? S.rcww2 : S.rcpw2);
-        double r2;
-        {
-#pragma clang fp contract(off)
-          const double dx = c_ref[lane][0] - xa0, dy = c_ref[lane][1] - xa1, dz = c_ref[lane][2] - xa2;
-          r2 = dx * dx;
-          r2 = r2 + dy * dy;
-          r2 = r2 + dz * dz;
+      if (b < S.ncgp && b != a) {
+        const int rb = group_ref_atom(S, b);
+        if (!S.excl[rb]) {
+          const bool b_sol = b < S.ncgp_solute;
+          const double cut2 = (a_sol && b_sol) ? S.rcpp2 : ((!a_sol && !b_sol) ? S.rcww2 : S.rcpw2);
+          const double r2 = dist4(x, ra, rb);
+          if (r2 <= cut2) in = true;
+          else far = S.use_lrf && r2 <= S.rclrf2;
         }
-        if (r2 <= cut2) in = true;
-        else far = S.use_lrf && r2 <= S.rclrf2;
       }
       const uint64_t m = __ballot(in);
       const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
       if (in && pos < S.cap) row[pos] = b;
       count += __popcll(m);
-      // --- LRF: the far groups' atoms spread over the lanes
-      const int fcnt = far ? c_off[lane + 1 < ng ? lane + 1 : QNB_WAVE] - c_off[lane] : 0;
+      // far groups' non-Q atoms over the lanes
+      int n0 = 0, fcnt = 0;
+      if (far) {
+        n0 = S.gnq_off[b];
+        fcnt = S.gnq_off[b + 1] - n0;
+      }
       int incl = fcnt;
 #pragma unroll
       for (int o2 = 1; o2 < QNB_WAVE; o2 <<= 1) {
@@ -231,33 +157,31 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
         if (lane >= o2) incl += v;
       }
       const int total = __shfl(incl, QNB_WAVE - 1, QNB_WAVE);
+      if (total == 0) continue;
       w_fp[wib][lane] = incl - fcnt;
+      w_nq[wib][lane] = n0;
       __builtin_amdgcn_wave_barrier();
-      if (total > 0) {
-        R part[20];
+      R part[20];
 #pragma unroll
-        for (int k = 0; k < 20; k++) part[k] = R(0);
-        for (int t = lane; t < total; t += QNB_WAVE) {
-          // owning far group: largest g with w_fp[g] <= t among groups with atoms
-          int lo = 0, hi = QNB_WAVE - 1;
+      for (int k = 0; k < 20; k++) part[k] = R(0);
+      for (int t = lane; t < total; t += QNB_WAVE) {
+        // owning group: largest g with w_fp[g] <= t (zero-count groups share
+        // their successor's start and lose the tie to it)
+        int lo = 0, hi = QNB_WAVE - 1;
 #pragma unroll
-          for (int it = 0; it < 6; it++) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (w_fp[wib][mid] <= t) lo = mid;
-            else hi = mid - 1;
-          }
-          const int slot = c_off[lo] + (t - w_fp[wib][lo]);
-          lrf_accumulate<R>(part, c_at[slot][0] - cx, c_at[slot][1] - cy, c_at[slot][2] - cz, c_at[slot][3]);
+        for (int it = 0; it < 6; it++) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (w_fp[wib][mid] <= t) lo = mid;
+          else hi = mid - 1;
         }
-#pragma unroll
-        for (int k = 0; k < 20; k++) acc[k] += (double)part[k];
+        const int i = S.gnq_atom[w_nq[wib][lo] + (t - w_fp[wib][lo])];
+        lrf_accumulate<R>(part, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
       }
+#pragma unroll
+      for (int k = 0; k < 20; k++) acc[k] += (double)part[k];
       __builtin_amdgcn_wave_barrier();
     }
-    b0 += ng;
   }
-
-  if (!active) return;
 #pragma unroll
   for (int k = 0; k < 20; k++) acc[k] = wave_sum(acc[k]);
   if (lane == 0) {
