@@ -224,17 +224,7 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
   UP(gswitch, s->cgp_iswitch, s->ncgp);
   UP(cgpatom, s->cgpatom, ncgpat);
   UP(atom_group, atom_group.data(), nat);
-  {
-    // non-Q atoms of every group, in group order (the LRF sources)
-    std::vector<int> off(s->ncgp + 1, 0), at;
-    for (int g = 0; g < s->ncgp; g++) {
-      for (int k = s->cgp_first[g]; k <= s->cgp_last[g]; k++)
-        if (s->iqatom[s->cgpatom[k]] == 0) at.push_back(s->cgpatom[k]);
-      off[g + 1] = (int)at.size();
-    }
-    UP(gnq_off, off.data(), off.size());
-    UP(gnq_atom, at.data(), at.size());
-  }
+
   UP(avdw, s->avdw, 3 * s->ntypes);
   UP(bvdw, s->bvdw, 3 * s->ntypes);
   UP(ljcod, s->ljcod, s->ntypes * s->ntypes);

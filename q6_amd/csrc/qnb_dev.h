@@ -50,8 +50,6 @@ struct DevSys {
   const int* gswitch;  // atom
   const int* cgpatom;
   const int* atom_group;
-  const int* gnq_off;   // [ncgp+1] each group's non-Q atoms in gnq_atom (LRF sources)
-  const int* gnq_atom;
   const double* avdw;  // [t*3+k]
   const double* bvdw;
   const int* ljcod;    // [ti + tj*ntypes], 1-based code

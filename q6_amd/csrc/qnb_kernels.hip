@@ -89,23 +89,16 @@ __device__ __forceinline__ int t3index(int a, int b, int c) {
 
 // ---------------------------------------------------------------------------
 // Pair-list update with LRF (make_pair_lists, nonbondene.f90:749-837, SPH):
-// one wavefront per receiver charge group a.  Lanes test 64 source groups b
-// at a time (q_dist4 of the reference atoms); a pair inside its class
+// one wavefront per receiver charge group a.  Lanes sweep every other group b
+// (LDS-free: group reference atoms are L2-resident); a pair inside its class
 // cut-off goes to a's neighbour row through a ballot/prefix compaction, a
-// pair beyond it (and inside RcLRF) adds b's non-Q charges to a's
-// expansion.  The far groups' non-Q atoms (per-group lists built at set-up)
-// are spread over the lanes through a wave prefix sum of their counts, so
-// lanes stay balanced; terms are summed in R per lane over a chunk of
-// groups, then added in FP64.
+// pair beyond it (and inside RcLRF) adds b's non-Q charges to a's expansion.
 template <typename R>
 __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ systems) {
   const DevSys& S = systems[blockIdx.y];
-  const int wib = threadIdx.x / QNB_WAVE;
-  const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / QNB_WAVE) + wib);
+  const int a = blockIdx.x * (blockDim.x / QNB_WAVE) + threadIdx.x / QNB_WAVE;
   if (a >= S.ncgp) return;
   const int lane = threadIdx.x % QNB_WAVE;
-  __shared__ int w_fp[512 / QNB_WAVE][QNB_WAVE];   // per wave: first far atom of each lane's group
-  __shared__ int w_nq[512 / QNB_WAVE][QNB_WAVE];   // per wave: start of that group's non-Q atom list
   const double* __restrict__ x = S.x;
   const int ra = group_ref_atom(S, a);
   const bool a_sol = a < S.ncgp_solute;
@@ -129,57 +122,40 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
   if (!S.excl[ra]) {
     for (int b0 = 0; b0 < S.ncgp; b0 += QNB_WAVE) {
       const int b = b0 + lane;
-      bool in = false, far = false;
+      bool in = false;
       if (b < S.ncgp && b != a) {
         const int rb = group_ref_atom(S, b);
         if (!S.excl[rb]) {
           const bool b_sol = b < S.ncgp_solute;
           const double cut2 = (a_sol && b_sol) ? S.rcpp2 : ((!a_sol && !b_sol) ? S.rcww2 : S.rcpw2);
           const double r2 = dist4(x, ra, rb);
-          if (r2 <= cut2) in = true;
-          else far = S.use_lrf && r2 <= S.rclrf2;
+          if (r2 <= cut2) {
+            in = true;
+          } else if (S.use_lrf && r2 <= S.rclrf2) {
+            R part[20];
+#pragma unroll
+            for (int k = 0; k < 20; k++) part[k] = R(0);
+            if (b_sol) {
+              for (int k = S.gfirst[b]; k <= S.glast[b]; k++) {
+                int i = S.cgpatom[k];
+                if (S.iqatom[i] != 0) continue;
+                lrf_accumulate<R>(part, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
+              }
+            } else {
+              for (int k = 0; k < S.sa; k++) {
+                int i = rb + k;
+                lrf_accumulate<R>(part, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
+              }
+            }
+#pragma unroll
+            for (int k = 0; k < 20; k++) acc[k] += (double)part[k];
+          }
         }
       }
       const uint64_t m = __ballot(in);
       const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
       if (in && pos < S.cap) row[pos] = b;
       count += __popcll(m);
-      // far groups' non-Q atoms over the lanes
-      int n0 = 0, fcnt = 0;
-      if (far) {
-        n0 = S.gnq_off[b];
-        fcnt = S.gnq_off[b + 1] - n0;
-      }
-      int incl = fcnt;
-#pragma unroll
-      for (int o2 = 1; o2 < QNB_WAVE; o2 <<= 1) {
-        const int v = __shfl_up(incl, o2, QNB_WAVE);
-        if (lane >= o2) incl += v;
-      }
-      const int total = __shfl(incl, QNB_WAVE - 1, QNB_WAVE);
-      if (total == 0) continue;
-      w_fp[wib][lane] = incl - fcnt;
-      w_nq[wib][lane] = n0;
-      __builtin_amdgcn_wave_barrier();
-      R part[20];
-#pragma unroll
-      for (int k = 0; k < 20; k++) part[k] = R(0);
-      for (int t = lane; t < total; t += QNB_WAVE) {
-        // owning group: largest g with w_fp[g] <= t (zero-count groups share
-        // their successor's start and lose the tie to it)
-        int lo = 0, hi = QNB_WAVE - 1;
-#pragma unroll
-        for (int it = 0; it < 6; it++) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (w_fp[wib][mid] <= t) lo = mid;
-          else hi = mid - 1;
-        }
-        const int i = S.gnq_atom[w_nq[wib][lo] + (t - w_fp[wib][lo])];
-        lrf_accumulate<R>(part, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
-      }
-#pragma unroll
-      for (int k = 0; k < 20; k++) acc[k] += (double)part[k];
-      __builtin_amdgcn_wave_barrier();
     }
   }
 #pragma unroll
