@@ -298,8 +298,10 @@ enum qnb_frame_field {
   QNB_F_PP_EL = 0, QNB_F_PP_VDW, QNB_F_PW_EL, QNB_F_PW_VDW, QNB_F_WW_EL, QNB_F_WW_VDW, QNB_F_LRF,
   QNB_F_RAD, QNB_F_POL,
   QNB_F_Q0 = 9,          /* per state s: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw at 9 + 6s (8 states) */
-  QNB_F_BOND = 57, QNB_F_ANGLE, QNB_F_EKIN, QNB_F_TEMP, QNB_F_TFREE, QNB_F_STEP
+  QNB_F_BOND = 57, QNB_F_ANGLE, QNB_F_EKIN, QNB_F_TEMP, QNB_F_TFREE, QNB_F_STEP,
+  QNB_F_OFFD = 63        /* EVB off-diagonal m: Hij at 63 + 2m, r_kl at 64 + 2m (m < QNB_MAX_OFFD) */
 };
+#define QNB_MAX_OFFD 4
 
 typedef struct qnb_batch qnb_batch;
 
@@ -324,6 +326,15 @@ int32_t qnb_batch_energy_history(qnb_batch *b, int32_t k, double *out, int32_t c
  * explicit lists would hold them: out[0..6] = nbpp, nbpw, nbww, nbqp, nbqw
  * (atom pairs, one state), nbqq, nbqqp (summed over states). */
 int32_t qnb_batch_pair_counts(qnb_batch *b, int32_t k, int64_t *out);
+
+/* EVB off-diagonal elements sampled with the energies (offdiag,
+ * src/nonbondene.f90:6238-6260): element m of instance k is
+ * H = A[m] exp(-mu[m] r), r the distance between Q-atoms kl[2m] and
+ * kl[2m+1] (0-based Q-atom numbers), and each sampled frame holds (H, r) at
+ * QNB_F_OFFD + 2m, as Qdyn6 stores offd(:)%Hij, %rkl for the energy file.
+ * n <= QNB_MAX_OFFD; n = 0 removes them. */
+int32_t qnb_batch_set_offdiag(qnb_batch *b, int32_t k, int32_t n, const int32_t *kl, const double *A,
+                              const double *mu);
 
 /* Relaxes the current structures before MD: nsteps steepest-descent moves
  * along the full gradient (nonbonded engine + solute bonds/angles), each atom

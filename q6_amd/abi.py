@@ -101,7 +101,8 @@ QNB_ADJ_NV = 7
 ADJ_DQ, ADJ_DA, ADJ_DB, ADJ_DA14, ADJ_DB14, ADJ_EEL, ADJ_EVDW = range(7)
 
 QNB_FRAME_LEN = 72
-F_Q0, F_BOND, F_ANGLE, F_EKIN, F_TEMP, F_TFREE, F_STEP = 9, 57, 58, 59, 60, 61, 62
+F_Q0, F_BOND, F_ANGLE, F_EKIN, F_TEMP, F_TFREE, F_STEP, F_OFFD = 9, 57, 58, 59, 60, 61, 62, 63
+QNB_MAX_OFFD = 4
 
 # every symbol include/qnb.h declares (checked by the CPU test suite)
 EXPORTS = [
@@ -110,7 +111,7 @@ EXPORTS = [
     "qnb_get_pairs", "qnb_get_lrf", "qnb_get_shell_state", "qnb_set_shell_state",
     "qnb_batch_create", "qnb_batch_destroy", "qnb_batch_run", "qnb_batch_sync", "qnb_batch_get_state",
     "qnb_batch_energy_history", "qnb_batch_pair_counts", "qnb_batch_time_kernel", "qnb_batch_minimize", "qnb_batch_autotune",
-    "qnb_adjoint", "qnb_batch_store_frames", "qnb_batch_frames", "qnb_batch_adjoint",
+    "qnb_adjoint", "qnb_batch_store_frames", "qnb_batch_frames", "qnb_batch_adjoint", "qnb_batch_set_offdiag",
 ]
 
 _lib = None
@@ -146,6 +147,7 @@ def _declare(lib):
     lib.qnb_batch_autotune.argtypes = [vp, C.c_int32, _pi]
     lib.qnb_adjoint.argtypes = [vp, C.POINTER(QnbQTangent), C.c_int32, _pd, _pd]
     lib.qnb_batch_store_frames.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.qnb_batch_set_offdiag.argtypes = [vp, C.c_int32, C.c_int32, _pi, _pd, _pd]
     lib.qnb_batch_frames.argtypes = [vp, C.c_int32, _pd, C.c_int32, C.POINTER(C.c_int32)]
     lib.qnb_batch_adjoint.argtypes = [vp, pp, _pd, C.c_int64, C.POINTER(C.c_int64)]
     for n in EXPORTS:

@@ -53,6 +53,10 @@ struct MdDev {
   int xs_every, xs_max;
   double* bpart;  // [nbpart][2] bond and angle energy per k_bonded block
   int nbpart;
+  // EVB off-diagonal elements sampled with the energies
+  int noffd;
+  int offd_k[QNB_MAX_OFFD], offd_l[QNB_MAX_OFFD];  // atom indices
+  double offd_A[QNB_MAX_OFFD], offd_mu[QNB_MAX_OFFD];
 };
 
 enum { ERR_NONFINITE = 1, ERR_SHAKE = 2, ERR_OVERFLOW = 4 };
@@ -287,6 +291,12 @@ __global__ void k_record(const DevSys* __restrict__ systems, const MdDev* __rest
     h[QNB_F_TFREE] = M.st[1];
     h[QNB_F_STEP] = (double)step;
     for (int z = QNB_F_STEP + 1; z < QNB_FRAME_LEN; z++) h[z] = 0.0;
+    // offdiag (nonbondene.f90:6238-6260): H = A exp(-mu r_kl)
+    for (int m = 0; m < M.noffd; m++) {
+      const double r = sqrt(dist4(S.x, M.offd_k[m], M.offd_l[m]));
+      h[QNB_F_OFFD + 2 * m] = M.offd_A[m] * exp(-M.offd_mu[m] * r);
+      h[QNB_F_OFFD + 2 * m + 1] = r;
+    }
     M.ctr[0] = f + 1;
   }
 }
@@ -548,7 +558,8 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
   }
   if ((rc = b->mem.alloc(&b->dsys, n)) || (rc = b->mem.alloc(&b->dmd, n))) return rc;
   // the MD records change after this point only in qnb_batch_store_frames
-  // (after a stream sync); every kernel that takes b->dmd runs after this copy
+  // and qnb_batch_set_offdiag (after a stream sync); every kernel that takes
+  // b->dmd runs after this copy
   HIPCHK(hipMemcpy(b->dmd, b->hm.data(), sizeof(MdDev) * n, hipMemcpyHostToDevice));
   // first list build, regrown until every row fits, then rows sized to 1.5x
   // the largest neighbour count seen (re-checked at every update; an overflow
@@ -681,6 +692,28 @@ extern "C" int32_t qnb_batch_energy_history(qnb_batch* b, int32_t k, double* out
   *nframes = ctr[0];
   if (out && ctr[0] <= cap && ctr[0] > 0)
     HIPCHK(hipMemcpy(out, b->hm[k].hist, sizeof(double) * QNB_FRAME_LEN * ctr[0], hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_set_offdiag(qnb_batch* b, int32_t k, int32_t n, const int32_t* kl, const double* A,
+                                         const double* mu) {
+  if (!b || k < 0 || k >= b->n) return fail(QNB_EINVAL, "bad instance");
+  if (n < 0 || n > QNB_MAX_OFFD || (n > 0 && (!kl || !A || !mu)))
+    return fail(QNB_EINVAL, "off-diagonals: 0..%d elements with pairs, A and mu", QNB_MAX_OFFD);
+  const DevSys& S = b->hs[k];
+  const HostTopo* T = b->topos[b->topo_of[k]].get();
+  for (int m = 0; m < 2 * n; m++)
+    if (kl[m] < 0 || kl[m] >= S.nqat) return fail(QNB_EINVAL, "off-diagonal Q-atom %d out of range", kl[m]);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  MdDev& M = b->hm[k];
+  M.noffd = n;
+  for (int m = 0; m < n; m++) {
+    M.offd_k[m] = T->iqseq[kl[2 * m]];
+    M.offd_l[m] = T->iqseq[kl[2 * m + 1]];
+    M.offd_A[m] = A[m];
+    M.offd_mu[m] = mu[m];
+  }
+  HIPCHK(hipMemcpy(b->dmd, b->hm.data(), sizeof(MdDev) * b->n, hipMemcpyHostToDevice));
   return 0;
 }
 

@@ -69,6 +69,15 @@ def frame_states(frames, nstates):
     return st
 
 
+def frame_offd(frames, ij):
+    """Per-frame off-diagonal records (i, j, Hij, rkl) from the batch energy
+    history: ij[m] = (i, j) states of off-diagonal m (1-based, as in the FEP
+    file), its (H, r) sampled at QNB_F_OFFD + 2m."""
+    frames = np.asarray(frames, np.float64)
+    return [[(i, j, float(f[abi.F_OFFD + 2 * m]), float(f[abi.F_OFFD + 2 * m + 1])) for m, (i, j) in enumerate(ij)]
+            for f in frames]
+
+
 def write_ene(path, lambdas, states, offd=(), offd_frames=None, version="q6_amd nonbonded engine"):
     """states: (nframes, nstates, 14) = total, q(bond, angle, torsion, improper),
     qx(el, vdw), qq(el, vdw), qp(el, vdw), qw(el, vdw), restraint.
@@ -120,4 +129,4 @@ def read_ene(path, nstates):
     return dict(canary=canary, version=version, lambdas=lam, states=st, offd=offd)
 
 
-__all__ = ["write_ene", "read_ene", "frame_states"]
+__all__ = ["write_ene", "read_ene", "frame_states", "frame_offd"]

@@ -39,6 +39,21 @@ def evb_ground(V, H12):
     return eg, c1
 
 
+def offdiag_coupling(rkl, A, mu, eta=0.0, r0=0.0):
+    """Distance-dependent EVB coupling of Qfep6 (src/qfep.f90:476-480):
+    H12 = A exp(-mu (r - r0)) exp(-eta (r - r0)^2), r the off-diagonal
+    distance r_kl each frame of the energy file carries."""
+    d = np.asarray(rkl, np.float64) - r0
+    return A * np.exp(-mu * d) * np.exp(-eta * d ** 2)
+
+
+def _window_h12(H12, i, nskip):
+    """H12 of window i: a constant, or per window an array over its frames."""
+    if np.ndim(H12) == 0:
+        return float(H12)
+    return np.asarray(H12[i], np.float64)[nskip:]
+
+
 def gap(V, coeff):
     return np.asarray(V, np.float64) @ np.asarray(coeff, np.float64)
 
@@ -120,13 +135,14 @@ def evb_window_free_energy(Vs, lambdas, H12, kT, nskip=0):
     """Ground-state free energy at every window, Qfep6 Part 2 with the whole
     window in one bin (src/qfep.f90:716-795): dGg_i = dG_i - kT ln <exp(-(E_g
     - eps_i)/kT)>_i, eps_i = sum_s lambda_s V_s, taken about the mean of
-    E_g - eps_i as the reference does."""
+    E_g - eps_i as the reference does.  H12 is a constant or, per window, an
+    array over its frames (offdiag_coupling)."""
     lam = np.asarray(lambdas, np.float64)
     _, _, dG = zwanzig(Vs, lam, kT, nskip)
     out = np.zeros(len(Vs))
     for i, V in enumerate(Vs):
         V = np.asarray(V, np.float64)[nskip:]
-        eg, _ = evb_ground(V, H12)
+        eg, _ = evb_ground(V, _window_h12(H12, i, nskip))
         dvg = eg - V @ lam[i]
         av = dvg.sum() / len(dvg)
         out[i] = dG[i] + (-kT * np.log(np.sum(np.exp(-(dvg - av) / kT)) / len(dvg)) + av)
@@ -149,7 +165,7 @@ def evb_window_free_energy_grad(Vs, dVs, dH, lambdas, H12, kT, nskip=0):
     return out
 
 
-def evb_pmf(Vs, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):
+def evb_pmf(Vs, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None, rkl=None):
     """Reaction free energy along the energy gap, Qfep6 Parts 2 and 3
     (src/qfep.f90:700-827), 2-state EVB.  The gap sum_s coeff_s V_s of every
     frame after nskip is binned between the smallest and largest gap of all
@@ -158,11 +174,14 @@ def evb_pmf(Vs, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):
     bin with at least nmin points: dGg = dG_i - kT ln<exp(-(E_g -
     eps_i)/kT)>_bin; Part 3 averages dGg over windows weighted by points.
     frame_bins (list of int arrays) overrides the binning, e.g. to hold it
-    fixed under a parameter perturbation.  Returns a dict: gap centres,
+    fixed under a parameter perturbation.  H12 is a constant or, per window,
+    an array over its frames (offdiag_coupling); rkl (per window, over its
+    frames) adds the bin averages of the off-diagonal distance Part 3 prints
+    as <r_xy> (qfep.f90:748, 780).  Returns a dict: gap centres,
     dGg[nwin, nbins] (ground state), dGv[nwin, nbins, nstates] (diabatic
     curves), c1[nwin, nbins] (mean state-1 weight), npts[nwin, nbins],
-    binned[nbins] and binned_c1[nbins] (Part 3), ptsum[nbins], frame_bins;
-    entries below nmin are NaN."""
+    binned[nbins], binned_c1[nbins] and binned_r[nbins] (Part 3; binned_r
+    NaN without rkl), ptsum[nbins], frame_bins; entries below nmin are NaN."""
     lam = np.asarray(lambdas, np.float64)
     coeff = np.asarray(coeff, np.float64)
     _, _, dG = zwanzig(Vs, lam, kT, nskip)
@@ -180,10 +199,12 @@ def evb_pmf(Vs, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):
     npts = np.zeros((n, nbins), np.int64)
     num = np.zeros(nbins)
     numc = np.zeros(nbins)
+    numr = np.zeros(nbins)
     ptsum = np.zeros(nbins, np.int64)
     for i, V in enumerate(Vs):
         V = np.asarray(V, np.float64)[nskip:]
-        eg, w1 = evb_ground(V, H12)
+        eg, w1 = evb_ground(V, _window_h12(H12, i, nskip))
+        r = np.asarray(rkl[i], np.float64)[nskip:] if rkl is not None else None
         veff = V @ lam[i]
         dvg = eg - veff
         dvv = V - veff[:, None]
@@ -201,13 +222,16 @@ def evb_pmf(Vs, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):
             c1[i, k] = w1[sel].sum() / c
             num[k] += dGg[i, k] * c
             numc[k] += c1[i, k] * c
+            if r is not None:
+                numr[k] += (r[sel].sum() / c) * c  # bin mean x points, as Part 3 accumulates
             ptsum[k] += c
     ok = ptsum >= max(nmin, 1)
     binned = np.where(ok, num / np.maximum(ptsum, 1), np.nan)
     binned_c1 = np.where(ok, numc / np.maximum(ptsum, 1), np.nan)
+    binned_r = np.where(ok & (rkl is not None), numr / np.maximum(ptsum, 1), np.nan)
     centres = gapmin + (np.arange(nbins) + 0.5) * xint
     return dict(centres=centres, dGg=dGg, dGv=dGv, c1=c1, npts=npts, binned=binned, binned_c1=binned_c1,
-                ptsum=ptsum, frame_bins=frame_bins)
+                binned_r=binned_r, ptsum=ptsum, frame_bins=frame_bins)
 
 
 def evb_pmf_grad(Vs, dVs, dH, lambdas, H12, kT, coeff, nbins, nmin, nskip=0, frame_bins=None):

@@ -141,6 +141,15 @@ class BatchMD:
         keys = ("pp", "pw", "ww", "qp", "qw", "qq", "qqp")
         return dict(zip(keys, (int(v) for v in out)))
 
+    def set_offdiag(self, k, pairs, A, mu):
+        """EVB off-diagonals of instance k sampled with the energies: pairs of
+        0-based Q-atom numbers (k, l); H = A exp(-mu r_kl) (offdiag)."""
+        kl = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1))
+        A = np.ascontiguousarray(A, np.float64)
+        mu = np.ascontiguousarray(mu, np.float64)
+        abi.check(self.lib, self.lib.qnb_batch_set_offdiag(self.h, k, len(A), _pi(kl), _pd(A), _pd(mu)),
+                  "qnb_batch_set_offdiag")
+
     def store_frames(self, every, max_frames):
         """Keep a coordinate snapshot every `every` steps (max_frames per
         instance) on the device, for the adjoint."""

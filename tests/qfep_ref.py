@@ -13,11 +13,19 @@ def available():
     return os.path.exists(QFEP)
 
 
-def run(workdir, files, nstates, kT, nskip, nbins, nmin, alphas, coeff, noffd=1, scale_hij=1.0):
+def run(workdir, files, nstates, kT, nskip, nbins, nmin, alphas, coeff, noffd=1, scale_hij=1.0, couplings=None):
+    """couplings: list of (i, j, A, mu, eta, r0); Qfep6 then computes Hij from
+    the r_kl of the files' first off-diagonal record instead of taking the
+    files' Hij."""
+    if couplings:
+        noffd = 0
     lines = [str(len(files)), f"{nstates} {noffd}", f"{kT} {nskip}", str(nbins), str(nmin)]
     lines += [str(a) for a in alphas[1:]]
     if noffd:
         lines.append(str(scale_hij))
+    else:
+        lines.append(str(len(couplings or [])))
+        lines += [" ".join(str(v) for v in c) for c in couplings or []]
     lines.append(" ".join(str(c) for c in coeff))
     lines += list(files)
     r = subprocess.run([QFEP], input="\n".join(lines) + "\n", cwd=workdir, capture_output=True, text=True,
@@ -47,5 +55,5 @@ def parse(out):
         elif part == 3:
             t = ln.split()
             p3.append(dict(bin=int(t[0]) - 1, gap=float(t[1]), dGg=float(t[2]), norm=float(t[3]), pts=int(t[4]),
-                           c1=float(t[5]), c2=float(t[6])))
+                           c1=float(t[5]), c2=float(t[6]), r=float(t[7])))
     return dict(part1=p1, part2=p2, part3=p3)

@@ -84,3 +84,34 @@ def test_md_stays_stable_and_rigid(small):
     print(f"[gpu] T first {T[0]:.1f} max {T.max():.1f} last {T[-1]:.1f}", flush=True)
     assert np.all(np.isfinite(e[:, :abi.F_STEP]))
     assert 200.0 < T[-1] < 450.0
+
+
+def test_offdiag_sampled_with_energies(small):
+    """EVB off-diagonal elements recorded with the energies: r_kl is the
+    distance of the two Q-atoms in the coordinates of the sampled step (the
+    stored frames of the same steps), H = A exp(-mu r_kl); an instance
+    without off-diagonals records zeros there."""
+    from q6_amd.md import BatchMD
+    s, v = small
+    iqseq = s.a["iqseq"]
+    pairs = [(0, 1), (2, len(iqseq) - 1)]
+    A, mu = np.array([30.0, 12.5]), np.array([0.45, 1.2])
+    print("\n[gpu] offdiag sampling", flush=True)
+    b = BatchMD([s, s], [np.array([0.5, 0.5])] * 2, [v, v], precision=abi.QNB_PREC_MIXED, nbcycle=10,
+                ene_every=10, max_frames=16)
+    b.set_offdiag(0, pairs, A, mu)
+    b.store_frames(10, 16)
+    b.run(60)
+    b.sync()
+    e0, e1, fr = b.energies(0), b.energies(1), b.frames(0)
+    with pytest.raises(abi.QnbError):
+        b.set_offdiag(0, [(0, len(iqseq))], [1.0], [0.0])
+    b.close()
+    assert len(e0) == len(fr) == 7
+    for m, (k, l) in enumerate(pairs):
+        r = np.linalg.norm(fr[:, iqseq[l]] - fr[:, iqseq[k]], axis=1)
+        H = A[m] * np.exp(-mu[m] * r)
+        assert np.allclose(e0[:, abi.F_OFFD + 2 * m + 1], r, rtol=1e-13, atol=0)
+        assert np.allclose(e0[:, abi.F_OFFD + 2 * m], H, rtol=1e-13, atol=0)
+    assert np.all(e0[:, abi.F_OFFD + 4:] == 0.0)
+    assert np.all(e1[:, abi.F_OFFD:] == 0.0)

@@ -85,3 +85,51 @@ def test_fep_and_evb_match_stock_qfep6(seed):
         assert r["pts"] == res["ptsum"][k]
         assert abs(r["dGg"] - res["binned"][k]) <= 6e-3
         assert abs(r["c1"] - res["binned_c1"][k]) <= 6e-4
+
+
+@pytest.mark.skipif(not qfep_ref.available(), reason="reference Qfep6 not built (make -C oracle/ref)")
+def test_distance_dependent_coupling_matches_stock_qfep6():
+    """Off-diagonal records carry a per-frame r_kl (as the batch samples it);
+    Qfep6 given A, mu, eta, r0 computes H12 from it (qfep.f90:476-480)."""
+    rng = np.random.default_rng(3)
+    nw, npts, nskip, nbins, nmin, kT = 6, 50, 4, 10, 3, 0.596
+    A, mu, eta, r0 = 35.0, 0.6, 0.05, 2.0
+    alpha = np.array([0.0, -2.0])
+    coeff = np.array([1.0, -1.0])
+    lam1 = np.linspace(1, 0, nw)
+    lams = np.stack([lam1, 1 - lam1], 1)
+    Vraw = [np.stack([rng.normal(-15 + 6 * i, 2.0, npts), rng.normal(12 - 5 * i, 2.0, npts)], 1) for i in range(nw)]
+    rkl = [rng.uniform(2.0, 4.5, npts) for _ in range(nw)]
+    with tempfile.TemporaryDirectory() as tmp:
+        files = []
+        for i, V in enumerate(Vraw):
+            st = np.zeros((npts, 2, 14))
+            st[:, :, 0] = V
+            st[:, :, 5] = V
+            files.append(f"w{i}.en")
+            enefile.write_ene(os.path.join(tmp, files[-1]), lams[i], st,
+                              offd_frames=[[(1, 2, 0.0, r)] for r in rkl[i]])
+        ref = qfep_ref.run(tmp, files, 2, kT, nskip, nbins, nmin, alpha, coeff, couplings=[(1, 2, A, mu, eta, r0)])
+    Vs = [fep.state_energies(V, alpha) for V in Vraw]
+    H = [fep.offdiag_coupling(r, A, mu, eta, r0) for r in rkl]
+    res = fep.evb_pmf(Vs, lams, H, kT, coeff, nbins, nmin, nskip, rkl=rkl)
+    rows = ref["part2"]
+    assert len(rows) == int(np.sum(res["npts"] >= nmin))
+    for r in rows:
+        i = int(np.argmin(np.abs(lam1 - r["lam"])))
+        k = r["bin"]
+        assert r["pts"] == res["npts"][i, k]
+        assert abs(r["dGg"] - res["dGg"][i, k]) <= 6e-3
+        assert abs(r["c1"] - res["c1"][i, k]) <= 6e-4
+    assert ref["part3"]
+    for r in ref["part3"]:
+        k = r["bin"]
+        assert r["pts"] == res["ptsum"][k]
+        assert abs(r["dGg"] - res["binned"][k]) <= 6e-3
+        assert abs(r["c1"] - res["binned_c1"][k]) <= 6e-4
+        assert abs(r["r"] - res["binned_r"][k]) <= 6e-3
+    # a constant coupling is the mu = eta = 0 case
+    Hc = [fep.offdiag_coupling(r, A, 0.0) for r in rkl]
+    a = fep.evb_window_free_energy(Vs, lams, Hc, kT, nskip)
+    b = fep.evb_window_free_energy(Vs, lams, A, kT, nskip)
+    assert np.array_equal(a, b)
