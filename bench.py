@@ -177,6 +177,9 @@ def main():
                     help="frames per window stored after the timed run for the dG precision check and the adjoint")
     ap.add_argument("--workload", choices=["windows", "uneven"], default="windows",
                     help="windows: 51 lambda windows of the 30 A system (headline); uneven: 16 spheres of 15-45 A")
+    ap.add_argument("--calibration-copy", type=int, default=0, metavar="MIB",
+                    help="end with one device-to-device copy of MIB MiB (MIB read + MIB written), the known-size "
+                         "reference for scaling hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: lambda windows shard across GPUs with no per-step exchange "
@@ -268,6 +271,28 @@ def main():
         "free_energy_precision": prec,
     }
     print(json.dumps(out))
+    if args.calibration_copy > 0:
+        calibration_copy(args.calibration_copy)
+
+
+def calibration_copy(mib):
+    """One hipMemcpy device-to-device of mib MiB (the runtime's copy kernel:
+    mib MiB read, mib MiB written) through the HIP runtime the engine loaded."""
+    import ctypes as C
+    path = next(ln.split()[-1] for ln in open("/proc/self/maps") if "libamdhip64" in ln)
+    hip = C.CDLL(path)
+    n = C.c_size_t(mib << 20)
+    a, c = C.c_void_p(), C.c_void_p()
+    for ptr in (a, c):
+        if hip.hipMalloc(C.byref(ptr), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    ok = hip.hipMemset(a, 1, n) == 0 and hip.hipDeviceSynchronize() == 0
+    ok = ok and hip.hipMemcpy(c, a, n, 3) == 0 and hip.hipDeviceSynchronize() == 0  # 3: device to device
+    hip.hipFree(a)
+    hip.hipFree(c)
+    if not ok:
+        raise RuntimeError("calibration copy failed")
+    print(f"# calibration copy: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
 
 
 def uneven(args):

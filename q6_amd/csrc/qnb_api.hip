@@ -20,7 +20,7 @@ template <typename R, int NS, bool WAT, bool E>
 __global__ void k_forces(const DevSys*, uint32_t, const double*);
 template <typename R, int NS>
 __global__ void k_qforces(const DevSys*, uint32_t, int);
-template <typename R>
+template <typename R, int NS>
 __global__ void k_qq(const DevSys*, uint32_t);
 __global__ void k_watpol(const DevSys*, const int*);
 __global__ void k_reduce_energies(const DevSys*, int, int);
@@ -477,7 +477,7 @@ static void launch_eval(qnb_handle* h, uint32_t mask) {
   }
   const int with_qq = S.nqat > 0 && (mask & (QNB_T_QQ | QNB_T_QQP));
   const int with_pol = S.nwat > 0 && S.wpol && S.nshell > 0 && (mask & QNB_T_POL);
-  if (with_qq) hipLaunchKernelGGL((k_qq<R>), dim3(1), dim3(64), 0, h->stream, h->dsys, mask);
+  if (with_qq) hipLaunchKernelGGL((k_qq<R, NS>), dim3(1), dim3(QQ_THREADS), 0, h->stream, h->dsys, mask);
   if (with_pol) hipLaunchKernelGGL(k_watpol, dim3(1), dim3(1024), 0, h->stream, h->dsys, (const int*)h->d_istep);
   hipLaunchKernelGGL(k_reduce_energies, dim3(1), dim3(1024), 0, h->stream, h->dsys, with_qq, with_pol);
 }

@@ -22,7 +22,7 @@ template <typename R, int NS, bool WAT, bool E>
 __global__ void k_forces(const DevSys*, uint32_t, const double*);
 template <typename R, int NS>
 __global__ void k_qforces(const DevSys*, uint32_t, int);
-template <typename R>
+template <typename R, int NS>
 __global__ void k_qq(const DevSys*, uint32_t);
 __global__ void k_watpol(const DevSys*, const int*);
 __global__ void k_reduce_energies(const DevSys*, int, int);
@@ -350,13 +350,20 @@ static int forces_wpb(int wpb) {
   return std::max(1, std::min(std::min(wpb, FC_WPB), cap));
 }
 
+// Blocks are dealt to the 8 XCDs round-robin in linear block order
+// (blockIdx.y * gridDim.x + blockIdx.x); with gridDim.x a multiple of 8 the
+// XCD of a block depends on blockIdx.x alone, so the blocks of every instance
+// that read the same rows of the shared tables (Q tables, topology) run on
+// the same XCD and find them in its L2.  The padding blocks exit at once.
+static unsigned xcd_cols(int blocks) { return (unsigned)((blocks + 7) / 8 * 8); }
+
 template <typename R, int NS, bool E>
 static void launch_forces_e(qnb_batch* b, int wpb) {
   if (b->maxw > 0)
-    hipLaunchKernelGGL((k_forces<R, NS, true, E>), dim3((b->maxw + wpb - 1) / wpb, b->n), dim3(64 * wpb),
+    hipLaunchKernelGGL((k_forces<R, NS, true, E>), dim3(xcd_cols((b->maxw + wpb - 1) / wpb), b->n), dim3(64 * wpb),
                        fc_wave_bytes<R>() * wpb, b->stream, b->dsys, kStepMask, (const double*)b->tfree);
   if (b->maxs > 0)
-    hipLaunchKernelGGL((k_forces<R, NS, false, E>), dim3((b->maxs + wpb - 1) / wpb, b->n), dim3(64 * wpb),
+    hipLaunchKernelGGL((k_forces<R, NS, false, E>), dim3(xcd_cols((b->maxs + wpb - 1) / wpb), b->n), dim3(64 * wpb),
                        fc_wave_bytes<R>() * wpb, b->stream, b->dsys, kStepMask, (const double*)b->tfree);
 }
 
@@ -389,9 +396,9 @@ static void enqueue_forces(qnb_batch* b, bool energy = true) {
   const uint32_t mask = kStepMask;
   launch_forces_t<R, NS>(b, b->wpb_forces, energy);
   if (b->anyq) {
-    hipLaunchKernelGGL((k_qforces<R, NS>), dim3((b->qwaves + wpb - 1) / wpb, B), dim3(64 * wpb), 0, b->stream,
-                       b->dsys, mask, b->qchunks);
-    hipLaunchKernelGGL((k_qq<R>), dim3(B), dim3(64), 0, b->stream, b->dsys, mask);
+    hipLaunchKernelGGL((k_qforces<R, NS>), dim3(xcd_cols((b->qwaves + wpb - 1) / wpb), B), dim3(64 * wpb), 0,
+                       b->stream, b->dsys, mask, b->qchunks);
+    hipLaunchKernelGGL((k_qq<R, NS>), dim3(B), dim3(QQ_THREADS), 0, b->stream, b->dsys, mask);
   }
   if (b->anypol) hipLaunchKernelGGL(k_watpol, dim3(B), dim3(1024), 0, b->stream, b->dsys, (const int*)b->istep);
 }

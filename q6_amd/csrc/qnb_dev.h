@@ -123,6 +123,14 @@ __device__ __forceinline__ bool want_energies(const DevSys& S) {
 
 // ---------------------------------------------------------------- helpers
 
+// load through the global address space: the DevSys arrays are reached
+// through generic pointers, whose loads (flat) the compiler can neither
+// separate from LDS traffic in its wait counts nor batch as freely
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, QNB_WAVE);
@@ -255,9 +263,17 @@ struct PairMath {
 // staged partner atoms and FC_CH receiver atoms per sweep; per-wave LDS:
 // partner offsets and receiver offsets (R x 3), partner index and tag,
 // receiver index
+// threads of the Q-Q kernel's block (one block per system)
+#define QQ_THREADS 256
+
 #define FC_WPB 8
+#ifndef FC_CAP
 #define FC_CAP 384
+#endif
 #define FC_CH 10
+#ifndef FC_OCC_ATTR
+#define FC_OCC_ATTR
+#endif
 template <typename R>
 __host__ __device__ constexpr size_t fc_wave_bytes() {
   return ((sizeof(R) * (3 * (FC_CAP + FC_CH) + 9 * FC_CH) + sizeof(double) * FC_CH +

@@ -382,10 +382,13 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
                                               double xr2, int* s_j, R (*s_d)[3], int* s_t, int* ngroups) {
   const int lane = threadIdx.x % QNB_WAVE;
   const int p = p0 + lane;
-  int b = -1, cnt = 0;
+  int b = -1, cnt = 0, f0 = 0;
   if (p < nn) {
-    b = row[p];
-    if (b < S.ncgp_solute ? use_sol : use_wat) cnt = S.glast[b] - S.gfirst[b] + 1;
+    b = ldg(row + p);
+    if (b < S.ncgp_solute ? use_sol : use_wat) {
+      f0 = ldg(S.gfirst + b);
+      cnt = ldg(S.glast + b) - f0 + 1;
+    }
   }
   int incl = cnt;
 #pragma unroll
@@ -398,17 +401,57 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
   *ngroups = ng;
   if (ng == 0) return 0;
   const int total = __shfl(incl, ng - 1, QNB_WAVE);
+  // every group of the batch skipped by the term mask: nothing staged (the
+  // slots below hold nothing of this batch)
+  if (total == 0) return 0;
+  // the group lanes lay out their slots (LDS only): position in cgpatom and
+  // the group part of the tag
   if (lane < ng && cnt > 0) {
-    const bool b_sol = b < S.ncgp_solute;
-    const int tag0 = (b_sol ? FT_SOL : 0) | (owns(a, b) ? FT_OWN : 0);
-    const int f0 = S.gfirst[b];
+    const int tag0 = (b < S.ncgp_solute ? FT_SOL : 0) | (owns(a, b) ? FT_OWN : 0);
     for (int m = 0, slot = incl - cnt; m < cnt; m++, slot++) {
-      const int j = S.cgpatom[f0 + m];
-      s_j[slot] = j;
-      s_d[slot][0] = (R)(S.x[3 * j] - xr0);
-      s_d[slot][1] = (R)(S.x[3 * j + 1] - xr1);
-      s_d[slot][2] = (R)(S.x[3 * j + 2] - xr2);
-      s_t[slot] = tag0 | (m & 3) | ((b_sol && S.iqatom[j] != 0) ? FT_Q : 0);
+      s_j[slot] = f0 + m;
+      s_t[slot] = tag0 | (m & 3);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  // every lane then fills slots lane, lane + 64, ...: FC_SPR slots per round,
+  // all atom-index loads of the round first, then all position loads, so the
+  // loads of a round are in flight together (slots past the end load the
+  // receiver's first atom, a valid index, and store nothing)
+  constexpr int FC_SPL = (FC_CAP + QNB_WAVE - 1) / QNB_WAVE;
+  constexpr int FC_SPR = 3;
+  static_assert(FC_SPL % FC_SPR == 0, "staging rounds must cover FC_CAP");
+  const int pos_pad = ldg(S.gfirst + a);
+#pragma unroll
+  for (int u0 = 0; u0 < FC_SPL; u0 += FC_SPR) {
+    int jj[FC_SPR], tg[FC_SPR], qa[FC_SPR];
+    double px[FC_SPR][3];
+#pragma unroll
+    for (int u = 0; u < FC_SPR; u++) {
+      const int k = lane + (u0 + u) * QNB_WAVE;
+      const bool in = k < total;
+      tg[u] = in ? s_t[k] : 0;
+      jj[u] = ldg(S.cgpatom + (in ? s_j[k] : pos_pad));
+    }
+#pragma unroll
+    for (int u = 0; u < FC_SPR; u++) {
+      const int j = jj[u];
+      px[u][0] = ldg(S.x + 3 * j);
+      px[u][1] = ldg(S.x + 3 * j + 1);
+      px[u][2] = ldg(S.x + 3 * j + 2);
+      qa[u] = ldg(S.iqatom + j);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < FC_SPR; u++) {
+      const int k = lane + (u0 + u) * QNB_WAVE;
+      if (k < total) {
+        s_j[k] = jj[u];
+        s_d[k][0] = (R)(px[u][0] - xr0);
+        s_d[k][1] = (R)(px[u][1] - xr1);
+        s_d[k][2] = (R)(px[u][2] - xr2);
+        s_t[k] = tg[u] | (((tg[u] & FT_SOL) && qa[u] != 0) ? FT_Q : 0);
+      }
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -421,7 +464,7 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
 // E: false for launches on steps that sample no energies (the host knows);
 // true evaluates the device-side sampling test
 template <typename R, int NS, bool WAT, bool E>
-__global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict__ systems, uint32_t mask,
+__global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys* __restrict__ systems, uint32_t mask,
                                                         const double* __restrict__ tfree) {
   const DevSys& S = systems[blockIdx.y];
   const int wib = threadIdx.x / QNB_WAVE;
@@ -531,7 +574,8 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
               const size_t tt = (size_t)j + (size_t)nsol * k;
               const R ux = dx - o[k][0], uy = dy - o[k][1], uz = dz - o[k][2];
               double ee = 0.0, ev = 0.0;
-              const R dv = PairMath<R>::nbe(ux, uy, uz, (R)S.pwA[tt], (R)S.pwB[tt], (R)S.pwEl[tt], ee, ev, false);
+              const R dv = PairMath<R>::nbe(ux, uy, uz, (R)ldg(S.pwA + tt), (R)ldg(S.pwB + tt), (R)ldg(S.pwEl + tt),
+                                            ee, ev, false);
               f[k][0] += ux * dv;
               f[k][1] += uy * dv;
               f[k][2] += uz * dv;
@@ -634,10 +678,10 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_forces(const DevSys* __restrict
             const R dx = s_pd[t][0], dy = s_pd[t][1], dz = s_pd[t][2];
             PPAtom Q{j, 0, 0u, 0u, 0.0};
             if (b_sol && lds_types) {
-              Q.t = S.iac[j];
-              Q.xm = S.exmask[j];
-              Q.m14 = S.m14mask[j];
-              Q.q = S.crg[j];
+              Q.t = ldg(S.iac + j);
+              Q.xm = ldg(S.exmask + j);
+              Q.m14 = ldg(S.m14mask + j);
+              Q.q = ldg(S.crg + j);
             }
 #pragma unroll
             for (int k = 0; k < CH; k++) {
@@ -873,17 +917,30 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
 // ---------------------------------------------------------------------------
 // Static Q-Q (nonbond_qq, nonbondene.f90:5013-5082, pair kernel nbe_qq
 // nonbonded.f90:111-150) and Q-bonded-neighbour (nonbond_qqp, 5085-5153)
-// lists: one wave per system; both atoms updated with atomics.
-template <typename R>
-__global__ __launch_bounds__(64) void k_qq(const DevSys* __restrict__ systems, uint32_t mask) {
+// lists: one block of QQ_THREADS per system, a thread per pair; both atoms
+// updated with atomics.  Energies per state are summed per wave and then over
+// the waves in order.
+template <int NS>
+__device__ __forceinline__ void qq_add(double (&eq)[NS][4], int s, int c0, double v0, double v1) {
+#pragma unroll
+  for (int k = 0; k < NS; k++)
+    if (k == s) {
+      eq[k][c0] += v0;
+      eq[k][c0 + 1] += v1;
+    }
+}
+
+template <typename R, int NS>
+__global__ __launch_bounds__(QQ_THREADS) void k_qq(const DevSys* __restrict__ systems, uint32_t mask) {
   const DevSys& S = systems[blockIdx.x];
   const int slot = S.ncgp;
-  const int lane = threadIdx.x;
+  const int t = threadIdx.x, lane = t % QNB_WAVE, wid = t / QNB_WAVE;
   const double* __restrict__ x = S.x;
-  double eq[QNB_MAXST][4];
-  for (int s = 0; s < QNB_MAXST; s++) eq[s][0] = eq[s][1] = eq[s][2] = eq[s][3] = 0.0;
+  double eq[NS][4];
+#pragma unroll
+  for (int s = 0; s < NS; s++) eq[s][0] = eq[s][1] = eq[s][2] = eq[s][3] = 0.0;
   if (mask & T_QQ)
-    for (int p = lane; p < S.nqq; p += QNB_WAVE) {
+    for (int p = t; p < S.nqq; p += QQ_THREADS) {
       const int s = S.qq_st[p];
       const int i = S.iqseq[S.qq_iq[p]], j = S.iqseq[S.qq_jq[p]];
       const double* pr = S.qq_par + 4 * p;
@@ -906,8 +963,7 @@ __global__ __launch_bounds__(64) void k_qq(const DevSys* __restrict__ systems, u
         Vb = (R)pr[1] * r6;
         dv = r2 * (-Vel - (R(12) * Va - R(6) * Vb) * r6 * r6_hc) * lam;
       }
-      eq[s][0] += (double)Vel;
-      eq[s][1] += (double)Va - (double)Vb;
+      qq_add<NS>(eq, s, 0, (double)Vel, (double)Va - (double)Vb);
       const double fx = (double)(rux * dv), fy = (double)(ruy * dv), fz = (double)(ruz * dv);
       unsafeAtomicAdd(&S.d[3 * i], -fx);
       unsafeAtomicAdd(&S.d[3 * i + 1], -fy);
@@ -917,7 +973,7 @@ __global__ __launch_bounds__(64) void k_qq(const DevSys* __restrict__ systems, u
       unsafeAtomicAdd(&S.d[3 * j + 2], fz);
     }
   if (mask & T_QQP)
-    for (int p = lane; p < S.nqqp; p += QNB_WAVE) {
+    for (int p = t; p < S.nqqp; p += QQ_THREADS) {
       const int s = S.qqp_st[p];
       const int iq = S.qqp_iq[p];
       const int i = S.iqseq[iq], j = S.qqp_j[p];
@@ -927,8 +983,9 @@ __global__ __launch_bounds__(64) void k_qq(const DevSys* __restrict__ systems, u
       R r2, r;
       rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
       R r6 = r2 * r2 * r2;
-      R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)pr[0], (R)pr[1], (R)pr[2], (R)pr[3], (R)S.lambda[s], eq[s][2],
-                                 eq[s][3]);
+      double el = 0.0, vdw = 0.0;
+      R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)pr[0], (R)pr[1], (R)pr[2], (R)pr[3], (R)S.lambda[s], el, vdw);
+      qq_add<NS>(eq, s, 2, el, vdw);
       const double fx = (double)(rux * dv), fy = (double)(ruy * dv), fz = (double)(ruz * dv);
       unsafeAtomicAdd(&S.d[3 * i], -fx);
       unsafeAtomicAdd(&S.d[3 * i + 1], -fy);
@@ -937,20 +994,26 @@ __global__ __launch_bounds__(64) void k_qq(const DevSys* __restrict__ systems, u
       unsafeAtomicAdd(&S.d[3 * j + 1], fy);
       unsafeAtomicAdd(&S.d[3 * j + 2], fz);
     }
-  double* ep = S.epart + (size_t)slot * QNB_NE;
-  for (int k = 0; k < E_Q0; k++)
-    if (lane == 0) ep[k] = 0.0;
-  for (int s = 0; s < QNB_MAXST; s++) {
-    double v0 = wave_sum(eq[s][0]), v1 = wave_sum(eq[s][1]), v2 = wave_sum(eq[s][2]), v3 = wave_sum(eq[s][3]);
-    if (lane == 0) {
-      ep[E_Q0 + 6 * s + 0] = v0;
-      ep[E_Q0 + 6 * s + 1] = v1;
-      ep[E_Q0 + 6 * s + 2] = v2;
-      ep[E_Q0 + 6 * s + 3] = v3;
-      ep[E_Q0 + 6 * s + 4] = 0.0;
-      ep[E_Q0 + 6 * s + 5] = 0.0;
+  if (!want_energies(S)) return;
+  __shared__ double red[QQ_THREADS / QNB_WAVE][NS][4];
+#pragma unroll
+  for (int s = 0; s < NS; s++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const double v = wave_sum(eq[s][c]);
+      if (lane == 0) red[wid][s][c] = v;
     }
-  }
+  __syncthreads();
+  if (t != 0) return;
+  double* ep = S.epart + (size_t)slot * QNB_NE;
+  for (int k = 0; k < E_Q0; k++) ep[k] = 0.0;
+  for (int k = E_Q0; k < QNB_NE; k++) ep[k] = 0.0;
+  for (int s = 0; s < NS; s++)
+    for (int c = 0; c < 4; c++) {
+      double v = 0.0;
+      for (int w = 0; w < QQ_THREADS / QNB_WAVE; w++) v += red[w][s][c];
+      ep[E_Q0 + 6 * s + c] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1119,9 +1182,6 @@ __global__ __launch_bounds__(1024) void k_watpol(const DevSys* __restrict__ syst
 }
 
 // ---------------------------------------------------------------------------
-// Fixed-order reduction of the energy partials: slots [0, ncgp) from
-// k_forces, plus the qq slot (ncgp) and the watpol slot (ncgp+1) when those
-// kernels ran.
 // Fixed-order reduction of the per-wave energy partials: wave c of the block
 // sums slots [c*n/16, (c+1)*n/16) with lane k on energy k (a wave reads one
 // 57-double slot per iteration), then thread k adds the 16 chunk sums in
@@ -1183,7 +1243,12 @@ QNB_INST(float, 1)
 QNB_INST(float, 2)
 QNB_INST(float, 4)
 QNB_INST(float, 8)
-template __global__ void k_qq<double>(const DevSys*, uint32_t);
-template __global__ void k_qq<float>(const DevSys*, uint32_t);
+#define QNB_INST_QQ(NS)                                           \
+  template __global__ void k_qq<double, NS>(const DevSys*, uint32_t); \
+  template __global__ void k_qq<float, NS>(const DevSys*, uint32_t);
+QNB_INST_QQ(1)
+QNB_INST_QQ(2)
+QNB_INST_QQ(4)
+QNB_INST_QQ(8)
 
 }  // namespace qnb
