@@ -130,6 +130,20 @@ template <typename T>
 __device__ __forceinline__ T ldg(const T* p) {
   return *(const __attribute__((address_space(1))) T*)p;
 }
+template <typename T>
+using gptr_t = const __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr_t<T> gp(const T* p) {
+  return (gptr_t<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gpm(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -142,9 +156,9 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 // q_dist4 (math.f90:254-262): |b - a|^2, ((dx^2 + dy^2) + dz^2), no contraction
 __device__ __forceinline__ double dist4(const double* __restrict__ x, int a, int b) {
 #pragma clang fp contract(off)
-  double dx = x[3 * b] - x[3 * a];
-  double dy = x[3 * b + 1] - x[3 * a + 1];
-  double dz = x[3 * b + 2] - x[3 * a + 2];
+  double dx = ldg(x + 3 * b) - ldg(x + 3 * a);
+  double dy = ldg(x + 3 * b + 1) - ldg(x + 3 * a + 1);
+  double dz = ldg(x + 3 * b + 2) - ldg(x + 3 * a + 2);
   double r = dx * dx;
   r = r + dy * dy;
   r = r + dz * dz;
@@ -170,32 +184,32 @@ __device__ __forceinline__ bool pp_param(const DevSys& S, int i, int j, double& 
   bool is14 = false;
   if (dk <= 25) {
     uint32_t bit = 1u << (dk - 1);
-    if (S.exmask[lo] & bit) return false;
-    is14 = (S.m14mask[lo] & bit) != 0;
+    if (ldg(S.exmask + lo) & bit) return false;
+    is14 = (ldg(S.m14mask + lo) & bit) != 0;
   } else {
     bool ex = false;
-    for (int m = S.lx_off[lo]; m < S.lx_off[lo + 1]; m++)
-      if (S.lx_j[m] == hi) {
-        if (S.lx_kind[m] == 0) ex = true;
+    for (int m = ldg(S.lx_off + lo); m < ldg(S.lx_off + (lo + 1)); m++)
+      if (ldg(S.lx_j + m) == hi) {
+        if (ldg(S.lx_kind + m) == 0) ex = true;
         else is14 = true;
       }
     if (ex) return false;
   }
-  int ti = S.iac[i], tj = S.iac[j];
-  int vdw = is14 ? 3 : S.ljcod[ti + tj * S.ntypes];
+  int ti = ldg(S.iac + i), tj = ldg(S.iac + j);
+  int vdw = is14 ? 3 : ldg(S.ljcod + (ti + tj * S.ntypes));
   int k = vdw - 1;
   if (S.ivdw_rule == 1) {
-    A = S.avdw[ti * 3 + k] * S.avdw[tj * 3 + k];
-    B = S.bvdw[ti * 3 + k] * S.bvdw[tj * 3 + k];
+    A = ldg(S.avdw + (ti * 3 + k)) * ldg(S.avdw + (tj * 3 + k));
+    B = ldg(S.bvdw + (ti * 3 + k)) * ldg(S.bvdw + (tj * 3 + k));
   } else {
-    double tA = S.avdw[ti * 3 + k] + S.avdw[tj * 3 + k];
+    double tA = ldg(S.avdw + (ti * 3 + k)) + ldg(S.avdw + (tj * 3 + k));
     tA = tA * tA;
     tA = tA * tA * tA;
-    double tB = S.bvdw[ti * 3 + k] * S.bvdw[tj * 3 + k];
+    double tB = ldg(S.bvdw + (ti * 3 + k)) * ldg(S.bvdw + (tj * 3 + k));
     A = (tA * tA) * tB;
     B = 2.0 * tA * tB;
   }
-  el = S.crg[i] * S.crg[j];
+  el = ldg(S.crg + i) * ldg(S.crg + j);
   bool set = fabs(A) > 1.0e-10 || fabs(B) > 1.0e-10 || fabs(el) > 1.0e-10;
   if (vdw == 3) el = el * S.el14;
   return set;

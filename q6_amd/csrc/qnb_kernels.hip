@@ -14,7 +14,7 @@
 namespace qnb {
 
 __device__ __forceinline__ int group_ref_atom(const DevSys& S, int g) {
-  return g < S.ncgp_solute ? S.gswitch[g] : S.nat_solute + S.sa * (g - S.ncgp_solute);
+  return g < S.ncgp_solute ? ldg(S.gswitch + g) : S.nat_solute + S.sa * (g - S.ncgp_solute);
 }
 
 // the owner side of a charge-group pair (a != b) under the parity rule
@@ -104,13 +104,14 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
   const bool a_sol = a < S.ncgp_solute;
   // centre of a (cgp_centers, nonbondene.f90:43-78): sum in atom order, then divide
   double cx = 0.0, cy = 0.0, cz = 0.0;
-  for (int k = S.gfirst[a]; k <= S.glast[a]; k++) {
-    int at = S.cgpatom[k];
-    cx = cx + x[3 * at];
-    cy = cy + x[3 * at + 1];
-    cz = cz + x[3 * at + 2];
+  const int ga0 = ldg(S.gfirst + a), ga1 = ldg(S.glast + a);
+  for (int k = ga0; k <= ga1; k++) {
+    int at = ldg(S.cgpatom + k);
+    cx = cx + ldg(x + 3 * at);
+    cy = cy + ldg(x + 3 * at + 1);
+    cz = cz + ldg(x + 3 * at + 2);
   }
-  double nat = (double)(S.glast[a] - S.gfirst[a] + 1);
+  double nat = (double)(ga1 - ga0 + 1);
   cx = cx / nat;
   cy = cy / nat;
   cz = cz / nat;
@@ -119,13 +120,13 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
   for (int k = 0; k < 20; k++) acc[k] = 0.0;
   int count = 0;
   int* __restrict__ row = S.nbr + (size_t)a * S.cap;
-  if (!S.excl[ra]) {
+  if (!ldg(S.excl + ra)) {
     for (int b0 = 0; b0 < S.ncgp; b0 += QNB_WAVE) {
       const int b = b0 + lane;
       bool in = false;
       if (b < S.ncgp && b != a) {
         const int rb = group_ref_atom(S, b);
-        if (!S.excl[rb]) {
+        if (!ldg(S.excl + rb)) {
           const bool b_sol = b < S.ncgp_solute;
           const double cut2 = (a_sol && b_sol) ? S.rcpp2 : ((!a_sol && !b_sol) ? S.rcww2 : S.rcpw2);
           const double r2 = dist4(x, ra, rb);
@@ -136,15 +137,18 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
 #pragma unroll
             for (int k = 0; k < 20; k++) part[k] = R(0);
             if (b_sol) {
-              for (int k = S.gfirst[b]; k <= S.glast[b]; k++) {
-                int i = S.cgpatom[k];
-                if (S.iqatom[i] != 0) continue;
-                lrf_accumulate<R>(part, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
+              const int k1 = ldg(S.glast + b);
+              for (int k = ldg(S.gfirst + b); k <= k1; k++) {
+                const int i = ldg(S.cgpatom + k);
+                if (ldg(S.iqatom + i) != 0) continue;
+                lrf_accumulate<R>(part, ldg(x + 3 * i) - cx, ldg(x + 3 * i + 1) - cy, ldg(x + 3 * i + 2) - cz,
+                                  ldg(S.crg + i));
               }
             } else {
               for (int k = 0; k < S.sa; k++) {
-                int i = rb + k;
-                lrf_accumulate<R>(part, x[3 * i] - cx, x[3 * i + 1] - cy, x[3 * i + 2] - cz, S.crg[i]);
+                const int i = rb + k;
+                lrf_accumulate<R>(part, ldg(x + 3 * i) - cx, ldg(x + 3 * i + 1) - cy, ldg(x + 3 * i + 2) - cz,
+                                  ldg(S.crg + i));
               }
             }
 #pragma unroll
@@ -154,7 +158,7 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
       }
       const uint64_t m = __ballot(in);
       const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
-      if (in && pos < S.cap) row[pos] = b;
+      if (in && pos < S.cap) stg(row + pos, b);
       count += __popcll(m);
     }
   }
@@ -246,21 +250,21 @@ __global__ __launch_bounds__(1024) void k_qlists(const DevSys* __restrict__ syst
 // ---------------------------------------------------------------------------
 // LRF Taylor application for one atom (lrf_taylor, nonbondene.f90:507-571)
 __device__ __forceinline__ void lrf_taylor_atom(const DevSys& S, int i, double* f, double& e) {
-  const double* L = S.lrf + (size_t)QNB_NLRF * S.atom_group[i];
-  const double* x = S.x;
+  const auto L = gp(S.lrf + (size_t)QNB_NLRF * ldg(S.atom_group + i));
+  const auto x = gp(S.x);
   const double drx = L[0] - x[3 * i], dry = L[1] - x[3 * i + 1], drz = L[2] - x[3 * i + 2];
-  const double* p1 = L + 4;
-  const double* p2 = L + 7;
-  const double* p3 = L + 16;
+  const auto p1 = L + 4;
+  const auto p2 = L + 7;
+  const auto p3 = L + 16;
   double t0 = drx * p2[0] + dry * p2[1] + drz * p2[2];
   double t1 = drx * p2[3] + dry * p2[4] + drz * p2[5];
   double t2 = drx * p2[6] + dry * p2[7] + drz * p2[8];
   double V = L[3] + (drx * p1[0] + dry * p1[1] + drz * p1[2]) + 0.5 * (drx * t0 + dry * t1 + drz * t2);
-  const double q = S.crg[i];
+  const double q = ldg(S.crg + i);
   e += 0.5 * q * V;
   double df[3];
   for (int k = 0; k < 3; k++) {
-    const double* P = p3 + 9 * k;
+    const auto P = p3 + 9 * k;
     double u0 = drx * P[0] + dry * P[1] + drz * P[2];
     double u1 = drx * P[3] + dry * P[4] + drz * P[5];
     double u2 = drx * P[6] + dry * P[7] + drz * P[8];
@@ -275,7 +279,7 @@ __device__ __forceinline__ void lrf_taylor_atom(const DevSys& S, int i, double* 
 // SCAAS radial restraint on one water oxygen (restrain_solvent,
 // nonbondene.f90:6466-6544, 3-site branch)
 __device__ __forceinline__ void radial_restraint(const DevSys& S, int i, double shift, double* f, double& e) {
-  const double* x = S.x;
+  const auto x = gp(S.x);
   double vx = x[3 * i] - S.xw[0], vy = x[3 * i + 1] - S.xw[1], vz = x[3 * i + 2] - S.xw[2];
   double b = sqrt(vx * vx + vy * vy + vz * vz);
   double db = b - (S.rwat - shift);
@@ -350,9 +354,9 @@ __device__ __forceinline__ bool pp_param_fast(const DevSys& S, const PPAtom& P, 
   } else if (xm >> 31) {
     const int lo = pl ? P.i : Q.i, hi = pl ? Q.i : P.i;
     bool ex = false;
-    for (int m = S.lx_off[lo]; m < S.lx_off[lo + 1]; m++)
-      if (S.lx_j[m] == hi) {
-        if (S.lx_kind[m] == 0) ex = true;
+    for (int m = ldg(S.lx_off + lo); m < ldg(S.lx_off + (lo + 1)); m++)
+      if (ldg(S.lx_j + m) == hi) {
+        if (ldg(S.lx_kind + m) == 0) ex = true;
         else is14 = true;
       }
     if (ex) return false;
@@ -478,10 +482,10 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
     if (lds_types) {
       const int nt = S.ntypes;
       for (int k = threadIdx.x; k < 3 * nt; k += blockDim.x) {
-        s_avdw[k] = S.avdw[k];
-        s_bvdw[k] = S.bvdw[k];
+        s_avdw[k] = ldg(S.avdw + k);
+        s_bvdw[k] = ldg(S.bvdw + k);
       }
-      for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) s_ljc[k] = (uint8_t)S.ljcod[k];
+      for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) s_ljc[k] = (uint8_t)ldg(S.ljcod + k);
     }
     __syncthreads();
   }
@@ -501,22 +505,22 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
   uint32_t* s_rx = reinterpret_cast<uint32_t*>(s_rt + FC_CH);
   uint32_t* s_rm = s_rx + FC_CH;
   const double* __restrict__ x = S.x;
-  double* __restrict__ d = S.d;
-  const int nn = min(S.nnbr[a], S.cap);
+  const auto d = gpm(S.d);
+  const int nn = min(ldg(S.nnbr + a), S.cap);
   const int* __restrict__ row = S.nbr + (size_t)a * S.cap;
   const bool want_e = E && want_energies(S);
   const int nsol = S.nat_solute, sa = S.sa, ncs = S.ncgp_solute;
   const bool a_sol = a < ncs;
   const int ra = group_ref_atom(S, a);
-  const bool aexcl = S.excl[ra];
+  const bool aexcl = ldg(S.excl + ra);
   double e[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) e[k] = 0.0;
   double eq[4 * NS];  // per state: qp_el, qp_vdw, qw_el, qw_vdw
 #pragma unroll
   for (int k = 0; k < 4 * NS; k++) eq[k] = 0.0;
-  const double xr0 = x[3 * ra], xr1 = x[3 * ra + 1], xr2 = x[3 * ra + 2];
-  const int g0 = S.gfirst[a], na = S.glast[a] - g0 + 1;
+  const double xr0 = ldg(x + (3 * ra)), xr1 = ldg(x + (3 * ra + 1)), xr2 = ldg(x + (3 * ra + 2));
+  const int g0 = ldg(S.gfirst + a), na = ldg(S.glast + a) - g0 + 1;
   const int nq = S.nqat, ns = S.nstates;
 
   if constexpr (WAT) {
@@ -525,14 +529,14 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
     // k with partner atom kb, and the O-O LJ pair
     R wel[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) wel[k] = (R)S.wwEl[k];
-    const R wA = (R)S.wwA[0], wB = (R)S.wwB[0];
+    for (int k = 0; k < 9; k++) wel[k] = (R)ldg(S.wwEl + k);
+    const R wA = (R)ldg(S.wwA + 0), wB = (R)ldg(S.wwB + 0);
     R o[3][3], f[3][3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      o[k][0] = (R)(x[3 * (ra + k)] - xr0);
-      o[k][1] = (R)(x[3 * (ra + k) + 1] - xr1);
-      o[k][2] = (R)(x[3 * (ra + k) + 2] - xr2);
+      o[k][0] = (R)(ldg(x + (3 * (ra + k))) - xr0);
+      o[k][1] = (R)(ldg(x + (3 * (ra + k) + 1)) - xr1);
+      o[k][2] = (R)(ldg(x + (3 * (ra + k) + 2)) - xr2);
       f[k][0] = f[k][1] = f[k][2] = R(0);
     }
     if (!aexcl) {
@@ -559,7 +563,8 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
                 else
                   dv = PairMath<R>::nbe_spc(ux, uy, uz, q, e[E_WW_EL], acc);
               } else {
-                dv = PairMath<R>::nbe(ux, uy, uz, (R)S.wwA[tt], (R)S.wwB[tt], (R)S.wwEl[tt], e[E_WW_EL],
+                dv = PairMath<R>::nbe(ux, uy, uz, (R)ldg(S.wwA + tt), (R)ldg(S.wwB + tt), (R)ldg(S.wwEl + tt),
+                                      e[E_WW_EL],
                                       e[E_WW_VDW], acc);
               }
               f[k][0] += ux * dv;
@@ -587,10 +592,10 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
     }
     // Q-atom - water, receiver side (nonbond_qw_spc nonbondene.f90:5658-5767;
     // nonbond_qw 5339-5418)
-    if ((mask & T_QW) && nq > 0 && S.qw_in[a - ncs]) {
+    if ((mask & T_QW) && nq > 0 && ldg(S.qw_in + (a - ncs))) {
       for (int iq = lane; iq < nq; iq += QNB_WAVE) {
-        const int q = S.iqseq[iq];
-        const R qx = (R)(x[3 * q] - xr0), qy = (R)(x[3 * q + 1] - xr1), qz = (R)(x[3 * q + 2] - xr2);
+        const int q = ldg(S.iqseq + iq);
+        const R qx = (R)(ldg(x + 3 * q) - xr0), qy = (R)(ldg(x + 3 * q + 1) - xr1), qz = (R)(ldg(x + 3 * q + 2) - xr2);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           // u = x_q - x_i, the partner-minus-receiver convention of the pair terms
@@ -604,7 +609,8 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
             for (int s = 0; s < NS; s++) {
               if (s >= ns) break;
               const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
-              dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)S.qwA[t], (R)S.qwB[t], (R)S.qwEl[t], (R)S.qwSc[t],
+              dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)ldg(S.qwA + t), (R)ldg(S.qwB + t), (R)ldg(S.qwEl + t),
+                                         (R)ldg(S.qwSc + t),
                                          (R)S.lambda[s], eq[4 * s + 2], eq[4 * s + 3], want_e);
             }
           } else {
@@ -612,7 +618,7 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
             for (int s = 0; s < NS; s++) {
               if (s >= ns) break;
               const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
-              const R Vel = (R)S.qwEl[t] * r;  // nbe_qspc (nonbonded.f90:226-243)
+              const R Vel = (R)ldg(S.qwEl + t) * r;  // nbe_qspc (nonbonded.f90:226-243)
               if (want_e) eq[4 * s + 2] += (double)Vel;
               dvs += -r2 * Vel * (R)S.lambda[s];
             }
@@ -636,29 +642,29 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
     }
   } else {
     // ------------------------------------------------ solute receiver
-    const bool qin = (mask & T_QP) && nq > 0 && S.qp_in[a];
+    const bool qin = (mask & T_QP) && nq > 0 && ldg(S.qp_in + a);
     constexpr int CH = sizeof(R) == 4 ? FC_CH : FC_CH / 2;
     for (int c0 = 0; c0 < na; c0 += CH) {
       const int nk = min(CH, na - c0);
       // receiver atoms of this sweep in LDS: index (-1: takes no pair terms)
       // and offset from the reference atom
       if (lane < nk) {
-        const int i = S.cgpatom[g0 + c0 + lane];
-        s_ia[lane] = (!aexcl && S.iqatom[i] == 0) ? i : -1;
-        s_o[lane][0] = (R)(x[3 * i] - xr0);
-        s_o[lane][1] = (R)(x[3 * i + 1] - xr1);
-        s_o[lane][2] = (R)(x[3 * i + 2] - xr2);
-        s_rt[lane] = S.iac[i];
-        s_rx[lane] = S.exmask[i];
-        s_rm[lane] = S.m14mask[i];
-        s_rq[lane] = S.crg[i];
+        const int i = ldg(S.cgpatom + (g0 + c0 + lane));
+        s_ia[lane] = (!aexcl && ldg(S.iqatom + i) == 0) ? i : -1;
+        s_o[lane][0] = (R)(ldg(x + (3 * i)) - xr0);
+        s_o[lane][1] = (R)(ldg(x + (3 * i + 1)) - xr1);
+        s_o[lane][2] = (R)(ldg(x + (3 * i + 2)) - xr2);
+        s_rt[lane] = ldg(S.iac + i);
+        s_rx[lane] = ldg(S.exmask + i);
+        s_rm[lane] = ldg(S.m14mask + i);
+        s_rq[lane] = ldg(S.crg + i);
         // the atom's solute-water parameters against the 3 water sites
         if (S.nwat > 0)
           for (int kb = 0; kb < 3; kb++) {
             const size_t tt = (size_t)i + (size_t)nsol * kb;
-            s_pw[lane][kb][0] = (R)S.pwA[tt];
-            s_pw[lane][kb][1] = (R)S.pwB[tt];
-            s_pw[lane][kb][2] = (R)S.pwEl[tt];
+            s_pw[lane][kb][0] = (R)ldg(S.pwA + tt);
+            s_pw[lane][kb][1] = (R)ldg(S.pwB + tt);
+            s_pw[lane][kb][2] = (R)ldg(S.pwEl + tt);
           }
       }
       __builtin_amdgcn_wave_barrier();
@@ -713,9 +719,10 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
         // pairs inside the group (ig == jgr, i < j, nonbondene.f90:1610)
         if (mask & T_PP) {
           for (int q = lane; q < na; q += QNB_WAVE) {
-            const int j = S.cgpatom[g0 + q];
-            if (S.iqatom[j] != 0) continue;
-            const R dx = (R)(x[3 * j] - xr0), dy = (R)(x[3 * j + 1] - xr1), dz = (R)(x[3 * j + 2] - xr2);
+            const int j = ldg(S.cgpatom + (g0 + q));
+            if (ldg(S.iqatom + j) != 0) continue;
+            const R dx = (R)(ldg(x + 3 * j) - xr0), dy = (R)(ldg(x + 3 * j + 1) - xr1),
+                    dz = (R)(ldg(x + 3 * j + 2) - xr2);
 #pragma unroll
             for (int k = 0; k < CH; k++) {
               if (k >= nk) break;
@@ -735,13 +742,14 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
       // Q-atom - solute atom, receiver side (nonbond_qp, nonbondene.f90:5157-5230)
       if (qin) {
         for (int iq = lane; iq < nq; iq += QNB_WAVE) {
-          const int q = S.iqseq[iq];
-          const R qx = (R)(x[3 * q] - xr0), qy = (R)(x[3 * q + 1] - xr1), qz = (R)(x[3 * q + 2] - xr2);
+          const int q = ldg(S.iqseq + iq);
+          const R qx = (R)(ldg(x + 3 * q) - xr0), qy = (R)(ldg(x + 3 * q + 1) - xr1),
+                  qz = (R)(ldg(x + 3 * q + 2) - xr2);
 #pragma unroll
           for (int k = 0; k < CH; k++) {
             if (k >= nk) break;
             const int i = s_ia[k];
-            if (i < 0 || S.qconn3[i]) continue;
+            if (i < 0 || ldg(S.qconn3 + i)) continue;
             const R ux = qx - s_o[k][0], uy = qy - s_o[k][1], uz = qz - s_o[k][2];
             R r2, r;
             rinv(ux * ux + uy * uy + uz * uz, r2, r);
@@ -751,7 +759,8 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
             for (int s = 0; s < NS; s++) {
               if (s >= ns) break;
               const size_t t = (size_t)i + (size_t)nsol * ((size_t)iq + (size_t)nq * s);
-              dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)S.qpA[t], (R)S.qpB[t], (R)S.qpEl[t], (R)S.qpSc[t],
+              dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)ldg(S.qpA + t), (R)ldg(S.qpB + t), (R)ldg(S.qpEl + t),
+                                         (R)ldg(S.qpSc + t),
                                          (R)S.lambda[s], eq[4 * s + 0], eq[4 * s + 1], want_e);
             }
             f[k][0] += ux * dvs;
@@ -766,7 +775,7 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
         if (k >= nk) break;
         const double fx = wave_sum((double)f[k][0]), fy = wave_sum((double)f[k][1]), fz = wave_sum((double)f[k][2]);
         if (lane == k) {
-          const int i = S.cgpatom[g0 + c0 + k];
+          const int i = ldg(S.cgpatom + (g0 + c0 + k));
           d[3 * i] = -fx;
           d[3 * i + 1] = -fy;
           d[3 * i + 2] = -fz;
@@ -815,19 +824,20 @@ __global__ __launch_bounds__(256) void k_atom_terms(const DevSys* __restrict__ s
   if (i >= S.natom) return;
   double f[3] = {0.0, 0.0, 0.0};
   double el = 0.0, er = 0.0;
-  if ((mask & T_LRF) && S.use_lrf && !S.excl[i] && S.iqatom[i] == 0) lrf_taylor_atom(S, i, f, el);
+  if ((mask & T_LRF) && S.use_lrf && !ldg(S.excl + i) && ldg(S.iqatom + i) == 0) lrf_taylor_atom(S, i, f, el);
   const int w = i - S.nat_solute;
-  if ((mask & T_RAD) && w >= 0 && w % S.sa == 0 && S.sa == 3 && !S.excl[i]) {
+  if ((mask & T_RAD) && w >= 0 && w % S.sa == 0 && S.sa == 3 && !ldg(S.excl + i)) {
     const double T = tfree[blockIdx.y];
     const double shift = S.fk_wsphere != 0.0 ? sqrt(S.Boltz * T / S.fk_wsphere) : 0.0;
     radial_restraint(S, i, shift, f, er);
   }
-  S.d[3 * i] += f[0];
-  S.d[3 * i + 1] += f[1];
-  S.d[3 * i + 2] += f[2];
+  const auto d = gpm(S.d);
+  d[3 * i] += f[0];
+  d[3 * i + 1] += f[1];
+  d[3 * i + 2] += f[2];
   if (want_energies(S)) {
-    S.eatom[2 * i] = el;
-    S.eatom[2 * i + 1] = er;
+    stg(S.eatom + 2 * i, el);
+    stg(S.eatom + 2 * i + 1, er);
   }
 }
 
@@ -842,11 +852,11 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
   const int lane = threadIdx.x % QNB_WAVE;
   const int iq = wv / chunks_per_q, ch = wv % chunks_per_q;
   if (iq >= S.nqat) return;
-  const int nqw = S.nq_lists[0], nqp = S.nq_lists[1];
+  const int nqw = ldg(S.nq_lists + 0), nqp = ldg(S.nq_lists + 1);
   const int nw_ch = (mask & T_QW) ? (nqw + QNB_WAVE - 1) / QNB_WAVE : 0;
   const int np_ch = (mask & T_QP) ? (nqp + QNB_WAVE - 1) / QNB_WAVE : 0;
   const double* __restrict__ x = S.x;
-  const int i = S.iqseq[iq];
+  const int i = ldg(S.iqseq + iq);
   const int nq = S.nqat, ns = S.nstates, sa = S.sa;
   double f[3] = {0.0, 0.0, 0.0};
   double dummy_e = 0.0, dummy_v = 0.0;
@@ -854,10 +864,11 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
     const int p = c * QNB_WAVE + lane;
     if (c < nw_ch) {
       if (p >= nqw) continue;
-      const int o = S.nat_solute + sa * S.qw_list[p];
+      const int o = S.nat_solute + sa * ldg(S.qw_list + p);
       for (int k = 0; k < sa; k++) {
         const int j = o + k;
-        const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
+        const double ux = ldg(x + 3 * j) - ldg(x + 3 * i), uy = ldg(x + 3 * j + 1) - ldg(x + 3 * i + 1),
+                     uz = ldg(x + 3 * j + 2) - ldg(x + 3 * i + 2);
         const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
         R r2, r;
         rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
@@ -868,7 +879,8 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
           for (int s = 0; s < NS; s++) {
             if (s >= ns) break;
             const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
-            dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)S.qwA[t], (R)S.qwB[t], (R)S.qwEl[t], (R)S.qwSc[t],
+            dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)ldg(S.qwA + t), (R)ldg(S.qwB + t), (R)ldg(S.qwEl + t),
+                                       (R)ldg(S.qwSc + t),
                                        (R)S.lambda[s], dummy_e, dummy_v, false);
           }
         } else {
@@ -876,7 +888,7 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
           for (int s = 0; s < NS; s++) {
             if (s >= ns) break;
             const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
-            dvs += -r2 * ((R)S.qwEl[t] * r) * (R)S.lambda[s];
+            dvs += -r2 * ((R)ldg(S.qwEl + t) * r) * (R)S.lambda[s];
           }
         }
         f[0] -= (double)(rux * dvs);
@@ -886,8 +898,9 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
     } else {
       const int pp = p - nw_ch * QNB_WAVE;
       if (pp >= nqp) continue;
-      const int j = S.qp_list[pp];
-      const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
+      const int j = ldg(S.qp_list + pp);
+      const double ux = ldg(x + 3 * j) - ldg(x + 3 * i), uy = ldg(x + 3 * j + 1) - ldg(x + 3 * i + 1),
+                   uz = ldg(x + 3 * j + 2) - ldg(x + 3 * i + 2);
       const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
       R r2, r;
       rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
@@ -896,7 +909,8 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
       for (int s = 0; s < NS; s++) {
         if (s >= ns) break;
         const size_t t = (size_t)j + (size_t)S.nat_solute * ((size_t)iq + (size_t)nq * s);
-        R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)S.qpA[t], (R)S.qpB[t], (R)S.qpEl[t], (R)S.qpSc[t],
+        R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)ldg(S.qpA + t), (R)ldg(S.qpB + t), (R)ldg(S.qpEl + t),
+                                   (R)ldg(S.qpSc + t),
                                    (R)S.lambda[s], dummy_e, dummy_v, false);
         f[0] -= (double)(rux * dv);
         f[1] -= (double)(ruy * dv);
