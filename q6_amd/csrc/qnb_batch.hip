@@ -24,7 +24,7 @@ template <typename R, int NS>
 __global__ void k_qforces(const DevSys*, uint32_t, int);
 template <typename R, int NS>
 __global__ void k_qq(const DevSys*, uint32_t);
-__global__ void k_watpol(const DevSys*, const int*);
+void launch_watpol(hipStream_t st, int nblk, int B, const DevSys* dsys, const int* istep);
 __global__ void k_reduce_energies(const DevSys*, int, int);
 __global__ void k_atom_terms(const DevSys*, uint32_t, const double*);
 
@@ -400,7 +400,7 @@ static void enqueue_forces(qnb_batch* b, bool energy = true) {
                        b->stream, b->dsys, mask, b->qchunks);
     hipLaunchKernelGGL((k_qq<R, NS>), dim3(B), dim3(QQ_THREADS), 0, b->stream, b->dsys, mask);
   }
-  if (b->anypol) hipLaunchKernelGGL(k_watpol, dim3(B), dim3(1024), 0, b->stream, b->dsys, (const int*)b->istep);
+  if (b->anypol) launch_watpol(b->stream, (b->maxw + WP_THREADS - 1) / WP_THREADS, B, b->dsys, (const int*)b->istep);
 }
 
 template <typename R>

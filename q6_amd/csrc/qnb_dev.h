@@ -108,9 +108,12 @@ struct DevSys {
   double* sh_state;  // [3*QNB_MAXSH]: theta_corr, avtheta, avn_insh
   double* w_theta;   // [nwat]
   int* w_shell;      // [nwat]
-  int* sh_list;      // [QNB_MAXSH][nwat]
-  int* sh_rank;      // [QNB_MAXSH][nwat]
-  int* sh_n;         // [QNB_MAXSH]
+  int* w_rank;       // [nwat] rank of the water in its shell
+  int* sh_list;      // [QNB_MAXSH][nwat] members of each shell
+  double* sh_lth;    // [QNB_MAXSH][nwat] their theta
+  int* sh_n;         // [QNB_MAXSH] members per shell
+  double* wp_part;   // [ceil(nwat / WP_THREADS)][1 + QNB_MAXSH] per-block sums
+  int* wp_ticket;    // [1] blocks done (reset by the last one)
   // energy sampling: energies are accumulated when ene_every <= 0 or when
   // the instance's step counter is a multiple of ene_every
   const int* step_ptr;
@@ -279,6 +282,8 @@ struct PairMath {
 // receiver index
 // threads of the Q-Q kernel's block (one block per system)
 #define QQ_THREADS 256
+// threads per block of the polarisation-restraint kernels (one per water)
+#define WP_THREADS 256
 
 #define FC_WPB 8
 #ifndef FC_CAP

@@ -1048,151 +1048,227 @@ __device__ __forceinline__ void wat_vectors(const DevSys& S, int i, double* rmu,
   for (int c = 0; c < 3; c++) rcu[c] = v[c] / rc;
 }
 
-__global__ __launch_bounds__(1024) void k_watpol(const DevSys* __restrict__ systems, const int* __restrict__ istep) {
-  const DevSys& S = systems[blockIdx.x];
-  const int slot = S.ncgp + 1;
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int nw = S.nwat, nsh = S.nshell;
-  const double pi = 3.141592653589793;
-  __shared__ int cnt[QNB_MAXSH];
-  __shared__ double red[1024 / QNB_WAVE][2 * QNB_MAXSH + 1];
-  if (t < QNB_MAXSH) cnt[t] = 0;
-  __syncthreads();
-  // theta and shell of every water, in water order per shell (atomic slot,
-  // then ordered by rank over (theta, index) so slot order does not matter)
-  for (int w = t; w < nw; w += nt) {
-    const int i = S.nat_solute + S.sa * w;
-    int sh = -1;
-    double th = 0.0;
-    if (!S.excl[i]) {
-      double rmu[3], rcu[3], rm, rc;
-      wat_vectors(S, i, rmu, rm, rcu, rc);
-      double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
-      scp = scp > 1.0 ? 1.0 : (scp < -1.0 ? -1.0 : scp);
-      th = acos(scp);
-      if (rc > S.sh_rout[nsh - 1] - S.sh_dr[nsh - 1]) {
-        int is;
-        for (is = nsh - 1; is >= 1; is--)
-          if (rc <= S.sh_rout[is]) break;
-        if (is < 1) is = 0;
-        sh = is;
+__device__ __forceinline__ bool has_pol(const DevSys& S) { return S.nwat > 0 && S.wpol && S.nshell > 0; }
+
+// Phase 1, one thread per water: theta and shell (-1: inside the innermost
+// shell, or excluded); the members of each shell are appended to its list
+// (slot order is immaterial: ranks are counted by value).  Block 0 first
+// applies the theta_corr feedback of the steps that are multiples of itdis
+// (nonbondene.f90:6653-6666).  The shell counts start at zero (set at
+// creation and cleared by the last block of phase 3).
+__global__ __launch_bounds__(WP_THREADS) void k_watpol_theta(const DevSys* __restrict__ systems,
+                                                             const int* __restrict__ istep) {
+  const DevSys& S = systems[blockIdx.y];
+  if (!has_pol(S)) return;
+  const int t = threadIdx.x, nsh = S.nshell, nw = S.nwat;
+  if (blockIdx.x == 0 && t == 0) {
+    double* st = S.sh_state;  // theta_corr[8], avtheta[8], avn_insh[8]
+    const int step = istep[blockIdx.y];
+    const int itd = (int)llround(S.itdis);
+    if (step > 0 && itd > 0 && step % itd == 0) {
+      for (int is = 0; is < nsh; is++) {
+        st[QNB_MAXSH + is] = st[QNB_MAXSH + is] / S.itdis;
+        st[2 * QNB_MAXSH + is] = st[2 * QNB_MAXSH + is] / S.itdis;
+        st[is] = st[is] + st[QNB_MAXSH + is] - acos(S.sh_cstb[is]);
+        st[QNB_MAXSH + is] = 0.0;
+        st[2 * QNB_MAXSH + is] = 0.0;
       }
     }
-    S.w_theta[w] = th;
-    S.w_shell[w] = sh;
-    if (sh >= 0) {
-      int pos = atomicAdd(&cnt[sh], 1);
-      S.sh_list[(size_t)sh * nw + pos] = w;
+  }
+  const int w = blockIdx.x * WP_THREADS + t;
+  if (w >= nw) return;
+  const int i = S.nat_solute + S.sa * w;
+  int sh = -1;
+  double th = 0.0;
+  if (!ldg(S.excl + i)) {
+    double rmu[3], rcu[3], rm, rc;
+    wat_vectors(S, i, rmu, rm, rcu, rc);
+    double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+    scp = scp > 1.0 ? 1.0 : (scp < -1.0 ? -1.0 : scp);
+    th = acos(scp);
+    if (rc > S.sh_rout[nsh - 1] - S.sh_dr[nsh - 1]) {
+      int is;
+      for (is = nsh - 1; is >= 1; is--)
+        if (rc <= S.sh_rout[is]) break;
+      if (is < 1) is = 0;
+      sh = is;
     }
   }
-  __syncthreads();
-  // rank of each member inside its shell; the comparands are staged through
-  // LDS in chunks of WP_CHUNK (theta and water index)
-  constexpr int WP_CHUNK = 1024;
-  __shared__ double c_th[WP_CHUNK];
-  __shared__ int c_w[WP_CHUNK];
-  for (int sh = 0; sh < nsh; sh++) {
-    const int n = cnt[sh];
-    const int* L = S.sh_list + (size_t)sh * nw;
-    for (int m0 = 0; m0 < n; m0 += nt) {
-      const int m = m0 + t;
-      const int w = m < n ? L[m] : -1;
-      const double th = m < n ? S.w_theta[w] : 0.0;
-      int r = 0;
-      for (int k0 = 0; k0 < n; k0 += WP_CHUNK) {
-        const int nk = min(WP_CHUNK, n - k0);
-        __syncthreads();
-        for (int k = t; k < nk; k += nt) {
-          const int w2 = L[k0 + k];
-          c_w[k] = w2;
-          c_th[k] = S.w_theta[w2];
+  stg(S.w_theta + w, th);
+  stg(S.w_shell + w, sh);
+  if (sh >= 0) {
+    const int pos = atomicAdd(S.sh_n + sh, 1);
+    stg(S.sh_list + (size_t)sh * nw + pos, w);
+    stg(S.sh_lth + (size_t)sh * nw + pos, th);
+  }
+}
+
+// Phase 2, one block per (shell, system): the rank of every member in the
+// shell's stable ascending order of theta (the reference's selection sort;
+// ties by water index).  Shells of up to WP_SORT members are sorted by
+// (theta, water index) with a bitonic network in LDS and the rank is the
+// sorted position; larger shells count, for every member, the members
+// before it (staged through LDS in chunks of WP_SORT).
+#define WP_RANK_THREADS 1024
+#define WP_SORT 4096
+__global__ __launch_bounds__(WP_RANK_THREADS) void k_watpol_rank(const DevSys* __restrict__ systems) {
+  const DevSys& S = systems[blockIdx.y];
+  const int sh = blockIdx.x;
+  if (!has_pol(S) || sh >= S.nshell) return;
+  __shared__ double c_th[WP_SORT];
+  __shared__ int c_w[WP_SORT];
+  const int t = threadIdx.x, nw = S.nwat;
+  const int n = S.sh_n[sh];
+  const int* L = S.sh_list + (size_t)sh * nw;
+  const double* Lth = S.sh_lth + (size_t)sh * nw;
+  if (n <= WP_SORT) {
+    int N = 2;
+    while (N < n) N <<= 1;
+    for (int k = t; k < N; k += WP_RANK_THREADS) {
+      c_th[k] = k < n ? ldg(Lth + k) : __builtin_inf();
+      c_w[k] = k < n ? ldg(L + k) : 0x7fffffff;
+    }
+    __syncthreads();
+    for (int k = 2; k <= N; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = t; i < N; i += WP_RANK_THREADS) {
+          const int l = i ^ j;
+          if (l > i) {
+            const double ti = c_th[i], tl = c_th[l];
+            const int wi = c_w[i], wl = c_w[l];
+            const bool gt = ti > tl || (ti == tl && wi > wl);  // (i) after (l)
+            if (((i & k) == 0) == gt) {
+              c_th[i] = tl;
+              c_th[l] = ti;
+              c_w[i] = wl;
+              c_w[l] = wi;
+            }
+          }
         }
         __syncthreads();
-        if (m < n)
-          for (int k = 0; k < nk; k++) {
-            const double t2 = c_th[k];
-            const int w2 = c_w[k];
-            r += (t2 < th) || (t2 == th && w2 < w);
-          }
       }
-      if (m < n) S.sh_rank[(size_t)sh * nw + m] = r;
-    }
+    for (int p = t; p < n; p += WP_RANK_THREADS) stg(S.w_rank + c_w[p], p);
+    return;
   }
-  // theta_corr feedback every itdis steps (nonbondene.f90:6653-6666)
-  double* st = S.sh_state;  // theta_corr[8], avtheta[8], avn_insh[8]
-  const int step = istep[blockIdx.x];
-  const int itd = (int)llround(S.itdis);
-  if (t == 0 && step > 0 && itd > 0 && step % itd == 0) {
-    for (int is = 0; is < nsh; is++) {
-      st[QNB_MAXSH + is] = st[QNB_MAXSH + is] / S.itdis;
-      st[2 * QNB_MAXSH + is] = st[2 * QNB_MAXSH + is] / S.itdis;
-      st[is] = st[is] + st[QNB_MAXSH + is] - acos(S.sh_cstb[is]);
-      st[QNB_MAXSH + is] = 0.0;
-      st[2 * QNB_MAXSH + is] = 0.0;
+  for (int m0 = 0; m0 < n; m0 += WP_RANK_THREADS) {
+    const int m = m0 + t;
+    const int w = m < n ? ldg(L + m) : -1;
+    const double th = m < n ? ldg(Lth + m) : 0.0;
+    int r = 0;
+    for (int k0 = 0; k0 < n; k0 += WP_SORT) {
+      const int nk = min(WP_SORT, n - k0);
+      __syncthreads();
+      for (int k = t; k < nk; k += WP_RANK_THREADS) {
+        c_w[k] = ldg(L + k0 + k);
+        c_th[k] = ldg(Lth + k0 + k);
+      }
+      __syncthreads();
+      if (m < n)
+        for (int k = 0; k < nk; k++) {
+          const double t2 = c_th[k];
+          r += (t2 < th) || (t2 == th && c_w[k] < w);
+        }
     }
+    if (m < n) stg(S.w_rank + w, r);
   }
-  __syncthreads();
-  double epol = 0.0;
-  double avt[QNB_MAXSH];
-  for (int is = 0; is < QNB_MAXSH; is++) avt[is] = 0.0;
-  for (int sh = 0; sh < nsh; sh++) {
-    const int n = cnt[sh];
+}
+
+// Phase 3, one thread per water of a shell: target angle from the rank, the
+// restraint energy and forces (adds to d), and per-block sums of the energy
+// and of theta per shell.  The last block of a system to finish (ticket)
+// adds the block sums in block order, accumulates avtheta and avn_insh and
+// writes the energy slot; it resets the ticket for the next launch.
+__global__ __launch_bounds__(WP_THREADS) void k_watpol_force(const DevSys* __restrict__ systems) {
+  const DevSys& S = systems[blockIdx.y];
+  const int slot = S.ncgp + 1;
+  const int t = threadIdx.x, lane = t % QNB_WAVE, wid = t / QNB_WAVE;
+  if (!has_pol(S)) {
+    if (blockIdx.x == 0 && t < QNB_NE) S.epart[(size_t)slot * QNB_NE + t] = 0.0;
+    return;
+  }
+  const int nsh = S.nshell, nw = S.nwat;
+  // blocks of this system (the grid is sized for the batch's largest)
+  const int nblk = (nw + WP_THREADS - 1) / WP_THREADS;
+  if ((int)blockIdx.x >= nblk) return;
+  const double pi = 3.141592653589793;
+  const int w = blockIdx.x * WP_THREADS + t;
+  const int sh = w < nw ? ldg(S.w_shell + w) : -1;
+  double epol = 0.0, th = 0.0;
+  if (sh >= 0) {
+    const double* st = S.sh_state;
+    const int n = S.sh_n[sh];
+    const int il = ldg(S.w_rank + w) + 1;
+    th = ldg(S.w_theta + w);
     const double tc = st[sh];
-    for (int m = t; m < n; m += nt) {
-      const int w = S.sh_list[(size_t)sh * nw + m];
-      const int il = S.sh_rank[(size_t)sh * nw + m] + 1;
-      const double th = S.w_theta[w];
-      double arg = 1.0 + (1.0 - 2.0 * (double)il) / (double)n;
-      double th0 = acos(arg);
-      th0 = th0 - 3.0 * sin(th0) * S.sh_cstb[sh] / 2.0;
-      if (th0 < 0.0) th0 = 0.0;
-      if (th0 > pi) th0 = pi;
-      avt[sh] += th;
-      const double dth = th - th0 + tc;
-      epol += 0.5 * S.fkwpol * (dth * dth);
-      const double dv = S.fkwpol * dth;
-      const int i = S.nat_solute + S.sa * w;
-      double rmu[3], rcu[3], rm, rc;
-      wat_vectors(S, i, rmu, rm, rcu, rc);
-      double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
-      scp = scp > 1.0 ? 1.0 : (scp < -1.0 ? -1.0 : scp);
-      double f0 = sin(acos(scp));
-      if (fabs(f0) < 1.0e-10) f0 = 1.0e-10;
-      f0 = -1.0 / f0;
-      f0 = dv * f0;
-      for (int c = 0; c < 3; c++) {
-        const double av = rcu[c] - rmu[c] * scp;
-        const double f1 = (av * (-2.0)) / rm, f3 = av / rm, f2 = (rmu[c] - rcu[c] * scp) / rc;
-        S.d[3 * i + c] += (f1 + f2) * f0;
-        S.d[3 * (i + 1) + c] += f3 * f0;
-        S.d[3 * (i + 2) + c] += f3 * f0;
-      }
+    double arg = 1.0 + (1.0 - 2.0 * (double)il) / (double)n;
+    double th0 = acos(arg);
+    th0 = th0 - 3.0 * sin(th0) * S.sh_cstb[sh] / 2.0;
+    if (th0 < 0.0) th0 = 0.0;
+    if (th0 > pi) th0 = pi;
+    const double dth = th - th0 + tc;
+    epol = 0.5 * S.fkwpol * (dth * dth);
+    const double dv = S.fkwpol * dth;
+    const int i = S.nat_solute + S.sa * w;
+    double rmu[3], rcu[3], rm, rc;
+    wat_vectors(S, i, rmu, rm, rcu, rc);
+    double scp = rmu[0] * rcu[0] + rmu[1] * rcu[1] + rmu[2] * rcu[2];
+    scp = scp > 1.0 ? 1.0 : (scp < -1.0 ? -1.0 : scp);
+    double f0 = sin(acos(scp));
+    if (fabs(f0) < 1.0e-10) f0 = 1.0e-10;
+    f0 = -1.0 / f0;
+    f0 = dv * f0;
+    const auto d = gpm(S.d);
+    for (int c = 0; c < 3; c++) {
+      const double av = rcu[c] - rmu[c] * scp;
+      const double f1 = (av * (-2.0)) / rm, f3 = av / rm, f2 = (rmu[c] - rcu[c] * scp) / rc;
+      d[3 * i + c] += (f1 + f2) * f0;
+      d[3 * (i + 1) + c] += f3 * f0;
+      d[3 * (i + 2) + c] += f3 * f0;
     }
   }
-  // block reductions: energy and per-shell theta sums
-  const int wid = t / QNB_WAVE, lane = t % QNB_WAVE;
+  // block sums: [0] energy, [1 + is] theta over the block's members of shell is
+  __shared__ double red[WP_THREADS / QNB_WAVE][1 + QNB_MAXSH];
+  __shared__ int last;
   double v = wave_sum(epol);
   if (lane == 0) red[wid][0] = v;
   for (int is = 0; is < nsh; is++) {
-    double s2 = wave_sum(avt[is]);
+    const double s2 = wave_sum(sh == is ? th : 0.0);
     if (lane == 0) red[wid][1 + is] = s2;
   }
   __syncthreads();
+  // thread 0 writes the block's sums, publishes them (device-scope fence)
+  // and takes a ticket; one fence per block
   if (t == 0) {
-    double etot = 0.0;
-    for (int k = 0; k < nt / QNB_WAVE; k++) etot += red[k][0];
-    for (int is = 0; is < nsh; is++) {
-      if (cnt[is] == 0) continue;
+    double* part = S.wp_part + (size_t)blockIdx.x * (1 + QNB_MAXSH);
+    for (int c = 0; c <= nsh; c++) {
       double s2 = 0.0;
-      for (int k = 0; k < nt / QNB_WAVE; k++) s2 += red[k][1 + is];
-      st[QNB_MAXSH + is] += s2 / (double)cnt[is];
-      st[2 * QNB_MAXSH + is] += (double)cnt[is];
+      for (int k = 0; k < WP_THREADS / QNB_WAVE; k++) s2 += red[k][c];
+      part[c] = s2;
     }
-    for (int is = 0; is < nsh; is++) S.sh_n[is] = cnt[is];
-    double* ep = S.epart + (size_t)slot * QNB_NE;
-    for (int k = 0; k < QNB_NE; k++) ep[k] = 0.0;
-    ep[E_POL] = etot;
+    __threadfence();
+    last = atomicAdd(S.wp_ticket, 1) == nblk - 1;
   }
+  __syncthreads();
+  if (!last || t != 0) return;
+  __threadfence();
+  double* st = S.sh_state;
+  double sum[1 + QNB_MAXSH];
+  for (int k = 0; k <= nsh; k++) sum[k] = 0.0;
+  for (int blk = 0; blk < nblk; blk++) {
+    const volatile double* pb = S.wp_part + (size_t)blk * (1 + QNB_MAXSH);
+    for (int k = 0; k <= nsh; k++) sum[k] += pb[k];
+  }
+  for (int is = 0; is < nsh; is++) {
+    const int n = S.sh_n[is];
+    S.sh_n[is] = 0;  // the next step's phase 1 counts from zero
+    if (n == 0) continue;
+    st[QNB_MAXSH + is] += sum[1 + is] / (double)n;
+    st[2 * QNB_MAXSH + is] += (double)n;
+  }
+  double* ep = S.epart + (size_t)slot * QNB_NE;
+  for (int k = 0; k < QNB_NE; k++) ep[k] = 0.0;
+  ep[E_POL] = sum[0];
+  atomicExch(S.wp_ticket, 0);
 }
 
 // ---------------------------------------------------------------------------
