@@ -25,14 +25,16 @@ __global__ void k_qq(const DevSys*, uint32_t);
 __global__ void k_watpol_theta(const DevSys*, const int*);
 __global__ void k_watpol_rank(const DevSys*);
 __global__ void k_watpol_force(const DevSys*);
+__global__ void k_watpol_finish(const DevSys*);
 
 // SCAAS polarisation restraint: theta and shells, ranks, forces (blocks of
-// WP_THREADS waters, grid.y the instance)
+// WP_THREADS waters, grid.y the instance), then the per-system sums
 void launch_watpol(hipStream_t st, int nblk, int B, const DevSys* dsys, const int* istep) {
   const dim3 g(nblk, B), t(WP_THREADS);
   hipLaunchKernelGGL(k_watpol_theta, g, t, 0, st, dsys, istep);
   hipLaunchKernelGGL(k_watpol_rank, dim3(QNB_MAXSH, B), dim3(1024), 0, st, dsys);
   hipLaunchKernelGGL(k_watpol_force, g, t, 0, st, dsys);
+  hipLaunchKernelGGL(k_watpol_finish, dim3(B), dim3(64), 0, st, dsys);
 }
 __global__ void k_reduce_energies(const DevSys*, int, int);
 __global__ void k_atom_terms(const DevSys*, uint32_t, const double*);
@@ -318,14 +320,13 @@ int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots) {
   AL(sh_lth, (size_t)QNB_MAXSH * nw1);
   AL(sh_n, QNB_MAXSH);
   AL(wp_part, (size_t)(1 + QNB_MAXSH) * ((nw1 + WP_THREADS - 1) / WP_THREADS));
-  AL(wp_ticket, 1);
+
 #undef AL
   double sh0[3 * QNB_MAXSH] = {0};
   for (int k = 0; k < S.nshell; k++) sh0[k] = p->wshell_theta_corr[k];
   HIPCHK(hipMemcpy(S.sh_state, sh0, sizeof sh0, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(S.nq_lists, 0, 2 * sizeof(int)));
   HIPCHK(hipMemset(S.need, 0, sizeof(int)));
-  HIPCHK(hipMemset(S.wp_ticket, 0, sizeof(int)));
   HIPCHK(hipMemset(S.sh_n, 0, sizeof(int) * QNB_MAXSH));
   HIPCHK(hipMemset(S.nnbr, 0, sizeof(int) * S.ncgp));
   HIPCHK(hipMemset(S.ered, 0, sizeof(double) * QNB_NE));

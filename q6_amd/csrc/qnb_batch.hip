@@ -53,6 +53,16 @@ struct MdDev {
   int xs_every, xs_max;
   double* bpart;  // [nbpart][2] bond and angle energy per k_bonded block
   int nbpart;
+  // atom-centred bonded terms: for atom i, items boff[i]..boff[i+1]-1 of
+  // bitem (2 * bond + role) and aoff[i]..aoff[i+1]-1 of aitem (3 * angle +
+  // role); atoms [0, nbat) carry all of them
+  const int* boff;
+  const int* bitem;
+  const int* aoff;
+  const int* aitem;
+  int nbat;
+  // fused integrator: per-block kinetic energy sums
+  double* tpart;  // [ceil((nwat + nat_solute) / 256)][2]
   // EVB off-diagonal elements sampled with the energies
   int noffd;
   int offd_k[QNB_MAX_OFFD], offd_l[QNB_MAX_OFFD];  // atom indices
@@ -73,59 +83,67 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // harmonic bonds and angles of the solute (bond, angle: src/bondene.f90:41-246)
-// Thread t of the grid row takes bond t and angle t; each block's energy
-// sums go to bpart[block] and k_record adds them in block order.
+// One thread per atom sums the terms of every bond and angle it belongs to
+// (each term is evaluated once per member atom), so the gradient needs no
+// atomics and does not depend on scheduling.  A bond's energy is counted by
+// its first atom, an angle's by its vertex; each block's energy sums go to
+// bpart[block] and k_record adds them in block order.
 __global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
   const DevSys& S = systems[blockIdx.y];
   const MdDev& M = mds[blockIdx.y];
   __shared__ double red[4];
-  const double* x = S.x;
-  double* d = S.d;
+  const auto x = gp(S.x);
   double eb = 0.0, ea = 0.0;
-  const int t0 = blockIdx.x * blockDim.x + threadIdx.x, tstride = gridDim.x * blockDim.x;
-  for (int b = t0; b < M.nbonds; b += tstride) {
-    const int i = M.bond_ij[2 * b], j = M.bond_ij[2 * b + 1];
-    const double ax = x[3 * j] - x[3 * i], ay = x[3 * j + 1] - x[3 * i + 1], az = x[3 * j + 2] - x[3 * i + 2];
-    const double r = sqrt(ax * ax + ay * ay + az * az);
-    const double db = r - M.bond_r0[b];
-    eb += 0.5 * M.bond_k[b] * db * db;
-    const double dv = M.bond_k[b] * db / r;
-    unsafeAtomicAdd(&d[3 * i], -ax * dv);
-    unsafeAtomicAdd(&d[3 * i + 1], -ay * dv);
-    unsafeAtomicAdd(&d[3 * i + 2], -az * dv);
-    unsafeAtomicAdd(&d[3 * j], ax * dv);
-    unsafeAtomicAdd(&d[3 * j + 1], ay * dv);
-    unsafeAtomicAdd(&d[3 * j + 2], az * dv);
-  }
-  for (int a = t0; a < M.nangles; a += tstride) {
-    const int i = M.angle_ijk[3 * a], j = M.angle_ijk[3 * a + 1], k = M.angle_ijk[3 * a + 2];
-    double u[3], w[3];
-    for (int c = 0; c < 3; c++) {
-      u[c] = x[3 * i + c] - x[3 * j + c];
-      w[c] = x[3 * k + c] - x[3 * j + c];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M.nbat) {
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int it = ldg(M.boff + i), e = ldg(M.boff + i + 1); it < e; it++) {
+      const int item = ldg(M.bitem + it), b = item >> 1, role = item & 1;
+      const int bi = ldg(M.bond_ij + 2 * b), bj = ldg(M.bond_ij + 2 * b + 1);
+      const double ax = x[3 * bj] - x[3 * bi], ay = x[3 * bj + 1] - x[3 * bi + 1], az = x[3 * bj + 2] - x[3 * bi + 2];
+      const double r = sqrt(ax * ax + ay * ay + az * az);
+      const double kb = ldg(M.bond_k + b), db = r - ldg(M.bond_r0 + b);
+      const double dv = kb * db / r;
+      const double sg = role == 0 ? -dv : dv;
+      f[0] += ax * sg;
+      f[1] += ay * sg;
+      f[2] += az * sg;
+      if (role == 0) eb += 0.5 * kb * db * db;
     }
-    const double ru = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    const double rw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-    double ct = (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / (ru * rw);
-    ct = fmin(1.0, fmax(-1.0, ct));
-    const double th = acos(ct);
-    // sin(theta) from the cross product, consistent with the numerator of the
-    // derivative below even for nearly linear angles, where sin(acos(ct))
-    // loses all digits; at a linear angle the derivative has no direction
-    const double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
-    const double st = sqrt(cx * cx + cy * cy + cz * cz) / (ru * rw);
-    const double da = th - M.angle_t0[a];
-    ea += 0.5 * M.angle_k[a] * da * da;
-    if (!(st > 1.0e-8)) continue;
-    const double dv = M.angle_k[a] * da;
-    // dtheta/dx_i = -(w/(ru rw) - ct u/ru^2)/sin(theta), likewise for k; j balances
-    for (int c = 0; c < 3; c++) {
-      const double gi = -(w[c] / (ru * rw) - ct * u[c] / (ru * ru)) / st;
-      const double gk = -(u[c] / (ru * rw) - ct * w[c] / (rw * rw)) / st;
-      unsafeAtomicAdd(&d[3 * i + c], dv * gi);
-      unsafeAtomicAdd(&d[3 * k + c], dv * gk);
-      unsafeAtomicAdd(&d[3 * j + c], -dv * (gi + gk));
+    for (int it = ldg(M.aoff + i), e = ldg(M.aoff + i + 1); it < e; it++) {
+      const int item = ldg(M.aitem + it), a = item / 3, role = item - 3 * a;
+      const int ai = ldg(M.angle_ijk + 3 * a), aj = ldg(M.angle_ijk + 3 * a + 1), ak = ldg(M.angle_ijk + 3 * a + 2);
+      double u[3], w[3];
+      for (int c = 0; c < 3; c++) {
+        u[c] = x[3 * ai + c] - x[3 * aj + c];
+        w[c] = x[3 * ak + c] - x[3 * aj + c];
+      }
+      const double ru = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      const double rw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+      double ct = (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / (ru * rw);
+      ct = fmin(1.0, fmax(-1.0, ct));
+      const double th = acos(ct);
+      // sin(theta) from the cross product, consistent with the numerator of
+      // the derivative below even for nearly linear angles, where
+      // sin(acos(ct)) loses all digits; at a linear angle the derivative has
+      // no direction
+      const double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
+      const double st = sqrt(cx * cx + cy * cy + cz * cz) / (ru * rw);
+      const double ka = ldg(M.angle_k + a), da = th - ldg(M.angle_t0 + a);
+      if (role == 1) ea += 0.5 * ka * da * da;
+      if (!(st > 1.0e-8)) continue;
+      const double dv = ka * da;
+      // dtheta/dx_i = -(w/(ru rw) - ct u/ru^2)/sin(theta), likewise for k; j balances
+      for (int c = 0; c < 3; c++) {
+        const double gi = -(w[c] / (ru * rw) - ct * u[c] / (ru * ru)) / st;
+        const double gk = -(u[c] / (ru * rw) - ct * w[c] / (rw * rw)) / st;
+        f[c] += dv * (role == 0 ? gi : (role == 2 ? gk : -(gi + gk)));
+      }
     }
+    const auto d = gpm(S.d);
+    d[3 * i] += f[0];
+    d[3 * i + 1] += f[1];
+    d[3 * i + 2] += f[2];
   }
   const double sb = block_sum(eb, red);
   const double sa = block_sum(ea, red);
@@ -135,19 +153,126 @@ __global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ syste
   }
 }
 
-// newvel_ber (src/md.f90:1451-1466): v = (v - d winv dt) s; xx = x; x += v dt
-__global__ __launch_bounds__(256) void k_leapfrog(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
+// One MD step's integration of every atom, fused: leap-frog with the
+// Berendsen factor of the previous step (newvel_ber, src/md.f90:1451-1466),
+// SHAKE of each rigid water (src/bondene.f90:1069-1151), v = (x - x_old)/dt
+// for every atom (src/md.f90:1787-1793) and per-block sums of the kinetic
+// energy (k_thermo adds them).  Thread t < nwat takes water t (3 atoms), the
+// next nat_solute threads one solute atom each.
+__global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
   const DevSys& S = systems[blockIdx.y];
   const MdDev& M = mds[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= S.natom) return;
-  const double s = M.st[0], dt = M.dt, wi = M.winv[i];
-  for (int c = 0; c < 3; c++) {
-    double v = (M.v[3 * i + c] - S.d[3 * i + c] * wi * dt) * s;
-    M.v[3 * i + c] = v;
-    M.xx[3 * i + c] = S.x[3 * i + c];
-    S.x[3 * i + c] = S.x[3 * i + c] + v * dt;
+  const int nthr = S.nwat + S.nat_solute;
+  const int nblk = (nthr + 255) / 256;
+  if ((int)blockIdx.x >= nblk) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const double s = M.st[0], dt = M.dt;
+  const auto xg = gpm(S.x);
+  const auto vg = gpm(M.v);
+  const auto dg = gp(S.d);
+  double ek = 0.0, ekf = 0.0;
+  if (t < S.nwat) {
+    const int o = S.nat_solute + 3 * t;
+    double xo[3][3], xn[3][3], wi[3];
+    for (int k = 0; k < 3; k++) {
+      wi[k] = ldg(M.winv + o + k);
+      for (int c = 0; c < 3; c++) {
+        const int q = 3 * (o + k) + c;
+        const double v = (vg[q] - dg[q] * wi[k] * dt) * s;
+        xo[k][c] = xg[q];
+        xn[k][c] = xo[k][c] + v * dt;
+      }
+    }
+    const int ci[3] = {0, 0, 1}, cj[3] = {1, 2, 2};
+    const double d2[3] = {M.roh2, M.roh2, M.rhh2};
+    bool ready[3] = {false, false, false};
+    int nits = 0;
+    for (; nits < 1000; nits++) {
+      for (int c = 0; c < 3; c++) {
+        if (ready[c]) continue;
+        const int i = ci[c], j = cj[c];
+        const double vx = xn[i][0] - xn[j][0], vy = xn[i][1] - xn[j][1], vz = xn[i][2] - xn[j][2];
+        const double diff = d2[c] - (vx * vx + vy * vy + vz * vz);
+        if (fabs(diff) < 0.0001 * d2[c]) ready[c] = true;
+        const double px = xo[i][0] - xo[j][0], py = xo[i][1] - xo[j][1], pz = xo[i][2] - xo[j][2];
+        const double scp = vx * px + vy * py + vz * pz;
+        const double corr = diff / (2.0 * scp * (wi[i] + wi[j]));
+        xn[i][0] += px * corr * wi[i];
+        xn[i][1] += py * corr * wi[i];
+        xn[i][2] += pz * corr * wi[i];
+        xn[j][0] -= px * corr * wi[j];
+        xn[j][1] -= py * corr * wi[j];
+        xn[j][2] -= pz * corr * wi[j];
+      }
+      if (ready[0] && ready[1] && ready[2]) break;
+    }
+    if (nits >= 1000) atomicOr(&M.ctr[1], ERR_SHAKE);
+    for (int k = 0; k < 3; k++) {
+      double v2 = 0.0;
+      for (int c = 0; c < 3; c++) {
+        const int q = 3 * (o + k) + c;
+        const double v = (xn[k][c] - xo[k][c]) / dt;
+        xg[q] = xn[k][c];
+        vg[q] = v;
+        v2 += v * v;
+      }
+      const double e = 0.5 * (1.0 / wi[k]) * v2;
+      ek += e;
+      if (!ldg(S.excl + o + k)) ekf += e;
+    }
+  } else if (t < nthr) {
+    const int i = t - S.nwat;
+    const double wi = ldg(M.winv + i);
+    double v2 = 0.0;
+    for (int c = 0; c < 3; c++) {
+      const int q = 3 * i + c;
+      const double v0 = (vg[q] - dg[q] * wi * dt) * s;
+      const double xo = xg[q], xn = xo + v0 * dt;
+      const double v = (xn - xo) / dt;
+      xg[q] = xn;
+      vg[q] = v;
+      v2 += v * v;
+    }
+    const double e = 0.5 * (1.0 / wi) * v2;
+    ek += e;
+    if (!ldg(S.excl + i)) ekf += e;
   }
+  __shared__ double red[4];
+  ek = block_sum(ek, red);
+  ekf = block_sum(ekf, red);
+  if (threadIdx.x == 0) {
+    M.tpart[2 * blockIdx.x] = ek;
+    M.tpart[2 * blockIdx.x + 1] = ekf;
+  }
+}
+
+// After k_integrate, one thread per system: the kinetic energy from the
+// block sums in block order, the temperature, Tfree and the Berendsen factor
+// of the next step (temperature, src/md.f90:1379-1447); advances the step
+// counter and records errors.
+__global__ __launch_bounds__(64) void k_thermo(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
+  const DevSys& S = systems[blockIdx.x];
+  const MdDev& M = mds[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  const int nblk = (S.nwat + S.nat_solute + 255) / 256;
+  double eks = 0.0, ekfs = 0.0;
+  for (int k = 0; k < nblk; k++) {
+    eks += M.tpart[2 * k];
+    ekfs += M.tpart[2 * k + 1];
+  }
+  const double T = 2.0 * eks / M.Boltz / M.ndegf;
+  const double Tf = 2.0 * ekfs / M.Boltz / M.ndegfree;
+  double sf = 0.0;
+  if (Tf != 0.0) sf = M.T0 / Tf - 1.0;
+  sf = sqrt(1.0 + M.dt_over_tau * sf);
+  M.st[0] = sf;
+  M.st[1] = Tf;
+  M.st[2] = T;
+  M.st[3] = eks;
+  *M.tfree = Tf;
+  if (!isfinite(T) || !isfinite(S.ered[E_PP_EL] + S.ered[E_WW_EL])) atomicOr(&M.ctr[1], ERR_NONFINITE);
+  if (*S.need > S.cap) atomicOr(&M.ctr[1], ERR_OVERFLOW);
+  *M.istep = *M.istep + 1;
 }
 
 // shake (src/bondene.f90:1069-1151) on each rigid water (O-H1, O-H2, H1-H2),
@@ -218,10 +343,10 @@ __global__ __launch_bounds__(256) void k_sd_move(const DevSys* __restrict__ syst
   S.x[3 * i + 2] -= gz * sc;
 }
 
-// temperature (src/md.f90:1379-1447), one temperature group: Temp, Tfree and
-// the Berendsen factor of the next step; advances the step counter
-__global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds,
-                                                     int advance) {
+// temperature (src/md.f90:1379-1447) of the starting velocities, one
+// temperature group: Temp, Tfree and the Berendsen factor of the first step
+// (during MD, k_integrate computes them)
+__global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
   const DevSys& S = systems[blockIdx.x];
   const MdDev& M = mds[blockIdx.x];
   __shared__ double red[4];
@@ -248,7 +373,6 @@ __global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ 
     *M.tfree = Tf;
     if (!isfinite(T) || !isfinite(S.ered[E_PP_EL] + S.ered[E_WW_EL])) atomicOr(&M.ctr[1], ERR_NONFINITE);
     if (*S.need > S.cap) atomicOr(&M.ctr[1], ERR_OVERFLOW);
-    if (advance) *M.istep = *M.istep + 1;
   }
 }
 
@@ -421,9 +545,8 @@ static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
   hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(1024), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
   hipLaunchKernelGGL(k_snap, dim3((3 * b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
-  hipLaunchKernelGGL(k_leapfrog, dim3((b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
-  hipLaunchKernelGGL(k_shake, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
-  hipLaunchKernelGGL(k_temperature, dim3(B), dim3(256), 0, b->stream, b->dsys, b->dmd, 1);
+  hipLaunchKernelGGL(k_integrate, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  hipLaunchKernelGGL(k_thermo, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
 }
 
 template <typename R>
@@ -546,7 +669,41 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
         (rc = b->mem.alloc(&M.hist, (size_t)std::max(m->max_frames, 1) * QNB_FRAME_LEN)) ||
         (rc = b->mem.alloc(&M.st, 8)) || (rc = b->mem.alloc(&M.ctr, 4)))
       return rc;
-    b->nbblk = std::max(b->nbblk, (std::max(m->nbonds, m->nangles) + 255) / 256);
+    // atom-centred incidence lists of the bonds and angles (k_bonded)
+    if (m->nbonds < 0 || m->nangles < 0 || (m->nbonds > 0 && !m->bond_ij) || (m->nangles > 0 && !m->angle_ijk))
+      return fail(QNB_EINVAL, "instance %d: bad bond/angle arrays", k);
+    {
+      std::vector<int> boff(S.natom + 1, 0), aoff(S.natom + 1, 0);
+      for (int q = 0; q < 2 * m->nbonds; q++) {
+        const int at = m->bond_ij[q];
+        if (at < 0 || at >= S.natom) return fail(QNB_EINVAL, "instance %d: bond atom %d out of range", k, at);
+        boff[at + 1]++;
+      }
+      for (int q = 0; q < 3 * m->nangles; q++) {
+        const int at = m->angle_ijk[q];
+        if (at < 0 || at >= S.natom) return fail(QNB_EINVAL, "instance %d: angle atom %d out of range", k, at);
+        aoff[at + 1]++;
+      }
+      int nbat = 0;
+      for (int i = 0; i < S.natom; i++) {
+        if (boff[i + 1] > 0 || aoff[i + 1] > 0) nbat = i + 1;
+        boff[i + 1] += boff[i];
+        aoff[i + 1] += aoff[i];
+      }
+      std::vector<int> bitem(std::max(boff[S.natom], 1)), aitem(std::max(aoff[S.natom], 1));
+      std::vector<int> bpos(boff.begin(), boff.end() - 1), apos(aoff.begin(), aoff.end() - 1);
+      for (int q = 0; q < 2 * m->nbonds; q++) bitem[bpos[m->bond_ij[q]]++] = q;  // q = 2 * bond + role
+      for (int q = 0; q < 3 * m->nangles; q++) aitem[apos[m->angle_ijk[q]]++] = q;  // q = 3 * angle + role
+      M.nbat = nbat;
+      if ((rc = b->mem.upload(&M.boff, boff.data(), boff.size())) ||
+          (rc = b->mem.upload(&M.bitem, bitem.data(), bitem.size())) ||
+          (rc = b->mem.upload(&M.aoff, aoff.data(), aoff.size())) ||
+          (rc = b->mem.upload(&M.aitem, aitem.data(), aitem.size())))
+        return rc;
+      b->nbblk = std::max(b->nbblk, (nbat + 255) / 256);
+      const int nib = (S.nwat + S.nat_solute + 255) / 256;
+      if ((rc = b->mem.alloc(&M.tpart, 2 * (size_t)std::max(nib, 1)))) return rc;
+    }
     HIPCHK(hipMemcpy(M.v, v0[k], sizeof(double) * 3 * S.natom, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(M.st, 0, 8 * sizeof(double)));
     HIPCHK(hipMemset(M.ctr, 0, 4 * sizeof(int)));
@@ -627,7 +784,7 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
   }
   if (b->anyq) hipLaunchKernelGGL(k_qlists, dim3(n), dim3(1024), 0, b->stream, b->dsys);
   // initial temperature (md_run calls temperature() before the loop)
-  hipLaunchKernelGGL(k_temperature, dim3(n), dim3(256), 0, b->stream, b->dsys, b->dmd, 0);
+  hipLaunchKernelGGL(k_temperature, dim3(n), dim3(256), 0, b->stream, b->dsys, b->dmd);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b->stream));
   *out = b.release();

@@ -113,7 +113,6 @@ struct DevSys {
   double* sh_lth;    // [QNB_MAXSH][nwat] their theta
   int* sh_n;         // [QNB_MAXSH] members per shell
   double* wp_part;   // [ceil(nwat / WP_THREADS)][1 + QNB_MAXSH] per-block sums
-  int* wp_ticket;    // [1] blocks done (reset by the last one)
   // energy sampling: energies are accumulated when ene_every <= 0 or when
   // the instance's step counter is a multiple of ene_every
   const int* step_ptr;

@@ -1055,7 +1055,7 @@ __device__ __forceinline__ bool has_pol(const DevSys& S) { return S.nwat > 0 && 
 // (slot order is immaterial: ranks are counted by value).  Block 0 first
 // applies the theta_corr feedback of the steps that are multiples of itdis
 // (nonbondene.f90:6653-6666).  The shell counts start at zero (set at
-// creation and cleared by the last block of phase 3).
+// creation and cleared by phase 4).
 __global__ __launch_bounds__(WP_THREADS) void k_watpol_theta(const DevSys* __restrict__ systems,
                                                              const int* __restrict__ istep) {
   const DevSys& S = systems[blockIdx.y];
@@ -1175,17 +1175,11 @@ __global__ __launch_bounds__(WP_RANK_THREADS) void k_watpol_rank(const DevSys* _
 
 // Phase 3, one thread per water of a shell: target angle from the rank, the
 // restraint energy and forces (adds to d), and per-block sums of the energy
-// and of theta per shell.  The last block of a system to finish (ticket)
-// adds the block sums in block order, accumulates avtheta and avn_insh and
-// writes the energy slot; it resets the ticket for the next launch.
+// and of theta per shell into wp_part.
 __global__ __launch_bounds__(WP_THREADS) void k_watpol_force(const DevSys* __restrict__ systems) {
   const DevSys& S = systems[blockIdx.y];
-  const int slot = S.ncgp + 1;
   const int t = threadIdx.x, lane = t % QNB_WAVE, wid = t / QNB_WAVE;
-  if (!has_pol(S)) {
-    if (blockIdx.x == 0 && t < QNB_NE) S.epart[(size_t)slot * QNB_NE + t] = 0.0;
-    return;
-  }
+  if (!has_pol(S)) return;
   const int nsh = S.nshell, nw = S.nwat;
   // blocks of this system (the grid is sized for the batch's largest)
   const int nblk = (nw + WP_THREADS - 1) / WP_THREADS;
@@ -1228,7 +1222,6 @@ __global__ __launch_bounds__(WP_THREADS) void k_watpol_force(const DevSys* __res
   }
   // block sums: [0] energy, [1 + is] theta over the block's members of shell is
   __shared__ double red[WP_THREADS / QNB_WAVE][1 + QNB_MAXSH];
-  __shared__ int last;
   double v = wave_sum(epol);
   if (lane == 0) red[wid][0] = v;
   for (int is = 0; is < nsh; is++) {
@@ -1236,8 +1229,6 @@ __global__ __launch_bounds__(WP_THREADS) void k_watpol_force(const DevSys* __res
     if (lane == 0) red[wid][1 + is] = s2;
   }
   __syncthreads();
-  // thread 0 writes the block's sums, publishes them (device-scope fence)
-  // and takes a ticket; one fence per block
   if (t == 0) {
     double* part = S.wp_part + (size_t)blockIdx.x * (1 + QNB_MAXSH);
     for (int c = 0; c <= nsh; c++) {
@@ -1245,30 +1236,34 @@ __global__ __launch_bounds__(WP_THREADS) void k_watpol_force(const DevSys* __res
       for (int k = 0; k < WP_THREADS / QNB_WAVE; k++) s2 += red[k][c];
       part[c] = s2;
     }
-    __threadfence();
-    last = atomicAdd(S.wp_ticket, 1) == nblk - 1;
   }
-  __syncthreads();
-  if (!last || t != 0) return;
-  __threadfence();
-  double* st = S.sh_state;
+}
+
+// Phase 4, one thread per system: adds the block sums in block order,
+// accumulates avtheta and avn_insh, writes the energy slot and clears the
+// shell counts for the next step's phase 1.
+__global__ __launch_bounds__(64) void k_watpol_finish(const DevSys* __restrict__ systems) {
+  const DevSys& S = systems[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  double* ep = S.epart + (size_t)(S.ncgp + 1) * QNB_NE;
+  for (int k = 0; k < QNB_NE; k++) ep[k] = 0.0;
+  if (!has_pol(S)) return;
+  const int nsh = S.nshell, nblk = (S.nwat + WP_THREADS - 1) / WP_THREADS;
   double sum[1 + QNB_MAXSH];
   for (int k = 0; k <= nsh; k++) sum[k] = 0.0;
   for (int blk = 0; blk < nblk; blk++) {
-    const volatile double* pb = S.wp_part + (size_t)blk * (1 + QNB_MAXSH);
+    const double* pb = S.wp_part + (size_t)blk * (1 + QNB_MAXSH);
     for (int k = 0; k <= nsh; k++) sum[k] += pb[k];
   }
+  double* st = S.sh_state;
   for (int is = 0; is < nsh; is++) {
     const int n = S.sh_n[is];
-    S.sh_n[is] = 0;  // the next step's phase 1 counts from zero
+    S.sh_n[is] = 0;
     if (n == 0) continue;
     st[QNB_MAXSH + is] += sum[1 + is] / (double)n;
     st[2 * QNB_MAXSH + is] += (double)n;
   }
-  double* ep = S.epart + (size_t)slot * QNB_NE;
-  for (int k = 0; k < QNB_NE; k++) ep[k] = 0.0;
   ep[E_POL] = sum[0];
-  atomicExch(S.wp_ticket, 0);
 }
 
 // ---------------------------------------------------------------------------
