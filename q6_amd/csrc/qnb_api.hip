@@ -265,6 +265,15 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
       UP(qpB, s->qp_B, n);
       UP(qpEl, s->qp_el, n);
       UP(qpSc, s->qp_sc, n);
+      if (S.prec == QNB_PREC_MIXED) {
+        std::vector<float> f(n);
+        const double* src[4] = {s->qp_A, s->qp_B, s->qp_el, s->qp_sc};
+        const float** dst[4] = {&S.qpA_f, &S.qpB_f, &S.qpEl_f, &S.qpSc_f};
+        for (int k = 0; k < 4; k++) {
+          for (size_t m = 0; m < n; m++) f[m] = (float)src[k][m];
+          if ((rc = M.upload(dst[k], f.data(), n))) return rc;
+        }
+      }
     }
     if (s->nwat > 0) {
       size_t n = (size_t)nq * sa * ns;

@@ -70,6 +70,10 @@ struct DevSys {
   const double* qpB;
   const double* qpEl;
   const double* qpSc;
+  const float* qpA_f;  // FP32 copies of qpA, qpB, qpEl, qpSc for the mixed path
+  const float* qpB_f;
+  const float* qpEl_f;
+  const float* qpSc_f;
   const double* qwA;  // [iq + nq*(j + sa*s)]
   const double* qwB;
   const double* qwEl;
@@ -132,6 +136,14 @@ template <typename T>
 __device__ __forceinline__ T ldg(const T* p) {
   return *(const __attribute__((address_space(1))) T*)p;
 }
+// a Q-solute table in the pair arithmetic's type: on the mixed path its FP32
+// copy (the same values the FP64 table gives after rounding to float)
+template <typename R>
+__device__ __forceinline__ const R* qp_tab(const double* d, const float* f) {
+  if constexpr (sizeof(R) == 4) return f;
+  else return d;
+}
+
 template <typename T>
 using gptr_t = const __attribute__((address_space(1))) T*;
 template <typename T>

@@ -741,6 +741,8 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
       }
       // Q-atom - solute atom, receiver side (nonbond_qp, nonbondene.f90:5157-5230)
       if (qin) {
+        const R *qtA = qp_tab<R>(S.qpA, S.qpA_f), *qtB = qp_tab<R>(S.qpB, S.qpB_f);
+        const R *qtE = qp_tab<R>(S.qpEl, S.qpEl_f), *qtS = qp_tab<R>(S.qpSc, S.qpSc_f);
         for (int iq = lane; iq < nq; iq += QNB_WAVE) {
           const int q = ldg(S.iqseq + iq);
           const R qx = (R)(ldg(x + 3 * q) - xr0), qy = (R)(ldg(x + 3 * q + 1) - xr1),
@@ -759,8 +761,7 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
             for (int s = 0; s < NS; s++) {
               if (s >= ns) break;
               const size_t t = (size_t)i + (size_t)nsol * ((size_t)iq + (size_t)nq * s);
-              dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)ldg(S.qpA + t), (R)ldg(S.qpB + t), (R)ldg(S.qpEl + t),
-                                         (R)ldg(S.qpSc + t),
+              dvs += PairMath<R>::nbe_qx(r2, r, r6, ldg(qtA + t), ldg(qtB + t), ldg(qtE + t), ldg(qtS + t),
                                          (R)S.lambda[s], eq[4 * s + 0], eq[4 * s + 1], want_e);
             }
             f[k][0] += ux * dvs;
@@ -860,6 +861,8 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
   const int nq = S.nqat, ns = S.nstates, sa = S.sa;
   double f[3] = {0.0, 0.0, 0.0};
   double dummy_e = 0.0, dummy_v = 0.0;
+  const R *qtA = qp_tab<R>(S.qpA, S.qpA_f), *qtB = qp_tab<R>(S.qpB, S.qpB_f);
+  const R *qtE = qp_tab<R>(S.qpEl, S.qpEl_f), *qtS = qp_tab<R>(S.qpSc, S.qpSc_f);
   for (int c = ch; c < nw_ch + np_ch; c += chunks_per_q) {
     const int p = c * QNB_WAVE + lane;
     if (c < nw_ch) {
@@ -909,8 +912,7 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
       for (int s = 0; s < NS; s++) {
         if (s >= ns) break;
         const size_t t = (size_t)j + (size_t)S.nat_solute * ((size_t)iq + (size_t)nq * s);
-        R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)ldg(S.qpA + t), (R)ldg(S.qpB + t), (R)ldg(S.qpEl + t),
-                                   (R)ldg(S.qpSc + t),
+        R dv = PairMath<R>::nbe_qx(r2, r, r6, ldg(qtA + t), ldg(qtB + t), ldg(qtE + t), ldg(qtS + t),
                                    (R)S.lambda[s], dummy_e, dummy_v, false);
         f[0] -= (double)(rux * dv);
         f[1] -= (double)(ruy * dv);
