@@ -293,6 +293,48 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
     UP(qqp_j, qqp_j.data(), qqp_j.size());
     UP(qqp_st, qqp_st.data(), qqp_st.size());
     UP(qqp_par, qqp_par.data(), qqp_par.size());
+    // atom-centred view of both lists for k_qq: entities are the Q-atoms and
+    // the distinct solute partners of the Q-bonded-neighbour pairs; entity e
+    // is atom qe_atom[e] and owns items qe_off[e]..qe_off[e+1]-1 of qe_item,
+    // each {first atom, second atom, pair, (state << 2) | (kind << 1) | side},
+    // kind 0 a Q-Q pair and 1 a Q-neighbour pair, side 0 the pair's first
+    // atom (which counts its energy)
+    std::vector<int> ent_atom(nq), cnt(nq, 0);
+    std::vector<int> ent_of_j(nsol1, -1);
+    for (int iq = 0; iq < nq; iq++) ent_atom[iq] = s->iqseq[iq];
+    for (size_t p = 0; p < qqp_j.size(); p++)
+      if (ent_of_j[qqp_j[p]] < 0) {
+        ent_of_j[qqp_j[p]] = (int)ent_atom.size();
+        ent_atom.push_back(qqp_j[p]);
+        cnt.push_back(0);
+      }
+    for (size_t p = 0; p < qq_iq.size(); p++) cnt[qq_iq[p]]++, cnt[qq_jq[p]]++;
+    for (size_t p = 0; p < qqp_iq.size(); p++) cnt[qqp_iq[p]]++, cnt[ent_of_j[qqp_j[p]]]++;
+    const int nent = (int)ent_atom.size();
+    std::vector<int> off(nent + 1, 0);
+    for (int e = 0; e < nent; e++) off[e + 1] = off[e] + cnt[e];
+    std::vector<int> item(4 * (size_t)std::max(off[nent], 1)), pos(off.begin(), off.end() - 1);
+    auto put = [&](int e, int a, int b, int p, int w) {
+      const int q = pos[e]++;
+      item[4 * q] = a;
+      item[4 * q + 1] = b;
+      item[4 * q + 2] = p;
+      item[4 * q + 3] = w;
+    };
+    for (size_t p = 0; p < qq_iq.size(); p++) {
+      const int a = s->iqseq[qq_iq[p]], b = s->iqseq[qq_jq[p]], st = qq_st[p] << 2;
+      put(qq_iq[p], a, b, (int)p, st);
+      put(qq_jq[p], a, b, (int)p, st | 1);
+    }
+    for (size_t p = 0; p < qqp_iq.size(); p++) {
+      const int a = s->iqseq[qqp_iq[p]], b = qqp_j[p], st = qqp_st[p] << 2;
+      put(qqp_iq[p], a, b, (int)p, st | 2);
+      put(ent_of_j[qqp_j[p]], a, b, (int)p, st | 3);
+    }
+    S.nqe = nent;
+    UP(qe_atom, ent_atom.data(), ent_atom.size());
+    UP(qe_off, off.data(), off.size());
+    UP(qe_item, item.data(), item.size());
   }
 #undef UP
   return 0;

@@ -90,6 +90,13 @@ struct DevSys {
   const int* qqp_st;
   const double* qqp_par;
   int nqqp;
+  // atom-centred view of the two lists (Q-atoms, then the distinct partners
+  // of the Q-neighbour pairs): items {first atom, second atom, pair,
+  // (state << 2) | (kind << 1) | side}
+  const int* qe_atom;
+  const int* qe_off;
+  const int* qe_item;  // [4 * items]
+  int nqe;
 
   // per-instance state
   double* x;
@@ -292,7 +299,7 @@ struct PairMath {
 // partner offsets and receiver offsets (R x 3), partner index and tag,
 // receiver index
 // threads of the Q-Q kernel's block (one block per system)
-#define QQ_THREADS 256
+#define QQ_THREADS 1024
 // threads per block of the polarisation-restraint kernels (one per water)
 #define WP_THREADS 256
 

@@ -933,9 +933,12 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
 // ---------------------------------------------------------------------------
 // Static Q-Q (nonbond_qq, nonbondene.f90:5013-5082, pair kernel nbe_qq
 // nonbonded.f90:111-150) and Q-bonded-neighbour (nonbond_qqp, 5085-5153)
-// lists: one block of QQ_THREADS per system, a thread per pair; both atoms
-// updated with atomics.  Energies per state are summed per wave and then over
-// the waves in order.
+// lists: one block of QQ_THREADS per system, a wavefront per atom of the
+// lists (entity) whose lanes evaluate the pairs the atom belongs to; the
+// wave's sum goes to the atom's gradient once (no atomics; each pair is
+// evaluated from both sides).  A pair's energy is counted by its first atom;
+// per state the energies are summed per wave and then over the waves in
+// order.
 template <int NS>
 __device__ __forceinline__ void qq_add(double (&eq)[NS][4], int s, int c0, double v0, double v1) {
 #pragma unroll
@@ -951,67 +954,68 @@ __global__ __launch_bounds__(QQ_THREADS) void k_qq(const DevSys* __restrict__ sy
   const DevSys& S = systems[blockIdx.x];
   const int slot = S.ncgp;
   const int t = threadIdx.x, lane = t % QNB_WAVE, wid = t / QNB_WAVE;
-  const double* __restrict__ x = S.x;
+  constexpr int NW = QQ_THREADS / QNB_WAVE;
+  const auto x = gp(S.x);
+  const auto d = gpm(S.d);
+  const bool want_e = want_energies(S);
   double eq[NS][4];
 #pragma unroll
   for (int s = 0; s < NS; s++) eq[s][0] = eq[s][1] = eq[s][2] = eq[s][3] = 0.0;
-  if (mask & T_QQ)
-    for (int p = t; p < S.nqq; p += QQ_THREADS) {
-      const int s = S.qq_st[p];
-      const int i = S.iqseq[S.qq_iq[p]], j = S.iqseq[S.qq_jq[p]];
-      const double* pr = S.qq_par + 4 * p;
+  // wave wid takes entities wid, wid + NW, ...; its lanes take the entity's pairs
+  for (int e = wid; e < S.nqe; e += NW) {
+    const int me = ldg(S.qe_atom + e);
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int it = ldg(S.qe_off + e) + lane, end = ldg(S.qe_off + e + 1); it < end; it += QNB_WAVE) {
+      const int i = ldg(S.qe_item + 4 * it), j = ldg(S.qe_item + 4 * it + 1), p = ldg(S.qe_item + 4 * it + 2);
+      const int w = ldg(S.qe_item + 4 * it + 3), s = w >> 2, side = w & 1;
+      const bool qq = (w & 2) == 0;
+      if (!(mask & (qq ? T_QQ : T_QQP))) continue;
+      const double* pr = (qq ? S.qq_par : S.qqp_par) + 4 * p;
       const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
       const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
       R r2, r;
       rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
-      R r6d = r2 * r2 * r2;
-      R r6_hc = recip(r6d);
-      R r6 = recip(r6_hc + (R)pr[3]);
-      R r12 = r6 * r6;
-      R lam = (R)S.lambda[s];
-      R Vel = (R)pr[2] * r, Va, Vb, dv;
-      if (S.qq_soft[p]) {
-        Vb = R(0);
-        Va = (R)pr[0] * exp(-(R)pr[1] / r);
-        dv = r2 * (-Vel - (R)pr[1] * Va / r) * lam;
+      const R pA = (R)ldg(pr), pB = (R)ldg(pr + 1), pel = (R)ldg(pr + 2), psc = (R)ldg(pr + 3);
+      const R lam = (R)S.lambda[s];
+      R dv;
+      if (qq) {
+        R r6d = r2 * r2 * r2;
+        R r6_hc = recip(r6d);
+        R r6 = recip(r6_hc + psc);
+        R r12 = r6 * r6;
+        R Vel = pel * r, Va, Vb;
+        if (ldg(S.qq_soft + p)) {
+          Vb = R(0);
+          Va = pA * exp(-pB / r);
+          dv = r2 * (-Vel - pB * Va / r) * lam;
+        } else {
+          Va = pA * r12;
+          Vb = pB * r6;
+          dv = r2 * (-Vel - (R(12) * Va - R(6) * Vb) * r6 * r6_hc) * lam;
+        }
+        if (want_e && side == 0) qq_add<NS>(eq, s, 0, (double)Vel, (double)Va - (double)Vb);
       } else {
-        Va = (R)pr[0] * r12;
-        Vb = (R)pr[1] * r6;
-        dv = r2 * (-Vel - (R(12) * Va - R(6) * Vb) * r6 * r6_hc) * lam;
+        double el = 0.0, vdw = 0.0;
+        dv = PairMath<R>::nbe_qx(r2, r, r2 * r2 * r2, pA, pB, pel, psc, lam, el, vdw, want_e && side == 0);
+        if (want_e && side == 0) qq_add<NS>(eq, s, 2, el, vdw);
       }
-      qq_add<NS>(eq, s, 0, (double)Vel, (double)Va - (double)Vb);
-      const double fx = (double)(rux * dv), fy = (double)(ruy * dv), fz = (double)(ruz * dv);
-      unsafeAtomicAdd(&S.d[3 * i], -fx);
-      unsafeAtomicAdd(&S.d[3 * i + 1], -fy);
-      unsafeAtomicAdd(&S.d[3 * i + 2], -fz);
-      unsafeAtomicAdd(&S.d[3 * j], fx);
-      unsafeAtomicAdd(&S.d[3 * j + 1], fy);
-      unsafeAtomicAdd(&S.d[3 * j + 2], fz);
+      // the pair's gradient is -u dv on its first atom, +u dv on the second
+      const double sg = side == 0 ? -1.0 : 1.0;
+      f[0] += sg * (double)(rux * dv);
+      f[1] += sg * (double)(ruy * dv);
+      f[2] += sg * (double)(ruz * dv);
     }
-  if (mask & T_QQP)
-    for (int p = t; p < S.nqqp; p += QQ_THREADS) {
-      const int s = S.qqp_st[p];
-      const int iq = S.qqp_iq[p];
-      const int i = S.iqseq[iq], j = S.qqp_j[p];
-      const double* pr = S.qqp_par + 4 * p;
-      const double ux = x[3 * j] - x[3 * i], uy = x[3 * j + 1] - x[3 * i + 1], uz = x[3 * j + 2] - x[3 * i + 2];
-      const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
-      R r2, r;
-      rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
-      R r6 = r2 * r2 * r2;
-      double el = 0.0, vdw = 0.0;
-      R dv = PairMath<R>::nbe_qx(r2, r, r6, (R)pr[0], (R)pr[1], (R)pr[2], (R)pr[3], (R)S.lambda[s], el, vdw);
-      qq_add<NS>(eq, s, 2, el, vdw);
-      const double fx = (double)(rux * dv), fy = (double)(ruy * dv), fz = (double)(ruz * dv);
-      unsafeAtomicAdd(&S.d[3 * i], -fx);
-      unsafeAtomicAdd(&S.d[3 * i + 1], -fy);
-      unsafeAtomicAdd(&S.d[3 * i + 2], -fz);
-      unsafeAtomicAdd(&S.d[3 * j], fx);
-      unsafeAtomicAdd(&S.d[3 * j + 1], fy);
-      unsafeAtomicAdd(&S.d[3 * j + 2], fz);
+    f[0] = wave_sum(f[0]);
+    f[1] = wave_sum(f[1]);
+    f[2] = wave_sum(f[2]);
+    if (lane == 0) {
+      d[3 * me] += f[0];
+      d[3 * me + 1] += f[1];
+      d[3 * me + 2] += f[2];
     }
-  if (!want_energies(S)) return;
-  __shared__ double red[QQ_THREADS / QNB_WAVE][NS][4];
+  }
+  if (!want_e) return;
+  __shared__ double red[NW][NS][4];
 #pragma unroll
   for (int s = 0; s < NS; s++)
 #pragma unroll
@@ -1022,12 +1026,11 @@ __global__ __launch_bounds__(QQ_THREADS) void k_qq(const DevSys* __restrict__ sy
   __syncthreads();
   if (t != 0) return;
   double* ep = S.epart + (size_t)slot * QNB_NE;
-  for (int k = 0; k < E_Q0; k++) ep[k] = 0.0;
-  for (int k = E_Q0; k < QNB_NE; k++) ep[k] = 0.0;
+  for (int k = 0; k < QNB_NE; k++) ep[k] = 0.0;
   for (int s = 0; s < NS; s++)
     for (int c = 0; c < 4; c++) {
       double v = 0.0;
-      for (int w = 0; w < QQ_THREADS / QNB_WAVE; w++) v += red[w][s][c];
+      for (int w = 0; w < NW; w++) v += red[w][s][c];
       ep[E_Q0 + 6 * s + c] = v;
     }
 }
