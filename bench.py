@@ -26,8 +26,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X peaks (vendor figures): FP64 vector 78.6 TFLOP/s, FP32 vector 157.3 TFLOP/s
-PEAK_FP64, PEAK_FP32 = 78.6, 157.3
+# MI355X peaks (vendor figures): FP64 vector 78.6 TFLOP/s, FP32 vector 157.3 TFLOP/s, HBM 8 TB/s
+PEAK_FP64, PEAK_FP32, PEAK_HBM_GBPS = 78.6, 157.3, 8000.0
+# HBM bytes per launch of the dominant kernels, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (scaled by a copy kernel of known size) on this
+# workload; the file gives the commands
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_cfg2_51windows_mixed.md")
 # algorithmic FLOPs per unit (div and sqrt count as one)
 F_LRF = 89        # LRF atom-centre update as restated in the engine (20 independent components)
 F_TEST = 8        # group-pair distance test of the list sweep
@@ -54,6 +58,19 @@ def lrf_units(s, x):
         far[a] = False
         units += int(nonq[far].sum())
     return units, ng * ng
+
+
+def measured_traffic(nw, natom):
+    """Bytes per launch from the counter summary, when it is for this workload."""
+    try:
+        for ln in open(TRAFFIC_FILE):
+            if ln.strip().startswith("traffic_bytes_per_launch "):
+                t = json.loads(ln.strip().split(" ", 1)[1])
+                if t.get("workload") == f"{nw} lambda windows, {natom} atoms, mixed":
+                    return t
+    except OSError:
+        pass
+    return None
 
 
 def force_flops(counts, ns):
@@ -231,8 +248,12 @@ def main():
     else:
         dom, t_dom, flop_dom = "k_forces", t_forces, forces_flop
     achieved = flop_dom / (t_dom * 1e-3) / 1e12
+    traffic = measured_traffic(nw, int(s.n["natom"]))
+    t_bytes = traffic.get(dom) if traffic else None
     roofline = {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP32,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4), "traffic": t_bytes,
+                "hbm_GBps": None if t_bytes is None else round(t_bytes / (t_dom * 1e-3) / 1e9, 1),
+                "hbm_frac": None if t_bytes is None else round(t_bytes / (t_dom * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
                 "ms_per_launch": {"k_lists_lrf": round(t_lists, 4), "k_forces": round(t_forces, 4)},
                 "flop_per_launch": {"k_lists_lrf": lists_flop, "k_forces": forces_flop}}
 

@@ -739,20 +739,22 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
           }
         }
       }
-      // Q-atom - solute atom, receiver side (nonbond_qp, nonbondene.f90:5157-5230)
+      // Q-atom - solute atom, receiver side (nonbond_qp, nonbondene.f90:5157-5230):
+      // lane (g, kq) = (lane / nk, lane % nk) takes receiver atom kq against
+      // Q-atoms g, g + G, ... (G = 64 / nk lanes per receiver atom), and its
+      // sum joins f[kq] before the reduction
       if (qin) {
         const R *qtA = qp_tab<R>(S.qpA, S.qpA_f), *qtB = qp_tab<R>(S.qpB, S.qpB_f);
         const R *qtE = qp_tab<R>(S.qpEl, S.qpEl_f), *qtS = qp_tab<R>(S.qpSc, S.qpSc_f);
-        for (int iq = lane; iq < nq; iq += QNB_WAVE) {
-          const int q = ldg(S.iqseq + iq);
-          const R qx = (R)(ldg(x + 3 * q) - xr0), qy = (R)(ldg(x + 3 * q + 1) - xr1),
-                  qz = (R)(ldg(x + 3 * q + 2) - xr2);
-#pragma unroll
-          for (int k = 0; k < CH; k++) {
-            if (k >= nk) break;
-            const int i = s_ia[k];
-            if (i < 0 || ldg(S.qconn3 + i)) continue;
-            const R ux = qx - s_o[k][0], uy = qy - s_o[k][1], uz = qz - s_o[k][2];
+        const int G = QNB_WAVE / nk, kq = lane % nk, g = lane / nk;
+        const int i = s_ia[kq];
+        R fq[3] = {R(0), R(0), R(0)};
+        if (g < G && i >= 0 && !ldg(S.qconn3 + i)) {
+          const R ox = s_o[kq][0], oy = s_o[kq][1], oz = s_o[kq][2];
+          for (int iq = g; iq < nq; iq += G) {
+            const int q = ldg(S.iqseq + iq);
+            const R ux = (R)(ldg(x + 3 * q) - xr0) - ox, uy = (R)(ldg(x + 3 * q + 1) - xr1) - oy,
+                    uz = (R)(ldg(x + 3 * q + 2) - xr2) - oz;
             R r2, r;
             rinv(ux * ux + uy * uy + uz * uz, r2, r);
             const R r6 = r2 * r2 * r2;
@@ -764,11 +766,18 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
               dvs += PairMath<R>::nbe_qx(r2, r, r6, ldg(qtA + t), ldg(qtB + t), ldg(qtE + t), ldg(qtS + t),
                                          (R)S.lambda[s], eq[4 * s + 0], eq[4 * s + 1], want_e);
             }
-            f[k][0] += ux * dvs;
-            f[k][1] += uy * dvs;
-            f[k][2] += uz * dvs;
+            fq[0] += ux * dvs;
+            fq[1] += uy * dvs;
+            fq[2] += uz * dvs;
           }
         }
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+          if (k == kq) {
+            f[k][0] += fq[0];
+            f[k][1] += fq[1];
+            f[k][2] += fq[2];
+          }
       }
       // reduce (the per-atom LRF Taylor term follows in k_atom_terms)
 #pragma unroll
