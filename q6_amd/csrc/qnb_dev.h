@@ -256,27 +256,29 @@ __device__ __forceinline__ float recip(float v) { return __builtin_amdgcn_rcpf(v
 // to R, and the returned scalar dv is in R; callers accumulate u*dv in FP64.
 template <typename R>
 struct PairMath {
-  // nbe (nonbonded.f90:45-75): LJ + Coulomb.  Adds V_el, V_a - V_b.
+  // nbe (nonbonded.f90:45-75): LJ + Coulomb.  Adds V_el, V_a - V_b to the
+  // accumulators (type T: FP64, or R for a short per-partner partial).
   // `want`: accumulate the energies (only the sampled steps need them)
-  static __device__ __forceinline__ R nbe(R ux, R uy, R uz, R A, R B, R q, double& el, double& vdw,
-                                          bool want = true) {
+  template <typename T>
+  static __device__ __forceinline__ R nbe(R ux, R uy, R uz, R A, R B, R q, T& el, T& vdw, bool want = true) {
     R r2, r;
     rinv(ux * ux + uy * uy + uz * uz, r2, r);
     R r6 = r2 * r2 * r2;
     R r12 = r6 * r6;
     R Vel = q * r, Va = A * r12, Vb = B * r6;
     if (want) {
-      el += (double)Vel;
-      vdw += (double)Va - (double)Vb;
+      el += (T)Vel;
+      vdw += (T)Va - (T)Vb;
     }
     return r2 * (-Vel - R(12) * Va + R(6) * Vb);
   }
   // nbe_spc (nonbonded.f90:247-267): Coulomb only
-  static __device__ __forceinline__ R nbe_spc(R ux, R uy, R uz, R q, double& el, bool want = true) {
+  template <typename T>
+  static __device__ __forceinline__ R nbe_spc(R ux, R uy, R uz, R q, T& el, bool want = true) {
     R r2, r;
     rinv(ux * ux + uy * uy + uz * uz, r2, r);
     R Vel = q * r;
-    if (want) el += (double)Vel;
+    if (want) el += (T)Vel;
     return -r2 * Vel;
   }
   // nbe_qx (nonbonded.f90:200-222), soft-core score sc, state weight lam

@@ -548,8 +548,10 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
           const int tag = s_pt[t], j = s_pj[t], kb = tag & 3;
           const R dx = s_pd[t][0], dy = s_pd[t][1], dz = s_pd[t][2];
           if (!(tag & FT_SOL)) {
-            // water-water: receiver atom k, partner atom kb
+            // water-water: receiver atom k, partner atom kb; the partner's
+            // three pair energies are summed in R, then added in FP64
             const bool acc = want_e && (tag & FT_OWN);
+            R pel = R(0), pvdw = R(0);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
               const int tt = k + 3 * kb;
@@ -559,17 +561,20 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
                 // SPC: Coulomb for every pair, LJ for O-O only
                 const R q = kb == 0 ? wel[k] : (kb == 1 ? wel[k + 3] : wel[k + 6]);
                 if (k == 0 && kb == 0)
-                  dv = PairMath<R>::nbe(ux, uy, uz, wA, wB, q, e[E_WW_EL], e[E_WW_VDW], acc);
+                  dv = PairMath<R>::nbe(ux, uy, uz, wA, wB, q, pel, pvdw, acc);
                 else
-                  dv = PairMath<R>::nbe_spc(ux, uy, uz, q, e[E_WW_EL], acc);
+                  dv = PairMath<R>::nbe_spc(ux, uy, uz, q, pel, acc);
               } else {
                 dv = PairMath<R>::nbe(ux, uy, uz, (R)ldg(S.wwA + tt), (R)ldg(S.wwB + tt), (R)ldg(S.wwEl + tt),
-                                      e[E_WW_EL],
-                                      e[E_WW_VDW], acc);
+                                      pel, pvdw, acc);
               }
               f[k][0] += ux * dv;
               f[k][1] += uy * dv;
               f[k][2] += uz * dv;
+            }
+            if (acc) {
+              e[E_WW_EL] += (double)pel;
+              e[E_WW_VDW] += (double)pvdw;
             }
           } else {
             // solute partner j: pw table, energy counted on the solute side
@@ -682,6 +687,9 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
             const bool b_sol = (tag & FT_SOL) != 0;
             const bool acc = want_e && (tag & FT_OWN);
             const R dx = s_pd[t][0], dy = s_pd[t][1], dz = s_pd[t][2];
+            // the partner's pair energies are summed in R over the receiver
+            // atoms, then added in FP64
+            R pel = R(0), pvdw = R(0);
             PPAtom Q{j, 0, 0u, 0u, 0.0};
             if (b_sol && lds_types) {
               Q.t = ldg(S.iac + j);
@@ -704,14 +712,21 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
                 } else if (!pp_param(S, i, j, A, B, el)) {
                   continue;
                 }
-                dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], acc);
+                dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, pel, pvdw, acc);
               } else {
-                dv = PairMath<R>::nbe(ux, uy, uz, s_pw[k][kb][0], s_pw[k][kb][1], s_pw[k][kb][2], e[E_PW_EL],
-                                      e[E_PW_VDW], want_e);
+                dv = PairMath<R>::nbe(ux, uy, uz, s_pw[k][kb][0], s_pw[k][kb][1], s_pw[k][kb][2], pel, pvdw,
+                                      want_e);
               }
               f[k][0] += ux * dv;
               f[k][1] += uy * dv;
               f[k][2] += uz * dv;
+            }
+            if (want_e) {
+              const double de = (double)pel, dw = (double)pvdw;
+              e[E_PP_EL] += b_sol ? de : 0.0;
+              e[E_PP_VDW] += b_sol ? dw : 0.0;
+              e[E_PW_EL] += b_sol ? 0.0 : de;
+              e[E_PW_VDW] += b_sol ? 0.0 : dw;
             }
           }
           __builtin_amdgcn_wave_barrier();
@@ -795,28 +810,33 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
     }
   }
 
-  // energy partials of this wave, on the sampled steps
+  // energy partials of this wave, on the sampled steps: only the terms this
+  // receiver class accumulates are reduced (waters: ww and the Q-water
+  // pairs; solute groups: pp, pw and the Q-solute pairs), the rest are zero
   if (!want_e) return;
   double* ep = S.epart + (size_t)a * QNB_NE;
+  constexpr int k0 = WAT ? E_WW_EL : E_PP_EL, k1 = WAT ? E_WW_VDW : E_PW_VDW;
+  constexpr int q0 = WAT ? 2 : 0;  // eq[4 s + q0 .. q0 + 1]
 #pragma unroll
   for (int k = 0; k < 9; k++) {
-    double v = wave_sum(e[k]);
+    double v = 0.0;
+    if (k >= k0 && k <= k1) v = wave_sum(e[k]);
     if (lane == 0) ep[k] = v;
   }
 #pragma unroll
   for (int s = 0; s < QNB_MAXST; s++) {
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    double v0 = 0.0, v1 = 0.0;
     if (s < NS) {
-#pragma unroll
-      for (int k = 0; k < 4; k++) v[k] = wave_sum(eq[4 * (s < NS ? s : 0) + k]);
+      v0 = wave_sum(eq[4 * (s < NS ? s : 0) + q0]);
+      v1 = wave_sum(eq[4 * (s < NS ? s : 0) + q0 + 1]);
     }
     if (lane == 0) {
       ep[E_Q0 + 6 * s + 0] = 0.0;
       ep[E_Q0 + 6 * s + 1] = 0.0;
-      ep[E_Q0 + 6 * s + 2] = v[0];
-      ep[E_Q0 + 6 * s + 3] = v[1];
-      ep[E_Q0 + 6 * s + 4] = v[2];
-      ep[E_Q0 + 6 * s + 5] = v[3];
+      ep[E_Q0 + 6 * s + 2] = WAT ? 0.0 : v0;
+      ep[E_Q0 + 6 * s + 3] = WAT ? 0.0 : v1;
+      ep[E_Q0 + 6 * s + 4] = WAT ? v0 : 0.0;
+      ep[E_Q0 + 6 * s + 5] = WAT ? v1 : 0.0;
     }
   }
 }
