@@ -468,9 +468,10 @@ static void launch_lists(qnb_batch* b, int wpb) {
 static const uint32_t kStepMask = QNB_T_NONBONDED | QNB_T_QQ | QNB_T_QQP | QNB_T_RAD | QNB_T_POL;
 
 // waves per block of the force kernel, kept within 64 KiB of LDS per block
+// (the per-wave partner slots plus the solute launch's type-pair tables)
 template <typename R>
 static int forces_wpb(int wpb) {
-  const int cap = (int)(65536 / fc_wave_bytes<R>());
+  const int cap = (int)((65536 - fc_static_bytes<R>()) / fc_wave_bytes<R>());
   return std::max(1, std::min(std::min(wpb, FC_WPB), cap));
 }
 

@@ -320,6 +320,12 @@ __host__ __device__ constexpr size_t fc_wave_bytes() {
 }
 // type tables of the solute-solute parameters held in LDS up to this many types
 #define FC_MAXT 64
+// static LDS of the solute-receiver force kernel: the LJ factors and the
+// ljcod table of up to FC_MAXT types
+template <typename R>
+__host__ __device__ constexpr size_t fc_static_bytes() {
+  return 2 * 3 * FC_MAXT * sizeof(double) + FC_MAXT * FC_MAXT;
+}
 
 // Adjoint of the per-state Q-atom energies over a set of frames: tangent
 // tables (derivative of each Q table entry with respect to the Q-side
