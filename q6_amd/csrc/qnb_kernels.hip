@@ -335,6 +335,19 @@ enum : uint32_t { T_WW = 1, T_QW = 2, T_PW = 4, T_PP = 8, T_QP = 16, T_LRF = 32,
 // masks (bit 31: partners beyond the 25-atom window, looked up in the long
 // list) and the type tables (LJ factors, ljcod) in LDS.  Same result as
 // pp_param.
+// the pair (lo, hi) in lo's long exclusion / 1-4 list: 1 excluded, 2 1-4,
+// 0 neither (kept out of line: rare, and its loop would stop the callers'
+// receiver-atom loops from unrolling)
+__device__ __noinline__ int long_pair_kind(const DevSys& S, int lo, int hi) {
+  bool ex = false, is14 = false;
+  for (int m = ldg(S.lx_off + lo); m < ldg(S.lx_off + lo + 1); m++)
+    if (ldg(S.lx_j + m) == hi) {
+      if (ldg(S.lx_kind + m) == 0) ex = true;
+      else is14 = true;
+    }
+  return ex ? 1 : (is14 ? 2 : 0);
+}
+
 struct PPAtom {
   int i, t;
   uint32_t xm, m14;
@@ -353,13 +366,9 @@ __device__ __forceinline__ bool pp_param_fast(const DevSys& S, const PPAtom& P, 
     is14 = (m14 & bit) != 0;
   } else if (xm >> 31) {
     const int lo = pl ? P.i : Q.i, hi = pl ? Q.i : P.i;
-    bool ex = false;
-    for (int m = ldg(S.lx_off + lo); m < ldg(S.lx_off + (lo + 1)); m++)
-      if (ldg(S.lx_j + m) == hi) {
-        if (ldg(S.lx_kind + m) == 0) ex = true;
-        else is14 = true;
-      }
-    if (ex) return false;
+    const int kind = long_pair_kind(S, lo, hi);
+    if (kind == 1) return false;
+    is14 = kind == 2;
   }
   const int vdw = is14 ? 3 : ljc[P.t + Q.t * S.ntypes];
   const int k = vdw - 1;
@@ -697,29 +706,38 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
               Q.m14 = ldg(S.m14mask + j);
               Q.q = ldg(S.crg + j);
             }
+            // receiver atoms k < nk with i >= 0 (no early exits, so the loop
+            // unrolls and f stays in registers)
 #pragma unroll
             for (int k = 0; k < CH; k++) {
-              if (k >= nk) break;
               const int i = s_ia[k];
-              if (i < 0) continue;
-              const R ux = dx - s_o[k][0], uy = dy - s_o[k][1], uz = dz - s_o[k][2];
-              R dv;
-              if (b_sol) {
-                double A, B, el;
-                if (lds_types) {
-                  const PPAtom P{i, s_rt[k], s_rx[k], s_rm[k], s_rq[k]};
-                  if (!pp_param_fast(S, P, Q, s_avdw, s_bvdw, s_ljc, A, B, el)) continue;
-                } else if (!pp_param(S, i, j, A, B, el)) {
-                  continue;
+              bool ok = k < nk && i >= 0;
+              R A = R(0), B = R(0), el = R(0);
+              if (ok) {
+                if (b_sol) {
+                  double Ad, Bd, eld;
+                  if (lds_types) {
+                    const PPAtom P{i, s_rt[k], s_rx[k], s_rm[k], s_rq[k]};
+                    ok = pp_param_fast(S, P, Q, s_avdw, s_bvdw, s_ljc, Ad, Bd, eld);
+                  } else {
+                    ok = pp_param(S, i, j, Ad, Bd, eld);
+                  }
+                  A = (R)Ad;
+                  B = (R)Bd;
+                  el = (R)eld;
+                } else {
+                  A = s_pw[k][kb][0];
+                  B = s_pw[k][kb][1];
+                  el = s_pw[k][kb][2];
                 }
-                dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, pel, pvdw, acc);
-              } else {
-                dv = PairMath<R>::nbe(ux, uy, uz, s_pw[k][kb][0], s_pw[k][kb][1], s_pw[k][kb][2], pel, pvdw,
-                                      want_e);
               }
-              f[k][0] += ux * dv;
-              f[k][1] += uy * dv;
-              f[k][2] += uz * dv;
+              if (ok) {
+                const R ux = dx - s_o[k][0], uy = dy - s_o[k][1], uz = dz - s_o[k][2];
+                const R dv = PairMath<R>::nbe(ux, uy, uz, A, B, el, pel, pvdw, b_sol ? acc : want_e);
+                f[k][0] += ux * dv;
+                f[k][1] += uy * dv;
+                f[k][2] += uz * dv;
+              }
             }
             if (want_e) {
               const double de = (double)pel, dw = (double)pvdw;
@@ -740,16 +758,16 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
                     dz = (R)(ldg(x + 3 * j + 2) - xr2);
 #pragma unroll
             for (int k = 0; k < CH; k++) {
-              if (k >= nk) break;
               const int i = s_ia[k];
-              if (i < 0 || i == j) continue;
-              double A, B, el;
-              if (!pp_param(S, i, j, A, B, el)) continue;
-              const R ux = dx - s_o[k][0], uy = dy - s_o[k][1], uz = dz - s_o[k][2];
-              const R dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], want_e && i < j);
-              f[k][0] += ux * dv;
-              f[k][1] += uy * dv;
-              f[k][2] += uz * dv;
+              double A = 0.0, B = 0.0, el = 0.0;
+              if (k < nk && i >= 0 && i != j && pp_param(S, i, j, A, B, el)) {
+                const R ux = dx - s_o[k][0], uy = dy - s_o[k][1], uz = dz - s_o[k][2];
+                const R dv =
+                    PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], want_e && i < j);
+                f[k][0] += ux * dv;
+                f[k][1] += uy * dv;
+                f[k][2] += uz * dv;
+              }
             }
           }
         }
@@ -797,7 +815,7 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
       // reduce (the per-atom LRF Taylor term follows in k_atom_terms)
 #pragma unroll
       for (int k = 0; k < CH; k++) {
-        if (k >= nk) break;
+        if (k >= nk) continue;
         const double fx = wave_sum((double)f[k][0]), fy = wave_sum((double)f[k][1]), fz = wave_sum((double)f[k][2]);
         if (lane == k) {
           const int i = ldg(S.cgpatom + (g0 + c0 + k));
