@@ -250,6 +250,17 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
     UP(pwA, s->pw_A, (size_t)nsol * sa);
     UP(pwB, s->pw_B, (size_t)nsol * sa);
     UP(pwEl, s->pw_el, (size_t)nsol * sa);
+    if (S.prec == QNB_PREC_MIXED && sa == 3) {
+      std::vector<float> f((size_t)nsol * PW_REC, 0.0f);
+      for (int i = 0; i < nsol; i++)
+        for (int k = 0; k < 3; k++) {
+          const size_t t = (size_t)i + (size_t)nsol * k;
+          f[(size_t)PW_REC * i + 3 * k] = (float)s->pw_A[t];
+          f[(size_t)PW_REC * i + 3 * k + 1] = (float)s->pw_B[t];
+          f[(size_t)PW_REC * i + 3 * k + 2] = (float)s->pw_el[t];
+        }
+      UP(pw_f, f.data(), f.size());
+    }
   }
   if (s->nwat > 0) {
     UP(wwA, s->ww_A, sa * sa);

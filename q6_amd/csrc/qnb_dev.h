@@ -61,6 +61,7 @@ struct DevSys {
   const double* pwA;       // [i + j*nat_solute]
   const double* pwB;
   const double* pwEl;
+  const float* pw_f;  // mixed path: [i][PW_REC] A, B, el of atom i with solvent sites 0..2 (FP32)
   const double* wwA;       // [la + ka*sa]
   const double* wwB;
   const double* wwEl;
@@ -300,6 +301,10 @@ struct PairMath {
 // staged partner atoms and FC_CH receiver atoms per sweep; per-wave LDS:
 // partner offsets and receiver offsets (R x 3), partner index and tag,
 // receiver index
+// floats per solute atom in the FP32 solute-water table (3 sites x A, B, el,
+// padded to 16-byte records)
+#define PW_REC 12
+
 // threads of the Q-Q kernel's block (one block per system)
 #define QQ_THREADS 1024
 // threads per block of the polarisation-restraint kernels (one per water)

@@ -586,15 +586,29 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
               e[E_WW_VDW] += (double)pvdw;
             }
           } else {
-            // solute partner j: pw table, energy counted on the solute side
+            // solute partner j: pw table (on the mixed path its FP32 copy,
+            // one 48-byte record per solute atom), energy counted on the
+            // solute side
             if (tag & FT_Q) continue;
+            R pw[9];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-              const size_t tt = (size_t)j + (size_t)nsol * k;
+              if constexpr (sizeof(R) == 4) {
+                pw[3 * k] = ldg(S.pw_f + PW_REC * j + 3 * k);
+                pw[3 * k + 1] = ldg(S.pw_f + PW_REC * j + 3 * k + 1);
+                pw[3 * k + 2] = ldg(S.pw_f + PW_REC * j + 3 * k + 2);
+              } else {
+                const size_t tt = (size_t)j + (size_t)nsol * k;
+                pw[3 * k] = (R)ldg(S.pwA + tt);
+                pw[3 * k + 1] = (R)ldg(S.pwB + tt);
+                pw[3 * k + 2] = (R)ldg(S.pwEl + tt);
+              }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
               const R ux = dx - o[k][0], uy = dy - o[k][1], uz = dz - o[k][2];
-              double ee = 0.0, ev = 0.0;
-              const R dv = PairMath<R>::nbe(ux, uy, uz, (R)ldg(S.pwA + tt), (R)ldg(S.pwB + tt), (R)ldg(S.pwEl + tt),
-                                            ee, ev, false);
+              R ee = R(0), ev = R(0);
+              const R dv = PairMath<R>::nbe(ux, uy, uz, pw[3 * k], pw[3 * k + 1], pw[3 * k + 2], ee, ev, false);
               f[k][0] += ux * dv;
               f[k][1] += uy * dv;
               f[k][2] += uz * dv;
