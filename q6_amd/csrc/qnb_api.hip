@@ -22,6 +22,7 @@ template <typename R, int NS>
 __global__ void k_qforces(const DevSys*, uint32_t, int);
 template <typename R, int NS>
 __global__ void k_qq(const DevSys*, uint32_t);
+__global__ void k_pack_xq(const DevSys*);
 __global__ void k_watpol_theta(const DevSys*, const int*);
 __global__ void k_watpol_rank(const DevSys*);
 __global__ void k_watpol_force(const DevSys*);
@@ -359,6 +360,7 @@ int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots) {
   } while (0)
   const int nat = S.natom, nsol1 = std::max(S.nat_solute, 1), nw1 = std::max(S.nwat, 1);
   AL(x, 3 * nat);
+  if (p->precision == QNB_PREC_MIXED) AL(xq, 4 * (size_t)nat);
   AL(d, 3 * nat);
   AL(lrf, (size_t)QNB_NLRF * S.ncgp);
   if (S.cap <= 0) S.cap = 256;
@@ -496,6 +498,8 @@ template <typename R>
 static void launch_lists(qnb_handle* h) {
   const int wpb = 4;
   dim3 grid((h->hs.ncgp + wpb - 1) / wpb, 1);
+  if (sizeof(R) == 4)
+    hipLaunchKernelGGL(k_pack_xq, dim3((h->hs.natom + 255) / 256, 1), dim3(256), 0, h->stream, h->dsys);
   hipLaunchKernelGGL(k_lists_lrf<R>, grid, dim3(64 * wpb), 0, h->stream, h->dsys);
 }
 

@@ -18,6 +18,7 @@ namespace qnb {
 template <typename R>
 __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
+__global__ void k_pack_xq(const DevSys*);
 template <typename R, int NS, bool WAT, bool E>
 __global__ void k_forces(const DevSys*, uint32_t, const double*);
 template <typename R, int NS>
@@ -461,8 +462,12 @@ struct qnb_batch {
 // launchers shared by the step, the minimiser, the timer and the autotuner
 static void launch_lists(qnb_batch* b, int wpb) {
   dim3 g((b->maxg + wpb - 1) / wpb, b->n), t(64 * wpb);
-  if (b->prec == QNB_PREC_MIXED) hipLaunchKernelGGL(k_lists_lrf<float>, g, t, 0, b->stream, b->dsys);
-  else hipLaunchKernelGGL(k_lists_lrf<double>, g, t, 0, b->stream, b->dsys);
+  if (b->prec == QNB_PREC_MIXED) {
+    hipLaunchKernelGGL(k_pack_xq, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys);
+    hipLaunchKernelGGL(k_lists_lrf<float>, g, t, 0, b->stream, b->dsys);
+  } else {
+    hipLaunchKernelGGL(k_lists_lrf<double>, g, t, 0, b->stream, b->dsys);
+  }
 }
 
 static const uint32_t kStepMask = QNB_T_NONBONDED | QNB_T_QQ | QNB_T_QQP | QNB_T_RAD | QNB_T_POL;
@@ -739,10 +744,7 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     HIPCHK(hipMemcpy(b->dsys, b->hs.data(), sizeof(DevSys) * n, hipMemcpyHostToDevice));
     for (int k = 0; k < n; k++) HIPCHK(hipMemset(b->hs[k].need, 0, sizeof(int)));
     const int wpb = 4;
-    if (b->prec == QNB_PREC_MIXED)
-      hipLaunchKernelGGL(k_lists_lrf<float>, dim3((b->maxg + wpb - 1) / wpb, n), dim3(64 * wpb), 0, b->stream, b->dsys);
-    else
-      hipLaunchKernelGGL(k_lists_lrf<double>, dim3((b->maxg + wpb - 1) / wpb, n), dim3(64 * wpb), 0, b->stream, b->dsys);
+    launch_lists(b.get(), wpb);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(b->stream));
     bool again = false;
@@ -778,10 +780,7 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
   HIPCHK(hipMemcpy(b->dsys, b->hs.data(), sizeof(DevSys) * n, hipMemcpyHostToDevice));
   {
     const int wpb = 4;
-    if (b->prec == QNB_PREC_MIXED)
-      hipLaunchKernelGGL(k_lists_lrf<float>, dim3((b->maxg + wpb - 1) / wpb, n), dim3(64 * wpb), 0, b->stream, b->dsys);
-    else
-      hipLaunchKernelGGL(k_lists_lrf<double>, dim3((b->maxg + wpb - 1) / wpb, n), dim3(64 * wpb), 0, b->stream, b->dsys);
+    launch_lists(b.get(), wpb);
   }
   if (b->anyq) hipLaunchKernelGGL(k_qlists, dim3(n), dim3(1024), 0, b->stream, b->dsys);
   // initial temperature (md_run calls temperature() before the loop)

@@ -101,6 +101,7 @@ struct DevSys {
 
   // per-instance state
   double* x;
+  float* xq;  // mixed path: [natom][4] x, y, z (FP32) and LRF charge, packed before each list update
   double* d;
   double* lrf;
   int* nbr;

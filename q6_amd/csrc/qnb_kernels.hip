@@ -70,6 +70,55 @@ __device__ __forceinline__ void lrf_accumulate(R* acc, double drx, double dry, d
   acc[19] += gz * zz - R(3) * h * dz;
 }
 
+// FP32 form for the mixed path: dr already formed in FP32 from the packed
+// positions (x_i and the centre each rounded once), q the atom's charge
+__device__ __forceinline__ void lrf_accumulate_f(float* acc, float dx, float dy, float dz, float q) {
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  float ri2, ri;
+  rinv(r2, ri2, ri);
+  const float f0 = q * ri * ri2;
+  const float f1 = 3.0f * f0 * ri2;
+  const float f2 = -f1 * ri2;
+  acc[0] += f0 * r2;
+  acc[1] -= dx * f0;
+  acc[2] -= dy * f0;
+  acc[3] -= dz * f0;
+  const float xx = dx * dx, xy = dx * dy, xz = dx * dz, yy = dy * dy, yz = dy * dz, zz = dz * dz;
+  acc[4] += xx * f1 - f0;
+  acc[5] += xy * f1;
+  acc[6] += xz * f1;
+  acc[7] += yy * f1 - f0;
+  acc[8] += yz * f1;
+  acc[9] += zz * f1 - f0;
+  const float g = 5.0f * f2, h = f2 * r2;
+  const float gx = g * dx, gy = g * dy, gz = g * dz;
+  acc[10] += gx * xx - 3.0f * h * dx;
+  acc[11] += gx * xy - h * dy;
+  acc[12] += gx * xz - h * dz;
+  acc[13] += gx * yy - h * dx;
+  acc[14] += gx * yz;
+  acc[15] += gx * zz - h * dx;
+  acc[16] += gy * yy - 3.0f * h * dy;
+  acc[17] += gy * yz - h * dz;
+  acc[18] += gy * zz - h * dy;
+  acc[19] += gz * zz - 3.0f * h * dz;
+}
+
+// Packs the mixed path's LRF sources before a list update: per atom x, y, z
+// rounded to FP32 and the charge (0 for Q-atoms, which take no part in the
+// expansion), 16 bytes, so the sweep loads one record per source atom.
+__global__ __launch_bounds__(256) void k_pack_xq(const DevSys* __restrict__ systems) {
+  const DevSys& S = systems[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S.natom) return;
+  const auto x = gp(S.x);
+  const auto o = gpm(S.xq);
+  o[4 * i] = (float)x[3 * i];
+  o[4 * i + 1] = (float)x[3 * i + 1];
+  o[4 * i + 2] = (float)x[3 * i + 2];
+  o[4 * i + 3] = ldg(S.iqatom + i) != 0 ? 0.0f : (float)ldg(S.crg + i);
+}
+
 // index of the symmetric third-rank component (a<=b<=c sorted) in acc[10..]
 __device__ __forceinline__ int t3index(int a, int b, int c) {
   // sort three indices
@@ -115,6 +164,7 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
   cx = cx / nat;
   cy = cy / nat;
   cz = cz / nat;
+  const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
   double acc[20];
 #pragma unroll
   for (int k = 0; k < 20; k++) acc[k] = 0.0;
@@ -136,7 +186,15 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
             R part[20];
 #pragma unroll
             for (int k = 0; k < 20; k++) part[k] = R(0);
-            if (b_sol) {
+            if constexpr (sizeof(R) == 4) {
+              // mixed path: packed FP32 positions and charges (Q-atoms carry 0)
+              const auto xq = gp(S.xq);
+              const int k0 = b_sol ? ldg(S.gfirst + b) : 0, k1 = b_sol ? ldg(S.glast + b) : S.sa - 1;
+              for (int k = k0; k <= k1; k++) {
+                const int i = b_sol ? ldg(S.cgpatom + k) : rb + k;
+                lrf_accumulate_f(part, xq[4 * i] - cxf, xq[4 * i + 1] - cyf, xq[4 * i + 2] - czf, xq[4 * i + 3]);
+              }
+            } else if (b_sol) {
               const int k1 = ldg(S.glast + b);
               for (int k = ldg(S.gfirst + b); k <= k1; k++) {
                 const int i = ldg(S.cgpatom + k);
