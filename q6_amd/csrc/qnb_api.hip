@@ -551,12 +551,11 @@ static void launch_eval(qnb_handle* h, uint32_t mask) {
   hipLaunchKernelGGL(k_atom_terms, dim3((S.natom + 255) / 256, 1), dim3(256), 0, h->stream, h->dsys, mask,
                      (const double*)h->d_tfree);
   if (S.nqat > 0 && (mask & (QNB_T_QW | QNB_T_QP))) {
-    // partners per Q-atom: at most nwat + nat_solute; about 8 per lane
+    // partners per Q-atom: at most nwat + nat_solute; about QF_PART / 64 per lane
     int maxp = S.nwat + S.nat_solute;
-    int chunks = std::max(1, std::min((maxp + 511) / 512, 64));
-    int waves = S.nqat * chunks;
-    hipLaunchKernelGGL((k_qforces<R, NS>), dim3((waves + wpb - 1) / wpb, 1), dim3(64 * wpb), 0, h->stream, h->dsys,
-                       mask, chunks);
+    int chunks = std::max(1, std::min((maxp + QF_PART - 1) / QF_PART, 64));
+    hipLaunchKernelGGL((k_qforces<R, NS>), dim3(qforces_blocks(S.nqat, chunks, wpb), 1), dim3(64 * wpb), 0,
+                       h->stream, h->dsys, mask, chunks);
   }
   const int with_qq = S.nqat > 0 && (mask & (QNB_T_QQ | QNB_T_QQP));
   const int with_pol = S.nwat > 0 && S.wpol && S.nshell > 0 && (mask & QNB_T_POL);

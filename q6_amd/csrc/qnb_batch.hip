@@ -447,7 +447,7 @@ struct qnb_batch {
   int maxg = 0, maxat = 0, maxsw = 0, maxw = 0, maxs = 0, ns_max = 1, prec = 0;
   int nbblk = 1;  // k_bonded blocks per instance
   bool anyq = false, anypol = false, static_q = true;
-  int qchunks = 1, qwaves = 0;
+  int qchunks = 1, nq_max = 0;
   int wpb_lists = 4, wpb_forces = 4;  // waves per block, picked by qnb_batch_autotune
   std::vector<int> nbcycle;
   std::vector<int> ene_every;  // per instance energy sampling interval
@@ -526,7 +526,7 @@ static void enqueue_forces(qnb_batch* b, bool energy = true) {
   const uint32_t mask = kStepMask;
   launch_forces_t<R, NS>(b, b->wpb_forces, energy);
   if (b->anyq) {
-    hipLaunchKernelGGL((k_qforces<R, NS>), dim3(xcd_cols((b->qwaves + wpb - 1) / wpb), B), dim3(64 * wpb), 0,
+    hipLaunchKernelGGL((k_qforces<R, NS>), dim3(qforces_blocks(b->nq_max, b->qchunks, wpb), B), dim3(64 * wpb), 0,
                        b->stream, b->dsys, mask, b->qchunks);
     hipLaunchKernelGGL((k_qq<R, NS>), dim3(B), dim3(QQ_THREADS), 0, b->stream, b->dsys, mask);
   }
@@ -634,12 +634,12 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     b->anyq = b->anyq || S.nqat > 0;
     b->anypol = b->anypol || (S.nwat > 0 && S.wpol && S.nshell > 0);
     b->static_q = b->static_q && (S.rcq2 > S.rexcl2);
-    // about 8 partners per lane
-    b->qchunks = std::max(b->qchunks, std::min((S.nwat + S.nat_solute + 511) / 512, 64));
+    // about QF_PART / 64 partners per lane of k_qforces
+    b->qchunks = std::max(b->qchunks, std::min((S.nwat + S.nat_solute + QF_PART - 1) / QF_PART, 64));
     b->nbcycle.push_back(md[k]->nbcycle);
     if (md[k]->nbcycle != md[0]->nbcycle) return fail(QNB_EINVAL, "all instances need the same nbcycle");
   }
-  for (int k = 0; k < n; k++) b->qwaves = std::max(b->qwaves, b->hs[k].nqat * b->qchunks);
+  for (int k = 0; k < n; k++) b->nq_max = std::max(b->nq_max, b->hs[k].nqat);
   HIPCHK(hipStreamCreate(&b->stream));
   if ((rc = b->mem.alloc(&b->tfree, n)) || (rc = b->mem.alloc(&b->istep, n))) return rc;
   HIPCHK(hipMemset(b->istep, 0, sizeof(int) * n));

@@ -306,6 +306,15 @@ struct PairMath {
 // padded to 16-byte records)
 #define PW_REC 12
 
+// partners per wave of the Q-atom force kernel (the Q-atom's partners are
+// split into chunks of about this many)
+#ifndef QF_PART
+#define QF_PART 512
+#endif
+// grid.x of the Q-atom force kernel: per XCD (8) and per Q-atom group of 8,
+// the blocks of one Q-atom's chunks (W waves per block)
+inline int qforces_blocks(int nq, int chunks, int W) { return 8 * ((chunks + W - 1) / W) * ((nq + 7) / 8); }
+
 // threads of the Q-Q kernel's block (one block per system)
 #define QQ_THREADS 1024
 // threads per block of the polarisation-restraint kernels (one per water)

@@ -962,16 +962,22 @@ __global__ __launch_bounds__(256) void k_atom_terms(const DevSys* __restrict__ s
 }
 
 // ---------------------------------------------------------------------------
-// Q-atom side of the Q-water and Q-solute pairs: wave = (Q-atom, 64 partners);
-// the wave's sum is added to the Q-atom's gradient with atomics (few per step).
+// Q-atom side of the Q-water and Q-solute pairs: wave = (Q-atom, chunk of its
+// partners, 64 at a time); the wave's sum is added to the Q-atom's gradient
+// with atomics (a few per Q-atom).
 template <typename R, int NS>
 __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ systems, uint32_t mask,
                                                  int chunks_per_q) {
   const DevSys& S = systems[blockIdx.y];
-  const int wv = blockIdx.x * (blockDim.x / QNB_WAVE) + threadIdx.x / QNB_WAVE;
+  // block x runs on XCD x % 8 and takes Q-atom iq = x % 8 + 8 * (x / 8 / bq),
+  // chunks (x / 8 % bq) * W .. + W - 1 (W waves per block, bq blocks per
+  // Q-atom): an XCD serves only the Q-atoms iq = its index mod 8, so the rows
+  // of the Q tables it reads stay in its L2 across the instances
+  const int W = blockDim.x / QNB_WAVE, bq = (chunks_per_q + W - 1) / W;
+  const int m = blockIdx.x / 8;
+  const int iq = blockIdx.x % 8 + 8 * (m / bq), ch = (m % bq) * W + threadIdx.x / QNB_WAVE;
   const int lane = threadIdx.x % QNB_WAVE;
-  const int iq = wv / chunks_per_q, ch = wv % chunks_per_q;
-  if (iq >= S.nqat) return;
+  if (iq >= S.nqat || ch >= chunks_per_q) return;
   const int nqw = ldg(S.nq_lists + 0), nqp = ldg(S.nq_lists + 1);
   const int nw_ch = (mask & T_QW) ? (nqw + QNB_WAVE - 1) / QNB_WAVE : 0;
   const int np_ch = (mask & T_QP) ? (nqp + QNB_WAVE - 1) / QNB_WAVE : 0;
