@@ -38,6 +38,22 @@ def test_first_frame_matches_single_engine(small):
         qw = f0[abi.F_Q0 + 6 * st + 4]
         assert abs(qw - EQ[st]["qw_el"]) <= 1e-10 * max(abs(EQ[st]["qw_el"]), 1.0)
     eng.close()
+    # harmonic bonds (with the switch-atom restraints) and angles at x0
+    # (bond, angle: src/bondene.f90:41-246), each term counted once
+    md, x = s.md, s.x
+    ij = np.concatenate([np.asarray(md["bonds"]).reshape(-1, 2), np.asarray(md["restraints"]).reshape(-1, 2)])
+    r0 = np.concatenate([np.broadcast_to(md["bond_r0"], len(md["bonds"])),
+                         np.broadcast_to(md["restraint_r0"], len(md["restraints"]))])
+    kb = np.concatenate([np.broadcast_to(md["bond_k"], len(md["bonds"])),
+                         np.broadcast_to(md["restraint_k"], len(md["restraints"]))])
+    r = np.linalg.norm(x[ij[:, 1]] - x[ij[:, 0]], axis=1)
+    eb = float(np.sum(0.5 * kb * (r - r0) ** 2))
+    ijk = np.asarray(md["angles"]).reshape(-1, 3)
+    u, w = x[ijk[:, 0]] - x[ijk[:, 1]], x[ijk[:, 2]] - x[ijk[:, 1]]
+    ct = np.clip(np.sum(u * w, 1) / (np.linalg.norm(u, axis=1) * np.linalg.norm(w, axis=1)), -1.0, 1.0)
+    ea = float(np.sum(0.5 * md["angle_k"] * (np.arccos(ct) - md["angle_t0"]) ** 2))
+    assert abs(f0[abi.F_BOND] - eb) <= 1e-10 * max(abs(eb), 1.0), (f0[abi.F_BOND], eb)
+    assert abs(f0[abi.F_ANGLE] - ea) <= 1e-10 * max(abs(ea), 1.0), (f0[abi.F_ANGLE], ea)
 
 
 def test_windows_are_independent_and_graph_matches_eager(small):
