@@ -71,37 +71,73 @@ __device__ __forceinline__ void lrf_accumulate(R* acc, double drx, double dry, d
 }
 
 // FP32 form for the mixed path: dr already formed in FP32 from the packed
-// positions (x_i and the centre each rounded once), q the atom's charge
-__device__ __forceinline__ void lrf_accumulate_f(float* acc, float dx, float dy, float dz, float q) {
+// positions (x_i and the centre each rounded once), q the atom's charge.
+// The terms are kept as raw sums that need no per-atom subtraction, p[0..19]
+// as acc[0..19] without the -f0 (phi2 diagonal) and -h dr (phi3) parts,
+// p[20] = sum f0, p[21..23] = sum h dr; lrf_flush_f forms the components in
+// FP64 when it adds a source group's sums.
+#define LRF_NF 24
+__device__ __forceinline__ void lrf_accumulate_f(float* p, float dx, float dy, float dz, float q) {
   const float r2 = dx * dx + dy * dy + dz * dz;
   float ri2, ri;
   rinv(r2, ri2, ri);
   const float f0 = q * ri * ri2;
   const float f1 = 3.0f * f0 * ri2;
   const float f2 = -f1 * ri2;
-  acc[0] += f0 * r2;
-  acc[1] -= dx * f0;
-  acc[2] -= dy * f0;
-  acc[3] -= dz * f0;
+  p[0] += f0 * r2;
+  p[1] += dx * f0;
+  p[2] += dy * f0;
+  p[3] += dz * f0;
   const float xx = dx * dx, xy = dx * dy, xz = dx * dz, yy = dy * dy, yz = dy * dz, zz = dz * dz;
-  acc[4] += xx * f1 - f0;
-  acc[5] += xy * f1;
-  acc[6] += xz * f1;
-  acc[7] += yy * f1 - f0;
-  acc[8] += yz * f1;
-  acc[9] += zz * f1 - f0;
+  p[4] += xx * f1;
+  p[5] += xy * f1;
+  p[6] += xz * f1;
+  p[7] += yy * f1;
+  p[8] += yz * f1;
+  p[9] += zz * f1;
+  p[20] += f0;
   const float g = 5.0f * f2, h = f2 * r2;
   const float gx = g * dx, gy = g * dy, gz = g * dz;
-  acc[10] += gx * xx - 3.0f * h * dx;
-  acc[11] += gx * xy - h * dy;
-  acc[12] += gx * xz - h * dz;
-  acc[13] += gx * yy - h * dx;
-  acc[14] += gx * yz;
-  acc[15] += gx * zz - h * dx;
-  acc[16] += gy * yy - 3.0f * h * dy;
-  acc[17] += gy * yz - h * dz;
-  acc[18] += gy * zz - h * dy;
-  acc[19] += gz * zz - 3.0f * h * dz;
+  p[10] += gx * xx;
+  p[11] += gx * xy;
+  p[12] += gx * xz;
+  p[13] += gx * yy;
+  p[14] += gx * yz;
+  p[15] += gx * zz;
+  p[16] += gy * yy;
+  p[17] += gy * yz;
+  p[18] += gy * zz;
+  p[19] += gz * zz;
+  p[21] += h * dx;
+  p[22] += h * dy;
+  p[23] += h * dz;
+}
+
+// adds one source group's raw FP32 sums to the FP64 expansion, forming
+// phi1 = -sum dr f0, phi2_aa = sum d_a d_a f1 - sum f0 and the -h dr parts
+// of phi3 in FP64
+__device__ __forceinline__ void lrf_flush_f(double* acc, const float* p) {
+  const double s0 = p[20], hx = p[21], hy = p[22], hz = p[23];
+  acc[0] += (double)p[0];
+  acc[1] -= (double)p[1];
+  acc[2] -= (double)p[2];
+  acc[3] -= (double)p[3];
+  acc[4] += (double)p[4] - s0;
+  acc[5] += (double)p[5];
+  acc[6] += (double)p[6];
+  acc[7] += (double)p[7] - s0;
+  acc[8] += (double)p[8];
+  acc[9] += (double)p[9] - s0;
+  acc[10] += (double)p[10] - 3.0 * hx;
+  acc[11] += (double)p[11] - hy;
+  acc[12] += (double)p[12] - hz;
+  acc[13] += (double)p[13] - hx;
+  acc[14] += (double)p[14];
+  acc[15] += (double)p[15] - hx;
+  acc[16] += (double)p[16] - 3.0 * hy;
+  acc[17] += (double)p[17] - hz;
+  acc[18] += (double)p[18] - hy;
+  acc[19] += (double)p[19] - 3.0 * hz;
 }
 
 // Packs the mixed path's LRF sources before a list update: per atom x, y, z
@@ -189,11 +225,15 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
             if constexpr (sizeof(R) == 4) {
               // mixed path: packed FP32 positions and charges (Q-atoms carry 0)
               const auto xq = gp(S.xq);
+              float pf[LRF_NF];
+#pragma unroll
+              for (int k = 0; k < LRF_NF; k++) pf[k] = 0.0f;
               const int k0 = b_sol ? ldg(S.gfirst + b) : 0, k1 = b_sol ? ldg(S.glast + b) : S.sa - 1;
               for (int k = k0; k <= k1; k++) {
                 const int i = b_sol ? ldg(S.cgpatom + k) : rb + k;
-                lrf_accumulate_f(part, xq[4 * i] - cxf, xq[4 * i + 1] - cyf, xq[4 * i + 2] - czf, xq[4 * i + 3]);
+                lrf_accumulate_f(pf, xq[4 * i] - cxf, xq[4 * i + 1] - cyf, xq[4 * i + 2] - czf, xq[4 * i + 3]);
               }
+              lrf_flush_f(acc, pf);
             } else if (b_sol) {
               const int k1 = ldg(S.glast + b);
               for (int k = ldg(S.gfirst + b); k <= k1; k++) {
@@ -209,8 +249,10 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
                                   ldg(S.crg + i));
               }
             }
+            if constexpr (sizeof(R) != 4) {
 #pragma unroll
-            for (int k = 0; k < 20; k++) acc[k] += (double)part[k];
+              for (int k = 0; k < 20; k++) acc[k] += (double)part[k];
+            }
           }
         }
       }
