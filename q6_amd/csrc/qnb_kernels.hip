@@ -863,27 +863,59 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
           }
           __builtin_amdgcn_wave_barrier();
         }
-        // pairs inside the group (ig == jgr, i < j, nonbondene.f90:1610)
+        // pairs inside the group (ig == jgr, i < j, nonbondene.f90:1610):
+        // lane (g, kq) = (lane / nk, lane % nk) takes receiver atom kq against
+        // group atoms q = g, g + G, ... (G = 64 / nk); an atom of this sweep
+        // has its data in LDS, others (groups larger than one sweep) are read
+        // from memory
         if (mask & T_PP) {
-          for (int q = lane; q < na; q += QNB_WAVE) {
-            const int j = ldg(S.cgpatom + (g0 + q));
-            if (ldg(S.iqatom + j) != 0) continue;
-            const R dx = (R)(ldg(x + 3 * j) - xr0), dy = (R)(ldg(x + 3 * j + 1) - xr1),
-                    dz = (R)(ldg(x + 3 * j + 2) - xr2);
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-              const int i = s_ia[k];
-              double A = 0.0, B = 0.0, el = 0.0;
-              if (k < nk && i >= 0 && i != j && pp_param(S, i, j, A, B, el)) {
-                const R ux = dx - s_o[k][0], uy = dy - s_o[k][1], uz = dz - s_o[k][2];
-                const R dv =
-                    PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], want_e && i < j);
-                f[k][0] += ux * dv;
-                f[k][1] += uy * dv;
-                f[k][2] += uz * dv;
+          const int G = QNB_WAVE / nk, kq = lane % nk, g = lane / nk;
+          const int i = s_ia[kq];
+          R fq[3] = {R(0), R(0), R(0)};
+          if (g < G && i >= 0) {
+            const R ox = s_o[kq][0], oy = s_o[kq][1], oz = s_o[kq][2];
+            for (int q = g; q < na; q += G) {
+              const int ql = q - c0;
+              const bool in_sweep = ql >= 0 && ql < nk;
+              int j;
+              R dx, dy, dz;
+              if (in_sweep) {
+                j = s_ia[ql];  // -1: a Q-atom, whose pairs belong to the Q kernels
+                dx = s_o[ql][0];
+                dy = s_o[ql][1];
+                dz = s_o[ql][2];
+              } else {
+                j = ldg(S.cgpatom + (g0 + q));
+                if (ldg(S.iqatom + j) != 0) continue;
+                dx = (R)(ldg(x + 3 * j) - xr0);
+                dy = (R)(ldg(x + 3 * j + 1) - xr1);
+                dz = (R)(ldg(x + 3 * j + 2) - xr2);
               }
+              if (j < 0 || j == i) continue;
+              double A, B, el;
+              bool ok;
+              if (in_sweep && lds_types) {
+                const PPAtom P{i, s_rt[kq], s_rx[kq], s_rm[kq], s_rq[kq]};
+                const PPAtom Q{j, s_rt[ql], s_rx[ql], s_rm[ql], s_rq[ql]};
+                ok = pp_param_fast(S, P, Q, s_avdw, s_bvdw, s_ljc, A, B, el);
+              } else {
+                ok = pp_param(S, i, j, A, B, el);
+              }
+              if (!ok) continue;
+              const R ux = dx - ox, uy = dy - oy, uz = dz - oz;
+              const R dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], want_e && i < j);
+              fq[0] += ux * dv;
+              fq[1] += uy * dv;
+              fq[2] += uz * dv;
             }
           }
+#pragma unroll
+          for (int k = 0; k < CH; k++)
+            if (k == kq) {
+              f[k][0] += fq[0];
+              f[k][1] += fq[1];
+              f[k][2] += fq[2];
+            }
         }
       }
       // Q-atom - solute atom, receiver side (nonbond_qp, nonbondene.f90:5157-5230):
