@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define QNB_ABI_VERSION 1
+#define QNB_ABI_VERSION 2
 
 enum qnb_status {
   QNB_OK = 0,
@@ -266,6 +266,51 @@ int32_t qnb_get_shell_state(qnb_handle *h, double *state);
 int32_t qnb_set_shell_state(qnb_handle *h, const double *state);
 
 /* ------------------------------------------------------------------------
+ * Q-atom bonded terms per state: the FEP file's changing bonds, angles,
+ * torsions and impropers with their Morse couplings (TYPE(QBOND_TYPE) ...
+ * src/qatom.f90:72-122, as get_fep leaves them).  Atom and term numbers are
+ * 0-based; cod[t*nstates + s] is the 1-based row of the term's library used in
+ * state s, 0 = term absent in s.  Angles in radians, as the reference holds
+ * them after reading (src/qatom.f90:1112, 1312, 1506). */
+typedef struct qnb_qbonded {
+  int32_t nbond, nangle, ntorsion, nimproper;
+  int32_t nbond_lib, nangle_lib, ntorsion_lib, nimproper_lib;
+  const int32_t *bond_ij, *bond_cod;   /* [nbond][2], [nbond][nstates]               */
+  const double *bond_lib;              /* [][4] Dmz, amz, r0, fk (qbondlib)           */
+  const int32_t *angle_ijk, *angle_cod;
+  const double *angle_lib;             /* [][4] fk, ang0, ureyfk, ureyr0 (qanglib)    */
+  const int32_t *torsion_ijkl, *torsion_cod;
+  const double *torsion_lib;           /* [][3] fk, rmult, deltor (qtorlib)           */
+  const int32_t *improper_ijkl, *improper_cod;
+  const double *improper_lib;          /* [][2] fk, imp0 (qimplib)                    */
+  /* Morse couplings [n][3]: term, bond, flag (iang_coupl / itor_coupl /
+   * iimp_coupl(1:3, :), src/qatom.f90:1688-1800); flag 1 scales the term by
+   * 1 - EMorseD(bond) instead of EMorseD(bond) */
+  int32_t nangle_coupl, ntorsion_coupl, nimproper_coupl;
+  const int32_t *angle_coupl, *torsion_coupl, *improper_coupl;
+  int32_t urey_bradley; /* 1: qurey_bradley after qangle (CHARMM, src/potene.f90:187) */
+} qnb_qbonded;
+
+/* Capacity of one Q bonded set (terms of all four kinds, and Q bonds). */
+#define QNB_QB_MAX_TERMS 4096
+#define QNB_QB_MAX_BONDS 1024
+
+/* Loads (or with qb = NULL / no terms, removes) the Q bonded terms of a
+ * handle and zeroes their Morse coupling state, as qatom_load_fep does
+ * (EMorseD = dMorse = 0, src/qatom.f90:1000-1004). */
+int32_t qnb_set_qbonded(qnb_handle *h, const qnb_qbonded *qb);
+
+/* One pass of pot_energy's per-state Q bonded loop (src/potene.f90:183-190):
+ * for each state qbond, qangle, [qurey_bradley], qtorsion, qimproper
+ * (src/bondene.f90:387-685) at x, ADDING the lambda-weighted gradient into d
+ * (natom x 3, on the host) and writing eq[s*4 + {0,1,2,3}] = EQ(s)%q%bond,
+ * %angle (Urey-Bradley included), %torsion, %improper.  A coupled term reads
+ * the Morse state (EMorseD, dMorse) its bond left at its last evaluation,
+ * in this call or an earlier one, exactly as the reference's module arrays
+ * carry it. */
+int32_t qnb_eval_qbonded(qnb_handle *h, const double *x, double *d, double *eq);
+
+/* ------------------------------------------------------------------------
  * Device-resident MD over a batch of instances (lambda windows of one system,
  * or independent systems of uneven size).  The step follows md_run
  * (src/md.f90:1657-1871) for leap-frog + Berendsen: nonbonded engine (list
@@ -293,13 +338,14 @@ typedef struct qnb_md_params {
 } qnb_md_params;
 
 /* per-frame record of the energy history (float64 each) */
-#define QNB_FRAME_LEN 72
+#define QNB_FRAME_LEN 104
 enum qnb_frame_field {
   QNB_F_PP_EL = 0, QNB_F_PP_VDW, QNB_F_PW_EL, QNB_F_PW_VDW, QNB_F_WW_EL, QNB_F_WW_VDW, QNB_F_LRF,
   QNB_F_RAD, QNB_F_POL,
   QNB_F_Q0 = 9,          /* per state s: qq_el qq_vdw qp_el qp_vdw qw_el qw_vdw at 9 + 6s (8 states) */
   QNB_F_BOND = 57, QNB_F_ANGLE, QNB_F_EKIN, QNB_F_TEMP, QNB_F_TFREE, QNB_F_STEP,
-  QNB_F_OFFD = 63        /* EVB off-diagonal m: Hij at 63 + 2m, r_kl at 64 + 2m (m < QNB_MAX_OFFD) */
+  QNB_F_OFFD = 63,       /* EVB off-diagonal m: Hij at 63 + 2m, r_kl at 64 + 2m (m < QNB_MAX_OFFD) */
+  QNB_F_QB0 = 71         /* per state s: Q bond, angle, torsion, improper at 71 + 4s (8 states) */
 };
 #define QNB_MAX_OFFD 4
 
@@ -335,6 +381,12 @@ int32_t qnb_batch_pair_counts(qnb_batch *b, int32_t k, int64_t *out);
  * n <= QNB_MAX_OFFD; n = 0 removes them. */
 int32_t qnb_batch_set_offdiag(qnb_batch *b, int32_t k, int32_t n, const int32_t *kl, const double *A,
                               const double *mu);
+
+/* Q bonded terms of instance k (qnb_qbonded above; NULL removes them): each
+ * MD step adds their lambda-weighted forces (potene.f90:183-190) and every
+ * sampled frame holds the per-state energies at QNB_F_QB0 + 4s.  Zeroes the
+ * instance's Morse coupling state. */
+int32_t qnb_batch_set_qbonded(qnb_batch *b, int32_t k, const qnb_qbonded *qb);
 
 /* Relaxes the current structures before MD: nsteps steepest-descent moves
  * along the full gradient (nonbonded engine + solute bonds/angles), each atom

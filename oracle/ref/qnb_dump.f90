@@ -18,6 +18,8 @@ program qnb_dump
   use MPIGLOB
   use NONBONDENE
   use POTENE
+  use BONDENE
+  use QATOM
   implicit none
   character(len=512) :: outname
   integer :: u, n, is, i, k, nq, ns
@@ -306,6 +308,12 @@ program qnb_dump
       call wd(u, 'theta', nwat, theta(1:nwat), 1, (/nwat/))
     end if
   end if
+  ! Q-atom bonded terms per state, as pot_energy's loop (src/potene.f90:183-190):
+  ! three passes at the same x, each into a zeroed d.  Passes 1 and 2 run the
+  ! loop without Urey-Bradley (pass 2 starts from the Morse coupling state
+  ! EMorseD/dMorse that pass 1 left); pass 3 adds qurey_bradley after qangle,
+  ! as the loop does for CHARMM force fields.
+  if (nqbond + nqangle + nqtor + nqimp > 0) call dump_qbonded(u)
   ! full potential energy (bonded terms included) for the printed known answers
   call pot_energy(E, EQ, .true.)
   call pack_x(d, natom, dtmp)
@@ -316,6 +324,62 @@ program qnb_dump
   write(*,'(a)') 'qnb_dump: wrote '//trim(outname)
 
 contains
+
+  subroutine dump_qbonded(u)
+    integer, intent(in) :: u
+    integer :: ip, st, j
+    real(kind=prec) :: eb(4, max_states)
+    call wi(u, 'qb_counts', 7, (/nqbond, nqangle, nqtor, nqimp, nang_coupl, ntor_coupl, nimp_coupl/), 1, (/7/))
+    if (nqbond > 0) then
+      call wi(u, 'qbnd', (2+ns)*nqbond, (/(int(qbnd(j)%i), int(qbnd(j)%j), int(qbnd(j)%cod(1:ns)), &
+              j=1,nqbond)/), 2, (/2+ns, nqbond/))
+      n = size(qbondlib)
+      call wd(u, 'qbondlib', 4*n, (/(qbondlib(j)%Dmz, qbondlib(j)%amz, qbondlib(j)%r0, qbondlib(j)%fk, &
+              j=1,n)/), 2, (/4, n/))
+    end if
+    if (nqangle > 0) then
+      call wi(u, 'qang', (3+ns)*nqangle, (/(int(qang(j)%i), int(qang(j)%j), int(qang(j)%k), &
+              int(qang(j)%cod(1:ns)), j=1,nqangle)/), 2, (/3+ns, nqangle/))
+      n = size(qanglib)
+      call wd(u, 'qanglib', 4*n, (/(qanglib(j)%fk, qanglib(j)%ang0, qanglib(j)%ureyfk, qanglib(j)%ureyr0, &
+              j=1,n)/), 2, (/4, n/))
+    end if
+    if (nqtor > 0) then
+      call wi(u, 'qtor', (4+ns)*nqtor, (/(int(qtor(j)%i), int(qtor(j)%j), int(qtor(j)%k), int(qtor(j)%l), &
+              int(qtor(j)%cod(1:ns)), j=1,nqtor)/), 2, (/4+ns, nqtor/))
+      n = size(qtorlib)
+      call wd(u, 'qtorlib', 3*n, (/(qtorlib(j)%fk, qtorlib(j)%rmult, qtorlib(j)%deltor, j=1,n)/), 2, (/3, n/))
+    end if
+    if (nqimp > 0) then
+      call wi(u, 'qimp', (4+ns)*nqimp, (/(int(qimp(j)%i), int(qimp(j)%j), int(qimp(j)%k), int(qimp(j)%l), &
+              int(qimp(j)%cod(1:ns)), j=1,nqimp)/), 2, (/4+ns, nqimp/))
+      n = size(qimplib)
+      call wd(u, 'qimplib', 2*n, (/(qimplib(j)%fk, qimplib(j)%imp0, j=1,n)/), 2, (/2, n/))
+    end if
+    if (nang_coupl > 0) call wi(u, 'iang_coupl', 3*nang_coupl, &
+            reshape(int(iang_coupl(1:3,1:nang_coupl)), (/3*nang_coupl/)), 2, (/3, nang_coupl/))
+    if (ntor_coupl > 0) call wi(u, 'itor_coupl', 3*ntor_coupl, &
+            reshape(int(itor_coupl(1:3,1:ntor_coupl)), (/3*ntor_coupl/)), 2, (/3, ntor_coupl/))
+    if (nimp_coupl > 0) call wi(u, 'iimp_coupl', 3*nimp_coupl, &
+            reshape(int(iimp_coupl(1:3,1:nimp_coupl)), (/3*nimp_coupl/)), 2, (/3, nimp_coupl/))
+    do ip = 1, 3
+      d(:) = zero
+      eb(:, :) = zero
+      do st = 1, ns
+        call qbond(eb(1, st), EQ(st)%lambda, st)
+        call qangle(eb(2, st), EQ(st)%lambda, st)
+        if (ip == 3) call qurey_bradley(eb(2, st), EQ(st)%lambda, st)
+        call qtorsion(eb(3, st), EQ(st)%lambda, st)
+        call qimproper(eb(4, st), EQ(st)%lambda, st)
+      end do
+      call dump_term(u, 'qb'//achar(48 + ip), reshape(eb(:, 1:ns), (/4*ns/)), 4*ns)
+      if (nqbond > 0) then
+        call wd(u, 'EMorseD'//achar(48 + ip), nqbond, EMorseD(1:nqbond), 1, (/nqbond/))
+        call wd(u, 'dMorse_i'//achar(48 + ip), 3*nqbond, (/(dMorse_i(j)%x, dMorse_i(j)%y, dMorse_i(j)%z, &
+                j=1,nqbond)/), 2, (/3, nqbond/))
+      end if
+    end do
+  end subroutine dump_qbonded
 
   subroutine whead(u, name, kind, ndim, dims)
     integer, intent(in) :: u, kind, ndim, dims(:)

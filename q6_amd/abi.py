@@ -100,9 +100,63 @@ class QnbQTangent(C.Structure):
 QNB_ADJ_NV = 7
 ADJ_DQ, ADJ_DA, ADJ_DB, ADJ_DA14, ADJ_DB14, ADJ_EEL, ADJ_EVDW = range(7)
 
-QNB_FRAME_LEN = 72
+QNB_FRAME_LEN = 104
 F_Q0, F_BOND, F_ANGLE, F_EKIN, F_TEMP, F_TFREE, F_STEP, F_OFFD = 9, 57, 58, 59, 60, 61, 62, 63
+F_QB0 = 71  # per state s: Q bond, angle, torsion, improper at 71 + 4s
 QNB_MAX_OFFD = 4
+QNB_QB_MAX_TERMS = 4096
+QNB_QB_MAX_BONDS = 1024
+
+
+class QnbQbonded(C.Structure):
+    _fields_ = [
+        ("nbond", C.c_int32), ("nangle", C.c_int32), ("ntorsion", C.c_int32), ("nimproper", C.c_int32),
+        ("nbond_lib", C.c_int32), ("nangle_lib", C.c_int32), ("ntorsion_lib", C.c_int32),
+        ("nimproper_lib", C.c_int32),
+        ("bond_ij", _pi), ("bond_cod", _pi), ("bond_lib", _pd),
+        ("angle_ijk", _pi), ("angle_cod", _pi), ("angle_lib", _pd),
+        ("torsion_ijkl", _pi), ("torsion_cod", _pi), ("torsion_lib", _pd),
+        ("improper_ijkl", _pi), ("improper_cod", _pi), ("improper_lib", _pd),
+        ("nangle_coupl", C.c_int32), ("ntorsion_coupl", C.c_int32), ("nimproper_coupl", C.c_int32),
+        ("angle_coupl", _pi), ("torsion_coupl", _pi), ("improper_coupl", _pi),
+        ("urey_bradley", C.c_int32),
+    ]
+
+
+# (struct prefix, term-atom count, library width) of the four Q bonded kinds
+QB_KINDS = (("bond", 2, 4), ("angle", 3, 4), ("torsion", 4, 3), ("improper", 4, 2))
+QB_IDX = {"bond": "bond_ij", "angle": "angle_ijk", "torsion": "torsion_ijkl", "improper": "improper_ijkl"}
+
+
+def qbonded_struct(tb, urey=False):
+    """qnb_qbonded from a table dict: <kind>_ij/_ijk/_ijkl (0-based atoms),
+    <kind>_cod (n, nstates) 1-based library rows (0 = absent), <kind>_lib
+    rows, <kind>_coupl (n, 3) term, bond, flag.  Returns (struct, arrays to
+    keep alive while the struct is in use)."""
+    import numpy as np
+    q = QnbQbonded()
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data_as(_pi if dt == np.int32 else _pd)
+
+    for kind, nat, nlib in QB_KINDS:
+        idx = np.asarray(tb.get(QB_IDX[kind], np.zeros((0, nat))), np.int32).reshape(-1, nat)
+        n = len(idx)
+        lib = np.asarray(tb.get(kind + "_lib", np.zeros((0, nlib))), float).reshape(-1, max(nlib, 1))[:, :nlib]
+        setattr(q, "n" + kind, n)
+        setattr(q, "n" + kind + "_lib", len(lib))
+        setattr(q, QB_IDX[kind], arr(idx, np.int32))
+        setattr(q, kind + "_cod", arr(np.asarray(tb.get(kind + "_cod", np.zeros((n, 1))), np.int32), np.int32))
+        setattr(q, kind + "_lib", arr(lib, np.float64))
+        if kind != "bond":
+            c = np.asarray(tb.get(kind + "_coupl", np.zeros((0, 3))), np.int32).reshape(-1, 3)
+            setattr(q, "n" + kind + "_coupl", len(c))
+            setattr(q, kind + "_coupl", arr(c, np.int32))
+    q.urey_bradley = 1 if urey else 0
+    return q, keep
 
 # every symbol include/qnb.h declares (checked by the CPU test suite)
 EXPORTS = [
@@ -112,6 +166,7 @@ EXPORTS = [
     "qnb_batch_create", "qnb_batch_destroy", "qnb_batch_run", "qnb_batch_sync", "qnb_batch_get_state",
     "qnb_batch_energy_history", "qnb_batch_pair_counts", "qnb_batch_time_kernel", "qnb_batch_minimize", "qnb_batch_autotune",
     "qnb_adjoint", "qnb_batch_store_frames", "qnb_batch_frames", "qnb_batch_adjoint", "qnb_batch_set_offdiag",
+    "qnb_set_qbonded", "qnb_eval_qbonded", "qnb_batch_set_qbonded",
 ]
 
 _lib = None
@@ -150,6 +205,9 @@ def _declare(lib):
     lib.qnb_batch_set_offdiag.argtypes = [vp, C.c_int32, C.c_int32, _pi, _pd, _pd]
     lib.qnb_batch_frames.argtypes = [vp, C.c_int32, _pd, C.c_int32, C.POINTER(C.c_int32)]
     lib.qnb_batch_adjoint.argtypes = [vp, pp, _pd, C.c_int64, C.POINTER(C.c_int64)]
+    lib.qnb_set_qbonded.argtypes = [vp, C.POINTER(QnbQbonded)]
+    lib.qnb_eval_qbonded.argtypes = [vp, _pd, _pd, _pd]
+    lib.qnb_batch_set_qbonded.argtypes = [vp, C.c_int32, C.POINTER(QnbQbonded)]
     for n in EXPORTS:
         if n not in ("qnb_abi_version", "qnb_last_error", "qnb_device_count"):
             getattr(lib, n).restype = C.c_int32
@@ -165,7 +223,7 @@ def load(path=None):
     if not os.path.exists(p):
         raise RuntimeError(f"HIP engine library not built: {p} (run __graft_entry__.build())")
     lib = _declare(C.CDLL(p))
-    if lib.qnb_abi_version() != 1:
+    if lib.qnb_abi_version() != 2:
         raise RuntimeError("qnb ABI version mismatch")
     if path is None:
         _lib = lib

@@ -411,6 +411,9 @@ struct qnb_handle {
   int* d_istep = nullptr;
   bool qlists_built = false;
   bool lists_built = false;
+  QbDev qb{};                  // Q bonded terms (host copy of the device record)
+  QbDev* dqb = nullptr;
+  std::vector<void*> qb_owned;
 };
 
 // host twin of the device pp_param filter, for list export
@@ -619,6 +622,44 @@ extern "C" int32_t qnb_eval_terms(qnb_handle* h, uint32_t mask, const double* x,
       if (mask & QNB_T_QW) { EQ[s].qw_el = q[4]; EQ[s].qw_vdw = q[5]; }
     }
   if (!finite) return fail(QNB_ENONFINITE, "non-finite energy");
+  return 0;
+}
+
+extern "C" int32_t qnb_set_qbonded(qnb_handle* h, const qnb_qbonded* qb) {
+  if (!h) return fail(QNB_EINVAL, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  qbonded_free(h->mem, h->qb_owned);
+  int rc = qbonded_build(qb, h->hs.nstates, h->hs.natom, h->hs.lambda, h->mem, h->qb_owned, h->qb);
+  if (rc) {
+    qbonded_free(h->mem, h->qb_owned);
+    h->qb = QbDev{};
+  }
+  if (!h->dqb) {
+    int rc2 = h->mem.alloc(&h->dqb, 1);
+    if (rc2) return rc2;
+  }
+  HIPCHK(hipMemcpy(h->dqb, &h->qb, sizeof(QbDev), hipMemcpyHostToDevice));
+  return rc;
+}
+
+extern "C" int32_t qnb_eval_qbonded(qnb_handle* h, const double* x, double* d, double* eq) {
+  if (!h) return fail(QNB_EINVAL, "null handle");
+  if (!d || !eq) return fail(QNB_EINVAL, "null gradient or energies");
+  const int ns = h->hs.nstates, nat = h->hs.natom;
+  for (int k = 0; k < 4 * ns; k++) eq[k] = 0.0;
+  if (!h->dqb || h->qb.nb + h->qb.na + h->qb.nt + h->qb.ni == 0) return 0;
+  int rc = upload_x(h, x);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(h->hs.d, 0, sizeof(double) * 3 * nat, h->stream));
+  launch_qbonded(h->stream, 1, h->dsys, h->dqb);
+  HIPCHK(hipGetLastError());
+  std::vector<double> dd(3 * (size_t)nat);
+  HIPCHK(hipMemcpyAsync(dd.data(), h->hs.d, sizeof(double) * 3 * nat, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(eq, h->qb.e, sizeof(double) * 4 * ns, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (size_t k = 0; k < dd.size(); k++) d[k] += dd[k];
+  for (int k = 0; k < 4 * ns; k++)
+    if (!std::isfinite(eq[k])) return fail(QNB_ENONFINITE, "non-finite Q bonded energy");
   return 0;
 }
 

@@ -68,6 +68,8 @@ struct MdDev {
   int noffd;
   int offd_k[QNB_MAX_OFFD], offd_l[QNB_MAX_OFFD];  // atom indices
   double offd_A[QNB_MAX_OFFD], offd_mu[QNB_MAX_OFFD];
+  // per-state Q bonded energies of the last step ([nstates][4]), or null
+  const double* qbe;
 };
 
 enum { ERR_NONFINITE = 1, ERR_SHAKE = 2, ERR_OVERFLOW = 4 };
@@ -422,6 +424,8 @@ __global__ void k_record(const DevSys* __restrict__ systems, const MdDev* __rest
       h[QNB_F_OFFD + 2 * m] = M.offd_A[m] * exp(-M.offd_mu[m] * r);
       h[QNB_F_OFFD + 2 * m + 1] = r;
     }
+    if (M.qbe)
+      for (int z = 0; z < 4 * S.nstates; z++) h[QNB_F_QB0 + z] = M.qbe[z];
     M.ctr[0] = f + 1;
   }
 }
@@ -451,6 +455,11 @@ struct qnb_batch {
   int wpb_lists = 4, wpb_forces = 4;  // waves per block, picked by qnb_batch_autotune
   std::vector<int> nbcycle;
   std::vector<int> ene_every;  // per instance energy sampling interval
+  // Q bonded terms per instance
+  std::vector<QbDev> hqb;
+  std::vector<std::vector<void*>> qb_owned;
+  QbDev* dqb = nullptr;
+  bool anyqb = false;
   long long steps_done = 0;
   ~qnb_batch() {
     if (cycle_exec) (void)hipGraphExecDestroy(cycle_exec);
@@ -548,6 +557,7 @@ static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
     default: enqueue_forces<R, 8>(b, energy); break;
   }
   hipLaunchKernelGGL(k_bonded, dim3(b->nbblk, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  if (b->anyqb) launch_qbonded(b->stream, B, b->dsys, b->dqb);
   hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(1024), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
   hipLaunchKernelGGL(k_snap, dim3((3 * b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
@@ -570,6 +580,7 @@ static void enqueue_min_step(qnb_batch* b, bool lists, double k0, double max_dis
     default: enqueue_forces<R, 8>(b); break;
   }
   hipLaunchKernelGGL(k_bonded, dim3(b->nbblk, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  if (b->anyqb) launch_qbonded(b->stream, B, b->dsys, b->dqb);
   hipLaunchKernelGGL(k_sd_move, dim3((b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd, k0, max_disp);
   hipLaunchKernelGGL(k_shake, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
 }
@@ -726,7 +737,15 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     if ((rc = b->mem.alloc(&M.bpart, 2 * (size_t)b->nbblk))) return rc;
     HIPCHK(hipMemset(M.bpart, 0, sizeof(double) * 2 * b->nbblk));
   }
-  if ((rc = b->mem.alloc(&b->dsys, n)) || (rc = b->mem.alloc(&b->dmd, n))) return rc;
+  if ((rc = b->mem.alloc(&b->dsys, n)) || (rc = b->mem.alloc(&b->dmd, n)) || (rc = b->mem.alloc(&b->dqb, n)))
+    return rc;
+  b->hqb.assign(n, QbDev{});
+  b->qb_owned.assign(n, {});
+  for (int k = 0; k < n; k++) {
+    b->hqb[k].ns = b->hs[k].nstates;
+    for (int st = 0; st < QNB_MAXST; st++) b->hqb[k].lambda[st] = b->hs[k].lambda[st];
+  }
+  HIPCHK(hipMemcpy(b->dqb, b->hqb.data(), sizeof(QbDev) * n, hipMemcpyHostToDevice));
   // the MD records change after this point only in qnb_batch_store_frames
   // and qnb_batch_set_offdiag (after a stream sync); every kernel that takes
   // b->dmd runs after this copy
@@ -879,6 +898,35 @@ extern "C" int32_t qnb_batch_set_offdiag(qnb_batch* b, int32_t k, int32_t n, con
   }
   HIPCHK(hipMemcpy(b->dmd, b->hm.data(), sizeof(MdDev) * b->n, hipMemcpyHostToDevice));
   return 0;
+}
+
+extern "C" int32_t qnb_batch_set_qbonded(qnb_batch* b, int32_t k, const qnb_qbonded* qb) {
+  if (!b || k < 0 || k >= b->n) return fail(QNB_EINVAL, "bad instance");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const DevSys& S = b->hs[k];
+  qbonded_free(b->mem, b->qb_owned[k]);
+  int rc = qbonded_build(qb, S.nstates, S.natom, S.lambda, b->mem, b->qb_owned[k], b->hqb[k]);
+  if (rc) {
+    qbonded_free(b->mem, b->qb_owned[k]);
+    b->hqb[k] = QbDev{};
+    b->hqb[k].ns = S.nstates;
+  }
+  const QbDev& Q = b->hqb[k];
+  b->hm[k].qbe = Q.nb + Q.na + Q.nt + Q.ni > 0 ? Q.e : nullptr;
+  b->anyqb = false;
+  for (const QbDev& q : b->hqb) b->anyqb = b->anyqb || q.nb + q.na + q.nt + q.ni > 0;
+  HIPCHK(hipMemcpy(b->dqb, b->hqb.data(), sizeof(QbDev) * b->n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->dmd, b->hm.data(), sizeof(MdDev) * b->n, hipMemcpyHostToDevice));
+  // the step graph holds the launch list: recapture with (or without) k_qbonded
+  if (b->cycle_exec) {
+    (void)hipGraphExecDestroy(b->cycle_exec);
+    b->cycle_exec = nullptr;
+  }
+  if (b->cycle_graph) {
+    (void)hipGraphDestroy(b->cycle_graph);
+    b->cycle_graph = nullptr;
+  }
+  return rc;
 }
 
 extern "C" int32_t qnb_batch_store_frames(qnb_batch* b, int32_t every, int32_t max_frames) {

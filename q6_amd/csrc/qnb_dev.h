@@ -132,6 +132,29 @@ struct DevSys {
   int ene_every;
 };
 
+// Q-atom bonded terms of one instance (qbond, qangle, qurey_bradley,
+// qtorsion, qimproper, src/bondene.f90:387-685), resolved per state on the
+// host.  Terms are numbered bonds, angles, torsions, impropers; each writes
+// its gradient contributions into fixed slots (zeros where absent), and every
+// atom the terms touch sums its slots in a fixed order.
+struct QbDev {
+  int nb, na, nt, ni, ns, urey;
+  int nslot, nat;
+  const int* atoms;     // term atoms: 2 per bond, 3 per angle, 4 per torsion / improper
+  const double* par;    // [nterm][ns][4] the state's library row
+  const uint8_t* pres;  // [nterm][ns] 1 = term present in the state
+  const int* cpl;       // [na + nt + ni][4]: bond, flag, Urey-Bradley bond, flag (bond -1 = uncoupled)
+  const int* slot0;     // [nterm] first slot of the term
+  const int* coff;      // [nat + 1] slots of distinct atom a: cslot[coff[a] .. coff[a+1])
+  const int* cslot;
+  const int* catom;     // [nat] atom number
+  double* morse;        // [nb][4] EMorseD, dMorse_i (x, y, z): kept between calls
+  double* slots;        // [nslot][3]
+  double* acc;          // [nat][3]
+  double* e;            // [ns][4] bond, angle, torsion, improper of the last call
+  double lambda[QNB_MAXST];
+};
+
 __device__ __forceinline__ bool want_energies(const DevSys& S) {
   return S.ene_every <= 0 || (*S.step_ptr % S.ene_every) == 0;
 }

@@ -75,5 +75,11 @@ int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots);
 // adjoint of the Q-atom energies: tangent upload and launch over frames
 int adjoint_upload(DevAlloc& M, const DevSys& S, const qnb_qtangent* t, AdjDev& A);
 void launch_adjoint(hipStream_t st, int prec, int ns_max, long waves, int B, const DevSys* dsys, const AdjDev* dadj);
+// Q-atom bonded terms: validation and device copies (buffers recorded in
+// `owned` so a later set can release them), and the per-instance launch
+int qbonded_build(const qnb_qbonded* q, int nstates, int natom, const double* lambda, DevAlloc& M,
+                  std::vector<void*>& owned, QbDev& out);
+void qbonded_free(DevAlloc& M, std::vector<void*>& owned);
+void launch_qbonded(hipStream_t st, int B, const DevSys* dsys, const QbDev* dqb);
 
 }  // namespace qnb

@@ -51,17 +51,20 @@ def _read_records(buf):
 
 def frame_states(frames, nstates):
     """Per-state energy records of the batch energy history (nframes x
-    QNB_FRAME_LEN): qq, qp, qw (el, vdw) per state; qx = qq + qp + qw;
-    total = qx (the nonbonded engine's workloads carry no Q bonded terms or
-    Q restraints, so those fields are zero, as Qdyn6 writes them then)."""
+    QNB_FRAME_LEN): q (bond, angle, torsion, improper) of the Q bonded terms,
+    qq, qp, qw (el, vdw) per state; qx = qq + qp + qw; total = q + qx
+    (EQ(s)%total, src/potene.f90:229-231; the engine has no Q restraints, so
+    that field is zero, as Qdyn6 writes it then)."""
     frames = np.asarray(frames, np.float64)
     nf = len(frames)
     st = np.zeros((nf, nstates, 14))
     for s in range(nstates):
         b = abi.F_Q0 + 6 * s
         qq, qp, qw = frames[:, b:b + 2], frames[:, b + 2:b + 4], frames[:, b + 4:b + 6]
+        q = frames[:, abi.F_QB0 + 4 * s:abi.F_QB0 + 4 * s + 4]
         qx = qq + qp + qw
-        st[:, s, 0] = qx.sum(1)     # total
+        st[:, s, 0] = q.sum(1) + qx.sum(1)     # total
+        st[:, s, 1:5] = q
         st[:, s, 5:7] = qx
         st[:, s, 7:9] = qq
         st[:, s, 9:11] = qp

@@ -34,6 +34,27 @@ class NonbondedEngine:
     def __del__(self):
         self.close()
 
+    # Q-atom bonded terms (qatom_load_fep's tables, src/qatom.f90:952-1800)
+    def set_qbonded(self, tables, urey=False):
+        """Loads the Q bonded terms (abi.qbonded_struct layout; None removes
+        them) and zeroes their Morse coupling state."""
+        if tables is None:
+            abi.check(self.lib, self.lib.qnb_set_qbonded(self.h, None), "qnb_set_qbonded")
+            return
+        q, keep = abi.qbonded_struct(tables, urey)
+        abi.check(self.lib, self.lib.qnb_set_qbonded(self.h, C.byref(q)), "qnb_set_qbonded")
+
+    # the per-state Q bonded loop of pot_energy (src/potene.f90:183-190)
+    def qbonded(self, x, d=None):
+        """Returns (E, d): E (nstates, 4) = EQ(s)%q%bond, %angle, %torsion,
+        %improper; the lambda-weighted gradient is added into d."""
+        x = np.ascontiguousarray(x, np.float64)
+        if d is None:
+            d = np.zeros_like(x)
+        e = np.zeros(4 * self.nst)
+        abi.check(self.lib, self.lib.qnb_eval_qbonded(self.h, _pd(x), _pd(d), _pd(e)), "qnb_eval_qbonded")
+        return e.reshape(self.nst, 4), d
+
     # make_pair_lists (src/nonbondene.f90:749-837)
     def make_pair_lists(self, x):
         x = np.ascontiguousarray(x, np.float64)

@@ -56,6 +56,20 @@ module qnb_bind
     type(c_ptr) :: qq_dq_i, qq_dA_i, qq_dB_i, qq_dq_j, qq_dA_j, qq_dB_j, qq_is14
   end type qnb_qtangent
 
+  ! Q bonded terms (c_loc of the arrays qatom_load_fep fills, converted to
+  ! 0-based atom/term numbers and per-term code rows)
+  type, bind(c) :: qnb_qbonded
+    integer(c_int32_t) :: nbond, nangle, ntorsion, nimproper
+    integer(c_int32_t) :: nbond_lib, nangle_lib, ntorsion_lib, nimproper_lib
+    type(c_ptr) :: bond_ij, bond_cod, bond_lib
+    type(c_ptr) :: angle_ijk, angle_cod, angle_lib
+    type(c_ptr) :: torsion_ijkl, torsion_cod, torsion_lib
+    type(c_ptr) :: improper_ijkl, improper_cod, improper_lib
+    integer(c_int32_t) :: nangle_coupl, ntorsion_coupl, nimproper_coupl
+    type(c_ptr) :: angle_coupl, torsion_coupl, improper_coupl
+    integer(c_int32_t) :: urey_bradley
+  end type qnb_qbonded
+
   interface
     integer(c_int32_t) function qnb_abi_version() bind(c, name='qnb_abi_version')
       import :: c_int32_t
@@ -139,6 +153,19 @@ module qnb_bind
       integer(c_int32_t), value :: nframes
       real(c_double), intent(in) :: frames(*)
       real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int32_t) function qnb_set_qbonded(h, qb) bind(c, name='qnb_set_qbonded')
+      import :: c_int32_t, c_ptr, qnb_qbonded
+      type(c_ptr), value :: h
+      type(qnb_qbonded), intent(in) :: qb
+    end function
+    ! pot_energy's Q bonded loop: adds into d(3, natom); eq(4, nstates)
+    integer(c_int32_t) function qnb_eval_qbonded(h, x, d, eq) bind(c, name='qnb_eval_qbonded')
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(inout) :: d(*)
+      real(c_double), intent(out) :: eq(*)
     end function
   end interface
 

@@ -150,6 +150,16 @@ class BatchMD:
         abi.check(self.lib, self.lib.qnb_batch_set_offdiag(self.h, k, len(A), _pi(kl), _pd(A), _pd(mu)),
                   "qnb_batch_set_offdiag")
 
+    def set_qbonded(self, k, tables, urey=False):
+        """Q-atom bonded terms of instance k (abi.qbonded_struct layout; None
+        removes them): their lambda-weighted forces act in every step and the
+        per-state energies are sampled at abi.F_QB0 + 4 s."""
+        if tables is None:
+            abi.check(self.lib, self.lib.qnb_batch_set_qbonded(self.h, k, None), "qnb_batch_set_qbonded")
+            return
+        q, keep = abi.qbonded_struct(tables, urey)
+        abi.check(self.lib, self.lib.qnb_batch_set_qbonded(self.h, k, C.byref(q)), "qnb_batch_set_qbonded")
+
     def store_frames(self, every, max_frames):
         """Keep a coordinate snapshot every `every` steps (max_frames per
         instance) on the device, for the adjoint."""
@@ -192,12 +202,13 @@ class BatchMD:
 
 
 def q_state_energies(frames, nstates):
-    """Per-state Q-atom nonbonded energies EQ(s)%qx = qq + qp + qw (el + vdw),
-    the state energies of the FEP/EVB analysis (src/potene.f90:226-229)."""
+    """Per-state energies EQ(s)%total of the FEP/EVB analysis
+    (src/potene.f90:226-231): the Q bonded terms plus EQ(s)%qx = qq + qp + qw
+    (el + vdw); the engine has no Q restraints."""
     out = np.zeros((len(frames), nstates))
     for s in range(nstates):
         b = abi.F_Q0 + 6 * s
-        out[:, s] = frames[:, b:b + 6].sum(1)
+        out[:, s] = frames[:, b:b + 6].sum(1) + frames[:, abi.F_QB0 + 4 * s:abi.F_QB0 + 4 * s + 4].sum(1)
     return out
 
 

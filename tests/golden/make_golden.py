@@ -27,8 +27,14 @@ Cases:
            q_atom 12 A (so the Q lists are radius-limited) -> exercises pp, pw,
            qp, qqp, qq, qw, ww, LRF and SCAAS together.
   lig_fep2_nolrf  as lig_fep2 with LRF off (cut-off-only list builders).
+  lig_qb2  lig_fep2 plus changing bonds (Morse, harmonic, one forming bond),
+           angles (one with Urey-Bradley terms), torsions and impropers with
+           Morse couplings (gamma and 1 - gamma, two couplings on one term);
+           the fixture keeps x, lambda, the Q bonded tables as Qdyn6 holds them
+           after get_fep, and three evaluations of the per-state loop
+           (src/potene.f90:183-190) with the Morse coupling state after each.
 
-Usage: python tests/golden/make_golden.py  (writes tests/golden/*.npz)
+Usage: python tests/golden/make_golden.py [case ...]  (writes tests/golden/<case>.npz)
 """
 import os
 import shutil
@@ -117,6 +123,75 @@ def fep2_text(top_charges):
     return "\n".join(out)
 
 
+# Q bonded sections of lig_qb2 (topology atom numbers; src/qatom.f90:952-1800
+# The one bond that changes presence (19-20) is not a topology bond: for a
+# topology bond that breaks, qatom_load_fep collects the topology torsions
+# through it into break_torsions(nqtor+1) without a bound check
+# (src/qatom.f90:1362-1385), which overruns the heap on this ligand.
+# formats: Morse types "Dmz alpha r0", harmonic "fk r0", angle "fk theta0
+# [urey_fk urey_r0]", torsion "fk multiplicity delta", improper "fk imp0").
+QB_TEXT = """
+[bond_types]
+1 95.0 2.1 1.05
+2 650.0 1.42
+3 80.0 1.8 1.95
+4 110.0 2.3 1.60
+
+[change_bonds]
+1 2 1 3
+1 25 2 2
+2 3 3 1
+19 20 0 4
+
+[angle_types]
+1 70.0 109.5
+2 60.0 120.0 25.0 2.3
+3 45.0 100.0
+
+[change_angles]
+2 1 22 1 2
+2 1 25 2 0
+1 2 3 0 3
+3 2 24 1 2
+
+[torsion_types]
+1 1.5 3.0 0.0
+2 0.8 2.0 180.0
+3 2.0 1.0 35.0
+
+[change_torsions]
+22 1 2 3 1 2
+25 1 2 24 3 0
+2 1 22 4 0 1
+
+[improper_types]
+1 10.0 0.0
+2 5.0 180.0
+
+[change_impropers]
+25 1 2 22 1 2
+1 2 3 24 2 0
+
+[angle_couplings]
+1 1
+3 1 1
+4 3
+1 4
+
+[torsion_couplings]
+1 1
+2 3 1
+2 4
+
+[improper_couplings]
+1 1
+2 4 1
+"""
+
+QB_KEYS = ("x", "lambda", "nstates", "natom", "qb_counts", "qbnd", "qbondlib", "qang", "qanglib", "qtor", "qtorlib",
+           "qimp", "qimplib", "iang_coupl", "itor_coupl", "iimp_coupl")
+
+
 def top_charges():
     lines = open(os.path.join(REF, "prep_SPH", "lig_w.top")).read().split("\n")
     for k, ln in enumerate(lines):
@@ -201,8 +276,19 @@ def main():
                                           extra_cut="solute_solute 16\nsolvent_solvent 9\nsolute_solvent 16\n",
                                           fep="fep2.fep", lambdas="[lambdas]\n0.7 0.3\n"), fep2_text(q)),
     }
+    cases["lig_qb2"] = (cases["lig_fep2"][0], fep2_text(q) + QB_TEXT)
+    only = sys.argv[1:]
     for name, (inp, fep) in cases.items():
+        if only and name not in only:
+            continue
         d, log = run_case(name, inp, fep)
+        if name == "lig_qb2":
+            keep = {k: np.asarray(v) for k, v in d.items()
+                    if k in QB_KEYS or k[:4] in ("E_qb", "d_qb") or k.startswith(("EMorseD", "dMorse_i"))}
+            path = os.path.join(HERE, name + ".npz")
+            np.savez_compressed(path, **keep)
+            print(name, "->", path, os.path.getsize(path) // 1024, "KiB;", "counts", d["qb_counts"])
+            continue
         if name == "lig_eq1":
             # the reference's own known answer (tests/basic_tests/eval_test.sh:97-101)
             el, vdw = d["E_qw"]

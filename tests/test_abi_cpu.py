@@ -25,7 +25,7 @@ def test_library_exports_every_symbol():
 
 def test_abi_version_and_device_count_without_gpu():
     lib = abi.load()
-    assert lib.qnb_abi_version() == 1
+    assert lib.qnb_abi_version() == 2
     assert lib.qnb_device_count() >= 0
 
 
@@ -34,8 +34,8 @@ def test_struct_sizes_match_header():
     src = r'''
 #include <stdio.h>
 #include "qnb.h"
-int main(){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(qnb_system), sizeof(qnb_params), sizeof(qnb_energies),
-              sizeof(qnb_qenergies), sizeof(qnb_md_params), sizeof(qnb_qtangent));return 0;}
+int main(){printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(qnb_system), sizeof(qnb_params), sizeof(qnb_energies),
+              sizeof(qnb_qenergies), sizeof(qnb_md_params), sizeof(qnb_qtangent), sizeof(qnb_qbonded));return 0;}
 '''
     import subprocess
     import tempfile
@@ -45,7 +45,7 @@ int main(){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(qnb_system), sizeof(qnb_pa
     out = subprocess.check_output([os.path.join(d, "t")]).decode().split()
     assert [int(v) for v in out] == [C.sizeof(abi.QnbSystem), C.sizeof(abi.QnbParams), C.sizeof(abi.QnbEnergies),
                                      C.sizeof(abi.QnbQEnergies), C.sizeof(abi.QnbMdParams),
-                                     C.sizeof(abi.QnbQTangent)]
+                                     C.sizeof(abi.QnbQTangent), C.sizeof(abi.QnbQbonded)]
 
 
 def test_create_fails_loudly_without_gpu():

@@ -21,8 +21,9 @@ program t
   type(qnb_energies) :: e
   type(qnb_qenergies) :: q
   type(qnb_qtangent) :: tg
-  print '(i0,1x,i0,1x,i0,1x,i0,1x,i0,1x,i0,1x,i0)', qnb_abi_version(), c_sizeof(s), c_sizeof(p), c_sizeof(e), &
-        c_sizeof(q), c_sizeof(tg), merge(1, 0, qnb_device_count() >= 0)
+  type(qnb_qbonded) :: qb
+  print '(i0,1x,i0,1x,i0,1x,i0,1x,i0,1x,i0,1x,i0,1x,i0)', qnb_abi_version(), c_sizeof(s), c_sizeof(p), &
+        c_sizeof(e), c_sizeof(q), c_sizeof(tg), c_sizeof(qb), merge(1, 0, qnb_device_count() >= 0)
 end program
 '''
 
@@ -37,5 +38,6 @@ def test_fortran_binding_links_and_matches_c_layout():
     subprocess.check_call(["amdflang", "-I", d, os.path.join(d, "t.f90"), os.path.join(d, "qnb_bind.o"),
                            "-L" + lib, "-lq6nb", "-Wl,-rpath," + lib, "-o", os.path.join(d, "t")])
     out = subprocess.check_output([os.path.join(d, "t")]).decode().split()
-    assert [int(v) for v in out] == [1, C.sizeof(abi.QnbSystem), C.sizeof(abi.QnbParams), C.sizeof(abi.QnbEnergies),
-                                     C.sizeof(abi.QnbQEnergies), C.sizeof(abi.QnbQTangent), 1]
+    assert [int(v) for v in out] == [2, C.sizeof(abi.QnbSystem), C.sizeof(abi.QnbParams), C.sizeof(abi.QnbEnergies),
+                                     C.sizeof(abi.QnbQEnergies), C.sizeof(abi.QnbQTangent),
+                                     C.sizeof(abi.QnbQbonded), 1]
