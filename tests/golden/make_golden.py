@@ -27,6 +27,13 @@ Cases:
            q_atom 12 A (so the Q lists are radius-limited) -> exercises pp, pw,
            qp, qqp, qq, qw, ww, LRF and SCAAS together.
   lig_fep2_nolrf  as lig_fep2 with LRF off (cut-off-only list builders).
+  lig_fep2_arith  lig_fep2 under the arithmetic combination rule: the
+           topology's rule flag set to 2 and every type's (sqrt A, sqrt B) pair
+           rewritten as the (R*/2, epsilon) pair with the same A_ii, B_ii
+           (topology and FEP types alike), so the geometric and arithmetic
+           systems agree on like-type pairs and differ on mixed ones.  The
+           rule also selects the general solvent loops (nonbond_ww/qw instead
+           of the _spc forms, src/potene.f90:347-356).
   lig_qb2  lig_fep2 plus changing bonds (Morse, harmonic, one forming bond),
            angles (one with Urey-Bradley terms), torsions and impropers with
            Morse couplings (gamma and 1 - gamma, two couplings on one term);
@@ -192,6 +199,86 @@ QB_KEYS = ("x", "lambda", "nstates", "natom", "qb_counts", "qbnd", "qbondlib", "
            "qimp", "qimplib", "iang_coupl", "itor_coupl", "iimp_coupl")
 
 
+def _to_arith(sa, sb):
+    """(sqrt A, sqrt B) of one type -> (R*/2, epsilon): Qdyn6 takes the square
+    root of epsilon on loading (FEP types: src/simprep.f90:1042-1047), so that
+    (a_i + a_j)^12 b_i b_j = A and 2 (a_i + a_j)^6 b_i b_j = B for i = j
+    (src/simprep.f90:3333-3340)."""
+    A, B = sa * sa, sb * sb
+    if A <= 0.0 or B <= 0.0:
+        return 0.0, 0.0
+    rstar = (2.0 * A / B) ** (1.0 / 6.0)
+    return 0.5 * rstar, B * B / (4.0 * A)
+
+
+def arith_top(text):
+    """The topology with the arithmetic rule and converted LJ parameters
+    (topology section 11, read by src/topo.f90:846-876: natyps, rule, 1-4
+    line, 'Masses:' line, masses, then for each LJ code a header and
+    natyps avdw values, a header and natyps bvdw values)."""
+    lines = text.split("\n")
+    k = next(i for i, ln in enumerate(lines) if "vdW combination rule" in ln)
+    ntyp = int(lines[k - 1].split()[0])
+    assert lines[k].split()[0] == "1"
+    lines[k] = lines[k].replace("1 =", "2 =", 1)
+    j = k + 3  # after the 1-4 line and the 'Masses:' line
+
+    def take(j):
+        vals, start = [], j
+        while len(vals) < ntyp:
+            vals += [float(t) for t in lines[j].split()]
+            j += 1
+        assert len(vals) == ntyp
+        return vals, start, j
+
+    _, _, j = take(j)
+    blocks = []
+    for _ in range(3):
+        a, a0, j = take(j + 1)
+        b, b0, j = take(j + 1)
+        blocks.append((a, a0, b, b0))
+    out = list(lines)
+    for a, a0, b, b0 in reversed(blocks):
+        conv = [_to_arith(x, y) for x, y in zip(a, b)]
+        # replace the value lines of each array (later blocks first, so the
+        # line numbers of the earlier ones stay valid)
+        nb_lines = _count_lines(lines, b0, ntyp)
+        out[b0:b0 + nb_lines] = fmt([c[1] for c in conv])
+        na_lines = _count_lines(lines, a0, ntyp)
+        out[a0:a0 + na_lines] = fmt([c[0] for c in conv])
+    return "\n".join(out)
+
+
+def fmt(vs):
+    return [" ".join("%.10f" % v for v in vs[i:i + 8]) for i in range(0, len(vs), 8)]
+
+
+def _count_lines(lines, j0, n):
+    got, j = 0, j0
+    while got < n:
+        got += len(lines[j].split())
+        j += 1
+    return j - j0
+
+
+def arith_fep(text):
+    """FEP [atom_types] rows name Ri Ei Ci ai Ri14 Ei14 mass: the (Ri, Ei)
+    pairs converted as the topology's; Ci, ai (exponential soft pairs) kept."""
+    out, inside = [], False
+    for ln in text.split("\n"):
+        if ln.startswith("["):
+            inside = ln.strip() == "[atom_types]"
+            out.append(ln)
+            continue
+        t = ln.split()
+        if inside and len(t) == 8:
+            r, e = _to_arith(float(t[1]), float(t[2]))
+            r14, e14 = _to_arith(float(t[5]), float(t[6]))
+            ln = " ".join([t[0], "%.10f" % r, "%.10f" % e, t[3], t[4], "%.10f" % r14, "%.10f" % e14, t[7]])
+        out.append(ln)
+    return "\n".join(out)
+
+
 def top_charges():
     lines = open(os.path.join(REF, "prep_SPH", "lig_w.top")).read().split("\n")
     for k, ln in enumerate(lines):
@@ -206,10 +293,14 @@ def top_charges():
     raise RuntimeError("charges not found")
 
 
-def run_case(name, inp_text, fep_text=None):
+def run_case(name, inp_text, fep_text=None, top_text=None):
     tmp = tempfile.mkdtemp(prefix="qnbgold_")
     try:
-        shutil.copy(os.path.join(REF, "prep_SPH", "lig_w.top"), tmp)
+        if top_text is None:
+            shutil.copy(os.path.join(REF, "prep_SPH", "lig_w.top"), tmp)
+        else:
+            with open(os.path.join(tmp, "lig_w.top"), "w") as f:
+                f.write(top_text)
         shutil.copy(os.path.join(REF, "prep_SPH", "lig_w.fep"), tmp)
         if fep_text is not None:
             with open(os.path.join(tmp, "fep2.fep"), "w") as f:
@@ -277,11 +368,13 @@ def main():
                                           fep="fep2.fep", lambdas="[lambdas]\n0.7 0.3\n"), fep2_text(q)),
     }
     cases["lig_qb2"] = (cases["lig_fep2"][0], fep2_text(q) + QB_TEXT)
+    cases["lig_fep2_arith"] = (cases["lig_fep2"][0], arith_fep(fep2_text(q)))
+    top_arith = arith_top(open(os.path.join(REF, "prep_SPH", "lig_w.top")).read())
     only = sys.argv[1:]
     for name, (inp, fep) in cases.items():
         if only and name not in only:
             continue
-        d, log = run_case(name, inp, fep)
+        d, log = run_case(name, inp, fep, top_arith if name.endswith("_arith") else None)
         if name == "lig_qb2":
             keep = {k: np.asarray(v) for k, v in d.items()
                     if k in QB_KEYS or k[:4] in ("E_qb", "d_qb") or k.startswith(("EMorseD", "dMorse_i"))}

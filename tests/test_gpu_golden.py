@@ -37,7 +37,7 @@ def term_energy(t, E, EQ):
     return [E[t]]
 
 
-@pytest.fixture(scope="module", params=["lig_eq1", "lig_fep2"])
+@pytest.fixture(scope="module", params=["lig_eq1", "lig_fep2", "lig_fep2_arith"])
 def gcase(request):
     print(f"\n[gpu] golden case {request.param}", flush=True)
     s = from_golden(golden_path(request.param))
@@ -139,7 +139,8 @@ def pair_energy_scales(s):
     return out
 
 
-def test_mixed_precision_error_budget():
+@pytest.mark.parametrize("case", ["lig_fep2", "lig_fep2_arith"])
+def test_mixed_precision_error_budget(case):
     """FP32 pair arithmetic with FP64 accumulation: the error against the FP64
     reference is reported and bounded.  Forces: 1e-4 of the largest gradient
     component.  Energy of a class: each pair energy carries an FP32 relative
@@ -147,7 +148,8 @@ def test_mixed_precision_error_budget():
     errs by a few ulp times S = sqrt(sum of squared pair energies); bound 16
     ulp x S.  (A relative bound on the sum itself is meaningless for classes
     like water-water Coulomb: -10.6 kcal/mol from 3.4e5 terms of about +-50.)"""
-    s = from_golden(golden_path("lig_fep2"))
+    print(f"\n[gpu] mixed precision {case}", flush=True)
+    s = from_golden(golden_path(case))
     eng = engine_for(s, precision=abi.QNB_PREC_MIXED)
     eng.make_pair_lists(s.x)
     d, E, EQ = eng.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree)
