@@ -75,6 +75,11 @@ typedef struct qnb_system {
                           src/potene.f90:347-356); 0: general loops     */
   int32_t nexlong;     /* long (> QNB_MAX_NBR_RANGE) exclusions          */
   int32_t n14long;     /* long 1-4 pairs                                 */
+  int32_t any_atom_cutoff; /* 1: the topology's iuse_switch_atom = 0: a charge-
+                          group pair is inside a cut-off when any atom pair
+                          is (nbpplis2, nbpwlis2, nbqplis2 and their _lrf
+                          forms, src/nonbondene.f90:796-825); 0 (default):
+                          switch atoms (iuse_switch_atom = 1)            */
 
   /* per atom [natom] */
   const double *crg;     /* scaled charge                                  */

@@ -326,8 +326,39 @@ static inline int parity_skip(int ig, int jg) {
   return (ig > jg && ((ig + jg) % 2 == 0)) || (ig < jg && ((ig + jg) % 2 == 1));
 }
 
+/* Group test of the iuse_switch_atom = 0 builders: the atom pairs of groups
+ * ig and jg are scanned ig-major until one lies within rc2 (nbpplis2
+ * nonbondene.f90:1031-1047, nbpplis2_lrf :1451-1468).  Returns 1 when one
+ * does; otherwise 0 with *r2last the squared distance of the last pair
+ * scanned (the two groups' last atoms), which the _lrf form compares with
+ * RcLRF^2 (:1499). */
+static int any_pair_inside(const qnb_system *s, const double *x, int ig, int jg, double rc2, double *r2last) {
+  double r2 = 1.0;
+  for (int ia = s->cgp_first[ig]; ia <= s->cgp_last[ig]; ia++)
+    for (int ja = s->cgp_first[jg]; ja <= s->cgp_last[jg]; ja++) {
+      r2 = dist4(v3(x, s->cgpatom[ia]), v3(x, s->cgpatom[ja]));
+      if (r2 <= rc2) return 1;
+    }
+  *r2last = r2;
+  return 0;
+}
+
+/* the same with a single point on the other side: the water's first atom
+ * (nbpwlis2 :2197-2208, nbpwlis2_lrf :2566-2577; LRF test :2601) or xpcent
+ * (nbqplis2 :3477-3488) */
+static int any_atom_inside(const qnb_system *s, const double *x, int ig, vec3 p, double rc2, double *r2last) {
+  double r2 = 1.0;
+  for (int ia = s->cgp_first[ig]; ia <= s->cgp_last[ig]; ia++) {
+    r2 = dist4(v3(x, s->cgpatom[ia]), p);
+    if (r2 <= rc2) return 1;
+  }
+  *r2last = r2;
+  return 0;
+}
+
 /* nbpplist_lrf (nonbondene.f90:1777-1908) and, for use_LRF = 0, nbpplist
- * (nonbondene.f90:1520-1639). */
+ * (nonbondene.f90:1520-1639); with any_atom_cutoff, nbpplis2_lrf (:1369-1488)
+ * and nbpplis2 (:950-1089). */
 static void nbpplist(orc_handle *h, const double *x) {
   const qnb_system *s = &h->s;
   const double rc2 = h->p.Rcpp * h->p.Rcpp, rl2 = h->p.RcLRF * h->p.RcLRF;
@@ -339,8 +370,14 @@ static void nbpplist(orc_handle *h, const double *x) {
       if (parity_skip(ig, jg)) continue;
       int js = s->cgp_iswitch[jg];
       if (s->excl[js]) continue;
-      double r2 = dist4(v3(x, is), v3(x, js));
-      if (r2 <= rc2) {
+      double r2 = 0.0;
+      int inside;
+      if (s->any_atom_cutoff) inside = any_pair_inside(s, x, ig, jg, rc2, &r2);
+      else {
+        r2 = dist4(v3(x, is), v3(x, js));
+        inside = r2 <= rc2;
+      }
+      if (inside) {
         for (int ia = s->cgp_first[ig]; ia <= s->cgp_last[ig]; ia++) {
           int i = s->cgpatom[ia];
           if (s->iqatom[i] != 0) continue;
@@ -364,12 +401,13 @@ static void nbpplist(orc_handle *h, const double *x) {
   }
 }
 
-/* nbpwlist_lrf (nonbondene.f90:2842-2955) and nbpwlist (2622-2725). */
+/* nbpwlist_lrf (nonbondene.f90:2842-2955) and nbpwlist (2622-2725); with
+ * any_atom_cutoff, nbpwlis2_lrf (:2493-2620) and nbpwlis2 (:2137-2230). */
 static void nbpwlist(orc_handle *h, const double *x) {
   const qnb_system *s = &h->s;
   const double rc2 = h->p.Rcpw * h->p.Rcpw, rl2 = h->p.RcLRF * h->p.RcLRF;
   const int sa = s->solv_atom, ns = s->nat_solute;
-  const int quirk = !h->p.use_LRF && (h->p.ref_quirks & 1);
+  const int quirk = !h->p.use_LRF && (h->p.ref_quirks & 1) && !s->any_atom_cutoff;
   h->pw.n = 0;
   for (int ig = 0; ig < s->ncgp_solute; ig++) {
     int is = s->cgp_iswitch[ig];
@@ -379,8 +417,14 @@ static void nbpwlist(orc_handle *h, const double *x) {
       int ja = ns + sa * jw;
       int jg = h->iwhich_cgp[ja];
       if (s->excl[ja]) continue;
-      double r2 = dist4(v3(x, ref), v3(x, ja));
-      if (r2 <= rc2) {
+      double r2 = 0.0;
+      int inside;
+      if (s->any_atom_cutoff) inside = any_atom_inside(s, x, ig, v3(x, ja), rc2, &r2);
+      else {
+        r2 = dist4(v3(x, ref), v3(x, ja));
+        inside = r2 <= rc2;
+      }
+      if (inside) {
         for (int ia = s->cgp_first[ig]; ia <= s->cgp_last[ig]; ia++) {
           int i = s->cgpatom[ia];
           if (s->iqatom[i] != 0) continue;
@@ -432,7 +476,8 @@ static void nbwwlist(orc_handle *h, const double *x) {
 
 /* nbqplist (nonbondene.f90:3647-3742): per-state parameters of every
  * (Q-atom, solute atom) pair whose group's switch atom is within Rcq of
- * xpcent; built once when Rcq > rexcl_o. */
+ * xpcent (with any_atom_cutoff, any atom of the group: nbqplis2
+ * :3408-3470); built once when Rcq > rexcl_o. */
 static void nbqplist(orc_handle *h, const double *x) {
   const qnb_system *s = &h->s;
   const double rq2 = h->p.Rcq * h->p.Rcq;
@@ -444,8 +489,11 @@ static void nbqplist(orc_handle *h, const double *x) {
   for (int ig = 0; ig < s->ncgp_solute; ig++) {
     int ia = s->cgp_iswitch[ig];
     if (s->excl[ia]) continue;
-    double r2 = dist4(v3(x, ia), xp);
-    if (r2 > rq2) continue;
+    if (s->any_atom_cutoff) {
+      double r2l;
+      if (!any_atom_inside(s, x, ig, xp, rq2, &r2l)) continue;
+    } else if (dist4(v3(x, ia), xp) > rq2)
+      continue;
     for (int k = s->cgp_first[ig]; k <= s->cgp_last[ig]; k++) {
       int i = s->cgpatom[k];
       if (i < nsol && s->qconn3[i]) continue;

@@ -33,7 +33,7 @@ class QnbSystem(C.Structure):
         ("natom", C.c_int32), ("nat_solute", C.c_int32), ("nwat", C.c_int32), ("solv_atom", C.c_int32),
         ("ncgp", C.c_int32), ("ncgp_solute", C.c_int32), ("nqat", C.c_int32), ("nstates", C.c_int32),
         ("ntypes", C.c_int32), ("ivdw_rule", C.c_int32), ("solvent_spc", C.c_int32),
-        ("nexlong", C.c_int32), ("n14long", C.c_int32),
+        ("nexlong", C.c_int32), ("n14long", C.c_int32), ("any_atom_cutoff", C.c_int32),
         ("crg", _pd), ("iac", _pi), ("iqatom", _pi), ("excl", _pu8), ("heavy", _pu8),
         ("cgp_first", _pi), ("cgp_last", _pi), ("cgp_iswitch", _pi), ("cgpatom", _pi),
         ("avdw", _pd), ("bvdw", _pd), ("ljcod", _pi), ("el14_scale", C.c_double),

@@ -85,12 +85,11 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevSys* __restrict__ syst
       }
     }
   }
-  // Q-solute: atoms of solute groups whose switch atom lies within Rcq,
-  // not 1-2/1-3 bonded to any Q-atom
+  // Q-solute: atoms of solute groups inside Rcq (switch atom, or any atom
+  // with iuse_switch_atom = 0), not 1-2/1-3 bonded to any Q-atom
   for (int j = lane; j < nsol; j += QNB_WAVE) {
     if (S.qconn3[j]) continue;
-    const int sw = S.gswitch[S.atom_group[j]];
-    if (S.excl[sw] || dist4p(x, sw, S.xp) > S.rcq2) continue;
+    if (!qp_group_in(S, x, S.atom_group[j])) continue;
     const R ux = (R)(x[3 * j] - xi0), uy = (R)(x[3 * j + 1] - xi1), uz = (R)(x[3 * j + 2] - xi2);
     R r2, r;
     rinv(ux * ux + uy * uy + uz * uz, r2, r);

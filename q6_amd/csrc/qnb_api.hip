@@ -13,7 +13,7 @@
 #include "qnb_host.h"
 
 namespace qnb {
-template <typename R>
+template <typename R, bool ANY>
 __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
 template <typename R, int NS, bool WAT, bool E>
@@ -103,6 +103,7 @@ void fill_params(DevSys& S, const qnb_system* s, const qnb_params* p) {
   S.nstates = ns;
   S.ntypes = s->ntypes;
   S.ivdw_rule = s->ivdw_rule;
+  S.any_atom = s->any_atom_cutoff ? 1 : 0;
   S.spc = s->solvent_spc;
   S.use_lrf = p->use_LRF;
   S.prec = p->precision;
@@ -503,7 +504,8 @@ static void launch_lists(qnb_handle* h) {
   dim3 grid((h->hs.ncgp + wpb - 1) / wpb, 1);
   if (sizeof(R) == 4)
     hipLaunchKernelGGL(k_pack_xq, dim3((h->hs.natom + 255) / 256, 1), dim3(256), 0, h->stream, h->dsys);
-  hipLaunchKernelGGL(k_lists_lrf<R>, grid, dim3(64 * wpb), 0, h->stream, h->dsys);
+  if (h->hs.any_atom) hipLaunchKernelGGL((k_lists_lrf<R, true>), grid, dim3(64 * wpb), 0, h->stream, h->dsys);
+  else hipLaunchKernelGGL((k_lists_lrf<R, false>), grid, dim3(64 * wpb), 0, h->stream, h->dsys);
 }
 
 extern "C" int32_t qnb_make_pair_lists(qnb_handle* h, const double* x) {

@@ -15,7 +15,7 @@
 #include "qnb_host.h"
 
 namespace qnb {
-template <typename R>
+template <typename R, bool ANY>
 __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
 __global__ void k_pack_xq(const DevSys*);
@@ -471,11 +471,15 @@ struct qnb_batch {
 // launchers shared by the step, the minimiser, the timer and the autotuner
 static void launch_lists(qnb_batch* b, int wpb) {
   dim3 g((b->maxg + wpb - 1) / wpb, b->n), t(64 * wpb);
+  bool any = false;
+  for (const DevSys& S : b->hs) any = any || S.any_atom;
   if (b->prec == QNB_PREC_MIXED) {
     hipLaunchKernelGGL(k_pack_xq, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys);
-    hipLaunchKernelGGL(k_lists_lrf<float>, g, t, 0, b->stream, b->dsys);
+    if (any) hipLaunchKernelGGL((k_lists_lrf<float, true>), g, t, 0, b->stream, b->dsys);
+    else hipLaunchKernelGGL((k_lists_lrf<float, false>), g, t, 0, b->stream, b->dsys);
   } else {
-    hipLaunchKernelGGL(k_lists_lrf<double>, g, t, 0, b->stream, b->dsys);
+    if (any) hipLaunchKernelGGL((k_lists_lrf<double, true>), g, t, 0, b->stream, b->dsys);
+    else hipLaunchKernelGGL((k_lists_lrf<double, false>), g, t, 0, b->stream, b->dsys);
   }
 }
 

@@ -29,6 +29,7 @@ enum {
 struct DevSys {
   // sizes
   int natom, nat_solute, nwat, sa, ncgp, ncgp_solute, nqat, nstates, ntypes, ivdw_rule, spc;
+  int any_atom;  // iuse_switch_atom = 0: group cut-off tests take any atom pair
   int use_lrf, prec, quirk;
   double rcpp2, rcpw2, rcww2, rcq2, rclrf2, rexcl2;
   double lambda[QNB_MAXST];
@@ -220,6 +221,18 @@ __device__ __forceinline__ double dist4p(const double* __restrict__ x, int a, co
   r = r + dy * dy;
   r = r + dz * dz;
   return r;
+}
+
+// Q-solute group test (nbqplist, nonbondene.f90:3647-3742): the group's
+// switch atom within Rcq of xpcent; with any_atom (nbqplis2 :3408-3500) any
+// atom of the group, scanned in group order
+__device__ __forceinline__ bool qp_group_in(const DevSys& S, const double* __restrict__ x, int g) {
+  const int sw = S.gswitch[g];
+  if (S.excl[sw]) return false;
+  if (!S.any_atom) return dist4p(x, sw, S.xp) <= S.rcq2;
+  for (int k = S.gfirst[g]; k <= S.glast[g]; k++)
+    if (dist4p(x, S.cgpatom[k], S.xp) <= S.rcq2) return true;
+  return false;
 }
 
 // Solute-solute pair parameters computed from atom data exactly as

@@ -17,6 +17,36 @@ __device__ __forceinline__ int group_ref_atom(const DevSys& S, int g) {
   return g < S.ncgp_solute ? ldg(S.gswitch + g) : S.nat_solute + S.sa * (g - S.ncgp_solute);
 }
 
+// Group test of the iuse_switch_atom = 0 builders for a pair with a solute
+// group: true when any atom pair (solute-solute, nbpplis2(_lrf)
+// nonbondene.f90:950-1089, 1369-1488) or any solute atom and the water's
+// first atom (nbpwlis2(_lrf) :2137-2230, 2493-2620) lies within cut2;
+// otherwise r2 is the squared distance of the last pair the reference scans
+// (the groups' last atoms; symmetric, so either group may be the receiver),
+// which its _lrf forms compare with RcLRF^2
+__device__ __forceinline__ bool any_atom_pair(const DevSys& S, const double* __restrict__ x, int a, int b, bool a_sol,
+                                            bool b_sol, int ra, int rb, double cut2, double& r2) {
+  r2 = 1.0;
+  if (a_sol && b_sol) {
+    const int a1 = ldg(S.glast + a), b0 = ldg(S.gfirst + b), b1 = ldg(S.glast + b);
+    for (int ia = ldg(S.gfirst + a); ia <= a1; ia++) {
+      const int i = ldg(S.cgpatom + ia);
+      for (int jb = b0; jb <= b1; jb++) {
+        r2 = dist4(x, i, ldg(S.cgpatom + jb));
+        if (r2 <= cut2) return true;
+      }
+    }
+    return false;
+  }
+  const int g = a_sol ? a : b, w = a_sol ? rb : ra;
+  const int g1 = ldg(S.glast + g);
+  for (int k = ldg(S.gfirst + g); k <= g1; k++) {
+    r2 = dist4(x, ldg(S.cgpatom + k), w);
+    if (r2 <= cut2) return true;
+  }
+  return false;
+}
+
 // the owner side of a charge-group pair (a != b) under the parity rule
 __device__ __forceinline__ bool owns(int a, int b) {
   return (a < b && ((a + b) & 1) == 0) || (a > b && ((a + b) & 1) == 1);
@@ -178,7 +208,9 @@ __device__ __forceinline__ int t3index(int a, int b, int c) {
 // (LDS-free: group reference atoms are L2-resident); a pair inside its class
 // cut-off goes to a's neighbour row through a ballot/prefix compaction, a
 // pair beyond it (and inside RcLRF) adds b's non-Q charges to a's expansion.
-template <typename R>
+// ANY: some system of the launch has iuse_switch_atom = 0 (any_atom), whose
+// group tests scan atom pairs; the switch-atom form is compiled without it.
+template <typename R, bool ANY>
 __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ systems) {
   const DevSys& S = systems[blockIdx.y];
   const int a = blockIdx.x * (blockDim.x / QNB_WAVE) + threadIdx.x / QNB_WAVE;
@@ -215,8 +247,15 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
         if (!ldg(S.excl + rb)) {
           const bool b_sol = b < S.ncgp_solute;
           const double cut2 = (a_sol && b_sol) ? S.rcpp2 : ((!a_sol && !b_sol) ? S.rcww2 : S.rcpw2);
-          const double r2 = dist4(x, ra, rb);
-          if (r2 <= cut2) {
+          double r2;
+          bool inside;
+          if (ANY && S.any_atom && (a_sol || b_sol))
+            inside = any_atom_pair(S, x, a, b, a_sol, b_sol, ra, rb, cut2, r2);
+          else {
+            r2 = dist4(x, ra, rb);
+            inside = r2 <= cut2;
+          }
+          if (inside) {
             in = true;
           } else if (S.use_lrf && r2 <= S.rclrf2) {
             R part[20];
@@ -322,8 +361,7 @@ __global__ __launch_bounds__(1024) void k_qlists(const DevSys* __restrict__ syst
     const int per = (S.ncgp_solute + nt - 1) / nt, g0 = t * per, g1 = min(S.ncgp_solute, g0 + per);
     int c = 0;
     for (int g = g0; g < g1; g++) {
-      int sw = S.gswitch[g];
-      bool in = !S.excl[sw] && dist4p(S.x, sw, S.xp) <= S.rcq2;
+      const bool in = qp_group_in(S, S.x, g);
       S.qp_in[g] = in;
       if (in)
         for (int k = S.gfirst[g]; k <= S.glast[g]; k++) c += !S.qconn3[S.cgpatom[k]];
@@ -1516,8 +1554,10 @@ __global__ __launch_bounds__(64 * QNB_RED_CHUNKS) void k_reduce_energies(const D
 }
 
 // explicit instantiations used by the host code
-template __global__ void k_lists_lrf<double>(const DevSys*);
-template __global__ void k_lists_lrf<float>(const DevSys*);
+template __global__ void k_lists_lrf<double, false>(const DevSys*);
+template __global__ void k_lists_lrf<float, false>(const DevSys*);
+template __global__ void k_lists_lrf<double, true>(const DevSys*);
+template __global__ void k_lists_lrf<float, true>(const DevSys*);
 #define QNB_INST(R, NS)                                                            \
   template __global__ void k_forces<R, NS, true, true>(const DevSys*, uint32_t, const double*); \
   template __global__ void k_forces<R, NS, false, true>(const DevSys*, uint32_t, const double*); \

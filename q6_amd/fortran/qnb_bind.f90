@@ -16,7 +16,7 @@ module qnb_bind
 
   type, bind(c) :: qnb_system
     integer(c_int32_t) :: natom, nat_solute, nwat, solv_atom, ncgp, ncgp_solute, nqat, nstates
-    integer(c_int32_t) :: ntypes, ivdw_rule, solvent_spc, nexlong, n14long
+    integer(c_int32_t) :: ntypes, ivdw_rule, solvent_spc, nexlong, n14long, any_atom_cutoff
     type(c_ptr) :: crg, iac, iqatom, excl, heavy
     type(c_ptr) :: cgp_first, cgp_last, cgp_iswitch, cgpatom
     type(c_ptr) :: avdw, bvdw, ljcod

@@ -67,6 +67,7 @@ class QSystem:
             setattr(s, k, int(self.n[k]))
         s.nexlong = int(self.n.get("nexlong", 0))
         s.n14long = int(self.n.get("n14long", 0))
+        s.any_atom_cutoff = int(self.n.get("any_atom_cutoff", 0))
         s.el14_scale = float(self.el14_scale)
         for k, dt in self.ARRAYS.items():
             ct = {np.float64: C.c_double, np.int32: C.c_int32, np.uint8: C.c_uint8}[np.dtype(dt).type]
@@ -120,6 +121,7 @@ def from_golden(path):
         s.n[k] = int(sc(k))
     s.n["ntypes"] = int(sc("natyps"))
     s.n["solvent_spc"] = 1 if (int(sc("solvent_type")) == 0 and s.n["ivdw_rule"] == 1) else 0
+    s.n["any_atom_cutoff"] = 1 if int(sc("iuse_switch_atom")) == 0 else 0
     s.el14_scale = float(sc("el14_scale"))
     nat, nsol = s.n["natom"], s.n["nat_solute"]
     a = s.a

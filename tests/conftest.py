@@ -8,7 +8,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-CASES = ["lig_eq1", "lig_fep2", "lig_fep2_nolrf", "lig_fep2_arith"]
+CASES = ["lig_eq1", "lig_fep2", "lig_fep2_nolrf", "lig_fep2_arith", "lig_fep2_anyatom", "lig_fep2_anyatom_nolrf"]
 
 
 def pytest_configure(config):

@@ -34,6 +34,11 @@ Cases:
            systems agree on like-type pairs and differ on mixed ones.  The
            rule also selects the general solvent loops (nonbond_ww/qw instead
            of the _spc forms, src/potene.f90:347-356).
+  lig_fep2_anyatom, lig_fep2_anyatom_nolrf  lig_fep2 (and its LRF-off form)
+           with the topology's switch-atom flag 0 (iuse_switch_atom = 0,
+           src/topo.f90:815): the nbpplis2/nbpwlis2/nbqplis2 builders and
+           their _lrf forms, where a charge-group pair is inside a cut-off
+           when any atom pair is.
   lig_qb2  lig_fep2 plus changing bonds (Morse, harmonic, one forming bond),
            angles (one with Urey-Bradley terms), torsions and impropers with
            Morse couplings (gamma and 1 - gamma, two couplings on one term);
@@ -261,6 +266,17 @@ def _count_lines(lines, j0, n):
     return j - j0
 
 
+def anyatom_top(text):
+    """The topology with the charge groups' switch-atom flag set to 0 (the
+    third number of the 'No. of charge groups' line, src/topo.f90:815)."""
+    lines = text.split("\n")
+    k = next(i for i, ln in enumerate(lines) if "No. of charge groups" in ln)
+    t = lines[k].split()
+    assert t[2] == "1", lines[k]
+    lines[k] = "%9s%8s%8s = " % (t[0], t[1], "0") + lines[k].split("= ", 1)[1]
+    return "\n".join(lines)
+
+
 def arith_fep(text):
     """FEP [atom_types] rows name Ri Ei Ci ai Ri14 Ei14 mass: the (Ri, Ei)
     pairs converted as the topology's; Ci, ai (exponential soft pairs) kept."""
@@ -369,12 +385,17 @@ def main():
     }
     cases["lig_qb2"] = (cases["lig_fep2"][0], fep2_text(q) + QB_TEXT)
     cases["lig_fep2_arith"] = (cases["lig_fep2"][0], arith_fep(fep2_text(q)))
-    top_arith = arith_top(open(os.path.join(REF, "prep_SPH", "lig_w.top")).read())
+    top_ref = open(os.path.join(REF, "prep_SPH", "lig_w.top")).read()
+    top_arith = arith_top(top_ref)
+    top_any = anyatom_top(top_ref)
+    cases["lig_fep2_anyatom"] = cases["lig_fep2"]
+    cases["lig_fep2_anyatom_nolrf"] = cases["lig_fep2_nolrf"]
+    tops = {"lig_fep2_arith": top_arith, "lig_fep2_anyatom": top_any, "lig_fep2_anyatom_nolrf": top_any}
     only = sys.argv[1:]
     for name, (inp, fep) in cases.items():
         if only and name not in only:
             continue
-        d, log = run_case(name, inp, fep, top_arith if name.endswith("_arith") else None)
+        d, log = run_case(name, inp, fep, tops.get(name))
         if name == "lig_qb2":
             keep = {k: np.asarray(v) for k, v in d.items()
                     if k in QB_KEYS or k[:4] in ("E_qb", "d_qb") or k.startswith(("EMorseD", "dMorse_i"))}

@@ -37,7 +37,7 @@ def term_energy(t, E, EQ):
     return [E[t]]
 
 
-@pytest.fixture(scope="module", params=["lig_eq1", "lig_fep2", "lig_fep2_arith"])
+@pytest.fixture(scope="module", params=["lig_eq1", "lig_fep2", "lig_fep2_arith", "lig_fep2_anyatom"])
 def gcase(request):
     print(f"\n[gpu] golden case {request.param}", flush=True)
     s = from_golden(golden_path(request.param))
@@ -116,6 +116,29 @@ def test_nolrf_lists_against_oracle():
     d, E, EQ = eng.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree)
     do, Eo, EQo = o.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree)
     assert np.max(np.abs(d - do)) <= REL * np.max(np.abs(do))
+    eng.close()
+
+
+def test_anyatom_nolrf_matches_reference():
+    """iuse_switch_atom = 0 with LRF off (nbpplis2, nbpwlis2, nbqplis2): these
+    builders have no switch-atom quirk, so every list equals the reference's,
+    in order, and every term is within 1e-10."""
+    print("\n[gpu] lig_fep2_anyatom_nolrf", flush=True)
+    s = from_golden(golden_path("lig_fep2_anyatom_nolrf"))
+    assert s.n["any_atom_cutoff"] == 1
+    eng = engine_for(s)
+    eng.make_pair_lists(s.x)
+    for which, key in ((abi.LIST_PP, "nbpp_ij"), (abi.LIST_PW, "nbpw_ij"), (abi.LIST_QP, "nbqp_ij")):
+        assert np.array_equal(eng.pairs(which), s.ref[key].T - 1), key
+    assert np.array_equal(eng.pairs(abi.LIST_WW), s.ref["ww_mol_pairs"].T)
+    for term, mask in TERMS:
+        if term == "lrf":
+            continue
+        d, E, EQ = eng.eval_terms(mask, s.x, Tfree=s.Tfree, istep=0)
+        dref, eref = s.ref["d_" + term].T, np.atleast_1d(s.ref["E_" + term])
+        e = np.array(term_energy(term, E, EQ))
+        assert np.max(np.abs(e - eref)) <= REL * max(np.max(np.abs(eref)), 1e-12), (term, e, eref)
+        assert np.max(np.abs(d - dref)) <= REL * max(np.max(np.abs(dref)), 1e-12), term
     eng.close()
 
 
