@@ -150,8 +150,7 @@ struct QbDev {
   const int* cslot;
   const int* catom;     // [nat] atom number
   double* morse;        // [nb][4] EMorseD, dMorse_i (x, y, z): kept between calls
-  double* slots;        // [nslot][3]
-  double* acc;          // [nat][3]
+  double* slots;        // [ns][nslot][3] gradient shares of each state's terms
   double* e;            // [ns][4] bond, angle, torsion, improper of the last call
   double lambda[QNB_MAXST];
 };

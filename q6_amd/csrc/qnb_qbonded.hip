@@ -119,8 +119,8 @@ __device__ __forceinline__ void coupling_slots(const QbDev& Q, int bond, double 
   st3(slot + 3, -dm * e);
 }
 
-__device__ void qb_bond(const QbDev& Q, const double* x, int b, int s, double lam, double& eb) {
-  double* slot = Q.slots + 3 * Q.slot0[b];
+__device__ void qb_bond(const QbDev& Q, double* sl, const double* x, int b, int s, double lam, double& eb) {
+  double* slot = sl + 3 * Q.slot0[b];
   if (!Q.pres[b * Q.ns + s]) {
     st3(slot, {0.0, 0.0, 0.0});
     st3(slot + 3, {0.0, 0.0, 0.0});
@@ -151,9 +151,9 @@ __device__ void qb_bond(const QbDev& Q, const double* x, int b, int s, double la
 }
 
 // angle term a (+ its Urey-Bradley term): energies into ea
-__device__ void qb_angle(const QbDev& Q, const double* x, int a, int s, double lam, double& ea) {
+__device__ void qb_angle(const QbDev& Q, double* sl, const double* x, int a, int s, double lam, double& ea) {
   const int t = Q.nb + a;
-  double* slot = Q.slots + 3 * Q.slot0[t];
+  double* slot = sl + 3 * Q.slot0[t];
   const int* c = Q.cpl + 4 * a;
   const int i = Q.atoms[2 * Q.nb + 3 * a], j = Q.atoms[2 * Q.nb + 3 * a + 1], k = Q.atoms[2 * Q.nb + 3 * a + 2];
   const bool on = Q.pres[t * Q.ns + s];
@@ -213,9 +213,10 @@ __device__ void qb_angle(const QbDev& Q, const double* x, int a, int s, double l
 }
 
 // torsion (improper = false) or improper term u of its kind
-__device__ void qb_dihedral(const QbDev& Q, const double* x, int u, bool improper, int s, double lam, double& e) {
+__device__ void qb_dihedral(const QbDev& Q, double* sl, const double* x, int u, bool improper, int s, double lam,
+                            double& e) {
   const int t = Q.nb + Q.na + (improper ? Q.nt : 0) + u;
-  double* slot = Q.slots + 3 * Q.slot0[t];
+  double* slot = sl + 3 * Q.slot0[t];
   const int* c = Q.cpl + 4 * (t - Q.nb);
   const int* at = Q.atoms + 2 * Q.nb + 3 * Q.na + 4 * ((improper ? Q.nt : 0) + u);
   const bool on = Q.pres[t * Q.ns + s];
@@ -254,21 +255,15 @@ __device__ void qb_dihedral(const QbDev& Q, const double* x, int u, bool imprope
   }
 }
 
-__device__ double qb_block_sum(double v, double* red) {
-  v = wave_sum(v);
-  const int w = threadIdx.x / QNB_WAVE, l = threadIdx.x % QNB_WAVE;
-  __syncthreads();
-  if (l == 0) red[w] = v;
-  __syncthreads();
-  double t = 0.0;
-  for (int k = 0; k < (int)(blockDim.x / QNB_WAVE); k++) t += red[k];
-  return t;
-}
-
 }  // namespace
 
-// blockIdx.x = instance.  The Morse state, slots and per-atom sums are read
-// back after __syncthreads by other threads of the same block.
+// blockIdx.x = instance.  Each state's terms write the state's own slots
+// ([state][slot][3]); the Morse state and the slots are read back after
+// __syncthreads by other threads of the same block.  Two barriers per state:
+// the terms read the Morse state the bonds just wrote, and the next state's
+// bonds may rewrite it only after every term of this state has read it.  The
+// per-wave energy sums of every state wait in LDS for one final reduction,
+// and every touched atom sums all its slots of all states at the end.
 __global__ __launch_bounds__(QB_THREADS) void k_qbonded(const DevSys* __restrict__ systems,
                                                        const QbDev* __restrict__ qbs) {
   const QbDev& Q = qbs[blockIdx.x];
@@ -276,33 +271,40 @@ __global__ __launch_bounds__(QB_THREADS) void k_qbonded(const DevSys* __restrict
   if (nterm == 0) return;
   const DevSys& S = systems[blockIdx.x];
   const double* x = S.x;
-  __shared__ double red[QB_THREADS / QNB_WAVE];
-  for (int a = threadIdx.x; a < Q.nat; a += blockDim.x) st3(Q.acc + 3 * a, {0.0, 0.0, 0.0});
+  constexpr int NW = QB_THREADS / QNB_WAVE;
+  __shared__ double ered[QNB_MAXST * 4][NW];
+  const int wave = threadIdx.x / QNB_WAVE, lane = threadIdx.x % QNB_WAVE;
   for (int s = 0; s < Q.ns; s++) {
     const double lam = Q.lambda[s];
+    double* sl = Q.slots + 3 * (size_t)Q.nslot * s;
     double e[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < Q.nb; b += blockDim.x) qb_bond(Q, x, b, s, lam, e[0]);
+    for (int b = threadIdx.x; b < Q.nb; b += blockDim.x) qb_bond(Q, sl, x, b, s, lam, e[0]);
     __syncthreads();
     for (int t = threadIdx.x; t < Q.na + Q.nt + Q.ni; t += blockDim.x) {
-      if (t < Q.na) qb_angle(Q, x, t, s, lam, e[1]);
-      else if (t < Q.na + Q.nt) qb_dihedral(Q, x, t - Q.na, false, s, lam, e[2]);
-      else qb_dihedral(Q, x, t - Q.na - Q.nt, true, s, lam, e[3]);
+      if (t < Q.na) qb_angle(Q, sl, x, t, s, lam, e[1]);
+      else if (t < Q.na + Q.nt) qb_dihedral(Q, sl, x, t - Q.na, false, s, lam, e[2]);
+      else qb_dihedral(Q, sl, x, t - Q.na - Q.nt, true, s, lam, e[3]);
     }
-    __syncthreads();
-    for (int a = threadIdx.x; a < Q.nat; a += blockDim.x) {
-      V3 sum = {0.0, 0.0, 0.0};
-      for (int m = Q.coff[a]; m < Q.coff[a + 1]; m++) sum = sum + ld3(Q.slots, Q.cslot[m]);
-      st3(Q.acc + 3 * a, ld3(Q.acc, a) + sum);
-    }
+#pragma unroll
     for (int k = 0; k < 4; k++) {
-      const double v = qb_block_sum(e[k], red);
-      if (threadIdx.x == 0) Q.e[4 * s + k] = v;
+      const double v = wave_sum(e[k]);
+      if (lane == 0) ered[4 * s + k][wave] = v;
     }
     __syncthreads();
   }
+  if ((int)threadIdx.x < 4 * Q.ns) {
+    double v = 0.0;
+    for (int w = 0; w < NW; w++) v += ered[threadIdx.x][w];
+    Q.e[threadIdx.x] = v;
+  }
   for (int a = threadIdx.x; a < Q.nat; a += blockDim.x) {
+    V3 sum = {0.0, 0.0, 0.0};
+    for (int s = 0; s < Q.ns; s++) {
+      const double* sl = Q.slots + 3 * (size_t)Q.nslot * s;
+      for (int m = Q.coff[a]; m < Q.coff[a + 1]; m++) sum = sum + ld3(sl, Q.cslot[m]);
+    }
     const int i = Q.catom[a];
-    st3(S.d + 3 * i, ld3(S.d, i) + ld3(Q.acc, a));
+    st3(S.d + 3 * i, ld3(S.d, i) + sum);
   }
 }
 
@@ -477,8 +479,7 @@ int qbonded_build(const qnb_qbonded* q, int nstates, int natom, const double* la
     }
   };
   scratch(&out.morse, 4 * (size_t)nb);
-  scratch(&out.slots, 3 * (size_t)nslot);
-  scratch(&out.acc, 3 * (size_t)nat);
+  scratch(&out.slots, 3 * (size_t)nslot * nstates);
   scratch(&out.e, 4 * (size_t)nstates);
   return rc;
 }
