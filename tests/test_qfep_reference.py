@@ -1,7 +1,8 @@
 """q6_amd.enefile and q6_amd.fep against the reference's own programs.
 
-  * The energy file stock Qdyn6 wrote for the 2-state lig_fep2 case
-    (tests/golden/lig_fep2_qdyn.en, made by tests/golden/make_enefile.py)
+  * The energy files stock Qdyn6 wrote for the 2-state lig_fep2 case, with
+    and without Q bonded terms (tests/golden/lig_fep2_qdyn.en,
+    lig_qb2_qdyn.en, made by tests/golden/make_enefile.py)
     reads back with consistent sums (EQ%total = Q bonded + qx + restraint,
     qx = qq + qp + qw, src/potene.f90:224-232), the run's lambdas and the
     off-diagonal H12 = 80 it was given, and the writer reproduces its
@@ -23,10 +24,13 @@ from conftest import ROOT
 from q6_amd import enefile, fep
 
 GOLD = os.path.join(ROOT, "tests", "golden", "lig_fep2_qdyn.en")
+# the same run with Q bonded terms in the FEP file: nonzero EQ%q fields
+GOLD_QB = os.path.join(ROOT, "tests", "golden", "lig_qb2_qdyn.en")
 
 
-def test_reader_on_qdyn6_energy_file():
-    d = enefile.read_ene(GOLD, 2)
+@pytest.mark.parametrize("path", [GOLD, GOLD_QB])
+def test_reader_on_qdyn6_energy_file(path):
+    d = enefile.read_ene(path, 2)
     assert d["canary"] == 1337
     assert np.array_equal(d["lambdas"], [0.7, 0.3])
     st = d["states"]
@@ -37,12 +41,33 @@ def test_reader_on_qdyn6_energy_file():
     assert all(len(o) == 1 and o[0][:3] == (1, 2, 80.0) and o[0][3] > 0 for o in d["offd"])
 
 
-def test_writer_reproduces_qdyn6_records():
-    d = enefile.read_ene(GOLD, 2)
+@pytest.mark.parametrize("path", [GOLD, GOLD_QB])
+def test_writer_reproduces_qdyn6_records(path):
+    d = enefile.read_ene(path, 2)
+    if path == GOLD_QB:
+        assert np.all(d["states"][:, :, 1:5] != 0.0)  # Q bond, angle, torsion, improper in both states
     with tempfile.TemporaryDirectory() as tmp:
         p = os.path.join(tmp, "rt.en")
         enefile.write_ene(p, d["lambdas"], d["states"], offd_frames=d["offd"], version=d["version"])
-        assert open(p, "rb").read() == open(GOLD, "rb").read()
+        assert open(p, "rb").read() == open(path, "rb").read()
+
+
+def test_frame_states_places_q_bonded_fields():
+    """Batch frames -> energy-file records: the four Q bonded energies of state
+    s (QNB_F_QB0 + 4s) go to q(1:4) and into EQ(s)%total with qx
+    (src/potene.f90:224-231); md.q_state_energies gives the same totals."""
+    from q6_amd import abi
+    from q6_amd.md import q_state_energies
+    rng = np.random.default_rng(3)
+    fr = rng.normal(0.0, 10.0, (5, abi.QNB_FRAME_LEN))
+    st = enefile.frame_states(fr, 2)
+    for s in range(2):
+        q = fr[:, abi.F_QB0 + 4 * s:abi.F_QB0 + 4 * s + 4]
+        qx = fr[:, abi.F_Q0 + 6 * s:abi.F_Q0 + 6 * s + 6].reshape(5, 3, 2).sum(1)
+        assert np.array_equal(st[:, s, 1:5], q)
+        assert np.allclose(st[:, s, 5:7], qx, rtol=0, atol=1e-12)
+        assert np.allclose(st[:, s, 0], q.sum(1) + qx.sum(1), rtol=0, atol=1e-12)
+        assert np.allclose(q_state_energies(fr, 2)[:, s], st[:, s, 0], rtol=0, atol=1e-12)
 
 
 @pytest.mark.skipif(not qfep_ref.available(), reason="reference Qfep6 not built (make -C oracle/ref)")
