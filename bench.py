@@ -199,8 +199,8 @@ def main():
                          "reference for scaling hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: lambda windows shard across GPUs with no per-step exchange "
-              "(see DESIGN.md); only --gpus 1 is measured.", file=sys.stderr)
+        print("bench.py runs on one GPU: lambda windows would shard across GPUs (whole windows per rank, "
+              "no per-step exchange), but only --gpus 1 is built and measured.", file=sys.stderr)
         sys.exit(2)
     if args.steps % 10 or args.warmup % 10:
         sys.exit("--steps and --warmup must be multiples of the 10-step list cycle")
