@@ -290,7 +290,7 @@ typedef struct qnb_qbonded {
   const double *improper_lib;          /* [][2] fk, imp0 (qimplib)                    */
   /* Morse couplings [n][3]: term, bond, flag (iang_coupl / itor_coupl /
    * iimp_coupl(1:3, :), src/qatom.f90:1688-1800); flag 1 scales the term by
-   * 1 - EMorseD(bond) instead of EMorseD(bond) */
+   * 1 - EMorseD(bond) instead of EMorseD(bond), any other value as 0 */
   int32_t nangle_coupl, ntorsion_coupl, nimproper_coupl;
   const int32_t *angle_coupl, *torsion_coupl, *improper_coupl;
   int32_t urey_bradley; /* 1: qurey_bradley after qangle (CHARMM, src/potene.f90:187) */

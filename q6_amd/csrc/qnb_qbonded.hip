@@ -325,7 +325,7 @@ void find_coupling(const int32_t* tab, int n, int term, bool first, int& bond, i
   for (int m = 0; m < n; m++)
     if (tab[3 * m] == term) {
       bond = tab[3 * m + 1];
-      flag = tab[3 * m + 2];
+      flag = tab[3 * m + 2] == 1 ? 1 : 0;  // only 1 selects 1 - EMorseD (src/bondene.f90:415)
       if (first) return;
     }
 }
@@ -349,10 +349,9 @@ int qbonded_build(const qnb_qbonded* q, int nstates, int natom, const double* la
   for (int k = 0; k < 3; k++) {
     if (ncp[k] < 0 || (ncp[k] > 0 && !cpt[k])) return fail(QNB_EINVAL, "Q bonded coupling table missing");
     for (int m = 0; m < ncp[k]; m++) {
-      const int t = cpt[k][3 * m], b = cpt[k][3 * m + 1], f = cpt[k][3 * m + 2];
-      if (t < 0 || t >= nterm_k[k] || b < 0 || b >= nb || (f != 0 && f != 1))
-        return fail(QNB_EINVAL, "Q bonded coupling %d of kind %d out of range (term %d, bond %d, flag %d)", m, k, t,
-                    b, f);
+      const int t = cpt[k][3 * m], b = cpt[k][3 * m + 1];
+      if (t < 0 || t >= nterm_k[k] || b < 0 || b >= nb)
+        return fail(QNB_EINVAL, "Q bonded coupling %d of kind %d out of range (term %d, bond %d)", m, k, t, b);
     }
   }
   // per-kind tables
