@@ -150,6 +150,8 @@ struct QbDev {
   const int* cslot;
   const int* catom;     // [nat] atom number
   double* morse;        // [nb][4] EMorseD, dMorse_i (x, y, z): kept between calls
+  double* snap;         // [nb][ns][4] the same as evaluated in each state of this call
+  double* eterm;        // [nterm][ns] energy of each term in each state
   double* slots;        // [ns][nslot][3] gradient shares of each state's terms
   double* e;            // [ns][4] bond, angle, torsion, improper of the last call
   double lambda[QNB_MAXST];

@@ -103,17 +103,25 @@ __device__ Dih dihedral_geo(V3 pa, V3 pb, V3 pc, V3 pd) {
   return r;
 }
 
+// The Morse state (EMorseD, dMorse_i) a term of state s reads for bond b: in
+// the reference's sequential loop, the value qbond left at the bond's last
+// evaluation, i.e. in the latest state s' <= s of this call where the bond is
+// present, else the value kept from the previous call
+__device__ __forceinline__ const double* morse_for(const QbDev& Q, int bond, int s) {
+  for (int s2 = s; s2 >= 0; s2--)
+    if (Q.pres[bond * Q.ns + s2]) return Q.snap + 4 * (bond * Q.ns + s2);
+  return Q.morse + 4 * bond;
+}
+
 // gamma of a coupled term: EMorseD(bond), or 1 - EMorseD with flag 1
-__device__ __forceinline__ double coupling_gamma(const QbDev& Q, int bond, int flag) {
-  if (bond < 0) return 1.0;
-  const double g = Q.morse[4 * bond];
+__device__ __forceinline__ double coupling_gamma(const double* m, int flag) {
+  const double g = m[0];
   return flag == 1 ? 1.0 - g : g;
 }
 
 // the coupled bond's share: d(i) += dMorse_i * e, d(j) += dMorse_j * e with
 // dMorse_j = -dMorse_i (src/bondene.f90:537-541)
-__device__ __forceinline__ void coupling_slots(const QbDev& Q, int bond, double e, double* slot) {
-  const double* m = Q.morse + 4 * bond;
+__device__ __forceinline__ void coupling_slots(const double* m, double e, double* slot) {
   const V3 dm = {m[1], m[2], m[3]};
   st3(slot, dm * e);
   st3(slot + 3, -dm * e);
@@ -137,7 +145,7 @@ __device__ void qb_bond(const QbDev& Q, double* sl, const double* x, int b, int 
   const double dv = (2.0 * Dmz * amz * (fexp - fexp * fexp) + fk * db) * lam / dist;
   st3(slot, -av * dv);
   st3(slot + 3, av * dv);
-  double* m = Q.morse + 4 * b;
+  double* m = Q.snap + 4 * (b * Q.ns + s);
   if (db > 0.0) {
     m[0] = -(fexp * fexp - 2.0 * fexp);
     const double c = 2.0 * amz * (fexp - fexp * fexp) * lam / dist;
@@ -161,8 +169,9 @@ __device__ void qb_angle(const QbDev& Q, double* sl, const double* x, int a, int
   const V3 xi = ld3(x, i), xk = ld3(x, k);
   V3 fa = {0.0, 0.0, 0.0}, fb = fa, fc = fa;
   double ae = 0.0;
+  const double* m0 = c[0] >= 0 ? morse_for(Q, c[0], s) : nullptr;
   if (on) {
-    const double g = coupling_gamma(Q, c[0], c[1]);
+    const double g = m0 ? coupling_gamma(m0, c[1]) : 1.0;
     const Ang A = angle_geo(xi, ld3(x, j), xk);
     const double da = A.t - p[1];
     ae = 0.5 * p[0] * da * da;
@@ -177,7 +186,7 @@ __device__ void qb_angle(const QbDev& Q, double* sl, const double* x, int a, int
   st3(slot + 6, fc);
   int q = 3;
   if (c[0] >= 0) {
-    if (on) coupling_slots(Q, c[0], ae, slot + 3 * q);
+    if (on) coupling_slots(m0, ae, slot + 3 * q);
     else {
       st3(slot + 3 * q, {0.0, 0.0, 0.0});
       st3(slot + 3 * q + 3, {0.0, 0.0, 0.0});
@@ -189,8 +198,9 @@ __device__ void qb_angle(const QbDev& Q, double* sl, const double* x, int a, int
   const bool uon = on && p[2] != 0.0;
   V3 ui = {0.0, 0.0, 0.0}, uk = ui;
   double eu = 0.0;
+  const double* m2 = c[2] >= 0 ? morse_for(Q, c[2], s) : nullptr;
   if (uon) {
-    const double g = coupling_gamma(Q, c[2], c[3]);
+    const double g = m2 ? coupling_gamma(m2, c[3]) : 1.0;
     const V3 av = xk - xi;
     const double dist = sqrt(dot(av, av));
     const double ru = dist - p[3];
@@ -204,7 +214,7 @@ __device__ void qb_angle(const QbDev& Q, double* sl, const double* x, int a, int
   st3(slot + 3 * q + 3, uk);
   q += 2;
   if (c[2] >= 0) {
-    if (uon) coupling_slots(Q, c[2], eu, slot + 3 * q);
+    if (uon) coupling_slots(m2, eu, slot + 3 * q);
     else {
       st3(slot + 3 * q, {0.0, 0.0, 0.0});
       st3(slot + 3 * q + 3, {0.0, 0.0, 0.0});
@@ -222,9 +232,10 @@ __device__ void qb_dihedral(const QbDev& Q, double* sl, const double* x, int u, 
   const bool on = Q.pres[t * Q.ns + s];
   V3 f[4] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
   double pe = 0.0;
+  const double* m0 = c[0] >= 0 ? morse_for(Q, c[0], s) : nullptr;
   if (on) {
     const double* p = Q.par + 4 * (t * Q.ns + s);
-    const double g = coupling_gamma(Q, c[0], c[1]);
+    const double g = m0 ? coupling_gamma(m0, c[1]) : 1.0;
     const Dih D = dihedral_geo(ld3(x, at[0]), ld3(x, at[1]), ld3(x, at[2]), ld3(x, at[3]));
     double dv;
     if (!improper) {
@@ -247,7 +258,7 @@ __device__ void qb_dihedral(const QbDev& Q, double* sl, const double* x, int u, 
   }
   for (int r = 0; r < 4; r++) st3(slot + 3 * r, f[r]);
   if (c[0] >= 0) {
-    if (on) coupling_slots(Q, c[0], pe, slot + 12);
+    if (on) coupling_slots(m0, pe, slot + 12);
     else {
       st3(slot + 12, {0.0, 0.0, 0.0});
       st3(slot + 15, {0.0, 0.0, 0.0});
@@ -257,13 +268,14 @@ __device__ void qb_dihedral(const QbDev& Q, double* sl, const double* x, int u, 
 
 }  // namespace
 
-// blockIdx.x = instance.  Each state's terms write the state's own slots
-// ([state][slot][3]); the Morse state and the slots are read back after
-// __syncthreads by other threads of the same block.  Two barriers per state:
-// the terms read the Morse state the bonds just wrote, and the next state's
-// bonds may rewrite it only after every term of this state has read it.  The
-// per-wave energy sums of every state wait in LDS for one final reduction,
-// and every touched atom sums all its slots of all states at the end.
+// blockIdx.x = instance.  Every state at once: first the Q bonds of all
+// states, each present (bond, state) writing its Morse snapshot; after a
+// barrier every (term, state), each coupled term reading the snapshot
+// morse_for() picks, which is what the sequential per-state loop gives it;
+// after a second barrier the kept Morse state takes each bond's last present
+// state, the per-state energies are summed in term order and every touched
+// atom adds its slots of all states to d.  Each (term, state) writes its own
+// energy and slots, so the result does not depend on timing.
 __global__ __launch_bounds__(QB_THREADS) void k_qbonded(const DevSys* __restrict__ systems,
                                                        const QbDev* __restrict__ qbs) {
   const QbDev& Q = qbs[blockIdx.x];
@@ -271,35 +283,45 @@ __global__ __launch_bounds__(QB_THREADS) void k_qbonded(const DevSys* __restrict
   if (nterm == 0) return;
   const DevSys& S = systems[blockIdx.x];
   const double* x = S.x;
-  constexpr int NW = QB_THREADS / QNB_WAVE;
-  __shared__ double ered[QNB_MAXST * 4][NW];
-  const int wave = threadIdx.x / QNB_WAVE, lane = threadIdx.x % QNB_WAVE;
-  for (int s = 0; s < Q.ns; s++) {
-    const double lam = Q.lambda[s];
-    double* sl = Q.slots + 3 * (size_t)Q.nslot * s;
-    double e[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < Q.nb; b += blockDim.x) qb_bond(Q, sl, x, b, s, lam, e[0]);
-    __syncthreads();
-    for (int t = threadIdx.x; t < Q.na + Q.nt + Q.ni; t += blockDim.x) {
-      if (t < Q.na) qb_angle(Q, sl, x, t, s, lam, e[1]);
-      else if (t < Q.na + Q.nt) qb_dihedral(Q, sl, x, t - Q.na, false, s, lam, e[2]);
-      else qb_dihedral(Q, sl, x, t - Q.na - Q.nt, true, s, lam, e[3]);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const double v = wave_sum(e[k]);
-      if (lane == 0) ered[4 * s + k][wave] = v;
-    }
-    __syncthreads();
+  const int ns = Q.ns;
+  for (int k = threadIdx.x; k < Q.nb * ns; k += blockDim.x) {
+    const int b = k / ns, s = k - b * ns;
+    double e = 0.0;
+    qb_bond(Q, Q.slots + 3 * (size_t)Q.nslot * s, x, b, s, Q.lambda[s], e);
+    stg(Q.eterm + k, e);
   }
-  if ((int)threadIdx.x < 4 * Q.ns) {
+  __syncthreads();
+  const int nrest = Q.na + Q.nt + Q.ni;
+  for (int k = threadIdx.x; k < nrest * ns; k += blockDim.x) {
+    const int t = k / ns, s = k - t * ns;
+    double* sl = Q.slots + 3 * (size_t)Q.nslot * s;
+    double e = 0.0;
+    if (t < Q.na) qb_angle(Q, sl, x, t, s, Q.lambda[s], e);
+    else if (t < Q.na + Q.nt) qb_dihedral(Q, sl, x, t - Q.na, false, s, Q.lambda[s], e);
+    else qb_dihedral(Q, sl, x, t - Q.na - Q.nt, true, s, Q.lambda[s], e);
+    stg(Q.eterm + (size_t)Q.nb * ns + k, e);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < Q.nb; b += blockDim.x)
+    for (int s = ns - 1; s >= 0; s--)
+      if (Q.pres[b * ns + s]) {
+        const double* m = Q.snap + 4 * (b * ns + s);
+        for (int c = 0; c < 4; c++) stg(Q.morse + 4 * b + c, m[c]);
+        break;
+      }
+  // energies: wave w sums kind k of state s for (s, k) = w, w + NW, ...
+  const int wave = threadIdx.x / QNB_WAVE, lane = threadIdx.x % QNB_WAVE;
+  const int lo[5] = {0, Q.nb, Q.nb + Q.na, Q.nb + Q.na + Q.nt, nterm};
+  for (int sk = wave; sk < 4 * ns; sk += QB_THREADS / QNB_WAVE) {
+    const int s = sk / 4, kind = sk - 4 * s;
     double v = 0.0;
-    for (int w = 0; w < NW; w++) v += ered[threadIdx.x][w];
-    Q.e[threadIdx.x] = v;
+    for (int t = lo[kind] + lane; t < lo[kind + 1]; t += QNB_WAVE) v += ldg(Q.eterm + (size_t)t * ns + s);
+    v = wave_sum(v);
+    if (lane == 0) stg(Q.e + sk, v);
   }
   for (int a = threadIdx.x; a < Q.nat; a += blockDim.x) {
     V3 sum = {0.0, 0.0, 0.0};
-    for (int s = 0; s < Q.ns; s++) {
+    for (int s = 0; s < ns; s++) {
       const double* sl = Q.slots + 3 * (size_t)Q.nslot * s;
       for (int m = Q.coff[a]; m < Q.coff[a + 1]; m++) sum = sum + ld3(sl, Q.cslot[m]);
     }
@@ -479,6 +501,8 @@ int qbonded_build(const qnb_qbonded* q, int nstates, int natom, const double* la
   };
   scratch(&out.morse, 4 * (size_t)nb);
   scratch(&out.slots, 3 * (size_t)nslot * nstates);
+  scratch(&out.snap, 4 * (size_t)nb * nstates);
+  scratch(&out.eterm, (size_t)nterm * nstates);
   scratch(&out.e, 4 * (size_t)nstates);
   return rc;
 }
