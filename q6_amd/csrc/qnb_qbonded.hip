@@ -1,17 +1,19 @@
 // qnb_qbonded.hip -- Q-atom bonded terms per state on the device.
 //
-// One block per instance runs pot_energy's per-state loop
-// (src/potene.f90:183-190): for each state, all Q bonds (qbond,
-// src/bondene.f90:506-551) and then every angle (+ Urey-Bradley), torsion and
+// One block per instance computes what pot_energy's per-state loop
+// (src/potene.f90:183-190) computes: for each state, the Q bonds (qbond,
+// src/bondene.f90:506-551), then every angle (+ Urey-Bradley), torsion and
 // improper (qangle :387-446, qurey_bradley :450-502, qtorsion :624-685,
-// qimproper :555-620) in parallel, one thread per term.  A coupled term reads
-// the Morse state of its bond (EMorseD, dMorse_i) that the bond phase of this
-// or an earlier state, or an earlier call, left: it lives in device memory
-// between calls as the reference's module arrays do.  Gradient contributions
-// go to fixed slots; every atom the terms touch sums its own slots in a fixed
-// order, so results do not depend on thread timing, and adds the total to d
-// once at the end.  The term counts are small (tens per FEP file), so one
-// block of 256 threads per instance is the whole launch.
+// qimproper :555-620).  All states run at once: a thread per (bond, state),
+// then a thread per (term, state).  A coupled term reads the Morse state of
+// its bond (EMorseD, dMorse_i) that the sequential loop would hold at that
+// point: the bond's evaluation in the latest state up to its own where the
+// bond is present, else the value kept from the previous call, which lives in
+// device memory between calls as the reference's module arrays do.  Gradient
+// contributions go to fixed slots per state; every atom the terms touch sums
+// its own slots in a fixed order, so results do not depend on thread timing,
+// and adds the total to d once at the end.  The term counts are small (tens
+// per FEP file), so one block of 256 threads per instance is the whole launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
