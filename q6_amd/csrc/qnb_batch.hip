@@ -460,6 +460,7 @@ struct qnb_batch {
   std::vector<std::vector<void*>> qb_owned;
   QbDev* dqb = nullptr;
   bool anyqb = false;
+  bool anysnap = false;  // some instance stores coordinate snapshots (k_snap launched)
   long long steps_done = 0;
   ~qnb_batch() {
     if (cycle_exec) (void)hipGraphExecDestroy(cycle_exec);
@@ -563,7 +564,8 @@ static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
   hipLaunchKernelGGL(k_bonded, dim3(b->nbblk, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   if (b->anyqb) launch_qbonded(b->stream, B, b->dsys, b->dqb);
   hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(1024), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
-  hipLaunchKernelGGL(k_snap, dim3((3 * b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  if (b->anysnap)
+    hipLaunchKernelGGL(k_snap, dim3((3 * b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_integrate, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_thermo, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
@@ -904,6 +906,19 @@ extern "C" int32_t qnb_batch_set_offdiag(qnb_batch* b, int32_t k, int32_t n, con
   return 0;
 }
 
+// the captured step graph embeds the launch list and shapes: a change to
+// either drops it, and the next run recaptures
+static void drop_step_graph(qnb_batch* b) {
+  if (b->cycle_exec) {
+    (void)hipGraphExecDestroy(b->cycle_exec);
+    b->cycle_exec = nullptr;
+  }
+  if (b->cycle_graph) {
+    (void)hipGraphDestroy(b->cycle_graph);
+    b->cycle_graph = nullptr;
+  }
+}
+
 extern "C" int32_t qnb_batch_set_qbonded(qnb_batch* b, int32_t k, const qnb_qbonded* qb) {
   if (!b || k < 0 || k >= b->n) return fail(QNB_EINVAL, "bad instance");
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -922,14 +937,7 @@ extern "C" int32_t qnb_batch_set_qbonded(qnb_batch* b, int32_t k, const qnb_qbon
   HIPCHK(hipMemcpy(b->dqb, b->hqb.data(), sizeof(QbDev) * b->n, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->dmd, b->hm.data(), sizeof(MdDev) * b->n, hipMemcpyHostToDevice));
   // the step graph holds the launch list: recapture with (or without) k_qbonded
-  if (b->cycle_exec) {
-    (void)hipGraphExecDestroy(b->cycle_exec);
-    b->cycle_exec = nullptr;
-  }
-  if (b->cycle_graph) {
-    (void)hipGraphDestroy(b->cycle_graph);
-    b->cycle_graph = nullptr;
-  }
+  drop_step_graph(b);
   return rc;
 }
 
@@ -954,6 +962,12 @@ extern "C" int32_t qnb_batch_store_frames(qnb_batch* b, int32_t every, int32_t m
     HIPCHK(hipMemcpy(M.ctr + 2, &zero, sizeof zero, hipMemcpyHostToDevice));
   }
   HIPCHK(hipMemcpy(b->dmd, b->hm.data(), sizeof(MdDev) * b->n, hipMemcpyHostToDevice));
+  // k_snap is launched only while snapshots are stored
+  const bool snap = every > 0 && max_frames > 0;
+  if (snap != b->anysnap) {
+    b->anysnap = snap;
+    drop_step_graph(b);
+  }
   return 0;
 }
 
