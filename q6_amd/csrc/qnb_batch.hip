@@ -563,10 +563,13 @@ static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
   }
   hipLaunchKernelGGL(k_bonded, dim3(b->nbblk, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   if (b->anyqb) launch_qbonded(b->stream, B, b->dsys, b->dqb);
-  hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(1024), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
+  // the reduction and the record do nothing on a step no instance samples
+  // (energy is false only then); the record also counts stored snapshots
+  if (energy)
+    hipLaunchKernelGGL(k_reduce_energies, dim3(B), dim3(1024), 0, b->stream, b->dsys, (int)b->anyq, (int)b->anypol);
   if (b->anysnap)
     hipLaunchKernelGGL(k_snap, dim3((3 * b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
-  hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
+  if (energy || b->anysnap) hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_integrate, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   hipLaunchKernelGGL(k_thermo, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
 }
