@@ -1338,10 +1338,26 @@ __global__ __launch_bounds__(WP_THREADS) void k_watpol_theta(const DevSys* __res
   }
   stg(S.w_theta + w, th);
   stg(S.w_shell + w, sh);
-  if (sh >= 0) {
-    const int pos = atomicAdd(S.sh_n + sh, 1);
-    stg(S.sh_list + (size_t)sh * nw + pos, w);
-    stg(S.sh_lth + (size_t)sh * nw + pos, th);
+  // append to the shell lists with one atomic per (wave, shell): the lanes of
+  // a shell take consecutive places (k_watpol_rank orders each shell by
+  // (theta, water index), so the order of appending does not matter)
+  const int lane = t % QNB_WAVE;
+  int pending = sh;
+  for (;;) {
+    const uint64_t act = __ballot(pending >= 0);
+    if (act == 0) break;
+    const int lead = __ffsll((unsigned long long)act) - 1;
+    const int s0 = __shfl(pending, lead, QNB_WAVE);
+    const uint64_t m = __ballot(pending == s0);
+    int base = 0;
+    if (lane == lead) base = atomicAdd(S.sh_n + s0, __popcll(m));
+    base = __shfl(base, lead, QNB_WAVE);
+    if (pending == s0) {
+      const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+      stg(S.sh_list + (size_t)s0 * nw + pos, w);
+      stg(S.sh_lth + (size_t)s0 * nw + pos, th);
+      pending = -1;
+    }
   }
 }
 
