@@ -33,7 +33,7 @@ __global__ void k_watpol_finish(const DevSys*);
 void launch_watpol(hipStream_t st, int nblk, int B, const DevSys* dsys, const int* istep) {
   const dim3 g(nblk, B), t(WP_THREADS);
   hipLaunchKernelGGL(k_watpol_theta, g, t, 0, st, dsys, istep);
-  hipLaunchKernelGGL(k_watpol_rank, dim3(QNB_MAXSH, B), dim3(1024), 0, st, dsys);
+  hipLaunchKernelGGL(k_watpol_rank, dim3(QNB_MAXSH, B), dim3(WP_RANK_THREADS), 0, st, dsys);
   hipLaunchKernelGGL(k_watpol_force, g, t, 0, st, dsys);
   hipLaunchKernelGGL(k_watpol_finish, dim3(B), dim3(64), 0, st, dsys);
 }

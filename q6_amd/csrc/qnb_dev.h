@@ -356,6 +356,11 @@ inline int qforces_blocks(int nq, int chunks, int W) { return 8 * ((chunks + W -
 #define QQ_THREADS 1024
 // threads per block of the polarisation-restraint kernels (one per water)
 #define WP_THREADS 256
+// threads of a polarisation-shell rank block (k_watpol_rank loops over any
+// shell size with this stride; the launch uses the same value)
+#ifndef WP_RANK_THREADS
+#define WP_RANK_THREADS 1024
+#endif
 
 #define FC_WPB 8
 #ifndef FC_CAP

@@ -1367,7 +1367,6 @@ __global__ __launch_bounds__(WP_THREADS) void k_watpol_theta(const DevSys* __res
 // (theta, water index) with a bitonic network in LDS and the rank is the
 // sorted position; larger shells count, for every member, the members
 // before it (staged through LDS in chunks of WP_SORT).
-#define WP_RANK_THREADS 1024
 #define WP_SORT 4096
 __global__ __launch_bounds__(WP_RANK_THREADS) void k_watpol_rank(const DevSys* __restrict__ systems) {
   const DevSys& S = systems[blockIdx.y];
