@@ -90,63 +90,108 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // (each term is evaluated once per member atom), so the gradient needs no
 // atomics and does not depend on scheduling.  A bond's energy is counted by
 // its first atom, an angle's by its vertex; each block's energy sums go to
-// bpart[block] and k_record adds them in block order.
+// bpart[block] and k_record adds them in block order.  bonded_atom is the
+// per-atom part, used by k_bonded (the minimiser) and k_atom_step (MD).
+__device__ __forceinline__ void bonded_atom(const DevSys& S, const MdDev& M, int i, double* f, double& eb,
+                                            double& ea) {
+  const auto x = gp(S.x);
+  for (int it = ldg(M.boff + i), e = ldg(M.boff + i + 1); it < e; it++) {
+    const int item = ldg(M.bitem + it), b = item >> 1, role = item & 1;
+    const int bi = ldg(M.bond_ij + 2 * b), bj = ldg(M.bond_ij + 2 * b + 1);
+    const double ax = x[3 * bj] - x[3 * bi], ay = x[3 * bj + 1] - x[3 * bi + 1], az = x[3 * bj + 2] - x[3 * bi + 2];
+    const double r = sqrt(ax * ax + ay * ay + az * az);
+    const double kb = ldg(M.bond_k + b), db = r - ldg(M.bond_r0 + b);
+    const double dv = kb * db / r;
+    const double sg = role == 0 ? -dv : dv;
+    f[0] += ax * sg;
+    f[1] += ay * sg;
+    f[2] += az * sg;
+    if (role == 0) eb += 0.5 * kb * db * db;
+  }
+  for (int it = ldg(M.aoff + i), e = ldg(M.aoff + i + 1); it < e; it++) {
+    const int item = ldg(M.aitem + it), a = item / 3, role = item - 3 * a;
+    const int ai = ldg(M.angle_ijk + 3 * a), aj = ldg(M.angle_ijk + 3 * a + 1), ak = ldg(M.angle_ijk + 3 * a + 2);
+    double u[3], w[3];
+    for (int c = 0; c < 3; c++) {
+      u[c] = x[3 * ai + c] - x[3 * aj + c];
+      w[c] = x[3 * ak + c] - x[3 * aj + c];
+    }
+    const double ru = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double rw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    double ct = (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / (ru * rw);
+    ct = fmin(1.0, fmax(-1.0, ct));
+    const double th = acos(ct);
+    // sin(theta) from the cross product, consistent with the numerator of
+    // the derivative below even for nearly linear angles, where
+    // sin(acos(ct)) loses all digits; at a linear angle the derivative has
+    // no direction
+    const double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
+    const double st = sqrt(cx * cx + cy * cy + cz * cz) / (ru * rw);
+    const double ka = ldg(M.angle_k + a), da = th - ldg(M.angle_t0 + a);
+    if (role == 1) ea += 0.5 * ka * da * da;
+    if (!(st > 1.0e-8)) continue;
+    const double dv = ka * da;
+    // dtheta/dx_i = -(w/(ru rw) - ct u/ru^2)/sin(theta), likewise for k; j balances
+    for (int c = 0; c < 3; c++) {
+      const double gi = -(w[c] / (ru * rw) - ct * u[c] / (ru * ru)) / st;
+      const double gk = -(u[c] / (ru * rw) - ct * w[c] / (rw * rw)) / st;
+      f[c] += dv * (role == 0 ? gi : (role == 2 ? gk : -(gi + gk)));
+    }
+  }
+}
+
+// solute bonds and angles alone (the minimiser's step)
 __global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds) {
   const DevSys& S = systems[blockIdx.y];
   const MdDev& M = mds[blockIdx.y];
   __shared__ double red[4];
-  const auto x = gp(S.x);
   double eb = 0.0, ea = 0.0;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < M.nbat) {
     double f[3] = {0.0, 0.0, 0.0};
-    for (int it = ldg(M.boff + i), e = ldg(M.boff + i + 1); it < e; it++) {
-      const int item = ldg(M.bitem + it), b = item >> 1, role = item & 1;
-      const int bi = ldg(M.bond_ij + 2 * b), bj = ldg(M.bond_ij + 2 * b + 1);
-      const double ax = x[3 * bj] - x[3 * bi], ay = x[3 * bj + 1] - x[3 * bi + 1], az = x[3 * bj + 2] - x[3 * bi + 2];
-      const double r = sqrt(ax * ax + ay * ay + az * az);
-      const double kb = ldg(M.bond_k + b), db = r - ldg(M.bond_r0 + b);
-      const double dv = kb * db / r;
-      const double sg = role == 0 ? -dv : dv;
-      f[0] += ax * sg;
-      f[1] += ay * sg;
-      f[2] += az * sg;
-      if (role == 0) eb += 0.5 * kb * db * db;
-    }
-    for (int it = ldg(M.aoff + i), e = ldg(M.aoff + i + 1); it < e; it++) {
-      const int item = ldg(M.aitem + it), a = item / 3, role = item - 3 * a;
-      const int ai = ldg(M.angle_ijk + 3 * a), aj = ldg(M.angle_ijk + 3 * a + 1), ak = ldg(M.angle_ijk + 3 * a + 2);
-      double u[3], w[3];
-      for (int c = 0; c < 3; c++) {
-        u[c] = x[3 * ai + c] - x[3 * aj + c];
-        w[c] = x[3 * ak + c] - x[3 * aj + c];
-      }
-      const double ru = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-      const double rw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-      double ct = (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / (ru * rw);
-      ct = fmin(1.0, fmax(-1.0, ct));
-      const double th = acos(ct);
-      // sin(theta) from the cross product, consistent with the numerator of
-      // the derivative below even for nearly linear angles, where
-      // sin(acos(ct)) loses all digits; at a linear angle the derivative has
-      // no direction
-      const double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
-      const double st = sqrt(cx * cx + cy * cy + cz * cz) / (ru * rw);
-      const double ka = ldg(M.angle_k + a), da = th - ldg(M.angle_t0 + a);
-      if (role == 1) ea += 0.5 * ka * da * da;
-      if (!(st > 1.0e-8)) continue;
-      const double dv = ka * da;
-      // dtheta/dx_i = -(w/(ru rw) - ct u/ru^2)/sin(theta), likewise for k; j balances
-      for (int c = 0; c < 3; c++) {
-        const double gi = -(w[c] / (ru * rw) - ct * u[c] / (ru * ru)) / st;
-        const double gk = -(u[c] / (ru * rw) - ct * w[c] / (rw * rw)) / st;
-        f[c] += dv * (role == 0 ? gi : (role == 2 ? gk : -(gi + gk)));
-      }
-    }
+    bonded_atom(S, M, i, f, eb, ea);
     const auto d = gpm(S.d);
     d[3 * i] += f[0];
     d[3 * i + 1] += f[1];
     d[3 * i + 2] += f[2];
+  }
+  const double sb = block_sum(eb, red);
+  const double sa = block_sum(ea, red);
+  if (threadIdx.x == 0 && blockIdx.x < M.nbpart) {
+    M.bpart[2 * blockIdx.x] = sb;
+    M.bpart[2 * blockIdx.x + 1] = sa;
+  }
+}
+
+// The per-atom terms of an MD step in one pass: LRF Taylor application and
+// the SCAAS radial restraint (as k_atom_terms with every term on) and the
+// solute bonds and angles, added to the atom's gradient in one update.
+__global__ __launch_bounds__(256) void k_atom_step(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds,
+                                                   const double* __restrict__ tfree) {
+  const DevSys& S = systems[blockIdx.y];
+  const MdDev& M = mds[blockIdx.y];
+  __shared__ double red[4];
+  double eb = 0.0, ea = 0.0;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < S.natom) {
+    double f[3] = {0.0, 0.0, 0.0};
+    double el = 0.0, er = 0.0;
+    if (S.use_lrf && !ldg(S.excl + i) && ldg(S.iqatom + i) == 0) lrf_taylor_atom(S, i, f, el);
+    const int w = i - S.nat_solute;
+    if (w >= 0 && w % S.sa == 0 && S.sa == 3 && !ldg(S.excl + i)) {
+      const double T = tfree[blockIdx.y];
+      const double shift = S.fk_wsphere != 0.0 ? sqrt(S.Boltz * T / S.fk_wsphere) : 0.0;
+      radial_restraint(S, i, shift, f, er);
+    }
+    if (i < M.nbat) bonded_atom(S, M, i, f, eb, ea);
+    const auto d = gpm(S.d);
+    d[3 * i] += f[0];
+    d[3 * i + 1] += f[1];
+    d[3 * i + 2] += f[2];
+    if (want_energies(S)) {
+      stg(S.eatom + 2 * i, el);
+      stg(S.eatom + 2 * i + 1, er);
+    }
   }
   const double sb = block_sum(eb, red);
   const double sa = block_sum(ea, red);
@@ -515,12 +560,13 @@ static void launch_forces_e(qnb_batch* b, int wpb) {
 // cannot tell, e.g. inside a graph whose period is not a multiple of every
 // instance's sampling interval)
 template <typename R, int NS>
-static void launch_forces_t(qnb_batch* b, int wpb, bool energy = true) {
+static void launch_forces_t(qnb_batch* b, int wpb, bool energy = true, bool atom_terms = true) {
   wpb = forces_wpb<R>(wpb);
   if (energy) launch_forces_e<R, NS, true>(b, wpb);
   else launch_forces_e<R, NS, false>(b, wpb);
-  hipLaunchKernelGGL(k_atom_terms, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys, kStepMask,
-                     (const double*)b->tfree);
+  if (atom_terms)
+    hipLaunchKernelGGL(k_atom_terms, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys, kStepMask,
+                       (const double*)b->tfree);
 }
 
 static void launch_forces(qnb_batch* b, int wpb, bool energy = true) {
@@ -534,11 +580,13 @@ static void launch_forces(qnb_batch* b, int wpb, bool energy = true) {
   }
 }
 
+// atom_terms = false: the caller runs the per-atom terms itself (the MD step
+// fuses them with the bonded terms in k_atom_step)
 template <typename R, int NS>
-static void enqueue_forces(qnb_batch* b, bool energy = true) {
+static void enqueue_forces(qnb_batch* b, bool energy = true, bool atom_terms = true) {
   const int wpb = 4, B = b->n;
   const uint32_t mask = kStepMask;
-  launch_forces_t<R, NS>(b, b->wpb_forces, energy);
+  launch_forces_t<R, NS>(b, b->wpb_forces, energy, atom_terms);
   if (b->anyq) {
     hipLaunchKernelGGL((k_qforces<R, NS>), dim3(qforces_blocks(b->nq_max, b->qchunks, wpb), B), dim3(64 * wpb), 0,
                        b->stream, b->dsys, mask, b->qchunks);
@@ -555,13 +603,14 @@ static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
     if (b->anyq && !b->static_q) hipLaunchKernelGGL(k_qlists, dim3(B), dim3(1024), 0, b->stream, b->dsys);
   }
   switch (b->ns_max) {
-    case 1: enqueue_forces<R, 1>(b, energy); break;
-    case 2: enqueue_forces<R, 2>(b, energy); break;
+    case 1: enqueue_forces<R, 1>(b, energy, false); break;
+    case 2: enqueue_forces<R, 2>(b, energy, false); break;
     case 3:
-    case 4: enqueue_forces<R, 4>(b, energy); break;
-    default: enqueue_forces<R, 8>(b, energy); break;
+    case 4: enqueue_forces<R, 4>(b, energy, false); break;
+    default: enqueue_forces<R, 8>(b, energy, false); break;
   }
-  hipLaunchKernelGGL(k_bonded, dim3(b->nbblk, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  hipLaunchKernelGGL(k_atom_step, dim3((b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd,
+                     (const double*)b->tfree);
   if (b->anyqb) launch_qbonded(b->stream, B, b->dsys, b->dqb);
   // the reduction and the record do nothing on a step no instance samples
   // (energy is false only then); the record also counts stored snapshots

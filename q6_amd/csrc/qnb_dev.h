@@ -224,6 +224,59 @@ __device__ __forceinline__ double dist4p(const double* __restrict__ x, int a, co
   return r;
 }
 
+// LRF Taylor application for one atom (lrf_taylor, nonbondene.f90:507-571)
+__device__ __forceinline__ void lrf_taylor_atom(const DevSys& S, int i, double* f, double& e) {
+  const auto L = gp(S.lrf + (size_t)QNB_NLRF * ldg(S.atom_group + i));
+  const auto x = gp(S.x);
+  const double drx = L[0] - x[3 * i], dry = L[1] - x[3 * i + 1], drz = L[2] - x[3 * i + 2];
+  const auto p1 = L + 4;
+  const auto p2 = L + 7;
+  const auto p3 = L + 16;
+  double t0 = drx * p2[0] + dry * p2[1] + drz * p2[2];
+  double t1 = drx * p2[3] + dry * p2[4] + drz * p2[5];
+  double t2 = drx * p2[6] + dry * p2[7] + drz * p2[8];
+  double V = L[3] + (drx * p1[0] + dry * p1[1] + drz * p1[2]) + 0.5 * (drx * t0 + dry * t1 + drz * t2);
+  const double q = ldg(S.crg + i);
+  e += 0.5 * q * V;
+  double df[3];
+  for (int k = 0; k < 3; k++) {
+    const auto P = p3 + 9 * k;
+    double u0 = drx * P[0] + dry * P[1] + drz * P[2];
+    double u1 = drx * P[3] + dry * P[4] + drz * P[5];
+    double u2 = drx * P[6] + dry * P[7] + drz * P[8];
+    df[k] = p1[k] + (drx * p2[3 * k] + dry * p2[3 * k + 1] + drz * p2[3 * k + 2]) +
+            0.5 * (drx * u0 + dry * u1 + drz * u2);
+  }
+  f[0] -= df[0] * q;
+  f[1] -= df[1] * q;
+  f[2] -= df[2] * q;
+}
+
+// SCAAS radial restraint on one water oxygen (restrain_solvent,
+// nonbondene.f90:6466-6544, 3-site branch)
+__device__ __forceinline__ void radial_restraint(const DevSys& S, int i, double shift, double* f, double& e) {
+  const auto x = gp(S.x);
+  double vx = x[3 * i] - S.xw[0], vy = x[3 * i + 1] - S.xw[1], vz = x[3 * i + 2] - S.xw[2];
+  double b = sqrt(vx * vx + vy * vy + vz * vz);
+  double db = b - (S.rwat - shift);
+  double erst, dv;
+  if (db > 0.0) {
+    erst = 0.5 * S.fk_wsphere * (db * db) - S.Dwmz;
+    dv = S.fk_wsphere * db / b;
+  } else if (b > 0.0) {
+    double fexp = exp(S.awmz * db);
+    erst = S.Dwmz * (fexp * fexp - 2.0 * fexp);
+    dv = -2.0 * S.Dwmz * S.awmz * (fexp - fexp * fexp) / b;
+  } else {
+    erst = 0.0;
+    dv = 0.0;
+  }
+  f[0] += vx * dv;
+  f[1] += vy * dv;
+  f[2] += vz * dv;
+  e += erst;
+}
+
 // Q-solute group test (nbqplist, nonbondene.f90:3647-3742): the group's
 // switch atom within Rcq of xpcent; with any_atom (nbqplis2 :3408-3500) any
 // atom of the group, scanned in group order
