@@ -107,6 +107,8 @@ def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True, positio
                 max_frames=(warmup + steps + extra) // 10 + 2, positions=positions)
     if tune:
         b.shapes = b.autotune()
+    if graph:
+        b.prepare_graph()
     b.run(warmup, graph=graph)
     b.sync()
     t0 = time.perf_counter()
@@ -116,14 +118,36 @@ def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True, positio
     return b, dt
 
 
-def free_energy_precision(bm, s, lambdas, nfr):
+def oracle_state_energies(s, frames_by_window, threads):
+    """FP64 oracle state energies EQ(s)%qq + %qp + %qw of every stored frame
+    (the Q lists are built once, as Qdyn6 does when Rcq exceeds the sphere),
+    one oracle per thread over the windows."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle.oracle import Oracle
+    from q6_amd import abi
+    qmask = abi.T_QP | abi.T_QW | abi.T_QQ | abi.T_QQP
+    ns = int(s.n["nstates"])
+
+    def one(frames):
+        o = Oracle(s)
+        o.make_pair_lists(frames[0])
+        E = np.zeros((len(frames), ns))
+        for f, x in enumerate(frames):
+            _, _, EQ = o.eval_terms(qmask, x)
+            E[f] = [sum(q.values()) for q in EQ]
+        return E
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, frames_by_window))
+
+
+def free_energy_precision(bm, s, lambdas, nfr, threads):
     """Runs nfr*10 more steps storing a frame and an energy record every 10;
-    the per-window state energies the mixed run recorded and the plain FP64
-    engine's state energies of the same frames give two Zwanzig dG curves
+    the per-window state energies the mixed run recorded and the FP64
+    oracle's state energies of the same frames give two Zwanzig dG curves
     over the same frames: their difference is the precision error of dG.
     Then times the adjoint over the stored frames (one launch, all windows)."""
-    from q6_amd import abi, adjoint, fep
-    from q6_amd.engine import NonbondedEngine
+    from q6_amd import adjoint, fep
     from q6_amd.md import q_state_energies
     nw = len(lambdas)
     bm.store_frames(10, nfr)
@@ -131,17 +155,9 @@ def free_energy_precision(bm, s, lambdas, nfr):
     bm.sync()
     frames = [bm.frames(k) for k in range(nw)]
     V_mixed = [q_state_energies(bm.energies(k)[-nfr:], 2) for k in range(nw)]
-    eng = NonbondedEngine(s)
-    eng.make_pair_lists(frames[0][0])
-    qmask = abi.T_QP | abi.T_QW | abi.T_QQ | abi.T_QQP
-    V64 = []
-    for k in range(nw):
-        E = np.zeros((nfr, 2))
-        for f in range(nfr):
-            _, _, EQ = eng.eval_terms(qmask, frames[k][f])
-            E[f] = [sum(q.values()) for q in EQ]
-        V64.append(E)
-    eng.close()
+    t0 = time.perf_counter()
+    V64 = oracle_state_energies(s, frames, threads)
+    t_orc = time.perf_counter() - t0
     kT = 0.001986 * 300.0
     lam = np.asarray(lambdas)
     _, _, dG_m = fep.zwanzig(V_mixed, lam, kT)
@@ -156,13 +172,14 @@ def free_energy_precision(bm, s, lambdas, nfr):
     g = adjoint.gradients([v for v, _ in VdV], [d for _, d in VdV], space, lam, kT, [1.0, -1.0])
     t_red = time.perf_counter() - t1
     E_adj = [v for v, _ in VdV]
-    return {"frames_per_window": nfr, "windows": nw,
-            "dG_end_mixed": round(float(dG_m[-1]), 4), "dG_end_fp64": round(float(dG_64[-1]), 4),
+    return {"frames_per_window": nfr, "windows": nw, "comparator": "FP64 oracle (C restatement) on the same frames",
+            "oracle_s": round(t_orc, 1), "oracle_threads": threads,
+            "dG_end_mixed": round(float(dG_m[-1]), 4), "dG_end_fp64_oracle": round(float(dG_64[-1]), 4),
             "max_abs_dG_diff_kcal_mol": float(np.abs(dG_m - dG_64).max()),
             "max_abs_state_energy_diff_kcal_mol": float(max(np.abs(a - b).max() for a, b in zip(V_mixed, V64))),
             "adjoint": {"launch_s": round(t_adj, 4), "frames": nw * nfr, "frames_per_s": round(nw * nfr / t_adj, 1),
                         "params": space.npar, "host_reduction_s": round(t_red, 4),
-                        "max_abs_E_adjoint_vs_fp64": float(max(np.abs(a - b).max() for a, b in zip(E_adj, V64))),
+                        "max_abs_E_adjoint_vs_oracle": float(max(np.abs(a - b).max() for a, b in zip(E_adj, V64))),
                         "grad_dG_norm": float(np.linalg.norm(g["dG"]))}}
 
 
@@ -180,6 +197,14 @@ def cpu_baseline(s, ncycles=1):
     dt = time.perf_counter() - t0
     steps = 10 * ncycles
     return steps / dt, dt, steps
+
+
+def host_cores():
+    """Host cores this process may run on (its CPU affinity), at most 16."""
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
 
 
 def main():
@@ -202,8 +227,8 @@ def main():
         print("bench.py runs on one GPU: lambda windows would shard across GPUs (whole windows per rank, "
               "no per-step exchange), but only --gpus 1 is built and measured.", file=sys.stderr)
         sys.exit(2)
-    if args.steps % 10 or args.warmup % 10:
-        sys.exit("--steps and --warmup must be multiples of the 10-step list cycle")
+    if args.steps < 1 or args.warmup < 0:
+        sys.exit("--steps must be positive and --warmup non-negative")
     if args.workload == "uneven":
         return uneven(args)
 
@@ -232,7 +257,7 @@ def main():
     t_forces = bm.time_kernel(1, 20)
     e = bm.energies(0)
     temps = e[:, abi.F_TEMP]
-    prec = free_energy_precision(bm, s, lambdas, args.frames) if args.frames > 0 else None
+    prec = free_energy_precision(bm, s, lambdas, args.frames, host_cores()) if args.frames > 0 else None
     if prec:
         print(f"# dG precision: max |dG_mixed - dG_fp64| = {prec['max_abs_dG_diff_kcal_mol']:.2e} kcal/mol; "
               f"adjoint {prec['adjoint']['frames']} frames in {prec['adjoint']['launch_s']:.3f} s",

@@ -365,7 +365,10 @@ int32_t qnb_batch_create(int32_t n, const qnb_system *const *sys, const qnb_para
                          qnb_batch **out);
 int32_t qnb_batch_destroy(qnb_batch *b);
 /* Advances every instance by nsteps MD steps (device-resident).  With
- * use_graph, each list-update cycle of nbcycle steps is one HIP-graph launch. */
+ * use_graph, each list-update cycle of nbcycle steps is one HIP-graph launch;
+ * the steps before the first cycle boundary and after the last one run as
+ * ordinary launches.  nsteps = 0 with use_graph captures and instantiates the
+ * cycle graph without advancing (so a timed run does not include it). */
 int32_t qnb_batch_run(qnb_batch *b, int32_t nsteps, int32_t use_graph);
 /* Waits for queued work; returns QNB_ENONFINITE / QNB_EOVERFLOW if an instance
  * went non-finite, a constraint failed or a list overflowed. */

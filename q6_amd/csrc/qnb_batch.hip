@@ -868,9 +868,27 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
   return 0;
 }
 
-extern "C" int32_t qnb_batch_run(qnb_batch* b, int32_t nsteps, int32_t use_graph) {
-  if (!b) return fail(QNB_EINVAL, "null batch");
+// captures one list cycle (nbcycle steps, starting with the list update) as a
+// graph, once; it is replayed until the launch list changes
+static int capture_cycle(qnb_batch* b) {
+  if (b->cycle_exec) return 0;
   const int nb = b->nbcycle[0];
+  // a cycle starts at a multiple of nb: its sampled steps sit at fixed
+  // offsets when every sampling interval divides nb
+  bool fixed = true;
+  for (int e : b->ene_every) fixed = fixed && e > 0 && nb % e == 0;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int s = 0; s < nb; s++) enqueue(b, s == 0, fixed ? energy_step(b, s) : true);
+  HIPCHK(hipStreamEndCapture(b->stream, &b->cycle_graph));
+  HIPCHK(hipGraphInstantiate(&b->cycle_exec, b->cycle_graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_run(qnb_batch* b, int32_t nsteps, int32_t use_graph) {
+  if (!b || nsteps < 0) return fail(QNB_EINVAL, "null batch or negative step count");
+  const int nb = b->nbcycle[0];
+  int rc;
+  if (nsteps == 0) return use_graph ? capture_cycle(b) : 0;
   int done = 0;
   // steps up to the next list update, eagerly
   while (done < nsteps && (b->steps_done % nb) != 0) {
@@ -879,16 +897,7 @@ extern "C" int32_t qnb_batch_run(qnb_batch* b, int32_t nsteps, int32_t use_graph
     done++;
   }
   if (use_graph && nsteps - done >= nb) {
-    if (!b->cycle_exec) {
-      // a cycle starts at a multiple of nb: its sampled steps sit at fixed
-      // offsets when every sampling interval divides nb
-      bool fixed = true;
-      for (int e : b->ene_every) fixed = fixed && e > 0 && nb % e == 0;
-      HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      for (int s = 0; s < nb; s++) enqueue(b, s == 0, fixed ? energy_step(b, s) : true);
-      HIPCHK(hipStreamEndCapture(b->stream, &b->cycle_graph));
-      HIPCHK(hipGraphInstantiate(&b->cycle_exec, b->cycle_graph, nullptr, nullptr, 0));
-    }
+    if ((rc = capture_cycle(b))) return rc;
     while (nsteps - done >= nb) {
       HIPCHK(hipGraphLaunch(b->cycle_exec, b->stream));
       b->steps_done += nb;

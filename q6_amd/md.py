@@ -105,6 +105,10 @@ class BatchMD:
     def run(self, nsteps, graph=True):
         abi.check(self.lib, self.lib.qnb_batch_run(self.h, nsteps, int(graph)), "qnb_batch_run")
 
+    def prepare_graph(self):
+        """Capture the list-cycle graph now (no step is taken)."""
+        abi.check(self.lib, self.lib.qnb_batch_run(self.h, 0, 1), "qnb_batch_run")
+
     def minimize(self, nsteps, max_disp=0.02):
         """Steepest-descent relaxation of the start structures (set-up aid)."""
         abi.check(self.lib, self.lib.qnb_batch_minimize(self.h, nsteps, max_disp), "qnb_batch_minimize")

@@ -200,7 +200,8 @@ def _exclusions(nat, bonds):
 
 def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q_sigma=0.3,
           cut=10.0, rcq=99.0, rclrf=99.0, use_lrf=True, lambdas=None, q_state0_zero=False,
-          name="synthetic", q_per_chain=None, compact=None):
+          name="synthetic", q_per_chain=None, compact=None, solute_dmin=2.2, water_dmin=(2.4, 1.8),
+          water_fill=None):
     """A sphere of `radius` A: `nchain` neutral bead chains of `nbead` atoms (one
     charge group each) in the central region, the first `n_q_chains` of them
     (the ones closest to the centre) made of Q-atoms with `nstates` states,
@@ -210,7 +211,11 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
     `compact`, each chain is folded within that radius of its first bead and
     its switch atom is the bead with the smallest largest distance to the
     others, so every atom of a group lies within `compact` of the switch atom
-    and no atom pair in contact is missed by the switch-atom cut-off."""
+    and no atom pair in contact is missed by the switch-atom cut-off.
+    `solute_dmin` is the smallest distance between non-bonded beads as the
+    chains are grown, `water_dmin` the smallest O and H distance from a kept
+    water to any solute atom, and `water_fill` the radius within which water
+    centres are kept (default radius - 0.5)."""
     rng = np.random.default_rng(seed)
     rng_lat = np.random.default_rng(seed + 1)
     rng_rot = np.random.default_rng(seed + 2)
@@ -220,7 +225,8 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
     eps = np.linspace(0.05, 0.2, 6)[::-1]
     bead_masses = [12.011, 14.007, 15.999, 12.011, 32.06, 14.007]
     if nchain:
-        spos, r_used = bead_chains(rng, nchain, nbead, chain_region or radius * 0.6, compact=compact)
+        spos, r_used = bead_chains(rng, nchain, nbead, chain_region or radius * 0.6, dmin=solute_dmin,
+                                   compact=compact)
     else:
         spos, r_used = np.zeros((0, 3)), 0.0
     # Q chains: the chains whose centres are closest to the sphere centre
@@ -229,7 +235,8 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
         order = np.argsort(np.einsum("ij,ij->i", cc, cc), kind="stable")
         spos = spos.reshape(nchain, nbead, 3)[order].reshape(-1, 3)
     nsol = nchain * nbead
-    wat = water_sphere(rng_lat, rng_rot, radius - 0.5, spos)
+    wat = water_sphere(rng_lat, rng_rot, radius - 0.5 if water_fill is None else water_fill, spos,
+                       dmin_o=water_dmin[0], dmin_h=water_dmin[1])
     nwat = len(wat)
     nat = nsol + 3 * nwat
     x = np.concatenate([spos, wat.reshape(-1, 3)]) if nwat else spos.copy()
@@ -616,14 +623,23 @@ def cfg1(seed=1):
     """20 A sphere of TIP3P (~1,100 waters) around a 12-atom neutral chain, all
     12 atoms Q-atoms in 2 states (state 1 charges N(0,0.4), state 2 zero)."""
     return build(20.0, 1, 12, 1, 2, seed, chain_region=2.0, q_sigma=0.4, lambdas=[0.5, 0.5],
-                 q_state0_zero=True, name="cfg1: 20 A TIP3P sphere + 12 Q-atoms")
+                 q_state0_zero=True, water_fill=20.0, name="cfg1: 20 A TIP3P sphere + 12 Q-atoms")
+
+
+# Dense packing of the workload systems: 4,000 solute atoms at about water's
+# atom density (0.1 per A^3) need non-bonded beads as close as 1.7 A while the
+# chains grow (the start is relaxed by steepest descent before MD); water fills
+# the whole sphere at 0.0334 per A^3 wherever its O stays 2.2 A and its H 1.6 A
+# from every solute atom.
+PACK = dict(solute_dmin=1.7, water_dmin=(2.2, 1.6))
 
 
 def cfg2(seed=11, lambdas=(0.5, 0.5)):
-    """30 A sphere: 400 neutral 10-atom bead-chain groups around the centre,
-    30 Q-atoms in 2 EVB states, TIP3P fill."""
+    """30 A sphere of about 11,000 atoms: 400 neutral 10-atom bead-chain
+    groups around the centre (they reach about 22.5 A), 30 Q-atoms in 2 EVB
+    states, TIP3P filling the rest of the sphere (about 2,290 waters)."""
     return build(30.0, 400, 10, 3, 2, seed, chain_region=18.0, q_sigma=0.3, lambdas=list(lambdas), compact=3.0,
-                 name="cfg2: 30 A sphere, 400x10-atom solute groups, 30 Q-atoms, TIP3P")
+                 water_fill=30.0, name="cfg2: 30 A sphere, 400x10-atom solute groups, 30 Q-atoms, TIP3P", **PACK)
 
 
 def describe(s):
@@ -634,17 +650,19 @@ def describe(s):
 
 def cfg5(seed=500, radii=None):
     """Uneven batch: 16 spheres of radius 15, 17, ..., 45 A built as cfg2
-    (ten-atom bead chains, solute about 0.3 of the atoms, TIP3P fill), each
-    with 10-60 Q-atoms (1-6 chains) in 2-3 states (seeded)."""
+    (ten-atom bead chains packed as cfg2's, TIP3P filling the sphere), about
+    0.1 atoms per A^3 of which 0.3 are solute (about 1,400 to 38,000 atoms),
+    each with 10-60 Q-atoms (1-6 chains) in 2-3 states (seeded)."""
     rng = np.random.default_rng(seed)
     radii = list(radii) if radii is not None else list(range(15, 47, 2))
     out = []
     for k, R in enumerate(radii):
         vol = 4.0 / 3.0 * math.pi * R ** 3
-        nchain = max(2, int(round(0.3 * 3 * RHO_WAT * vol / 10 / 1.15)))
+        nchain = max(2, int(round(0.3 * 0.1 * vol / 10)))
         nq = int(rng.integers(1, 7))
         nst = int(rng.integers(2, 4))
         lam = np.full(nst, 1.0 / nst)
         out.append(build(float(R), nchain, 10, min(nq, nchain), nst, seed + 10 * k, chain_region=0.55 * R,
-                         q_sigma=0.3, lambdas=lam, compact=3.0, name=f"cfg5[{k}]: {R} A sphere"))
+                         q_sigma=0.3, lambdas=lam, compact=3.0, water_fill=float(R),
+                         name=f"cfg5[{k}]: {R} A sphere", **PACK))
     return out
