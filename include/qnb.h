@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define QNB_ABI_VERSION 2
+#define QNB_ABI_VERSION 3
 
 enum qnb_status {
   QNB_OK = 0,
@@ -340,6 +340,9 @@ typedef struct qnb_md_params {
   const double *angle_t0, *angle_k;   /* radians; E = k/2 (t - t0)^2            */
   const double *winv;       /* [natom] inverse masses                            */
   double water_roh, water_rhh; /* rigid 3-site water geometry (SHAKE constraints) */
+  int32_t separate_scaling;    /* 1: solute and solvent are separate Berendsen groups
+                                  (Qdyn6's default, src/md.f90:303, src/simprep.f90:3908-3923);
+                                  0: one group for all atoms                          */
 } qnb_md_params;
 
 /* per-frame record of the energy history (float64 each) */
@@ -370,6 +373,19 @@ int32_t qnb_batch_destroy(qnb_batch *b);
  * ordinary launches.  nsteps = 0 with use_graph captures and instantiates the
  * cycle graph without advancing (so a timed run does not include it). */
 int32_t qnb_batch_run(qnb_batch *b, int32_t nsteps, int32_t use_graph);
+/* One integration step of every instance from a caller-supplied gradient:
+ * d[k] (natom*3) replaces instance k's gradient, then leap-frog with each
+ * temperature group's Berendsen factor (newvel_ber, src/md.f90:1451-1466),
+ * SHAKE of the rigid waters and v = (x - x_old)/dt (src/md.f90:1787-1793,
+ * src/bondene.f90:1069-1151) and temperature() for the next factor
+ * (src/md.f90:1379-1447): the part of md_run's step after pot_energy, for a
+ * caller that computes the forces itself.  The step counter advances. */
+int32_t qnb_batch_integrate(qnb_batch *b, const double *const *d);
+/* Thermostat state of instance k after the last step (temperature(),
+ * src/md.f90:1379-1447): out[0..7] = Temp, Tfree, kinetic energy, tfree of
+ * groups 1 and 2, Berendsen factor of groups 1 and 2 for the next step,
+ * number of groups (1 or 2; unused group entries are 0). */
+int32_t qnb_batch_thermostat(qnb_batch *b, int32_t k, double *out);
 /* Waits for queued work; returns QNB_ENONFINITE / QNB_EOVERFLOW if an instance
  * went non-finite, a constraint failed or a list overflowed. */
 int32_t qnb_batch_sync(qnb_batch *b);

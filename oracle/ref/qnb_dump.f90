@@ -31,6 +31,9 @@ program qnb_dump
   real(kind=prec) :: e_lrf
   type(NB_ENERGIES), allocatable :: eq_nb(:)
   real(kind=prec), allocatable :: lam(:)
+  integer :: iseed0, tg, kc, ic, mol
+  real(kind=prec), allocatable :: v_mx(:), x_pc(:), sf0(:)
+  type(qr_vec), allocatable :: x0v(:), v0v(:)
 
   nodeid = 0
   numnodes = 1
@@ -50,8 +53,14 @@ program qnb_dump
   call make_nbqqlist
   call shrink_topology
   call init_trj
+  iseed0 = iseed
   if (iseed > 0) then
     call maxwell
+    ! velocities as maxwell draws them and the coordinates before the initial
+    ! constraint, to pin the seeded start (maxwell, randm, gauss, shake)
+    allocate(v_mx(3*natom), x_pc(3*natom))
+    call pack_x(v, natom, v_mx)
+    call pack_x(x, natom, x_pc)
     call initial_constraint(const_method)
     call stop_cm_translation
   end if
@@ -133,6 +142,7 @@ program qnb_dump
   ! ---------------- types and exclusions ----------------
   call wd(u, 'avdw', 3*natyps, (/((iaclib(i)%avdw(k), k=1,3), i=1,natyps)/), 2, (/3, natyps/))
   call wd(u, 'bvdw', 3*natyps, (/((iaclib(i)%bvdw(k), k=1,3), i=1,natyps)/), 2, (/3, natyps/))
+  call wd(u, 'type_mass', natyps, iaclib(1:natyps)%mass, 1, (/natyps/))
   call wi(u, 'ljcod', natyps*natyps, reshape(ljcod(1:natyps,1:natyps), (/natyps*natyps/)), 2, (/natyps, natyps/))
   if (nat_solute > 0) then
     n = max_nbr_range*nat_solute
@@ -320,6 +330,110 @@ program qnb_dump
   call wd(u, 'd_total', 3*natom, dtmp, 2, (/3, natom/))
   call wd(u, 'E_summary', 12, (/E%potential, E%LRF, E%pp%el, E%pp%vdw, E%pw%el, E%pw%vdw, &
           E%ww%el, E%ww%vdw, E%qx%el, E%qx%vdw, E%restraint%solvent_radial, E%restraint%water_pol/), 1, (/12/))
+  ! the remaining inputs and outputs of pot_energy's summary (src/potene.f90:224-251)
+  call wd(u, 'E_bonded', 14, (/E%p%bond, E%w%bond, E%p%angle, E%w%angle, E%p%torsion, E%p%improper, &
+          E%q%bond, E%q%angle, E%q%torsion, E%q%improper, E%restraint%fix, E%restraint%shell, &
+          E%restraint%protein, E%restraint%total/), 1, (/14/))
+  if (ns > 0) call wd(u, 'EQ_summary', 16*ns, (/(EQ(is)%lambda, EQ(is)%total, EQ(is)%q%bond, EQ(is)%q%angle, &
+          EQ(is)%q%torsion, EQ(is)%q%improper, EQ(is)%qx%el, EQ(is)%qx%vdw, EQ(is)%qq%el, EQ(is)%qq%vdw, &
+          EQ(is)%qp%el, EQ(is)%qp%vdw, EQ(is)%qw%el, EQ(is)%qw%vdw, EQ(is)%restraint, 0.0_prec, is=1,ns)/), &
+          2, (/16, ns/))
+
+  ! ---------------- the seeded start ----------------
+  call wi1(u, 'iseed', iseed0)
+  call wd1(u, 'Tmaxw', Tmaxw)
+  if (allocated(v_mx)) then
+    call wd(u, 'v_maxwell', 3*natom, v_mx, 2, (/3, natom/))
+    call wd(u, 'x_preconstraint', 3*natom, x_pc, 2, (/3, natom/))
+  end if
+
+  ! ---------------- one MD step (md_run, src/md.f90:1774-1797) ----------------
+  ! from the x, v above and d = the gradient pot_energy just left (d_total):
+  ! leap-frog with each temperature group's Berendsen factor, shake, the new
+  ! velocities and temperature()
+  call wd1(u, 'Temp0', Temp0)
+  call wd1(u, 'tau_T', tau_T)
+  call wi1(u, 'ntgroups', ntgroups)
+  call wi(u, 'tgroup_range', 2*ntgroups, (/(tscale(tg)%starta, tscale(tg)%enda, tg=1,ntgroups)/), 2, (/2, ntgroups/))
+  call wi(u, 'tgroup_ndeg', 4*ntgroups, (/(tscale(tg)%Ndegf, tscale(tg)%Ndegfree, tscale(tg)%shake, &
+          tscale(tg)%nexcl, tg=1,ntgroups)/), 2, (/4, ntgroups/))
+  call wd(u, 'tgroup_exclshk', ntgroups, tscale(1:ntgroups)%exclshk, 1, (/ntgroups/))
+  call wi(u, 'ndegf_total', 2, (/Ndegf, Ndegfree/), 1, (/2/))
+  ! the constraints in the order shake visits them: (i, j) and dist2
+  n = 0
+  do mol = 1, const_molecules
+    n = n + const_mol(mol)%nconstraints
+  end do
+  call wi1(u, 'nconstraints', n)
+  if (n > 0) then
+    deallocate(itmp)
+    allocate(itmp(2*n))
+    deallocate(dtmp)
+    allocate(dtmp(max(n, 3*natom)))
+    kc = 0
+    do mol = 1, const_molecules
+      do ic = 1, const_mol(mol)%nconstraints
+        kc = kc + 1
+        itmp(2*kc-1) = const_mol(mol)%bond%bond(ic)%i
+        itmp(2*kc) = const_mol(mol)%bond%bond(ic)%j
+        dtmp(kc) = const_mol(mol)%bond%bond(ic)%dist2
+      end do
+    end do
+    call wi(u, 'const_ij', 2*n, itmp(1:2*n), 2, (/2, n/))
+    call wd(u, 'const_dist2', n, dtmp(1:n), 1, (/n/))
+    deallocate(dtmp)
+    allocate(dtmp(3*natom))
+  end if
+  allocate(x0v(natom), v0v(natom), sf0(ntgroups))
+  x0v(1:natom) = x(1:natom)
+  v0v(1:natom) = v(1:natom)
+  sf0(1:ntgroups) = tscale(1:ntgroups)%sfact
+  call wd(u, 'md1_sfact_in', ntgroups, sf0, 1, (/ntgroups/))
+  do tg = 1, ntgroups
+    call newvel_ber(dt, tscale(tg)%sfact, tscale(tg)%starta, tscale(tg)%enda)
+  end do
+  if (constraints > 0) then
+    n = constraint(const_method, xx, x)
+    v(1:natom) = (x(1:natom) - xx(1:natom)) / dt
+  end if
+  call temperature(tscale(1:ntgroups), Ekinmax_loc, .false.)
+  call pack_x(x, natom, dtmp)
+  call wd(u, 'md1_x', 3*natom, dtmp, 2, (/3, natom/))
+  call pack_x(v, natom, dtmp)
+  call wd(u, 'md1_v', 3*natom, dtmp, 2, (/3, natom/))
+  call wd(u, 'md1_temp', 3 + 3*ntgroups, (/Temp, Tfree, E%kinetic, (tscale(tg)%temp, tscale(tg)%tfree, &
+          tscale(tg)%sfact, tg=1,ntgroups)/), 1, (/3 + 3*ntgroups/))
+  x(1:natom) = x0v(1:natom)
+  v(1:natom) = v0v(1:natom)
+
+  ! ---------------- the polarisation restraint over several steps ----------------
+  ! watpol at istep = 1..7 with itdis = 3 from a zero shell state, each call
+  ! into a zeroed d, with the waters displaced by a pattern both sides can
+  ! form exactly (multiples of 2^-7 A), so the theta_corr feedback at istep
+  ! 3 and 6 (src/nonbondene.f90:6653-6666) runs on changing averages
+  if (nwat > 0 .and. wpol_restr) then
+    wshell(1:nwpolr_shell)%theta_corr = zero
+    wshell(1:nwpolr_shell)%avtheta = zero
+    wshell(1:nwpolr_shell)%avn_insh = zero
+    itdis = 3.0_prec
+    call wd1(u, 'wp_itdis', itdis)
+    do k = 1, 7
+      do i = nat_solute + 1, natom
+        x(i)%x = x0v(i)%x + real(mod(7*i + 3*k, 11) - 5, kind=prec) * 0.0078125_prec
+        x(i)%y = x0v(i)%y + real(mod(5*i + 2*k, 13) - 6, kind=prec) * 0.0078125_prec
+        x(i)%z = x0v(i)%z + real(mod(3*i + 5*k, 7) - 3, kind=prec) * 0.0078125_prec
+      end do
+      istep = k
+      d(:) = zero
+      e_rs = RESTRAINT_ENERGIES(zero, zero, zero, zero, zero, zero)
+      call watpol(e_rs, .true.)
+      call dump_term(u, 'wp'//achar(48 + k), (/e_rs%water_pol/), 1)
+      n = nwpolr_shell
+      call wd(u, 'wp'//achar(48 + k)//'_shell', 3*n, (/(wshell(is)%theta_corr, wshell(is)%avtheta, &
+              wshell(is)%avn_insh, is=1,n)/), 2, (/3, n/))
+    end do
+    x(1:natom) = x0v(1:natom)
+  end if
   close(u)
   write(*,'(a)') 'qnb_dump: wrote '//trim(outname)
 

@@ -84,6 +84,7 @@ class QnbMdParams(C.Structure):
         ("bond_ij", _pi), ("bond_r0", _pd), ("bond_k", _pd),
         ("angle_ijk", _pi), ("angle_t0", _pd), ("angle_k", _pd),
         ("winv", _pd), ("water_roh", C.c_double), ("water_rhh", C.c_double),
+        ("separate_scaling", C.c_int32),
     ]
 
 
@@ -158,12 +159,14 @@ def qbonded_struct(tb, urey=False):
     q.urey_bradley = 1 if urey else 0
     return q, keep
 
+ABI_VERSION = 3
+
 # every symbol include/qnb.h declares (checked by the CPU test suite)
 EXPORTS = [
     "qnb_abi_version", "qnb_last_error", "qnb_device_count", "qnb_create", "qnb_destroy",
     "qnb_make_pair_lists", "qnb_nonbonded", "qnb_qq", "qnb_restraints", "qnb_eval_terms",
     "qnb_get_pairs", "qnb_get_lrf", "qnb_get_shell_state", "qnb_set_shell_state",
-    "qnb_batch_create", "qnb_batch_destroy", "qnb_batch_run", "qnb_batch_sync", "qnb_batch_get_state",
+    "qnb_batch_create", "qnb_batch_destroy", "qnb_batch_run", "qnb_batch_integrate", "qnb_batch_thermostat", "qnb_batch_sync", "qnb_batch_get_state",
     "qnb_batch_energy_history", "qnb_batch_pair_counts", "qnb_batch_time_kernel", "qnb_batch_minimize", "qnb_batch_autotune",
     "qnb_adjoint", "qnb_batch_store_frames", "qnb_batch_frames", "qnb_batch_adjoint", "qnb_batch_set_offdiag",
     "qnb_set_qbonded", "qnb_eval_qbonded", "qnb_batch_set_qbonded",
@@ -193,6 +196,8 @@ def _declare(lib):
     lib.qnb_batch_create.argtypes = [C.c_int32, pp, pp, pp, pp, pp, C.POINTER(vp)]
     lib.qnb_batch_destroy.argtypes = [vp]
     lib.qnb_batch_run.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.qnb_batch_integrate.argtypes = [vp, pp]
+    lib.qnb_batch_thermostat.argtypes = [vp, C.c_int32, _pd]
     lib.qnb_batch_sync.argtypes = [vp]
     lib.qnb_batch_get_state.argtypes = [vp, C.c_int32, _pd, _pd]
     lib.qnb_batch_energy_history.argtypes = [vp, C.c_int32, _pd, C.c_int32, C.POINTER(C.c_int32)]
@@ -223,7 +228,7 @@ def load(path=None):
     if not os.path.exists(p):
         raise RuntimeError(f"HIP engine library not built: {p} (run __graft_entry__.build())")
     lib = _declare(C.CDLL(p))
-    if lib.qnb_abi_version() != 2:
+    if lib.qnb_abi_version() != ABI_VERSION:
         raise RuntimeError("qnb ABI version mismatch")
     if path is None:
         _lib = lib

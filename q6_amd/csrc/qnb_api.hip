@@ -23,6 +23,7 @@ __global__ void k_qforces(const DevSys*, uint32_t, int);
 template <typename R, int NS>
 __global__ void k_qq(const DevSys*, uint32_t);
 __global__ void k_pack_xq(const DevSys*);
+__global__ void k_pw_first(const DevSys*);
 __global__ void k_watpol_theta(const DevSys*, const int*);
 __global__ void k_watpol_rank(const DevSys*);
 __global__ void k_watpol_force(const DevSys*);
@@ -63,8 +64,6 @@ int validate_system(const qnb_system* s, const qnb_params* p) {
   if (s->nat_solute + s->nwat * s->solv_atom != s->natom) return fail(QNB_EINVAL, "natom != nat_solute + nwat*solv_atom");
   if (s->ncgp != s->ncgp_solute + s->nwat) return fail(QNB_EINVAL, "each solvent molecule must be one charge group");
   if (p->nwpolr_shell < 0 || p->nwpolr_shell > QNB_MAX_SHELLS) return fail(QNB_EINVAL, "nwpolr_shell out of range");
-  if (!p->use_LRF && (p->ref_quirks & 1) && s->nwat > 0 && s->ncgp_solute > 0)
-    return fail(QNB_EUNSUPPORTED, "the nbpwlist index-reuse reproduction is only available in the oracle");
   if (s->ntypes <= 0) return fail(QNB_EINVAL, "no atom types");
   for (int i = 0; i < s->natom; i++)
     if (s->iac[i] < 0 || s->iac[i] >= s->ntypes) return fail(QNB_EINVAL, "atom %d has type %d out of range", i, s->iac[i]);
@@ -367,6 +366,7 @@ int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots) {
   if (S.cap <= 0) S.cap = 256;
   AL(nbr, (size_t)S.cap * S.ncgp);
   AL(nnbr, S.ncgp);
+  AL(pw_first, std::max(S.ncgp_solute, 1));
   AL(need, 1);
   AL(qw_in, nw1);
   AL(qp_in, std::max(S.ncgp_solute, 1));
@@ -504,6 +504,8 @@ static void launch_lists(qnb_handle* h) {
   dim3 grid((h->hs.ncgp + wpb - 1) / wpb, 1);
   if (sizeof(R) == 4)
     hipLaunchKernelGGL(k_pack_xq, dim3((h->hs.natom + 255) / 256, 1), dim3(256), 0, h->stream, h->dsys);
+  if (pw_quirk(h->hs))
+    hipLaunchKernelGGL(k_pw_first, dim3((h->hs.ncgp_solute + 3) / 4, 1), dim3(256), 0, h->stream, h->dsys);
   if (h->hs.any_atom) hipLaunchKernelGGL((k_lists_lrf<R, true>), grid, dim3(64 * wpb), 0, h->stream, h->dsys);
   else hipLaunchKernelGGL((k_lists_lrf<R, false>), grid, dim3(64 * wpb), 0, h->stream, h->dsys);
 }

@@ -19,6 +19,7 @@ template <typename R, bool ANY>
 __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
 __global__ void k_pack_xq(const DevSys*);
+__global__ void k_pw_first(const DevSys*);
 template <typename R, int NS, bool WAT, bool E>
 __global__ void k_forces(const DevSys*, uint32_t, const double*);
 template <typename R, int NS>
@@ -42,11 +43,17 @@ struct MdDev {
   const double* angle_k;
   const double* winv;
   double roh2, rhh2;
+  // temperature groups (temperature, src/md.f90:1379-1447): with separate
+  // scaling, group 0 = solute atoms [0, nat_solute), group 1 = the solvent;
+  // otherwise one group.  Degrees of freedom as init_constraints counts them
+  // (src/simprep.f90:2439-2448).
+  int ngroup;
   double ndegf, ndegfree;
+  double gndegf[2], gndegfree[2];
   double* v;
   double* xx;
   double* hist;   // [max_frames][QNB_FRAME_LEN]
-  double* st;     // [8]: sfact, Tfree, Temp, Ekin, Ebond, Eangle
+  double* st;     // [8]: sfact of group 0, Tfree, Temp, Ekin, tfree of groups 0 and 1, sfact of group 1
   int* ctr;       // [4]: frame, error flags
   double* tfree;  // &tfree_all[k]
   int* istep;     // &istep_all[k]
@@ -63,7 +70,7 @@ struct MdDev {
   const int* aitem;
   int nbat;
   // fused integrator: per-block kinetic energy sums
-  double* tpart;  // [ceil((nwat + nat_solute) / 256)][2]
+  double* tpart;  // [ceil((nwat + nat_solute) / 256)][4]: Ekin, free Ekin of groups 0 and 1
   // EVB off-diagonal elements sampled with the energies
   int noffd;
   int offd_k[QNB_MAX_OFFD], offd_l[QNB_MAX_OFFD];  // atom indices
@@ -214,11 +221,14 @@ __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sy
   const int nblk = (nthr + 255) / 256;
   if ((int)blockIdx.x >= nblk) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const double s = M.st[0], dt = M.dt;
+  const double dt = M.dt;
   const auto xg = gpm(S.x);
   const auto vg = gpm(M.v);
   const auto dg = gp(S.d);
+  // waters belong to the solvent group, solute atoms to group 0
   double ek = 0.0, ekf = 0.0;
+  const bool solvent = t < S.nwat;
+  const double s = (solvent && M.ngroup == 2) ? M.st[6] : M.st[0];
   if (t < S.nwat) {
     const int o = S.nat_solute + 3 * t;
     double xo[3][3], xn[3][3], wi[3];
@@ -286,11 +296,39 @@ __global__ __launch_bounds__(256) void k_integrate(const DevSys* __restrict__ sy
     if (!ldg(S.excl + i)) ekf += e;
   }
   __shared__ double red[4];
-  ek = block_sum(ek, red);
-  ekf = block_sum(ekf, red);
+  const bool g1 = solvent && M.ngroup == 2;
+  const double ek0 = block_sum(g1 ? 0.0 : ek, red);
+  const double ekf0 = block_sum(g1 ? 0.0 : ekf, red);
+  const double ek1 = block_sum(g1 ? ek : 0.0, red);
+  const double ekf1 = block_sum(g1 ? ekf : 0.0, red);
   if (threadIdx.x == 0) {
-    M.tpart[2 * blockIdx.x] = ek;
-    M.tpart[2 * blockIdx.x + 1] = ekf;
+    M.tpart[4 * blockIdx.x] = ek0;
+    M.tpart[4 * blockIdx.x + 1] = ekf0;
+    M.tpart[4 * blockIdx.x + 2] = ek1;
+    M.tpart[4 * blockIdx.x + 3] = ekf1;
+  }
+}
+
+// Temperatures and Berendsen factors from the kinetic energies of the
+// temperature groups (temperature, src/md.f90:1412-1443): Temp and Tfree over
+// all atoms, each group's free temperature, and sfact = sqrt(1 + dt/tau
+// (T0/tfree - 1)); a group with tfree = 0 keeps its previous sfact inside the
+// square root, as the reference's update does.
+__device__ void thermo_update(const MdDev& M, const double* ek, const double* ekf) {
+  const double eks = M.ngroup == 2 ? ek[0] + ek[1] : ek[0];
+  const double ekfs = M.ngroup == 2 ? ekf[0] + ekf[1] : ekf[0];
+  const double T = 2.0 * eks / M.Boltz / M.ndegf;
+  const double Tf = 2.0 * ekfs / M.Boltz / M.ndegfree;
+  M.st[1] = Tf;
+  M.st[2] = T;
+  M.st[3] = eks;
+  *M.tfree = Tf;
+  for (int g = 0; g < M.ngroup; g++) {
+    const double tf = 2.0 * ekf[g] / M.Boltz / M.gndegfree[g];
+    double sf = M.st[g == 0 ? 0 : 6];
+    if (tf != 0.0) sf = M.T0 / tf - 1.0;
+    M.st[g == 0 ? 0 : 6] = sqrt(1.0 + M.dt_over_tau * sf);
+    M.st[4 + g] = tf;
   }
 }
 
@@ -303,21 +341,15 @@ __global__ __launch_bounds__(64) void k_thermo(const DevSys* __restrict__ system
   const MdDev& M = mds[blockIdx.x];
   if (threadIdx.x != 0) return;
   const int nblk = (S.nwat + S.nat_solute + 255) / 256;
-  double eks = 0.0, ekfs = 0.0;
+  double ek[2] = {0.0, 0.0}, ekf[2] = {0.0, 0.0};
   for (int k = 0; k < nblk; k++) {
-    eks += M.tpart[2 * k];
-    ekfs += M.tpart[2 * k + 1];
+    ek[0] += M.tpart[4 * k];
+    ekf[0] += M.tpart[4 * k + 1];
+    ek[1] += M.tpart[4 * k + 2];
+    ekf[1] += M.tpart[4 * k + 3];
   }
-  const double T = 2.0 * eks / M.Boltz / M.ndegf;
-  const double Tf = 2.0 * ekfs / M.Boltz / M.ndegfree;
-  double sf = 0.0;
-  if (Tf != 0.0) sf = M.T0 / Tf - 1.0;
-  sf = sqrt(1.0 + M.dt_over_tau * sf);
-  M.st[0] = sf;
-  M.st[1] = Tf;
-  M.st[2] = T;
-  M.st[3] = eks;
-  *M.tfree = Tf;
+  thermo_update(M, ek, ekf);
+  const double T = M.st[2];
   if (!isfinite(T) || !isfinite(S.ered[E_PP_EL] + S.ered[E_WW_EL])) atomicOr(&M.ctr[1], ERR_NONFINITE);
   if (*S.need > S.cap) atomicOr(&M.ctr[1], ERR_OVERFLOW);
   *M.istep = *M.istep + 1;
@@ -398,27 +430,23 @@ __global__ __launch_bounds__(256) void k_temperature(const DevSys* __restrict__ 
   const DevSys& S = systems[blockIdx.x];
   const MdDev& M = mds[blockIdx.x];
   __shared__ double red[4];
-  double ek = 0.0, ekf = 0.0;
+  double e0 = 0.0, ef0 = 0.0, e1 = 0.0, ef1 = 0.0;
   for (int i = threadIdx.x; i < S.natom; i += blockDim.x) {
     const double m = 1.0 / M.winv[i];
     const double v2 = M.v[3 * i] * M.v[3 * i] + M.v[3 * i + 1] * M.v[3 * i + 1] + M.v[3 * i + 2] * M.v[3 * i + 2];
     const double e = 0.5 * m * v2;
-    ek += e;
-    if (!S.excl[i]) ekf += e;
+    const bool g1 = M.ngroup == 2 && i >= S.nat_solute;
+    (g1 ? e1 : e0) += e;
+    if (!S.excl[i]) (g1 ? ef1 : ef0) += e;
   }
-  ek = block_sum(ek, red);
-  ekf = block_sum(ekf, red);
+  double ek[2], ekf[2];
+  ek[0] = block_sum(e0, red);
+  ekf[0] = block_sum(ef0, red);
+  ek[1] = block_sum(e1, red);
+  ekf[1] = block_sum(ef1, red);
   if (threadIdx.x == 0) {
-    const double T = 2.0 * ek / M.Boltz / M.ndegf;
-    const double Tf = 2.0 * ekf / M.Boltz / M.ndegfree;
-    double sf = 0.0;
-    if (Tf != 0.0) sf = M.T0 / Tf - 1.0;
-    sf = sqrt(1.0 + M.dt_over_tau * sf);
-    M.st[0] = sf;
-    M.st[1] = Tf;
-    M.st[2] = T;
-    M.st[3] = ek;
-    *M.tfree = Tf;
+    thermo_update(M, ek, ekf);
+    const double T = M.st[2];
     if (!isfinite(T) || !isfinite(S.ered[E_PP_EL] + S.ered[E_WW_EL])) atomicOr(&M.ctr[1], ERR_NONFINITE);
     if (*S.need > S.cap) atomicOr(&M.ctr[1], ERR_OVERFLOW);
   }
@@ -517,8 +545,14 @@ struct qnb_batch {
 // launchers shared by the step, the minimiser, the timer and the autotuner
 static void launch_lists(qnb_batch* b, int wpb) {
   dim3 g((b->maxg + wpb - 1) / wpb, b->n), t(64 * wpb);
-  bool any = false;
-  for (const DevSys& S : b->hs) any = any || S.any_atom;
+  bool any = false, quirk = false;
+  int maxsg = 0;
+  for (const DevSys& S : b->hs) {
+    any = any || S.any_atom;
+    quirk = quirk || pw_quirk(S);
+    maxsg = std::max(maxsg, S.ncgp_solute);
+  }
+  if (quirk) hipLaunchKernelGGL(k_pw_first, dim3((maxsg + 3) / 4, b->n), dim3(256), 0, b->stream, b->dsys);
   if (b->prec == QNB_PREC_MIXED) {
     hipLaunchKernelGGL(k_pack_xq, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys);
     if (any) hipLaunchKernelGGL((k_lists_lrf<float, true>), g, t, 0, b->stream, b->dsys);
@@ -595,6 +629,13 @@ static void enqueue_forces(qnb_batch* b, bool energy = true, bool atom_terms = t
   if (b->anypol) launch_watpol(b->stream, (b->maxw + WP_THREADS - 1) / WP_THREADS, B, b->dsys, (const int*)b->istep);
 }
 
+// leap-frog + SHAKE + new velocities, then the temperatures and Berendsen
+// factors of the next step
+static void launch_integrate(qnb_batch* b) {
+  hipLaunchKernelGGL(k_integrate, dim3((b->maxsw + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys, b->dmd);
+  hipLaunchKernelGGL(k_thermo, dim3(b->n), dim3(64), 0, b->stream, b->dsys, b->dmd);
+}
+
 template <typename R>
 static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
   const int B = b->n;
@@ -619,8 +660,7 @@ static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
   if (b->anysnap)
     hipLaunchKernelGGL(k_snap, dim3((3 * b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   if (energy || b->anysnap) hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
-  hipLaunchKernelGGL(k_integrate, dim3((b->maxsw + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
-  hipLaunchKernelGGL(k_thermo, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
+  launch_integrate(b);
 }
 
 template <typename R>
@@ -719,7 +759,7 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     memset(&M, 0, sizeof M);
     const DevSys& S = b->hs[k];
     M.dt = 0.020462 * m->dt_fs;
-    M.dt_over_tau = m->dt_fs / m->tau_T_fs;
+    M.dt_over_tau = M.dt / (0.020462 * m->tau_T_fs);  // dt / tau_T in internal units (src/md.f90:207, 231)
     M.T0 = m->T0;
     M.Boltz = par[k]->Boltz;
     M.nbcycle = m->nbcycle;
@@ -729,12 +769,30 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     M.nangles = m->nangles;
     M.roh2 = m->water_roh * m->water_roh;
     M.rhh2 = m->water_rhh * m->water_rhh;
-    const double nconstr = 3.0 * S.nwat;
-    M.ndegf = 3.0 * S.natom - nconstr;
-    int nexcl = 0;
-    std::vector<uint8_t> ex(S.natom);
-    for (int i = 0; i < S.natom; i++) nexcl += sys[k]->excl[i] != 0;
-    M.ndegfree = M.ndegf - 3.0 * nexcl;
+    // degrees of freedom (init_constraints, src/simprep.f90:2194-2250,
+    // 2439-2448): three constraints per rigid water; a constraint counts half
+    // for each excluded atom it joins (exclshk), which for a water adds its
+    // number of excluded atoms
+    M.ngroup = (m->separate_scaling && S.nat_solute > 0 && S.nat_solute < S.natom) ? 2 : 1;
+    {
+      int nex[2] = {0, 0};
+      double exsh[2] = {0.0, 0.0};
+      for (int i = 0; i < S.natom; i++) {
+        const int g = (M.ngroup == 2 && i >= S.nat_solute) ? 1 : 0;
+        if (sys[k]->excl[i]) {
+          nex[g]++;
+          if (i >= S.nat_solute) exsh[g] += 1.0;
+        }
+      }
+      const int nat_g[2] = {M.ngroup == 2 ? S.nat_solute : S.natom, S.natom - S.nat_solute};
+      const int shk_g[2] = {M.ngroup == 2 ? 0 : 3 * S.nwat, 3 * S.nwat};
+      for (int g = 0; g < M.ngroup; g++) {
+        M.gndegf[g] = 3.0 * nat_g[g] - shk_g[g];
+        M.gndegfree[g] = (double)(long long)(M.gndegf[g] - 3.0 * nex[g] + exsh[g]);
+      }
+      M.ndegf = 3.0 * S.natom - 3.0 * S.nwat;
+      M.ndegfree = (double)(long long)(M.ndegf - 3.0 * (nex[0] + nex[1]) + (exsh[0] + exsh[1]));
+    }
     if ((rc = b->mem.upload(&M.bond_ij, m->bond_ij, 2 * (size_t)m->nbonds)) ||
         (rc = b->mem.upload(&M.bond_r0, m->bond_r0, m->nbonds)) || (rc = b->mem.upload(&M.bond_k, m->bond_k, m->nbonds)) ||
         (rc = b->mem.upload(&M.angle_ijk, m->angle_ijk, 3 * (size_t)m->nangles)) ||
@@ -777,7 +835,7 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
         return rc;
       b->nbblk = std::max(b->nbblk, (nbat + 255) / 256);
       const int nib = (S.nwat + S.nat_solute + 255) / 256;
-      if ((rc = b->mem.alloc(&M.tpart, 2 * (size_t)std::max(nib, 1)))) return rc;
+      if ((rc = b->mem.alloc(&M.tpart, 4 * (size_t)std::max(nib, 1)))) return rc;
     }
     HIPCHK(hipMemcpy(M.v, v0[k], sizeof(double) * 3 * S.natom, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(M.st, 0, 8 * sizeof(double)));
@@ -910,6 +968,36 @@ extern "C" int32_t qnb_batch_run(qnb_batch* b, int32_t nsteps, int32_t use_graph
     done++;
   }
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_integrate(qnb_batch* b, const double* const* d) {
+  if (!b || !d) return fail(QNB_EINVAL, "null batch or gradients");
+  for (int k = 0; k < b->n; k++)
+    if (!d[k]) return fail(QNB_EINVAL, "instance %d: null gradient", k);
+  for (int k = 0; k < b->n; k++)
+    HIPCHK(hipMemcpyAsync(b->hs[k].d, d[k], sizeof(double) * 3 * b->hs[k].natom, hipMemcpyHostToDevice, b->stream));
+  launch_integrate(b);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->steps_done++;
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_thermostat(qnb_batch* b, int32_t k, double* out) {
+  if (!b || k < 0 || k >= b->n || !out) return fail(QNB_EINVAL, "bad instance or output");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  double st[8];
+  HIPCHK(hipMemcpy(st, b->hm[k].st, sizeof st, hipMemcpyDeviceToHost));
+  const int ng = b->hm[k].ngroup;
+  out[0] = st[2];
+  out[1] = st[1];
+  out[2] = st[3];
+  out[3] = st[4];
+  out[4] = ng == 2 ? st[5] : 0.0;
+  out[5] = st[0];
+  out[6] = ng == 2 ? st[6] : 0.0;
+  out[7] = ng;
   return 0;
 }
 

@@ -107,6 +107,11 @@ struct DevSys {
   double* lrf;
   int* nbr;
   int* nnbr;
+  // reference test-point reuse of the LRF-off switch-atom solute-water list
+  // (quirk bit 0 with use_lrf = 0): per solute group, the first water within
+  // the cut-off of the switch atom (nwat when none); later waters are tested
+  // against atom glast + 1
+  int* pw_first;
   int cap;
   int* need;        // [1] max neighbours seen (overflow detection)
   uint8_t* qw_in;   // [nwat] water on the Q-water list
@@ -200,6 +205,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// the LRF-off switch-atom solute-water list takes nbpwlist's test-point reuse
+__host__ __device__ __forceinline__ bool pw_quirk(const DevSys& S) {
+  return (S.quirk & 1) && !S.use_lrf && !S.any_atom && S.nwat > 0 && S.ncgp_solute > 0;
+}
 
 // q_dist4 (math.f90:254-262): |b - a|^2, ((dx^2 + dy^2) + dz^2), no contraction
 __device__ __forceinline__ double dist4(const double* __restrict__ x, int a, int b) {

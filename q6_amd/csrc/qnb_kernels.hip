@@ -47,6 +47,37 @@ __device__ __forceinline__ bool any_atom_pair(const DevSys& S, const double* __r
   return false;
 }
 
+// First water (index order) within the cut-off of each solute group's switch
+// atom, for the LRF-off switch-atom solute-water list: nbpwlist
+// (nonbondene.f90:2665-2691) reuses its switch-atom variable `ia` as the loop
+// index over the group's atoms, so once a water is inside, the distance test
+// of every later water is made from atom cgp(ig)%last + 1.  One wavefront per
+// solute group; excluded groups and waters are skipped as the reference does.
+__global__ __launch_bounds__(256) void k_pw_first(const DevSys* __restrict__ systems) {
+  const DevSys& S = systems[blockIdx.y];
+  const int g = blockIdx.x * (blockDim.x / QNB_WAVE) + threadIdx.x / QNB_WAVE;
+  if (g >= S.ncgp_solute || !pw_quirk(S)) return;
+  const int lane = threadIdx.x % QNB_WAVE;
+  const int ia = ldg(S.gswitch + g);
+  int first = S.nwat;
+  if (!ldg(S.excl + ia)) {
+    for (int w0 = 0; w0 < S.nwat; w0 += QNB_WAVE) {
+      const int w = w0 + lane;
+      bool in = false;
+      if (w < S.nwat) {
+        const int ja = S.nat_solute + S.sa * w;
+        in = !ldg(S.excl + ja) && dist4(S.x, ia, ja) <= S.rcpw2;
+      }
+      const uint64_t m = __ballot(in);
+      if (m) {
+        first = w0 + __ffsll((long long)m) - 1;
+        break;
+      }
+    }
+  }
+  if (lane == 0) S.pw_first[g] = first;
+}
+
 // the owner side of a charge-group pair (a != b) under the parity rule
 __device__ __forceinline__ bool owns(int a, int b) {
   return (a < b && ((a + b) & 1) == 0) || (a > b && ((a + b) & 1) == 1);
@@ -251,7 +282,14 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
           bool inside;
           if (ANY && S.any_atom && (a_sol || b_sol))
             inside = any_atom_pair(S, x, a, b, a_sol, b_sol, ra, rb, cut2, r2);
-          else {
+          else if (a_sol != b_sol && pw_quirk(S)) {
+            // nbpwlist's test point: the switch atom up to the group's first
+            // water inside, atom glast + 1 after it (LRF off: r2 unused)
+            const int g = a_sol ? a : b, w = (a_sol ? b : a) - S.ncgp_solute;
+            const int wf = ldg(S.pw_first + g);
+            r2 = 0.0;
+            inside = w == wf || (w > wf && dist4(x, ldg(S.glast + g) + 1, a_sol ? rb : ra) <= cut2);
+          } else {
             r2 = dist4(x, ra, rb);
             inside = r2 <= cut2;
           }

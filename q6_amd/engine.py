@@ -135,3 +135,48 @@ class NonbondedEngine:
         out = np.zeros((nf, self.nst, nq, abi.QNB_ADJ_NV))
         abi.check(self.lib, self.lib.qnb_adjoint(self.h, C.byref(tangent), nf, _pd(fr), _pd(out)), "qnb_adjoint")
         return out
+
+
+def pot_energy_summary(E, EQ, lambdas, bonded=None, EQ_bonded=None, EQ_restraint=None):
+    """The energy summary of pot_energy (src/potene.f90:224-251) in its
+    order of operations.
+
+    E: class energies of the nonbonded engine (pp/pw/ww el and vdw, lrf,
+    solvent_radial, water_pol); EQ[s]: per-state qq/qp/qw el and vdw (qp
+    includes the Q-neighbour pairs); lambdas[s].  The terms the engine does
+    not compute come from the caller: bonded = dict of p_bond, w_bond,
+    p_angle, w_angle, p_torsion, p_improper, restraint_fix,
+    restraint_shell and restraint_protein (the position/distance restraint
+    sum p_restrain leaves before the states' restraints are added); EQ_bonded[s] = (bond, angle, torsion, improper) of the
+    Q-atom bonded terms; EQ_restraint[s] = the state's restraint energy.
+    Returns (Esum, EQsum): Esum holds q_bond..q_improper, qx_el, qx_vdw,
+    restraint_protein, restraint_total and potential; EQsum[s] holds qx_el,
+    qx_vdw and total."""
+    bonded = bonded or {}
+    ns = len(EQ)
+    qb = [(0.0, 0.0, 0.0, 0.0)] * ns if EQ_bonded is None else [tuple(map(float, b)) for b in EQ_bonded]
+    qr = [0.0] * ns if EQ_restraint is None else [float(r) for r in EQ_restraint]
+    out = dict(q_bond=0.0, q_angle=0.0, q_torsion=0.0, q_improper=0.0, qx_el=0.0, qx_vdw=0.0,
+               restraint_protein=float(bonded.get("restraint_protein", 0.0)))
+    eqs = []
+    for s in range(ns):
+        q, lam = EQ[s], float(lambdas[s])
+        qx_el = q["qq_el"] + q["qp_el"] + q["qw_el"]
+        qx_vdw = q["qq_vdw"] + q["qp_vdw"] + q["qw_vdw"]
+        total = qb[s][0] + qb[s][1] + qb[s][2] + qb[s][3] + qx_el + qx_vdw + qr[s]
+        eqs.append(dict(qx_el=qx_el, qx_vdw=qx_vdw, total=total))
+        out["q_bond"] = out["q_bond"] + qb[s][0] * lam
+        out["q_angle"] = out["q_angle"] + qb[s][1] * lam
+        out["q_torsion"] = out["q_torsion"] + qb[s][2] * lam
+        out["q_improper"] = out["q_improper"] + qb[s][3] * lam
+        out["qx_el"] = out["qx_el"] + qx_el * lam
+        out["qx_vdw"] = out["qx_vdw"] + qx_vdw * lam
+        out["restraint_protein"] = out["restraint_protein"] + qr[s] * lam
+    g = lambda k: float(bonded.get(k, 0.0))  # noqa: E731
+    out["restraint_total"] = (g("restraint_fix") + g("restraint_shell") + out["restraint_protein"]
+                              + E["solvent_radial"] + E["water_pol"])
+    out["potential"] = (g("p_bond") + g("w_bond") + g("p_angle") + g("w_angle") + g("p_torsion")
+                        + g("p_improper") + E["pp_el"] + E["pp_vdw"] + E["pw_el"] + E["pw_vdw"] + E["ww_el"]
+                        + E["ww_vdw"] + out["q_bond"] + out["q_angle"] + out["q_torsion"] + out["q_improper"]
+                        + out["qx_el"] + out["qx_vdw"] + out["restraint_total"] + E["lrf"])
+    return out, eqs

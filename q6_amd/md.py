@@ -38,10 +38,14 @@ def scaas_cstb(s, lam):
 
 class BatchMD:
     def __init__(self, systems, lambdas, velocities, precision=abi.QNB_PREC_FP64, dt_fs=1.0, T0=300.0,
-                 tau_T_fs=100.0, nbcycle=10, ene_every=10, max_frames=1000, positions=None):
+                 tau_T_fs=100.0, nbcycle=10, ene_every=10, max_frames=1000, positions=None,
+                 separate_scaling=True, water_geometry=None):
         """systems[k]: QSystem of instance k (repeat the same object for the
         windows of one system: they share its device topology); lambdas[k]:
-        state weights; velocities[k]: (natom, 3) initial velocities."""
+        state weights; velocities[k]: (natom, 3) initial velocities.
+        separate_scaling: solute and solvent as separate Berendsen groups
+        (Qdyn6's default); water_geometry: (r_OH, r_HH) of the SHAKE
+        constraints when they differ from the system's."""
         self.lib = abi.load()
         n = len(systems)
         self.n = n
@@ -74,9 +78,10 @@ class BatchMD:
                         angle_t0=np.full(len(angles), md["angle_t0"]), angle_k=np.full(len(angles), md["angle_k"]),
                         winv=1.0 / np.asarray(md["mass"], float))
             arrs = {a: np.ascontiguousarray(v) for a, v in arrs.items()}
+            roh, rhh = water_geometry if water_geometry is not None else (md["water_roh"], md["water_rhh"])
             m = abi.QnbMdParams(dt_fs=dt_fs, T0=T0, tau_T_fs=tau_T_fs, nbcycle=nbcycle, ene_every=ene_every,
                                 max_frames=max_frames, nbonds=len(bonds), nangles=len(angles),
-                                water_roh=md["water_roh"], water_rhh=md["water_rhh"])
+                                water_roh=roh, water_rhh=rhh, separate_scaling=int(bool(separate_scaling)))
             m.bond_ij, m.angle_ijk = _pi(arrs["bond_ij"]), _pi(arrs["angle_ijk"])
             for a in ("bond_r0", "bond_k", "angle_t0", "angle_k", "winv"):
                 setattr(m, a, _pd(arrs[a]))
@@ -104,6 +109,23 @@ class BatchMD:
 
     def run(self, nsteps, graph=True):
         abi.check(self.lib, self.lib.qnb_batch_run(self.h, nsteps, int(graph)), "qnb_batch_run")
+
+    def integrate(self, grads):
+        """One leap-frog + SHAKE + temperature step of every instance from the
+        given gradients (grads[k]: (natom, 3))."""
+        gs = [np.ascontiguousarray(g, np.float64) for g in grads]
+        arr = (C.c_void_p * self.n)(*[g.ctypes.data for g in gs])
+        abi.check(self.lib, self.lib.qnb_batch_integrate(self.h, arr), "qnb_batch_integrate")
+
+    def thermostat(self, k):
+        """Instance k's thermostat state after the last step: dict with Temp,
+        Tfree, Ekin and per temperature group the free temperature `tfree` and
+        the Berendsen factor `sfact` of the next step."""
+        out = np.zeros(8)
+        abi.check(self.lib, self.lib.qnb_batch_thermostat(self.h, k, _pd(out)), "qnb_batch_thermostat")
+        ng = int(out[7])
+        return {"Temp": out[0], "Tfree": out[1], "Ekin": out[2], "ngroups": ng,
+                "tfree": out[3:3 + ng].copy(), "sfact": out[5:5 + ng].copy()}
 
     def prepare_graph(self):
         """Capture the list-cycle graph now (no step is taken)."""

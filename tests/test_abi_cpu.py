@@ -25,7 +25,7 @@ def test_library_exports_every_symbol():
 
 def test_abi_version_and_device_count_without_gpu():
     lib = abi.load()
-    assert lib.qnb_abi_version() == 2
+    assert lib.qnb_abi_version() == abi.ABI_VERSION
     assert lib.qnb_device_count() >= 0
 
 

@@ -38,6 +38,6 @@ def test_fortran_binding_links_and_matches_c_layout():
     subprocess.check_call(["amdflang", "-I", d, os.path.join(d, "t.f90"), os.path.join(d, "qnb_bind.o"),
                            "-L" + lib, "-lq6nb", "-Wl,-rpath," + lib, "-o", os.path.join(d, "t")])
     out = subprocess.check_output([os.path.join(d, "t")]).decode().split()
-    assert [int(v) for v in out] == [2, C.sizeof(abi.QnbSystem), C.sizeof(abi.QnbParams), C.sizeof(abi.QnbEnergies),
+    assert [int(v) for v in out] == [abi.ABI_VERSION, C.sizeof(abi.QnbSystem), C.sizeof(abi.QnbParams), C.sizeof(abi.QnbEnergies),
                                      C.sizeof(abi.QnbQEnergies), C.sizeof(abi.QnbQTangent),
                                      C.sizeof(abi.QnbQbonded), 1]

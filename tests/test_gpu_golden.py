@@ -102,20 +102,41 @@ def test_full_nonbonded_call(gcase):
         assert abs(E[t + "_vdw"] - s.ref["E_" + t][1]) <= REL * max(abs(s.ref["E_" + t][1]), 1.0)
 
 
-def test_nolrf_lists_against_oracle():
-    """LRF off: cut-off-only builders.  The engine follows the switch-atom
-    distance test; the oracle without its reference-quirk flag does the same."""
-    print("\n[gpu] lig_fep2_nolrf", flush=True)
+@pytest.mark.parametrize("quirks", [1, 0])
+def test_nolrf_lists(quirks):
+    """LRF off, switch atoms (nbpplist, nbpwlist, nbwwlist, nbqplist).  With
+    the reference's own mode (ref_quirks bit 0: nbpwlist's distance test
+    moves to atom cgp(ig)%last + 1 after a group's first water inside,
+    nonbondene.f90:2686-2691) every list equals stock Qdyn6's, in order, and
+    every term is within 1e-10 of its dump.  Without it the solute-water list
+    follows the switch-atom test for every water; that mode is checked
+    against the oracle in the same mode."""
+    print(f"\n[gpu] lig_fep2_nolrf ref_quirks={quirks}", flush=True)
     s = from_golden(golden_path("lig_fep2_nolrf"))
-    eng = engine_for(s)
+    eng = engine_for(s, ref_quirks=quirks)
     eng.make_pair_lists(s.x)
-    o = Oracle(s, ref_quirks=0)
+    o = Oracle(s, ref_quirks=quirks)
     o.make_pair_lists(s.x)
     for which in (abi.LIST_PP, abi.LIST_PW, abi.LIST_WW, abi.LIST_QP, abi.LIST_QW):
         assert np.array_equal(eng.pairs(which), o.pairs(which)), which
-    d, E, EQ = eng.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree)
-    do, Eo, EQo = o.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree)
-    assert np.max(np.abs(d - do)) <= REL * np.max(np.abs(do))
+    if quirks:
+        for which, key in ((abi.LIST_PP, "nbpp_ij"), (abi.LIST_PW, "nbpw_ij"), (abi.LIST_QP, "nbqp_ij")):
+            assert np.array_equal(eng.pairs(which), s.ref[key].T - 1), key
+        assert np.array_equal(eng.pairs(abi.LIST_WW), s.ref["ww_mol_pairs"].T)
+        assert len(s.ref["nbpw_ij"].T) == 51072
+        for term, mask in TERMS:
+            if term == "lrf":
+                continue
+            d, E, EQ = eng.eval_terms(mask, s.x, Tfree=s.Tfree, istep=0)
+            dref, eref = s.ref["d_" + term].T, np.atleast_1d(s.ref["E_" + term])
+            e = np.array(term_energy(term, E, EQ))
+            assert np.max(np.abs(e - eref)) <= REL * max(np.max(np.abs(eref)), 1e-12), (term, e, eref)
+            assert np.max(np.abs(d - dref)) <= REL * max(np.max(np.abs(dref)), 1e-12), term
+    else:
+        assert len(eng.pairs(abi.LIST_PW)) == 60201
+        d, E, EQ = eng.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree)
+        do, Eo, EQo = o.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree)
+        assert np.max(np.abs(d - do)) <= REL * np.max(np.abs(do))
     eng.close()
 
 
@@ -191,3 +212,50 @@ def test_mixed_precision_error_budget(case):
                   flush=True)
             assert e <= 16 * ulp * S, (t, suf, E[t + suf], ref, S)
     eng.close()
+
+
+@pytest.mark.parametrize("case", ["lig_eq1", "lig_fep2"])
+def test_watpol_sequence(case):
+    """The polarisation restraint over istep = 1..7 with itdis = 3 from a zero
+    shell state (the theta_corr feedback of src/nonbondene.f90:6653-6666 runs
+    at istep 3 and 6): energy, gradient and shell state of every call within
+    1e-10 of stock Qdyn6's."""
+    from conftest import watpol_frames
+    print(f"\n[gpu] watpol sequence {case}", flush=True)
+    s = from_golden(golden_path(case))
+    eng = engine_for(s, itdis=float(s.ref["wp_itdis"]))
+    eng.make_pair_lists(s.x)
+    nsh = int(s.par["nwpolr_shell"])
+    eng.set_shell_state(np.zeros(3 * nsh))
+    for k in range(1, 8):
+        x = watpol_frames(s, k)
+        d, E, _ = eng.eval_terms(abi.T_POL, x, Tfree=s.Tfree, istep=k)
+        eref, dref = float(s.ref[f"E_wp{k}"]), s.ref[f"d_wp{k}"].T
+        assert abs(E["water_pol"] - eref) <= REL * abs(eref), (k, E["water_pol"], eref)
+        assert np.max(np.abs(d - dref)) <= REL * np.max(np.abs(dref)), k
+        st, sref = eng.shell_state(), s.ref[f"wp{k}_shell"].reshape(-1)
+        assert np.max(np.abs(st - sref)) <= REL * np.max(np.abs(sref)), (k, st, sref)
+    eng.close()
+
+
+@pytest.mark.parametrize("case", ["lig_fep2", "lig_fep2_arith"])
+def test_pot_energy_summary(case):
+    """The engine's class energies of one full evaluation, summed by
+    pot_energy_summary with the caller's bonded terms, give Qdyn6's E%qx,
+    EQ(s)%total and E%potential within 1e-10 (src/potene.f90:224-251)."""
+    from conftest import golden_summary_inputs
+    from q6_amd.engine import pot_energy_summary
+    print(f"\n[gpu] pot_energy summary {case}", flush=True)
+    s = from_golden(golden_path(case))
+    eng = engine_for(s)
+    eng.make_pair_lists(s.x)
+    d, E, EQ = eng.eval_terms(abi.T_ALL, s.x, Tfree=s.Tfree, istep=0)
+    eng.close()
+    bonded, qb, qr, lam = golden_summary_inputs(s)
+    out, eqo = pot_energy_summary(E, EQ, lam, bonded, qb, qr)
+    es, eqs = s.ref["E_summary"], s.ref["EQ_summary"]
+    scale = max(abs(es[0]), np.abs(es[1:]).max())
+    assert abs(out["qx_el"] - es[8]) <= REL * scale and abs(out["qx_vdw"] - es[9]) <= REL * scale
+    for st, q in enumerate(eqs.T):
+        assert abs(eqo[st]["total"] - q[1]) <= REL * scale, st
+    assert abs(out["potential"] - es[0]) <= REL * scale, (out["potential"], es[0])

@@ -131,3 +131,48 @@ def test_offdiag_sampled_with_energies(small):
         assert np.allclose(e0[:, abi.F_OFFD + 2 * m], H, rtol=1e-13, atol=0)
     assert np.all(e0[:, abi.F_OFFD + 4:] == 0.0)
     assert np.all(e1[:, abi.F_OFFD:] == 0.0)
+
+
+def golden_md_system(case):
+    """A golden system with the MD data the batch needs: masses from the
+    topology types; no bonds or angles of its own (the step under test takes
+    the full gradient pot_energy left, d_total)."""
+    from conftest import golden_path
+    from q6_amd.system import from_golden
+    s = from_golden(golden_path(case))
+    s.md = dict(mass=s.ref["type_mass"][s.a["iac"]], bonds=np.zeros((0, 2), np.int32), bond_r0=1.0, bond_k=0.0,
+                angles=np.zeros((0, 3), np.int32), angle_t0=0.0, angle_k=0.0,
+                water_roh=float(np.sqrt(s.ref["const_dist2"][0])), water_rhh=float(np.sqrt(s.ref["const_dist2"][2])))
+    return s
+
+
+@pytest.mark.parametrize("case", ["lig_eq1", "lig_fep2"])
+def test_one_md_step_matches_qdyn6(case):
+    """One step of md_run after pot_energy (src/md.f90:1774-1797) on the
+    device from Qdyn6's own start and gradient: leap-frog with the solute
+    and solvent Berendsen factors, SHAKE of the waters, v = (x - x_old)/dt
+    and temperature().  Positions, velocities, Temp, Tfree, the kinetic
+    energy and each group's tfree and next factor within 1e-10 of stock
+    Qdyn6's step (the initial factors too)."""
+    from q6_amd.md import BatchMD
+    print(f"\n[gpu] one MD step vs Qdyn6 {case}", flush=True)
+    s = golden_md_system(case)
+    assert int(s.ref["ntgroups"]) == 2
+    dt_fs = float(s.ref["dt"]) / 0.020462
+    b = BatchMD([s], [np.asarray(s.par["lambda"])], [s.ref["v"].T], dt_fs=dt_fs, T0=float(s.ref["Temp0"]),
+                tau_T_fs=float(s.ref["tau_T"]) / 0.020462, nbcycle=10, ene_every=0, max_frames=2)
+    th0 = b.thermostat(0)
+    assert th0["ngroups"] == 2
+    assert np.max(np.abs(th0["sfact"] - s.ref["md1_sfact_in"])) <= 1e-12
+    b.integrate([s.ref["d_total"].T])
+    b.sync()
+    x, v = b.state(0)
+    th = b.thermostat(0)
+    b.close()
+    xr, vr = s.ref["md1_x"].T, s.ref["md1_v"].T
+    assert np.max(np.abs(x - xr)) <= 1e-10 * np.max(np.abs(xr))
+    assert np.max(np.abs(v - vr)) <= 1e-10 * np.max(np.abs(vr)), np.max(np.abs(v - vr))
+    t = s.ref["md1_temp"]
+    got = [th["Temp"], th["Tfree"], th["Ekin"], th["tfree"][0], th["sfact"][0], th["tfree"][1], th["sfact"][1]]
+    ref = [t[0], t[1], t[2], t[4], t[5], t[7], t[8]]
+    assert np.allclose(got, ref, rtol=1e-10, atol=0), (got, ref)

@@ -80,3 +80,47 @@ def test_terms_exact(case, term, mask):
     d, E, EQ = o.eval_terms(mask, s.x, Tfree=s.Tfree, istep=0)
     assert np.array_equal(d, s.ref["d_" + term].T)
     assert np.array_equal(np.array(term_energy(term, E, EQ)), np.atleast_1d(s.ref["E_" + term]))
+
+
+def test_watpol_sequence_exact(case):
+    """watpol over istep = 1..7 with itdis = 3 from a zero shell state, the
+    waters moving between calls: energies, gradients and the shell state
+    (theta_corr feedback at istep 3 and 6, running averages) equal stock
+    Qdyn6's bit for bit (src/nonbondene.f90:6548-6746)."""
+    from conftest import watpol_frames
+    name, s, _ = case
+    if "E_wp1" not in s.ref:
+        pytest.skip("no polarisation restraint in this case")
+    o = Oracle(s, ref_quirks=1, itdis=float(s.ref["wp_itdis"]))
+    nsh = int(s.par["nwpolr_shell"])
+    o.set_shell_state(np.zeros(3 * nsh))
+    corr_moved = False
+    for k in range(1, 8):
+        x = watpol_frames(s, k)
+        d, E, _ = o.eval_terms(abi.T_POL, x, Tfree=s.Tfree, istep=k)
+        assert E["water_pol"] == float(s.ref[f"E_wp{k}"]), k
+        assert np.array_equal(d, s.ref[f"d_wp{k}"].T), k
+        st = o.shell_state()
+        assert np.array_equal(st, s.ref[f"wp{k}_shell"].reshape(-1)), k
+        corr_moved = corr_moved or np.any(st[:nsh] != 0.0)
+    assert corr_moved  # the feedback branch ran
+
+
+def test_pot_energy_summary_exact(case):
+    """pot_energy's summary (src/potene.f90:224-251) as the engine's host code
+    forms it, from the dumped class energies and the caller's bonded terms,
+    equals Qdyn6's E%qx, EQ(s)%qx, EQ(s)%total and E%potential bit for bit."""
+    from conftest import golden_summary_inputs
+    from q6_amd.engine import pot_energy_summary
+    name, s, _ = case
+    es, eqs = s.ref["E_summary"], s.ref["EQ_summary"]
+    bonded, qb, qr, lam = golden_summary_inputs(s)
+    E = dict(pp_el=es[2], pp_vdw=es[3], pw_el=es[4], pw_vdw=es[5], ww_el=es[6], ww_vdw=es[7], lrf=es[1],
+             solvent_radial=es[10], water_pol=es[11])
+    EQ = [dict(qq_el=q[8], qq_vdw=q[9], qp_el=q[10], qp_vdw=q[11], qw_el=q[12], qw_vdw=q[13]) for q in eqs.T]
+    out, eqo = pot_energy_summary(E, EQ, lam, bonded, qb, qr)
+    assert out["qx_el"] == es[8] and out["qx_vdw"] == es[9]
+    for st, q in enumerate(eqs.T):
+        assert eqo[st]["qx_el"] == q[6] and eqo[st]["qx_vdw"] == q[7] and eqo[st]["total"] == q[1], st
+    assert out["restraint_total"] == s.ref["E_bonded"][13]
+    assert out["potential"] == es[0]
