@@ -653,16 +653,18 @@ def cfg5(seed=500, radii=None):
     (ten-atom bead chains packed as cfg2's, TIP3P filling the sphere), about
     0.1 atoms per A^3 of which 0.3 are solute (about 1,400 to 38,000 atoms),
     each with 10-60 Q-atoms (1-6 chains) in 2-3 states (seeded)."""
+    return [cfg5_member(k, seed, radii) for k in range(len(radii) if radii is not None else 16)]
+
+
+def cfg5_member(k, seed=500, radii=None):
+    """Member k of the uneven batch (the same system cfg5 builds as its k-th)."""
     rng = np.random.default_rng(seed)
     radii = list(radii) if radii is not None else list(range(15, 47, 2))
-    out = []
-    for k, R in enumerate(radii):
-        vol = 4.0 / 3.0 * math.pi * R ** 3
-        nchain = max(2, int(round(0.3 * 0.1 * vol / 10)))
-        nq = int(rng.integers(1, 7))
-        nst = int(rng.integers(2, 4))
-        lam = np.full(nst, 1.0 / nst)
-        out.append(build(float(R), nchain, 10, min(nq, nchain), nst, seed + 10 * k, chain_region=0.55 * R,
-                         q_sigma=0.3, lambdas=lam, compact=3.0, water_fill=float(R),
-                         name=f"cfg5[{k}]: {R} A sphere", **PACK))
-    return out
+    draws = [(int(rng.integers(1, 7)), int(rng.integers(2, 4))) for _ in radii]
+    R = radii[k]
+    nq, nst = draws[k]
+    vol = 4.0 / 3.0 * math.pi * R ** 3
+    nchain = max(2, int(round(0.3 * 0.1 * vol / 10)))
+    lam = np.full(nst, 1.0 / nst)
+    return build(float(R), nchain, 10, min(nq, nchain), nst, seed + 10 * k, chain_region=0.55 * R,
+                 q_sigma=0.3, lambdas=lam, compact=3.0, water_fill=float(R), name=f"cfg5[{k}]: {R} A sphere", **PACK)

@@ -125,7 +125,7 @@ def test_batched_adjoint_gradients_match_fd_on_fp64_path(md_case):
     theta0 = space.vector()
     h = 1e-4
     print("\n[gpu] adjoint fd over", space.npar, "parameters", flush=True)
-    worst = {}
+    worst, fds, ans = {}, {}, {}
     try:
         for p in range(space.npar):
             res = []
@@ -140,14 +140,25 @@ def test_batched_adjoint_gradients_match_fd_on_fp64_path(md_case):
             for key in ("dG", "gap", "G_evb", "pmf"):
                 fd = (np.asarray(res[0][key]) - np.asarray(res[1][key])) / (2 * h)
                 an = g[key][p] if key == "dG" else np.asarray(g[key])[..., p]
-                ok = np.isfinite(fd)
-                err = np.abs(np.asarray(an)[ok] - fd[ok])
-                tol = 1e-4 * np.maximum(np.abs(fd[ok]), 1.0)
-                worst[(key, p)] = float((err / tol).max()) if err.size else 0.0
-                assert np.all(err <= tol), (key, space.names()[p], an, fd)
+                fds.setdefault(key, []).append(np.atleast_1d(fd))
+                ans.setdefault(key, []).append(np.atleast_1d(np.asarray(an, float)))
     finally:
         space.apply(s)
-    print("[gpu] worst |adjoint - fd| / tol:", max(worst.values()), flush=True)
+    # 1e-4 relative: every derivative against its finite difference, with a
+    # floor at 1e-2 of the largest derivative of the same estimator (where the
+    # finite difference itself carries noise of that order), and the whole
+    # gradient of each estimator in norm
+    for key in fds:
+        fd, an = np.stack(fds[key], -1), np.stack(ans[key], -1)
+        ok = np.isfinite(fd)
+        fdk, ank = fd[ok], an[ok]
+        big = np.max(np.abs(fdk))
+        tol = 1e-4 * np.maximum(np.abs(fdk), 1e-2 * big)
+        err = np.abs(ank - fdk)
+        worst[key] = float((err / tol).max())
+        assert np.all(err <= tol), (key, np.argmax(err / tol), ank, fdk)
+        assert np.linalg.norm(ank - fdk) <= 1e-4 * np.linalg.norm(fdk), key
+    print("[gpu] worst |adjoint - fd| / tol per estimator:", worst, flush=True)
 
 
 def test_uneven_batch_matches_single_system_entry():

@@ -186,8 +186,8 @@ def pair_energy_scales(s):
 @pytest.mark.parametrize("case", ["lig_fep2", "lig_fep2_arith"])
 def test_mixed_precision_error_budget(case):
     """FP32 pair arithmetic with FP64 accumulation: the error against the FP64
-    reference is reported and bounded.  Forces: 1e-4 of the largest gradient
-    component.  Energy of a class: each pair energy carries an FP32 relative
+    reference is reported and bounded.  Forces: 3e-6 of the largest gradient
+    component (about ten times the 2.7e-7 observed on lig_fep2).  Energy of a class: each pair energy carries an FP32 relative
     error of a few ulp (2^-23), independent from pair to pair, so the class sum
     errs by a few ulp times S = sqrt(sum of squared pair energies); bound 16
     ulp x S.  (A relative bound on the sum itself is meaningless for classes
@@ -200,7 +200,7 @@ def test_mixed_precision_error_budget(case):
     dref = sum(s.ref["d_" + t].T for t, _ in TERMS)
     err = np.max(np.abs(d - dref)) / np.max(np.abs(dref))
     print(f"\n[gpu] mixed precision max force error {err:.2e}", flush=True)
-    assert err < 1e-4
+    assert err < 3e-6
     ulp = 2.0 ** -23
     scales = pair_energy_scales(s)
     for t in ("ww", "pw", "pp"):
