@@ -477,6 +477,20 @@ enum qnb_adj_field {
  * gives E_s of the frame. */
 int32_t qnb_adjoint(qnb_handle *h, const qnb_qtangent *tan, int32_t nframes, const double *frames, double *out);
 
+/* qnb_adjoint for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments and results as qnb_adjoint;
+ * the tangent tables, a chunk of frames, the kernel's parameter record and
+ * the chunk's output are staged in a fixed device workspace of
+ * QNB_ADJOINT_STATIC_BYTES per GPU (held until the process ends) instead of
+ * allocated memory, so no call allocates or frees.  Returns QNB_ENOMEM, with
+ * the sizes in the message, when the tangent tables plus one frame and its
+ * output do not fit.  Every call on one GPU shares that GPU's workspace:
+ * calls that overlap on one GPU race. */
+#define QNB_ADJOINT_STATIC_BYTES (32u << 20)
+int32_t qnb_adjoint_static(qnb_handle *h, const qnb_qtangent *tan, int32_t nframes, const double *frames,
+                           double *out);
+
 /* Coordinate snapshots during qnb_batch_run: every `every` steps (at the
  * point where the step's energies are sampled) up to max_frames per
  * instance, kept in device memory.  every = 0 stops storing. */

@@ -168,7 +168,7 @@ EXPORTS = [
     "qnb_get_pairs", "qnb_get_lrf", "qnb_get_shell_state", "qnb_set_shell_state",
     "qnb_batch_create", "qnb_batch_destroy", "qnb_batch_run", "qnb_batch_integrate", "qnb_batch_thermostat", "qnb_batch_sync", "qnb_batch_get_state",
     "qnb_batch_energy_history", "qnb_batch_pair_counts", "qnb_batch_time_kernel", "qnb_batch_minimize", "qnb_batch_autotune",
-    "qnb_adjoint", "qnb_batch_store_frames", "qnb_batch_frames", "qnb_batch_adjoint", "qnb_batch_set_offdiag",
+    "qnb_adjoint", "qnb_adjoint_static", "qnb_batch_store_frames", "qnb_batch_frames", "qnb_batch_adjoint", "qnb_batch_set_offdiag",
     "qnb_set_qbonded", "qnb_eval_qbonded", "qnb_batch_set_qbonded",
 ]
 
@@ -206,6 +206,7 @@ def _declare(lib):
     lib.qnb_batch_minimize.argtypes = [vp, C.c_int32, C.c_double]
     lib.qnb_batch_autotune.argtypes = [vp, C.c_int32, _pi]
     lib.qnb_adjoint.argtypes = [vp, C.POINTER(QnbQTangent), C.c_int32, _pd, _pd]
+    lib.qnb_adjoint_static.argtypes = [vp, C.POINTER(QnbQTangent), C.c_int32, _pd, _pd]
     lib.qnb_batch_store_frames.argtypes = [vp, C.c_int32, C.c_int32]
     lib.qnb_batch_set_offdiag.argtypes = [vp, C.c_int32, C.c_int32, _pi, _pd, _pd]
     lib.qnb_batch_frames.argtypes = [vp, C.c_int32, _pd, C.c_int32, C.POINTER(C.c_int32)]

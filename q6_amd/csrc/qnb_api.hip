@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -402,7 +403,35 @@ int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots) {
 
 using namespace qnb;
 
+// ---------------------------------------------------------------------------
+// Fixed device workspace of qnb_adjoint_static (callers that cannot
+// allocate): the tangent tables, a chunk of frames, the kernel's parameter
+// record and the chunk's output are staged here.  Each GPU holds its own copy
+// for the life of the process.  In host code `scratch` only names the symbol:
+// its device address on each GPU comes from hipGetSymbolAddress, looked up
+// once per device when a handle is created there (qnb_create), and only that
+// address is used.  Every call on one GPU shares the array; calls that overlap
+// on one GPU race.
+static __device__ __align__(256) unsigned char scratch[QNB_ADJOINT_STATIC_BYTES];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device `dev` (cached per device)
+static int scratch_lookup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(QNB_EDEVICE, "device %d beyond the workspace table", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(QNB_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+  g_scratch_ptr[dev] = (unsigned char*)ptr;
+  return 0;
+}
+
 struct qnb_handle {
+  int device = 0;  // the device current at qnb_create
   DevSys hs;  // host copy of the device record (device pointers)
   DevSys* dsys = nullptr;
   DevAlloc mem;
@@ -475,7 +504,11 @@ extern "C" int32_t qnb_create(const qnb_system* s, const qnb_params* p, qnb_hand
   if (rc) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(QNB_EDEVICE, "no HIP device");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(QNB_EDEVICE, "no current HIP device");
+  if ((rc = scratch_lookup(dev))) return rc;
   qnb_handle* h = new qnb_handle();
+  h->device = dev;
   DevSys& S = h->hs;
   memset(&S, 0, sizeof S);
   fill_params(S, s, p);
@@ -697,6 +730,80 @@ extern "C" int32_t qnb_adjoint(qnb_handle* h, const qnb_qtangent* tan, int32_t n
   if ((rc = tmp.alloc(&dfr, fl * std::min(nframes, chunk)))) return rc;
   if ((rc = tmp.alloc(&dout, ol * std::min(nframes, chunk)))) return rc;
   if ((rc = tmp.alloc(&dadj, 1))) return rc;
+  for (int f0 = 0; f0 < nframes; f0 += chunk) {
+    const int nf = std::min(chunk, nframes - f0);
+    A.frames = dfr;
+    A.nframes = nf;
+    A.out = dout;
+    HIPCHK(hipMemcpyAsync(dfr, frames + fl * f0, sizeof(double) * fl * nf, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(dadj, &A, sizeof A, hipMemcpyHostToDevice, h->stream));
+    launch_adjoint(h->stream, S.prec, S.nstates, (long)nf * S.nqat, 1, h->dsys, dadj);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out + ol * f0, dout, sizeof(double) * ol * nf, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  return 0;
+}
+
+// qnb_adjoint without device allocation: the same kernel, with every buffer
+// it needs beyond the caller's inputs and outputs in the fixed workspace,
+// laid out as [parameter record | tangent tables | frames | output], each
+// 256-byte aligned; frames go through in chunks that fit.
+extern "C" int32_t qnb_adjoint_static(qnb_handle* h, const qnb_qtangent* tan, int32_t nframes, const double* frames,
+                                      double* out) {
+  if (!h) return fail(QNB_EINVAL, "null handle");
+  if (nframes < 0 || (nframes > 0 && (!frames || !out))) return fail(QNB_EINVAL, "bad frames or output");
+  if (!tan) return fail(QNB_EINVAL, "null tangent tables");
+  const DevSys& S = h->hs;
+  if (S.nqat <= 0 || nframes == 0) return 0;
+  unsigned char* base = (h->device >= 0 && h->device < kMaxDevices) ? g_scratch_ptr[h->device] : nullptr;
+  if (!base) return fail(QNB_EDEVICE, "no workspace address for device %d", h->device);
+  const size_t nq = S.nqat, ns = S.nstates, nsol = S.nat_solute, sa = S.sa;
+  const size_t np = nsol * nq * ns, nw = S.nwat > 0 ? nq * sa * ns : 0, nqq = nq * nq * ns;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  // the tangent tables (layouts of the Q tables, qnb_qtangent)
+  struct Tab {
+    const void* src;
+    size_t bytes;
+    const char* name;
+  };
+  const Tab tabs[] = {{tan->qp_dq, 8 * np, "qp_dq"},     {tan->qp_dA, 8 * np, "qp_dA"},
+                      {tan->qp_dB, 8 * np, "qp_dB"},     {tan->qp_is14, np, "qp_is14"},
+                      {tan->qw_dq, 8 * nw, "qw_dq"},     {tan->qw_dA, 8 * nw, "qw_dA"},
+                      {tan->qw_dB, 8 * nw, "qw_dB"},     {tan->qq_dq_i, 8 * nqq, "qq_dq_i"},
+                      {tan->qq_dA_i, 8 * nqq, "qq_dA_i"}, {tan->qq_dB_i, 8 * nqq, "qq_dB_i"},
+                      {tan->qq_dq_j, 8 * nqq, "qq_dq_j"}, {tan->qq_dA_j, 8 * nqq, "qq_dA_j"},
+                      {tan->qq_dB_j, 8 * nqq, "qq_dB_j"}, {tan->qq_is14, nqq, "qq_is14"}};
+  size_t off = al(sizeof(AdjDev));
+  size_t toff[14];
+  for (int t = 0; t < 14; t++) {
+    if (tabs[t].bytes > 0 && !tabs[t].src) return fail(QNB_EINVAL, "tangent table %s missing", tabs[t].name);
+    toff[t] = off;
+    off += al(tabs[t].bytes);
+  }
+  const size_t fl = 3 * (size_t)S.natom, ol = ns * nq * QNB_ADJ_NV;
+  const size_t per = al(8 * fl) + al(8 * ol);  // one frame and its output (upper bound per frame)
+  if (off + per > QNB_ADJOINT_STATIC_BYTES)
+    return fail(QNB_ENOMEM,
+                "qnb_adjoint_static: tangent tables (%zu bytes) plus one frame and its output (%zu bytes) exceed the "
+                "static workspace of %u bytes; use qnb_adjoint",
+                off, per, (unsigned)QNB_ADJOINT_STATIC_BYTES);
+  const size_t room = QNB_ADJOINT_STATIC_BYTES - off;
+  int chunk = (int)std::min<size_t>((size_t)nframes, std::min<size_t>(room / (8 * fl + 8 * ol), 256));
+  while (chunk > 1 && al(8 * fl * chunk) + al(8 * ol * chunk) > room) chunk--;
+  double* dfr = (double*)(base + off);
+  double* dout = (double*)(base + off + al(8 * fl * chunk));
+  AdjDev A{};
+  const void** dst[14] = {(const void**)&A.qp_dq,   (const void**)&A.qp_dA,   (const void**)&A.qp_dB,
+                          (const void**)&A.qp_is14, (const void**)&A.qw_dq,   (const void**)&A.qw_dA,
+                          (const void**)&A.qw_dB,   (const void**)&A.qq_dq_i, (const void**)&A.qq_dA_i,
+                          (const void**)&A.qq_dB_i, (const void**)&A.qq_dq_j, (const void**)&A.qq_dA_j,
+                          (const void**)&A.qq_dB_j, (const void**)&A.qq_is14};
+  for (int t = 0; t < 14; t++) {
+    *dst[t] = base + toff[t];
+    if (tabs[t].bytes) HIPCHK(hipMemcpyAsync(base + toff[t], tabs[t].src, tabs[t].bytes, hipMemcpyHostToDevice, h->stream));
+  }
+  AdjDev* dadj = (AdjDev*)base;
   for (int f0 = 0; f0 < nframes; f0 += chunk) {
     const int nf = std::min(chunk, nframes - f0);
     A.frames = dfr;

@@ -136,6 +136,17 @@ class NonbondedEngine:
         abi.check(self.lib, self.lib.qnb_adjoint(self.h, C.byref(tangent), nf, _pd(fr), _pd(out)), "qnb_adjoint")
         return out
 
+    def adjoint_static(self, tangent, frames):
+        """As adjoint(), through the entry point that allocates no device
+        memory (qnb_adjoint_static: a fixed per-GPU workspace)."""
+        fr = np.ascontiguousarray(frames, np.float64)
+        nf = fr.shape[0]
+        nq = int(self.sys.n["nqat"])
+        out = np.zeros((nf, self.nst, nq, abi.QNB_ADJ_NV))
+        abi.check(self.lib, self.lib.qnb_adjoint_static(self.h, C.byref(tangent), nf, _pd(fr), _pd(out)),
+                  "qnb_adjoint_static")
+        return out
+
 
 def pot_energy_summary(E, EQ, lambdas, bonded=None, EQ_bonded=None, EQ_restraint=None):
     """The energy summary of pot_energy (src/potene.f90:224-251) in its
