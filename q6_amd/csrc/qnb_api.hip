@@ -17,8 +17,10 @@ namespace qnb {
 template <typename R, bool ANY>
 __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
-template <typename R, int NS, bool WAT, bool E>
-__global__ void k_forces(const DevSys*, uint32_t, const double*);
+template <typename R, int NS, bool E, int CAP>
+__global__ void k_wforces(const DevSys*, uint32_t, int, int, int);
+template <typename R, int NS, bool E, int CAP>
+__global__ void k_sforces(const DevSys*, uint32_t, int, int, int);
 template <typename R, int NS>
 __global__ void k_qforces(const DevSys*, uint32_t, int);
 template <typename R, int NS>
@@ -245,6 +247,39 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
   UP(ljcod, s->ljcod, s->ntypes * s->ntypes);
   UP(exmask, T.exmask.data(), T.exmask.size());
   UP(m14mask, T.m14mask.data(), T.m14mask.size());
+  {
+    // fast-path data of the solute receivers (DevSys::gspan ... ppf)
+    const int ncs1 = std::max(s->ncgp_solute, 1);
+    std::vector<int> span(2 * ncs1, 0);
+    std::vector<uint8_t> glong(ncs1, 0);
+    for (int g = 0; g < s->ncgp_solute; g++) {
+      int lo = nat, hi = -1;
+      for (int k = s->cgp_first[g]; k <= s->cgp_last[g]; k++) {
+        const int i = s->cgpatom[k];
+        lo = std::min(lo, i);
+        hi = std::max(hi, i);
+        if (i < nsol && (T.exmask[i] >> 31)) glong[g] = 1;
+      }
+      span[2 * g] = lo;
+      span[2 * g + 1] = hi;
+    }
+    UP(gspan, span.data(), span.size());
+    UP(glong, glong.data(), glong.size());
+    bool code1 = s->ivdw_rule == 1;
+    for (int k = 0; k < s->ntypes * s->ntypes && code1; k++) code1 = s->ljcod[k] == 1;
+    S.ppfast = code1 ? 1 : 0;
+    std::vector<double> pd(4 * (size_t)nsol1, 0.0);
+    std::vector<float> pf(4 * (size_t)nsol1, 0.0f);
+    for (int i = 0; i < nsol; i++) {
+      const int t = s->iac[i];
+      pd[4 * i] = s->avdw[3 * t];
+      pd[4 * i + 1] = s->bvdw[3 * t];
+      pd[4 * i + 2] = s->crg[i];
+      for (int c = 0; c < 3; c++) pf[4 * i + c] = (float)pd[4 * i + c];
+    }
+    UP(ppd, pd.data(), pd.size());
+    UP(ppf, pf.data(), pf.size());
+  }
   UP(lx_off, T.lx_off.data(), T.lx_off.size());
   UP(lx_j, T.lx_j.data(), T.lx_j.size());
   UP(lx_kind, T.lx_kind.data(), T.lx_kind.size());
@@ -432,6 +467,7 @@ static int scratch_lookup(int dev) {
 
 struct qnb_handle {
   int device = 0;  // the device current at qnb_create
+  bool small_groups = true;  // every solute group fits the water kernel's small staging
   DevSys hs;  // host copy of the device record (device pointers)
   DevSys* dsys = nullptr;
   DevAlloc mem;
@@ -509,6 +545,7 @@ extern "C" int32_t qnb_create(const qnb_system* s, const qnb_params* p, qnb_hand
   if ((rc = scratch_lookup(dev))) return rc;
   qnb_handle* h = new qnb_handle();
   h->device = dev;
+  h->small_groups = max_solute_group(s->cgp_first, s->cgp_last, s->ncgp_solute) <= WF_CAP_SMALL;
   DevSys& S = h->hs;
   memset(&S, 0, sizeof S);
   fill_params(S, s, p);
@@ -582,12 +619,24 @@ template <typename R, int NS>
 static void launch_eval(qnb_handle* h, uint32_t mask) {
   const DevSys& S = h->hs;
   const int wpb = 4;
-  if (S.nwat > 0)
-    hipLaunchKernelGGL((k_forces<R, NS, true, true>), dim3((S.nwat + wpb - 1) / wpb, 1), dim3(64 * wpb),
-                       fc_wave_bytes<R>() * wpb, h->stream, h->dsys, mask, (const double*)h->d_tfree);
-  if (S.ncgp_solute > 0)
-    hipLaunchKernelGGL((k_forces<R, NS, false, true>), dim3((S.ncgp_solute + wpb - 1) / wpb, 1), dim3(64 * wpb),
-                       fc_wave_bytes<R>() * wpb, h->stream, h->dsys, mask, (const double*)h->d_tfree);
+  if (S.nwat > 0) {
+    const int nb = (S.nwat + wpb - 1) / wpb;
+    if (h->small_groups)
+      hipLaunchKernelGGL((k_wforces<R, NS, true, WF_CAP_SMALL>), dim3(nb, 1), dim3(64 * wpb),
+                         (wf_wave_bytes<R, WF_CAP_SMALL>() * wpb), h->stream, h->dsys, mask, 0, nb, 1);
+    else
+      hipLaunchKernelGGL((k_wforces<R, NS, true, FC_CAP>), dim3(nb, 1), dim3(64 * wpb),
+                         (wf_wave_bytes<R, FC_CAP>() * wpb), h->stream, h->dsys, mask, 0, nb, 1);
+  }
+  if (S.ncgp_solute > 0) {
+    const int nb = (S.ncgp_solute + wpb - 1) / wpb;
+    if (h->small_groups)
+      hipLaunchKernelGGL((k_sforces<R, NS, true, WF_CAP_SMALL>), dim3(nb, 1), dim3(64 * wpb),
+                         (sf_wave_bytes<R, WF_CAP_SMALL>() * wpb), h->stream, h->dsys, mask, 0, nb, 1);
+    else
+      hipLaunchKernelGGL((k_sforces<R, NS, true, FC_CAP>), dim3(nb, 1), dim3(64 * wpb), (sf_wave_bytes<R, FC_CAP>() * wpb),
+                         h->stream, h->dsys, mask, 0, nb, 1);
+  }
   hipLaunchKernelGGL(k_atom_terms, dim3((S.natom + 255) / 256, 1), dim3(256), 0, h->stream, h->dsys, mask,
                      (const double*)h->d_tfree);
   if (S.nqat > 0 && (mask & (QNB_T_QW | QNB_T_QP))) {

@@ -20,8 +20,10 @@ __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
 __global__ void k_pack_xq(const DevSys*);
 __global__ void k_pw_first(const DevSys*);
-template <typename R, int NS, bool WAT, bool E>
-__global__ void k_forces(const DevSys*, uint32_t, const double*);
+template <typename R, int NS, bool E, int CAP>
+__global__ void k_wforces(const DevSys*, uint32_t, int, int, int);
+template <typename R, int NS, bool E, int CAP>
+__global__ void k_sforces(const DevSys*, uint32_t, int, int, int);
 template <typename R, int NS>
 __global__ void k_qforces(const DevSys*, uint32_t, int);
 template <typename R, int NS>
@@ -526,6 +528,9 @@ struct qnb_batch {
   bool anyq = false, anypol = false, static_q = true;
   int qchunks = 1, nq_max = 0;
   int wpb_lists = 4, wpb_forces = 4;  // waves per block, picked by qnb_batch_autotune
+  int imap_water = 1;                 // block placement of the water-receiver kernel (map_block), autotuned
+  int imap_solute = 0;                // block placement of the solute-receiver kernel, autotuned
+  bool small_groups = true;           // every solute group fits the water kernel's small staging
   std::vector<int> nbcycle;
   std::vector<int> ene_every;  // per instance energy sampling interval
   // Q bonded terms per instance
@@ -580,14 +585,31 @@ static int forces_wpb(int wpb) {
 // the same XCD and find them in its L2.  The padding blocks exit at once.
 static unsigned xcd_cols(int blocks) { return (unsigned)((blocks + 7) / 8 * 8); }
 
+// water receivers (k_wforces) with the block placement imap (map_block)
+template <typename R, int NS, bool E, int CAP>
+static void launch_water(qnb_batch* b, int wpb, int imap) {
+  const int nb = (b->maxw + wpb - 1) / wpb;
+  const dim3 grid = imap == 0 ? dim3(xcd_cols(nb), b->n) : dim3((unsigned)(8 * (((long)nb * b->n + 7) / 8)), 1);
+  hipLaunchKernelGGL((k_wforces<R, NS, E, CAP>), grid, dim3(64 * wpb), (wf_wave_bytes<R, CAP>() * wpb), b->stream,
+                     b->dsys, kStepMask, imap, nb, b->n);
+}
+
 template <typename R, int NS, bool E>
 static void launch_forces_e(qnb_batch* b, int wpb) {
-  if (b->maxw > 0)
-    hipLaunchKernelGGL((k_forces<R, NS, true, E>), dim3(xcd_cols((b->maxw + wpb - 1) / wpb), b->n), dim3(64 * wpb),
-                       fc_wave_bytes<R>() * wpb, b->stream, b->dsys, kStepMask, (const double*)b->tfree);
-  if (b->maxs > 0)
-    hipLaunchKernelGGL((k_forces<R, NS, false, E>), dim3(xcd_cols((b->maxs + wpb - 1) / wpb), b->n), dim3(64 * wpb),
-                       fc_wave_bytes<R>() * wpb, b->stream, b->dsys, kStepMask, (const double*)b->tfree);
+  if (b->maxw > 0) {
+    if (b->small_groups) launch_water<R, NS, E, WF_CAP_SMALL>(b, wpb, b->imap_water);
+    else launch_water<R, NS, E, FC_CAP>(b, wpb, b->imap_water);
+  }
+  if (b->maxs > 0) {
+    const int nb = (b->maxs + wpb - 1) / wpb, imap = b->imap_solute;
+    const dim3 grid = imap == 0 ? dim3(xcd_cols(nb), b->n) : dim3((unsigned)(8 * (((long)nb * b->n + 7) / 8)), 1);
+    if (b->small_groups)
+      hipLaunchKernelGGL((k_sforces<R, NS, E, WF_CAP_SMALL>), grid, dim3(64 * wpb),
+                         (sf_wave_bytes<R, WF_CAP_SMALL>() * wpb), b->stream, b->dsys, kStepMask, imap, nb, b->n);
+    else
+      hipLaunchKernelGGL((k_sforces<R, NS, E, FC_CAP>), grid, dim3(64 * wpb), (sf_wave_bytes<R, FC_CAP>() * wpb),
+                         b->stream, b->dsys, kStepMask, imap, nb, b->n);
+  }
 }
 
 // energy: the step samples energies in at least one instance (or the host
@@ -737,6 +759,8 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
     b->maxg = std::max(b->maxg, S.ncgp);
     b->maxw = std::max(b->maxw, S.nwat);
     b->maxs = std::max(b->maxs, S.ncgp_solute);
+    b->small_groups = b->small_groups &&
+                      max_solute_group(sys[k]->cgp_first, sys[k]->cgp_last, sys[k]->ncgp_solute) <= WF_CAP_SMALL;
     b->maxat = std::max(b->maxat, S.natom);
     b->maxsw = std::max(b->maxsw, S.nwat + S.nat_solute);
     b->ns_max = std::max(b->ns_max, S.nstates);
@@ -1287,26 +1311,39 @@ extern "C" int32_t qnb_batch_autotune(qnb_batch* b, int32_t reps, int32_t* wpb_o
   const int cand[4] = {1, 2, 4, 8};
   for (int which = 0; which < 2; which++) {
     float best = 1e30f;
-    int pick = 4;
-    for (int c : cand) {
-      if (which == 0) launch_lists(b, c);
-      else launch_forces(b, c, false);
-      HIPCHK(hipEventRecord(e0, b->stream));
-      for (int r = 0; r < reps; r++) {
+    int pick = 4, pick_map = b->imap_water | (b->imap_solute << 1);
+    // the force kernels also try both block placements (map_block) of the
+    // water receivers (bit 0) and of the solute receivers (bit 1)
+    for (int imap = 0; imap < (which == 0 ? 1 : 4); imap++) {
+      if (which == 1) {
+        b->imap_water = imap & 1;
+        b->imap_solute = imap >> 1;
+      }
+      for (int c : cand) {
         if (which == 0) launch_lists(b, c);
         else launch_forces(b, c, false);
-      }
-      HIPCHK(hipEventRecord(e1, b->stream));
-      HIPCHK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        pick = c;
+        HIPCHK(hipEventRecord(e0, b->stream));
+        for (int r = 0; r < reps; r++) {
+          if (which == 0) launch_lists(b, c);
+          else launch_forces(b, c, false);
+        }
+        HIPCHK(hipEventRecord(e1, b->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          pick = c;
+          pick_map = imap;
+        }
       }
     }
     if (which == 0) b->wpb_lists = pick;
-    else b->wpb_forces = pick;
+    else {
+      b->wpb_forces = pick;
+      b->imap_water = pick_map & 1;
+      b->imap_solute = pick_map >> 1;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -59,6 +59,17 @@ struct DevSys {
   const int* lx_off;       // [nat_solute+1] long exclusion/1-4 partners of the lower atom
   const int* lx_j;
   const uint8_t* lx_kind;  // 0 excluded, 1 1-4
+  // solute-receiver fast path (k_sforces): a partner group b whose atom
+  // indices lie more than 25 from every atom of the receiver group a, with
+  // no long-range exclusion or 1-4 partner in either group, holds no
+  // excluded or 1-4 pair with a; with the geometric rule and LJ code 1 for
+  // every type pair, A, B and qq of such a pair are products of per-atom
+  // factors
+  const int* gspan;       // [ncgp_solute][2]: smallest and largest atom index of the group
+  const uint8_t* glong;   // [ncgp_solute]: an atom of the group has long-range exclusion / 1-4 partners
+  int ppfast;             // geometric rule and ljcod = 1 throughout
+  const double* ppd;      // [nat_solute][4]: avdw(t, 1), bvdw(t, 1), charge, 0 (FP64 path)
+  const float* ppf;       // [nat_solute][4]: the same in FP32 (mixed path)
   const double* pwA;       // [i + j*nat_solute]
   const double* pwB;
   const double* pwEl;
@@ -202,6 +213,30 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, QNB_WAVE);
   return v;
+}
+
+// Sum over the wavefront without LDS instructions, for call sites where every
+// lane is active: DPP moves within each row of 16 lanes (quad_perm xor 1 and
+// xor 2, row_half_mirror, row_mirror), then the four row sums are read with
+// v_readlane and added in a fixed order, so the result (the same in every
+// lane) is deterministic.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_all(double v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  v += dpp_move<0x140>(v);  // row_mirror
+  return ((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48);
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -439,6 +474,50 @@ __host__ __device__ constexpr size_t fc_wave_bytes() {
            sizeof(int) * (2 * FC_CAP + 4 * FC_CH)) + 15) / 16 * 16;
 }
 // type tables of the solute-solute parameters held in LDS up to this many types
+// Water-receiver force kernel (k_wforces): solute partner atoms are staged in
+// rounds of at most CAP slots per wave (WF_CAP_SMALL when every solute group
+// of the launch fits, else FC_CAP); partner waters need no staging.
+#define WF_CAP_SMALL 128
+// per-wave LDS of the solute-receiver kernel (k_sforces) with CAP partner
+// slots: partner offsets, indices and tags, then the sweep's receiver data
+template <typename R, int CAP>
+__host__ __device__ constexpr size_t sf_wave_bytes() {
+  return ((sizeof(R) * (3 * (CAP + FC_CH) + 9 * FC_CH) + sizeof(double) * FC_CH +
+           sizeof(int) * (2 * CAP + 4 * FC_CH)) + 15) / 16 * 16;
+}
+template <typename R, int CAP>
+__host__ __device__ constexpr size_t wf_wave_bytes() {
+  return (sizeof(R) * 3 * CAP + sizeof(int) * 2 * CAP + 15) / 16 * 16;
+}
+
+// Placement of a batch's blocks on the XCDs.  Blocks are dealt to the 8 XCDs
+// round-robin in linear block order.  imap 0: grid (nb, ninst), block x of
+// every instance on the same XCD (instances share the topology's rows in
+// L2).  imap 1: a 1-D grid of 8 * ceil(nb * ninst / 8) blocks whose k-th
+// block on XCD x takes work item x * per + k of the instance-major order, so
+// each instance's blocks run on one or two XCDs and its coordinates stay in
+// their L2.  Returns false for padding blocks.
+__device__ __forceinline__ bool map_block(int imap, int nb, int ninst, int& inst, int& blk) {
+  if (imap == 0) {
+    inst = blockIdx.y;
+    blk = blockIdx.x;
+    return blk < nb;
+  }
+  const long total = (long)nb * ninst, per = (total + 7) / 8, L = blockIdx.x;
+  const long g = (L % 8) * per + L / 8;
+  if (g >= total) return false;
+  inst = (int)(g / nb);
+  blk = (int)(g % nb);
+  return true;
+}
+
+// largest solute charge group of a system (gfirst/glast span)
+inline int max_solute_group(const int* gfirst, const int* glast, int ncgp_solute) {
+  int m = 0;
+  for (int g = 0; g < ncgp_solute; g++) m = glast[g] - gfirst[g] + 1 > m ? glast[g] - gfirst[g] + 1 : m;
+  return m;
+}
+
 #define FC_MAXT 64
 // static LDS of the solute-receiver force kernel: the LJ factors and the
 // ljcod table of up to FC_MAXT types

@@ -269,6 +269,13 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
   for (int k = 0; k < 20; k++) acc[k] = 0.0;
   int count = 0;
   int* __restrict__ row = S.nbr + (size_t)a * S.cap;
+  // mixed path: raw FP32 sums of this lane's far atoms since the last flush
+  // to the FP64 expansion; the wave flushes together once a lane holds 10
+  // or more atoms (a solute group, or three or four waters)
+  float pf[LRF_NF];
+#pragma unroll
+  for (int k = 0; k < LRF_NF; k++) pf[k] = 0.0f;
+  int pf_atoms = 0;
   if (!ldg(S.excl + ra)) {
     for (int b0 = 0; b0 < S.ncgp; b0 += QNB_WAVE) {
       const int b = b0 + lane;
@@ -302,15 +309,12 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
             if constexpr (sizeof(R) == 4) {
               // mixed path: packed FP32 positions and charges (Q-atoms carry 0)
               const auto xq = gp(S.xq);
-              float pf[LRF_NF];
-#pragma unroll
-              for (int k = 0; k < LRF_NF; k++) pf[k] = 0.0f;
               const int k0 = b_sol ? ldg(S.gfirst + b) : 0, k1 = b_sol ? ldg(S.glast + b) : S.sa - 1;
               for (int k = k0; k <= k1; k++) {
                 const int i = b_sol ? ldg(S.cgpatom + k) : rb + k;
                 lrf_accumulate_f(pf, xq[4 * i] - cxf, xq[4 * i + 1] - cyf, xq[4 * i + 2] - czf, xq[4 * i + 3]);
               }
-              lrf_flush_f(acc, pf);
+              pf_atoms += k1 - k0 + 1;
             } else if (b_sol) {
               const int k1 = ldg(S.glast + b);
               for (int k = ldg(S.gfirst + b); k <= k1; k++) {
@@ -337,10 +341,19 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
       const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
       if (in && pos < S.cap) stg(row + pos, b);
       count += __popcll(m);
+      if constexpr (sizeof(R) == 4) {
+        if (__ballot(pf_atoms >= 10)) {
+          lrf_flush_f(acc, pf);
+#pragma unroll
+          for (int k = 0; k < LRF_NF; k++) pf[k] = 0.0f;
+          pf_atoms = 0;
+        }
+      }
     }
   }
+  if constexpr (sizeof(R) == 4) lrf_flush_f(acc, pf);
 #pragma unroll
-  for (int k = 0; k < 20; k++) acc[k] = wave_sum(acc[k]);
+  for (int k = 0; k < 20; k++) acc[k] = wave_sum_all(acc[k]);
   if (lane == 0) {
     S.nnbr[a] = count;
     if (count > S.cap) atomicMax(S.need, count);
@@ -453,6 +466,7 @@ enum : uint32_t { T_WW = 1, T_QW = 2, T_PW = 4, T_PP = 8, T_QP = 16, T_LRF = 32,
 #define FT_SOL 4
 #define FT_OWN 8
 #define FT_Q 16
+#define FT_FAR 32  // solute partner group that holds no excluded or 1-4 pair with the receiver group
 // pp_param from data at hand: the two atoms' types, charges and exclusion
 // masks (bit 31: partners beyond the 25-atom window, looked up in the long
 // list) and the type tables (LJ factors, ljcod) in LDS.  Same result as
@@ -511,7 +525,7 @@ __device__ __forceinline__ bool pp_param_fast(const DevSys& S, const PPAtom& P, 
   return set;
 }
 
-template <typename R>
+template <typename R, int CAP = FC_CAP>
 __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int* __restrict__ row, int p0, int nn,
                                               uint32_t use_sol, uint32_t use_wat, double xr0, double xr1,
                                               double xr2, int* s_j, R (*s_d)[3], int* s_t, int* ngroups) {
@@ -532,7 +546,7 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
     if (lane >= o2) incl += v;
   }
   // inclusive sums never decrease, so the groups that fit are a prefix
-  const int ng = __popcll(__ballot(incl <= FC_CAP));
+  const int ng = __popcll(__ballot(incl <= CAP));
   *ngroups = ng;
   if (ng == 0) return 0;
   const int total = __shfl(incl, ng - 1, QNB_WAVE);
@@ -540,9 +554,16 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
   // slots below hold nothing of this batch)
   if (total == 0) return 0;
   // the group lanes lay out their slots (LDS only): position in cgpatom and
-  // the group part of the tag
+  // the group part of the tag (FT_FAR for a solute group whose atoms are all
+  // more than 25 indices from the solute receiver's, neither group having
+  // long-range exclusion / 1-4 partners)
   if (lane < ng && cnt > 0) {
-    const int tag0 = (b < S.ncgp_solute ? FT_SOL : 0) | (owns(a, b) ? FT_OWN : 0);
+    int tag0 = (b < S.ncgp_solute ? FT_SOL : 0) | (owns(a, b) ? FT_OWN : 0);
+    if (b < S.ncgp_solute && a < S.ncgp_solute && !ldg(S.glong + a) && !ldg(S.glong + b)) {
+      const int alo = ldg(S.gspan + 2 * a), ahi = ldg(S.gspan + 2 * a + 1);
+      const int blo = ldg(S.gspan + 2 * b), bhi = ldg(S.gspan + 2 * b + 1);
+      if (blo - ahi > 25 || alo - bhi > 25) tag0 |= FT_FAR;
+    }
     for (int m = 0, slot = incl - cnt; m < cnt; m++, slot++) {
       s_j[slot] = f0 + m;
       s_t[slot] = tag0 | (m & 3);
@@ -553,9 +574,9 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
   // all atom-index loads of the round first, then all position loads, so the
   // loads of a round are in flight together (slots past the end load the
   // receiver's first atom, a valid index, and store nothing)
-  constexpr int FC_SPL = (FC_CAP + QNB_WAVE - 1) / QNB_WAVE;
-  constexpr int FC_SPR = 3;
-  static_assert(FC_SPL % FC_SPR == 0, "staging rounds must cover FC_CAP");
+  constexpr int FC_SPL = (CAP + QNB_WAVE - 1) / QNB_WAVE;
+  constexpr int FC_SPR = FC_SPL % 3 == 0 ? 3 : (FC_SPL % 2 == 0 ? 2 : 1);
+  static_assert(FC_SPL % FC_SPR == 0, "staging rounds must cover the slots");
   const int pos_pad = ldg(S.gfirst + a);
 #pragma unroll
   for (int u0 = 0; u0 < FC_SPL; u0 += FC_SPR) {
@@ -593,45 +614,58 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
   return total;
 }
 
-// WAT: the water receivers (groups ncgp_solute..ncgp-1), else the solute
-// receivers; the two launches write disjoint parts of d and epart and get
-// their own register allocation
-// E: false for launches on steps that sample no energies (the host knows);
-// true evaluates the device-side sampling test
-template <typename R, int NS, bool WAT, bool E>
-__global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys* __restrict__ systems, uint32_t mask,
-                                                        const double* __restrict__ tfree) {
-  const DevSys& S = systems[blockIdx.y];
+// ---------------------------------------------------------------------------
+// Solute receivers: one wavefront per solute charge group a, its atoms taken
+// in sweeps of up to CH (receiver data in LDS and registers).  The row is
+// sorted, so the partner solute groups come first and the waters after them.
+// Per sweep:
+//  * partner solute groups through the staged atom expansion, one partner
+//    atom per lane against the sweep's receiver atoms (nonbond_pp,
+//    nonbondene.f90:4694-4760).  A group that can hold no excluded or 1-4
+//    pair with a (FT_FAR) takes A, B and qq as products of per-atom factors
+//    when the system uses the geometric rule with LJ code 1 throughout
+//    (ppfast): the products pp_param forms for such a pair, same operands;
+//    every other pair goes through pp_param (exclusions, 1-4, ljcod, rule);
+//  * partner waters with the fixed mapping of k_wforces (lane l takes site
+//    l % 3 of water l / 3 of the round), the pw parameters of that site for
+//    every receiver atom held in registers (nonbond_pw 4856-4916);
+//  * the pairs inside the group (ig == jgr, i < j, nonbondene.f90:1610) and
+//    the receiver side of the Q-atom - solute pairs (nonbond_qp 5157-5230).
+// Writes d of the group's atoms, and the wave's pp, pw and qp energy
+// partials on the sampled steps (a pp pair's energy on its owner side).
+template <typename R, int NS, bool E, int CAP>
+__global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_sforces(const DevSys* __restrict__ systems, uint32_t mask,
+                                                                     int imap, int nb, int ninst) {
+  int inst, blk;
+  if (!map_block(imap, nb, ninst, inst, blk)) return;
+  const DevSys& S = systems[inst];
   const int wib = threadIdx.x / QNB_WAVE;
-  const int a = __builtin_amdgcn_readfirstlane((WAT ? S.ncgp_solute : 0) + blockIdx.x * (blockDim.x / QNB_WAVE) + wib);
-  // solute receivers: LJ factor and ljcod tables in LDS (every thread of the
-  // block takes part before any wave leaves)
+  const int a = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x / QNB_WAVE) + wib);
+  // LJ factor and ljcod tables in LDS (every thread of the block takes part
+  // before any wave leaves)
   __shared__ double s_avdw[3 * FC_MAXT], s_bvdw[3 * FC_MAXT];
   __shared__ uint8_t s_ljc[FC_MAXT * FC_MAXT];
-  const bool lds_types = !WAT && S.ntypes <= FC_MAXT;
-  if constexpr (!WAT) {
-    if (lds_types) {
-      const int nt = S.ntypes;
-      for (int k = threadIdx.x; k < 3 * nt; k += blockDim.x) {
-        s_avdw[k] = ldg(S.avdw + k);
-        s_bvdw[k] = ldg(S.bvdw + k);
-      }
-      for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) s_ljc[k] = (uint8_t)ldg(S.ljcod + k);
+  const bool lds_types = S.ntypes <= FC_MAXT;
+  if (lds_types) {
+    const int nt = S.ntypes;
+    for (int k = threadIdx.x; k < 3 * nt; k += blockDim.x) {
+      s_avdw[k] = ldg(S.avdw + k);
+      s_bvdw[k] = ldg(S.bvdw + k);
     }
-    __syncthreads();
+    for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) s_ljc[k] = (uint8_t)ldg(S.ljcod + k);
   }
-  if (a >= (WAT ? S.ncgp : S.ncgp_solute)) return;
+  __syncthreads();
+  if (a >= S.ncgp_solute) return;
   const int lane = threadIdx.x % QNB_WAVE;
-  // per-wave LDS (dynamic, fc_wave_bytes<R>() per wave of the block)
   extern __shared__ __align__(16) unsigned char fc_smem[];
-  unsigned char* wbase = fc_smem + (size_t)wib * fc_wave_bytes<R>();
+  unsigned char* wbase = fc_smem + (size_t)wib * sf_wave_bytes<R, CAP>();
   R(*s_pd)[3] = reinterpret_cast<R(*)[3]>(wbase);
-  R(*s_o)[3] = reinterpret_cast<R(*)[3]>(wbase + sizeof(R) * 3 * FC_CAP);
-  R(*s_pw)[3][3] = reinterpret_cast<R(*)[3][3]>(wbase + sizeof(R) * 3 * (FC_CAP + FC_CH));
-  double* s_rq = reinterpret_cast<double*>(wbase + sizeof(R) * (3 * (FC_CAP + FC_CH) + 9 * FC_CH));
+  R(*s_o)[3] = reinterpret_cast<R(*)[3]>(wbase + sizeof(R) * 3 * CAP);
+  R(*s_pw)[3][3] = reinterpret_cast<R(*)[3][3]>(wbase + sizeof(R) * 3 * (CAP + FC_CH));
+  double* s_rq = reinterpret_cast<double*>(wbase + sizeof(R) * (3 * (CAP + FC_CH) + 9 * FC_CH));
   int* s_pj = reinterpret_cast<int*>(s_rq + FC_CH);
-  int* s_pt = s_pj + FC_CAP;
-  int* s_ia = s_pt + FC_CAP;
+  int* s_pt = s_pj + CAP;
+  int* s_ia = s_pt + CAP;
   int* s_rt = s_ia + FC_CH;
   uint32_t* s_rx = reinterpret_cast<uint32_t*>(s_rt + FC_CH);
   uint32_t* s_rm = s_rx + FC_CH;
@@ -641,335 +675,224 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
   const int* __restrict__ row = S.nbr + (size_t)a * S.cap;
   const bool want_e = E && want_energies(S);
   const int nsol = S.nat_solute, sa = S.sa, ncs = S.ncgp_solute;
-  const bool a_sol = a < ncs;
-  const int ra = group_ref_atom(S, a);
+  const int ra = ldg(S.gswitch + a);
   const bool aexcl = ldg(S.excl + ra);
-  double e[9];
+  const bool fast = S.ppfast != 0;
+  double e_ppel = 0.0, e_ppvdw = 0.0, e_pwel = 0.0, e_pwvdw = 0.0;
+  double eq[2 * NS];  // per state: qp_el, qp_vdw
 #pragma unroll
-  for (int k = 0; k < 9; k++) e[k] = 0.0;
-  double eq[4 * NS];  // per state: qp_el, qp_vdw, qw_el, qw_vdw
-#pragma unroll
-  for (int k = 0; k < 4 * NS; k++) eq[k] = 0.0;
+  for (int k = 0; k < 2 * NS; k++) eq[k] = 0.0;
   const double xr0 = ldg(x + (3 * ra)), xr1 = ldg(x + (3 * ra + 1)), xr2 = ldg(x + (3 * ra + 2));
   const int g0 = ldg(S.gfirst + a), na = ldg(S.glast + a) - g0 + 1;
   const int nq = S.nqat, ns = S.nstates;
-
-  if constexpr (WAT) {
-    // ------------------------------------------------ water receiver (3 atoms)
-    // water-water parameters, wave-uniform: charge products of receiver atom
-    // k with partner atom kb, and the O-O LJ pair
-    R wel[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) wel[k] = (R)ldg(S.wwEl + k);
-    const R wA = (R)ldg(S.wwA + 0), wB = (R)ldg(S.wwB + 0);
-    R o[3][3], f[3][3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      o[k][0] = (R)(ldg(x + (3 * (ra + k))) - xr0);
-      o[k][1] = (R)(ldg(x + (3 * (ra + k) + 1)) - xr1);
-      o[k][2] = (R)(ldg(x + (3 * (ra + k) + 2)) - xr2);
-      f[k][0] = f[k][1] = f[k][2] = R(0);
+  // length of the row's solute prefix
+  int nsr = 0;
+  for (int p0 = 0; p0 < nn; p0 += QNB_WAVE) {
+    const int p = p0 + lane;
+    const uint64_t m = __ballot(p < nn && ldg(row + p) < ncs);
+    nsr += __popcll(m);
+    if (m != ~0ull) break;
+  }
+  const bool qin = (mask & T_QP) && nq > 0 && ldg(S.qp_in + a);
+  constexpr int CH = sizeof(R) == 4 ? FC_CH : FC_CH / 2;
+  for (int c0 = 0; c0 < na; c0 += CH) {
+    const int nk = min(CH, na - c0);
+    // receiver atoms of this sweep in LDS: index (-1: takes no pair terms)
+    // and offset from the reference atom
+    if (lane < nk) {
+      const int i = ldg(S.cgpatom + (g0 + c0 + lane));
+      s_ia[lane] = (!aexcl && ldg(S.iqatom + i) == 0) ? i : -1;
+      s_o[lane][0] = (R)(ldg(x + (3 * i)) - xr0);
+      s_o[lane][1] = (R)(ldg(x + (3 * i + 1)) - xr1);
+      s_o[lane][2] = (R)(ldg(x + (3 * i + 2)) - xr2);
+      s_rt[lane] = ldg(S.iac + i);
+      s_rx[lane] = ldg(S.exmask + i);
+      s_rm[lane] = ldg(S.m14mask + i);
+      s_rq[lane] = ldg(S.crg + i);
+      // the atom's solute-water parameters against the 3 water sites
+      if (S.nwat > 0)
+        for (int kb = 0; kb < 3; kb++) {
+          const size_t tt = (size_t)i + (size_t)nsol * kb;
+          s_pw[lane][kb][0] = (R)ldg(S.pwA + tt);
+          s_pw[lane][kb][1] = (R)ldg(S.pwB + tt);
+          s_pw[lane][kb][2] = (R)ldg(S.pwEl + tt);
+        }
     }
+    __builtin_amdgcn_wave_barrier();
+    // the sweep's receiver data in registers: usable flag, offset, per-atom
+    // LJ factors and charge of the fast path
+    bool rok[CH];
+    R ro[CH][3], rA[CH], rB[CH], rQ[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+      const int i = k < nk ? s_ia[k] : -1;
+      rok[k] = i >= 0;
+      ro[k][0] = s_o[k][0];
+      ro[k][1] = s_o[k][1];
+      ro[k][2] = s_o[k][2];
+      if constexpr (sizeof(R) == 4) {
+        rA[k] = rok[k] ? ldg(S.ppf + 4 * i) : 0.0f;
+        rB[k] = rok[k] ? ldg(S.ppf + 4 * i + 1) : 0.0f;
+        rQ[k] = rok[k] ? ldg(S.ppf + 4 * i + 2) : 0.0f;
+      } else {
+        rA[k] = rok[k] ? ldg(S.ppd + 4 * i) : 0.0;
+        rB[k] = rok[k] ? ldg(S.ppd + 4 * i + 1) : 0.0;
+        rQ[k] = rok[k] ? ldg(S.ppd + 4 * i + 2) : 0.0;
+      }
+    }
+    R f[CH][3];
+#pragma unroll
+    for (int k = 0; k < CH; k++) f[k][0] = f[k][1] = f[k][2] = R(0);
     if (!aexcl) {
-      for (int p0 = 0, ng = 0; p0 < nn; p0 += ng) {
-        const int total = stage_partners<R>(S, a, row, p0, nn, mask & T_PW, mask & T_WW, xr0, xr1, xr2, s_pj,
-                                            s_pd, s_pt, &ng);
-        if (ng == 0) break;
-        for (int t = lane; t < total; t += QNB_WAVE) {
-          const int tag = s_pt[t], j = s_pj[t], kb = tag & 3;
-          const R dx = s_pd[t][0], dy = s_pd[t][1], dz = s_pd[t][2];
-          if (!(tag & FT_SOL)) {
-            // water-water: receiver atom k, partner atom kb; the partner's
-            // three pair energies are summed in R, then added in FP64
-            const bool acc = want_e && (tag & FT_OWN);
-            R pel = R(0), pvdw = R(0);
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-              const int tt = k + 3 * kb;
-              const R ux = dx - o[k][0], uy = dy - o[k][1], uz = dz - o[k][2];
-              R dv;
-              if (S.spc) {
-                // SPC: Coulomb for every pair, LJ for O-O only
-                const R q = kb == 0 ? wel[k] : (kb == 1 ? wel[k + 3] : wel[k + 6]);
-                if (k == 0 && kb == 0)
-                  dv = PairMath<R>::nbe(ux, uy, uz, wA, wB, q, pel, pvdw, acc);
-                else
-                  dv = PairMath<R>::nbe_spc(ux, uy, uz, q, pel, acc);
-              } else {
-                dv = PairMath<R>::nbe(ux, uy, uz, (R)ldg(S.wwA + tt), (R)ldg(S.wwB + tt), (R)ldg(S.wwEl + tt),
-                                      pel, pvdw, acc);
-              }
-              f[k][0] += ux * dv;
-              f[k][1] += uy * dv;
-              f[k][2] += uz * dv;
-            }
-            if (acc) {
-              e[E_WW_EL] += (double)pel;
-              e[E_WW_VDW] += (double)pvdw;
-            }
-          } else {
-            // solute partner j: pw table (on the mixed path its FP32 copy,
-            // one 48-byte record per solute atom), energy counted on the
-            // solute side
-            if (tag & FT_Q) continue;
-            R pw[9];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-              if constexpr (sizeof(R) == 4) {
-                pw[3 * k] = ldg(S.pw_f + PW_REC * j + 3 * k);
-                pw[3 * k + 1] = ldg(S.pw_f + PW_REC * j + 3 * k + 1);
-                pw[3 * k + 2] = ldg(S.pw_f + PW_REC * j + 3 * k + 2);
-              } else {
-                const size_t tt = (size_t)j + (size_t)nsol * k;
-                pw[3 * k] = (R)ldg(S.pwA + tt);
-                pw[3 * k + 1] = (R)ldg(S.pwB + tt);
-                pw[3 * k + 2] = (R)ldg(S.pwEl + tt);
-              }
-            }
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-              const R ux = dx - o[k][0], uy = dy - o[k][1], uz = dz - o[k][2];
-              R ee = R(0), ev = R(0);
-              const R dv = PairMath<R>::nbe(ux, uy, uz, pw[3 * k], pw[3 * k + 1], pw[3 * k + 2], ee, ev, false);
-              f[k][0] += ux * dv;
-              f[k][1] += uy * dv;
-              f[k][2] += uz * dv;
-            }
-          }
-        }
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
-    // Q-atom - water, receiver side (nonbond_qw_spc nonbondene.f90:5658-5767;
-    // nonbond_qw 5339-5418)
-    if ((mask & T_QW) && nq > 0 && ldg(S.qw_in + (a - ncs))) {
-      for (int iq = lane; iq < nq; iq += QNB_WAVE) {
-        const int q = ldg(S.iqseq + iq);
-        const R qx = (R)(ldg(x + 3 * q) - xr0), qy = (R)(ldg(x + 3 * q + 1) - xr1), qz = (R)(ldg(x + 3 * q + 2) - xr2);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          // u = x_q - x_i, the partner-minus-receiver convention of the pair terms
-          const R ux = qx - o[k][0], uy = qy - o[k][1], uz = qz - o[k][2];
-          R r2, r;
-          rinv(ux * ux + uy * uy + uz * uz, r2, r);
-          R dvs = R(0);
-          if (k == 0 || !S.spc) {
-            const R r6 = r2 * r2 * r2;
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-              if (s >= ns) break;
-              const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
-              dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)ldg(S.qwA + t), (R)ldg(S.qwB + t), (R)ldg(S.qwEl + t),
-                                         (R)ldg(S.qwSc + t),
-                                         (R)S.lambda[s], eq[4 * s + 2], eq[4 * s + 3], want_e);
-            }
-          } else {
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-              if (s >= ns) break;
-              const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
-              const R Vel = (R)ldg(S.qwEl + t) * r;  // nbe_qspc (nonbonded.f90:226-243)
-              if (want_e) eq[4 * s + 2] += (double)Vel;
-              dvs += -r2 * Vel * (R)S.lambda[s];
-            }
-          }
-          f[k][0] += ux * dvs;
-          f[k][1] += uy * dvs;
-          f[k][2] += uz * dvs;
-        }
-      }
-    }
-    // reduce (the per-atom LRF Taylor and restraint terms follow in k_atom_terms)
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const double fx = wave_sum((double)f[k][0]), fy = wave_sum((double)f[k][1]), fz = wave_sum((double)f[k][2]);
-      if (lane == k) {
-        const int i = ra + k;
-        d[3 * i] = -fx;
-        d[3 * i + 1] = -fy;
-        d[3 * i + 2] = -fz;
-      }
-    }
-  } else {
-    // ------------------------------------------------ solute receiver
-    const bool qin = (mask & T_QP) && nq > 0 && ldg(S.qp_in + a);
-    constexpr int CH = sizeof(R) == 4 ? FC_CH : FC_CH / 2;
-    for (int c0 = 0; c0 < na; c0 += CH) {
-      const int nk = min(CH, na - c0);
-      // receiver atoms of this sweep in LDS: index (-1: takes no pair terms)
-      // and offset from the reference atom
-      if (lane < nk) {
-        const int i = ldg(S.cgpatom + (g0 + c0 + lane));
-        s_ia[lane] = (!aexcl && ldg(S.iqatom + i) == 0) ? i : -1;
-        s_o[lane][0] = (R)(ldg(x + (3 * i)) - xr0);
-        s_o[lane][1] = (R)(ldg(x + (3 * i + 1)) - xr1);
-        s_o[lane][2] = (R)(ldg(x + (3 * i + 2)) - xr2);
-        s_rt[lane] = ldg(S.iac + i);
-        s_rx[lane] = ldg(S.exmask + i);
-        s_rm[lane] = ldg(S.m14mask + i);
-        s_rq[lane] = ldg(S.crg + i);
-        // the atom's solute-water parameters against the 3 water sites
-        if (S.nwat > 0)
-          for (int kb = 0; kb < 3; kb++) {
-            const size_t tt = (size_t)i + (size_t)nsol * kb;
-            s_pw[lane][kb][0] = (R)ldg(S.pwA + tt);
-            s_pw[lane][kb][1] = (R)ldg(S.pwB + tt);
-            s_pw[lane][kb][2] = (R)ldg(S.pwEl + tt);
-          }
-      }
-      __builtin_amdgcn_wave_barrier();
-      R f[CH][3];
-#pragma unroll
-      for (int k = 0; k < CH; k++) f[k][0] = f[k][1] = f[k][2] = R(0);
-      if (!aexcl) {
-        for (int p0 = 0, ng = 0; p0 < nn; p0 += ng) {
-          const int total = stage_partners<R>(S, a, row, p0, nn, mask & T_PP, mask & T_PW, xr0, xr1, xr2,
-                                              s_pj, s_pd, s_pt, &ng);
+      // ---- partner solute groups, one staged atom per lane
+      if (mask & T_PP) {
+        for (int p0 = 0, ng = 0; p0 < nsr; p0 += ng) {
+          const int total = stage_partners<R, CAP>(S, a, row, p0, nsr, 1u, 0u, xr0, xr1, xr2, s_pj, s_pd, s_pt, &ng);
           if (ng == 0) break;
           for (int t = lane; t < total; t += QNB_WAVE) {
-            const int tag = s_pt[t], j = s_pj[t], kb = tag & 3;
+            const int tag = s_pt[t], j = s_pj[t];
             if (tag & FT_Q) continue;
-            const bool b_sol = (tag & FT_SOL) != 0;
             const bool acc = want_e && (tag & FT_OWN);
             const R dx = s_pd[t][0], dy = s_pd[t][1], dz = s_pd[t][2];
-            // the partner's pair energies are summed in R over the receiver
-            // atoms, then added in FP64
             R pel = R(0), pvdw = R(0);
-            PPAtom Q{j, 0, 0u, 0u, 0.0};
-            if (b_sol && lds_types) {
-              Q.t = ldg(S.iac + j);
-              Q.xm = ldg(S.exmask + j);
-              Q.m14 = ldg(S.m14mask + j);
-              Q.q = ldg(S.crg + j);
-            }
-            // receiver atoms k < nk with i >= 0 (no early exits, so the loop
-            // unrolls and f stays in registers)
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-              const int i = s_ia[k];
-              bool ok = k < nk && i >= 0;
-              R A = R(0), B = R(0), el = R(0);
-              if (ok) {
-                if (b_sol) {
-                  double Ad, Bd, eld;
-                  if (lds_types) {
-                    const PPAtom P{i, s_rt[k], s_rx[k], s_rm[k], s_rq[k]};
-                    ok = pp_param_fast(S, P, Q, s_avdw, s_bvdw, s_ljc, Ad, Bd, eld);
-                  } else {
-                    ok = pp_param(S, i, j, Ad, Bd, eld);
-                  }
-                  A = (R)Ad;
-                  B = (R)Bd;
-                  el = (R)eld;
-                } else {
-                  A = s_pw[k][kb][0];
-                  B = s_pw[k][kb][1];
-                  el = s_pw[k][kb][2];
-                }
+            if (fast && (tag & FT_FAR)) {
+              // no exclusion or 1-4 pair possible: A, B, qq from the factors
+              R pA, pB, pQ;
+              if constexpr (sizeof(R) == 4) {
+                pA = ldg(S.ppf + 4 * j);
+                pB = ldg(S.ppf + 4 * j + 1);
+                pQ = ldg(S.ppf + 4 * j + 2);
+              } else {
+                pA = ldg(S.ppd + 4 * j);
+                pB = ldg(S.ppd + 4 * j + 1);
+                pQ = ldg(S.ppd + 4 * j + 2);
               }
-              if (ok) {
-                const R ux = dx - s_o[k][0], uy = dy - s_o[k][1], uz = dz - s_o[k][2];
-                const R dv = PairMath<R>::nbe(ux, uy, uz, A, B, el, pel, pvdw, b_sol ? acc : want_e);
+#pragma unroll
+              for (int k = 0; k < CH; k++) {
+                if (!rok[k]) continue;
+                const R A = rA[k] * pA, B = rB[k] * pB, el = rQ[k] * pQ;
+                // pp_param's `set` test (QREAL_EPS, src/sizes.f90:50); in FP32
+                // a pair of zero parameters adds exactly zero, so it is skipped
+                // only on the FP64 path
+                if constexpr (sizeof(R) == 8)
+                  if (!(fabs(A) > R(1.0e-10) || fabs(B) > R(1.0e-10) || fabs(el) > R(1.0e-10))) continue;
+                const R ux = dx - ro[k][0], uy = dy - ro[k][1], uz = dz - ro[k][2];
+                const R dv = PairMath<R>::nbe(ux, uy, uz, A, B, el, pel, pvdw, acc);
+                f[k][0] += ux * dv;
+                f[k][1] += uy * dv;
+                f[k][2] += uz * dv;
+              }
+            } else {
+              PPAtom Q{j, 0, 0u, 0u, 0.0};
+              if (lds_types) {
+                Q.t = ldg(S.iac + j);
+                Q.xm = ldg(S.exmask + j);
+                Q.m14 = ldg(S.m14mask + j);
+                Q.q = ldg(S.crg + j);
+              }
+#pragma unroll
+              for (int k = 0; k < CH; k++) {
+                if (!rok[k]) continue;
+                const int i = s_ia[k];
+                double Ad, Bd, eld;
+                bool ok;
+                if (lds_types) {
+                  const PPAtom P{i, s_rt[k], s_rx[k], s_rm[k], s_rq[k]};
+                  ok = pp_param_fast(S, P, Q, s_avdw, s_bvdw, s_ljc, Ad, Bd, eld);
+                } else {
+                  ok = pp_param(S, i, j, Ad, Bd, eld);
+                }
+                if (!ok) continue;
+                const R ux = dx - ro[k][0], uy = dy - ro[k][1], uz = dz - ro[k][2];
+                const R dv = PairMath<R>::nbe(ux, uy, uz, (R)Ad, (R)Bd, (R)eld, pel, pvdw, acc);
                 f[k][0] += ux * dv;
                 f[k][1] += uy * dv;
                 f[k][2] += uz * dv;
               }
             }
-            if (want_e) {
-              const double de = (double)pel, dw = (double)pvdw;
-              e[E_PP_EL] += b_sol ? de : 0.0;
-              e[E_PP_VDW] += b_sol ? dw : 0.0;
-              e[E_PW_EL] += b_sol ? 0.0 : de;
-              e[E_PW_VDW] += b_sol ? 0.0 : dw;
+            if (acc) {
+              e_ppel += (double)pel;
+              e_ppvdw += (double)pvdw;
             }
           }
           __builtin_amdgcn_wave_barrier();
         }
-        // pairs inside the group (ig == jgr, i < j, nonbondene.f90:1610):
-        // lane (g, kq) = (lane / nk, lane % nk) takes receiver atom kq against
-        // group atoms q = g, g + G, ... (G = 64 / nk); an atom of this sweep
-        // has its data in LDS, others (groups larger than one sweep) are read
-        // from memory
-        if (mask & T_PP) {
-          const int G = QNB_WAVE / nk, kq = lane % nk, g = lane / nk;
-          const int i = s_ia[kq];
-          R fq[3] = {R(0), R(0), R(0)};
-          if (g < G && i >= 0) {
-            const R ox = s_o[kq][0], oy = s_o[kq][1], oz = s_o[kq][2];
-            for (int q = g; q < na; q += G) {
-              const int ql = q - c0;
-              const bool in_sweep = ql >= 0 && ql < nk;
-              int j;
-              R dx, dy, dz;
-              if (in_sweep) {
-                j = s_ia[ql];  // -1: a Q-atom, whose pairs belong to the Q kernels
-                dx = s_o[ql][0];
-                dy = s_o[ql][1];
-                dz = s_o[ql][2];
-              } else {
-                j = ldg(S.cgpatom + (g0 + q));
-                if (ldg(S.iqatom + j) != 0) continue;
-                dx = (R)(ldg(x + 3 * j) - xr0);
-                dy = (R)(ldg(x + 3 * j + 1) - xr1);
-                dz = (R)(ldg(x + 3 * j + 2) - xr2);
-              }
-              if (j < 0 || j == i) continue;
-              double A, B, el;
-              bool ok;
-              if (in_sweep && lds_types) {
-                const PPAtom P{i, s_rt[kq], s_rx[kq], s_rm[kq], s_rq[kq]};
-                const PPAtom Q{j, s_rt[ql], s_rx[ql], s_rm[ql], s_rq[ql]};
-                ok = pp_param_fast(S, P, Q, s_avdw, s_bvdw, s_ljc, A, B, el);
-              } else {
-                ok = pp_param(S, i, j, A, B, el);
-              }
-              if (!ok) continue;
-              const R ux = dx - ox, uy = dy - oy, uz = dz - oz;
-              const R dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e[E_PP_EL], e[E_PP_VDW], want_e && i < j);
-              fq[0] += ux * dv;
-              fq[1] += uy * dv;
-              fq[2] += uz * dv;
-            }
-          }
+      }
+      // ---- partner waters: lane l takes site kb = l % 3 of water l / 3 of the round
+      if ((mask & T_PW) && nsr < nn && lane < 3 * (QNB_WAVE / 3)) {
+        const int kb = lane % 3;
+        R pw[CH][3];
 #pragma unroll
-          for (int k = 0; k < CH; k++)
-            if (k == kq) {
-              f[k][0] += fq[0];
-              f[k][1] += fq[1];
-              f[k][2] += fq[2];
-            }
+        for (int k = 0; k < CH; k++) {
+          pw[k][0] = rok[k] ? s_pw[k][kb][0] : R(0);
+          pw[k][1] = rok[k] ? s_pw[k][kb][1] : R(0);
+          pw[k][2] = rok[k] ? s_pw[k][kb][2] : R(0);
+        }
+        for (int p = nsr + lane / 3; p < nn; p += QNB_WAVE / 3) {
+          const int b = ldg(row + p);
+          const int j = nsol + sa * (b - ncs) + kb;
+          const R dx = (R)(ldg(x + 3 * j) - xr0), dy = (R)(ldg(x + 3 * j + 1) - xr1),
+                  dz = (R)(ldg(x + 3 * j + 2) - xr2);
+          R pel = R(0), pvdw = R(0);
+#pragma unroll
+          for (int k = 0; k < CH; k++) {
+            if (!rok[k]) continue;
+            const R ux = dx - ro[k][0], uy = dy - ro[k][1], uz = dz - ro[k][2];
+            const R dv = PairMath<R>::nbe(ux, uy, uz, pw[k][0], pw[k][1], pw[k][2], pel, pvdw, want_e);
+            f[k][0] += ux * dv;
+            f[k][1] += uy * dv;
+            f[k][2] += uz * dv;
+          }
+          if (want_e) {
+            e_pwel += (double)pel;
+            e_pwvdw += (double)pvdw;
+          }
         }
       }
-      // Q-atom - solute atom, receiver side (nonbond_qp, nonbondene.f90:5157-5230):
-      // lane (g, kq) = (lane / nk, lane % nk) takes receiver atom kq against
-      // Q-atoms g, g + G, ... (G = 64 / nk lanes per receiver atom), and its
-      // sum joins f[kq] before the reduction
-      if (qin) {
-        const R *qtA = qp_tab<R>(S.qpA, S.qpA_f), *qtB = qp_tab<R>(S.qpB, S.qpB_f);
-        const R *qtE = qp_tab<R>(S.qpEl, S.qpEl_f), *qtS = qp_tab<R>(S.qpSc, S.qpSc_f);
+      __builtin_amdgcn_wave_barrier();
+      // ---- pairs inside the group: lane (g, kq) = (lane / nk, lane % nk)
+      // takes receiver atom kq against group atoms q = g, g + G, ...
+      // (G = 64 / nk); an atom of this sweep has its data in LDS, others
+      // (groups larger than one sweep) are read from memory
+      if (mask & T_PP) {
         const int G = QNB_WAVE / nk, kq = lane % nk, g = lane / nk;
         const int i = s_ia[kq];
         R fq[3] = {R(0), R(0), R(0)};
-        if (g < G && i >= 0 && !ldg(S.qconn3 + i)) {
+        if (g < G && i >= 0) {
           const R ox = s_o[kq][0], oy = s_o[kq][1], oz = s_o[kq][2];
-          for (int iq = g; iq < nq; iq += G) {
-            const int q = ldg(S.iqseq + iq);
-            const R ux = (R)(ldg(x + 3 * q) - xr0) - ox, uy = (R)(ldg(x + 3 * q + 1) - xr1) - oy,
-                    uz = (R)(ldg(x + 3 * q + 2) - xr2) - oz;
-            R r2, r;
-            rinv(ux * ux + uy * uy + uz * uz, r2, r);
-            const R r6 = r2 * r2 * r2;
-            R dvs = R(0);
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-              if (s >= ns) break;
-              const size_t t = (size_t)i + (size_t)nsol * ((size_t)iq + (size_t)nq * s);
-              dvs += PairMath<R>::nbe_qx(r2, r, r6, ldg(qtA + t), ldg(qtB + t), ldg(qtE + t), ldg(qtS + t),
-                                         (R)S.lambda[s], eq[4 * s + 0], eq[4 * s + 1], want_e);
+          for (int q = g; q < na; q += G) {
+            const int ql = q - c0;
+            const bool in_sweep = ql >= 0 && ql < nk;
+            int j;
+            R dx, dy, dz;
+            if (in_sweep) {
+              j = s_ia[ql];  // -1: a Q-atom, whose pairs belong to the Q kernels
+              dx = s_o[ql][0];
+              dy = s_o[ql][1];
+              dz = s_o[ql][2];
+            } else {
+              j = ldg(S.cgpatom + (g0 + q));
+              if (ldg(S.iqatom + j) != 0) continue;
+              dx = (R)(ldg(x + 3 * j) - xr0);
+              dy = (R)(ldg(x + 3 * j + 1) - xr1);
+              dz = (R)(ldg(x + 3 * j + 2) - xr2);
             }
-            fq[0] += ux * dvs;
-            fq[1] += uy * dvs;
-            fq[2] += uz * dvs;
+            if (j < 0 || j == i) continue;
+            double A, B, el;
+            bool ok;
+            if (in_sweep && lds_types) {
+              const PPAtom P{i, s_rt[kq], s_rx[kq], s_rm[kq], s_rq[kq]};
+              const PPAtom Q{j, s_rt[ql], s_rx[ql], s_rm[ql], s_rq[ql]};
+              ok = pp_param_fast(S, P, Q, s_avdw, s_bvdw, s_ljc, A, B, el);
+            } else {
+              ok = pp_param(S, i, j, A, B, el);
+            }
+            if (!ok) continue;
+            const R ux = dx - ox, uy = dy - oy, uz = dz - oz;
+            const R dv = PairMath<R>::nbe(ux, uy, uz, (R)A, (R)B, (R)el, e_ppel, e_ppvdw, want_e && i < j);
+            fq[0] += ux * dv;
+            fq[1] += uy * dv;
+            fq[2] += uz * dv;
           }
         }
 #pragma unroll
@@ -980,49 +903,290 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_forces(const DevSys
             f[k][2] += fq[2];
           }
       }
-      // reduce (the per-atom LRF Taylor term follows in k_atom_terms)
+    }
+    // ---- Q-atom - solute atom, receiver side: lane (g, kq) = (lane / nk,
+    // lane % nk) takes receiver atom kq against Q-atoms g, g + G, ...
+    if (qin) {
+      const R *qtA = qp_tab<R>(S.qpA, S.qpA_f), *qtB = qp_tab<R>(S.qpB, S.qpB_f);
+      const R *qtE = qp_tab<R>(S.qpEl, S.qpEl_f), *qtS = qp_tab<R>(S.qpSc, S.qpSc_f);
+      const int G = QNB_WAVE / nk, kq = lane % nk, g = lane / nk;
+      const int i = s_ia[kq];
+      R fq[3] = {R(0), R(0), R(0)};
+      if (g < G && i >= 0 && !ldg(S.qconn3 + i)) {
+        const R ox = s_o[kq][0], oy = s_o[kq][1], oz = s_o[kq][2];
+        for (int iq = g; iq < nq; iq += G) {
+          const int q = ldg(S.iqseq + iq);
+          const R ux = (R)(ldg(x + 3 * q) - xr0) - ox, uy = (R)(ldg(x + 3 * q + 1) - xr1) - oy,
+                  uz = (R)(ldg(x + 3 * q + 2) - xr2) - oz;
+          R r2, r;
+          rinv(ux * ux + uy * uy + uz * uz, r2, r);
+          const R r6 = r2 * r2 * r2;
+          R dvs = R(0);
 #pragma unroll
-      for (int k = 0; k < CH; k++) {
-        if (k >= nk) continue;
-        const double fx = wave_sum((double)f[k][0]), fy = wave_sum((double)f[k][1]), fz = wave_sum((double)f[k][2]);
-        if (lane == k) {
-          const int i = ldg(S.cgpatom + (g0 + c0 + k));
-          d[3 * i] = -fx;
-          d[3 * i + 1] = -fy;
-          d[3 * i + 2] = -fz;
+          for (int s = 0; s < NS; s++) {
+            if (s >= ns) break;
+            const size_t t = (size_t)i + (size_t)nsol * ((size_t)iq + (size_t)nq * s);
+            dvs += PairMath<R>::nbe_qx(r2, r, r6, ldg(qtA + t), ldg(qtB + t), ldg(qtE + t), ldg(qtS + t),
+                                       (R)S.lambda[s], eq[2 * s], eq[2 * s + 1], want_e);
+          }
+          fq[0] += ux * dvs;
+          fq[1] += uy * dvs;
+          fq[2] += uz * dvs;
         }
       }
-      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = 0; k < CH; k++)
+        if (k == kq) {
+          f[k][0] += fq[0];
+          f[k][1] += fq[1];
+          f[k][2] += fq[2];
+        }
     }
+    // reduce (the per-atom LRF Taylor term follows in k_atom_terms)
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+      if (k >= nk) continue;
+      const double fx = wave_sum_all((double)f[k][0]), fy = wave_sum_all((double)f[k][1]), fz = wave_sum_all((double)f[k][2]);
+      if (lane == k) {
+        const int i = ldg(S.cgpatom + (g0 + c0 + k));
+        d[3 * i] = -fx;
+        d[3 * i + 1] = -fy;
+        d[3 * i + 2] = -fz;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
   }
-
-  // energy partials of this wave, on the sampled steps: only the terms this
-  // receiver class accumulates are reduced (waters: ww and the Q-water
-  // pairs; solute groups: pp, pw and the Q-solute pairs), the rest are zero
   if (!want_e) return;
   double* ep = S.epart + (size_t)a * QNB_NE;
-  constexpr int k0 = WAT ? E_WW_EL : E_PP_EL, k1 = WAT ? E_WW_VDW : E_PW_VDW;
-  constexpr int q0 = WAT ? 2 : 0;  // eq[4 s + q0 .. q0 + 1]
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    double v = 0.0;
-    if (k >= k0 && k <= k1) v = wave_sum(e[k]);
-    if (lane == 0) ep[k] = v;
-  }
+  const double v0 = wave_sum_all(e_ppel), v1 = wave_sum_all(e_ppvdw), v2 = wave_sum_all(e_pwel), v3 = wave_sum_all(e_pwvdw);
+  if (lane == 0)
+    for (int k = 0; k < 9; k++)
+      ep[k] = k == E_PP_EL ? v0 : (k == E_PP_VDW ? v1 : (k == E_PW_EL ? v2 : (k == E_PW_VDW ? v3 : 0.0)));
 #pragma unroll
   for (int s = 0; s < QNB_MAXST; s++) {
-    double v0 = 0.0, v1 = 0.0;
+    double q0 = 0.0, q1 = 0.0;
     if (s < NS) {
-      v0 = wave_sum(eq[4 * (s < NS ? s : 0) + q0]);
-      v1 = wave_sum(eq[4 * (s < NS ? s : 0) + q0 + 1]);
+      q0 = wave_sum_all(eq[2 * (s < NS ? s : 0)]);
+      q1 = wave_sum_all(eq[2 * (s < NS ? s : 0) + 1]);
     }
     if (lane == 0) {
       ep[E_Q0 + 6 * s + 0] = 0.0;
       ep[E_Q0 + 6 * s + 1] = 0.0;
-      ep[E_Q0 + 6 * s + 2] = WAT ? 0.0 : v0;
-      ep[E_Q0 + 6 * s + 3] = WAT ? 0.0 : v1;
-      ep[E_Q0 + 6 * s + 4] = WAT ? v0 : 0.0;
-      ep[E_Q0 + 6 * s + 5] = WAT ? v1 : 0.0;
+      ep[E_Q0 + 6 * s + 2] = q0;
+      ep[E_Q0 + 6 * s + 3] = q1;
+      ep[E_Q0 + 6 * s + 4] = 0.0;
+      ep[E_Q0 + 6 * s + 5] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Water receivers (rigid 3-site solvent): one wavefront per water a.  Its
+// neighbour row is sorted, so the solute groups (indices below ncgp_solute)
+// come first, then the waters.  Solute partners go through the staged atom
+// expansion, one partner atom per lane against the three receiver atoms with
+// the pw parameters (nonbond_pw, nonbondene.f90:4856-4916; energy counted on
+// the solute side).  Partner waters need no staging: their atoms are
+// consecutive, so in each round lane l takes atom l % 3 of the row's water
+// (l / 3) of the round, 21 waters per round (lane 63 idle), reading the
+// position straight from x, and evaluates it against the three receiver
+// atoms (nonbond_ww_spc nonbondene.f90:5886-5975: O-O LJ + Coulomb, the
+// other pairs Coulomb only; nonbond_ww 5509-5573 for other solvents).  Then
+// the receiver side of the Q-atom - water pairs.  Writes d of the three
+// atoms, and the wave's ww and qw energy partials on the sampled steps.
+template <typename R, int NS, bool E, int CAP>
+__global__ __launch_bounds__(64 * FC_WPB) void k_wforces(const DevSys* __restrict__ systems, uint32_t mask,
+                                                         int imap, int nb, int ninst) {
+  int inst, blk;
+  if (!map_block(imap, nb, ninst, inst, blk)) return;
+  const DevSys& S = systems[inst];
+  const int wib = threadIdx.x / QNB_WAVE;
+  const int a = __builtin_amdgcn_readfirstlane(S.ncgp_solute + blk * (int)(blockDim.x / QNB_WAVE) + wib);
+  if (a >= S.ncgp) return;
+  const int lane = threadIdx.x % QNB_WAVE;
+  extern __shared__ __align__(16) unsigned char wf_smem[];
+  unsigned char* wbase = wf_smem + (size_t)wib * wf_wave_bytes<R, CAP>();
+  R(*s_pd)[3] = reinterpret_cast<R(*)[3]>(wbase);
+  int* s_pj = reinterpret_cast<int*>(wbase + sizeof(R) * 3 * CAP);
+  int* s_pt = s_pj + CAP;
+  const double* __restrict__ x = S.x;
+  const auto d = gpm(S.d);
+  const int nn = min(ldg(S.nnbr + a), S.cap);
+  const int* __restrict__ row = S.nbr + (size_t)a * S.cap;
+  const bool want_e = E && want_energies(S);
+  const int nsol = S.nat_solute, sa = S.sa, ncs = S.ncgp_solute;
+  const int ra = nsol + sa * (a - ncs);
+  const bool aexcl = ldg(S.excl + ra);
+  const int nq = S.nqat, ns = S.nstates;
+  double e_el = 0.0, e_vdw = 0.0;
+  double eq[2 * NS];  // per state: qw_el, qw_vdw
+#pragma unroll
+  for (int k = 0; k < 2 * NS; k++) eq[k] = 0.0;
+  const double xr0 = ldg(x + 3 * ra), xr1 = ldg(x + 3 * ra + 1), xr2 = ldg(x + 3 * ra + 2);
+  R o[3][3], f[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    o[k][0] = (R)(ldg(x + (3 * (ra + k))) - xr0);
+    o[k][1] = (R)(ldg(x + (3 * (ra + k) + 1)) - xr1);
+    o[k][2] = (R)(ldg(x + (3 * (ra + k) + 2)) - xr2);
+    f[k][0] = f[k][1] = f[k][2] = R(0);
+  }
+  if (!aexcl) {
+    // length of the row's solute prefix
+    int nsr = 0;
+    for (int p0 = 0; p0 < nn; p0 += QNB_WAVE) {
+      const int p = p0 + lane;
+      const uint64_t m = __ballot(p < nn && ldg(row + p) < ncs);
+      nsr += __popcll(m);
+      if (m != ~0ull) break;
+    }
+    // solute partners, one staged atom per lane
+    if (mask & T_PW) {
+      for (int p0 = 0, ng = 0; p0 < nsr; p0 += ng) {
+        const int total = stage_partners<R, CAP>(S, a, row, p0, nsr, 1u, 0u, xr0, xr1, xr2, s_pj, s_pd, s_pt, &ng);
+        if (ng == 0) break;
+        for (int t = lane; t < total; t += QNB_WAVE) {
+          if (s_pt[t] & FT_Q) continue;
+          const int j = s_pj[t];
+          const R dx = s_pd[t][0], dy = s_pd[t][1], dz = s_pd[t][2];
+          R pw[9];
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            if constexpr (sizeof(R) == 4) {
+              pw[3 * k] = ldg(S.pw_f + PW_REC * j + 3 * k);
+              pw[3 * k + 1] = ldg(S.pw_f + PW_REC * j + 3 * k + 1);
+              pw[3 * k + 2] = ldg(S.pw_f + PW_REC * j + 3 * k + 2);
+            } else {
+              const size_t tt = (size_t)j + (size_t)nsol * k;
+              pw[3 * k] = (R)ldg(S.pwA + tt);
+              pw[3 * k + 1] = (R)ldg(S.pwB + tt);
+              pw[3 * k + 2] = (R)ldg(S.pwEl + tt);
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            const R ux = dx - o[k][0], uy = dy - o[k][1], uz = dz - o[k][2];
+            R ee = R(0), ev = R(0);
+            const R dv = PairMath<R>::nbe(ux, uy, uz, pw[3 * k], pw[3 * k + 1], pw[3 * k + 2], ee, ev, false);
+            f[k][0] += ux * dv;
+            f[k][1] += uy * dv;
+            f[k][2] += uz * dv;
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    // partner waters: lane l takes atom kb = l % 3 of water (l / 3) of the round
+    if (mask & T_WW) {
+      R wel[3], wA = R(0), wB = R(0), gA[3], gB[3];
+      const int kb = lane % 3, pw0 = lane / 3;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        wel[k] = (R)ldg(S.wwEl + k + 3 * kb);  // receiver atom k with partner atom kb
+        gA[k] = (R)ldg(S.wwA + k + 3 * kb);
+        gB[k] = (R)ldg(S.wwB + k + 3 * kb);
+      }
+      if (S.spc && kb == 0) {
+        // SPC: LJ for O-O only; the other lanes take the same path with zero
+        // LJ parameters, so the k = 0 pair does not branch on kb
+        wA = (R)ldg(S.wwA);
+        wB = (R)ldg(S.wwB);
+      }
+      for (int p = nsr + pw0; p < nn; p += QNB_WAVE / 3) {
+        if (lane >= 3 * (QNB_WAVE / 3)) break;
+        const int b = ldg(row + p);
+        const int j = nsol + sa * (b - ncs) + kb;
+        const R dx = (R)(ldg(x + 3 * j) - xr0), dy = (R)(ldg(x + 3 * j + 1) - xr1), dz = (R)(ldg(x + 3 * j + 2) - xr2);
+        const bool acc = want_e && owns(a, b);
+        R pel = R(0), pvdw = R(0);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const R ux = dx - o[k][0], uy = dy - o[k][1], uz = dz - o[k][2];
+          R dv;
+          if (S.spc) {
+            if (k == 0) dv = PairMath<R>::nbe(ux, uy, uz, wA, wB, wel[0], pel, pvdw, acc);
+            else dv = PairMath<R>::nbe_spc(ux, uy, uz, wel[k], pel, acc);
+          } else {
+            dv = PairMath<R>::nbe(ux, uy, uz, gA[k], gB[k], wel[k], pel, pvdw, acc);
+          }
+          f[k][0] += ux * dv;
+          f[k][1] += uy * dv;
+          f[k][2] += uz * dv;
+        }
+        if (acc) {
+          e_el += (double)pel;
+          e_vdw += (double)pvdw;
+        }
+      }
+    }
+  }
+  // Q-atom - water, receiver side (nonbond_qw_spc nonbondene.f90:5658-5767;
+  // nonbond_qw 5339-5418): lane per Q-atom, the water's three sites in turn
+  // (an SPC hydrogen takes the Coulomb-only nbe_qspc, nonbonded.f90:226-243)
+  if ((mask & T_QW) && nq > 0 && ldg(S.qw_in + (a - ncs))) {
+    for (int iq = lane; iq < nq; iq += QNB_WAVE) {
+      const int q = ldg(S.iqseq + iq);
+      const R qx = (R)(ldg(x + 3 * q) - xr0), qy = (R)(ldg(x + 3 * q + 1) - xr1), qz = (R)(ldg(x + 3 * q + 2) - xr2);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const R ux = qx - o[k][0], uy = qy - o[k][1], uz = qz - o[k][2];
+        R r2, r;
+        rinv(ux * ux + uy * uy + uz * uz, r2, r);
+        R dvs = R(0);
+        if (k == 0 || !S.spc) {
+          const R r6 = r2 * r2 * r2;
+#pragma unroll
+          for (int s = 0; s < NS; s++) {
+            if (s >= ns) break;
+            const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
+            dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)ldg(S.qwA + t), (R)ldg(S.qwB + t), (R)ldg(S.qwEl + t),
+                                       (R)ldg(S.qwSc + t), (R)S.lambda[s], eq[2 * s], eq[2 * s + 1], want_e);
+          }
+        } else {
+#pragma unroll
+          for (int s = 0; s < NS; s++) {
+            if (s >= ns) break;
+            const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * s);
+            const R Vel = (R)ldg(S.qwEl + t) * r;
+            if (want_e) eq[2 * s] += (double)Vel;
+            dvs += -r2 * Vel * (R)S.lambda[s];
+          }
+        }
+        f[k][0] += ux * dvs;
+        f[k][1] += uy * dvs;
+        f[k][2] += uz * dvs;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double fx = wave_sum_all((double)f[k][0]), fy = wave_sum_all((double)f[k][1]), fz = wave_sum_all((double)f[k][2]);
+    if (lane == k) {
+      const int i = ra + k;
+      d[3 * i] = -fx;
+      d[3 * i + 1] = -fy;
+      d[3 * i + 2] = -fz;
+    }
+  }
+  if (!want_e) return;
+  double* ep = S.epart + (size_t)a * QNB_NE;
+  const double vel = wave_sum_all(e_el), vvdw = wave_sum_all(e_vdw);
+  if (lane == 0)
+    for (int k = 0; k < 9; k++) ep[k] = k == E_WW_EL ? vel : (k == E_WW_VDW ? vvdw : 0.0);
+#pragma unroll
+  for (int s = 0; s < QNB_MAXST; s++) {
+    double v0 = 0.0, v1 = 0.0;
+    if (s < NS) {
+      v0 = wave_sum_all(eq[2 * (s < NS ? s : 0)]);
+      v1 = wave_sum_all(eq[2 * (s < NS ? s : 0) + 1]);
+    }
+    if (lane == 0) {
+      ep[E_Q0 + 6 * s + 0] = 0.0;
+      ep[E_Q0 + 6 * s + 1] = 0.0;
+      ep[E_Q0 + 6 * s + 2] = 0.0;
+      ep[E_Q0 + 6 * s + 3] = 0.0;
+      ep[E_Q0 + 6 * s + 4] = v0;
+      ep[E_Q0 + 6 * s + 5] = v1;
     }
   }
 }
@@ -1558,11 +1722,15 @@ template __global__ void k_lists_lrf<float, false>(const DevSys*);
 template __global__ void k_lists_lrf<double, true>(const DevSys*);
 template __global__ void k_lists_lrf<float, true>(const DevSys*);
 #define QNB_INST(R, NS)                                                            \
-  template __global__ void k_forces<R, NS, true, true>(const DevSys*, uint32_t, const double*); \
-  template __global__ void k_forces<R, NS, false, true>(const DevSys*, uint32_t, const double*); \
-  template __global__ void k_forces<R, NS, true, false>(const DevSys*, uint32_t, const double*); \
-  template __global__ void k_forces<R, NS, false, false>(const DevSys*, uint32_t, const double*); \
-  template __global__ void k_qforces<R, NS>(const DevSys*, uint32_t, int);
+  template __global__ void k_qforces<R, NS>(const DevSys*, uint32_t, int);                       \
+  template __global__ void k_sforces<R, NS, true, WF_CAP_SMALL>(const DevSys*, uint32_t, int, int, int); \
+  template __global__ void k_sforces<R, NS, false, WF_CAP_SMALL>(const DevSys*, uint32_t, int, int, int); \
+  template __global__ void k_sforces<R, NS, true, FC_CAP>(const DevSys*, uint32_t, int, int, int);       \
+  template __global__ void k_sforces<R, NS, false, FC_CAP>(const DevSys*, uint32_t, int, int, int);      \
+  template __global__ void k_wforces<R, NS, true, WF_CAP_SMALL>(const DevSys*, uint32_t, int, int, int); \
+  template __global__ void k_wforces<R, NS, false, WF_CAP_SMALL>(const DevSys*, uint32_t, int, int, int); \
+  template __global__ void k_wforces<R, NS, true, FC_CAP>(const DevSys*, uint32_t, int, int, int);       \
+  template __global__ void k_wforces<R, NS, false, FC_CAP>(const DevSys*, uint32_t, int, int, int);
 QNB_INST(double, 1)
 QNB_INST(double, 2)
 QNB_INST(double, 4)
