@@ -143,7 +143,10 @@ program qnb_dump
   call wd(u, 'avdw', 3*natyps, (/((iaclib(i)%avdw(k), k=1,3), i=1,natyps)/), 2, (/3, natyps/))
   call wd(u, 'bvdw', 3*natyps, (/((iaclib(i)%bvdw(k), k=1,3), i=1,natyps)/), 2, (/3, natyps/))
   call wd(u, 'type_mass', natyps, iaclib(1:natyps)%mass, 1, (/natyps/))
-  call wi(u, 'ljcod', natyps*natyps, reshape(ljcod(1:natyps,1:natyps), (/natyps*natyps/)), 2, (/natyps, natyps/))
+  ! ljcod is allocated num_atyp = maxval(iac) square (src/simprep.f90:4555-4557),
+  ! which can be fewer than the topology's types: write what is allocated
+  n = size(ljcod, 1)
+  call wi(u, 'ljcod', n*n, reshape(ljcod(1:n,1:n), (/n*n/)), 2, (/n, n/))
   if (nat_solute > 0) then
     n = max_nbr_range*nat_solute
     call wi(u, 'listex', n, reshape(merge(1, 0, listex(1:max_nbr_range,1:nat_solute)), (/n/)), &

@@ -443,7 +443,9 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
                 restraint_r0=np.array([r for _, _, r in rst]), restraint_k=20.0,
                 angles=np.array(angles, np.int32).reshape(-1, 3), angle_t0=math.radians(109.5), angle_k=80.0,
                 water_roh=TIP3P_ROH, water_rhh=2 * TIP3P_ROH * math.sin(math.radians(TIP3P_HOH) / 2),
-                chain_region=r_used, qcrg=qcrg, qA=qA, qB=qB, qctx=qctx)
+                chain_region=r_used, qcrg=qcrg, qA=qA, qB=qB, qctx=qctx,
+                crg_unscaled=crg_unscaled, type_mass=mass_t.copy(), nchain=nchain, nbead=nbead,
+                type_names=["OW", "HW"] + ["C%d" % (t + 1) for t in range(6)])
     # Q-atom LJ types (the FEP file's Q atom types): one per bead type that
     # carries a Q-atom, sigma/eps as the topology type's
     q_types = np.unique(iac[iqseq]) if nq else np.zeros(0, np.int32)

@@ -136,7 +136,12 @@ def from_golden(path):
     a["cgpatom"] = g["cgpatom"] - 1
     a["avdw"] = g["avdw"].ravel(order="F")
     a["bvdw"] = g["bvdw"].ravel(order="F")
-    a["ljcod"] = g["ljcod"].ravel(order="F")
+    # ljcod covers types 1..maxval(iac) (src/simprep.f90:4555-4560); types
+    # above carry no atoms and keep the default code 1
+    lj = np.ones((s.n["ntypes"], s.n["ntypes"]), g["ljcod"].dtype)
+    nl = g["ljcod"].shape[0]
+    lj[:nl, :nl] = g["ljcod"]
+    a["ljcod"] = lj.ravel(order="F")
     if nsol > 0:
         a["listex"] = g["listex"].ravel(order="F")
         a["list14"] = g["list14"].ravel(order="F")
