@@ -31,7 +31,7 @@ PEAK_FP64, PEAK_FP32, PEAK_HBM_GBPS = 78.6, 157.3, 8000.0
 # HBM bytes per launch of the dominant kernels, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (scaled by a copy kernel of known size) on this
 # workload; the file gives the commands
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_cfg2_51windows_mixed.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_51windows_10861atoms_mixed.md")
 # algorithmic FLOPs per unit (div and sqrt count as one)
 F_LRF = 89        # LRF atom-centre update as restated in the engine (20 independent components)
 F_TEST = 8        # group-pair distance test of the list sweep
@@ -73,10 +73,17 @@ def measured_traffic(nw, natom):
     return None
 
 
-def force_flops(counts, ns):
-    ww_mol = counts["ww"] // 9
-    return (ww_mol * (F_LJC + 8 * F_COUL) + (counts["pp"] + counts["pw"]) * F_LJC
-            + counts["qp"] * (F_Q_SHARED + F_Q_STATE * ns)
+def pair_flops(counts):
+    """Algorithmic FLOPs of the non-Q pairs (pp, pw, ww), each pair once: the
+    work of the receiver-side pass k_wforces + k_sforces (which evaluates
+    every pair from both ends)."""
+    return (counts["ww"] // 9) * (F_LJC + 8 * F_COUL) + (counts["pp"] + counts["pw"]) * F_LJC
+
+
+def q_flops(counts, ns):
+    """Algorithmic FLOPs of the Q-atom pairs (qp, qw: per-state energies and
+    forces; qq, qqp), each pair once."""
+    return (counts["qp"] * (F_Q_SHARED + F_Q_STATE * ns)
             + (counts["qw"] // 3) * ((F_Q_SHARED + F_Q_STATE * ns) + 2 * (F_Q_SHARED + F_QH_STATE * ns))
             + (counts["qq"] + counts["qqp"]) * (F_Q_SHARED + F_Q_STATE))
 
@@ -199,6 +206,52 @@ def cpu_baseline(s, ncycles=1):
     return steps / dt, dt, steps
 
 
+def reference_baseline(s, lambdas, nproc, nsteps):
+    """Stock Qdyn6 (serial; oracle/_ref/Qdyn6, built from the reference
+    sources) on nproc of the windows at once, one process per host core, each
+    `nsteps` MD steps of the same system from the same start: the topology,
+    FEP file and inputs come from q6_amd.qfiles, whose tables Qdyn6 reproduces
+    exactly.  Returns (ns/day summed over the processes from each one's own
+    main-loop timer, src/md.f90:1924-1926; the slowest main loop; the wall
+    time), or None when the build is absent or a run fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    from q6_amd import qfiles
+    exe = os.path.join(ROOT, "oracle", "_ref", "Qdyn6")
+    if not os.access(exe, os.X_OK):
+        return None
+    tmp = tempfile.mkdtemp(prefix="q6cpu_")
+    try:
+        qfiles.write_top(s, os.path.join(tmp, "sys.top"))
+        qfiles.write_fep(s, os.path.join(tmp, "sys.fep"), offdiag=[(1, 2, 1, 2, 80.0, 0.0)])
+        pick = np.unique(np.linspace(0, len(lambdas) - 1, nproc).round().astype(int))
+        runs = []
+        for k, w in enumerate(pick):
+            d = os.path.join(tmp, "w%d" % k)
+            os.makedirs(d)
+            qfiles.write_inp(s, os.path.join(d, "run.inp"), "../sys.top", "../sys.fep", nsteps, lambdas=lambdas[w],
+                             seed=101 + k)
+            runs.append(d)
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([exe, "run.inp"], cwd=d, stdout=open(os.path.join(d, "log"), "w"),
+                                  stderr=subprocess.STDOUT) for d in runs]
+        rcs = [p.wait() for p in procs]
+        wall = time.perf_counter() - t0
+        loops = []
+        for d, rc in zip(runs, rcs):
+            txt = open(os.path.join(d, "log")).read()
+            t = [ln.split(":")[1].split("(")[0] for ln in txt.splitlines() if ln.startswith("Total time of main loop")]
+            if rc != 0 or "ABNORMAL" in txt or not t:
+                print("# Qdyn6 run failed (rc %d):\n%s" % (rc, txt[-1500:]), file=sys.stderr, flush=True)
+                return None
+            loops.append(float(t[0]))
+        ns_day = sum(nsteps / max(t, 1e-3) for t in loops) * 1.0 * 0.0864
+        return ns_day, max(loops), wall, len(runs)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def host_cores():
     """Host cores this process may run on (its CPU affinity), at most 16."""
     try:
@@ -214,6 +267,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--windows", type=int, default=51)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="MD steps per stock-Qdyn6 process of the CPU baseline (about 1 s per step at 10,861 atoms)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--frames", type=int, default=200,
                     help="frames per window stored after the timed run for the dG precision check and the adjoint")
@@ -247,14 +302,14 @@ def main():
                     extra=10 * args.frames)
     steps_per_s = args.steps / tm
     ns_day = nw * steps_per_s * 1.0 * 0.0864
-    counts = bm.pair_counts(0)
-    pairs_step = s.pair_count_per_step(counts)
+    all_counts = [bm.pair_counts(k) for k in range(nw)]  # list lengths of each window's last list update
+    pairs_step = sum(s.pair_count_per_step(c) for c in all_counts)
     print(f"# batched mixed: {args.steps} steps x {nw} windows in {tm:.3f} s -> {ns_day:.1f} ns/day aggregate; "
-          f"pairs/step/window {pairs_step}", file=sys.stderr, flush=True)
+          f"pairs/step over the windows {pairs_step}", file=sys.stderr, flush=True)
     # kernel timings on the same stream (HIP events), same batch
     bm_shapes = bm.shapes
-    t_lists = bm.time_kernel(0, 20)
-    t_forces = bm.time_kernel(1, 20)
+    t_k = {"k_lists_lrf": bm.time_kernel(bm.KERNEL_LISTS, 20), "k_wforces": bm.time_kernel(bm.KERNEL_WFORCES, 20),
+           "k_sforces": bm.time_kernel(bm.KERNEL_SFORCES, 20), "k_qforces": bm.time_kernel(bm.KERNEL_QFORCES, 20)}
     e = bm.energies(0)
     temps = e[:, abi.F_TEMP]
     prec = free_energy_precision(bm, s, lambdas, args.frames, host_cores()) if args.frames > 0 else None
@@ -264,23 +319,36 @@ def main():
               file=sys.stderr, flush=True)
     bm.close()
 
+    # algorithmic FLOPs per launch over all windows; the list sweep runs once
+    # per 10-step cycle, the force kernels every step
     units, tests = lrf_units(s, s.x)
-    lists_flop = nw * (units * F_LRF + tests * F_TEST)
-    forces_flop = nw * force_flops(counts, int(s.n["nstates"]))
-    share_lists = t_lists / 10.0
-    if share_lists >= t_forces:
-        dom, t_dom, flop_dom = "k_lists_lrf", t_lists, lists_flop
-    else:
-        dom, t_dom, flop_dom = "k_forces", t_forces, forces_flop
-    achieved = flop_dom / (t_dom * 1e-3) / 1e12
+    ns = int(s.n["nstates"])
+    flop = {"k_lists_lrf": nw * (units * F_LRF + tests * F_TEST),
+            "k_wforces+k_sforces": sum(pair_flops(c) for c in all_counts),
+            "k_qforces": sum(q_flops(c, ns) for c in all_counts)}
+    ms = {"k_lists_lrf": t_k["k_lists_lrf"], "k_wforces+k_sforces": t_k["k_wforces"] + t_k["k_sforces"],
+          "k_qforces": t_k["k_qforces"]}
+    per_step = {"k_lists_lrf": ms["k_lists_lrf"] / 10.0, "k_wforces+k_sforces": ms["k_wforces+k_sforces"],
+                "k_qforces": ms["k_qforces"]}
     traffic = measured_traffic(nw, int(s.n["natom"]))
-    t_bytes = traffic.get(dom) if traffic else None
-    roofline = {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP32,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32, 4), "traffic": t_bytes,
-                "hbm_GBps": None if t_bytes is None else round(t_bytes / (t_dom * 1e-3) / 1e9, 1),
-                "hbm_frac": None if t_bytes is None else round(t_bytes / (t_dom * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
-                "ms_per_launch": {"k_lists_lrf": round(t_lists, 4), "k_forces": round(t_forces, 4)},
-                "flop_per_launch": {"k_lists_lrf": lists_flop, "k_forces": forces_flop}}
+
+    def entry(k):
+        a = flop[k] / (ms[k] * 1e-3) / 1e12
+        tb = None
+        if traffic:
+            parts = k.split("+")
+            if all(p in traffic for p in parts):
+                tb = int(sum(traffic[p] for p in parts))
+        return {"kernel": k, "achieved": round(a, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
+                "frac": round(a / PEAK_FP32, 4), "traffic": tb, "ms_per_launch": round(ms[k], 4),
+                "ms_per_step": round(per_step[k], 4), "flop_per_launch": int(flop[k]),
+                "hbm_GBps": None if tb is None else round(tb / (ms[k] * 1e-3) / 1e9, 1),
+                "hbm_frac": None if tb is None else round(tb / (ms[k] * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4)}
+    # the dominant kernel is the one with the largest share of the step
+    dom = max(per_step, key=per_step.get)
+    roofline = dict(bound="valu", **entry(dom))
+    roofline["kernels"] = {k: entry(k) for k in flop}
+    roofline["ms_per_launch_each"] = {k: round(v, 4) for k, v in t_k.items()}
 
     plain = None
     if not args.no_plain:
@@ -293,10 +361,23 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        sps, tc, nsteps = cpu_baseline(s, ncycles=5)
-        cpu = {"value": round(sps * 1.0 * 0.0864, 4), "unit": "ns/day", "cores": 1, "kind": "port",
-               "sample": f"oracle (C restatement, 1 thread) on one window of the same system: 1 list/LRF update + "
-                         f"{nsteps} full nonbonded+restraint evaluations ({tc:.1f} s); integration not included"}
+        nproc = min(host_cores(), nw)
+        ref = reference_baseline(s, lambdas, nproc, args.cpu_steps)
+        if ref is not None:
+            v, tloop, wall, nrun = ref
+            cpu = {"value": round(v, 4), "unit": "ns/day", "cores": nrun, "kind": "reference",
+                   "sample": f"stock Qdyn6 (serial, built from the reference sources) on {nrun} of the {nw} windows "
+                             f"at once, one process per core, {args.cpu_steps} MD steps each of the same system from "
+                             f"the same start; ns/day from each run's main-loop timer (slowest {tloop:.1f} s, "
+                             f"{wall:.1f} s wall with set-up)"}
+        else:
+            sps, tc, nsteps = cpu_baseline(s, ncycles=5)
+            cpu = {"value": round(sps * 1.0 * 0.0864, 4), "unit": "ns/day", "cores": 1, "kind": "port",
+                   "sample": f"oracle (C restatement, 1 thread) on one window of the same system: 1 list/LRF "
+                             f"update + {nsteps} full nonbonded+restraint evaluations ({tc:.1f} s); integration "
+                             f"not included (stock Qdyn6 build absent or failed)"}
+        print(f"# cpu baseline ({cpu['kind']}): {cpu['value']} ns/day on {cpu['cores']} cores", file=sys.stderr,
+              flush=True)
 
     out = {
         "metric": "ns/day + pair-interactions/s, batched mixed-prec lambda-windows vs plain FP64, 1 GPU",
@@ -308,7 +389,7 @@ def main():
                                f"1 fs, lists/LRF every 10 steps",
                    "windows": nw, "natom": int(s.n["natom"]), "precision": "mixed", "graph": True,
                    "launch_shape_waves_per_block": list(bm_shapes)},
-        "pair_interactions_per_s": round(nw * pairs_step * steps_per_s, 1),
+        "pair_interactions_per_s": round(pairs_step * steps_per_s, 1),
         "plain_fp64_ns_day": None if plain is None else round(plain, 2),
         "speedup_vs_plain_fp64": None if plain is None else round(ns_day / plain, 2),
         "temperature_range_K": [round(float(temps.min()), 1), round(float(temps.max()), 1)] if len(temps) else None,

@@ -425,12 +425,16 @@ int32_t qnb_batch_minimize(qnb_batch *b, int32_t nsteps, double max_disp);
 int32_t qnb_batch_autotune(qnb_batch *b, int32_t reps, int32_t *wpb_out);
 
 /* Times one kernel of the step on the batch's stream with HIP events: which =
- * 0 the list/LRF update (k_lists_lrf), 1 the receiver-side force kernels
- * (k_forces for water and for solute receivers, in the form used on steps
- * that sample no energies, then the per-atom LRF/restraint kernel); the
- * kernels are launched `reps` times over all instances (all
- * recompute their outputs from the current coordinates, so the state does not
- * change) and *ms_per_launch receives the average launch duration. */
+ * 0 the list/LRF update (k_lists_lrf); 1 the receiver-side pair forces
+ * (k_wforces for water receivers, k_sforces for solute receivers, in the form
+ * used on steps that sample no energies) followed by the per-atom
+ * LRF/restraint kernel; 2 k_wforces alone; 3 k_sforces alone; 4 the Q-atom
+ * side of the Q-solute and Q-water pairs (k_qforces; QNB_EINVAL without
+ * Q-atoms).  The kernels are launched `reps` times over all instances and
+ * *ms_per_launch receives the average launch duration.  The positions,
+ * velocities and recorded energies do not change; the gradient is left with
+ * the timed launches' sums (k_qforces adds into it) and is recomputed by the
+ * next step's force pass before anything reads it. */
 int32_t qnb_batch_time_kernel(qnb_batch *b, int32_t which, int32_t reps, double *ms_per_launch);
 
 /* ------------------------------------------------------------------------

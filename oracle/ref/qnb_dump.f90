@@ -13,6 +13,7 @@
 !
 ! Usage:  qnb_dump out.bin input.inp   (Qdyn6 reads its input from the last argument)
 program qnb_dump
+  use, intrinsic :: iso_c_binding
   use MD
   use SIMPREP
   use MPIGLOB
@@ -108,6 +109,7 @@ program qnb_dump
   call wd1(u, 'fkwpol', fkwpol)
   call wd1(u, 'itdis', itdis)
   call wd1(u, 'dt', dt)
+  call wd1(u, 'crgQtot', crgQtot)
   call wd(u, 'xpcent', 3, (/xpcent%x, xpcent%y, xpcent%z/), 1, (/3/))
   call wd(u, 'xwcent', 3, (/xwcent%x, xwcent%y, xwcent%z/), 1, (/3/))
   if (nstates > 0) call wd(u, 'lambda', nstates, EQ(1:nstates)%lambda, 1, (/nstates/))
@@ -229,6 +231,9 @@ program qnb_dump
     call wd(u, 'wshell_cstb', n, wshell(1:n)%cstb, 1, (/n/))
     call wd(u, 'wshell_theta_corr', n, wshell(1:n)%theta_corr, 1, (/n/))
   end if
+
+  ! ---------------- the engine's tables as the Fortran fill module makes them ----
+  call dump_fill(u)
 
   ! ---------------- pair lists ----------------
   call make_pair_lists(Rcq, Rcq**2, RcLRF**2, Rcpp**2, Rcpw**2, Rcww**2)
@@ -441,6 +446,132 @@ program qnb_dump
   write(*,'(a)') 'qnb_dump: wrote '//trim(outname)
 
 contains
+
+  ! qnb_system and qnb_params as q6_amd/fortran/qnb_fill.f90 fills them from
+  ! the state above, every array read back through its C pointer with the
+  ! lengths the header gives, as records fill_<field> (bytes as int32)
+  subroutine dump_fill(u)
+    use qnb_bind
+    use qnb_fill
+    integer, intent(in) :: u
+    type(qnb_system) :: fs
+    type(qnb_params) :: fp
+    integer :: na, nsl, nq_, ns_, sa_, nt_, ncg
+    call qnb_fill_system(fs, fp, QNB_PREC_FP64)
+    na = fs%natom
+    nsl = fs%nat_solute
+    nq_ = fs%nqat
+    ns_ = max(fs%nstates, 1)
+    sa_ = fs%solv_atom
+    nt_ = fs%ntypes
+    ncg = fs%ncgp
+    call wi1(u, 'fill_natom', int(fs%natom))
+    call wi1(u, 'fill_nat_solute', int(fs%nat_solute))
+    call wi1(u, 'fill_nwat', int(fs%nwat))
+    call wi1(u, 'fill_solv_atom', int(fs%solv_atom))
+    call wi1(u, 'fill_ncgp', int(fs%ncgp))
+    call wi1(u, 'fill_ncgp_solute', int(fs%ncgp_solute))
+    call wi1(u, 'fill_nqat', int(fs%nqat))
+    call wi1(u, 'fill_nstates', int(fs%nstates))
+    call wi1(u, 'fill_ntypes', int(fs%ntypes))
+    call wi1(u, 'fill_ivdw_rule', int(fs%ivdw_rule))
+    call wi1(u, 'fill_solvent_spc', int(fs%solvent_spc))
+    call wi1(u, 'fill_nexlong', int(fs%nexlong))
+    call wi1(u, 'fill_n14long', int(fs%n14long))
+    call wi1(u, 'fill_any_atom_cutoff', int(fs%any_atom_cutoff))
+    call wd1(u, 'fill_el14_scale', real(fs%el14_scale, prec))
+    call fd(u, 'fill_crg', fs%crg, na)
+    call fi(u, 'fill_iac', fs%iac, na)
+    call fi(u, 'fill_iqatom', fs%iqatom, na)
+    call fb(u, 'fill_excl', fs%excl, na)
+    call fb(u, 'fill_heavy', fs%heavy, na)
+    call fi(u, 'fill_cgp_first', fs%cgp_first, ncg)
+    call fi(u, 'fill_cgp_last', fs%cgp_last, ncg)
+    call fi(u, 'fill_cgp_iswitch', fs%cgp_iswitch, ncg)
+    call fi(u, 'fill_cgpatom', fs%cgpatom, cgp(ncg)%last)
+    call fd(u, 'fill_avdw', fs%avdw, 3*nt_)
+    call fd(u, 'fill_bvdw', fs%bvdw, 3*nt_)
+    call fi(u, 'fill_ljcod', fs%ljcod, nt_*nt_)
+    if (nsl > 0) then
+      call fb(u, 'fill_listex', fs%listex, 25*nsl)
+      call fb(u, 'fill_list14', fs%list14, 25*nsl)
+      if (fs%nexlong > 0) call fi(u, 'fill_listexlong', fs%listexlong, 2*fs%nexlong)
+      if (fs%n14long > 0) call fi(u, 'fill_list14long', fs%list14long, 2*fs%n14long)
+    end if
+    if (fs%nwat > 0 .and. nsl > 0) then
+      call fd(u, 'fill_pw_A', fs%pw_A, nsl*sa_)
+      call fd(u, 'fill_pw_B', fs%pw_B, nsl*sa_)
+      call fd(u, 'fill_pw_el', fs%pw_el, nsl*sa_)
+    end if
+    if (fs%nwat > 0) then
+      call fd(u, 'fill_ww_A', fs%ww_A, sa_*sa_)
+      call fd(u, 'fill_ww_B', fs%ww_B, sa_*sa_)
+      call fd(u, 'fill_ww_el', fs%ww_el, sa_*sa_)
+    end if
+    if (nq_ > 0) then
+      call fi(u, 'fill_iqseq', fs%iqseq, nq_)
+      if (nsl > 0) then
+        call fb(u, 'fill_qconn3', fs%qconn3, nsl)
+        call fd(u, 'fill_qp_A', fs%qp_A, nsl*nq_*ns_)
+        call fd(u, 'fill_qp_B', fs%qp_B, nsl*nq_*ns_)
+        call fd(u, 'fill_qp_el', fs%qp_el, nsl*nq_*ns_)
+        call fd(u, 'fill_qp_sc', fs%qp_sc, nsl*nq_*ns_)
+        call fb(u, 'fill_qp_set', fs%qp_set, nsl*nq_*ns_)
+      end if
+      call fd(u, 'fill_qq_A', fs%qq_A, nq_*nq_*ns_)
+      call fd(u, 'fill_qq_B', fs%qq_B, nq_*nq_*ns_)
+      call fd(u, 'fill_qq_el', fs%qq_el, nq_*nq_*ns_)
+      call fd(u, 'fill_qq_sc', fs%qq_sc, nq_*nq_*ns_)
+      call fb(u, 'fill_qq_set', fs%qq_set, nq_*nq_*ns_)
+      call fb(u, 'fill_qq_soft', fs%qq_soft, nq_*nq_*ns_)
+      if (fs%nwat > 0) then
+        call fd(u, 'fill_qw_A', fs%qw_A, nq_*sa_*ns_)
+        call fd(u, 'fill_qw_B', fs%qw_B, nq_*sa_*ns_)
+        call fd(u, 'fill_qw_el', fs%qw_el, nq_*sa_*ns_)
+        call fd(u, 'fill_qw_sc', fs%qw_sc, nq_*sa_*ns_)
+      end if
+    end if
+    call wd(u, 'fill_cutoffs', 5, (/real(fp%Rcpp, prec), real(fp%Rcpw, prec), real(fp%Rcww, prec), &
+            real(fp%Rcq, prec), real(fp%RcLRF, prec)/), 1, (/5/))
+    call wi(u, 'fill_flags', 5, (/int(fp%use_LRF), int(fp%precision), int(fp%ref_quirks), int(fp%wpol_restr), &
+            int(fp%nwpolr_shell)/), 1, (/5/))
+    call wd(u, 'fill_xpcent', 3, real(fp%xpcent, prec), 1, (/3/))
+    call wd(u, 'fill_xwcent', 3, real(fp%xwcent, prec), 1, (/3/))
+    call wd(u, 'fill_lambda', QNB_MAX_STATES, real(fp%lambda, prec), 1, (/QNB_MAX_STATES/))
+    call wd(u, 'fill_scaas', 8, (/real(fp%rexcl_o, prec), real(fp%rwat, prec), real(fp%fk_wsphere, prec), &
+            real(fp%Dwmz, prec), real(fp%awmz, prec), real(fp%Boltz, prec), real(fp%fkwpol, prec), &
+            real(fp%itdis, prec)/), 1, (/8/))
+    call wd(u, 'fill_wshell', 4*QNB_MAX_SHELLS, real((/fp%wshell_rout, fp%wshell_dr, fp%wshell_cstb, &
+            fp%wshell_theta_corr/), prec), 2, (/QNB_MAX_SHELLS, 4/))
+    call qnb_fill_release()
+  end subroutine dump_fill
+
+  subroutine fd(u, name, p, n)
+    integer, intent(in) :: u, n
+    character(len=*), intent(in) :: name
+    type(c_ptr), intent(in) :: p
+    real(c_double), pointer :: a(:)
+    call c_f_pointer(p, a, (/n/))
+    call wd(u, name, n, real(a, prec), 1, (/n/))
+  end subroutine fd
+
+  subroutine fi(u, name, p, n)
+    integer, intent(in) :: u, n
+    character(len=*), intent(in) :: name
+    type(c_ptr), intent(in) :: p
+    integer(c_int32_t), pointer :: a(:)
+    call c_f_pointer(p, a, (/n/))
+    call wi(u, name, n, int(a), 1, (/n/))
+  end subroutine fi
+
+  subroutine fb(u, name, p, n)
+    integer, intent(in) :: u, n
+    character(len=*), intent(in) :: name
+    type(c_ptr), intent(in) :: p
+    integer(c_int8_t), pointer :: a(:)
+    call c_f_pointer(p, a, (/n/))
+    call wi(u, name, n, int(a), 1, (/n/))
+  end subroutine fb
 
   subroutine dump_qbonded(u)
     integer, intent(in) :: u

@@ -595,12 +595,12 @@ static void launch_water(qnb_batch* b, int wpb, int imap) {
 }
 
 template <typename R, int NS, bool E>
-static void launch_forces_e(qnb_batch* b, int wpb) {
-  if (b->maxw > 0) {
+static void launch_forces_e(qnb_batch* b, int wpb, int parts) {
+  if (b->maxw > 0 && (parts & 1)) {
     if (b->small_groups) launch_water<R, NS, E, WF_CAP_SMALL>(b, wpb, b->imap_water);
     else launch_water<R, NS, E, FC_CAP>(b, wpb, b->imap_water);
   }
-  if (b->maxs > 0) {
+  if (b->maxs > 0 && (parts & 2)) {
     const int nb = (b->maxs + wpb - 1) / wpb, imap = b->imap_solute;
     const dim3 grid = imap == 0 ? dim3(xcd_cols(nb), b->n) : dim3((unsigned)(8 * (((long)nb * b->n + 7) / 8)), 1);
     if (b->small_groups)
@@ -614,26 +614,48 @@ static void launch_forces_e(qnb_batch* b, int wpb) {
 
 // energy: the step samples energies in at least one instance (or the host
 // cannot tell, e.g. inside a graph whose period is not a multiple of every
-// instance's sampling interval)
+// instance's sampling interval); parts: 1 water receivers, 2 solute receivers
 template <typename R, int NS>
-static void launch_forces_t(qnb_batch* b, int wpb, bool energy = true, bool atom_terms = true) {
+static void launch_forces_t(qnb_batch* b, int wpb, bool energy = true, bool atom_terms = true, int parts = 3) {
   wpb = forces_wpb<R>(wpb);
-  if (energy) launch_forces_e<R, NS, true>(b, wpb);
-  else launch_forces_e<R, NS, false>(b, wpb);
+  if (energy) launch_forces_e<R, NS, true>(b, wpb, parts);
+  else launch_forces_e<R, NS, false>(b, wpb, parts);
   if (atom_terms)
     hipLaunchKernelGGL(k_atom_terms, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys, kStepMask,
                        (const double*)b->tfree);
 }
 
-static void launch_forces(qnb_batch* b, int wpb, bool energy = true) {
+static void launch_forces(qnb_batch* b, int wpb, bool energy = true, bool atom_terms = true, int parts = 3) {
   const bool mixed = b->prec == QNB_PREC_MIXED;
+#define QNB_LF(NS)                                                 \
+  (mixed ? launch_forces_t<float, NS>(b, wpb, energy, atom_terms, parts) \
+         : launch_forces_t<double, NS>(b, wpb, energy, atom_terms, parts))
   switch (b->ns_max) {
-    case 1: mixed ? launch_forces_t<float, 1>(b, wpb, energy) : launch_forces_t<double, 1>(b, wpb, energy); break;
-    case 2: mixed ? launch_forces_t<float, 2>(b, wpb, energy) : launch_forces_t<double, 2>(b, wpb, energy); break;
+    case 1: QNB_LF(1); break;
+    case 2: QNB_LF(2); break;
     case 3:
-    case 4: mixed ? launch_forces_t<float, 4>(b, wpb, energy) : launch_forces_t<double, 4>(b, wpb, energy); break;
-    default: mixed ? launch_forces_t<float, 8>(b, wpb, energy) : launch_forces_t<double, 8>(b, wpb, energy); break;
+    case 4: QNB_LF(4); break;
+    default: QNB_LF(8); break;
   }
+#undef QNB_LF
+}
+
+// the Q-atom side of the Q-solute and Q-water pairs alone (kernel timing)
+static void launch_qforces(qnb_batch* b) {
+  const bool mixed = b->prec == QNB_PREC_MIXED;
+  const int wpb = 4;
+  const dim3 grid(qforces_blocks(b->nq_max, b->qchunks, wpb), b->n), block(64 * wpb);
+#define QNB_LQ(NS)                                                                                     \
+  if (mixed) hipLaunchKernelGGL((k_qforces<float, NS>), grid, block, 0, b->stream, b->dsys, kStepMask, b->qchunks); \
+  else hipLaunchKernelGGL((k_qforces<double, NS>), grid, block, 0, b->stream, b->dsys, kStepMask, b->qchunks)
+  switch (b->ns_max) {
+    case 1: QNB_LQ(1); break;
+    case 2: QNB_LQ(2); break;
+    case 3:
+    case 4: QNB_LQ(4); break;
+    default: QNB_LQ(8); break;
+  }
+#undef QNB_LQ
 }
 
 // atom_terms = false: the caller runs the per-atom terms itself (the MD step
@@ -1263,13 +1285,19 @@ extern "C" int32_t qnb_batch_pair_counts(qnb_batch* b, int32_t k, int64_t* out) 
 }
 
 extern "C" int32_t qnb_batch_time_kernel(qnb_batch* b, int32_t which, int32_t reps, double* ms_per_launch) {
-  if (!b || reps <= 0 || !ms_per_launch) return fail(QNB_EINVAL, "bad arguments");
+  if (!b || reps <= 0 || !ms_per_launch || which < 0 || which > 4) return fail(QNB_EINVAL, "bad arguments");
+  if (which == 4 && !b->anyq) return fail(QNB_EINVAL, "no instance has Q-atoms");
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   auto launch = [&]() {
-    if (which == 0) launch_lists(b, b->wpb_lists);
-    else launch_forces(b, b->wpb_forces, false);
+    switch (which) {
+      case 0: launch_lists(b, b->wpb_lists); break;
+      case 1: launch_forces(b, b->wpb_forces, false); break;
+      case 2: launch_forces(b, b->wpb_forces, false, false, 1); break;
+      case 3: launch_forces(b, b->wpb_forces, false, false, 2); break;
+      default: launch_qforces(b); break;
+    }
   };
   launch();  // warm-up
   HIPCHK(hipEventRecord(e0, b->stream));

@@ -221,7 +221,12 @@ class BatchMD:
         abi.check(self.lib, self.lib.qnb_batch_frames(self.h, k, None, 0, C.byref(nf)), "qnb_batch_frames")
         return nf.value
 
+    # kernels qnb_batch_time_kernel times
+    KERNEL_LISTS, KERNEL_FORCES, KERNEL_WFORCES, KERNEL_SFORCES, KERNEL_QFORCES = 0, 1, 2, 3, 4
+
     def time_kernel(self, which, reps):
+        """Average launch duration (ms) of one kernel of the step over all
+        instances, timed with HIP events on the batch's stream."""
         ms = C.c_double()
         abi.check(self.lib, self.lib.qnb_batch_time_kernel(self.h, which, reps, C.byref(ms)), "time_kernel")
         return ms.value

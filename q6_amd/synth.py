@@ -406,8 +406,18 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
     rwat = radius
     Dwmz = 0.26 * math.exp(-0.19 * (rwat - 15.0)) + 0.74
     awmz = 0.2 / (1.0 + math.exp(0.4 * (rwat - 25.0))) + 0.3
-    crgQtot = float((qcrg_s * lam[None, :]).sum()) if nq else 0.0
-    mu_w = -chg_solv[0] * TIP3P_ROH * math.cos(math.radians(TIP3P_HOH) / 2)
+    # FEP-file (unscaled) charges summed in the reference's order: Q-atoms outer,
+    # states inner (src/simprep.f90:4640-4646)
+    crgQtot = 0.0
+    for i in range(nq):
+        for st in range(nstates):
+            crgQtot = crgQtot + float(qcrg[i, st]) * float(lam[st])
+    # water dipole as the reference forms it (src/simprep.f90:4745-4751): the
+    # oxygen's topology charge times the length of the topology's LAST bond
+    # and the cosine of half its last angle; water bonds are listed O-H1,
+    # O-H2, H1-H2, so that length is the H-H distance
+    rhh = 2 * TIP3P_ROH * math.sin(math.radians(TIP3P_HOH) / 2)
+    mu_w = -TIP3P_Q[0] * rhh * math.cos(math.radians(TIP3P_HOH) / 2)
     nsh = int(-0.5 + math.sqrt(2.0 * (3.0001 / 0.5) + 0.25))
     rout = rwat
     sh_rout, sh_dr, sh_cstb = [], [], []

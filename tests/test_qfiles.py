@@ -3,7 +3,7 @@
 q6_amd.qfiles writes a synthetic system as Q topology, FEP and MD input files.
 Stock Qdyn6 modules built from the reference sources (oracle/_ref/qnb_dump,
 made by `make -C oracle/ref` in the development container) read them and dump
-their set-up and every nonbonded term.  Two checks per workload:
+their set-up and every nonbonded term.  Three checks per workload:
 
   * every table Qdyn6 derives (scaled charges, types and LJ codes, charge
     groups and switch atoms, exclusion and 1-4 windows, the water and
@@ -12,7 +12,10 @@ their set-up and every nonbonded term.  Two checks per workload:
   * the oracle, given the generator's tables and Qdyn6's coordinates (the
     text round trip can move a coordinate by an ulp), reproduces Qdyn6's
     energy and gradient of every term bit for bit at the workload's size:
-    3,387 atoms (20 A sphere) and the benchmarked 10,861 atoms (30 A).
+    3,387 atoms (20 A sphere) and the benchmarked 10,861 atoms (30 A);
+  * the Fortran fill module (q6_amd/fortran/qnb_fill.f90), linked into the
+    dump, fills the engine's structs from that Qdyn6 state with the same
+    bytes the Python host passes.
 """
 import os
 import subprocess
@@ -55,9 +58,43 @@ def test_tables_equal_generator(written):
     for k in s.a:
         assert np.array_equal(np.asarray(s.a[k]), np.asarray(g.a[k])), k
     for k in ("Rcpp", "Rcpw", "Rcww", "Rcq", "RcLRF", "rexcl_o", "rwat", "fk_wsphere", "fkwpol", "lambda",
-              "xpcent", "xwcent", "use_LRF", "wpol_restr"):
+              "xpcent", "xwcent", "use_LRF", "wpol_restr", "nwpolr_shell", "Dwmz", "awmz", "wshell_rout", "wshell_dr",
+              "wshell_cstb"):
         assert np.array_equal(np.asarray(s.par[k], float), np.asarray(g.par[k], float)), k
     assert np.max(np.abs(s.x - g.x)) <= 1e-14 * np.max(np.abs(s.x))
+
+
+def test_fortran_fill_equals_host_tables(written):
+    """q6_amd/fortran/qnb_fill.f90, run inside the same Qdyn6 set-up (the
+    dump links it and reads every array back through its C pointer), fills
+    qnb_system and qnb_params with exactly the arrays and values the Python
+    host passes to qnb_create for the same system: same bytes, same layout."""
+    s, g, _ = written
+    r = g.ref
+    for k in ("natom", "nat_solute", "nwat", "solv_atom", "ncgp", "ncgp_solute", "nqat", "nstates", "ntypes",
+              "ivdw_rule", "solvent_spc"):
+        assert int(r["fill_" + k]) == int(s.n[k]), k
+    assert int(r["fill_any_atom_cutoff"]) == int(s.n.get("any_atom_cutoff", 0))
+    assert int(r["fill_nexlong"]) == int(s.n.get("nexlong", 0))
+    assert int(r["fill_n14long"]) == int(s.n.get("n14long", 0))
+    assert float(r["fill_el14_scale"]) == float(s.el14_scale)
+    sysc, parc = s.to_structs()
+    for k, dt in s.ARRAYS.items():
+        want = np.asarray(s.a[k]).ravel()
+        if "fill_" + k not in r:
+            assert want.size == 1 and not want.any(), k  # a table the system does not have
+            continue
+        got = np.asarray(r["fill_" + k]).ravel().astype(dt)
+        assert got.dtype == want.dtype and np.array_equal(got, want), k
+    assert np.array_equal(r["fill_cutoffs"], [parc.Rcpp, parc.Rcpw, parc.Rcww, parc.Rcq, parc.RcLRF])
+    assert list(r["fill_flags"]) == [parc.use_LRF, abi.QNB_PREC_FP64, 1, parc.wpol_restr, parc.nwpolr_shell]
+    assert np.array_equal(r["fill_xpcent"], list(parc.xpcent)) and np.array_equal(r["fill_xwcent"], list(parc.xwcent))
+    assert np.array_equal(r["fill_lambda"], list(parc.lambda_))
+    assert np.array_equal(r["fill_scaas"], [parc.rexcl_o, parc.rwat, parc.fk_wsphere, parc.Dwmz, parc.awmz, parc.Boltz,
+                                            parc.fkwpol, parc.itdis])
+    wsh = np.asarray(r["fill_wshell"]).T  # (shell, field) as dumped
+    for row, nm in zip(wsh, ("wshell_rout", "wshell_dr", "wshell_cstb", "wshell_theta_corr")):
+        assert np.array_equal(row, list(getattr(parc, nm))), nm
 
 
 @pytest.mark.parametrize("term,mask", [("ww", abi.T_WW), ("qw", abi.T_QW), ("pw", abi.T_PW), ("pp", abi.T_PP),
