@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define QNB_ABI_VERSION 3
+#define QNB_ABI_VERSION 4
 
 enum qnb_status {
   QNB_OK = 0,
@@ -507,6 +507,55 @@ int32_t qnb_batch_frames(qnb_batch *b, int32_t k, double *out, int32_t cap, int3
  * nframes_k x nstates_k x nqat_k x QNB_ADJ_NV); *n the number of float64
  * written (also when it exceeds cap, then nothing is copied). */
 int32_t qnb_batch_adjoint(qnb_batch *b, const qnb_qtangent *const *tan, double *out, int64_t cap, int64_t *n);
+
+/* ------------------------------------------------------------------------
+ * Free-energy analysis on the device: Qfep6's Part 1 (Zwanzig forward and
+ * reverse averages and the dG chain, src/qfep.f90:587-648), Part 2 (EVB
+ * ground-state reaction free energy and diabatic curves per window and
+ * energy-gap bin, :700-795, two states) and Part 3 (bin averages over the
+ * windows, :797-827), every sum over the frames in the reference's order.
+ * Replaces running Qfep6 on the energy files of a batch. */
+typedef struct qnb_fep_params {
+  double kT;                       /* R T in kcal/mol (Qfep6's rt)              */
+  int32_t nskip;                   /* frames skipped at the start of a window   */
+  int32_t nbins;                   /* energy-gap bins; 0: Part 1 only           */
+  int32_t nmin;                    /* fewest points of a reported bin           */
+  int32_t reserved;
+  double alpha[QNB_MAX_STATES];    /* alpha shifts (alpha[0] is taken as 0)     */
+  double coeff[QNB_MAX_STATES];    /* energy-gap coefficients                   */
+  /* H12 = A exp(-mu (r - r0)) exp(-eta (r - r0)^2) (qfep.f90:476-480), r the
+   * frame's distance of the first off-diagonal, 0 where none is recorded;
+   * c1 is computed only where off-diagonals are recorded, as in Qfep6 */
+  double A, mu, eta, r0;
+} qnb_fep_params;
+
+/* Results, caller-owned host arrays (any may be NULL); n windows, B bins,
+ * S states.  Unreported entries are NaN. */
+typedef struct qnb_fep_out {
+  double *dgf;        /* [n] window i -> i+1 (i < n-1)                        */
+  double *dgr;        /* [n] window i -> i-1 (i > 0)                          */
+  double *dG;         /* [n] dG_0 = 0                                         */
+  double *gap_range;  /* [2] smallest and largest gap after nskip             */
+  double *dGg;        /* [n][B] ground-state free energy per window and bin   */
+  double *dGv;        /* [n][B][S] diabatic free energies                     */
+  double *c1, *c2;    /* [n][B] mean c1^2, c2^2                               */
+  double *r;          /* [n][B] mean first off-diagonal distance              */
+  int32_t *npts;      /* [n][B] points in the bin                             */
+  double *bin_dGg, *bin_c1, *bin_c2, *bin_r; /* [B] Part 3 averages          */
+  int32_t *ptsum;     /* [B] points over the reported windows                 */
+} qnb_fep_out;
+
+/* Over a batch's recorded energy frames, one window per instance in order
+ * (lambdas from the instances; every instance must have the same number of
+ * states and more than nskip frames).  The state energies are formed from the
+ * frames as pot_energy forms EQ(s)%total (src/potene.f90:224-231). */
+int32_t qnb_batch_free_energy(qnb_batch *b, const qnb_fep_params *p, qnb_fep_out *out);
+/* Over state energies the caller holds (e.g. read from Qdyn6 energy files):
+ * window w has nframes[w] frames of nstates EQ(s)%total values at V[w] (host,
+ * frame-major), optionally rkl[w][nframes[w]] (the first off-diagonal's
+ * distance; rkl or rkl[w] NULL: none) and lambdas[w*nstates .. +nstates). */
+int32_t qnb_free_energy(int32_t nwin, int32_t nstates, const int32_t *nframes, const double *const *V,
+                        const double *const *rkl, const double *lambdas, const qnb_fep_params *p, qnb_fep_out *out);
 
 #ifdef __cplusplus
 }

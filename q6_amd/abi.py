@@ -159,7 +159,20 @@ def qbonded_struct(tb, urey=False):
     q.urey_bradley = 1 if urey else 0
     return q, keep
 
-ABI_VERSION = 3
+class QnbFepParams(C.Structure):
+    _fields_ = [("kT", C.c_double), ("nskip", C.c_int32), ("nbins", C.c_int32), ("nmin", C.c_int32),
+                ("reserved", C.c_int32), ("alpha", C.c_double * QNB_MAX_STATES),
+                ("coeff", C.c_double * QNB_MAX_STATES), ("A", C.c_double), ("mu", C.c_double),
+                ("eta", C.c_double), ("r0", C.c_double)]
+
+
+class QnbFepOut(C.Structure):
+    _fields_ = [("dgf", _pd), ("dgr", _pd), ("dG", _pd), ("gap_range", _pd), ("dGg", _pd), ("dGv", _pd),
+                ("c1", _pd), ("c2", _pd), ("r", _pd), ("npts", _pi), ("bin_dGg", _pd), ("bin_c1", _pd),
+                ("bin_c2", _pd), ("bin_r", _pd), ("ptsum", _pi)]
+
+
+ABI_VERSION = 4
 
 # every symbol include/qnb.h declares (checked by the CPU test suite)
 EXPORTS = [
@@ -169,7 +182,7 @@ EXPORTS = [
     "qnb_batch_create", "qnb_batch_destroy", "qnb_batch_run", "qnb_batch_integrate", "qnb_batch_thermostat", "qnb_batch_sync", "qnb_batch_get_state",
     "qnb_batch_energy_history", "qnb_batch_pair_counts", "qnb_batch_time_kernel", "qnb_batch_minimize", "qnb_batch_autotune",
     "qnb_adjoint", "qnb_adjoint_static", "qnb_batch_store_frames", "qnb_batch_frames", "qnb_batch_adjoint", "qnb_batch_set_offdiag",
-    "qnb_set_qbonded", "qnb_eval_qbonded", "qnb_batch_set_qbonded",
+    "qnb_set_qbonded", "qnb_eval_qbonded", "qnb_batch_set_qbonded", "qnb_batch_free_energy", "qnb_free_energy",
 ]
 
 _lib = None
@@ -211,6 +224,9 @@ def _declare(lib):
     lib.qnb_batch_set_offdiag.argtypes = [vp, C.c_int32, C.c_int32, _pi, _pd, _pd]
     lib.qnb_batch_frames.argtypes = [vp, C.c_int32, _pd, C.c_int32, C.POINTER(C.c_int32)]
     lib.qnb_batch_adjoint.argtypes = [vp, pp, _pd, C.c_int64, C.POINTER(C.c_int64)]
+    lib.qnb_batch_free_energy.argtypes = [vp, C.POINTER(QnbFepParams), C.POINTER(QnbFepOut)]
+    lib.qnb_free_energy.argtypes = [C.c_int32, C.c_int32, _pi, C.POINTER(_pd), C.POINTER(_pd), _pd,
+                                    C.POINTER(QnbFepParams), C.POINTER(QnbFepOut)]
     lib.qnb_set_qbonded.argtypes = [vp, C.POINTER(QnbQbonded)]
     lib.qnb_eval_qbonded.argtypes = [vp, _pd, _pd, _pd]
     lib.qnb_batch_set_qbonded.argtypes = [vp, C.c_int32, C.POINTER(QnbQbonded)]

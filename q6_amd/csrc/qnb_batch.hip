@@ -1312,6 +1312,31 @@ extern "C" int32_t qnb_batch_time_kernel(qnb_batch* b, int32_t which, int32_t re
   return 0;
 }
 
+extern "C" int32_t qnb_batch_free_energy(qnb_batch* b, const qnb_fep_params* p, qnb_fep_out* out) {
+  if (!b || !out) return fail(QNB_EINVAL, "bad arguments");
+  const int ns = b->hs[0].nstates;
+  for (const DevSys& S : b->hs)
+    if (S.nstates != ns) return fail(QNB_EINVAL, "instances differ in their number of states");
+  FepPar P;
+  if (int rc = fep_params(p, ns, &P)) return rc;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<FepWin> hw(b->n);
+  for (int k = 0; k < b->n; k++) {
+    int nf = 0;
+    HIPCHK(hipMemcpy(&nf, b->hm[k].ctr, sizeof(int), hipMemcpyDeviceToHost));
+    nf = std::min(nf, b->hm[k].max_frames);
+    if (nf <= P.nskip) return fail(QNB_EINVAL, "instance %d has %d frames, not more than the %d skipped", k, nf, P.nskip);
+    FepWin& W = hw[k];
+    W.hist = b->hm[k].hist;
+    W.V = nullptr;
+    W.rkl = nullptr;
+    W.nf = nf;
+    W.noffd = b->hm[k].noffd;
+    for (int s = 0; s < QNB_MAX_STATES; s++) W.lam[s] = b->hs[k].lambda[s];
+  }
+  return fep_run(b->stream, hw, P, out);
+}
+
 extern "C" int32_t qnb_batch_minimize(qnb_batch* b, int32_t nsteps, double max_disp) {
   if (!b || nsteps < 0 || !(max_disp > 0.0)) return fail(QNB_EINVAL, "bad arguments");
   const double k0 = 1.0e-3;  // A per kcal/mol/A

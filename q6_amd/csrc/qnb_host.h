@@ -72,6 +72,29 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M);
 bool host_pp_param(const HostTopo& T, int i, int j);
 // per-instance device state (coordinates, gradient, lists, LRF, energies)
 int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots);
+// free-energy analysis (qnb_fep.hip): one window is either a batch
+// instance's recorded frames (hist) or state energies the caller supplies
+// (V: EQ(s)%total per frame; rkl: the first off-diagonal's distance per frame
+// or none)
+struct FepWin {
+  const double* hist;  // [nf][QNB_FRAME_LEN] or null
+  const double* V;     // [nf][ns] or null
+  const double* rkl;   // [nf] or null
+  int nf;              // frames
+  int noffd;           // off-diagonals recorded per frame
+  double lam[QNB_MAX_STATES];
+};
+struct FepPar {
+  double kT;
+  int nskip, nbins, nmin, ns;
+  double alpha[QNB_MAX_STATES], coeff[QNB_MAX_STATES];
+  double A, mu, eta, r0;
+};
+int fep_params(const qnb_fep_params* p, int ns, FepPar* P);
+// runs the analysis on stream st for windows whose data sit in device memory
+// (hw: host copies of the window records) and copies the results to out
+int fep_run(hipStream_t st, const std::vector<FepWin>& hw, const FepPar& P, qnb_fep_out* out);
+
 // adjoint of the Q-atom energies: tangent upload and launch over frames
 int adjoint_upload(DevAlloc& M, const DevSys& S, const qnb_qtangent* t, AdjDev& A);
 void launch_adjoint(hipStream_t st, int prec, int ns_max, long waves, int B, const DevSys* dsys, const AdjDev* dadj);

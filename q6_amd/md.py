@@ -18,22 +18,14 @@ _pi = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
 
 
 def scaas_cstb(s, lam):
-    """Polarisation-shell constant cstb for the lambda-weighted Q charge
-    (wat_sphere crgQtot, src/simprep.f90:4640-4650; wat_shells
-    src/simprep.f90:4883)."""
-    qcrg = s.md.get("qcrg")
+    """Polarisation-shell constants cstb of a synthetic system at the state
+    weights lam (wat_sphere crgQtot and wat_shells, src/simprep.f90:4640-4650,
+    4863-4883; q6_amd.synth.shell_cstb)."""
+    qcrg = s.md.get("qcrg") if s.md else None
     if qcrg is None or qcrg.size == 0:
         return np.asarray(s.par["wshell_cstb"])
-    from .synth import RHO_WAT, SQRT332, TIP3P_HOH, TIP3P_ROH, TIP3P_Q
-    import math
-    crgQtot = float((qcrg * SQRT332 * np.asarray(lam)[None, :]).sum())
-    mu_w = -TIP3P_Q[0] * SQRT332 * TIP3P_ROH * math.cos(math.radians(TIP3P_HOH) / 2)
-    out = []
-    for rout, dr in zip(s.par["wshell_rout"], s.par["wshell_dr"]):
-        ri = rout - dr
-        rshell = (0.5 * (rout ** 3 + ri ** 3)) ** (1.0 / 3.0)
-        out.append(crgQtot * (1.0 - 1.0 / 80.0) / (RHO_WAT * mu_w * 4.0 * math.pi * rshell ** 2))
-    return np.array(out)
+    from .synth import shell_cstb
+    return shell_cstb(qcrg, lam, s.par["wshell_rout"], s.par["wshell_dr"])
 
 
 class BatchMD:
@@ -221,6 +213,16 @@ class BatchMD:
         abi.check(self.lib, self.lib.qnb_batch_frames(self.h, k, None, 0, C.byref(nf)), "qnb_batch_frames")
         return nf.value
 
+    def free_energy(self, kT, nskip=0, nbins=0, nmin=0, alpha=None, coeff=None, A=0.0, mu=0.0, eta=0.0, r0=0.0):
+        """Qfep6's analysis on the GPU (qnb_batch_free_energy) of the frames
+        every instance has recorded, one window per instance: Zwanzig dG, and
+        with nbins > 0 the EVB reaction free energy along the energy gap."""
+        ns = int(self.systems[0].n["nstates"])
+
+        def call(p, out):
+            abi.check(self.lib, self.lib.qnb_batch_free_energy(self.h, p, out), "qnb_batch_free_energy")
+        return _fep_call(call, self.n, ns, kT, nskip, nbins, nmin, alpha, coeff, A, mu, eta, r0)
+
     # kernels qnb_batch_time_kernel times
     KERNEL_LISTS, KERNEL_FORCES, KERNEL_WFORCES, KERNEL_SFORCES, KERNEL_QFORCES = 0, 1, 2, 3, 4
 
@@ -230,6 +232,60 @@ class BatchMD:
         ms = C.c_double()
         abi.check(self.lib, self.lib.qnb_batch_time_kernel(self.h, which, reps, C.byref(ms)), "time_kernel")
         return ms.value
+
+
+def _fep_call(call, n, ns, kT, nskip, nbins, nmin, alpha, coeff, A, mu, eta, r0):
+    """Fills qnb_fep_params, allocates the result arrays, runs `call(p, out)`
+    and returns the results as a dict (dgf[n-1] forward, dgr[n-1] reverse
+    from window i+1 to i, as q6_amd.fep.zwanzig returns them)."""
+    p = abi.QnbFepParams(kT=float(kT), nskip=int(nskip), nbins=int(nbins), nmin=int(nmin), A=float(A),
+                         mu=float(mu), eta=float(eta), r0=float(r0))
+    al = np.zeros(abi.QNB_MAX_STATES) if alpha is None else np.asarray(alpha, float)
+    co = np.zeros(abi.QNB_MAX_STATES) if coeff is None else np.asarray(coeff, float)
+    for k in range(len(al)):
+        p.alpha[k] = float(al[k])
+    for k in range(len(co)):
+        p.coeff[k] = float(co[k])
+    B = max(int(nbins), 0)
+    r = dict(dgf=np.zeros(n), dgr=np.zeros(n), dG=np.zeros(n), gap_range=np.zeros(2), dGg=np.zeros((n, B)),
+             dGv=np.zeros((n, B, ns)), c1=np.zeros((n, B)), c2=np.zeros((n, B)), r=np.zeros((n, B)),
+             npts=np.zeros((n, B), np.int32), binned=np.zeros(B), binned_c1=np.zeros(B), binned_c2=np.zeros(B),
+             binned_r=np.zeros(B), ptsum=np.zeros(B, np.int32))
+    out = abi.QnbFepOut(dgf=_pd(r["dgf"]), dgr=_pd(r["dgr"]), dG=_pd(r["dG"]), gap_range=_pd(r["gap_range"]))
+    if B:
+        for f, k in (("dGg", "dGg"), ("dGv", "dGv"), ("c1", "c1"), ("c2", "c2"), ("r", "r"), ("bin_dGg", "binned"),
+                     ("bin_c1", "binned_c1"), ("bin_c2", "binned_c2"), ("bin_r", "binned_r")):
+            setattr(out, f, _pd(r[k]))
+        out.npts = _pi(r["npts"])
+        out.ptsum = _pi(r["ptsum"])
+    call(C.byref(p), C.byref(out))
+    r["dgf"], r["dgr"] = r["dgf"][:n - 1], r["dgr"][1:]
+    if B:
+        lo, hi = r["gap_range"]
+        r["centres"] = lo + (np.arange(B) + 0.5) * ((hi - lo) / B)
+    return r
+
+
+def free_energy(Vs, lambdas, kT, rkl=None, nskip=0, nbins=0, nmin=0, alpha=None, coeff=None, A=0.0, mu=0.0, eta=0.0,
+                r0=0.0):
+    """Qfep6's analysis on the GPU (qnb_free_energy) of state-energy series
+    the caller holds: Vs[w] (nframes_w, nstates) EQ(s)%total per frame of
+    window w, lambdas (nwin, nstates), rkl[w] (nframes_w,) the first
+    off-diagonal's distance (or None).  Returns the dict of _fep_call."""
+    lib = abi.load()
+    lam = np.ascontiguousarray(lambdas, np.float64)
+    n, ns = lam.shape
+    Vs = [np.ascontiguousarray(v, np.float64) for v in Vs]
+    nf = np.array([len(v) for v in Vs], np.int32)
+    Vp = (C.POINTER(C.c_double) * n)(*[_pd(v) for v in Vs])
+    rk = None
+    if rkl is not None:
+        rkl = [np.ascontiguousarray(v, np.float64) for v in rkl]
+        rk = (C.POINTER(C.c_double) * n)(*[_pd(v) for v in rkl])
+
+    def call(p, out):
+        abi.check(lib, lib.qnb_free_energy(n, ns, _pi(nf), Vp, rk, _pd(lam), p, out), "qnb_free_energy")
+    return _fep_call(call, n, ns, kT, nskip, nbins, nmin, alpha, coeff, A, mu, eta, r0)
 
 
 def q_state_energies(frames, nstates):
@@ -243,4 +299,4 @@ def q_state_energies(frames, nstates):
     return out
 
 
-__all__ = ["BatchMD", "q_state_energies", "scaas_cstb", "BOLTZ"]
+__all__ = ["BatchMD", "free_energy", "q_state_energies", "scaas_cstb", "BOLTZ"]

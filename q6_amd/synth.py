@@ -198,6 +198,34 @@ def _exclusions(nat, bonds):
     return ex, p14, adj
 
 
+def shell_cstb(qcrg, lam, wshell_rout, wshell_dr):
+    """Born-correction constant cstb of each polarisation shell as wat_shells
+    forms it (src/simprep.f90:4863-4883) for TIP3P solvent:
+      * crgQtot from the FEP file's (unscaled) Q charges, Q-atoms outer and
+        states inner (wat_sphere, src/simprep.f90:4640-4646);
+      * the water dipole mu_w from the oxygen's charge, the length of the
+        topology's LAST bond and the cosine of half its last angle
+        (:4745-4751); water bonds are listed O-H1, O-H2, H1-H2, so that
+        length is the H-H distance;
+      * cstb = crgQtot (1 - 1/eps) / (rho mu_w 4 pi rshell^2), rshell the
+        radius that halves the shell's volume."""
+    qcrg = np.asarray(qcrg, float).reshape(len(qcrg), -1) if np.size(qcrg) else np.zeros((0, 1))
+    lam = np.asarray(lam, float)
+    crgQtot = 0.0
+    for i in range(qcrg.shape[0]):
+        for st in range(qcrg.shape[1]):
+            crgQtot = crgQtot + float(qcrg[i, st]) * float(lam[st])
+    rhh = 2 * TIP3P_ROH * math.sin(math.radians(TIP3P_HOH) / 2)
+    mu_w = -TIP3P_Q[0] * rhh * math.cos(math.radians(TIP3P_HOH) / 2)
+    eps_diel = 1.0 - 1.0 / 80.0
+    out = []
+    for rout, dr in zip(wshell_rout, wshell_dr):
+        ri = rout - dr
+        rshell = (0.5 * (rout ** 3 + ri ** 3)) ** (1.0 / 3.0)
+        out.append(crgQtot * eps_diel / (RHO_WAT * mu_w * 4.0 * math.pi * rshell ** 2))
+    return np.array(out)
+
+
 def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q_sigma=0.3,
           cut=10.0, rcq=99.0, rclrf=99.0, use_lrf=True, lambdas=None, q_state0_zero=False,
           name="synthetic", q_per_chain=None, compact=None, solute_dmin=2.2, water_dmin=(2.4, 1.8),
@@ -406,30 +434,15 @@ def build(radius, nchain, nbead, n_q_chains, nstates, seed, chain_region=None, q
     rwat = radius
     Dwmz = 0.26 * math.exp(-0.19 * (rwat - 15.0)) + 0.74
     awmz = 0.2 / (1.0 + math.exp(0.4 * (rwat - 25.0))) + 0.3
-    # FEP-file (unscaled) charges summed in the reference's order: Q-atoms outer,
-    # states inner (src/simprep.f90:4640-4646)
-    crgQtot = 0.0
-    for i in range(nq):
-        for st in range(nstates):
-            crgQtot = crgQtot + float(qcrg[i, st]) * float(lam[st])
-    # water dipole as the reference forms it (src/simprep.f90:4745-4751): the
-    # oxygen's topology charge times the length of the topology's LAST bond
-    # and the cosine of half its last angle; water bonds are listed O-H1,
-    # O-H2, H1-H2, so that length is the H-H distance
-    rhh = 2 * TIP3P_ROH * math.sin(math.radians(TIP3P_HOH) / 2)
-    mu_w = -TIP3P_Q[0] * rhh * math.cos(math.radians(TIP3P_HOH) / 2)
     nsh = int(-0.5 + math.sqrt(2.0 * (3.0001 / 0.5) + 0.25))
     rout = rwat
-    sh_rout, sh_dr, sh_cstb = [], [], []
-    eps_diel = 1.0 - 1.0 / 80.0
+    sh_rout, sh_dr = [], []
     for k in range(1, nsh + 1):
         dr = 0.5 * k
-        ri = rout - dr
-        rshell = (0.5 * (rout ** 3 + ri ** 3)) ** (1.0 / 3.0)
         sh_rout.append(rout)
         sh_dr.append(dr)
-        sh_cstb.append(crgQtot * eps_diel / (RHO_WAT * mu_w * 4.0 * math.pi * rshell ** 2))
         rout -= dr
+    sh_cstb = list(shell_cstb(qcrg, lam, sh_rout, sh_dr))
     s.par = dict(Rcpp=cut, Rcpw=cut, Rcww=cut, Rcq=rcq, RcLRF=rclrf, use_LRF=int(use_lrf), rexcl_o=radius,
                  rwat=rwat, fk_wsphere=60.0, Dwmz=Dwmz, awmz=awmz, Boltz=BOLTZ, fkwpol=20.0, itdis=100.0,
                  wpol_restr=1, nwpolr_shell=nsh, xpcent=np.zeros(3), xwcent=np.zeros(3),
