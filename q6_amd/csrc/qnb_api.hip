@@ -107,6 +107,13 @@ void fill_params(DevSys& S, const qnb_system* s, const qnb_params* p) {
   S.ivdw_rule = s->ivdw_rule;
   S.any_atom = s->any_atom_cutoff ? 1 : 0;
   S.spc = s->solvent_spc;
+  S.pw_coul_h = 0;
+  if (s->nwat > 0 && s->nat_solute > 0) {
+    bool zero = true;
+    for (size_t t = (size_t)s->nat_solute; t < (size_t)s->nat_solute * s->solv_atom && zero; t++)
+      zero = s->pw_A[t] == 0.0 && s->pw_B[t] == 0.0;
+    S.pw_coul_h = zero ? 1 : 0;
+  }
   S.use_lrf = p->use_LRF;
   S.prec = p->precision;
   S.quirk = p->ref_quirks;

@@ -29,6 +29,9 @@ enum {
 struct DevSys {
   // sizes
   int natom, nat_solute, nwat, sa, ncgp, ncgp_solute, nqat, nstates, ntypes, ivdw_rule, spc;
+  // 1: every solute-water LJ parameter of solvent atoms 1.. is zero (their
+  // pw pairs reduce exactly to Coulomb)
+  int pw_coul_h;
   int any_atom;  // iuse_switch_atom = 0: group cut-off tests take any atom pair
   int use_lrf, prec, quirk;
   double rcpp2, rcpw2, rcww2, rcq2, rclrf2, rexcl2;
@@ -237,6 +240,162 @@ __device__ __forceinline__ double wave_sum_all(double v) {
   v += dpp_move<0x141>(v);  // row_half_mirror
   v += dpp_move<0x140>(v);  // row_mirror
   return ((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48);
+}
+
+// Sums N per-lane values over the wave at once ("transposed" reduction): a
+// step pairs every lane with a partner and, while each lane still holds
+// several values, the lane keeps half of them (the half its lane bit names)
+// and adds the partner's copy of that half, so the number of values per
+// lane halves each step.  Pairings: row_mirror (lane bit 3), row_half_mirror
+// (bit 2), quad_perm xor 2 (bit 1), xor 1 (bit 0), then v_permlane16_swap
+// (bit 4: rows 0<->1, 2<->3) and v_permlane32_swap (bit 5: halves); the
+// mirrors come first because they also flip the lower bits, which must not
+// yet decide anything.  Once a lane holds one value the remaining steps are
+// plain butterflies.  Returns, in every lane, the full sum of value
+// wave_sum_tr_index<N>(lane).  About 7 VALU per value (13% of N separate
+// wave_sum_all reductions for N = 30).
+__host__ __device__ constexpr int pow2_ceil(int n) { return n <= 1 ? 1 : 2 * pow2_ceil((n + 1) / 2); }
+__host__ __device__ constexpr int log2_exact(int p) { return p <= 1 ? 0 : 1 + log2_exact(p / 2); }
+
+template <int CTRL, int C>
+__device__ __forceinline__ void tr_dpp_step(double* v, bool upper) {
+  if constexpr (C >= 2) {
+#pragma unroll
+    for (int j = 0; j < C / 2; j++) {
+      const double x = v[j], y = v[j + C / 2];
+      const double send = upper ? x : y;  // the half the partner keeps
+      const double keep = upper ? y : x;
+      v[j] = keep + dpp_move<CTRL>(send);
+    }
+  } else {
+    v[0] += dpp_move<CTRL>(v[0]);
+  }
+}
+
+__device__ __forceinline__ void swap16(double& x, double& y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y), false,
+                                                   false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y), false,
+                                                   false);
+  x = __hiloint2double((int)hi[0], (int)lo[0]);
+  y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ void swap32(double& x, double& y) {
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y), false,
+                                                   false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y), false,
+                                                   false);
+  x = __hiloint2double((int)hi[0], (int)lo[0]);
+  y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+// rows (lane bit 4) or halves (bit 5): after the swap, x + y holds in every
+// lane the pair's sum of the half that lane keeps
+template <int C, bool HALVES>
+__device__ __forceinline__ void tr_swap_step(double* v) {
+  if constexpr (C >= 2) {
+#pragma unroll
+    for (int j = 0; j < C / 2; j++) {
+      double x = v[j], y = v[j + C / 2];
+      if constexpr (HALVES) swap32(x, y);
+      else swap16(x, y);
+      v[j] = x + y;
+    }
+  } else {
+    double x = v[0], y = v[0];
+    if constexpr (HALVES) swap32(x, y);
+    else swap16(x, y);
+    v[0] = x + y;
+  }
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// per-lane partials of type T (float or double); the first step exchanges
+// them in T and adds in FP64 ((double)a + (double)b, exact conversions), so
+// float partials are never summed in float and the wide FP64 copy of the N
+// values is never live at once
+template <typename T, int N>
+__device__ __forceinline__ double wave_sum_tr(const T (&in)[N], int lane) {
+  constexpr int P = pow2_ceil(N);
+  static_assert(P <= 64, "at most 64 values");
+  static_assert(QNB_NE <= 64, "one energy record per wave store");
+  double v[P];
+  const bool up = lane & 8;
+  if constexpr (P >= 2) {
+#pragma unroll
+    for (int j = 0; j < P / 2; j++) {
+      const T x = in[j], y = j + P / 2 < N ? in[j + P / 2] : T(0);
+      const T send = up ? x : y;
+      const T keep = up ? y : x;
+      T got;
+      if constexpr (sizeof(T) == 4) got = dpp_move_f<0x140>(send);
+      else got = dpp_move<0x140>(send);
+      v[j] = (double)keep + (double)got;
+    }
+  } else {
+    T got;
+    if constexpr (sizeof(T) == 4) got = dpp_move_f<0x140>(in[0]);
+    else got = dpp_move<0x140>(in[0]);
+    v[0] = (double)in[0] + (double)got;
+  }
+  tr_dpp_step<0x141, (P >= 2 ? P / 2 : 1)>(v, lane & 4);
+  tr_dpp_step<0x4E, (P >= 4 ? P / 4 : 1)>(v, lane & 2);
+  tr_dpp_step<0xB1, (P >= 8 ? P / 8 : 1)>(v, lane & 1);
+  tr_swap_step<(P >= 16 ? P / 16 : 1), false>(v);
+  tr_swap_step<(P >= 32 ? P / 32 : 1), true>(v);
+  return v[0];
+}
+
+// the value whose sum lane `lane` holds after wave_sum_tr<N>, and whether
+// the lane is that value's one writer (its bits that decided nothing are 0)
+template <int N>
+__device__ __forceinline__ int wave_sum_tr_index(int lane) {
+  constexpr int L = log2_exact(pow2_ceil(N));
+  const int bits[6] = {3, 2, 1, 0, 4, 5};
+  int j = 0;
+#pragma unroll
+  for (int s = 0; s < L; s++) j = (j << 1) | ((lane >> bits[s]) & 1);
+  return j;
+}
+template <int N>
+__device__ __forceinline__ bool wave_sum_tr_writer(int lane) {
+  constexpr int L = log2_exact(pow2_ceil(N));
+  constexpr int bits[6] = {3, 2, 1, 0, 4, 5};
+  int decided = 0;
+#pragma unroll
+  for (int s = 0; s < L; s++) decided |= 1 << bits[s];
+  return (lane & ~decided) == 0 && wave_sum_tr_index<N>(lane) < N;
+}
+
+// the lane that holds value j after wave_sum_tr<N> and writes it
+template <int N>
+__host__ __device__ constexpr int wave_sum_tr_lane(int j) {
+  constexpr int L = log2_exact(pow2_ceil(N));
+  constexpr int bits[6] = {3, 2, 1, 0, 4, 5};
+  int lane = 0;
+  for (int s = 0; s < L; s++) lane |= ((j >> (L - 1 - s)) & 1) << bits[s];
+  return lane;
+}
+
+// A wave's energy record ep[0 .. QNB_NE) in one coalesced store from lanes
+// 0 .. QNB_NE-1: the N per-lane partials are summed with one transposed
+// reduction, each sum is read from its lane, and entry slot[j] takes sum j,
+// every other entry 0.
+template <int N>
+__device__ __forceinline__ void store_energy_record(double* ep, const double (&part)[N], const int (&slot)[N],
+                                                    int lane) {
+  const double sum = wave_sum_tr(part, lane);
+  double v = 0.0;
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const double vj = lane_value(sum, wave_sum_tr_lane<N>(j));
+    if (lane == slot[j]) v = vj;
+  }
+  if (lane < QNB_NE) *(__attribute__((address_space(1))) double*)(ep + lane) = v;
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -465,8 +624,11 @@ inline int qforces_blocks(int nq, int chunks, int W) { return 8 * ((chunks + W -
 #define FC_CAP 384
 #endif
 #define FC_CH 10
+// solute receivers: at most 168 VGPRs, so three waves fit a SIMD (the
+// no-energy FP32 form fits; the energy-sampling and FP64 forms spill a few
+// bytes to scratch)
 #ifndef FC_OCC_ATTR
-#define FC_OCC_ATTR
+#define FC_OCC_ATTR __attribute__((amdgpu_waves_per_eu(3)))
 #endif
 template <typename R>
 __host__ __device__ constexpr size_t fc_wave_bytes() {

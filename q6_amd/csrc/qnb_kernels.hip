@@ -942,42 +942,37 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_sforces(const DevSy
           f[k][2] += fq[2];
         }
     }
-    // reduce (the per-atom LRF Taylor term follows in k_atom_terms)
+    // reduce all 3 CH force components at once (the per-atom LRF Taylor term
+    // follows in k_atom_terms); component c of sweep atom k is value 3k + c
+    {
+      R fv[3 * CH];
 #pragma unroll
-    for (int k = 0; k < CH; k++) {
-      if (k >= nk) continue;
-      const double fx = wave_sum_all((double)f[k][0]), fy = wave_sum_all((double)f[k][1]), fz = wave_sum_all((double)f[k][2]);
-      if (lane == k) {
-        const int i = ldg(S.cgpatom + (g0 + c0 + k));
-        d[3 * i] = -fx;
-        d[3 * i + 1] = -fy;
-        d[3 * i + 2] = -fz;
+      for (int k = 0; k < CH; k++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) fv[3 * k + c] = f[k][c];
+      const double fs = wave_sum_tr(fv, lane);
+      const int j = wave_sum_tr_index<3 * CH>(lane);
+      if (wave_sum_tr_writer<3 * CH>(lane) && j / 3 < nk) {
+        const int i = ldg(S.cgpatom + (g0 + c0 + j / 3));
+        d[3 * i + j % 3] = -fs;
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
   if (!want_e) return;
-  double* ep = S.epart + (size_t)a * QNB_NE;
-  const double v0 = wave_sum_all(e_ppel), v1 = wave_sum_all(e_ppvdw), v2 = wave_sum_all(e_pwel), v3 = wave_sum_all(e_pwvdw);
-  if (lane == 0)
-    for (int k = 0; k < 9; k++)
-      ep[k] = k == E_PP_EL ? v0 : (k == E_PP_VDW ? v1 : (k == E_PW_EL ? v2 : (k == E_PW_VDW ? v3 : 0.0)));
+  // pp and pw energies, then each state's qp el and vdW
+  double part[4 + 2 * NS];
+  int slot[4 + 2 * NS];
+  part[0] = e_ppel, slot[0] = E_PP_EL;
+  part[1] = e_ppvdw, slot[1] = E_PP_VDW;
+  part[2] = e_pwel, slot[2] = E_PW_EL;
+  part[3] = e_pwvdw, slot[3] = E_PW_VDW;
 #pragma unroll
-  for (int s = 0; s < QNB_MAXST; s++) {
-    double q0 = 0.0, q1 = 0.0;
-    if (s < NS) {
-      q0 = wave_sum_all(eq[2 * (s < NS ? s : 0)]);
-      q1 = wave_sum_all(eq[2 * (s < NS ? s : 0) + 1]);
-    }
-    if (lane == 0) {
-      ep[E_Q0 + 6 * s + 0] = 0.0;
-      ep[E_Q0 + 6 * s + 1] = 0.0;
-      ep[E_Q0 + 6 * s + 2] = q0;
-      ep[E_Q0 + 6 * s + 3] = q1;
-      ep[E_Q0 + 6 * s + 4] = 0.0;
-      ep[E_Q0 + 6 * s + 5] = 0.0;
-    }
+  for (int s = 0; s < NS; s++) {
+    part[4 + 2 * s] = eq[2 * s], slot[4 + 2 * s] = E_Q0 + 6 * s + 2;
+    part[5 + 2 * s] = eq[2 * s + 1], slot[5 + 2 * s] = E_Q0 + 6 * s + 3;
   }
+  store_energy_record(S.epart + (size_t)a * QNB_NE, part, slot, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1067,7 +1062,10 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_wforces(const DevSys* __restric
           for (int k = 0; k < 3; k++) {
             const R ux = dx - o[k][0], uy = dy - o[k][1], uz = dz - o[k][2];
             R ee = R(0), ev = R(0);
-            const R dv = PairMath<R>::nbe(ux, uy, uz, pw[3 * k], pw[3 * k + 1], pw[3 * k + 2], ee, ev, false);
+            // sites without LJ: nbe with A = B = 0 is nbe_spc exactly
+            const R dv = (k > 0 && S.pw_coul_h)
+                             ? PairMath<R>::nbe_spc(ux, uy, uz, pw[3 * k + 2], ee, false)
+                             : PairMath<R>::nbe(ux, uy, uz, pw[3 * k], pw[3 * k + 1], pw[3 * k + 2], ee, ev, false);
             f[k][0] += ux * dv;
             f[k][1] += uy * dv;
             f[k][2] += uz * dv;
@@ -1158,37 +1156,29 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_wforces(const DevSys* __restric
       }
     }
   }
+  {
+    // the nine force components in one transposed reduction; component c of
+    // atom k is value 3k + c, i.e. d[3 * ra + j]
+    R fv[9];
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const double fx = wave_sum_all((double)f[k][0]), fy = wave_sum_all((double)f[k][1]), fz = wave_sum_all((double)f[k][2]);
-    if (lane == k) {
-      const int i = ra + k;
-      d[3 * i] = -fx;
-      d[3 * i + 1] = -fy;
-      d[3 * i + 2] = -fz;
-    }
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) fv[3 * k + c] = f[k][c];
+    const double fs = wave_sum_tr(fv, lane);
+    if (wave_sum_tr_writer<9>(lane)) d[3 * ra + wave_sum_tr_index<9>(lane)] = -fs;
   }
   if (!want_e) return;
-  double* ep = S.epart + (size_t)a * QNB_NE;
-  const double vel = wave_sum_all(e_el), vvdw = wave_sum_all(e_vdw);
-  if (lane == 0)
-    for (int k = 0; k < 9; k++) ep[k] = k == E_WW_EL ? vel : (k == E_WW_VDW ? vvdw : 0.0);
+  // ww energies, then each state's qw el and vdW
+  double part[2 + 2 * NS];
+  int slot[2 + 2 * NS];
+  part[0] = e_el, slot[0] = E_WW_EL;
+  part[1] = e_vdw, slot[1] = E_WW_VDW;
 #pragma unroll
-  for (int s = 0; s < QNB_MAXST; s++) {
-    double v0 = 0.0, v1 = 0.0;
-    if (s < NS) {
-      v0 = wave_sum_all(eq[2 * (s < NS ? s : 0)]);
-      v1 = wave_sum_all(eq[2 * (s < NS ? s : 0) + 1]);
-    }
-    if (lane == 0) {
-      ep[E_Q0 + 6 * s + 0] = 0.0;
-      ep[E_Q0 + 6 * s + 1] = 0.0;
-      ep[E_Q0 + 6 * s + 2] = 0.0;
-      ep[E_Q0 + 6 * s + 3] = 0.0;
-      ep[E_Q0 + 6 * s + 4] = v0;
-      ep[E_Q0 + 6 * s + 5] = v1;
-    }
+  for (int s = 0; s < NS; s++) {
+    part[2 + 2 * s] = eq[2 * s], slot[2 + 2 * s] = E_Q0 + 6 * s + 4;
+    part[3 + 2 * s] = eq[2 * s + 1], slot[3 + 2 * s] = E_Q0 + 6 * s + 5;
   }
+  store_energy_record(S.epart + (size_t)a * QNB_NE, part, slot, lane);
 }
 
 // ---------------------------------------------------------------------------
