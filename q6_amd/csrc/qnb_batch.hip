@@ -521,6 +521,8 @@ struct qnb_batch {
   std::vector<std::unique_ptr<HostTopo>> topos;
   std::vector<int> topo_of;  // instance -> index into topos
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // second branch of the MD step's force pass (default priority)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraphExec_t cycle_exec = nullptr;
   hipGraph_t cycle_graph = nullptr;
   int maxg = 0, maxat = 0, maxsw = 0, maxw = 0, maxs = 0, ns_max = 1, prec = 0;
@@ -543,6 +545,9 @@ struct qnb_batch {
   ~qnb_batch() {
     if (cycle_exec) (void)hipGraphExecDestroy(cycle_exec);
     if (cycle_graph) (void)hipGraphDestroy(cycle_graph);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -587,18 +592,18 @@ static unsigned xcd_cols(int blocks) { return (unsigned)((blocks + 7) / 8 * 8); 
 
 // water receivers (k_wforces) with the block placement imap (map_block)
 template <typename R, int NS, bool E, int CAP>
-static void launch_water(qnb_batch* b, int wpb, int imap) {
+static void launch_water(qnb_batch* b, int wpb, int imap, hipStream_t st) {
   const int nb = (b->maxw + wpb - 1) / wpb;
   const dim3 grid = imap == 0 ? dim3(xcd_cols(nb), b->n) : dim3((unsigned)(8 * (((long)nb * b->n + 7) / 8)), 1);
-  hipLaunchKernelGGL((k_wforces<R, NS, E, CAP>), grid, dim3(64 * wpb), (wf_wave_bytes<R, CAP>() * wpb), b->stream,
-                     b->dsys, kStepMask, imap, nb, b->n);
+  hipLaunchKernelGGL((k_wforces<R, NS, E, CAP>), grid, dim3(64 * wpb), (wf_wave_bytes<R, CAP>() * wpb), st, b->dsys,
+                     kStepMask, imap, nb, b->n);
 }
 
 template <typename R, int NS, bool E>
-static void launch_forces_e(qnb_batch* b, int wpb, int parts) {
+static void launch_forces_e(qnb_batch* b, int wpb, int parts, hipStream_t wst) {
   if (b->maxw > 0 && (parts & 1)) {
-    if (b->small_groups) launch_water<R, NS, E, WF_CAP_SMALL>(b, wpb, b->imap_water);
-    else launch_water<R, NS, E, FC_CAP>(b, wpb, b->imap_water);
+    if (b->small_groups) launch_water<R, NS, E, WF_CAP_SMALL>(b, wpb, b->imap_water, wst);
+    else launch_water<R, NS, E, FC_CAP>(b, wpb, b->imap_water, wst);
   }
   if (b->maxs > 0 && (parts & 2)) {
     const int nb = (b->maxs + wpb - 1) / wpb, imap = b->imap_solute;
@@ -616,10 +621,12 @@ static void launch_forces_e(qnb_batch* b, int wpb, int parts) {
 // cannot tell, e.g. inside a graph whose period is not a multiple of every
 // instance's sampling interval); parts: 1 water receivers, 2 solute receivers
 template <typename R, int NS>
-static void launch_forces_t(qnb_batch* b, int wpb, bool energy = true, bool atom_terms = true, int parts = 3) {
+static void launch_forces_t(qnb_batch* b, int wpb, bool energy = true, bool atom_terms = true, int parts = 3,
+                            hipStream_t wst = nullptr) {
   wpb = forces_wpb<R>(wpb);
-  if (energy) launch_forces_e<R, NS, true>(b, wpb, parts);
-  else launch_forces_e<R, NS, false>(b, wpb, parts);
+  if (!wst) wst = b->stream;
+  if (energy) launch_forces_e<R, NS, true>(b, wpb, parts, wst);
+  else launch_forces_e<R, NS, false>(b, wpb, parts, wst);
   if (atom_terms)
     hipLaunchKernelGGL(k_atom_terms, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys, kStepMask,
                        (const double*)b->tfree);
@@ -660,6 +667,28 @@ static void launch_qforces(qnb_batch* b) {
 
 // atom_terms = false: the caller runs the per-atom terms itself (the MD step
 // fuses them with the bonded terms in k_atom_step)
+// The MD step's force pass on two streams: the water receivers and the
+// polarisation restraint on the side stream, the solute receivers, the
+// Q-atom side and the Q-Q pairs on the batch stream.  The two branches write
+// disjoint gradient rows (waters / solute atoms) and disjoint energy slots;
+// the batch stream waits for the side branch before the per-atom terms.
+template <typename R, int NS>
+static void enqueue_forces_forked(qnb_batch* b, bool energy) {
+  const int wpb = 4, B = b->n;
+  (void)hipEventRecord(b->ev_fork, b->stream);
+  (void)hipStreamWaitEvent(b->side, b->ev_fork, 0);
+  if (b->maxw > 0) launch_forces_t<R, NS>(b, b->wpb_forces, energy, false, 1, b->side);
+  if (b->anypol) launch_watpol(b->side, (b->maxw + WP_THREADS - 1) / WP_THREADS, B, b->dsys, (const int*)b->istep);
+  launch_forces_t<R, NS>(b, b->wpb_forces, energy, false, 2);
+  if (b->anyq) {
+    hipLaunchKernelGGL((k_qforces<R, NS>), dim3(qforces_blocks(b->nq_max, b->qchunks, wpb), B), dim3(64 * wpb), 0,
+                       b->stream, b->dsys, kStepMask, b->qchunks);
+    hipLaunchKernelGGL((k_qq<R, NS>), dim3(B), dim3(QQ_THREADS), 0, b->stream, b->dsys, kStepMask);
+  }
+  (void)hipEventRecord(b->ev_join, b->side);
+  (void)hipStreamWaitEvent(b->stream, b->ev_join, 0);
+}
+
 template <typename R, int NS>
 static void enqueue_forces(qnb_batch* b, bool energy = true, bool atom_terms = true) {
   const int wpb = 4, B = b->n;
@@ -688,11 +717,11 @@ static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
     if (b->anyq && !b->static_q) hipLaunchKernelGGL(k_qlists, dim3(B), dim3(1024), 0, b->stream, b->dsys);
   }
   switch (b->ns_max) {
-    case 1: enqueue_forces<R, 1>(b, energy, false); break;
-    case 2: enqueue_forces<R, 2>(b, energy, false); break;
+    case 1: enqueue_forces_forked<R, 1>(b, energy); break;
+    case 2: enqueue_forces_forked<R, 2>(b, energy); break;
     case 3:
-    case 4: enqueue_forces<R, 4>(b, energy, false); break;
-    default: enqueue_forces<R, 8>(b, energy, false); break;
+    case 4: enqueue_forces_forked<R, 4>(b, energy); break;
+    default: enqueue_forces_forked<R, 8>(b, energy); break;
   }
   hipLaunchKernelGGL(k_atom_step, dim3((b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd,
                      (const double*)b->tfree);
@@ -796,6 +825,9 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
   }
   for (int k = 0; k < n; k++) b->nq_max = std::max(b->nq_max, b->hs[k].nqat);
   HIPCHK(hipStreamCreate(&b->stream));
+  HIPCHK(hipStreamCreate(&b->side));
+  HIPCHK(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
   if ((rc = b->mem.alloc(&b->tfree, n)) || (rc = b->mem.alloc(&b->istep, n))) return rc;
   HIPCHK(hipMemset(b->istep, 0, sizeof(int) * n));
   // MD records
