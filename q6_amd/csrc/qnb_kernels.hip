@@ -1234,7 +1234,10 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
   const double* __restrict__ x = S.x;
   const int i = ldg(S.iqseq + iq);
   const int nq = S.nqat, ns = S.nstates, sa = S.sa;
-  double f[3] = {0.0, 0.0, 0.0};
+  const double xi0 = ldg(x + 3 * i), xi1 = ldg(x + 3 * i + 1), xi2 = ldg(x + 3 * i + 2);
+  // per-lane partials in R (a lane takes one partner per chunk), summed over
+  // the wave in FP64
+  R f[3] = {R(0), R(0), R(0)};
   double dummy_e = 0.0, dummy_v = 0.0;
   const R *qtA = qp_tab<R>(S.qpA, S.qpA_f), *qtB = qp_tab<R>(S.qpB, S.qpB_f);
   const R *qtE = qp_tab<R>(S.qpEl, S.qpEl_f), *qtS = qp_tab<R>(S.qpSc, S.qpSc_f);
@@ -1245,8 +1248,7 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
       const int o = S.nat_solute + sa * ldg(S.qw_list + p);
       for (int k = 0; k < sa; k++) {
         const int j = o + k;
-        const double ux = ldg(x + 3 * j) - ldg(x + 3 * i), uy = ldg(x + 3 * j + 1) - ldg(x + 3 * i + 1),
-                     uz = ldg(x + 3 * j + 2) - ldg(x + 3 * i + 2);
+        const double ux = ldg(x + 3 * j) - xi0, uy = ldg(x + 3 * j + 1) - xi1, uz = ldg(x + 3 * j + 2) - xi2;
         const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
         R r2, r;
         rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
@@ -1269,16 +1271,15 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
             dvs += -r2 * ((R)ldg(S.qwEl + t) * r) * (R)S.lambda[s];
           }
         }
-        f[0] -= (double)(rux * dvs);
-        f[1] -= (double)(ruy * dvs);
-        f[2] -= (double)(ruz * dvs);
+        f[0] -= rux * dvs;
+        f[1] -= ruy * dvs;
+        f[2] -= ruz * dvs;
       }
     } else {
       const int pp = p - nw_ch * QNB_WAVE;
       if (pp >= nqp) continue;
       const int j = ldg(S.qp_list + pp);
-      const double ux = ldg(x + 3 * j) - ldg(x + 3 * i), uy = ldg(x + 3 * j + 1) - ldg(x + 3 * i + 1),
-                   uz = ldg(x + 3 * j + 2) - ldg(x + 3 * i + 2);
+      const double ux = ldg(x + 3 * j) - xi0, uy = ldg(x + 3 * j + 1) - xi1, uz = ldg(x + 3 * j + 2) - xi2;
       const R rux = (R)ux, ruy = (R)uy, ruz = (R)uz;
       R r2, r;
       rinv(rux * rux + ruy * ruy + ruz * ruz, r2, r);
@@ -1289,20 +1290,15 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
         const size_t t = (size_t)j + (size_t)S.nat_solute * ((size_t)iq + (size_t)nq * s);
         R dv = PairMath<R>::nbe_qx(r2, r, r6, ldg(qtA + t), ldg(qtB + t), ldg(qtE + t), ldg(qtS + t),
                                    (R)S.lambda[s], dummy_e, dummy_v, false);
-        f[0] -= (double)(rux * dv);
-        f[1] -= (double)(ruy * dv);
-        f[2] -= (double)(ruz * dv);
+        f[0] -= rux * dv;
+        f[1] -= ruy * dv;
+        f[2] -= ruz * dv;
       }
     }
   }
-  f[0] = wave_sum(f[0]);
-  f[1] = wave_sum(f[1]);
-  f[2] = wave_sum(f[2]);
-  if (lane == 0 && (f[0] != 0.0 || f[1] != 0.0 || f[2] != 0.0)) {
-    unsafeAtomicAdd(&S.d[3 * i], f[0]);
-    unsafeAtomicAdd(&S.d[3 * i + 1], f[1]);
-    unsafeAtomicAdd(&S.d[3 * i + 2], f[2]);
-  }
+  const double fs = wave_sum_tr(f, lane);
+  const int c = wave_sum_tr_index<3>(lane);
+  if (wave_sum_tr_writer<3>(lane) && fs != 0.0) unsafeAtomicAdd(&S.d[3 * i + c], fs);
 }
 
 // ---------------------------------------------------------------------------
