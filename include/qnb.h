@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define QNB_ABI_VERSION 4
+#define QNB_ABI_VERSION 5
 
 enum qnb_status {
   QNB_OK = 0,
@@ -507,6 +507,35 @@ int32_t qnb_batch_frames(qnb_batch *b, int32_t k, double *out, int32_t cap, int3
  * nframes_k x nstates_k x nqat_k x QNB_ADJ_NV); *n the number of float64
  * written (also when it exceeds cap, then nothing is copied). */
 int32_t qnb_batch_adjoint(qnb_batch *b, const qnb_qtangent *const *tan, double *out, int64_t cap, int64_t *n);
+
+/* ------------------------------------------------------------------------
+ * Group contributions (calculate_exc, src/nonbondene.f90:6825-6920): energy
+ * arrays that repeat the per-state Q-atom energies with the interactions
+ * between the Q-atoms and a group of solute atoms removed, as Qdyn6 writes
+ * them after the first array of an energy file for the input's
+ * [group_contribution] lines (src/md.f90:1011-1070; a line of type "all"
+ * gives three arrays: full, electro, vdw).  Array a: mask[a*nat_solute + i]
+ * = 1 for the group's solute atoms, type[a] one of QNB_GC_*.  The pairs are
+ * those exc_assign lists (src/simprep.f90:2868-2947) from the Q tables of
+ * `sys` (the system the engine was created from), with no cut-off.  Results
+ * per array a and state s: QNB_GC_NV float64 = qq el, qq vdW, qp el, qp vdW,
+ * qx el, qx vdW, total. */
+enum qnb_gc_type { QNB_GC_FULL = 1, QNB_GC_ELECTRO = 2, QNB_GC_VDW = 3 };
+#define QNB_GC_NV 7
+/* Sets (narr = 0: removes) the engine's arrays. */
+int32_t qnb_set_groups(qnb_handle *h, const qnb_system *sys, int32_t narr, const uint8_t *mask, const int32_t *type);
+/* calculate_exc at x (natom*3, host) from the per-state energies pot_energy
+ * left: eq_in[s*11 + 0..10] = qq el, qq vdW, qp el, qp vdW, qw el, qw vdW, Q
+ * bond, angle, torsion, improper, restraint of state s; out[(a*nstates + s)
+ * * QNB_GC_NV + k].  Pair by pair in the reference's order in FP64. */
+int32_t qnb_calculate_exc(qnb_handle *h, const double *x, const double *eq_in, double *out);
+/* The same for batch instance k, computed on every sampled step from the
+ * step's energy record; frames recorded before the arrays were set hold 0. */
+int32_t qnb_batch_set_groups(qnb_batch *b, int32_t k, const qnb_system *sys, int32_t narr, const uint8_t *mask,
+                             const int32_t *type);
+/* Copies instance k's group energies: nframes x narr x nstates x QNB_GC_NV
+ * float64 (nothing when *nframes exceeds cap frames). */
+int32_t qnb_batch_group_energies(qnb_batch *b, int32_t k, double *out, int32_t cap, int32_t *nframes);
 
 /* ------------------------------------------------------------------------
  * Free-energy analysis on the device: Qfep6's Part 1 (Zwanzig forward and

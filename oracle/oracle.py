@@ -35,6 +35,8 @@ def _lib():
     lib.orc_make_pair_lists.argtypes = [vp, C.POINTER(C.c_double)]
     lib.orc_eval_terms.argtypes = [vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
                                    C.c_int, C.POINTER(abi.QnbEnergies), C.POINTER(abi.QnbQEnergies)]
+    lib.orc_calculate_exc.argtypes = [vp, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_uint8),
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.orc_get_pairs.restype = C.c_int64
     lib.orc_get_pairs.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.c_int64]
     lib.orc_get_list_params.restype = C.c_int64
@@ -77,6 +79,21 @@ class Oracle:
         EQ = (self.abi.QnbQEnergies * self.nst)()
         self.lib.orc_eval_terms(self.h, mask, _pd(x), _pd(d), Tfree, istep, C.byref(E), EQ)
         return d, E.as_dict(), [q.as_dict() for q in EQ]
+
+    def calculate_exc(self, x, masks, types, eq_in):
+        """calculate_exc: masks (narr, nat_solute) 0/1, types (narr,) 1 full,
+        2 electro, 3 vdW; eq_in (nstates, 11) qq el, vdW, qp el, vdW, qw el,
+        vdW, Q bond, angle, torsion, improper, restraint.  Returns (narr,
+        nstates, 7): qq el, vdW, qp el, vdW, qx el, vdW, total."""
+        x = np.ascontiguousarray(x, np.float64)
+        masks = np.ascontiguousarray(masks, np.uint8)
+        types = np.ascontiguousarray(types, np.int32)
+        eq_in = np.ascontiguousarray(eq_in, np.float64)
+        narr = len(types)
+        out = np.zeros((narr, self.nst, 7))
+        self.lib.orc_calculate_exc(self.h, _pd(x), narr, masks.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                   types.ctypes.data_as(C.POINTER(C.c_int32)), _pd(eq_in), _pd(out))
+        return out
 
     def pairs(self, which):
         n = self.lib.orc_get_pairs(self.h, which, None, 0)

@@ -1028,3 +1028,91 @@ void orc_get_theta(orc_handle *h, double *theta, int32_t *n_insh) {
   memcpy(theta, h->theta, sizeof(double) * h->s.nwat);
   for (int k = 0; k < h->p.nwpolr_shell; k++) n_insh[k] = h->n_insh[k];
 }
+
+/* calculate_exc (nonbondene.f90:6825-6920) with the lists exc_assign builds
+ * (simprep.f90:2868-2947).  For each energy array a (mask over the solute
+ * atoms, type 1 full / 2 electro / 3 vdW) and state s, the Q energies of
+ * eq_in[s] (qq el, vdW, qp el, vdW, qw el, vdW, Q bond, angle, torsion,
+ * improper, restraint: what pot_energy left) minus the interactions of the
+ * Q-atoms with the masked atoms, pair by pair in list order: first the
+ * masked Q-atoms (Q-Q pairs, nbe_qq), then the masked atoms bonded to a
+ * Q-atom (qconn <= 3, nbe_qx), then the other masked atoms (nbe_qx).  The
+ * lists are built atom-major, Q-atom-minor; a Q-Q pair enters when its
+ * qq_precomp entry is set, a bonded-neighbour pair when its qp_precomp entry
+ * is set, and a plain Q-solute pair when qp_precomp(iq, iq, s) is set, the
+ * entry exc_assign tests (atom index = Q-atom number).  out[a][s][7]: qq el,
+ * vdW, qp el, vdW, qx el, vdW, total. */
+static void exc_sub(int type, double Vel, double Va, double Vb, double *el, double *vdw) {
+  if (type == 1) {
+    *el = *el - Vel;
+    *vdw = *vdw - (Va - Vb);
+  } else if (type == 2) {
+    *el = *el - Vel;
+  } else if (type == 3) {
+    *vdw = *vdw - (Va - Vb);
+  }
+}
+
+int orc_calculate_exc(orc_handle *h, const double *x, int narr, const uint8_t *masks, const int32_t *types,
+                      const double *eq_in, double *out) {
+  const qnb_system *s = &h->s;
+  const int nsol = s->nat_solute, nq = s->nqat, nst = s->nstates;
+  for (int a = 0; a < narr; a++) {
+    const uint8_t *mask = masks + (size_t)a * nsol;
+    for (int st = 0; st < nst; st++) {
+      const double *e = eq_in + 11 * st;
+      double qqel = e[0], qqvdw = e[1], qpel = e[2], qpvdw = e[3];
+      for (int ig = 0; ig < nsol; ig++) { /* Q-atoms of the group */
+        if (!mask[ig] || s->iqatom[ig] == 0) continue;
+        const int jq = s->iqatom[ig] - 1;
+        for (int iq = 0; iq < nq; iq++) {
+          const size_t k = (size_t)iq + (size_t)nq * ((size_t)jq + (size_t)nq * st);
+          if (!s->qq_set[k]) continue;
+          const int i = s->iqseq[iq], j = s->iqseq[jq];
+          vec3 t = {x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1], x[3 * j + 2] - x[3 * i + 2]};
+          double r2 = 1.0 / sq(t), r = sqrt(r2), r6d = r2 * r2 * r2;
+          double r6_hc = 1.0 / r6d;
+          double r6 = r6_hc + s->qq_sc[k];
+          r6 = 1.0 / r6;
+          double r12 = r6 * r6;
+          double Vel = s->qq_el[k] * r, Va, Vb;
+          if (s->qq_soft[k]) {
+            Vb = 0.0;
+            Va = s->qq_A[k] * exp(-s->qq_B[k] / r);
+          } else {
+            Va = s->qq_A[k] * r12;
+            Vb = s->qq_B[k] * r6;
+          }
+          exc_sub(types[a], Vel, Va, Vb, &qqel, &qqvdw);
+        }
+      }
+      for (int pass = 0; pass < 2; pass++) { /* bonded neighbours, then the others */
+        for (int ig = 0; ig < nsol; ig++) {
+          if (!mask[ig] || s->iqatom[ig] != 0) continue;
+          if ((pass == 0) != (s->qconn3[ig] != 0)) continue;
+          for (int iq = 0; iq < nq; iq++) {
+            const size_t k = (size_t)ig + (size_t)nsol * ((size_t)iq + (size_t)nq * st);
+            const size_t kt = pass == 0 ? k : (size_t)iq + (size_t)nsol * ((size_t)iq + (size_t)nq * st);
+            if (!s->qp_set[kt]) continue;
+            const int i = s->iqseq[iq];
+            vec3 t = {x[3 * ig] - x[3 * i], x[3 * ig + 1] - x[3 * i + 1], x[3 * ig + 2] - x[3 * i + 2]};
+            double r2 = 1.0 / sq(t), r = sqrt(r2), r6 = r2 * r2 * r2;
+            const double pr[4] = {s->qp_A[k], s->qp_B[k], s->qp_el[k], s->qp_sc[k]};
+            double Vel, Va, Vb;
+            (void)nbe_qx(r2, r, r6, pr, h->p.lambda[st], &Vel, &Va, &Vb);
+            exc_sub(types[a], Vel, Va, Vb, &qpel, &qpvdw);
+          }
+        }
+      }
+      double *o = out + 7 * ((size_t)a * nst + st);
+      o[0] = qqel;
+      o[1] = qqvdw;
+      o[2] = qpel;
+      o[3] = qpvdw;
+      o[4] = qqel + qpel + e[4];
+      o[5] = qqvdw + qpvdw + e[5];
+      o[6] = e[6] + e[7] + e[8] + e[9] + o[4] + o[5] + e[10];
+    }
+  }
+  return 0;
+}

@@ -23,6 +23,8 @@ void orc_set_lrf(orc_handle *h, const double *lrf);
 void orc_get_shell_state(orc_handle *h, double *st);
 void orc_set_shell_state(orc_handle *h, const double *st);
 void orc_get_theta(orc_handle *h, double *theta, int32_t *n_insh);
+int orc_calculate_exc(orc_handle *h, const double *x, int narr, const uint8_t *masks, const int32_t *types,
+                      const double *eq_in, double *out);
 
 #ifdef __cplusplus
 }

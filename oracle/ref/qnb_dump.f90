@@ -23,9 +23,11 @@ program qnb_dump
   use QATOM
   implicit none
   character(len=512) :: outname
+  character(len=16) :: mdflag
+  logical :: do_md
   integer :: u, n, is, i, k, nq, ns
   real(kind=prec) :: Ekinmax_loc
-  integer, allocatable :: itmp(:)
+  integer, allocatable :: itmp(:), iexc(:)
   real(kind=prec), allocatable :: dtmp(:)
   type(NB_ENERGIES) :: e_nb
   type(RESTRAINT_ENERGIES) :: e_rs
@@ -33,15 +35,22 @@ program qnb_dump
   type(NB_ENERGIES), allocatable :: eq_nb(:)
   real(kind=prec), allocatable :: lam(:)
   integer :: iseed0, tg, kc, ic, mol
-  real(kind=prec), allocatable :: v_mx(:), x_pc(:), sf0(:)
+  real(kind=prec), allocatable :: v_mx(:), x_pc(:), sf0(:), dtmp_exc(:, :, :)
   type(qr_vec), allocatable :: x0v(:), v0v(:)
 
   nodeid = 0
   numnodes = 1
   call md_startup
-  if (command_argument_count() < 2) stop 'usage: qnb_dump out.bin input.inp'
+  if (command_argument_count() < 2) stop 'usage: qnb_dump out.bin [nomd] input.inp'
   call get_command_argument(1, outname)
-  ! initialize() reads the input file named by the first argument (src/md.f90:161-170)
+  ! "nomd": leave out the MD-step record (a system whose start is not ready
+  ! for dynamics, e.g. an unrelaxed topology)
+  do_md = .true.
+  if (command_argument_count() >= 3) then
+    call get_command_argument(2, mdflag)
+    do_md = trim(mdflag) /= 'nomd'
+  end if
+  ! initialize() reads the input file named by the last argument (src/md.f90:161-170)
   if (.not. initialize()) call die('Invalid data in input file')
   call open_files(md=.true.)
   call topology
@@ -347,6 +356,46 @@ program qnb_dump
           EQ(is)%qp%el, EQ(is)%qp%vdw, EQ(is)%qw%el, EQ(is)%qw%vdw, EQ(is)%restraint, 0.0_prec, is=1,ns)/), &
           2, (/16, ns/))
 
+  ! ---------------- group contributions (calculate_exc) ----------------
+  ! the energy arrays 2.. of an energy step: per array (a group and a type)
+  ! and state the Q energies with the group's interactions removed
+  ! (src/nonbondene.f90:6825-6920 on the EQ pot_energy just left)
+  if (use_excluded_groups) then
+    call calculate_exc
+    n = ene_header%arrays - 1
+    allocate(iexc(max(n, nat_solute)))
+    do k = 2, ene_header%arrays
+      select case (ene_header%types(k))
+      case (FULL)
+        iexc(k - 1) = 1
+      case (ELECTRO)
+        iexc(k - 1) = 2
+      case (VDW)
+        iexc(k - 1) = 3
+      case default
+        iexc(k - 1) = 0
+      end select
+    end do
+    call wi(u, 'exc_type', n, iexc(1:n), 1, (/n/))
+    call wi(u, 'exc_group', n, int(ene_header%gcnum(2:ene_header%arrays)), 1, (/n/))
+    do k = 1, ngroups_gc
+      do i = 1, nat_solute
+        iexc(i) = merge(1, 0, logical(ST_gc(k)%gcmask%mask(i)))
+      end do
+      call wi(u, 'exc_mask'//char(48 + k), nat_solute, iexc(1:nat_solute), 1, (/nat_solute/))
+    end do
+    deallocate(iexc)
+    allocate(dtmp_exc(7, ns, n))
+    do k = 2, ene_header%arrays
+      do is = 1, ns
+        dtmp_exc(:, is, k - 1) = (/EQ_save(is)%qq(k)%el, EQ_save(is)%qq(k)%vdw, EQ_save(is)%qp(k)%el, &
+            EQ_save(is)%qp(k)%vdw, EQ_save(is)%qx(k)%el, EQ_save(is)%qx(k)%vdw, EQ_save(is)%total(k)/)
+      end do
+    end do
+    call wd(u, 'exc_EQ', 7*ns*n, reshape(dtmp_exc, (/7*ns*n/)), 3, (/7, ns, n/))
+    deallocate(dtmp_exc)
+  end if
+
   ! ---------------- the seeded start ----------------
   call wi1(u, 'iseed', iseed0)
   call wd1(u, 'Tmaxw', Tmaxw)
@@ -355,7 +404,13 @@ program qnb_dump
     call wd(u, 'x_preconstraint', 3*natom, x_pc, 2, (/3, natom/))
   end if
 
+  ! positions and velocities the later records start from
+  allocate(x0v(natom), v0v(natom), sf0(ntgroups))
+  x0v(1:natom) = x(1:natom)
+  v0v(1:natom) = v(1:natom)
+
   ! ---------------- one MD step (md_run, src/md.f90:1774-1797) ----------------
+  if (do_md) then
   ! from the x, v above and d = the gradient pot_energy just left (d_total):
   ! leap-frog with each temperature group's Berendsen factor, shake, the new
   ! velocities and temperature()
@@ -392,9 +447,6 @@ program qnb_dump
     deallocate(dtmp)
     allocate(dtmp(3*natom))
   end if
-  allocate(x0v(natom), v0v(natom), sf0(ntgroups))
-  x0v(1:natom) = x(1:natom)
-  v0v(1:natom) = v(1:natom)
   sf0(1:ntgroups) = tscale(1:ntgroups)%sfact
   call wd(u, 'md1_sfact_in', ntgroups, sf0, 1, (/ntgroups/))
   do tg = 1, ntgroups
@@ -413,6 +465,7 @@ program qnb_dump
           tscale(tg)%sfact, tg=1,ntgroups)/), 1, (/3 + 3*ntgroups/))
   x(1:natom) = x0v(1:natom)
   v(1:natom) = v0v(1:natom)
+  end if
 
   ! ---------------- the polarisation restraint over several steps ----------------
   ! watpol at istep = 1..7 with itdis = 3 from a zero shell state, each call

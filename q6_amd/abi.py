@@ -159,6 +159,11 @@ def qbonded_struct(tb, urey=False):
     q.urey_bradley = 1 if urey else 0
     return q, keep
 
+# group contributions: array types and values per (array, state)
+GC_FULL, GC_ELECTRO, GC_VDW = 1, 2, 3
+GC_NV = 7
+
+
 class QnbFepParams(C.Structure):
     _fields_ = [("kT", C.c_double), ("nskip", C.c_int32), ("nbins", C.c_int32), ("nmin", C.c_int32),
                 ("reserved", C.c_int32), ("alpha", C.c_double * QNB_MAX_STATES),
@@ -172,7 +177,7 @@ class QnbFepOut(C.Structure):
                 ("bin_c2", _pd), ("bin_r", _pd), ("ptsum", _pi)]
 
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # every symbol include/qnb.h declares (checked by the CPU test suite)
 EXPORTS = [
@@ -183,6 +188,7 @@ EXPORTS = [
     "qnb_batch_energy_history", "qnb_batch_pair_counts", "qnb_batch_time_kernel", "qnb_batch_minimize", "qnb_batch_autotune",
     "qnb_adjoint", "qnb_adjoint_static", "qnb_batch_store_frames", "qnb_batch_frames", "qnb_batch_adjoint", "qnb_batch_set_offdiag",
     "qnb_set_qbonded", "qnb_eval_qbonded", "qnb_batch_set_qbonded", "qnb_batch_free_energy", "qnb_free_energy",
+    "qnb_set_groups", "qnb_calculate_exc", "qnb_batch_set_groups", "qnb_batch_group_energies",
 ]
 
 _lib = None
@@ -227,6 +233,10 @@ def _declare(lib):
     lib.qnb_batch_free_energy.argtypes = [vp, C.POINTER(QnbFepParams), C.POINTER(QnbFepOut)]
     lib.qnb_free_energy.argtypes = [C.c_int32, C.c_int32, _pi, C.POINTER(_pd), C.POINTER(_pd), _pd,
                                     C.POINTER(QnbFepParams), C.POINTER(QnbFepOut)]
+    lib.qnb_set_groups.argtypes = [vp, C.POINTER(QnbSystem), C.c_int32, C.POINTER(C.c_uint8), _pi]
+    lib.qnb_calculate_exc.argtypes = [vp, _pd, _pd, _pd]
+    lib.qnb_batch_set_groups.argtypes = [vp, C.c_int32, C.POINTER(QnbSystem), C.c_int32, C.POINTER(C.c_uint8), _pi]
+    lib.qnb_batch_group_energies.argtypes = [vp, C.c_int32, _pd, C.c_int32, C.POINTER(C.c_int32)]
     lib.qnb_set_qbonded.argtypes = [vp, C.POINTER(QnbQbonded)]
     lib.qnb_eval_qbonded.argtypes = [vp, _pd, _pd, _pd]
     lib.qnb_batch_set_qbonded.argtypes = [vp, C.c_int32, C.POINTER(QnbQbonded)]

@@ -4,7 +4,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SOURCES = ["csrc/qnb_kernels.hip", "csrc/qnb_api.hip", "csrc/qnb_batch.hip", "csrc/qnb_adjoint.hip",
-           "csrc/qnb_qbonded.hip", "csrc/qnb_fep.hip"]
+           "csrc/qnb_qbonded.hip", "csrc/qnb_fep.hip", "csrc/qnb_exc.hip"]
 OUT = os.path.join(HERE, "libq6nb.so")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

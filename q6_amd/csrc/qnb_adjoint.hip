@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_adjoint(const DevSys* __restrict__ syst
   // Q-solute: atoms of solute groups inside Rcq (switch atom, or any atom
   // with iuse_switch_atom = 0), not 1-2/1-3 bonded to any Q-atom
   for (int j = lane; j < nsol; j += QNB_WAVE) {
-    if (S.qconn3[j]) continue;
+    if (S.qconn3[j] || S.atom_group[j] < 0) continue;
     if (!qp_group_in(S, x, S.atom_group[j])) continue;
     const R ux = (R)(x[3 * j] - xi0), uy = (R)(x[3 * j + 1] - xi1), uz = (R)(x[3 * j + 2] - xi2);
     R r2, r;

@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
 #include "qnb_host.h"
 
 namespace qnb {
@@ -164,8 +166,12 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
   std::vector<int> atom_group(nat, -1);
   for (int g = 0; g < s->ncgp; g++)
     for (int k = s->cgp_first[g]; k <= s->cgp_last[g]; k++) atom_group[s->cgpatom[k]] = g;
+  // Qdyn6 leaves excluded solute atoms out of every charge group when a
+  // topology is cut to a sphere; no pair list reaches them, and the per-atom
+  // pass, not a pair kernel, is the first writer of their gradient
   for (int i = 0; i < nat; i++)
-    if (atom_group[i] < 0) return fail(QNB_EINVAL, "atom %d is in no charge group", i);
+    if (atom_group[i] < 0 && !(i < nsol && s->excl[i] && s->iqatom[i] == 0))
+      return fail(QNB_EINVAL, "atom %d is in no charge group", i);
   const int nsol1 = std::max(nsol, 1);
   T.exmask.assign(nsol1, 0u);
   T.m14mask.assign(nsol1, 0u);
@@ -487,6 +493,8 @@ struct qnb_handle {
   QbDev qb{};                  // Q bonded terms (host copy of the device record)
   QbDev* dqb = nullptr;
   std::vector<void*> qb_owned;
+  ExcDev exc{};                         // group-contribution arrays (host copy of device pointers)
+  std::unique_ptr<DevAlloc> exc_mem;    // their lists, freed when the arrays are replaced
 };
 
 // host twin of the device pp_param filter, for list export
@@ -753,6 +761,41 @@ extern "C" int32_t qnb_eval_qbonded(qnb_handle* h, const double* x, double* d, d
   for (size_t k = 0; k < dd.size(); k++) d[k] += dd[k];
   for (int k = 0; k < 4 * ns; k++)
     if (!std::isfinite(eq[k])) return fail(QNB_ENONFINITE, "non-finite Q bonded energy");
+  return 0;
+}
+
+extern "C" int32_t qnb_set_groups(qnb_handle* h, const qnb_system* sys, int32_t narr, const uint8_t* mask,
+                                  const int32_t* type) {
+  if (!h || !sys) return fail(QNB_EINVAL, "null handle or system");
+  if (sys->natom != h->hs.natom || sys->nat_solute != h->hs.nat_solute || sys->nqat != h->hs.nqat ||
+      sys->nstates != h->hs.nstates)
+    return fail(QNB_EINVAL, "groups: not the system the engine was created from");
+  ExcHost X;
+  if (int rc = exc_build(sys, narr, mask, type, X)) return rc;
+  auto mem = std::make_unique<DevAlloc>();
+  ExcDev D{};
+  if (int rc = exc_upload(X, D, *mem)) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->exc = D;
+  h->exc_mem = std::move(mem);
+  return 0;
+}
+
+extern "C" int32_t qnb_calculate_exc(qnb_handle* h, const double* x, const double* eq_in, double* out) {
+  if (!h) return fail(QNB_EINVAL, "null handle");
+  const int n = h->exc.narr * h->exc.ns;
+  if (n == 0) return 0;
+  if (!x || !eq_in || !out) return fail(QNB_EINVAL, "null coordinates, energies or output");
+  if (int rc = upload_x(h, x)) return rc;
+  DevAlloc tmp;
+  double *de = nullptr, *dout = nullptr;
+  if (int rc = tmp.alloc(&de, (size_t)11 * h->exc.ns)) return rc;
+  if (int rc = tmp.alloc(&dout, (size_t)QNB_GC_NV * n)) return rc;
+  HIPCHK(hipMemcpyAsync(de, eq_in, sizeof(double) * 11 * h->exc.ns, hipMemcpyHostToDevice, h->stream));
+  launch_exc_eval(h->stream, h->exc, h->hs.x, de, dout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * QNB_GC_NV * n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
 }
 

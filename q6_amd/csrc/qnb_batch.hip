@@ -193,10 +193,7 @@ __global__ __launch_bounds__(256) void k_atom_step(const DevSys* __restrict__ sy
       radial_restraint(S, i, shift, f, er);
     }
     if (i < M.nbat) bonded_atom(S, M, i, f, eb, ea);
-    const auto d = gpm(S.d);
-    d[3 * i] += f[0];
-    d[3 * i + 1] += f[1];
-    d[3 * i + 2] += f[2];
+    store_atom_gradient(S, i, f);
     if (want_energies(S)) {
       stg(S.eatom + 2 * i, el);
       stg(S.eatom + 2 * i + 1, er);
@@ -541,6 +538,11 @@ struct qnb_batch {
   QbDev* dqb = nullptr;
   bool anyqb = false;
   bool anysnap = false;  // some instance stores coordinate snapshots (k_snap launched)
+  // group-contribution arrays per instance (k_exc_record after the energy record)
+  std::vector<ExcDev> hexc;
+  std::vector<std::unique_ptr<DevAlloc>> exc_mem;
+  ExcDev* dexc = nullptr;
+  int exc_threads = 0;  // largest narr * nstates over the instances (0: not launched)
   long long steps_done = 0;
   ~qnb_batch() {
     if (cycle_exec) (void)hipGraphExecDestroy(cycle_exec);
@@ -551,6 +553,9 @@ struct qnb_batch {
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+__global__ void k_exc_record(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds,
+                             const ExcDev* __restrict__ excs);
 
 // launchers shared by the step, the minimiser, the timer and the autotuner
 static void launch_lists(qnb_batch* b, int wpb) {
@@ -733,6 +738,9 @@ static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
   if (b->anysnap)
     hipLaunchKernelGGL(k_snap, dim3((3 * b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd);
   if (energy || b->anysnap) hipLaunchKernelGGL(k_record, dim3(B), dim3(64), 0, b->stream, b->dsys, b->dmd);
+  if (energy && b->exc_threads > 0)
+    hipLaunchKernelGGL(k_exc_record, dim3((b->exc_threads + 63) / 64, B), dim3(64), 0, b->stream, b->dsys, b->dmd,
+                       (const ExcDev*)b->dexc);
   launch_integrate(b);
 }
 
@@ -1100,6 +1108,29 @@ extern "C" int32_t qnb_batch_get_state(qnb_batch* b, int32_t k, double* x, doubl
   return 0;
 }
 
+// batch, after k_record on a sampled step: instance blockIdx.y, its frame of
+// this step (the record just written: frame ctr[0] - 1 with this step's
+// number); the state energies come from that record
+__global__ void k_exc_record(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds,
+                             const ExcDev* __restrict__ excs) {
+  const DevSys& S = systems[blockIdx.y];
+  const MdDev& M = mds[blockIdx.y];
+  const ExcDev& D = excs[blockIdx.y];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= D.narr * D.ns || M.ene_every <= 0) return;
+  const int step = *M.istep;
+  if (step % M.ene_every != 0) return;
+  const int f = M.ctr[0] - 1;
+  if (f < 0 || f >= D.max_frames) return;
+  const double* h = M.hist + (size_t)f * QNB_FRAME_LEN;
+  if (h[QNB_F_STEP] != (double)step) return;
+  const int a = t / D.ns, st = t % D.ns;
+  const double* q = h + QNB_F_Q0 + 6 * st;
+  const double* b = h + QNB_F_QB0 + 4 * st;
+  const double e[11] = {q[0], q[1], q[2], q[3], q[4], q[5], b[0], b[1], b[2], b[3], 0.0};
+  exc_walk(D, S.x, a, st, e, D.hist + ((size_t)f * D.narr * D.ns + t) * QNB_GC_NV);
+}
+
 extern "C" int32_t qnb_batch_energy_history(qnb_batch* b, int32_t k, double* out, int32_t cap, int32_t* nframes) {
   if (!b || k < 0 || k >= b->n || !nframes) return fail(QNB_EINVAL, "bad instance");
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -1108,6 +1139,54 @@ extern "C" int32_t qnb_batch_energy_history(qnb_batch* b, int32_t k, double* out
   *nframes = ctr[0];
   if (out && ctr[0] <= cap && ctr[0] > 0)
     HIPCHK(hipMemcpy(out, b->hm[k].hist, sizeof(double) * QNB_FRAME_LEN * ctr[0], hipMemcpyDeviceToHost));
+  return 0;
+}
+
+static void drop_step_graph(qnb_batch* b);
+
+extern "C" int32_t qnb_batch_set_groups(qnb_batch* b, int32_t k, const qnb_system* sys, int32_t narr,
+                                        const uint8_t* mask, const int32_t* type) {
+  if (!b || k < 0 || k >= b->n || !sys) return fail(QNB_EINVAL, "bad instance or system");
+  const DevSys& S = b->hs[k];
+  if (sys->natom != S.natom || sys->nat_solute != S.nat_solute || sys->nqat != S.nqat || sys->nstates != S.nstates)
+    return fail(QNB_EINVAL, "groups: not the system instance %d was created from", k);
+  ExcHost X;
+  if (int rc = exc_build(sys, narr, mask, type, X)) return rc;
+  auto mem = std::make_unique<DevAlloc>();
+  ExcDev D{};
+  if (int rc = exc_upload(X, D, *mem)) return rc;
+  D.max_frames = b->hm[k].max_frames;
+  D.hist = nullptr;
+  if (narr > 0) {
+    const size_t nv = (size_t)std::max(D.max_frames, 1) * narr * S.nstates * QNB_GC_NV;
+    if (int rc = mem->alloc(&D.hist, nv)) return rc;
+    HIPCHK(hipMemset(D.hist, 0, sizeof(double) * nv));
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->hexc.empty()) {
+    b->hexc.assign(b->n, ExcDev{});
+    b->exc_mem.resize(b->n);
+    if (int rc = b->mem.alloc(&b->dexc, (size_t)b->n)) return rc;
+  }
+  b->hexc[k] = D;
+  b->exc_mem[k] = std::move(mem);
+  HIPCHK(hipMemcpy(b->dexc, b->hexc.data(), sizeof(ExcDev) * b->n, hipMemcpyHostToDevice));
+  b->exc_threads = 0;
+  for (const ExcDev& E : b->hexc) b->exc_threads = std::max(b->exc_threads, E.narr * E.ns);
+  drop_step_graph(b);  // the step graph holds the launch list
+  return 0;
+}
+
+extern "C" int32_t qnb_batch_group_energies(qnb_batch* b, int32_t k, double* out, int32_t cap, int32_t* nframes) {
+  if (!b || k < 0 || k >= b->n || !nframes) return fail(QNB_EINVAL, "bad instance");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  int ctr[4];
+  HIPCHK(hipMemcpy(ctr, b->hm[k].ctr, sizeof ctr, hipMemcpyDeviceToHost));
+  *nframes = ctr[0];
+  if (b->hexc.empty() || b->hexc[k].narr == 0) return 0;
+  const ExcDev& D = b->hexc[k];
+  if (out && ctr[0] <= cap && ctr[0] > 0)
+    HIPCHK(hipMemcpy(out, D.hist, sizeof(double) * QNB_GC_NV * D.narr * D.ns * ctr[0], hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/qnb.h"
+
 #define QNB_WAVE 64
 #define QNB_NLRF 43
 #define QNB_MAXST 8
@@ -398,6 +400,70 @@ __device__ __forceinline__ void store_energy_record(double* ep, const double (&p
   if (lane < QNB_NE) *(__attribute__((address_space(1))) double*)(ep + lane) = v;
 }
 
+// group contributions (calculate_exc): one pair of an energy array's list,
+// kind 0 Q-Q (nbe_qq; soft: the exponential soft pair), 1 Q-solute (nbe_qx);
+// i the Q-atom's index, j the partner's; the pair's A, B, el and soft-core
+// score as the Q tables hold them
+struct ExcEnt {
+  int kind, i, j, soft;
+  double A, B, el, sc;
+};
+// one system's arrays: off[a*ns + s] .. off[a*ns + s + 1] the pairs of array
+// a in state s, type[a] (QNB_GC_*); hist: the batch's per-frame results
+// [frame][narr][ns][QNB_GC_NV]
+struct ExcDev {
+  int narr, ns, max_frames;
+  const int* off;
+  const ExcEnt* ent;
+  const int* type;
+  double* hist;
+};
+
+// the walk of one (array, state): e[11] the state's energies (qq el, vdW, qp
+// el, vdW, qw el, vdW, Q bond, angle, torsion, improper, restraint), o[7]
+__device__ __forceinline__ void exc_walk(const ExcDev& D, const double* __restrict__ x, int a, int st,
+                                         const double* e, double* o) {
+#pragma clang fp contract(off)
+  double qqel = e[0], qqvdw = e[1], qpel = e[2], qpvdw = e[3];
+  const int ty = D.type[a];
+  const int p0 = D.off[a * D.ns + st], p1 = D.off[a * D.ns + st + 1];
+  for (int p = p0; p < p1; p++) {
+    const ExcEnt& t = D.ent[p];
+    const double ux = x[3 * t.j] - x[3 * t.i], uy = x[3 * t.j + 1] - x[3 * t.i + 1], uz = x[3 * t.j + 2] - x[3 * t.i + 2];
+    const double r2 = 1.0 / (ux * ux + uy * uy + uz * uz), r = sqrt(r2), r6d = r2 * r2 * r2;
+    const double r6_hc = 1.0 / r6d;
+    double r6 = r6_hc + t.sc;
+    r6 = 1.0 / r6;
+    const double r12 = r6 * r6;
+    const double Vel = t.el * r;
+    double Va, Vb;
+    if (t.soft) {  // nbe_qq's exponential soft pair (nonbonded.f90:138-141)
+      Vb = 0.0;
+      Va = t.A * exp(-t.B / r);
+    } else {
+      Va = t.A * r12;
+      Vb = t.B * r6;
+    }
+    double* el = t.kind == 0 ? &qqel : &qpel;
+    double* vdw = t.kind == 0 ? &qqvdw : &qpvdw;
+    if (ty == QNB_GC_FULL) {
+      *el = *el - Vel;
+      *vdw = *vdw - (Va - Vb);
+    } else if (ty == QNB_GC_ELECTRO) {
+      *el = *el - Vel;
+    } else {
+      *vdw = *vdw - (Va - Vb);
+    }
+  }
+  o[0] = qqel;
+  o[1] = qqvdw;
+  o[2] = qpel;
+  o[3] = qpvdw;
+  o[4] = qqel + qpel + e[4];
+  o[5] = qqvdw + qpvdw + e[5];
+  o[6] = e[6] + e[7] + e[8] + e[9] + o[4] + o[5] + e[10];
+}
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 // the LRF-off switch-atom solute-water list takes nbpwlist's test-point reuse
@@ -426,6 +492,22 @@ __device__ __forceinline__ double dist4p(const double* __restrict__ x, int a, co
   r = r + dy * dy;
   r = r + dz * dz;
   return r;
+}
+
+// Adds an atom's per-atom terms to its gradient.  A solute atom in no charge
+// group (an excluded atom of a topology cut to a sphere) is reached by no pair
+// kernel, so this pass is the first writer of its gradient and stores it.
+__device__ __forceinline__ void store_atom_gradient(const DevSys& S, int i, const double* f) {
+  const auto d = gpm(S.d);
+  if (i < S.nat_solute && ldg(S.atom_group + i) < 0) {
+    d[3 * i] = f[0];
+    d[3 * i + 1] = f[1];
+    d[3 * i + 2] = f[2];
+  } else {
+    d[3 * i] += f[0];
+    d[3 * i + 1] += f[1];
+    d[3 * i + 2] += f[2];
+  }
 }
 
 // LRF Taylor application for one atom (lrf_taylor, nonbondene.f90:507-571)

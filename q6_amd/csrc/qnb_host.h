@@ -72,6 +72,16 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M);
 bool host_pp_param(const HostTopo& T, int i, int j);
 // per-instance device state (coordinates, gradient, lists, LRF, energies)
 int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots);
+// group contributions (qnb_exc.hip): the pair lists of each energy array
+// and state, built on the host from the system's Q tables
+struct ExcHost {
+  int narr = 0, ns = 0;
+  std::vector<int> off, type;
+  std::vector<ExcEnt> ent;
+};
+int exc_build(const qnb_system* s, int narr, const uint8_t* mask, const int32_t* type, ExcHost& X);
+int exc_upload(const ExcHost& X, ExcDev& D, DevAlloc& M);
+void launch_exc_eval(hipStream_t st, const ExcDev& D, const double* x, const double* eq_in, double* out);
 // free-energy analysis (qnb_fep.hip): one window is either a batch
 // instance's recorded frames (hist) or state energies the caller supplies
 // (V: EQ(s)%total per frame; rkl: the first off-diagonal's distance per frame

@@ -1185,7 +1185,7 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_wforces(const DevSys* __restric
 // Per-atom terms after the pair kernels, one thread per atom: the LRF Taylor
 // term (lrf_taylor, nonbondene.f90:507-571) of every non-Q, non-excluded
 // atom and the SCAAS radial restraint (restrain_solvent, 6466-6544) of every
-// non-excluded water oxygen; adds to d.  On the sampled steps the atom's two
+// non-excluded water oxygen; adds to d (store_atom_gradient).  On the sampled steps the atom's two
 // energies go to eatom[2i], eatom[2i+1] for the fixed-order reduction.
 __global__ __launch_bounds__(256) void k_atom_terms(const DevSys* __restrict__ systems, uint32_t mask,
                                                     const double* __restrict__ tfree) {
@@ -1201,10 +1201,7 @@ __global__ __launch_bounds__(256) void k_atom_terms(const DevSys* __restrict__ s
     const double shift = S.fk_wsphere != 0.0 ? sqrt(S.Boltz * T / S.fk_wsphere) : 0.0;
     radial_restraint(S, i, shift, f, er);
   }
-  const auto d = gpm(S.d);
-  d[3 * i] += f[0];
-  d[3 * i + 1] += f[1];
-  d[3 * i + 2] += f[2];
+  store_atom_gradient(S, i, f);
   if (want_energies(S)) {
     stg(S.eatom + 2 * i, el);
     stg(S.eatom + 2 * i + 1, er);

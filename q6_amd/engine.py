@@ -34,6 +34,29 @@ class NonbondedEngine:
     def __del__(self):
         self.close()
 
+    # group contributions (calculate_exc, src/nonbondene.f90:6825-6920)
+    def set_groups(self, masks, types):
+        """Energy arrays of the group contribution: masks (narr, nat_solute)
+        0/1 over the solute atoms, types (narr,) abi.GC_FULL / GC_ELECTRO /
+        GC_VDW; an empty list removes them."""
+        masks = np.ascontiguousarray(np.asarray(masks, np.uint8).reshape(len(types), self._s.nat_solute))
+        types = np.ascontiguousarray(types, np.int32)
+        self._gc = (masks, types)
+        abi.check(self.lib, self.lib.qnb_set_groups(self.h, C.byref(self._s), len(types),
+                                                     masks.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                     types.ctypes.data_as(C.POINTER(C.c_int32))), "qnb_set_groups")
+
+    def calculate_exc(self, x, eq_in):
+        """eq_in (nstates, 11): qq el, vdW, qp el, vdW, qw el, vdW, Q bond,
+        angle, torsion, improper, restraint per state.  Returns (narr,
+        nstates, 7): qq el, vdW, qp el, vdW, qx el, vdW, total."""
+        x = np.ascontiguousarray(x, np.float64)
+        eq_in = np.ascontiguousarray(eq_in, np.float64)
+        narr = len(self._gc[1]) if getattr(self, "_gc", None) else 0
+        out = np.zeros((narr, self.nst, abi.GC_NV))
+        abi.check(self.lib, self.lib.qnb_calculate_exc(self.h, _pd(x), _pd(eq_in), _pd(out)), "qnb_calculate_exc")
+        return out
+
     # Q-atom bonded terms (qatom_load_fep's tables, src/qatom.f90:952-1800)
     def set_qbonded(self, tables, urey=False):
         """Loads the Q bonded terms (abi.qbonded_struct layout; None removes

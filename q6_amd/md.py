@@ -51,6 +51,8 @@ class BatchMD:
                 st, _ = s.to_structs(precision=precision)
                 sys_structs[key] = st
                 self._keep.append(st)
+            self._sys_struct = getattr(self, "_sys_struct", {})
+            self._sys_struct[k] = sys_structs[key]
             lam = np.asarray(lambdas[k], float)
             _, par = s.to_structs(precision=precision, **{"lambda": lam, "wshell_cstb": scaas_cstb(s, lam),
                                                           "itdis": 100.0 / dt_fs})
@@ -212,6 +214,30 @@ class BatchMD:
         nf = C.c_int32()
         abi.check(self.lib, self.lib.qnb_batch_frames(self.h, k, None, 0, C.byref(nf)), "qnb_batch_frames")
         return nf.value
+
+    def set_groups(self, k, masks, types):
+        """Group-contribution arrays of instance k (qnb_batch_set_groups):
+        masks (narr, nat_solute) 0/1, types (narr,) abi.GC_*; computed on
+        every sampled step from that step's energy record."""
+        masks = np.ascontiguousarray(np.asarray(masks, np.uint8).reshape(len(types), self._sys_struct[k].nat_solute))
+        types = np.ascontiguousarray(types, np.int32)
+        self._gc = getattr(self, "_gc", {})
+        self._gc[k] = (masks, types)
+        abi.check(self.lib, self.lib.qnb_batch_set_groups(self.h, k, C.byref(self._sys_struct[k]), len(types),
+                                                           masks.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                           types.ctypes.data_as(C.POINTER(C.c_int32))),
+                  "qnb_batch_set_groups")
+
+    def group_energies(self, k):
+        """(nframes, narr, nstates, 7) group energies of instance k's frames."""
+        narr = len(self._gc[k][1]) if getattr(self, "_gc", None) and k in self._gc else 0
+        ns = int(self.systems[k].n["nstates"])
+        nf = C.c_int32()
+        abi.check(self.lib, self.lib.qnb_batch_group_energies(self.h, k, None, 0, C.byref(nf)), "group_energies")
+        out = np.zeros((max(nf.value, 1), narr, ns, abi.GC_NV))
+        abi.check(self.lib, self.lib.qnb_batch_group_energies(self.h, k, _pd(out), nf.value, C.byref(nf)),
+                  "group_energies")
+        return out[:nf.value]
 
     def free_energy(self, kT, nskip=0, nbins=0, nmin=0, alpha=None, coeff=None, A=0.0, mu=0.0, eta=0.0, r0=0.0):
         """Qfep6's analysis on the GPU (qnb_batch_free_energy) of the frames

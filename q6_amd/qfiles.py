@@ -157,8 +157,12 @@ def write_fep(s, path, offdiag=None):
 
 
 def write_inp(s, path, top, fep, steps, lambdas=None, dt_fs=1.0, T=300.0, tau_T=100.0, seed=4, nbcycle=10,
-              energy_every=0, final="final.re", energy_file=None):
-    """MD input for the synthetic system's cut-offs, sphere and restraints."""
+              energy_every=0, final="final.re", energy_file=None, group_contribution=None):
+    """MD input for the synthetic system's cut-offs, sphere and restraints.
+    group_contribution: lines of the [group_contribution] section
+    (src/md.f90:1011-1070), e.g. "residue full 3" (residues are the solute
+    chains, then the waters); Qdyn6 computes them only with an energy
+    interval and file."""
     p, md = s.par, s.md
     lam = np.asarray(lambdas if lambdas is not None else p["lambda"], float)
     L = ["[MD]", "steps %d" % steps, "stepsize %s" % _g(dt_fs), "temperature %s" % _g(T),
@@ -176,6 +180,8 @@ def write_inp(s, path, top, fep, steps, lambdas=None, dt_fs=1.0, T=300.0, tau_T=
     L += ["", "[files]", "topology %s" % top, "fep %s" % fep, "final %s" % final]
     if energy_file:
         L.append("energy %s" % energy_file)
+    if group_contribution:
+        L += ["", "[group_contribution]"] + list(group_contribution)
     L += ["", "[lambdas]", " ".join(_g(v) for v in lam)]
     rst = np.asarray(md.get("restraints", np.zeros((0, 2))), int).reshape(-1, 2)
     if len(rst):

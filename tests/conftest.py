@@ -8,7 +8,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-CASES = ["lig_eq1", "lig_fep2", "lig_fep2_nolrf", "lig_fep2_arith", "lig_fep2_anyatom", "lig_fep2_anyatom_nolrf"]
+CASES = ["lig_eq1", "lig_fep2", "lig_fep2_nolrf", "lig_fep2_arith", "lig_fep2_anyatom", "lig_fep2_anyatom_nolrf",
+         "cjp_excl"]
 
 
 def pytest_configure(config):
@@ -43,3 +44,16 @@ def golden_summary_inputs(s):
                   restraint_fix=eb[10], restraint_shell=eb[11], restraint_protein=eb[12] - float(
                       (eqs[14] * eqs[0]).sum()))
     return bonded, eqs[2:6].T, eqs[14], eqs[0]
+
+
+def exc_inputs(s):
+    """calculate_exc's inputs and stock Qdyn6's results from a golden dump
+    with group contributions: masks (narr, nat_solute), types (narr,), the
+    per-state energies pot_energy left (nstates, 11) and the reference's
+    arrays (narr, nstates, 7)."""
+    import numpy as np
+    r = s.ref
+    eqs = np.asarray(r["EQ_summary"])
+    eq_in = np.stack([eqs[[8, 9, 10, 11, 12, 13, 2, 3, 4, 5, 14], st] for st in range(eqs.shape[1])])
+    masks = np.stack([np.asarray(r["exc_mask%d" % g]) for g in np.atleast_1d(r["exc_group"])]).astype(np.uint8)
+    return masks, np.atleast_1d(r["exc_type"]).astype(np.int32), eq_in, np.asarray(r["exc_EQ"]).transpose(2, 1, 0)

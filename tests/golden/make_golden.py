@@ -39,6 +39,13 @@ Cases:
            src/topo.f90:815): the nbpplis2/nbpwlis2/nbqplis2 builders and
            their _lrf forms, where a charge-group pair is inside a cut-off
            when any atom pair is.
+  cjp_excl the reference's group-contribution test system (2CJP epoxide
+           hydrolase, 5,622 atoms, 40 Q-atoms in 2 EVB states with soft
+           pairs, OPLS-AA, 1,875 excluded atoms; tests/exclude_tests), with
+           its five [group_contribution] lines and three atom groups with
+           Q-atoms and their bonded neighbours (twelve energy arrays after
+           the first): every list and term, and calculate_exc's energies per
+           array and state.  Stored without the per-pair list parameters.
   lig_qb2  lig_fep2 plus changing bonds (Morse, harmonic, one forming bond),
            angles (one with Urey-Bradley terms), torsions and impropers with
            Morse couplings (gamma and 1 - gamma, two couplings on one term);
@@ -332,13 +339,17 @@ def run_case(name, inp_text, fep_text=None, top_text=None):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
-def compact(d):
-    """Pack a dump into fixture arrays (0-based atom indices everywhere)."""
+def compact(d, fill=True, list_params=True):
+    """Pack a dump into fixture arrays (0-based atom indices everywhere).
+    fill=False leaves out the fill module's copies (fill_*); list_params=False
+    the per-pair list parameters (nb*_par)."""
     out = {}
     for k, v in d.items():
         if k.startswith("nbww"):
             continue
         if k.endswith("_ij") and k.startswith("nb"):
+            continue
+        if (not fill and k.startswith("fill_")) or (not list_params and k.startswith("nb") and k.endswith("_par")):
             continue
         out[k] = np.asarray(v)
     # 0-based copies of index arrays; Fortran 1-based values are kept verbatim
@@ -359,7 +370,8 @@ def compact(d):
         out["ww_mol_pairs"] = np.stack([iw, jw]).astype(np.int32)
         par = d["nbww_par"].reshape(3, sa * sa, -1, order="F").transpose(0, 2, 1)
         assert np.all(par == par[:, :1, :])
-        out["ww_blk_par"] = par[:, 0, :]
+        if list_params:
+            out["ww_blk_par"] = par[:, 0, :]
     else:
         out["ww_mol_pairs"] = np.zeros((2, 0), np.int32)
     for lst in ("nbpp", "nbpw", "nbqp", "nbqw"):
@@ -368,6 +380,79 @@ def compact(d):
         if k.startswith("nbqq") and k.endswith("_ij"):
             out[k] = d[k].astype(np.int32)
     return out
+
+
+# The reference's group-contribution test system (tests/exclude_tests): the
+# 2CJP epoxide hydrolase in a 20 A sphere (5,622 atoms, OPLS-AA), its 40-atom
+# 2-state EVB FEP file, and the [group_contribution] lines of
+# tests/exclude_tests/inputs/excl/gen_inps.pl:42-47, plus three atom groups
+# with a Q-atom and atoms bonded to Q-atoms (the reference's groups hold
+# neither); the cut-offs of that script (:20-24), lambda (0.5, 0.5).  The
+# Q-atom is the last one (Q-atom 40, atom 5124): qq_int_comp counts a group's
+# Q-Q pairs with either atom in the group (src/simprep.f90:3212-3221), but
+# exc_assign lists (iq, jq) only where qq_precomp(iq, jq) is set, which
+# precompute_set_values_qq does for iq < jq alone (:3444-3500), so for any
+# other Q-atom calculate_exc would walk list entries never filled.  The topology's start is not relaxed for
+# dynamics, so the dump leaves out its MD step ("nomd").
+EXCL = "/root/reference/tests/exclude_tests/inputs"
+CJP_INP = """[MD]
+steps                          10
+stepsize                       1.0
+temperature                    300
+bath_coupling                  100
+separate_scaling               on
+lrf                            on
+random_seed                    1
+initial_temperature            300
+
+[cut-offs]
+solute_solute                  10
+solvent_solvent                10
+solute_solvent                 10
+q_atom                         99
+
+[sphere]
+
+[intervals]
+non_bond                       30
+output                         100
+energy                         10
+
+[group_contribution]
+residue all 153
+residue electro 153
+residue vdw 153
+residue full 234
+residue all 153 234
+atom full 5124 1624 1800
+atom electro 4719 4732 2000
+atom vdw 5124 1631
+
+[files]
+topology                       2cjpFH_ionres_oplsa.top
+final                          final.re
+energy                         run.en
+fep                            lig.fep
+
+[lambdas]
+0.5 0.5
+"""
+
+
+def run_cjp():
+    tmp = tempfile.mkdtemp(prefix="qnbgold_")
+    try:
+        for f in ("2cjpFH_ionres_oplsa.top", "lig.fep"):
+            shutil.copy(os.path.join(EXCL, f), tmp)
+        with open(os.path.join(tmp, "run.inp"), "w") as f:
+            f.write(CJP_INP)
+        log = subprocess.run([DUMP, "out.bin", "nomd", "run.inp"], cwd=tmp, capture_output=True, text=True)
+        if log.returncode != 0 or not os.path.exists(os.path.join(tmp, "out.bin")):
+            print(log.stdout[-3000:], log.stderr[-3000:])
+            raise RuntimeError("qnb_dump failed for cjp_excl")
+        return read_dump(os.path.join(tmp, "out.bin"))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def main():
@@ -392,6 +477,13 @@ def main():
     cases["lig_fep2_anyatom_nolrf"] = cases["lig_fep2_nolrf"]
     tops = {"lig_fep2_arith": top_arith, "lig_fep2_anyatom": top_any, "lig_fep2_anyatom_nolrf": top_any}
     only = sys.argv[1:]
+    if not only or "cjp_excl" in only:
+        d = run_cjp()
+        arrs = compact(d, fill=False, list_params=False)
+        path = os.path.join(HERE, "cjp_excl.npz")
+        np.savez_compressed(path, **arrs)
+        print("cjp_excl ->", path, os.path.getsize(path) // 1024, "KiB;", "pp", d["nbpp_ij"].shape[1], "pw",
+              d["nbpw_ij"].shape[1], "qp", d["nbqp_ij"].shape[1], "arrays", len(d["exc_type"]) + 1)
     for name, (inp, fep) in cases.items():
         if only and name not in only:
             continue
@@ -407,7 +499,7 @@ def main():
             # the reference's own known answer (tests/basic_tests/eval_test.sh:97-101)
             el, vdw = d["E_qw"]
             assert "%.2f %.2f" % (el, vdw) == "3.12 139.43", (el, vdw)
-        arrs = compact(d)
+        arrs = compact(d, fill=False)
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, **arrs)
         print(name, "->", path, os.path.getsize(path) // 1024, "KiB;",

@@ -37,7 +37,7 @@ def term_energy(t, E, EQ):
     return [E[t]]
 
 
-@pytest.fixture(scope="module", params=["lig_eq1", "lig_fep2", "lig_fep2_arith", "lig_fep2_anyatom"])
+@pytest.fixture(scope="module", params=["lig_eq1", "lig_fep2", "lig_fep2_arith", "lig_fep2_anyatom", "cjp_excl"])
 def gcase(request):
     print(f"\n[gpu] golden case {request.param}", flush=True)
     s = from_golden(golden_path(request.param))
@@ -259,3 +259,23 @@ def test_pot_energy_summary(case):
     for st, q in enumerate(eqs.T):
         assert abs(eqo[st]["total"] - q[1]) <= REL * scale, st
     assert abs(out["potential"] - es[0]) <= REL * scale, (out["potential"], es[0])
+
+
+def test_calculate_exc_matches_reference():
+    """qnb_calculate_exc on the reference's group-contribution system (2CJP,
+    twelve energy arrays with residue groups, the last Q-atom and atoms bonded
+    to Q-atoms) against stock Qdyn6's calculate_exc: every value within 1e-12
+    relative (FP64 in the reference's pair order; the soft pairs' exp is the
+    device's)."""
+    from conftest import exc_inputs
+    print("\n[gpu] calculate_exc cjp_excl", flush=True)
+    s = from_golden(golden_path("cjp_excl"))
+    masks, types, eq_in, ref = exc_inputs(s)
+    eng = engine_for(s)
+    eng.set_groups(masks, types)
+    got = eng.calculate_exc(s.x, eq_in)
+    eng.set_groups(np.zeros((0, s.n["nat_solute"])), [])
+    assert eng.calculate_exc(s.x, eq_in).shape[0] == 0
+    eng.close()
+    assert got.shape == ref.shape
+    assert np.all(np.abs(got - ref) <= 1e-12 * np.maximum(np.abs(ref), 1.0)), np.max(np.abs(got - ref))

@@ -176,3 +176,42 @@ def test_one_md_step_matches_qdyn6(case):
     got = [th["Temp"], th["Tfree"], th["Ekin"], th["tfree"][0], th["sfact"][0], th["tfree"][1], th["sfact"][1]]
     ref = [t[0], t[1], t[2], t[4], t[5], t[7], t[8]]
     assert np.allclose(got, ref, rtol=1e-10, atol=0), (got, ref)
+
+
+def test_group_contributions_recorded():
+    """Group-contribution arrays (calculate_exc) recorded on every sampled
+    step of a batch: for each frame, array and state they equal the oracle's
+    calculate_exc at the stored coordinates of that step, from the same
+    recorded per-state energies (FP64 on both sides, within 1e-12)."""
+    from oracle.oracle import Oracle
+    from q6_amd import synth
+    from q6_amd.md import BatchMD
+    s = synth.cfg2()
+    nsol, ns = s.n["nat_solute"], s.n["nstates"]
+    iqseq = s.a["iqseq"]
+    masks = np.zeros((3, nsol), np.uint8)
+    masks[0, :10] = 1                           # a chain of Q-atoms
+    masks[1, [5] + list(range(100, 110))] = 1   # one Q-atom, a plain chain
+    masks[2, [iqseq[-1]] + list(range(200, 210))] = 1
+    types = np.array([abi.GC_FULL, abi.GC_ELECTRO, abi.GC_VDW], np.int32)
+    lams = [np.full(ns, 1.0 / ns), np.eye(ns)[0]]
+    v = synth.maxwell_velocities(s, 300.0, 9)
+    print("\n[gpu] batch group contributions", flush=True)
+    b = BatchMD([s, s], lams, [v, v], precision=abi.QNB_PREC_MIXED, nbcycle=10, ene_every=5, max_frames=12)
+    b.minimize(200, 0.02)  # the seeded start has close contacts
+    b.set_groups(1, masks, types)
+    b.store_frames(5, 12)
+    b.run(30)
+    b.sync()
+    ge, en, fr = b.group_energies(1), b.energies(1), b.frames(1)
+    assert b.group_energies(0).shape[1] == 0
+    b.close()
+    assert len(ge) == len(en) == len(fr) == 6
+    o = Oracle(s)
+    for f in range(len(fr)):
+        row = en[f]
+        eq_in = np.array([[*row[abi.F_Q0 + 6 * st:abi.F_Q0 + 6 * st + 6], *row[abi.F_QB0 + 4 * st:abi.F_QB0 + 4 * st + 4],
+                           0.0] for st in range(ns)])
+        ref = o.calculate_exc(fr[f], masks, types, eq_in)
+        assert np.all(np.abs(ge[f] - ref) <= 1e-12 * np.maximum(np.abs(ref), 1.0)), (f, np.max(np.abs(ge[f] - ref)))
+    assert np.any(ge[:, 0, :, 0] != en[:, [abi.F_Q0 + 6 * st for st in range(ns)]])  # Q-Q energy removed

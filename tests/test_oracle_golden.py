@@ -59,6 +59,8 @@ def test_lists_exact_order(case, which, key):
 
 def test_list_parameters(case):
     name, s, o = case
+    if "nbpp_par" not in s.ref:
+        pytest.skip("fixture stored without list parameters")
     par = o.list_params(abi.LIST_PP, 3)
     assert np.array_equal(par.T, s.ref["nbpp_par"].reshape(3, -1))
     par = o.list_params(abi.LIST_PW, 3)
@@ -124,3 +126,19 @@ def test_pot_energy_summary_exact(case):
         assert eqo[st]["qx_el"] == q[6] and eqo[st]["qx_vdw"] == q[7] and eqo[st]["total"] == q[1], st
     assert out["restraint_total"] == s.ref["E_bonded"][13]
     assert out["potential"] == es[0]
+
+
+def test_calculate_exc_exact(case):
+    """calculate_exc (src/nonbondene.f90:6825-6920) on the reference's
+    group-contribution system: every energy array (the reference test's
+    residue groups, and groups with the last Q-atom and atoms bonded to
+    Q-atoms) and state equals stock Qdyn6's bit for bit."""
+    from conftest import exc_inputs
+    name, s, o = case
+    if "exc_EQ" not in s.ref:
+        pytest.skip("no group contributions in this case")
+    masks, types, eq_in, ref = exc_inputs(s)
+    iq, q3 = s.a["iqatom"][:s.n["nat_solute"]], s.a["qconn3"]
+    m = masks.astype(bool)
+    assert (m & (iq != 0)).any() and (m & (iq == 0) & (q3 != 0)).any() and (m & (iq == 0) & (q3 == 0)).any()
+    assert np.array_equal(o.calculate_exc(s.x, masks, types, eq_in), ref)
