@@ -172,6 +172,15 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
   for (int i = 0; i < nat; i++)
     if (atom_group[i] < 0 && !(i < nsol && s->excl[i] && s->iqatom[i] == 0))
       return fail(QNB_EINVAL, "atom %d is in no charge group", i);
+  // charge groups are consecutive runs of cgpatom, as the topology reader
+  // lays them out
+  std::vector<int> gslot(ncgpat);
+  for (int g = 0; g < s->ncgp; g++) {
+    if (s->cgp_first[g] != (g == 0 ? 0 : s->cgp_last[g - 1] + 1) || s->cgp_last[g] < s->cgp_first[g])
+      return fail(QNB_EINVAL, "charge group %d is not the run after group %d in cgpatom", g, g - 1);
+    for (int k = s->cgp_first[g]; k <= s->cgp_last[g]; k++) gslot[k] = g & 63;
+  }
+  S.ncgpat = ncgpat;
   const int nsol1 = std::max(nsol, 1);
   T.exmask.assign(nsol1, 0u);
   T.m14mask.assign(nsol1, 0u);
@@ -254,6 +263,7 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
   UP(gswitch, s->cgp_iswitch, s->ncgp);
   UP(cgpatom, s->cgpatom, ncgpat);
   UP(atom_group, atom_group.data(), nat);
+  UP(gslot, gslot.data(), ncgpat);
 
   UP(avdw, s->avdw, 3 * s->ntypes);
   UP(bvdw, s->bvdw, 3 * s->ntypes);
@@ -409,7 +419,9 @@ int instance_alloc(DevSys& S, DevAlloc& M, const qnb_params* p, int nslots) {
   } while (0)
   const int nat = S.natom, nsol1 = std::max(S.nat_solute, 1), nw1 = std::max(S.nwat, 1);
   AL(x, 3 * nat);
-  if (p->precision == QNB_PREC_MIXED) AL(xq, 4 * (size_t)nat);
+  if (p->precision == QNB_PREC_MIXED) AL(xqg, 4 * (size_t)S.ncgpat);
+  else AL(xqgd, 4 * (size_t)S.ncgpat);
+  AL(gref, 4 * (size_t)S.ncgp);
   AL(d, 3 * nat);
   AL(lrf, (size_t)QNB_NLRF * S.ncgp);
   if (S.cap <= 0) S.cap = 256;
@@ -587,8 +599,7 @@ template <typename R>
 static void launch_lists(qnb_handle* h) {
   const int wpb = 4;
   dim3 grid((h->hs.ncgp + wpb - 1) / wpb, 1);
-  if (sizeof(R) == 4)
-    hipLaunchKernelGGL(k_pack_xq, dim3((h->hs.natom + 255) / 256, 1), dim3(256), 0, h->stream, h->dsys);
+  hipLaunchKernelGGL(k_pack_xq, dim3((h->hs.ncgpat + 255) / 256, 1), dim3(256), 0, h->stream, h->dsys);
   if (pw_quirk(h->hs))
     hipLaunchKernelGGL(k_pw_first, dim3((h->hs.ncgp_solute + 3) / 4, 1), dim3(256), 0, h->stream, h->dsys);
   if (h->hs.any_atom) hipLaunchKernelGGL((k_lists_lrf<R, true>), grid, dim3(64 * wpb), 0, h->stream, h->dsys);

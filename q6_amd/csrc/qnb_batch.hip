@@ -568,8 +568,8 @@ static void launch_lists(qnb_batch* b, int wpb) {
     maxsg = std::max(maxsg, S.ncgp_solute);
   }
   if (quirk) hipLaunchKernelGGL(k_pw_first, dim3((maxsg + 3) / 4, b->n), dim3(256), 0, b->stream, b->dsys);
+  hipLaunchKernelGGL(k_pack_xq, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys);
   if (b->prec == QNB_PREC_MIXED) {
-    hipLaunchKernelGGL(k_pack_xq, dim3((b->maxat + 255) / 256, b->n), dim3(256), 0, b->stream, b->dsys);
     if (any) hipLaunchKernelGGL((k_lists_lrf<float, true>), g, t, 0, b->stream, b->dsys);
     else hipLaunchKernelGGL((k_lists_lrf<float, false>), g, t, 0, b->stream, b->dsys);
   } else {

@@ -56,6 +56,11 @@ struct DevSys {
   const int* gswitch;  // atom
   const int* cgpatom;
   const int* atom_group;
+  // per group-ordered atom k (cgpatom order; groups are consecutive runs):
+  // the atom's group modulo 64, its bit in the far-group ballot word of the
+  // list sweep's 64-group chunk
+  const int* gslot;
+  int ncgpat;
   const double* avdw;  // [t*3+k]
   const double* bvdw;
   const int* ljcod;    // [ti + tj*ntypes], 1-based code
@@ -118,7 +123,15 @@ struct DevSys {
 
   // per-instance state
   double* x;
-  float* xq;  // mixed path: [natom][4] x, y, z (FP32) and LRF charge, packed before each list update
+  // LRF sources in cgpatom order, packed before each list update: x, y, z
+  // and the charge (0 for Q-atoms, which take no part in the expansion);
+  // FP32 for the mixed path, FP64 for the plain one (the other is null)
+  float* xqg;
+  double* xqgd;
+  // [ncgp][4] per group, packed with them: its reference atom's position
+  // (switch atom of a solute group, oxygen of a water) and 1 if that atom is
+  // excluded, else 0
+  double* gref;
   double* d;
   double* lrf;
   int* nbr;

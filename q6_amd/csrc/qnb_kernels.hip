@@ -133,32 +133,42 @@ __device__ __forceinline__ void lrf_accumulate(R* acc, double drx, double dry, d
 
 // FP32 form for the mixed path: dr already formed in FP32 from the packed
 // positions (x_i and the centre each rounded once), q the atom's charge.
-// The terms are kept as raw sums that need no per-atom subtraction, p[0..19]
-// as acc[0..19] without the -f0 (phi2 diagonal) and -h dr (phi3) parts,
-// p[20] = sum f0, p[21..23] = sum h dr; lrf_flush_f forms the components in
-// FP64 when it adds a source group's sums.
+// The lane keeps raw sums that need no per-atom constant or subtraction:
+//   p[0] = sum q/r,  p[1..3] = sum f0 d_a,  p[4..9] = sum w1 d_a d_b,
+//   p[10..19] = sum w2 d_a d_b d_c,  p[20] = sum f0,  p[21..23] = sum w1 d_a
+// with f0 = q/r^3, w1 = f0/r^2, w2 = w1/r^2 (component order as acc);
+// lrf_flush_f forms the expansion from them in FP64:
+//   phi1 = -p[1..3], phi2_ab = 3 p_ab - delta_ab p[20],
+//   phi3_abc = -15 p_abc + 3 (delta_ab H_c + delta_ac H_b + delta_bc H_a)
+// which is lrf_update's field1 = 3 f0/r^2, field2 = -field1/r^2 and its
+// 5 field2 d d d term with the -field2 r^2 (delta d) parts.
 #define LRF_NF 24
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void lrf_accumulate_f(float* p, float dx, float dy, float dz, float q) {
   const float r2 = dx * dx + dy * dy + dz * dz;
   float ri2, ri;
   rinv(r2, ri2, ri);
-  const float f0 = q * ri * ri2;
-  const float f1 = 3.0f * f0 * ri2;
-  const float f2 = -f1 * ri2;
-  p[0] += f0 * r2;
+  const float qri = q * ri;
+  p[0] += qri;
+  const float f0 = qri * ri2;
+  const float w1 = f0 * ri2;
+  const float w2 = w1 * ri2;
   p[1] += dx * f0;
   p[2] += dy * f0;
   p[3] += dz * f0;
-  const float xx = dx * dx, xy = dx * dy, xz = dx * dz, yy = dy * dy, yz = dy * dz, zz = dz * dz;
-  p[4] += xx * f1;
-  p[5] += xy * f1;
-  p[6] += xz * f1;
-  p[7] += yy * f1;
-  p[8] += yz * f1;
-  p[9] += zz * f1;
   p[20] += f0;
-  const float g = 5.0f * f2, h = f2 * r2;
-  const float gx = g * dx, gy = g * dy, gz = g * dz;
+  const float xx = dx * dx, xy = dx * dy, xz = dx * dz, yy = dy * dy, yz = dy * dz, zz = dz * dz;
+  p[4] += xx * w1;
+  p[5] += xy * w1;
+  p[6] += xz * w1;
+  p[7] += yy * w1;
+  p[8] += yz * w1;
+  p[9] += zz * w1;
+  p[21] += dx * w1;
+  p[22] += dy * w1;
+  p[23] += dz * w1;
+  const float gx = w2 * dx, gy = w2 * dy, gz = w2 * dz;
   p[10] += gx * xx;
   p[11] += gx * xy;
   p[12] += gx * xz;
@@ -169,51 +179,67 @@ __device__ __forceinline__ void lrf_accumulate_f(float* p, float dx, float dy, f
   p[17] += gy * yz;
   p[18] += gy * zz;
   p[19] += gz * zz;
-  p[21] += h * dx;
-  p[22] += h * dy;
-  p[23] += h * dz;
 }
 
-// adds one source group's raw FP32 sums to the FP64 expansion, forming
-// phi1 = -sum dr f0, phi2_aa = sum d_a d_a f1 - sum f0 and the -h dr parts
-// of phi3 in FP64
+// adds a lane's raw FP32 sums to its FP64 expansion
 __device__ __forceinline__ void lrf_flush_f(double* acc, const float* p) {
   const double s0 = p[20], hx = p[21], hy = p[22], hz = p[23];
   acc[0] += (double)p[0];
   acc[1] -= (double)p[1];
   acc[2] -= (double)p[2];
   acc[3] -= (double)p[3];
-  acc[4] += (double)p[4] - s0;
-  acc[5] += (double)p[5];
-  acc[6] += (double)p[6];
-  acc[7] += (double)p[7] - s0;
-  acc[8] += (double)p[8];
-  acc[9] += (double)p[9] - s0;
-  acc[10] += (double)p[10] - 3.0 * hx;
-  acc[11] += (double)p[11] - hy;
-  acc[12] += (double)p[12] - hz;
-  acc[13] += (double)p[13] - hx;
-  acc[14] += (double)p[14];
-  acc[15] += (double)p[15] - hx;
-  acc[16] += (double)p[16] - 3.0 * hy;
-  acc[17] += (double)p[17] - hz;
-  acc[18] += (double)p[18] - hy;
-  acc[19] += (double)p[19] - 3.0 * hz;
+  acc[4] += 3.0 * (double)p[4] - s0;
+  acc[5] += 3.0 * (double)p[5];
+  acc[6] += 3.0 * (double)p[6];
+  acc[7] += 3.0 * (double)p[7] - s0;
+  acc[8] += 3.0 * (double)p[8];
+  acc[9] += 3.0 * (double)p[9] - s0;
+  acc[10] += 9.0 * hx - 15.0 * (double)p[10];
+  acc[11] += 3.0 * hy - 15.0 * (double)p[11];
+  acc[12] += 3.0 * hz - 15.0 * (double)p[12];
+  acc[13] += 3.0 * hx - 15.0 * (double)p[13];
+  acc[14] -= 15.0 * (double)p[14];
+  acc[15] += 3.0 * hx - 15.0 * (double)p[15];
+  acc[16] += 9.0 * hy - 15.0 * (double)p[16];
+  acc[17] += 3.0 * hz - 15.0 * (double)p[17];
+  acc[18] += 3.0 * hy - 15.0 * (double)p[18];
+  acc[19] += 9.0 * hz - 15.0 * (double)p[19];
 }
 
-// Packs the mixed path's LRF sources before a list update: per atom x, y, z
-// rounded to FP32 and the charge (0 for Q-atoms, which take no part in the
-// expansion), 16 bytes, so the sweep loads one record per source atom.
+// Packs the LRF sources before a list update, one record per atom in
+// cgpatom order (a charge group's atoms are one run): x, y, z and the charge,
+// 0 for Q-atoms (lrf_update skips them); FP32 for the mixed path (each
+// coordinate rounded once), FP64 for the plain one.  Also packs each group's
+// reference atom and its exclusion flag for the group tests of the sweep.
 __global__ __launch_bounds__(256) void k_pack_xq(const DevSys* __restrict__ systems) {
   const DevSys& S = systems[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= S.natom) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S.ncgpat) return;
+  const int i = ldg(S.cgpatom + k);
   const auto x = gp(S.x);
-  const auto o = gpm(S.xq);
-  o[4 * i] = (float)x[3 * i];
-  o[4 * i + 1] = (float)x[3 * i + 1];
-  o[4 * i + 2] = (float)x[3 * i + 2];
-  o[4 * i + 3] = ldg(S.iqatom + i) != 0 ? 0.0f : (float)ldg(S.crg + i);
+  const double q = ldg(S.iqatom + i) != 0 ? 0.0 : ldg(S.crg + i);
+  if (S.xqg) {
+    const auto o = gpm(S.xqg);
+    o[4 * k] = (float)x[3 * i];
+    o[4 * k + 1] = (float)x[3 * i + 1];
+    o[4 * k + 2] = (float)x[3 * i + 2];
+    o[4 * k + 3] = (float)q;
+  }
+  if (S.xqgd) {
+    const auto o = gpm(S.xqgd);
+    o[4 * k] = x[3 * i];
+    o[4 * k + 1] = x[3 * i + 1];
+    o[4 * k + 2] = x[3 * i + 2];
+    o[4 * k + 3] = q;
+  }
+  if (k < S.ncgp) {  // thread k also packs group k's reference atom
+    const int rb = group_ref_atom(S, k);
+    const auto o = gpm(S.gref);
+    o[4 * k] = x[3 * rb];
+    o[4 * k + 1] = x[3 * rb + 1];
+    o[4 * k + 2] = x[3 * rb + 2];
+    o[4 * k + 3] = ldg(S.excl + rb) ? 1.0 : 0.0;
+  }
 }
 
 // index of the symmetric third-rank component (a<=b<=c sorted) in acc[10..]
@@ -233,12 +259,25 @@ __device__ __forceinline__ int t3index(int a, int b, int c) {
   return 19;
 }
 
+// dist4 on packed reference positions: |b - a|^2, ((dx^2 + dy^2) + dz^2)
+__device__ __forceinline__ double dist4v(const d4v& a, const d4v& b) {
+#pragma clang fp contract(off)
+  const double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+  double r = dx * dx;
+  r = r + dy * dy;
+  r = r + dz * dz;
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // Pair-list update with LRF (make_pair_lists, nonbondene.f90:749-837, SPH):
-// one wavefront per receiver charge group a.  Lanes sweep every other group b
-// (LDS-free: group reference atoms are L2-resident); a pair inside its class
-// cut-off goes to a's neighbour row through a ballot/prefix compaction, a
-// pair beyond it (and inside RcLRF) adds b's non-Q charges to a's expansion.
+// one wavefront per receiver charge group a, sweeping the other groups in
+// chunks of 64 (lane = group b; reference atoms are L2-resident).  A pair
+// inside its class cut-off goes to a's neighbour row through a ballot/prefix
+// compaction; the pairs beyond it (and inside RcLRF) form the chunk's far
+// word, and the wave then adds the charges of those groups' atoms to a's
+// expansion lane by atom over the chunk's run of packed records (lrf_update,
+// nonbondene.f90:628-725), so every lane has an atom whatever the group sizes.
 // ANY: some system of the launch has iuse_switch_atom = 0 (any_atom), whose
 // group tests scan atom pairs; the switch-atom form is compiled without it.
 template <typename R, bool ANY>
@@ -269,84 +308,83 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
   for (int k = 0; k < 20; k++) acc[k] = 0.0;
   int count = 0;
   int* __restrict__ row = S.nbr + (size_t)a * S.cap;
-  // mixed path: raw FP32 sums of this lane's far atoms since the last flush
-  // to the FP64 expansion; the wave flushes together once a lane holds 10
-  // or more atoms (a solute group, or three or four waters)
+  // mixed path: the lane's raw FP32 sums since the last flush to the FP64
+  // expansion, flushed after a chunk once a lane has been offered LRF_FLUSH
+  // atoms (at most LRF_FLUSH - 1 plus one chunk's share)
+  constexpr int LRF_FLUSH = 10;
   float pf[LRF_NF];
 #pragma unroll
   for (int k = 0; k < LRF_NF; k++) pf[k] = 0.0f;
-  int pf_atoms = 0;
+  int offered = 0;
   if (!ldg(S.excl + ra)) {
+    const auto gref = (gptr_t<d4v>)S.gref;
+    const d4v A = gref[a];
+    // the cut-off of a pair with a solute group and with a water, given a
+    const double cut_sol = a_sol ? S.rcpp2 : S.rcpw2, cut_wat = a_sol ? S.rcpw2 : S.rcww2;
+    const bool quirk = pw_quirk(S), any_atom = ANY && S.any_atom, use_lrf = S.use_lrf;
+    const double rclrf2 = S.rclrf2;
     for (int b0 = 0; b0 < S.ncgp; b0 += QNB_WAVE) {
       const int b = b0 + lane;
-      bool in = false;
-      if (b < S.ncgp && b != a) {
-        const int rb = group_ref_atom(S, b);
-        if (!ldg(S.excl + rb)) {
-          const bool b_sol = b < S.ncgp_solute;
-          const double cut2 = (a_sol && b_sol) ? S.rcpp2 : ((!a_sol && !b_sol) ? S.rcww2 : S.rcpw2);
-          double r2;
-          bool inside;
-          if (ANY && S.any_atom && (a_sol || b_sol))
-            inside = any_atom_pair(S, x, a, b, a_sol, b_sol, ra, rb, cut2, r2);
-          else if (a_sol != b_sol && pw_quirk(S)) {
-            // nbpwlist's test point: the switch atom up to the group's first
-            // water inside, atom glast + 1 after it (LRF off: r2 unused)
-            const int g = a_sol ? a : b, w = (a_sol ? b : a) - S.ncgp_solute;
-            const int wf = ldg(S.pw_first + g);
-            r2 = 0.0;
-            inside = w == wf || (w > wf && dist4(x, ldg(S.glast + g) + 1, a_sol ? rb : ra) <= cut2);
-          } else {
-            r2 = dist4(x, ra, rb);
-            inside = r2 <= cut2;
-          }
-          if (inside) {
-            in = true;
-          } else if (S.use_lrf && r2 <= S.rclrf2) {
-            R part[20];
-#pragma unroll
-            for (int k = 0; k < 20; k++) part[k] = R(0);
-            if constexpr (sizeof(R) == 4) {
-              // mixed path: packed FP32 positions and charges (Q-atoms carry 0)
-              const auto xq = gp(S.xq);
-              const int k0 = b_sol ? ldg(S.gfirst + b) : 0, k1 = b_sol ? ldg(S.glast + b) : S.sa - 1;
-              for (int k = k0; k <= k1; k++) {
-                const int i = b_sol ? ldg(S.cgpatom + k) : rb + k;
-                lrf_accumulate_f(pf, xq[4 * i] - cxf, xq[4 * i + 1] - cyf, xq[4 * i + 2] - czf, xq[4 * i + 3]);
-              }
-              pf_atoms += k1 - k0 + 1;
-            } else if (b_sol) {
-              const int k1 = ldg(S.glast + b);
-              for (int k = ldg(S.gfirst + b); k <= k1; k++) {
-                const int i = ldg(S.cgpatom + k);
-                if (ldg(S.iqatom + i) != 0) continue;
-                lrf_accumulate<R>(part, ldg(x + 3 * i) - cx, ldg(x + 3 * i + 1) - cy, ldg(x + 3 * i + 2) - cz,
-                                  ldg(S.crg + i));
-              }
-            } else {
-              for (int k = 0; k < S.sa; k++) {
-                const int i = rb + k;
-                lrf_accumulate<R>(part, ldg(x + 3 * i) - cx, ldg(x + 3 * i + 1) - cy, ldg(x + 3 * i + 2) - cz,
-                                  ldg(S.crg + i));
-              }
-            }
-            if constexpr (sizeof(R) != 4) {
-#pragma unroll
-              for (int k = 0; k < 20; k++) acc[k] += (double)part[k];
-            }
-          }
+      const d4v G = gref[min(b, S.ncgp - 1)];
+      bool in = false, far = false;
+      if (b < S.ncgp && b != a && G.w == 0.0) {
+        const bool b_sol = b < S.ncgp_solute;
+        const double cut2 = b_sol ? cut_sol : cut_wat;
+        double r2;
+        bool inside;
+        if (any_atom && (a_sol || b_sol)) {
+          inside = any_atom_pair(S, x, a, b, a_sol, b_sol, ra, group_ref_atom(S, b), cut2, r2);
+        } else if (a_sol != b_sol && quirk) {
+          // nbpwlist's test point: the switch atom up to the group's first
+          // water inside, atom glast + 1 after it (LRF off: r2 unused)
+          const int g = a_sol ? a : b, w = (a_sol ? b : a) - S.ncgp_solute;
+          const int wf = ldg(S.pw_first + g);
+          r2 = 0.0;
+          inside = w == wf ||
+                   (w > wf && dist4(x, ldg(S.glast + g) + 1, a_sol ? group_ref_atom(S, b) : ra) <= cut2);
+        } else {
+          r2 = dist4v(A, G);
+          inside = r2 <= cut2;
         }
+        in = inside;
+        far = !inside && use_lrf && r2 <= rclrf2;
       }
       const uint64_t m = __ballot(in);
       const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
       if (in && pos < S.cap) stg(row + pos, b);
       count += __popcll(m);
+      const uint64_t fw = __ballot(far);
+      if (fw != 0) {
+        // the chunk's atoms: the run from b0's first to its last group's last
+        const int k0 = ldg(S.gfirst + b0), k1 = ldg(S.glast + min(b0 + QNB_WAVE, S.ncgp) - 1) + 1;
+        const auto gslot = gp(S.gslot);
+        // one iteration ahead: the next record and its group slot are in
+        // flight while this one is evaluated
+        using rec_t = std::conditional_t<sizeof(R) == 4, f4v, d4v>;
+        const auto recs = (gptr_t<rec_t>)(sizeof(R) == 4 ? (const void*)S.xqg : (const void*)S.xqgd);
+        // (loads past the run read its last record and are not used)
+        int k = k0 + lane;
+        rec_t r = recs[min(k, k1 - 1)];
+        int sl = gslot[min(k, k1 - 1)];
+        for (; k - lane < k1; k += QNB_WAVE) {
+          const int kn = min(k + QNB_WAVE, k1 - 1);
+          const rec_t rn = recs[kn];
+          const int sn = gslot[kn];
+          if (k < k1 && ((fw >> sl) & 1ull)) {
+            if constexpr (sizeof(R) == 4) lrf_accumulate_f(pf, r.x - cxf, r.y - cyf, r.z - czf, r.w);
+            else lrf_accumulate<double>(acc, r.x - cx, r.y - cy, r.z - cz, r.w);
+          }
+          r = rn;
+          sl = sn;
+        }
+        offered += (k1 - k0 + QNB_WAVE - 1) / QNB_WAVE;
+      }
       if constexpr (sizeof(R) == 4) {
-        if (__ballot(pf_atoms >= 10)) {
+        if (offered >= LRF_FLUSH) {
           lrf_flush_f(acc, pf);
 #pragma unroll
-          for (int k = 0; k < LRF_NF; k++) pf[k] = 0.0f;
-          pf_atoms = 0;
+          for (int c = 0; c < LRF_NF; c++) pf[c] = 0.0f;
+          offered = 0;
         }
       }
     }
