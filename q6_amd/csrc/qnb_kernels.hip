@@ -135,16 +135,18 @@ __device__ __forceinline__ void lrf_accumulate(R* acc, double drx, double dry, d
 // positions (x_i and the centre each rounded once), q the atom's charge.
 // The lane keeps raw sums that need no per-atom constant or subtraction:
 //   p[0] = sum q/r,  p[1..3] = sum f0 d_a,  p[4..9] = sum w1 d_a d_b,
-//   p[10..19] = sum w2 d_a d_b d_c,  p[20] = sum f0,  p[21..23] = sum w1 d_a
-// with f0 = q/r^3, w1 = f0/r^2, w2 = w1/r^2 (component order as acc);
-// lrf_flush_f forms the expansion from them in FP64:
-//   phi1 = -p[1..3], phi2_ab = 3 p_ab - delta_ab p[20],
-//   phi3_abc = -15 p_abc + 3 (delta_ab H_c + delta_ac H_b + delta_bc H_a)
-// which is lrf_update's field1 = 3 f0/r^2, field2 = -field1/r^2 and its
-// 5 field2 d d d term with the -field2 r^2 (delta d) parts.
-#define LRF_NF 24
+//   p[10..19] = sum w2 d_a d_b d_c
+// with f0 = q/r^3, w1 = f0/r^2, w2 = w1/r^2 (component order as acc).
+// lrf_update's field1 = 3 f0/r^2 and its 5 field2 d d d term (field2 =
+// -field1/r^2) give phi2_ab = 3 p_ab - delta_ab sum f0 and
+// phi3_abc = -15 p_abc + 3 (delta_ab H_c + delta_ac H_b + delta_bc H_a),
+// H_a = sum w1 d_a; as w1 r^2 = f0 and w2 r^2 = w1, sum f0 is the trace of
+// p[4..9] and H_a the trace of p[10..19] over the other two indices, which
+// lrf_flush_f forms in FP64.
+#define LRF_NF 20
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void lrf_accumulate_f(float* p, float dx, float dy, float dz, float q) {
   const float r2 = dx * dx + dy * dy + dz * dz;
   float ri2, ri;
@@ -157,7 +159,6 @@ __device__ __forceinline__ void lrf_accumulate_f(float* p, float dx, float dy, f
   p[1] += dx * f0;
   p[2] += dy * f0;
   p[3] += dz * f0;
-  p[20] += f0;
   const float xx = dx * dx, xy = dx * dy, xz = dx * dz, yy = dy * dy, yz = dy * dz, zz = dz * dz;
   p[4] += xx * w1;
   p[5] += xy * w1;
@@ -165,9 +166,6 @@ __device__ __forceinline__ void lrf_accumulate_f(float* p, float dx, float dy, f
   p[7] += yy * w1;
   p[8] += yz * w1;
   p[9] += zz * w1;
-  p[21] += dx * w1;
-  p[22] += dy * w1;
-  p[23] += dz * w1;
   const float gx = w2 * dx, gy = w2 * dy, gz = w2 * dz;
   p[10] += gx * xx;
   p[11] += gx * xy;
@@ -183,27 +181,50 @@ __device__ __forceinline__ void lrf_accumulate_f(float* p, float dx, float dy, f
 
 // adds a lane's raw FP32 sums to its FP64 expansion
 __device__ __forceinline__ void lrf_flush_f(double* acc, const float* p) {
-  const double s0 = p[20], hx = p[21], hy = p[22], hz = p[23];
-  acc[0] += (double)p[0];
-  acc[1] -= (double)p[1];
-  acc[2] -= (double)p[2];
-  acc[3] -= (double)p[3];
-  acc[4] += 3.0 * (double)p[4] - s0;
-  acc[5] += 3.0 * (double)p[5];
-  acc[6] += 3.0 * (double)p[6];
-  acc[7] += 3.0 * (double)p[7] - s0;
-  acc[8] += 3.0 * (double)p[8];
-  acc[9] += 3.0 * (double)p[9] - s0;
-  acc[10] += 9.0 * hx - 15.0 * (double)p[10];
-  acc[11] += 3.0 * hy - 15.0 * (double)p[11];
-  acc[12] += 3.0 * hz - 15.0 * (double)p[12];
-  acc[13] += 3.0 * hx - 15.0 * (double)p[13];
-  acc[14] -= 15.0 * (double)p[14];
-  acc[15] += 3.0 * hx - 15.0 * (double)p[15];
-  acc[16] += 9.0 * hy - 15.0 * (double)p[16];
-  acc[17] += 3.0 * hz - 15.0 * (double)p[17];
-  acc[18] += 3.0 * hy - 15.0 * (double)p[18];
-  acc[19] += 9.0 * hz - 15.0 * (double)p[19];
+  double q[LRF_NF];
+#pragma unroll
+  for (int k = 0; k < LRF_NF; k++) q[k] = (double)p[k];
+  const double s0 = q[4] + q[7] + q[9];
+  const double hx = q[10] + q[13] + q[15], hy = q[11] + q[16] + q[18], hz = q[12] + q[17] + q[19];
+  acc[0] += q[0];
+  acc[1] -= q[1];
+  acc[2] -= q[2];
+  acc[3] -= q[3];
+  acc[4] += 3.0 * q[4] - s0;
+  acc[5] += 3.0 * q[5];
+  acc[6] += 3.0 * q[6];
+  acc[7] += 3.0 * q[7] - s0;
+  acc[8] += 3.0 * q[8];
+  acc[9] += 3.0 * q[9] - s0;
+  acc[10] += 9.0 * hx - 15.0 * q[10];
+  acc[11] += 3.0 * hy - 15.0 * q[11];
+  acc[12] += 3.0 * hz - 15.0 * q[12];
+  acc[13] += 3.0 * hx - 15.0 * q[13];
+  acc[14] -= 15.0 * q[14];
+  acc[15] += 3.0 * hx - 15.0 * q[15];
+  acc[16] += 9.0 * hy - 15.0 * q[16];
+  acc[17] += 3.0 * hz - 15.0 * q[17];
+  acc[18] += 3.0 * hy - 15.0 * q[18];
+  acc[19] += 9.0 * hz - 15.0 * q[19];
+}
+
+// Raw buffer loads (32-bit byte offsets from a base in scalar registers; an
+// offset at or past `bytes` reads 0): the sweep's packed arrays are indexed
+// without 64-bit address arithmetic or clamping.
+typedef unsigned int u4v __attribute__((vector_size(16)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, size_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ f4v buf_f4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ d4v buf_d4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  const d2v lo = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  const d2v hi = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0));
+  return d4v{lo.x, lo.y, hi.x, hi.y};
+}
+__device__ __forceinline__ int buf_i(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
 }
 
 // Packs the LRF sources before a list update, one record per atom in
@@ -259,6 +280,13 @@ __device__ __forceinline__ int t3index(int a, int b, int c) {
   return 19;
 }
 
+// one packed LRF source record: FP32 on the mixed path, FP64 on the plain one
+template <typename R>
+__device__ __forceinline__ auto lrf_rec(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(R) == 4) return buf_f4(r, off);
+  else return buf_d4(r, off);
+}
+
 // dist4 on packed reference positions: |b - a|^2, ((dx^2 + dy^2) + dz^2)
 __device__ __forceinline__ double dist4v(const d4v& a, const d4v& b) {
 #pragma clang fp contract(off)
@@ -311,21 +339,24 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
   // mixed path: the lane's raw FP32 sums since the last flush to the FP64
   // expansion, flushed after a chunk once a lane has been offered LRF_FLUSH
   // atoms (at most LRF_FLUSH - 1 plus one chunk's share)
-  constexpr int LRF_FLUSH = 10;
+  constexpr int LRF_FLUSH = 16;
   float pf[LRF_NF];
 #pragma unroll
   for (int k = 0; k < LRF_NF; k++) pf[k] = 0.0f;
   int offered = 0;
   if (!ldg(S.excl + ra)) {
-    const auto gref = (gptr_t<d4v>)S.gref;
-    const d4v A = gref[a];
+    const auto grs = buf_rsrc(S.gref, (size_t)S.ncgp * 32);
+    const d4v A = buf_d4(grs, (uint32_t)a * 32);
+    const auto rs = buf_rsrc(sizeof(R) == 4 ? (const void*)S.xqg : (const void*)S.xqgd,
+                             (size_t)S.ncgpat * (sizeof(R) == 4 ? 16 : 32));
+    const auto gs = buf_rsrc(S.gslot, (size_t)S.ncgpat * 4);
     // the cut-off of a pair with a solute group and with a water, given a
     const double cut_sol = a_sol ? S.rcpp2 : S.rcpw2, cut_wat = a_sol ? S.rcpw2 : S.rcww2;
     const bool quirk = pw_quirk(S), any_atom = ANY && S.any_atom, use_lrf = S.use_lrf;
     const double rclrf2 = S.rclrf2;
     for (int b0 = 0; b0 < S.ncgp; b0 += QNB_WAVE) {
       const int b = b0 + lane;
-      const d4v G = gref[min(b, S.ncgp - 1)];
+      const d4v G = buf_d4(grs, (uint32_t)b * 32);  // 0 past the last group
       bool in = false, far = false;
       if (b < S.ncgp && b != a && G.w == 0.0) {
         const bool b_sol = b < S.ncgp_solute;
@@ -357,19 +388,18 @@ __global__ __launch_bounds__(512) void k_lists_lrf(const DevSys* __restrict__ sy
       if (fw != 0) {
         // the chunk's atoms: the run from b0's first to its last group's last
         const int k0 = ldg(S.gfirst + b0), k1 = ldg(S.glast + min(b0 + QNB_WAVE, S.ncgp) - 1) + 1;
-        const auto gslot = gp(S.gslot);
         // one iteration ahead: the next record and its group slot are in
-        // flight while this one is evaluated
-        using rec_t = std::conditional_t<sizeof(R) == 4, f4v, d4v>;
-        const auto recs = (gptr_t<rec_t>)(sizeof(R) == 4 ? (const void*)S.xqg : (const void*)S.xqgd);
-        // (loads past the run read its last record and are not used)
+        // flight while this one is evaluated (reads past the run are unused)
+        constexpr uint32_t RB = sizeof(R) == 4 ? 16 : 32;
         int k = k0 + lane;
-        rec_t r = recs[min(k, k1 - 1)];
-        int sl = gslot[min(k, k1 - 1)];
+        uint32_t off = (uint32_t)k * RB, offs = (uint32_t)k * 4;
+        auto r = lrf_rec<R>(rs, off);
+        int sl = buf_i(gs, offs);
         for (; k - lane < k1; k += QNB_WAVE) {
-          const int kn = min(k + QNB_WAVE, k1 - 1);
-          const rec_t rn = recs[kn];
-          const int sn = gslot[kn];
+          off += QNB_WAVE * RB;
+          offs += QNB_WAVE * 4;
+          const auto rn = lrf_rec<R>(rs, off);
+          const int sn = buf_i(gs, offs);
           if (k < k1 && ((fw >> sl) & 1ull)) {
             if constexpr (sizeof(R) == 4) lrf_accumulate_f(pf, r.x - cxf, r.y - cyf, r.z - czf, r.w);
             else lrf_accumulate<double>(acc, r.x - cx, r.y - cy, r.z - cz, r.w);
