@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define QNB_ABI_VERSION 5
+#define QNB_ABI_VERSION 6
 
 enum qnb_status {
   QNB_OK = 0,
@@ -140,6 +140,18 @@ typedef struct qnb_system {
   /* qw_* [nstates][solv_atom][nqat]: (iq, j, s) at [iq + nqat*(j +
    * solv_atom*s)] (qw_precomp, src/simprep.f90:3272-3292) */
   const double *qw_A, *qw_B, *qw_el, *qw_sc;
+
+  /* nonbonded pairs inside each solvent molecule (nonbnd_solv_int,
+   * prep_sim_precompute_solvent src/simprep.f90:4004-4138; evaluated by
+   * nonbond_solvent_internal src/nonbondene.f90:6082-6154 for every solvent
+   * molecule, into the solvent-solvent energies): n_solv_int pairs,
+   * solv_int_ij [n][2] the two atoms' 0-based offsets in the molecule,
+   * solv_int_par [n][3] vdW A, vdW B and the charge product.  Qdyn6 builds
+   * the list only when the topology has solvent torsions (0 pairs
+   * otherwise); pairs listed twice are evaluated twice, as there. */
+  int32_t n_solv_int;
+  const int32_t *solv_int_ij;
+  const double *solv_int_par;
 } qnb_system;
 
 /* Run parameters: cutoffs (Angstrom, not squared), sphere, LRF, SCAAS. */
@@ -245,9 +257,12 @@ enum qnb_term {
   QNB_T_QQ = 1 << 6,  /* nonbond_qq                    :5013 */
   QNB_T_QQP = 1 << 7, /* nonbond_qqp                   :5085 */
   QNB_T_RAD = 1 << 8, /* restrain_solvent              :6466 */
-  QNB_T_POL = 1 << 9  /* watpol                        :6548 */
+  QNB_T_POL = 1 << 9,  /* watpol                       :6548 */
+  QNB_T_SINT = 1 << 10 /* nonbond_solvent_internal     :6082; its energies go
+                          to ww_el / ww_vdw as there (added to nonbond_ww's
+                          when QNB_T_WW is selected too) */
 };
-#define QNB_T_NONBONDED (QNB_T_WW | QNB_T_QW | QNB_T_PW | QNB_T_PP | QNB_T_QP | QNB_T_LRF)
+#define QNB_T_NONBONDED (QNB_T_WW | QNB_T_QW | QNB_T_PW | QNB_T_PP | QNB_T_QP | QNB_T_LRF | QNB_T_SINT)
 
 /* Evaluates the selected terms (mask of qnb_term) at x, adds their gradient
  * into d and writes their energies; unselected fields are left untouched.

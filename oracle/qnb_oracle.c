@@ -614,6 +614,30 @@ static void nonbond_ww(orc_handle *h, const double *x, double *d, double *el, do
   }
 }
 
+/* nonbond_solvent_internal (nonbondene.f90:6082-6154): the listed pairs
+ * inside every solvent molecule, excluded molecules included, with q_dist2
+ * and the routine's own grouping of dv, into the solvent-solvent energies */
+static void nonbond_solvent_internal(orc_handle *h, const double *x, double *d, double *el, double *vdw) {
+  const qnb_system *s = &h->s;
+  for (int iw = 0; iw < s->nwat; iw++) {
+    const int is = s->nat_solute + s->solv_atom * iw;
+    for (int ip = 0; ip < s->n_solv_int; ip++) {
+      const int ia = is + s->solv_int_ij[2 * ip], ib = is + s->solv_int_ij[2 * ip + 1];
+      const double *pr = s->solv_int_par + 3 * ip;
+      vec3 t = {x[3 * ib] - x[3 * ia], x[3 * ib + 1] - x[3 * ia + 1], x[3 * ib + 2] - x[3 * ia + 2]};
+      double r2 = 1.0 / sq(t);
+      double r = sqrt(r2);
+      double r6 = r2 * r2 * r2;
+      double r12 = r6 * r6;
+      double Vel = pr[2] * r, Va = pr[0] * r12, Vb = pr[1] * r6;
+      double dv = r2 * (-Vel) + r2 * (-12.0 * Va + 6.0 * Vb);
+      *el = *el + Vel;
+      *vdw = *vdw + Va - Vb;
+      dupd(d, ia, ib, t, dv);
+    }
+  }
+}
+
 /* nbe_qx (nonbonded.f90:200-222) with q_dist3's r2, r, r6 */
 static inline double nbe_qx(double r2, double r, double r6in, const double *pr, double lam, double *Vel,
                             double *Va, double *Vb) {
@@ -954,6 +978,7 @@ int orc_eval_terms(orc_handle *h, uint32_t mask, const double *x, double *d, dou
       if (mask & QNB_T_QW) nonbond_qw(h, x, d, EQ);
       if (mask & QNB_T_WW) nonbond_ww(h, x, d, &E->ww_el, &E->ww_vdw);
     }
+    if ((mask & QNB_T_SINT) && s->n_solv_int > 0) nonbond_solvent_internal(h, x, d, &E->ww_el, &E->ww_vdw);
     if (mask & QNB_T_PW) nonbond_list(&h->pw, x, d, &E->pw_el, &E->pw_vdw);
   }
   if (mask & QNB_T_PP) nonbond_list(&h->pp, x, d, &E->pp_el, &E->pp_vdw);

@@ -290,6 +290,18 @@ program qnb_dump
       call nonbond_ww(e_nb)
     end if
     call dump_term(u, 'ww', (/e_nb%el, e_nb%vdw/), 2)
+    if (ntors > ntors_solute) then
+      ! pairs inside each solvent molecule (pot_energy_nonbonds calls it
+      ! after the solvent loops when the solvent has torsions), with the list
+      ! prep_sim_precompute_solvent built
+      d(:) = zero; e_nb = NB_ENERGIES(zero, zero)
+      call nonbond_solvent_internal(e_nb)
+      call dump_term(u, 'solv_int', (/e_nb%el, e_nb%vdw/), 2)
+      n = num_solv_int
+      call wi(u, 'solv_int_ij', 2*n, (/(nonbnd_solv_int(i)%i, nonbnd_solv_int(i)%j, i=1,n)/), 2, (/2, n/))
+      call wd(u, 'solv_int_par', 3*n, (/(nonbnd_solv_int(i)%vdWA, nonbnd_solv_int(i)%vdWB, &
+              nonbnd_solv_int(i)%elec, i=1,n)/), 2, (/3, n/))
+    end if
     if (nq > 0) then
       d(:) = zero; eq_nb(:) = NB_ENERGIES(zero, zero)
       if (ivdw_rule == VDW_GEOMETRIC .and. solvent_type == SOLVENT_SPC) then
@@ -583,6 +595,11 @@ contains
         call fd(u, 'fill_qw_el', fs%qw_el, nq_*sa_*ns_)
         call fd(u, 'fill_qw_sc', fs%qw_sc, nq_*sa_*ns_)
       end if
+    end if
+    call wi1(u, 'fill_n_solv_int', int(fs%n_solv_int))
+    if (fs%n_solv_int > 0) then
+      call fi(u, 'fill_solv_int_ij', fs%solv_int_ij, 2*fs%n_solv_int)
+      call fd(u, 'fill_solv_int_par', fs%solv_int_par, 3*fs%n_solv_int)
     end if
     call wd(u, 'fill_cutoffs', 5, (/real(fp%Rcpp, prec), real(fp%Rcpw, prec), real(fp%Rcww, prec), &
             real(fp%Rcq, prec), real(fp%RcLRF, prec)/), 1, (/5/))

@@ -19,9 +19,9 @@ QNB_PREC_MIXED = 1
 
 LIST_PP, LIST_PW, LIST_WW, LIST_QP, LIST_QW, LIST_QQ, LIST_QQP = range(7)
 
-T_WW, T_QW, T_PW, T_PP, T_QP, T_LRF, T_QQ, T_QQP, T_RAD, T_POL = (1 << k for k in range(10))
-T_NONBONDED = T_WW | T_QW | T_PW | T_PP | T_QP | T_LRF
-T_ALL = (1 << 10) - 1
+T_WW, T_QW, T_PW, T_PP, T_QP, T_LRF, T_QQ, T_QQP, T_RAD, T_POL, T_SINT = (1 << k for k in range(11))
+T_NONBONDED = T_WW | T_QW | T_PW | T_PP | T_QP | T_LRF | T_SINT
+T_ALL = (1 << 11) - 1
 
 _pd = C.POINTER(C.c_double)
 _pi = C.POINTER(C.c_int32)
@@ -44,6 +44,7 @@ class QnbSystem(C.Structure):
         ("qp_A", _pd), ("qp_B", _pd), ("qp_el", _pd), ("qp_sc", _pd), ("qp_set", _pu8),
         ("qq_A", _pd), ("qq_B", _pd), ("qq_el", _pd), ("qq_sc", _pd), ("qq_set", _pu8), ("qq_soft", _pu8),
         ("qw_A", _pd), ("qw_B", _pd), ("qw_el", _pd), ("qw_sc", _pd),
+        ("n_solv_int", C.c_int32), ("solv_int_ij", _pi), ("solv_int_par", _pd),
     ]
 
 
@@ -177,7 +178,7 @@ class QnbFepOut(C.Structure):
                 ("bin_c2", _pd), ("bin_r", _pd), ("ptsum", _pi)]
 
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 # every symbol include/qnb.h declares (checked by the CPU test suite)
 EXPORTS = [

@@ -21,6 +21,8 @@ __global__ void k_lists_lrf(const DevSys*);
 __global__ void k_qlists(const DevSys*);
 template <typename R, int NS, bool E, int CAP>
 __global__ void k_wforces(const DevSys*, uint32_t, int, int, int);
+template <typename R, int NS>
+__global__ void k_gforces(const DevSys*, uint32_t, int, int, int);
 template <typename R, int NS, bool E, int CAP>
 __global__ void k_sforces(const DevSys*, uint32_t, int, int, int);
 template <typename R, int NS>
@@ -64,8 +66,17 @@ int validate_system(const qnb_system* s, const qnb_params* p) {
   if (!s || !p) return fail(QNB_EINVAL, "null system or params");
   if (s->natom <= 0 || s->nat_solute < 0 || s->nat_solute > s->natom) return fail(QNB_EINVAL, "bad atom counts");
   if (s->nstates < 1 || s->nstates > QNB_MAX_STATES) return fail(QNB_EINVAL, "nstates must be 1..%d", QNB_MAX_STATES);
-  if (s->nwat > 0 && s->solv_atom != 3)
-    return fail(QNB_EUNSUPPORTED, "only 3-site solvent is supported (solv_atom=%d)", s->solv_atom);
+  if (s->nwat > 0 && (s->solv_atom < 1 || s->solv_atom > QNB_WAVE))
+    return fail(QNB_EUNSUPPORTED, "solvent molecules of 1..%d atoms are supported (solv_atom=%d)", QNB_WAVE,
+                s->solv_atom);
+  if (s->nwat > 0 && s->solv_atom != 3 && p->wpol_restr)
+    return fail(QNB_EUNSUPPORTED, "the polarisation restraint needs 3-site water (solv_atom=%d)", s->solv_atom);
+  if (s->n_solv_int < 0 || (s->n_solv_int > 0 && (!s->solv_int_ij || !s->solv_int_par)))
+    return fail(QNB_EINVAL, "solvent internal pairs: count and arrays");
+  for (int k = 0; k < s->n_solv_int; k++)
+    for (int e = 0; e < 2; e++)
+      if (s->solv_int_ij[2 * k + e] < 0 || s->solv_int_ij[2 * k + e] >= s->solv_atom)
+        return fail(QNB_EINVAL, "solvent internal pair %d: atom offset %d out of range", k, s->solv_int_ij[2 * k + e]);
   if (s->nat_solute + s->nwat * s->solv_atom != s->natom) return fail(QNB_EINVAL, "natom != nat_solute + nwat*solv_atom");
   if (s->ncgp != s->ncgp_solute + s->nwat) return fail(QNB_EINVAL, "each solvent molecule must be one charge group");
   if (p->nwpolr_shell < 0 || p->nwpolr_shell > QNB_MAX_SHELLS) return fail(QNB_EINVAL, "nwpolr_shell out of range");
@@ -326,6 +337,11 @@ int topo_build(DevSys& S, const qnb_system* s, HostTopo& T, DevAlloc& M) {
     UP(wwA, s->ww_A, sa * sa);
     UP(wwB, s->ww_B, sa * sa);
     UP(wwEl, s->ww_el, sa * sa);
+    S.n_sint = s->n_solv_int;
+    if (S.n_sint > 0) {
+      UP(sint_ij, s->solv_int_ij, 2 * (size_t)S.n_sint);
+      UP(sint_par, s->solv_int_par, 3 * (size_t)S.n_sint);
+    }
   }
   UP(qconn3, T.qconn3.data(), T.qconn3.size());
   if (nq > 0) {
@@ -645,7 +661,10 @@ template <typename R, int NS>
 static void launch_eval(qnb_handle* h, uint32_t mask) {
   const DevSys& S = h->hs;
   const int wpb = 4;
-  if (S.nwat > 0) {
+  if (S.nwat > 0 && S.sa != 3) {
+    const int nb = (S.nwat + FC_WPB - 1) / FC_WPB;
+    hipLaunchKernelGGL((k_gforces<R, NS>), dim3(nb, 1), dim3(64 * FC_WPB), 0, h->stream, h->dsys, mask, 0, nb, 1);
+  } else if (S.nwat > 0) {
     const int nb = (S.nwat + wpb - 1) / wpb;
     if (h->small_groups)
       hipLaunchKernelGGL((k_wforces<R, NS, true, WF_CAP_SMALL>), dim3(nb, 1), dim3(64 * wpb),
@@ -712,7 +731,7 @@ extern "C" int32_t qnb_eval_terms(qnb_handle* h, uint32_t mask, const double* x,
   if (E) {
     if (mask & QNB_T_PP) { E->pp_el = er[E_PP_EL]; E->pp_vdw = er[E_PP_VDW]; }
     if (mask & QNB_T_PW) { E->pw_el = er[E_PW_EL]; E->pw_vdw = er[E_PW_VDW]; }
-    if (mask & QNB_T_WW) { E->ww_el = er[E_WW_EL]; E->ww_vdw = er[E_WW_VDW]; }
+    if (mask & (QNB_T_WW | QNB_T_SINT)) { E->ww_el = er[E_WW_EL]; E->ww_vdw = er[E_WW_VDW]; }
     if (mask & QNB_T_LRF) E->lrf = er[E_LRF];
     if (mask & QNB_T_RAD) E->solvent_radial = er[E_RAD];
     if (mask & QNB_T_POL) E->water_pol = er[E_POL];

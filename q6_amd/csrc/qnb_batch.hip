@@ -186,11 +186,11 @@ __global__ __launch_bounds__(256) void k_atom_step(const DevSys* __restrict__ sy
     double f[3] = {0.0, 0.0, 0.0};
     double el = 0.0, er = 0.0;
     if (S.use_lrf && !ldg(S.excl + i) && ldg(S.iqatom + i) == 0) lrf_taylor_atom(S, i, f, el);
-    const int w = i - S.nat_solute;
-    if (w >= 0 && w % S.sa == 0 && S.sa == 3 && !ldg(S.excl + i)) {
+    const int src = radial_source(S, i);
+    if (src >= 0) {
       const double T = tfree[blockIdx.y];
       const double shift = S.fk_wsphere != 0.0 ? sqrt(S.Boltz * T / S.fk_wsphere) : 0.0;
-      radial_restraint(S, i, shift, f, er);
+      radial_restraint(S, src, shift, f, er);
     }
     if (i < M.nbat) bonded_atom(S, M, i, f, eb, ea);
     store_atom_gradient(S, i, f);
@@ -797,6 +797,9 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
   int rc = 0;
   for (int k = 0; k < n; k++) {
     if ((rc = validate_system(sys[k], par[k]))) return rc;
+    if (sys[k]->nwat > 0 && sys[k]->solv_atom != 3)
+      return fail(QNB_EUNSUPPORTED, "batched MD integrates rigid 3-site water only (instance %d: solv_atom=%d)", k,
+                  sys[k]->solv_atom);
     if (par[k]->precision != b->prec) return fail(QNB_EINVAL, "all instances need the same precision");
     DevSys& S = b->hs[k];
     memset(&S, 0, sizeof S);

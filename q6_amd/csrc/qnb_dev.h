@@ -87,6 +87,11 @@ struct DevSys {
   const double* wwA;       // [la + ka*sa]
   const double* wwB;
   const double* wwEl;
+  // pairs inside each solvent molecule (nonbond_solvent_internal): offsets
+  // [n_sint][2] and A, B, charge product [n_sint][3]
+  int n_sint;
+  const int* sint_ij;
+  const double* sint_par;
   const int* iqseq;
   const uint8_t* qconn3;
   const double* qpA;  // [i + nsol*(iq + nq*s)]
@@ -553,9 +558,9 @@ __device__ __forceinline__ void lrf_taylor_atom(const DevSys& S, int i, double* 
 
 // SCAAS radial restraint on one water oxygen (restrain_solvent,
 // nonbondene.f90:6466-6544, 3-site branch)
-__device__ __forceinline__ void radial_restraint(const DevSys& S, int i, double shift, double* f, double& e) {
+__device__ __forceinline__ void radial_restraint(const DevSys& S, int src, double shift, double* f, double& e) {
   const auto x = gp(S.x);
-  double vx = x[3 * i] - S.xw[0], vy = x[3 * i + 1] - S.xw[1], vz = x[3 * i + 2] - S.xw[2];
+  double vx = x[3 * src] - S.xw[0], vy = x[3 * src + 1] - S.xw[1], vz = x[3 * src + 2] - S.xw[2];
   double b = sqrt(vx * vx + vy * vy + vz * vz);
   double db = b - (S.rwat - shift);
   double erst, dv;
@@ -574,6 +579,21 @@ __device__ __forceinline__ void radial_restraint(const DevSys& S, int i, double 
   f[1] += vy * dv;
   f[2] += vz * dv;
   e += erst;
+}
+
+// The atom whose distance from xwcent drives atom i's SCAAS radial term
+// (restrain_solvent, nonbondene.f90:6466-6544), or -1: a 3-site water's first
+// atom acts on itself; in other solvents every heavy atom (by the first
+// molecule's pattern, is_heavy_solv) of a non-excluded molecule takes the
+// term at the distance of the molecule's first atom, as the reference
+// measures it.
+__device__ __forceinline__ int radial_source(const DevSys& S, int i) {
+  const int w = i - S.nat_solute;
+  if (w < 0) return -1;
+  const int isolv = w % S.sa, first = i - isolv;
+  if (ldg(S.excl + first)) return -1;
+  if (S.sa == 3) return isolv == 0 ? i : -1;
+  return ldg(S.heavy + S.nat_solute + isolv) ? first : -1;
 }
 
 // Q-solute group test (nbqplist, nonbondene.f90:3647-3742): the group's

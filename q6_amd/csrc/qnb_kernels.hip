@@ -505,8 +505,9 @@ __global__ __launch_bounds__(1024) void k_qlists(const DevSys* __restrict__ syst
 }
 
 // term bits (include/qnb.h enum qnb_term)
-enum : uint32_t { T_WW = 1, T_QW = 2, T_PW = 4, T_PP = 8, T_QP = 16, T_LRF = 32, T_QQ = 64, T_QQP = 128,
-                  T_RAD = 256, T_POL = 512 };
+enum : uint32_t { T_WW = QNB_T_WW, T_QW = QNB_T_QW, T_PW = QNB_T_PW, T_PP = QNB_T_PP, T_QP = QNB_T_QP,
+                  T_LRF = QNB_T_LRF, T_QQ = QNB_T_QQ, T_QQP = QNB_T_QQP, T_RAD = QNB_T_RAD, T_POL = QNB_T_POL,
+                  T_SINT = QNB_T_SINT };
 
 // ---------------------------------------------------------------------------
 // Receiver-side forces: one wavefront per charge group a.  The wave expands
@@ -778,7 +779,7 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_sforces(const DevSy
       s_rm[lane] = ldg(S.m14mask + i);
       s_rq[lane] = ldg(S.crg + i);
       // the atom's solute-water parameters against the 3 water sites
-      if (S.nwat > 0)
+      if (S.nwat > 0 && sa == 3)
         for (int kb = 0; kb < 3; kb++) {
           const size_t tt = (size_t)i + (size_t)nsol * kb;
           s_pw[lane][kb][0] = (R)ldg(S.pwA + tt);
@@ -887,7 +888,7 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_sforces(const DevSy
         }
       }
       // ---- partner waters: lane l takes site kb = l % 3 of water l / 3 of the round
-      if ((mask & T_PW) && nsr < nn && lane < 3 * (QNB_WAVE / 3)) {
+      if ((mask & T_PW) && nsr < nn && sa == 3 && lane < 3 * (QNB_WAVE / 3)) {
         const int kb = lane % 3;
         R pw[CH][3];
 #pragma unroll
@@ -907,6 +908,34 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_sforces(const DevSy
             if (!rok[k]) continue;
             const R ux = dx - ro[k][0], uy = dy - ro[k][1], uz = dz - ro[k][2];
             const R dv = PairMath<R>::nbe(ux, uy, uz, pw[k][0], pw[k][1], pw[k][2], pel, pvdw, want_e);
+            f[k][0] += ux * dv;
+            f[k][1] += uy * dv;
+            f[k][2] += uz * dv;
+          }
+          if (want_e) {
+            e_pwel += (double)pel;
+            e_pwvdw += (double)pvdw;
+          }
+        }
+      }
+      // ---- partner molecules of any other solvent: lane l takes site
+      // kb = l % sa of molecule l / sa of the round, 64 / sa molecules per
+      // round, the parameters read per receiver atom
+      if ((mask & T_PW) && nsr < nn && sa != 3 && lane < sa * (QNB_WAVE / sa)) {
+        const int kb = lane % sa, M = QNB_WAVE / sa;
+        for (int p = nsr + lane / sa; p < nn; p += M) {
+          const int b = ldg(row + p);
+          const int j = nsol + sa * (b - ncs) + kb;
+          const R dx = (R)(ldg(x + 3 * j) - xr0), dy = (R)(ldg(x + 3 * j + 1) - xr1),
+                  dz = (R)(ldg(x + 3 * j + 2) - xr2);
+          R pel = R(0), pvdw = R(0);
+#pragma unroll
+          for (int k = 0; k < CH; k++) {
+            if (!rok[k]) continue;
+            const size_t tt = (size_t)s_ia[k] + (size_t)nsol * kb;
+            const R ux = dx - ro[k][0], uy = dy - ro[k][1], uz = dz - ro[k][2];
+            const R dv = PairMath<R>::nbe(ux, uy, uz, (R)ldg(S.pwA + tt), (R)ldg(S.pwB + tt), (R)ldg(S.pwEl + tt),
+                                          pel, pvdw, want_e);
             f[k][0] += ux * dv;
             f[k][1] += uy * dv;
             f[k][2] += uz * dv;
@@ -1250,11 +1279,187 @@ __global__ __launch_bounds__(64 * FC_WPB) void k_wforces(const DevSys* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Receivers of a solvent other than 3-site water (any solv_atom up to 64):
+// one wavefront per solvent molecule a; lane l takes receiver atom
+// k = l % sa and, with M = 64 / sa lanes per atom, every M-th partner atom
+// from m = l / sa on, in the partner order of the reference's loops:
+//  * solute partner groups (the row's prefix): their non-Q atoms with the pw
+//    parameters (nonbond_pw, nonbondene.f90:4856-4916; energy counted on the
+//    solute side);
+//  * partner molecules: the sa x sa atom pairs with ww_precomp
+//    (nonbond_ww 5509-5573), energy on the owner side of the pair;
+//  * the Q-atoms (receiver side of nonbond_qw 5339-5418);
+//  * the pairs inside the molecule (nonbond_solvent_internal 6082-6154),
+//    excluded molecules included: the lane of each end atom evaluates the
+//    pair, the first end counts its energy.
+// The lanes of an atom add their FP64 partials in lane order (LDS); writes d
+// of the molecule's atoms and the wave's ww and qw energy partials on the
+// sampled steps.
+template <typename R, int NS>
+__global__ __launch_bounds__(64 * FC_WPB) void k_gforces(const DevSys* __restrict__ systems, uint32_t mask,
+                                                         int imap, int nb, int ninst) {
+  int inst, blk;
+  if (!map_block(imap, nb, ninst, inst, blk)) return;
+  const DevSys& S = systems[inst];
+  const int wib = threadIdx.x / QNB_WAVE;
+  const int a = __builtin_amdgcn_readfirstlane(S.ncgp_solute + blk * (int)(blockDim.x / QNB_WAVE) + wib);
+  if (a >= S.ncgp) return;
+  __shared__ double s_f[FC_WPB][QNB_WAVE][3];
+  const int lane = threadIdx.x % QNB_WAVE;
+  const int sa = S.sa, M = QNB_WAVE / sa;
+  const int k = lane % sa, m = lane / sa;
+  const bool active = m < M;
+  const double* __restrict__ x = S.x;
+  const int nsol = S.nat_solute, ncs = S.ncgp_solute, nq = S.nqat, ns = S.nstates;
+  const int base = nsol + sa * (a - ncs), ia = base + k;
+  const bool aexcl = ldg(S.excl + base);
+  const bool want_e = want_energies(S);
+  const int nn = min(ldg(S.nnbr + a), S.cap);
+  const int* __restrict__ row = S.nbr + (size_t)a * S.cap;
+  const double xi0 = ldg(x + 3 * ia), xi1 = ldg(x + 3 * ia + 1), xi2 = ldg(x + 3 * ia + 2);
+  double f[3] = {0.0, 0.0, 0.0};
+  double e_el = 0.0, e_vdw = 0.0;
+  double eq[2 * NS];
+#pragma unroll
+  for (int c = 0; c < 2 * NS; c++) eq[c] = 0.0;
+  if (active && !aexcl) {
+    for (int p = 0; p < nn; p++) {
+      const int b = ldg(row + p);
+      if (b < ncs) {
+        if (!(mask & T_PW)) continue;
+        const int t1 = ldg(S.glast + b);
+        for (int t = ldg(S.gfirst + b) + m; t <= t1; t += M) {
+          const int j = ldg(S.cgpatom + t);
+          if (ldg(S.iqatom + j) != 0) continue;
+          const R ux = (R)(ldg(x + 3 * j) - xi0), uy = (R)(ldg(x + 3 * j + 1) - xi1),
+                  uz = (R)(ldg(x + 3 * j + 2) - xi2);
+          const size_t tt = (size_t)j + (size_t)nsol * k;
+          R ee = R(0), ev = R(0);
+          const R dv = PairMath<R>::nbe(ux, uy, uz, (R)ldg(S.pwA + tt), (R)ldg(S.pwB + tt), (R)ldg(S.pwEl + tt),
+                                        ee, ev, false);
+          f[0] += (double)(ux * dv);
+          f[1] += (double)(uy * dv);
+          f[2] += (double)(uz * dv);
+        }
+      } else {
+        if (!(mask & T_WW)) continue;
+        const bool own = want_e && owns(a, b);
+        const int jb = nsol + sa * (b - ncs);
+        for (int kb = m; kb < sa; kb += M) {
+          const int j = jb + kb;
+          const R ux = (R)(ldg(x + 3 * j) - xi0), uy = (R)(ldg(x + 3 * j + 1) - xi1),
+                  uz = (R)(ldg(x + 3 * j + 2) - xi2);
+          const int t = k + kb * sa;
+          R ee = R(0), ev = R(0);
+          const R dv = (S.spc && (k != 0 || kb != 0))
+                           ? PairMath<R>::nbe_spc(ux, uy, uz, (R)ldg(S.wwEl + t), ee, own)
+                           : PairMath<R>::nbe(ux, uy, uz, (R)ldg(S.wwA + t), (R)ldg(S.wwB + t), (R)ldg(S.wwEl + t),
+                                              ee, ev, own);
+          f[0] += (double)(ux * dv);
+          f[1] += (double)(uy * dv);
+          f[2] += (double)(uz * dv);
+          if (own) {
+            e_el += (double)ee;
+            e_vdw += (double)ev;
+          }
+        }
+      }
+    }
+    if ((mask & T_QW) && nq > 0 && ldg(S.qw_in + (a - ncs))) {
+      for (int iq = m; iq < nq; iq += M) {
+        const int q = ldg(S.iqseq + iq);
+        const R ux = (R)(ldg(x + 3 * q) - xi0), uy = (R)(ldg(x + 3 * q + 1) - xi1), uz = (R)(ldg(x + 3 * q + 2) - xi2);
+        R r2, r;
+        rinv(ux * ux + uy * uy + uz * uz, r2, r);
+        R dvs = R(0);
+        if (k == 0 || !S.spc) {
+          const R r6 = r2 * r2 * r2;
+#pragma unroll
+          for (int st = 0; st < NS; st++) {
+            if (st >= ns) break;
+            const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * st);
+            dvs += PairMath<R>::nbe_qx(r2, r, r6, (R)ldg(S.qwA + t), (R)ldg(S.qwB + t), (R)ldg(S.qwEl + t),
+                                       (R)ldg(S.qwSc + t), (R)S.lambda[st], eq[2 * st], eq[2 * st + 1], want_e);
+          }
+        } else {
+#pragma unroll
+          for (int st = 0; st < NS; st++) {
+            if (st >= ns) break;
+            const size_t t = (size_t)iq + (size_t)nq * ((size_t)k + (size_t)sa * st);
+            const R Vel = (R)ldg(S.qwEl + t) * r;
+            if (want_e) eq[2 * st] += (double)Vel;
+            dvs += -r2 * Vel * (R)S.lambda[st];
+          }
+        }
+        f[0] += (double)(ux * dvs);
+        f[1] += (double)(uy * dvs);
+        f[2] += (double)(uz * dvs);
+      }
+    }
+  }
+  if ((mask & T_SINT) && S.n_sint > 0 && active && m == 0) {
+    for (int ip = 0; ip < S.n_sint; ip++) {
+      const int oa = ldg(S.sint_ij + 2 * ip), ob = ldg(S.sint_ij + 2 * ip + 1);
+      if (oa != k && ob != k) continue;
+      const int i0 = base + oa, i1 = base + ob;
+      const R vx = (R)(ldg(x + 3 * i1) - ldg(x + 3 * i0)), vy = (R)(ldg(x + 3 * i1 + 1) - ldg(x + 3 * i0 + 1)),
+              vz = (R)(ldg(x + 3 * i1 + 2) - ldg(x + 3 * i0 + 2));
+      R r2, r;
+      rinv(vx * vx + vy * vy + vz * vz, r2, r);
+      const R r6 = r2 * r2 * r2, r12 = r6 * r6;
+      const R Vel = (R)ldg(S.sint_par + 3 * ip + 2) * r, Va = (R)ldg(S.sint_par + 3 * ip) * r12,
+              Vb = (R)ldg(S.sint_par + 3 * ip + 1) * r6;
+      const R dv = r2 * (-Vel) + r2 * (R(-12) * Va + R(6) * Vb);
+      // d(ia) -= v dv, d(ib) += v dv; the lane holds the negative gradient
+      // part f (d = -f), as for the pairs above (u = partner - receiver)
+      const double sgn = oa == k ? 1.0 : -1.0;
+      if (oa == k && ob == k) continue;  // a self pair carries no force
+      f[0] += sgn * (double)(vx * dv);
+      f[1] += sgn * (double)(vy * dv);
+      f[2] += sgn * (double)(vz * dv);
+      if (want_e && oa == k) {
+        e_el += (double)Vel;
+        e_vdw += (double)(Va - Vb);
+      }
+    }
+  }
+  s_f[wib][lane][0] = f[0];
+  s_f[wib][lane][1] = f[1];
+  s_f[wib][lane][2] = f[2];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (lane < sa) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    for (int mm = 0; mm < M; mm++) {
+      t0 += s_f[wib][lane + sa * mm][0];
+      t1 += s_f[wib][lane + sa * mm][1];
+      t2 += s_f[wib][lane + sa * mm][2];
+    }
+    const auto d = gpm(S.d);
+    d[3 * ia] = -t0;
+    d[3 * ia + 1] = -t1;
+    d[3 * ia + 2] = -t2;
+  }
+  if (!want_e) return;
+  double part[2 + 2 * NS];
+  int slot[2 + 2 * NS];
+  part[0] = e_el, slot[0] = E_WW_EL;
+  part[1] = e_vdw, slot[1] = E_WW_VDW;
+#pragma unroll
+  for (int st = 0; st < NS; st++) {
+    part[2 + 2 * st] = eq[2 * st], slot[2 + 2 * st] = E_Q0 + 6 * st + 4;
+    part[3 + 2 * st] = eq[2 * st + 1], slot[3 + 2 * st] = E_Q0 + 6 * st + 5;
+  }
+  store_energy_record(S.epart + (size_t)a * QNB_NE, part, slot, lane);
+}
+
+// ---------------------------------------------------------------------------
 // Per-atom terms after the pair kernels, one thread per atom: the LRF Taylor
 // term (lrf_taylor, nonbondene.f90:507-571) of every non-Q, non-excluded
-// atom and the SCAAS radial restraint (restrain_solvent, 6466-6544) of every
-// non-excluded water oxygen; adds to d (store_atom_gradient).  On the sampled steps the atom's two
-// energies go to eatom[2i], eatom[2i+1] for the fixed-order reduction.
+// atom and the SCAAS radial restraint (restrain_solvent, 6466-6544) of the
+// solvent atoms radial_source names; adds to d (store_atom_gradient).  On the
+// sampled steps the atom's two energies go to eatom[2i], eatom[2i+1] for the
+// fixed-order reduction.
 __global__ __launch_bounds__(256) void k_atom_terms(const DevSys* __restrict__ systems, uint32_t mask,
                                                     const double* __restrict__ tfree) {
   const DevSys& S = systems[blockIdx.y];
@@ -1263,11 +1468,11 @@ __global__ __launch_bounds__(256) void k_atom_terms(const DevSys* __restrict__ s
   double f[3] = {0.0, 0.0, 0.0};
   double el = 0.0, er = 0.0;
   if ((mask & T_LRF) && S.use_lrf && !ldg(S.excl + i) && ldg(S.iqatom + i) == 0) lrf_taylor_atom(S, i, f, el);
-  const int w = i - S.nat_solute;
-  if ((mask & T_RAD) && w >= 0 && w % S.sa == 0 && S.sa == 3 && !ldg(S.excl + i)) {
+  const int src = (mask & T_RAD) ? radial_source(S, i) : -1;
+  if (src >= 0) {
     const double T = tfree[blockIdx.y];
     const double shift = S.fk_wsphere != 0.0 ? sqrt(S.Boltz * T / S.fk_wsphere) : 0.0;
-    radial_restraint(S, i, shift, f, er);
+    radial_restraint(S, src, shift, f, er);
   }
   store_atom_gradient(S, i, f);
   if (want_energies(S)) {
@@ -1781,7 +1986,8 @@ template __global__ void k_lists_lrf<float, true>(const DevSys*);
   template __global__ void k_wforces<R, NS, true, WF_CAP_SMALL>(const DevSys*, uint32_t, int, int, int); \
   template __global__ void k_wforces<R, NS, false, WF_CAP_SMALL>(const DevSys*, uint32_t, int, int, int); \
   template __global__ void k_wforces<R, NS, true, FC_CAP>(const DevSys*, uint32_t, int, int, int);       \
-  template __global__ void k_wforces<R, NS, false, FC_CAP>(const DevSys*, uint32_t, int, int, int);
+  template __global__ void k_wforces<R, NS, false, FC_CAP>(const DevSys*, uint32_t, int, int, int);       \
+  template __global__ void k_gforces<R, NS>(const DevSys*, uint32_t, int, int, int);
 QNB_INST(double, 1)
 QNB_INST(double, 2)
 QNB_INST(double, 4)

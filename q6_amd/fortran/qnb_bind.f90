@@ -12,7 +12,8 @@ module qnb_bind
   integer(c_int32_t), parameter :: QNB_OK = 0, QNB_PREC_FP64 = 0, QNB_PREC_MIXED = 1
   integer(c_int32_t), parameter :: QNB_MAX_STATES = 8, QNB_MAX_SHELLS = 8
   integer(c_int32_t), parameter :: QNB_T_WW = 1, QNB_T_QW = 2, QNB_T_PW = 4, QNB_T_PP = 8, QNB_T_QP = 16, &
-                                   QNB_T_LRF = 32, QNB_T_QQ = 64, QNB_T_QQP = 128, QNB_T_RAD = 256, QNB_T_POL = 512
+                                   QNB_T_LRF = 32, QNB_T_QQ = 64, QNB_T_QQP = 128, QNB_T_RAD = 256, QNB_T_POL = 512, &
+                                   QNB_T_SINT = 1024
 
   type, bind(c) :: qnb_system
     integer(c_int32_t) :: natom, nat_solute, nwat, solv_atom, ncgp, ncgp_solute, nqat, nstates
@@ -27,6 +28,8 @@ module qnb_bind
     type(c_ptr) :: qp_A, qp_B, qp_el, qp_sc, qp_set
     type(c_ptr) :: qq_A, qq_B, qq_el, qq_sc, qq_set, qq_soft
     type(c_ptr) :: qw_A, qw_B, qw_el, qw_sc
+    integer(c_int32_t) :: n_solv_int
+    type(c_ptr) :: solv_int_ij, solv_int_par
   end type qnb_system
 
   type, bind(c) :: qnb_params

@@ -17,6 +17,7 @@
 !                                                       src/simprep.f90:3069-3317
 !   Q-atoms, qconn                                      src/qatom.f90, src/globals.f90:427
 !   cut-offs, sphere, SCAAS and polarisation shells     src/globals.f90, src/topo.f90:132
+!   pairs inside a solvent molecule (nonbnd_solv_int)   src/simprep.f90:4004-4138
 module qnb_fill
   use, intrinsic :: iso_c_binding
   use qnb_bind
@@ -30,7 +31,8 @@ module qnb_fill
   real(c_double), allocatable, target, save :: f_pw(:, :), f_ww(:, :), f_qp(:, :), f_qq(:, :), f_qw(:, :)
   integer(c_int32_t), allocatable, target, save :: f_iac(:), f_iqatom(:), f_ljcod(:)
   integer(c_int32_t), allocatable, target, save :: f_first(:), f_last(:), f_iswitch(:), f_cgpatom(:)
-  integer(c_int32_t), allocatable, target, save :: f_exlong(:), f_14long(:), f_iqseq(:)
+  integer(c_int32_t), allocatable, target, save :: f_exlong(:), f_14long(:), f_iqseq(:), f_sint_ij(:)
+  real(c_double), allocatable, target, save :: f_sint_par(:)
   integer(c_int8_t), allocatable, target, save :: f_excl(:), f_heavy(:), f_listex(:), f_list14(:)
   integer(c_int8_t), allocatable, target, save :: f_qconn3(:), f_qp_set(:), f_qq_set(:), f_qq_soft(:)
   ! a valid address for tables a system does not have (never read then)
@@ -56,6 +58,7 @@ contains
     if (allocated(f_qp)) deallocate(f_qp, f_qp_set)
     if (allocated(f_qq)) deallocate(f_qq, f_qq_set, f_qq_soft)
     if (allocated(f_qw)) deallocate(f_qw)
+    if (allocated(f_sint_ij)) deallocate(f_sint_ij, f_sint_par)
   end subroutine
 
   ! sys, par: filled for Qdyn6's current state; precision: QNB_PREC_FP64 or
@@ -306,6 +309,26 @@ contains
       sys%qw_B = c_loc(f_qw(1, 2))
       sys%qw_el = c_loc(f_qw(1, 3))
       sys%qw_sc = c_loc(f_qw(1, 4))
+    end if
+
+    ! pairs inside a solvent molecule: listed only when the solvent has
+    ! torsions (nonbond_solvent_internal runs only then); offsets 0-based
+    sys%n_solv_int = 0
+    sys%solv_int_ij = c_loc(f_dummy_i)
+    sys%solv_int_par = c_loc(f_dummy_d)
+    if (nwat > 0 .and. ntors > ntors_solute .and. allocated(nonbnd_solv_int)) then
+      n = num_solv_int
+      allocate(f_sint_ij(2*n), f_sint_par(3*n))
+      do i = 1, n
+        f_sint_ij(2*i-1) = nonbnd_solv_int(i)%i - 1
+        f_sint_ij(2*i) = nonbnd_solv_int(i)%j - 1
+        f_sint_par(3*i-2) = nonbnd_solv_int(i)%vdWA
+        f_sint_par(3*i-1) = nonbnd_solv_int(i)%vdWB
+        f_sint_par(3*i) = nonbnd_solv_int(i)%elec
+      end do
+      sys%n_solv_int = n
+      sys%solv_int_ij = c_loc(f_sint_ij)
+      sys%solv_int_par = c_loc(f_sint_par)
     end if
 
     ! run parameters

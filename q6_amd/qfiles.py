@@ -45,35 +45,58 @@ def write_top(s, path, title="synthetic"):
     ntypes = int(n["ntypes"])
     L = ["Q topology file", "TITLE      " + title, "DATE       0", "VERSION    5.06", "FORCEFIELD synthetic",
          "FF_TYPE    1", "PRM_FILE   none", "END        of header"]
+    mol = md.get("solvent")  # a general solvent molecule (synth.replace_solvent), else TIP3P
+    sa = int(n["solv_atom"])
     L.append("%d %d %d %d = Total no. of atoms, no. of solute atoms, atoms per solvent molecule, solvent dielectric"
-             % (nat, nsol, 3, 80000))
+             % (nat, nsol, sa, 80000))
     L += _rows(np.asarray(s.x).ravel(), 6)
     L.append("%d = No. of integer atom codes. iac's:" % nat)
     L += _rows(np.asarray(a["iac"]) + 1, 16, str)
     # bonds: solute chains (code 1), water O-H (2) and H-H (3)
     bonds = np.asarray(md["bonds"], int).reshape(-1, 2)
     bl = [(i + 1, j + 1, 1) for i, j in bonds]
-    for w in range(nwat):
-        o = nsol + 3 * w + 1
-        bl += [(o, o + 1, 2), (o, o + 2, 2), (o + 1, o + 2, 3)]
+    if mol is None:
+        for w in range(nwat):
+            o = nsol + 3 * w + 1
+            bl += [(o, o + 1, 2), (o, o + 2, 2), (o + 1, o + 2, 3)]
+        bpar = [(450.0, md["water_roh"]), (450.0, md["water_rhh"])]
+    else:
+        for w in range(nwat):
+            o = nsol + sa * w + 1
+            bl += [(o + i, o + j, 2 + c) for i, j, c in mol["bonds"]]
+        bpar = mol["bond_par"]
     L.append("%d %d = No. of bonds, no. of solute bonds. i - j - icode:" % (len(bl), len(bonds)))
     L += _rows([v for b in bl for v in b], 15, str)
-    L.append("3 = No. of bond codes. Parameters:")
+    L.append("%d = No. of bond codes. Parameters:" % (1 + len(bpar)))
     L.append("1 %s %s" % (_g(md["bond_k"]), _g(md["bond_r0"])))
-    L.append("2 %s %s" % (_g(450.0), _g(md["water_roh"])))
-    L.append("3 %s %s" % (_g(450.0), _g(md["water_rhh"])))
+    L += ["%d %s %s" % (2 + c, _g(k), _g(r0)) for c, (k, r0) in enumerate(bpar)]
     angles = np.asarray(md["angles"], int).reshape(-1, 3)
     al = [(i + 1, j + 1, k + 1, 1) for i, j, k in angles]
-    for w in range(nwat):
-        o = nsol + 3 * w + 1
-        al.append((o + 1, o, o + 2, 2))
+    if mol is None:
+        for w in range(nwat):
+            o = nsol + 3 * w + 1
+            al.append((o + 1, o, o + 2, 2))
+        apar = [(55.0, 104.52)]
+    else:
+        for w in range(nwat):
+            o = nsol + sa * w + 1
+            al += [(o + i, o + j, o + k, 2 + c) for i, j, k, c in mol["angles"]]
+        apar = mol["angle_par"]
     L.append("%d %d = No. of angles, no. of solute angles. i - j - k - icode:" % (len(al), len(angles)))
     L += _rows([v for t in al for v in t], 12, str)
-    L.append("2 = No. of angle codes. Parameters:")
+    L.append("%d = No. of angle codes. Parameters:" % (1 + len(apar)))
     L.append("1 %s %s" % (_g(md["angle_k"]), _g(np.degrees(md["angle_t0"]))))
-    L.append("2 %s %s" % (_g(55.0), _g(104.52)))
-    L.append("0 0 = No. of torsions, solute torsions.")
-    L.append("0 = No. of torsion codes. Parameters:")
+    L += ["%d %s %s" % (2 + c, _g(k), _g(t0)) for c, (k, t0) in enumerate(apar)]
+    tl, tpar = [], []
+    if mol is not None:
+        for w in range(nwat):
+            o = nsol + sa * w + 1
+            tl += [(o + i, o + j, o + k, o + l, 1 + c) for i, j, k, l, c in mol["torsions"]]
+        tpar = mol["tor_par"]
+    L.append("%d 0 = No. of torsions, solute torsions." % len(tl))
+    L += _rows([v for t in tl for v in t], 10, str)
+    L.append("%d = No. of torsion codes. Parameters:" % len(tpar))
+    L += ["%d %s %s %s %d" % (1 + c, _g(fk), _g(rm), _g(dt), int(pth)) for c, (fk, rm, dt, pth) in enumerate(tpar)]
     L.append("0 0 = No. of impropers, solute impr.")
     L.append("0 1 = No. of improper codes, type (1=harmonic,2=periodic). Parameters:")
     L.append("%d = No. of atomic charges" % nat)
@@ -112,11 +135,11 @@ def write_top(s, path, title="synthetic"):
     pex = np.asarray(a["listexlong"]).ravel()[:2 * nexlong].reshape(-1, 2)
     L += ["%d %d" % (i + 1, j + 1) for i, j in pex]
     # residues and molecules: one per chain (charge group) and one per water
-    starts = [int(cgpatom[first[g]]) + 1 for g in range(ncs)] + [nsol + 3 * w + 1 for w in range(nwat)]
+    starts = [int(cgpatom[first[g]]) + 1 for g in range(ncs)] + [nsol + sa * w + 1 for w in range(nwat)]
     L.append("%d %d = No. of residues, No of solute residues. start atoms:" % (len(starts), ncs))
     L += _rows(starts, 13, str)
     L.append("Sequence:")
-    names = ["CHN"] * ncs + ["HOH"] * nwat
+    names = ["CHN"] * ncs + [("HOH" if mol is None else mol["name"])] * nwat
     L += [" ".join("%-4s" % v for v in names[i:i + 16]) for i in range(0, len(names), 16)]
     L.append("%d = No. of separate molecules. start atoms:" % len(starts))
     L += _rows(starts, 13, str)
@@ -125,9 +148,10 @@ def write_top(s, path, title="synthetic"):
     L += [" ".join("%-8s" % v for v in tn[i:i + 8]) for i in range(0, ntypes, 8)]
     L.append("%d = No. of SYBYL atom types:" % ntypes)
     L += [" ".join("%-5s" % v for v in ["O.3", "H"] + ["C.3"] * (ntypes - 2))]
-    L.append("0 = solvent type (0=SPC,1=3-atom,2=general)")
+    L.append("%d = solvent type (0=SPC,1=3-atom,2=general)" % (0 if mol is None else 1))
     p = s.par
-    L.append("%s %s %s = Exclusion, solvent radii, Solvent density" % (_g(p["rexcl_o"]), _g(p["rwat"]), _g(0.0334)))
+    L.append("%s %s %s = Exclusion, solvent radii, Solvent density"
+             % (_g(p["rexcl_o"]), _g(p["rwat"]), _g(p.get("solvent_density", 0.0334))))
     L.append("%s %s %s = Solute centre" % tuple(_g(v) for v in p["xpcent"]))
     L.append("%s %s %s = Solvent centre" % tuple(_g(v) for v in p["xwcent"]))
     excl = np.asarray(a["excl"])[:nat]
@@ -172,7 +196,8 @@ def write_inp(s, path, top, fep, steps, lambdas=None, dt_fs=1.0, T=300.0, tau_T=
          "[cut-offs]", "solute_solute %s" % _g(p["Rcpp"]), "solvent_solvent %s" % _g(p["Rcww"]),
          "solute_solvent %s" % _g(p["Rcpw"]), "q_atom %s" % _g(p["Rcq"]), "lrf %s" % _g(p["RcLRF"]), "",
          "[sphere]", "shell_force 10.0", "",
-         "[solvent]", "radial_force %s" % _g(p["fk_wsphere"]), "polarisation on",
+         "[solvent]", "radial_force %s" % _g(p["fk_wsphere"]),
+         "polarisation %s" % ("on" if p.get("wpol_restr", 1) else "off"),
          "polarisation_force %s" % _g(p["fkwpol"]), "pol_update 100", "",
          "[intervals]", "non_bond %d" % nbcycle, "output %d" % max(steps, 1)]
     if energy_every:

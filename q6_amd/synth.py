@@ -631,6 +631,214 @@ def set_q_params(s, qcrg=None, qA=None, qB=None):
     return s.finalize()
 
 
+def _zmat(rows):
+    """Cartesian positions from a z-matrix: rows (ref_bond, r, ref_angle,
+    theta_deg, ref_dihedral, phi_deg), None for the first atoms' missing
+    references."""
+    pos = []
+    for k, (a, r, b, th, c, ph) in enumerate(rows):
+        if k == 0:
+            pos.append(np.zeros(3))
+            continue
+        if k == 1:
+            pos.append(pos[a] + np.array([r, 0.0, 0.0]))
+            continue
+        pa, pb = pos[a], pos[b]
+        pc = pos[c] if c is not None else pb + np.array([0.0, 1.0, 0.0])
+        bc = pa - pb
+        bc /= np.linalg.norm(bc)
+        n = np.cross(pb - pc, bc)
+        if np.linalg.norm(n) < 1e-8:
+            n = np.cross(bc, np.array([0.0, 0.0, 1.0]))
+        n /= np.linalg.norm(n)
+        m = np.cross(n, bc)
+        t, f = math.radians(th), math.radians(ph)
+        d = np.array([-r * math.cos(t), r * math.sin(t) * math.cos(f), r * math.sin(t) * math.sin(f)])
+        pos.append(pa + d[0] * bc + d[1] * m + d[2] * n)
+    return np.array(pos)
+
+
+def ethanol():
+    """A nine-site flexible solvent molecule (OPLS-like ethanol): C2 first
+    (the group's switch atom), then C1, O, HO, three H on C1, two on C2.
+    Bonds, angles and the twelve torsions make three kinds of internal
+    nonbonded pairs for nonbond_solvent_internal: 1-4 pairs through the
+    torsions (LJ code 3, scaled Coulomb), and the three H(C1)-HO 1-5 pairs,
+    which Qdyn6's list holds in both orders."""
+    geo = _zmat([(None, 0, None, 0, None, 0),          # C2
+                 (0, 1.529, None, 0, None, 0),          # C1
+                 (0, 1.410, 1, 109.5, None, 0),         # O
+                 (2, 0.945, 0, 108.5, 1, 180.0),        # HO
+                 (1, 1.090, 0, 110.7, 2, 60.0),         # H11
+                 (1, 1.090, 0, 110.7, 2, 180.0),        # H12
+                 (1, 1.090, 0, 110.7, 2, 300.0),        # H13
+                 (0, 1.090, 1, 110.7, 2, 120.0),        # H21
+                 (0, 1.090, 1, 110.7, 2, 240.0)])       # H22
+    # types: name, sigma, eps, mass (1-4 LJ at half the normal factors)
+    types = [("CT2", 3.5, 0.066, 12.011), ("CT1", 3.5, 0.066, 12.011), ("OHE", 3.12, 0.17, 15.999),
+             ("HOE", 0.0, 0.0, 1.008), ("HCE", 2.5, 0.03, 1.008)]
+    atom_type = [0, 1, 2, 3, 4, 4, 4, 4, 4]
+    charge = [0.145, -0.18, -0.683, 0.418, 0.06, 0.06, 0.06, 0.03, 0.03]
+    bonds = [(0, 1, 0), (0, 2, 1), (2, 3, 2), (1, 4, 3), (1, 5, 3), (1, 6, 3), (0, 7, 3), (0, 8, 3)]
+    bond_par = [(536.0, 1.529), (640.0, 1.410), (1106.0, 0.945), (680.0, 1.090)]
+    angles = [(1, 0, 2, 0), (1, 0, 7, 1), (1, 0, 8, 1), (2, 0, 7, 1), (2, 0, 8, 1), (7, 0, 8, 2),
+              (0, 1, 4, 1), (0, 1, 5, 1), (0, 1, 6, 1), (4, 1, 5, 2), (4, 1, 6, 2), (5, 1, 6, 2), (0, 2, 3, 3)]
+    angle_par = [(100.0, 109.5), (75.0, 110.7), (66.0, 107.8), (110.0, 108.5)]
+    tors = [(i, 0, 1, l, 0) for i in (2, 7, 8) for l in (4, 5, 6)] + [(i, 0, 2, 3, 1) for i in (1, 7, 8)]
+    tor_par = [(0.3, 3.0, 0.0, 1.0), (0.45, 3.0, 0.0, 1.0)]
+    return dict(name="EOH", geo=geo, types=types, atom_type=atom_type, charge=charge, bonds=bonds,
+                bond_par=bond_par, angles=angles, angle_par=angle_par, torsions=tors, tor_par=tor_par,
+                atom_names=["C2", "C1", "O", "HO", "H11", "H12", "H13", "H21", "H22"])
+
+
+def solvent_internal_pairs(mol, avdw_solv, bvdw_solv, chg_solv, ljcode, el14, rule=1):
+    """Nonbonded pairs inside one solvent molecule as Qdyn6 lists them
+    (prep_sim_precompute_solvent, src/simprep.f90:4004-4138): every ordered
+    pair (i, j), i != j, starts with its LJ code; bonded and angle-end pairs
+    are dropped both ways; a torsion's end pair (i, l) becomes a 1-4 pair
+    (code 3) in that order and (l, i) is dropped; the rest stay in both
+    orders.  Returns (ij [n, 2] 0-based, par [n, 3] A, B, charge product)."""
+    sa = len(mol["atom_type"])
+    inter = np.array([[ljcode[i][j] for j in range(sa)] for i in range(sa)], int)
+    np.fill_diagonal(inter, 0)
+    for i, j, _ in mol["bonds"]:
+        inter[i, j] = inter[j, i] = 0
+    for i, _, k, _ in mol["angles"]:
+        inter[i, k] = inter[k, i] = 0
+    for i, _, _, l, _ in mol["torsions"]:
+        if inter[i, l] != 0 or inter[l, i] != 0:
+            inter[i, l] = 3
+            inter[l, i] = 0
+    ij, par = [], []
+    for i in range(sa):
+        for j in range(sa):
+            c = inter[i, j]
+            if c == 0:
+                continue
+            if rule == 1:
+                A = avdw_solv[i, c - 1] * avdw_solv[j, c - 1]
+                B = bvdw_solv[i, c - 1] * bvdw_solv[j, c - 1]
+            else:
+                tA = avdw_solv[i, c - 1] + avdw_solv[j, c - 1]
+                tA = tA ** 2
+                tA = tA * tA * tA
+                tB = bvdw_solv[i, c - 1] * bvdw_solv[j, c - 1]
+                A, B = (tA ** 2) * tB, 2.0 * tA * tB
+            el = chg_solv[i] * chg_solv[j]
+            if c == 3:
+                el = el * el14
+            ij.append((i, j))
+            par.append((A, B, el))
+    return np.array(ij, np.int32).reshape(-1, 2), np.array(par, float).reshape(-1, 3)
+
+
+def replace_solvent(s, mol, density, seed, fill=None, dmin_solute=2.6, dmin_solvent=2.2):
+    """The system s (from build) with its water replaced by copies of the
+    solvent molecule `mol` (e.g. ethanol()): centres on a jittered cubic
+    lattice at `density` molecules per A^3 within `fill` (default the sphere
+    radius - 1), random orientations, dropped where a heavy atom comes within
+    dmin_solute of a solute atom or dmin_solvent of a kept molecule's heavy
+    atom.  Topology types, charges, groups, the pw/ww/qw tables, the
+    solvent's internal nonbonded pairs and the MD data follow; the solvent is
+    Qdyn6's SOLVENT_ALLATOM type, without the polarisation restraint."""
+    rng = np.random.default_rng(seed)
+    n, a, md = s.n, s.a, s.md
+    nsol = int(n["nat_solute"])
+    sa = len(mol["atom_type"])
+    radius = float(s.par["rwat"])
+    fill = radius - 1.0 if fill is None else fill
+    spos = s.x[:nsol]
+    geo = mol["geo"] - mol["geo"].mean(0)
+    heavy_m = np.array([mol["types"][t][3] >= 4.0 for t in mol["atom_type"]])
+    lat = density ** (-1.0 / 3.0)
+    m = int(math.ceil(fill / lat)) + 1
+    g = np.arange(-m, m + 1) * lat
+    cen = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    cen = cen + rng.uniform(-0.3, 0.3, cen.shape)
+    cen = cen[np.einsum("ij,ij->i", cen, cen) <= fill * fill]
+    rot = _rand_rot(rng, len(cen))
+    gs, gw = _Grid(3.0), _Grid(3.0)
+    for k, p in enumerate(spos):
+        gs.add(p, k)
+    mols = []
+    for c, R in zip(cen, rot):
+        at = c + geo @ R.T
+        hv = at[heavy_m]
+        if any(gs.near(p, dmin_solute) for p in hv) or any(gw.near(p, dmin_solvent) for p in hv):
+            continue
+        for p in hv:
+            gw.add(p, len(mols))
+        mols.append(at)
+    nw = len(mols)
+    nat = nsol + sa * nw
+    x = np.concatenate([spos] + mols) if nw else spos.copy()
+    # types: the solute's own, then the solvent's (the water types stay, unused)
+    nt0 = int(n["ntypes"])
+    avdw = np.asarray(a["avdw"]).reshape(nt0, 3)
+    bvdw = np.asarray(a["bvdw"]).reshape(nt0, 3)
+    ntypes = nt0 + len(mol["types"])
+    avdw = np.vstack([avdw, np.zeros((len(mol["types"]), 3))])
+    bvdw = np.vstack([bvdw, np.zeros((len(mol["types"]), 3))])
+    mass_t = np.concatenate([md["type_mass"], np.zeros(len(mol["types"]))])
+    for k, (_, sig, eps, mass) in enumerate(mol["types"]):
+        A, B = _lj(sig, eps) if eps > 0 else (0.0, 0.0)
+        avdw[nt0 + k] = (A, A, 0.5 * A)
+        bvdw[nt0 + k] = (B, B, 0.5 * B)
+        mass_t[nt0 + k] = mass
+    iac = np.concatenate([np.asarray(a["iac"])[:nsol], np.tile(nt0 + np.array(mol["atom_type"]), nw)]).astype(np.int32)
+    crg_u = np.concatenate([md["crg_unscaled"][:nsol], np.tile(mol["charge"], nw)])
+    crg = crg_u * SQRT332
+    heavy = (mass_t[iac] >= 4.0).astype(np.uint8)
+    ncs = int(n["ncgp_solute"])
+    first = np.concatenate([np.asarray(a["cgp_first"])[:ncs], nsol + sa * np.arange(nw)]).astype(np.int32)
+    last = np.concatenate([np.asarray(a["cgp_last"])[:ncs], nsol + sa * np.arange(nw) + sa - 1]).astype(np.int32)
+    sw = np.concatenate([np.asarray(a["cgp_iswitch"])[:ncs], nsol + sa * np.arange(nw)]).astype(np.int32)
+    aLJ = avdw[nt0 + np.array(mol["atom_type"])]
+    bLJ = bvdw[nt0 + np.array(mol["atom_type"])]
+    chg_solv = np.array(mol["charge"]) * SQRT332
+    ww_A = np.array([[aLJ[la, 0] * aLJ[ka, 0] for ka in range(sa)] for la in range(sa)])
+    ww_B = np.array([[bLJ[la, 0] * bLJ[ka, 0] for ka in range(sa)] for la in range(sa)])
+    ww_el = np.array([[chg_solv[la] * chg_solv[ka] for ka in range(sa)] for la in range(sa)])
+    nsol1 = max(nsol, 1)
+    iqatom = np.concatenate([np.asarray(a["iqatom"])[:nsol], np.zeros(sa * nw, np.int32)]).astype(np.int32)
+    pw_A, pw_B, pw_el = (np.zeros((nsol1, sa)) for _ in range(3))
+    for i in range(nsol):
+        if iqatom[i]:
+            continue
+        for j in range(sa):
+            pw_A[i, j] = avdw[iac[i], 0] * aLJ[j, 0]
+            pw_B[i, j] = bvdw[iac[i], 0] * bLJ[j, 0]
+            pw_el[i, j] = crg[i] * chg_solv[j]
+    ctx = dict(md["qctx"], aLJ_solv=aLJ, bLJ_solv=bLJ, chg_solv=chg_solv, iac=iac, crg=crg, iqatom=iqatom,
+               avdw=avdw, bvdw=bvdw)
+    ij, par = solvent_internal_pairs(mol, aLJ, bLJ, chg_solv, [[1] * sa for _ in range(sa)], s.el14_scale)
+    a.update(crg=crg, iac=iac, iqatom=iqatom, excl=np.zeros(nat, np.uint8), heavy=heavy, cgp_first=first,
+             cgp_last=last, cgp_iswitch=sw, cgpatom=np.arange(nat, dtype=np.int32), avdw=avdw.ravel(),
+             bvdw=bvdw.ravel(), ljcod=np.ones(ntypes * ntypes, np.int32), pw_A=pw_A.ravel(order="F"),
+             pw_B=pw_B.ravel(order="F"), pw_el=pw_el.ravel(order="F"), ww_A=ww_A.ravel(order="F"),
+             ww_B=ww_B.ravel(order="F"), ww_el=ww_el.ravel(order="F"), solv_int_ij=ij.ravel(),
+             solv_int_par=par.ravel())
+    a.update(q_tables(ctx, md["qcrg"], md["qA"], md["qB"]))
+    n.update(natom=nat, nwat=nw, solv_atom=sa, ncgp=ncs + nw, ntypes=ntypes, solvent_spc=0, n_solv_int=len(ij))
+    s.x = x
+    s.par.update(wpol_restr=0, nwpolr_shell=0, wshell_rout=np.zeros(0), wshell_dr=np.zeros(0),
+                 wshell_cstb=np.zeros(0), wshell_theta_corr=np.zeros(0), solvent_density=density)
+    md.update(mass=mass_t[iac], type_mass=mass_t, crg_unscaled=crg_u, qctx=ctx, solvent=mol,
+              type_names=list(md["type_names"]) + [t[0] for t in mol["types"]])
+    s.name = s.name + " in " + mol["name"]
+    return s.finalize()
+
+
+def cfg_ethanol(seed=41):
+    """18 A sphere of a flexible nine-site solvent (ethanol()) around six
+    10-atom bead chains, the innermost with 10 Q-atoms in 2 states: the
+    general-solvent loops (nonbond_ww, nonbond_qw, nonbond_pw over nine
+    sites) and the solvent's internal pairs."""
+    s = build(18.0, 6, 10, 1, 2, seed, chain_region=6.0, q_sigma=0.4, lambdas=[0.6, 0.4], compact=3.0,
+              water_fill=17.0, name="18 A sphere, 6x10-atom chains, 10 Q-atoms", **PACK)
+    return replace_solvent(s, ethanol(), 0.0085, seed + 7)
+
+
 def maxwell_velocities(s, T, seed):
     """Seeded Maxwell-Boltzmann velocities in Q units (A per internal time unit,
     dt_internal = 0.020462 * fs, src/md.f90:207), net momentum removed."""

@@ -33,6 +33,7 @@ class QSystem:
         "qp_A": _F, "qp_B": _F, "qp_el": _F, "qp_sc": _F, "qp_set": _U8,
         "qq_A": _F, "qq_B": _F, "qq_el": _F, "qq_sc": _F, "qq_set": _U8, "qq_soft": _U8,
         "qw_A": _F, "qw_B": _F, "qw_el": _F, "qw_sc": _F,
+        "solv_int_ij": _INT, "solv_int_par": _F,
     }
     SIZES = ("natom", "nat_solute", "nwat", "solv_atom", "ncgp", "ncgp_solute", "nqat", "nstates",
              "ntypes", "ivdw_rule", "solvent_spc")
@@ -68,6 +69,7 @@ class QSystem:
         s.nexlong = int(self.n.get("nexlong", 0))
         s.n14long = int(self.n.get("n14long", 0))
         s.any_atom_cutoff = int(self.n.get("any_atom_cutoff", 0))
+        s.n_solv_int = int(self.n.get("n_solv_int", 0))
         s.el14_scale = float(self.el14_scale)
         for k, dt in self.ARRAYS.items():
             ct = {np.float64: C.c_double, np.int32: C.c_int32, np.uint8: C.c_uint8}[np.dtype(dt).type]
@@ -163,6 +165,10 @@ def from_golden(path):
                   "qq_soft", "qw_A", "qw_B", "qw_el", "qw_sc"):
             if k in g.files:
                 a[k] = g[k].ravel(order="F")
+    if "solv_int_ij" in g.files:  # nonbnd_solv_int of a topology with solvent torsions
+        a["solv_int_ij"] = g["solv_int_ij"].ravel(order="F") - 1
+        a["solv_int_par"] = g["solv_int_par"].ravel(order="F")
+        s.n["n_solv_int"] = g["solv_int_ij"].shape[1]
     s.x = g["x"].T.copy()
     s.v = g["v"].T.copy()
     s.winv = g["winv"].copy()

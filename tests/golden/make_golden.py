@@ -455,6 +455,23 @@ def run_cjp():
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def run_synth(name, system):
+    """A synthetic system written as Q files (q6_amd.qfiles) and set up by
+    the dump driver (no MD step)."""
+    from q6_amd import qfiles
+    tmp = tempfile.mkdtemp(prefix="qnbgold_")
+    try:
+        inp = qfiles.write_case(system, tmp, 10, name=name, seed=0)
+        log = subprocess.run([DUMP, "out.bin", "nomd", os.path.basename(inp)], cwd=tmp, capture_output=True,
+                             text=True)
+        if log.returncode != 0 or not os.path.exists(os.path.join(tmp, "out.bin")):
+            print(log.stdout[-3000:], log.stderr[-3000:])
+            raise RuntimeError("qnb_dump failed for " + name)
+        return read_dump(os.path.join(tmp, "out.bin"))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     if not os.path.exists(DUMP):
         sys.exit("build the reference first: make -C oracle/ref")
@@ -484,6 +501,17 @@ def main():
         np.savez_compressed(path, **arrs)
         print("cjp_excl ->", path, os.path.getsize(path) // 1024, "KiB;", "pp", d["nbpp_ij"].shape[1], "pw",
               d["nbpw_ij"].shape[1], "qp", d["nbqp_ij"].shape[1], "arrays", len(d["exc_type"]) + 1)
+    if not only or "eoh" in only:
+        # a nine-site flexible solvent (q6_amd.synth.cfg_ethanol): the general
+        # solvent loops and nonbond_solvent_internal
+        sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+        from q6_amd import synth
+        d = run_synth("eoh", synth.cfg_ethanol())
+        arrs = compact(d, fill=False)
+        path = os.path.join(HERE, "eoh.npz")
+        np.savez_compressed(path, **arrs)
+        print("eoh ->", path, os.path.getsize(path) // 1024, "KiB;", "pw", d["nbpw_ij"].shape[1], "ww",
+              d["nbww_ij"].shape[1], "qw", d["nbqw_ij"].shape[1], "solvent internal pairs", d["solv_int_ij"].shape[1])
     for name, (inp, fep) in cases.items():
         if only and name not in only:
             continue

@@ -15,12 +15,14 @@ from q6_amd.system import from_golden
 
 TERMS = [("ww", abi.T_WW), ("qw", abi.T_QW), ("pw", abi.T_PW), ("pp", abi.T_PP), ("qp", abi.T_QP),
          ("lrf", abi.T_LRF), ("qq", abi.T_QQ), ("qqp", abi.T_QQP), ("solvent_radial", abi.T_RAD),
-         ("water_pol", abi.T_POL)]
+         ("water_pol", abi.T_POL), ("solv_int", abi.T_SINT)]
 
 
 def term_energy(t, E, EQ):
     if t in ("ww", "pw", "pp"):
         return [E[t + "_el"], E[t + "_vdw"]]
+    if t == "solv_int":  # nonbond_solvent_internal adds into the solvent-solvent energies
+        return [E["ww_el"], E["ww_vdw"]]
     if t in ("qw", "qp", "qq"):
         return sum([[q[t + "_el"], q[t + "_vdw"]] for q in EQ], [])
     if t == "qqp":
