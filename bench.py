@@ -113,7 +113,7 @@ def run_md(systems, lambdas, vels, precision, steps, warmup, graph=True, positio
     b = BatchMD(systems, lambdas, vels, precision=precision, nbcycle=10, ene_every=10,
                 max_frames=(warmup + steps + extra) // 10 + 2, positions=positions)
     if tune:
-        b.shapes = b.autotune()
+        b.shapes = b.autotune(reps=10)
     if graph:
         b.prepare_graph()
     b.run(warmup, graph=graph)

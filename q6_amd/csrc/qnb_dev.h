@@ -747,7 +747,7 @@ inline int qforces_blocks(int nq, int chunks, int W) { return 8 * ((chunks + W -
 #endif
 template <typename R>
 __host__ __device__ constexpr size_t fc_wave_bytes() {
-  return ((sizeof(R) * (3 * (FC_CAP + FC_CH) + 9 * FC_CH) + sizeof(double) * FC_CH +
+  return ((sizeof(R) * (3 * (2 * FC_CAP + FC_CH) + 9 * FC_CH) + sizeof(double) * FC_CH +
            sizeof(int) * (2 * FC_CAP + 4 * FC_CH)) + 15) / 16 * 16;
 }
 // type tables of the solute-solute parameters held in LDS up to this many types
@@ -756,10 +756,11 @@ __host__ __device__ constexpr size_t fc_wave_bytes() {
 // of the launch fits, else FC_CAP); partner waters need no staging.
 #define WF_CAP_SMALL 128
 // per-wave LDS of the solute-receiver kernel (k_sforces) with CAP partner
-// slots: partner offsets, indices and tags, then the sweep's receiver data
+// slots: partner offsets, the sweep's receiver data, the partners' LJ and
+// charge factors, then partner indices and tags and receiver records
 template <typename R, int CAP>
 __host__ __device__ constexpr size_t sf_wave_bytes() {
-  return ((sizeof(R) * (3 * (CAP + FC_CH) + 9 * FC_CH) + sizeof(double) * FC_CH +
+  return ((sizeof(R) * (3 * (2 * CAP + FC_CH) + 9 * FC_CH) + sizeof(double) * FC_CH +
            sizeof(int) * (2 * CAP + 4 * FC_CH)) + 15) / 16 * 16;
 }
 template <typename R, int CAP>

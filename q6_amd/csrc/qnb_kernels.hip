@@ -594,10 +594,11 @@ __device__ __forceinline__ bool pp_param_fast(const DevSys& S, const PPAtom& P, 
   return set;
 }
 
-template <typename R, int CAP = FC_CAP>
+template <typename R, int CAP = FC_CAP, bool PPF = false>
 __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int* __restrict__ row, int p0, int nn,
                                               uint32_t use_sol, uint32_t use_wat, double xr0, double xr1,
-                                              double xr2, int* s_j, R (*s_d)[3], int* s_t, int* ngroups) {
+                                              double xr2, int* s_j, R (*s_d)[3], int* s_t, int* ngroups,
+                                              R (*s_f)[3] = nullptr) {
   const int lane = threadIdx.x % QNB_WAVE;
   const int p = p0 + lane;
   int b = -1, cnt = 0, f0 = 0;
@@ -658,6 +659,7 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
       tg[u] = in ? s_t[k] : 0;
       jj[u] = ldg(S.cgpatom + (in ? s_j[k] : pos_pad));
     }
+    R pf[FC_SPR][3];
 #pragma unroll
     for (int u = 0; u < FC_SPR; u++) {
       const int j = jj[u];
@@ -665,6 +667,15 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
       px[u][1] = ldg(S.x + 3 * j + 1);
       px[u][2] = ldg(S.x + 3 * j + 2);
       qa[u] = ldg(S.iqatom + j);
+      if constexpr (PPF) {
+        // the solute partner's LJ factors and charge (ppf / ppd rows); the
+        // row of the receiver's first atom stands in for a solvent atom
+        const int jp = j < S.nat_solute ? j : ldg(S.cgpatom + pos_pad);
+        const R* t = qp_tab<R>(S.ppd, S.ppf) + 4 * (size_t)jp;
+        pf[u][0] = ldg(t);
+        pf[u][1] = ldg(t + 1);
+        pf[u][2] = ldg(t + 2);
+      }
     }
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -676,6 +687,11 @@ __device__ __forceinline__ int stage_partners(const DevSys& S, int a, const int*
         s_d[k][1] = (R)(px[u][1] - xr1);
         s_d[k][2] = (R)(px[u][2] - xr2);
         s_t[k] = tg[u] | (((tg[u] & FT_SOL) && qa[u] != 0) ? FT_Q : 0);
+        if constexpr (PPF) {
+          s_f[k][0] = pf[u][0];
+          s_f[k][1] = pf[u][1];
+          s_f[k][2] = pf[u][2];
+        }
       }
     }
   }
@@ -731,7 +747,8 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_sforces(const DevSy
   R(*s_pd)[3] = reinterpret_cast<R(*)[3]>(wbase);
   R(*s_o)[3] = reinterpret_cast<R(*)[3]>(wbase + sizeof(R) * 3 * CAP);
   R(*s_pw)[3][3] = reinterpret_cast<R(*)[3][3]>(wbase + sizeof(R) * 3 * (CAP + FC_CH));
-  double* s_rq = reinterpret_cast<double*>(wbase + sizeof(R) * (3 * (CAP + FC_CH) + 9 * FC_CH));
+  R(*s_pf)[3] = reinterpret_cast<R(*)[3]>(wbase + sizeof(R) * (3 * (CAP + FC_CH) + 9 * FC_CH));
+  double* s_rq = reinterpret_cast<double*>(wbase + sizeof(R) * (3 * (2 * CAP + FC_CH) + 9 * FC_CH));
   int* s_pj = reinterpret_cast<int*>(s_rq + FC_CH);
   int* s_pt = s_pj + CAP;
   int* s_ia = s_pt + CAP;
@@ -816,7 +833,8 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_sforces(const DevSy
       // ---- partner solute groups, one staged atom per lane
       if (mask & T_PP) {
         for (int p0 = 0, ng = 0; p0 < nsr; p0 += ng) {
-          const int total = stage_partners<R, CAP>(S, a, row, p0, nsr, 1u, 0u, xr0, xr1, xr2, s_pj, s_pd, s_pt, &ng);
+          const int total = stage_partners<R, CAP, true>(S, a, row, p0, nsr, 1u, 0u, xr0, xr1, xr2, s_pj, s_pd, s_pt,
+                                                         &ng, s_pf);
           if (ng == 0) break;
           for (int t = lane; t < total; t += QNB_WAVE) {
             const int tag = s_pt[t], j = s_pj[t];
@@ -826,16 +844,8 @@ __global__ __launch_bounds__(64 * FC_WPB) FC_OCC_ATTR void k_sforces(const DevSy
             R pel = R(0), pvdw = R(0);
             if (fast && (tag & FT_FAR)) {
               // no exclusion or 1-4 pair possible: A, B, qq from the factors
-              R pA, pB, pQ;
-              if constexpr (sizeof(R) == 4) {
-                pA = ldg(S.ppf + 4 * j);
-                pB = ldg(S.ppf + 4 * j + 1);
-                pQ = ldg(S.ppf + 4 * j + 2);
-              } else {
-                pA = ldg(S.ppd + 4 * j);
-                pB = ldg(S.ppd + 4 * j + 1);
-                pQ = ldg(S.ppd + 4 * j + 2);
-              }
+              // (staged with the partner)
+              const R pA = s_pf[t][0], pB = s_pf[t][1], pQ = s_pf[t][2];
 #pragma unroll
               for (int k = 0; k < CH; k++) {
                 if (!rok[k]) continue;
