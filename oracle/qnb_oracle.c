@@ -840,31 +840,57 @@ static void restrain_solvent(orc_handle *h, const double *x, double *d, double T
   }
 }
 
-/* dipole direction and radial direction of a 3-site water (watpol, the
- * solv_atom == 3 branch, nonbondene.f90:6579-6604) */
+/* dipole direction and radial direction of a solvent molecule whose first
+ * atom is i (watpol, nonbondene.f90:6572-6604): a 3-site solvent takes
+ * H1 + H2 - 2 O and the radial vector of O; any other takes the
+ * charge-weighted sum of its positions (the charges of the first solvent
+ * molecule, chg_solv) and the radial vector of its centroid. */
 static void wat_vectors(const orc_handle *h, const double *x, int i, vec3 *rmu, double *rm, vec3 *rcu, double *rc) {
   const qnb_params *p = &h->p;
-  vec3 m = {0.0, 0.0, 0.0};
-  for (int k = 1; k < 3; k++) {
-    m.x = m.x + x[3 * (i + k)];
-    m.y = m.y + x[3 * (i + k) + 1];
-    m.z = m.z + x[3 * (i + k) + 2];
+  const int sa = h->s.solv_atom;
+  vec3 m = {0.0, 0.0, 0.0}, c;
+  if (sa == 3) {
+    for (int k = 1; k < 3; k++) {
+      m.x = m.x + x[3 * (i + k)];
+      m.y = m.y + x[3 * (i + k) + 1];
+      m.z = m.z + x[3 * (i + k) + 2];
+    }
+    m.x = m.x - x[3 * i] * 2.0;
+    m.y = m.y - x[3 * i + 1] * 2.0;
+    m.z = m.z - x[3 * i + 2] * 2.0;
+    c.x = x[3 * i] - p->xwcent[0];
+    c.y = x[3 * i + 1] - p->xwcent[1];
+    c.z = x[3 * i + 2] - p->xwcent[2];
+  } else {
+    vec3 g = {0.0, 0.0, 0.0};
+    const double *chg = h->s.crg + h->s.nat_solute;
+    for (int k = 0; k < sa; k++) {
+      g.x = g.x + x[3 * (i + k)];
+      g.y = g.y + x[3 * (i + k) + 1];
+      g.z = g.z + x[3 * (i + k) + 2];
+      m.x = m.x + x[3 * (i + k)] * chg[k];
+      m.y = m.y + x[3 * (i + k) + 1] * chg[k];
+      m.z = m.z + x[3 * (i + k) + 2] * chg[k];
+    }
+    g.x = g.x / (double)sa;
+    g.y = g.y / (double)sa;
+    g.z = g.z / (double)sa;
+    c.x = g.x - p->xwcent[0];
+    c.y = g.y - p->xwcent[1];
+    c.z = g.z - p->xwcent[2];
   }
-  m.x = m.x - x[3 * i] * 2.0;
-  m.y = m.y - x[3 * i + 1] * 2.0;
-  m.z = m.z - x[3 * i + 2] * 2.0;
   *rm = sqrt(sq(m));
   rmu->x = m.x / *rm;
   rmu->y = m.y / *rm;
   rmu->z = m.z / *rm;
-  vec3 c = {x[3 * i] - p->xwcent[0], x[3 * i + 1] - p->xwcent[1], x[3 * i + 2] - p->xwcent[2]};
   *rc = sqrt(sq(c));
   rcu->x = c.x / *rc;
   rcu->y = c.y / *rc;
   rcu->z = c.z / *rc;
 }
 
-/* watpol (nonbondene.f90:6548-6746), 3-site solvent, md = .true. */
+/* watpol (nonbondene.f90:6548-6746), md = .true.; the force on a molecule's
+ * first atom is (f1 + f2) f0 and on each other atom f3 f0, for any solvent */
 static void watpol(orc_handle *h, const double *x, double *d, int istep, double *E) {
   const qnb_system *s = &h->s;
   const qnb_params *p = &h->p;
@@ -952,7 +978,7 @@ static void watpol(orc_handle *h, const double *x, double *d, int istep, double 
       d[3 * i] = d[3 * i] + (f1.x + f2.x) * f0;
       d[3 * i + 1] = d[3 * i + 1] + (f1.y + f2.y) * f0;
       d[3 * i + 2] = d[3 * i + 2] + (f1.z + f2.z) * f0;
-      for (int k = 1; k < 3; k++) {
+      for (int k = 1; k < s->solv_atom; k++) {
         d[3 * (i + k)] = d[3 * (i + k)] + f3.x * f0;
         d[3 * (i + k) + 1] = d[3 * (i + k) + 1] + f3.y * f0;
         d[3 * (i + k) + 2] = d[3 * (i + k) + 2] + f3.z * f0;
@@ -984,11 +1010,9 @@ int orc_eval_terms(orc_handle *h, uint32_t mask, const double *x, double *d, dou
   if (mask & QNB_T_PP) nonbond_list(&h->pp, x, d, &E->pp_el, &E->pp_vdw);
   if (mask & QNB_T_QP) nonbond_qp(h, x, d, EQ);
   if ((mask & QNB_T_LRF) && h->p.use_LRF) lrf_taylor(h, x, d, &E->lrf);
-  if (s->nwat > 0 && s->solv_atom == 3) {
+  if (s->nwat > 0) {
     if (mask & QNB_T_RAD) restrain_solvent(h, x, d, Tfree, &E->solvent_radial);
     if ((mask & QNB_T_POL) && h->p.wpol_restr) watpol(h, x, d, istep, &E->water_pol);
-  } else if (s->nwat > 0 && (mask & QNB_T_RAD)) {
-    restrain_solvent(h, x, d, Tfree, &E->solvent_radial);
   }
   if (mask & QNB_T_QQ) nonbond_qq(h, x, d, EQ);
   if (mask & QNB_T_QQP) nonbond_qqp(h, x, d, EQ);

@@ -69,8 +69,6 @@ int validate_system(const qnb_system* s, const qnb_params* p) {
   if (s->nwat > 0 && (s->solv_atom < 1 || s->solv_atom > QNB_WAVE))
     return fail(QNB_EUNSUPPORTED, "solvent molecules of 1..%d atoms are supported (solv_atom=%d)", QNB_WAVE,
                 s->solv_atom);
-  if (s->nwat > 0 && s->solv_atom != 3 && p->wpol_restr)
-    return fail(QNB_EUNSUPPORTED, "the polarisation restraint needs 3-site water (solv_atom=%d)", s->solv_atom);
   if (s->n_solv_int < 0 || (s->n_solv_int > 0 && (!s->solv_int_ij || !s->solv_int_par)))
     return fail(QNB_EINVAL, "solvent internal pairs: count and arrays");
   for (int k = 0; k < s->n_solv_int; k++)

@@ -1687,19 +1687,37 @@ __global__ __launch_bounds__(QQ_THREADS) void k_qq(const DevSys* __restrict__ sy
 }
 
 // ---------------------------------------------------------------------------
-// SCAAS polarisation restraint (watpol, nonbondene.f90:6548-6746, 3-site
-// solvent, md = .true.): one block per system.  The reference's per-shell
+// SCAAS polarisation restraint (watpol, nonbondene.f90:6548-6746,
+// md = .true.): one block per system.  The reference's per-shell
 // selection sort by theta is a stable ascending sort; here each member's rank
 // is counted directly (ties broken by water index, as the selection sort does).
+// Dipole and radial directions of the molecule whose first atom is i
+// (nonbondene.f90:6572-6604): a 3-site solvent takes H1 + H2 - 2 O and the
+// radial vector of O; any other the charge-weighted sum of its positions
+// (charges of the first solvent molecule) and its centroid's radial vector.
 __device__ __forceinline__ void wat_vectors(const DevSys& S, int i, double* rmu, double& rm, double* rcu,
                                             double& rc) {
   const double* x = S.x;
-  double m[3];
-  for (int c = 0; c < 3; c++) m[c] = (x[3 * (i + 1) + c] + x[3 * (i + 2) + c]) - x[3 * i + c] * 2.0;
+  const int sa = S.sa;
+  double m[3], v[3];
+  if (sa == 3) {
+    for (int c = 0; c < 3; c++) m[c] = (x[3 * (i + 1) + c] + x[3 * (i + 2) + c]) - x[3 * i + c] * 2.0;
+    for (int c = 0; c < 3; c++) v[c] = x[3 * i + c] - S.xw[c];
+  } else {
+    const double* chg = S.crg + S.nat_solute;
+    double g[3] = {0.0, 0.0, 0.0};
+    m[0] = m[1] = m[2] = 0.0;
+    for (int k = 0; k < sa; k++) {
+      const double q = ldg(chg + k);
+      for (int c = 0; c < 3; c++) {
+        g[c] = g[c] + x[3 * (i + k) + c];
+        m[c] = m[c] + x[3 * (i + k) + c] * q;
+      }
+    }
+    for (int c = 0; c < 3; c++) v[c] = g[c] / (double)sa - S.xw[c];
+  }
   rm = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
   for (int c = 0; c < 3; c++) rmu[c] = m[c] / rm;
-  double v[3];
-  for (int c = 0; c < 3; c++) v[c] = x[3 * i + c] - S.xw[c];
   rc = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
   for (int c = 0; c < 3; c++) rcu[c] = v[c] / rc;
 }
@@ -1887,8 +1905,7 @@ __global__ __launch_bounds__(WP_THREADS) void k_watpol_force(const DevSys* __res
       const double av = rcu[c] - rmu[c] * scp;
       const double f1 = (av * (-2.0)) / rm, f3 = av / rm, f2 = (rmu[c] - rcu[c] * scp) / rc;
       d[3 * i + c] += (f1 + f2) * f0;
-      d[3 * (i + 1) + c] += f3 * f0;
-      d[3 * (i + 2) + c] += f3 * f0;
+      for (int k = 1; k < S.sa; k++) d[3 * (i + k) + c] += f3 * f0;
     }
   }
   // block sums: [0] energy, [1 + is] theta over the block's members of shell is

@@ -9,7 +9,7 @@ if ROOT not in sys.path:
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 CASES = ["lig_eq1", "lig_fep2", "lig_fep2_nolrf", "lig_fep2_arith", "lig_fep2_anyatom", "lig_fep2_anyatom_nolrf",
-         "cjp_excl", "eoh"]
+         "cjp_excl", "eoh", "eoh_pol"]
 
 
 def pytest_configure(config):

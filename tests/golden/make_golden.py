@@ -512,6 +512,19 @@ def main():
         np.savez_compressed(path, **arrs)
         print("eoh ->", path, os.path.getsize(path) // 1024, "KiB;", "pw", d["nbpw_ij"].shape[1], "ww",
               d["nbww_ij"].shape[1], "qw", d["nbqw_ij"].shape[1], "solvent internal pairs", d["solv_int_ij"].shape[1])
+    if not only or "eoh_pol" in only:
+        # the same solvent with the polarisation restraint: watpol's
+        # charge-weighted dipole about the molecule's centroid
+        sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+        from q6_amd import synth
+        s = synth.cfg_ethanol()
+        s.par["wpol_restr"] = 1
+        d = run_synth("eoh_pol", s)
+        arrs = compact(d, fill=False)
+        path = os.path.join(HERE, "eoh_pol.npz")
+        np.savez_compressed(path, **arrs)
+        print("eoh_pol ->", path, os.path.getsize(path) // 1024, "KiB;", "shells", d["nwpolr_shell"],
+              "in shells", d["wshell_n_insh"])
     for name, (inp, fep) in cases.items():
         if only and name not in only:
             continue

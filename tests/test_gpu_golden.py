@@ -40,7 +40,7 @@ def term_energy(t, E, EQ):
 
 
 @pytest.fixture(scope="module", params=["lig_eq1", "lig_fep2", "lig_fep2_arith", "lig_fep2_anyatom", "cjp_excl",
-                                        "eoh"])
+                                        "eoh", "eoh_pol"])
 def gcase(request):
     print(f"\n[gpu] golden case {request.param}", flush=True)
     s = from_golden(golden_path(request.param))
@@ -239,7 +239,7 @@ def test_mixed_precision_error_budget(case):
     eng.close()
 
 
-@pytest.mark.parametrize("case", ["lig_eq1", "lig_fep2"])
+@pytest.mark.parametrize("case", ["lig_eq1", "lig_fep2", "eoh_pol"])
 def test_watpol_sequence(case):
     """The polarisation restraint over istep = 1..7 with itdis = 3 from a zero
     shell state (the theta_corr feedback of src/nonbondene.f90:6653-6666 runs

@@ -56,11 +56,14 @@ def test_cjp_matches_qdyn6(tmp_path):
     assert np.array_equal(s.x, g.ref["x_preconstraint"].T)
 
 
-def test_general_solvent_matches_qdyn6(tmp_path):
+@pytest.mark.parametrize("case", ["eoh", "eoh_pol"])
+def test_general_solvent_matches_qdyn6(case, tmp_path):
     from q6_amd import qfiles, synth
-    inp = qfiles.write_case(synth.cfg_ethanol(), str(tmp_path), 10, name="eoh", seed=0)
+    sy = synth.cfg_ethanol()
+    sy.par["wpol_restr"] = int(case == "eoh_pol")
+    inp = qfiles.write_case(sy, str(tmp_path), 10, name="eoh", seed=0)
     s = qread.build_system(inp)
-    g = from_golden(golden_path("eoh"))
+    g = from_golden(golden_path(case))
     assert compare(s, g) == []
     assert s.n["n_solv_int"] == 18
 
