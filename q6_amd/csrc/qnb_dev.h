@@ -738,7 +738,9 @@ inline int qforces_blocks(int nq, int chunks, int W) { return 8 * ((chunks + W -
 #ifndef FC_CAP
 #define FC_CAP 384
 #endif
+#ifndef FC_CH
 #define FC_CH 10
+#endif
 // solute receivers: at most 168 VGPRs, so three waves fit a SIMD (the
 // no-energy FP32 form fits; the energy-sampling and FP64 forms spill a few
 // bytes to scratch)
