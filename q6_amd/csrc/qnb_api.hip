@@ -26,7 +26,7 @@ __global__ void k_gforces(const DevSys*, uint32_t, int, int, int);
 template <typename R, int NS, bool E, int CAP>
 __global__ void k_sforces(const DevSys*, uint32_t, int, int, int);
 template <typename R, int NS>
-__global__ void k_qforces(const DevSys*, uint32_t, int);
+__global__ void k_qforces(const DevSys*, uint32_t, int, double*, int);
 template <typename R, int NS>
 __global__ void k_qq(const DevSys*, uint32_t);
 __global__ void k_pack_xq(const DevSys*);
@@ -687,7 +687,7 @@ static void launch_eval(qnb_handle* h, uint32_t mask) {
     int maxp = S.nwat + S.nat_solute;
     int chunks = std::max(1, std::min((maxp + QF_PART - 1) / QF_PART, 64));
     hipLaunchKernelGGL((k_qforces<R, NS>), dim3(qforces_blocks(S.nqat, chunks, wpb), 1), dim3(64 * wpb), 0,
-                       h->stream, h->dsys, mask, chunks);
+                       h->stream, h->dsys, mask, chunks, (double*)nullptr, 0);
   }
   const int with_qq = S.nqat > 0 && (mask & (QNB_T_QQ | QNB_T_QQP));
   const int with_pol = S.nwat > 0 && S.wpol && S.nshell > 0 && (mask & QNB_T_POL);

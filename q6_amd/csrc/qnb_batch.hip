@@ -25,7 +25,7 @@ __global__ void k_wforces(const DevSys*, uint32_t, int, int, int);
 template <typename R, int NS, bool E, int CAP>
 __global__ void k_sforces(const DevSys*, uint32_t, int, int, int);
 template <typename R, int NS>
-__global__ void k_qforces(const DevSys*, uint32_t, int);
+__global__ void k_qforces(const DevSys*, uint32_t, int, double*, int);
 template <typename R, int NS>
 __global__ void k_qq(const DevSys*, uint32_t);
 void launch_watpol(hipStream_t st, int nblk, int B, const DevSys* dsys, const int* istep);
@@ -176,7 +176,8 @@ __global__ __launch_bounds__(256) void k_bonded(const DevSys* __restrict__ syste
 // the SCAAS radial restraint (as k_atom_terms with every term on) and the
 // solute bonds and angles, added to the atom's gradient in one update.
 __global__ __launch_bounds__(256) void k_atom_step(const DevSys* __restrict__ systems, const MdDev* __restrict__ mds,
-                                                   const double* __restrict__ tfree) {
+                                                   const double* __restrict__ tfree, double* __restrict__ dq,
+                                                   int dq_stride) {
   const DevSys& S = systems[blockIdx.y];
   const MdDev& M = mds[blockIdx.y];
   __shared__ double red[4];
@@ -193,6 +194,15 @@ __global__ __launch_bounds__(256) void k_atom_step(const DevSys* __restrict__ sy
       radial_restraint(S, src, shift, f, er);
     }
     if (i < M.nbat) bonded_atom(S, M, i, f, eb, ea);
+    // the Q-atom side's rows of this step, cleared for the next one
+    const int iq = i < S.nat_solute ? ldg(S.iqatom + i) : 0;
+    if (iq > 0) {
+      double* q = dq + (size_t)blockIdx.y * dq_stride + 3 * (iq - 1);
+      for (int c = 0; c < 3; c++) {
+        f[c] += q[c];
+        q[c] = 0.0;
+      }
+    }
     store_atom_gradient(S, i, f);
     if (want_energies(S)) {
       stg(S.eatom + 2 * i, el);
@@ -526,6 +536,7 @@ struct qnb_batch {
   int nbblk = 1;  // k_bonded blocks per instance
   bool anyq = false, anypol = false, static_q = true;
   int qchunks = 1, nq_max = 0;
+  double* dq = nullptr;  // [n][nq_max][3]: the Q-atom side's gradient rows of the MD step
   int wpb_lists = 4, wpb_forces = 4;  // waves per block, picked by qnb_batch_autotune
   int imap_water = 1;                 // block placement of the water-receiver kernel (map_block), autotuned
   int imap_solute = 0;                // block placement of the solute-receiver kernel, autotuned
@@ -658,8 +669,11 @@ static void launch_qforces(qnb_batch* b) {
   const int wpb = 4;
   const dim3 grid(qforces_blocks(b->nq_max, b->qchunks, wpb), b->n), block(64 * wpb);
 #define QNB_LQ(NS)                                                                                     \
-  if (mixed) hipLaunchKernelGGL((k_qforces<float, NS>), grid, block, 0, b->stream, b->dsys, kStepMask, b->qchunks); \
-  else hipLaunchKernelGGL((k_qforces<double, NS>), grid, block, 0, b->stream, b->dsys, kStepMask, b->qchunks)
+  if (mixed)                                                                                                  \
+    hipLaunchKernelGGL((k_qforces<float, NS>), grid, block, 0, b->stream, b->dsys, kStepMask, b->qchunks,     \
+                       (double*)nullptr, 0);                                                                  \
+  else hipLaunchKernelGGL((k_qforces<double, NS>), grid, block, 0, b->stream, b->dsys, kStepMask, b->qchunks, \
+                          (double*)nullptr, 0)
   switch (b->ns_max) {
     case 1: QNB_LQ(1); break;
     case 2: QNB_LQ(2); break;
@@ -672,24 +686,26 @@ static void launch_qforces(qnb_batch* b) {
 
 // atom_terms = false: the caller runs the per-atom terms itself (the MD step
 // fuses them with the bonded terms in k_atom_step)
-// The MD step's force pass on two streams: the water receivers and the
-// polarisation restraint on the side stream, the solute receivers, the
-// Q-atom side and the Q-Q pairs on the batch stream.  The two branches write
-// disjoint gradient rows (waters / solute atoms) and disjoint energy slots;
-// the batch stream waits for the side branch before the per-atom terms.
+// The MD step's force pass on two streams: the Q-atom side (first: its few
+// blocks start beside the solute receivers), the water receivers and the
+// polarisation restraint on the side stream, the solute receivers and the
+// Q-Q pairs on the batch stream.  The two branches write
+// disjoint gradient rows (waters and the Q-atom side's own rows dq / solute
+// atoms) and disjoint energy slots; the batch stream waits for the side
+// branch before the per-atom terms, which add dq to the Q-atoms' gradients.
 template <typename R, int NS>
 static void enqueue_forces_forked(qnb_batch* b, bool energy) {
   const int wpb = 4, B = b->n;
   (void)hipEventRecord(b->ev_fork, b->stream);
   (void)hipStreamWaitEvent(b->side, b->ev_fork, 0);
-  if (b->maxw > 0) launch_forces_t<R, NS>(b, b->wpb_forces, energy, false, 1, b->side);
-  if (b->anypol) launch_watpol(b->side, (b->maxw + WP_THREADS - 1) / WP_THREADS, B, b->dsys, (const int*)b->istep);
-  launch_forces_t<R, NS>(b, b->wpb_forces, energy, false, 2);
-  if (b->anyq) {
+  if (b->anyq)
     hipLaunchKernelGGL((k_qforces<R, NS>), dim3(qforces_blocks(b->nq_max, b->qchunks, wpb), B), dim3(64 * wpb), 0,
-                       b->stream, b->dsys, kStepMask, b->qchunks);
-    hipLaunchKernelGGL((k_qq<R, NS>), dim3(B), dim3(QQ_THREADS), 0, b->stream, b->dsys, kStepMask);
-  }
+                       b->side, b->dsys, kStepMask, b->qchunks, b->dq, 3 * b->nq_max);
+  if (b->maxw > 0) launch_forces_t<R, NS>(b, b->wpb_forces, energy, false, 1, b->side);
+  if (b->anypol)
+    launch_watpol(b->side, (b->maxw + WP_THREADS - 1) / WP_THREADS, B, b->dsys, (const int*)b->istep);
+  launch_forces_t<R, NS>(b, b->wpb_forces, energy, false, 2);
+  if (b->anyq) hipLaunchKernelGGL((k_qq<R, NS>), dim3(B), dim3(QQ_THREADS), 0, b->stream, b->dsys, kStepMask);
   (void)hipEventRecord(b->ev_join, b->side);
   (void)hipStreamWaitEvent(b->stream, b->ev_join, 0);
 }
@@ -701,7 +717,7 @@ static void enqueue_forces(qnb_batch* b, bool energy = true, bool atom_terms = t
   launch_forces_t<R, NS>(b, b->wpb_forces, energy, atom_terms);
   if (b->anyq) {
     hipLaunchKernelGGL((k_qforces<R, NS>), dim3(qforces_blocks(b->nq_max, b->qchunks, wpb), B), dim3(64 * wpb), 0,
-                       b->stream, b->dsys, mask, b->qchunks);
+                       b->stream, b->dsys, mask, b->qchunks, (double*)nullptr, 0);
     hipLaunchKernelGGL((k_qq<R, NS>), dim3(B), dim3(QQ_THREADS), 0, b->stream, b->dsys, mask);
   }
   if (b->anypol) launch_watpol(b->stream, (b->maxw + WP_THREADS - 1) / WP_THREADS, B, b->dsys, (const int*)b->istep);
@@ -729,7 +745,7 @@ static void enqueue_step(qnb_batch* b, bool lists, bool energy) {
     default: enqueue_forces_forked<R, 8>(b, energy); break;
   }
   hipLaunchKernelGGL(k_atom_step, dim3((b->maxat + 255) / 256, B), dim3(256), 0, b->stream, b->dsys, b->dmd,
-                     (const double*)b->tfree);
+                     (const double*)b->tfree, b->dq, 3 * b->nq_max);
   if (b->anyqb) launch_qbonded(b->stream, B, b->dsys, b->dqb);
   // the reduction and the record do nothing on a step no instance samples
   // (energy is false only then); the record also counts stored snapshots
@@ -841,6 +857,8 @@ extern "C" int32_t qnb_batch_create(int32_t n, const qnb_system* const* sys, con
   HIPCHK(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
   if ((rc = b->mem.alloc(&b->tfree, n)) || (rc = b->mem.alloc(&b->istep, n))) return rc;
   HIPCHK(hipMemset(b->istep, 0, sizeof(int) * n));
+  if ((rc = b->mem.alloc(&b->dq, (size_t)n * 3 * std::max(b->nq_max, 1)))) return rc;
+  HIPCHK(hipMemset(b->dq, 0, sizeof(double) * n * 3 * std::max(b->nq_max, 1)));
   // MD records
   for (int k = 0; k < n; k++) {
     const qnb_md_params* m = md[k];

@@ -1494,10 +1494,13 @@ __global__ __launch_bounds__(256) void k_atom_terms(const DevSys* __restrict__ s
 // ---------------------------------------------------------------------------
 // Q-atom side of the Q-water and Q-solute pairs: wave = (Q-atom, chunk of its
 // partners, 64 at a time); the wave's sum is added to the Q-atom's gradient
-// with atomics (a few per Q-atom).
+// with atomics (a few per Q-atom), or, with dq, to the system's row
+// dq[blockIdx.y * dq_stride + 3 iq ..] (the MD step runs this beside the
+// solute receivers, which store the same atoms' rows, and adds dq in its
+// per-atom pass).
 template <typename R, int NS>
 __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ systems, uint32_t mask,
-                                                 int chunks_per_q) {
+                                                 int chunks_per_q, double* __restrict__ dq, int dq_stride) {
   const DevSys& S = systems[blockIdx.y];
   // block x runs on XCD x % 8 and takes Q-atom iq = x % 8 + 8 * (x / 8 / bq),
   // chunks (x / 8 % bq) * W .. + W - 1 (W waves per block, bq blocks per
@@ -1578,7 +1581,8 @@ __global__ __launch_bounds__(256) void k_qforces(const DevSys* __restrict__ syst
   }
   const double fs = wave_sum_tr(f, lane);
   const int c = wave_sum_tr_index<3>(lane);
-  if (wave_sum_tr_writer<3>(lane) && fs != 0.0) unsafeAtomicAdd(&S.d[3 * i + c], fs);
+  if (wave_sum_tr_writer<3>(lane) && fs != 0.0)
+    unsafeAtomicAdd(dq ? dq + (size_t)blockIdx.y * dq_stride + 3 * iq + c : &S.d[3 * i + c], fs);
 }
 
 // ---------------------------------------------------------------------------
@@ -2005,7 +2009,7 @@ template __global__ void k_lists_lrf<float, false>(const DevSys*);
 template __global__ void k_lists_lrf<double, true>(const DevSys*);
 template __global__ void k_lists_lrf<float, true>(const DevSys*);
 #define QNB_INST(R, NS)                                                            \
-  template __global__ void k_qforces<R, NS>(const DevSys*, uint32_t, int);                       \
+  template __global__ void k_qforces<R, NS>(const DevSys*, uint32_t, int, double*, int);                       \
   template __global__ void k_sforces<R, NS, true, WF_CAP_SMALL>(const DevSys*, uint32_t, int, int, int); \
   template __global__ void k_sforces<R, NS, false, WF_CAP_SMALL>(const DevSys*, uint32_t, int, int, int); \
   template __global__ void k_sforces<R, NS, true, FC_CAP>(const DevSys*, uint32_t, int, int, int);       \
