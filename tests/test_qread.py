@@ -110,3 +110,17 @@ def test_refusals(tmp_path):
         f.write(text.replace("lrf\t99", "lrf\t5"))
     with pytest.raises(qread.QFileError):
         qread.build_system(inp)
+
+
+@pytest.mark.parametrize("cfg", ["cfg1", "cfg2"])
+def test_workloads_round_trip(cfg, tmp_path):
+    """The benchmark workloads written by q6_amd.qfiles and read back give
+    the generator's own tables (3,063 and 10,861 atoms); stock Qdyn6 builds
+    the same tables from these files (tests/test_qfiles.py)."""
+    from q6_amd import qfiles, synth
+    s = getattr(synth, cfg)()
+    r = qread.build_system(qfiles.write_case(s, str(tmp_path), 10, name=cfg, seed=0))
+    s.finalize()
+    assert [k for k in compare(r, s) if k != "winv"] == []
+    assert np.array_equal(r.winv, 1.0 / np.asarray(s.md["mass"], float))
+    assert np.array_equal(r.x, s.x)
